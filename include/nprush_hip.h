@@ -1,0 +1,150 @@
+/*
+ * nprush_hip.h -- C ABI of libnprush_hip.so, the MI355X (gfx950) splat
+ * rasterizer and optimizer loop.
+ *
+ * Plain C types only: pointers, sizes, int32 status codes.  No exceptions cross
+ * the boundary.  Host pointers unless the name ends in _dev (device pointers on
+ * the handle's GPU).  Work is enqueued on the handle's own HIP stream; every
+ * nps_read_* call and nps_sync() wait for it.  One handle is not thread-safe;
+ * separate handles are independent.  Status: 0 = ok, < 0 = error (see
+ * nps_last_error()).
+ *
+ * The entry points replace the WebGPU objects of the reference
+ * (wooningeire/nprush):
+ *   GpuSplatOptimizerManager        src/lib/gpu/GpuSplatOptimizerManager.ts
+ *   GpuSplatForwardPipelineManager  src/lib/gpu/GpuSplatForwardPipelineManager.ts
+ * Each declaration names the reference method or buffer it stands for.
+ */
+#ifndef NPRUSH_HIP_H
+#define NPRUSH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NPS_OK 0
+#define NPS_ERR_ARG (-1)
+#define NPS_ERR_HIP (-2)
+#define NPS_ERR_STATE (-3)
+#define NPS_ERR_CAPACITY (-4)
+
+/* Per-splat layout (16 fp32, 64 B), splat_backward.wgsl:1-6:
+ * [x y z sx] [r g b opacity] [qw qx qy qz] [sy shape_a shape_b sz]. */
+#define NPS_PARAMS_PER_SPLAT 16
+
+/*
+ * Tunables, with the reference's values as defaults (nps_config_default).
+ * Replaces GPU_CONSTANTS + injectWgslConstants (src/lib/gpu/constants.ts:5-55)
+ * and the host cadence (GpuSplatOptimizerManager.ts:380-385).
+ */
+typedef struct nps_config {
+    int32_t max_tile_splats; /* SPLAT_MAX_TILE_SPLATS = 1024 (constants.ts:35) */
+    int32_t adc_period;      /* SPLAT_ADC_PERIOD = 25 (constants.ts:30; OPT:381) */
+    int32_t adc_enabled;     /* 1: nps_dispatch runs ADC every adc_period steps */
+    int32_t device;          /* HIP device ordinal */
+    int32_t reserved[12];
+} nps_config;
+
+typedef struct nps_ctx* nps_handle;
+
+/* Fill *cfg with the reference defaults. */
+int nps_config_default(nps_config* cfg);
+
+/* new GpuSplatOptimizerManager({device, format, numSplats}) (OPT:43-125):
+ * allocates splats (16N f32), grads (16N i32), Adam m/v (16N f32 each) + t,
+ * grad_accum (N f32).  Splats start zeroed (all dead); write them with
+ * nps_write_splats. */
+int nps_create(const nps_config* cfg, int32_t num_splats, nps_handle* out);
+void nps_destroy(nps_handle h);
+const char* nps_last_error(nps_handle h);
+int32_t nps_num_splats(nps_handle h);
+
+/* queue.writeBuffer(splatBuffer) / readback (OPT:88-93). n must equal num_splats. */
+int nps_write_splats(nps_handle h, const float* splats16n, int32_t n);
+int nps_read_splats(nps_handle h, float* splats16n, int32_t n);
+/* gradBuffer (OPT:95-99): i32 fixed point, 16 per splat. */
+int nps_write_grads(nps_handle h, const int32_t* grads16n, int32_t n);
+int nps_read_grads(nps_handle h, int32_t* grads16n, int32_t n);
+/* adamBuffer (OPT:101-106): m, v (16N each) and the step counter t. */
+int nps_write_adam(nps_handle h, const float* m16n, const float* v16n, float t, int32_t n);
+int nps_read_adam(nps_handle h, float* m16n, float* v16n, float* t, int32_t n);
+/* adcBuffer (OPT:108-112): grad_accum, N floats. */
+int nps_write_accum(nps_handle h, const float* accum, int32_t n);
+int nps_read_accum(nps_handle h, float* accum, int32_t n);
+
+/* writeSplatVPMatrix(mat, invMat, blurEnabled) (OPT:281-301): column-major
+ * view-projection and its inverse. */
+int nps_set_camera(nps_handle h, const float vp[16], const float vp_inv[16], int32_t blur_enabled);
+
+/* setBackwardTarget(colorView, depthView, w, h) (OPT:303-317): rgba is H*W*4
+ * floats (the .rgb of an rgba8unorm texel is used), depth is H*W floats (the
+ * .r of the depth texel).  Copied into the handle's device buffers. */
+int nps_set_target(nps_handle h, const float* rgba, const float* depth, int32_t w, int32_t hgt);
+/* Same with device pointers (copied device-to-device). */
+int nps_set_target_dev(nps_handle h, const float* rgba_dev, const float* depth_dev, int32_t w, int32_t hgt);
+
+/* dispatchWorkgroups(ceil(W/16), ceil(H/16)) of splat_backward.wgsl (OPT:372-374):
+ * binning, forward composite, L2 loss seeds, reverse pass, fixed-point grads. */
+int nps_backward(nps_handle h);
+/* splat_step.wgsl (OPT:376-378): fused Adam; increments t. */
+int nps_step(nps_handle h);
+/* splat_adc.wgsl (OPT:380-385): clone/split into dead slots. */
+int nps_adc(nps_handle h);
+/* GpuSplatOptimizerManager.dispatch (OPT:365-388): backward, step, and ADC on
+ * every adc_period-th call.  Returns NPS_ERR_STATE when no target is set
+ * (the reference returns silently, OPT:366). */
+int nps_dispatch(nps_handle h);
+int nps_sync(nps_handle h);
+
+/* Outputs of the last nps_backward.  The reference never materialises them
+ * (splat_backward.wgsl:230-248); exposed for parity checks and metrics.
+ * c_pred: H*W*3, d_pred: H*W (either may be NULL). */
+int nps_read_image(nps_handle h, float* c_pred, float* d_pred);
+/* loss[0] = sum over pixels of |C-tgt|^2 + fg*(D-tgt_depth)^2 (the loss whose
+ * gradient the reference seeds, BW:250-257); loss[1] = sum |C-tgt| (L1). */
+int nps_read_loss(nps_handle h, double loss[2]);
+/* Tile lists of the last nps_backward: ranges[2*T] = [start, end) into the
+ * sorted list (uncapped), ids[M] = splat ids in tile order.  Pass NULL to query
+ * sizes in *num_tiles / *num_ids. */
+int nps_read_tiles(nps_handle h, uint32_t* ranges, uint32_t* ids, int32_t* num_tiles, int64_t* num_ids);
+/* Counters of the last backward/ADC: stats[0]=keys, [1]=tiles over the cap,
+ * [2]=pairs composited, [3]=ADC qualifiers, [4]=ADC children, [5]=ADC splits,
+ * [6]=ADC clones, [7]=ADC cascade rounds. */
+int nps_read_stats(nps_handle h, int64_t stats[8]);
+/* Per-phase GPU time (ms, HIP events on the handle's stream) of the last
+ * backward/step/adc when timing is enabled: [0]=preprocess+keys, [1]=sort,
+ * [2]=ranges, [3]=composite fwd+bwd, [4]=step, [5]=adc, [6]=whole backward. */
+int nps_enable_timing(nps_handle h, int32_t on);
+int nps_read_timing(nps_handle h, float ms[8]);
+
+/*
+ * new GpuSplatForwardPipelineManager({device, numSplats, splatBuffer})
+ * (GpuSplatForwardPipelineManager.ts:17-92): borrows the optimizer handle's
+ * splat buffer.  Instanced-quad render of splat_forward.wgsl, reverse index
+ * order, "over" blending into colour and encoded-depth targets.
+ */
+typedef struct npf_ctx* npf_handle;
+int npf_create(nps_handle splats_owner, npf_handle* out);
+void npf_destroy(npf_handle f);
+/* writeVPMatrix(mat) (FWDM:94-102) */
+int npf_write_vp(npf_handle f, const float vp[16]);
+/* setTarget(colorView, depthView, w, h) (FWDM:104-112): the targets live in
+ * the handle (H*W*4 floats each). */
+int npf_set_target(npf_handle f, int32_t w, int32_t hgt);
+/* dispatch(encoder, clear) (FWDM:114-137).  quantize != 0 rounds every
+ * channel to k/255 after each blend, as an rgba8unorm target does. */
+int npf_dispatch(npf_handle f, int32_t clear, int32_t quantize);
+int npf_read_target(npf_handle f, float* color, float* depth);
+int npf_write_target(npf_handle f, const float* color, const float* depth);
+
+/* Deterministic transcendentals used by the kernels, evaluated on the GPU
+ * over x[n] (and y[n] for pow): op 0=exp2 1=log2 2=pow 3=exp 4=log 5=sin.
+ * For tests (bit-identity with the CPU oracle). */
+int nps_debug_math(int32_t device, int32_t op, const float* x, const float* y, float* out, int32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,13 @@
+"""MI355X (gfx950) splat rasterizer and optimizer loop of nprush.
+
+Hot path in hand-written HIP kernels (nprush_amd/csrc) behind a C ABI
+(include/nprush_hip.h, libnprush_hip.so); Python classes mirroring the
+reference's GpuSplatOptimizerManager / GpuSplatForwardPipelineManager.
+"""
+from . import camera, scenes  # noqa: F401
+from .managers import (  # noqa: F401
+    GpuSplatForwardPipelineManager,
+    GpuSplatOptimizerManager,
+    debug_math,
+    reference_init,
+)

@@ -1,0 +1,126 @@
+"""ctypes binding of libnprush_hip.so (include/nprush_hip.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (nprush_amd/csrc/Makefile).
+Loading fails loudly when it is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnprush_hip.so")
+
+
+class NpsConfig(C.Structure):
+    _fields_ = [
+        ("max_tile_splats", C.c_int32),
+        ("adc_period", C.c_int32),
+        ("adc_enabled", C.c_int32),
+        ("device", C.c_int32),
+        ("reserved", C.c_int32 * 12),
+    ]
+
+
+_P = C.c_void_p
+_F = C.POINTER(C.c_float)
+_I = C.POINTER(C.c_int32)
+_U = C.POINTER(C.c_uint32)
+_L = C.POINTER(C.c_int64)
+_D = C.POINTER(C.c_double)
+i32 = C.c_int32
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "nps_config_default": (i32, [C.POINTER(NpsConfig)]),
+    "nps_create": (i32, [C.POINTER(NpsConfig), i32, C.POINTER(_P)]),
+    "nps_destroy": (None, [_P]),
+    "nps_last_error": (C.c_char_p, [_P]),
+    "nps_num_splats": (i32, [_P]),
+    "nps_write_splats": (i32, [_P, _F, i32]),
+    "nps_read_splats": (i32, [_P, _F, i32]),
+    "nps_write_grads": (i32, [_P, _I, i32]),
+    "nps_read_grads": (i32, [_P, _I, i32]),
+    "nps_write_adam": (i32, [_P, _F, _F, C.c_float, i32]),
+    "nps_read_adam": (i32, [_P, _F, _F, _F, i32]),
+    "nps_write_accum": (i32, [_P, _F, i32]),
+    "nps_read_accum": (i32, [_P, _F, i32]),
+    "nps_set_camera": (i32, [_P, _F, _F, i32]),
+    "nps_set_target": (i32, [_P, _F, _F, i32, i32]),
+    "nps_set_target_dev": (i32, [_P, _P, _P, i32, i32]),
+    "nps_backward": (i32, [_P]),
+    "nps_step": (i32, [_P]),
+    "nps_adc": (i32, [_P]),
+    "nps_dispatch": (i32, [_P]),
+    "nps_sync": (i32, [_P]),
+    "nps_read_image": (i32, [_P, _F, _F]),
+    "nps_read_loss": (i32, [_P, _D]),
+    "nps_read_tiles": (i32, [_P, _U, _U, _I, _L]),
+    "nps_read_stats": (i32, [_P, _L]),
+    "nps_enable_timing": (i32, [_P, i32]),
+    "nps_read_timing": (i32, [_P, _F]),
+    "npf_create": (i32, [_P, C.POINTER(_P)]),
+    "npf_destroy": (None, [_P]),
+    "npf_write_vp": (i32, [_P, _F]),
+    "npf_set_target": (i32, [_P, i32, i32]),
+    "npf_dispatch": (i32, [_P, i32, i32]),
+    "npf_read_target": (i32, [_P, _F, _F]),
+    "npf_write_target": (i32, [_P, _F, _F]),
+    "nps_debug_math": (i32, [i32, i32, _F, _F, _F, i32]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() (no CPU fallback exists)")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_F)
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_I)
+
+
+def uptr(a: np.ndarray):
+    assert a.dtype == np.uint32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_U)
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_D)
+
+
+def lptr(a: np.ndarray):
+    assert a.dtype == np.int64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_L)
+
+
+class NpsError(RuntimeError):
+    pass
+
+
+def check(rc: int, handle=None, what: str = ""):
+    if rc != 0:
+        msg = ""
+        if handle is not None:
+            m = load().nps_last_error(handle)
+            msg = m.decode() if m else ""
+        raise NpsError(f"{what} failed ({rc}): {msg}")

@@ -1,0 +1,377 @@
+// k_composite.hip -- fused per-tile forward composite, L2 loss seeds and
+// reverse pass with fixed-point gradients (REF semantics of
+// splat_backward.wgsl:213-455).
+//
+// One 256-thread workgroup per 16x16 tile, one thread per pixel.  The tile list
+// (ascending splat ids, first `cap` of them) is walked in batches of 64 splat
+// records staged in LDS.
+//   forward : per pixel front-to-back alpha compositing; the transmittance T
+//             is written to a per-(tile, chunk, pixel) checkpoint every 16
+//             entries (coalesced, 1 KB per tile per chunk);
+//   reverse : batches and 16-entry chunks in reverse order.  For each chunk the
+//             alphas and T values are recomputed from the checkpoint into LDS
+//             (bit-identical to the forward values, same expressions), then
+//             walked backwards.  The reference keeps alphas[1024]/Ts[1025] per
+//             thread instead (8 KB of private memory per pixel).
+//   grads   : per pair the 16 gradients are converted to i32 exactly as the
+//             reference's atomicAdd(i32(g * FP)) does; a wave sums them with a
+//             17-shuffle transpose reduction (integer adds, so the sum equals
+//             the reference's order-independent atomics), 16 lanes add the
+//             totals into an LDS accumulator per batch, and each batch ends with
+//             one global atomic per non-zero (splat, parameter).
+// Waves whose 64 pixels all have alpha < 0.001 for a splat skip it, which is
+// exactly the reference's per-pixel `continue` (BW:263).
+#include "nps_common.h"
+#include "nps_kernels.h"
+
+namespace nps {
+
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t i, uint32_t n) {
+    // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+    // contiguous run of tiles so neighbouring tiles share its L2.
+    uint32_t per = n / 8u, rem = n % 8u;
+    uint32_t xcd = i % 8u, local = i / 8u;
+    return xcd < rem ? xcd * (per + 1u) + local : rem * (per + 1u) + (xcd - rem) * per + local;
+}
+
+// Wave-wide sum of 16 values; lane l ends with the total of value
+// ((l>>5)&1)*8 + ((l>>4)&1)*4 + ((l>>3)&1)*2 + ((l>>2)&1).
+__device__ __forceinline__ uint32_t wave_reduce16(uint32_t (&v)[16], int lane) {
+    {
+        bool hi = lane & 32;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            uint32_t send = hi ? v[k] : v[k + 8];
+            uint32_t keep = hi ? v[k + 8] : v[k];
+            v[k] = keep + (uint32_t)__shfl_xor((int)send, 32, 64);
+        }
+    }
+    {
+        bool hi = lane & 16;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            uint32_t send = hi ? v[k] : v[k + 4];
+            uint32_t keep = hi ? v[k + 4] : v[k];
+            v[k] = keep + (uint32_t)__shfl_xor((int)send, 16, 64);
+        }
+    }
+    {
+        bool hi = lane & 8;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            uint32_t send = hi ? v[k] : v[k + 2];
+            uint32_t keep = hi ? v[k + 2] : v[k];
+            v[k] = keep + (uint32_t)__shfl_xor((int)send, 8, 64);
+        }
+    }
+    {
+        bool hi = lane & 4;
+        uint32_t send = hi ? v[0] : v[1];
+        uint32_t keep = hi ? v[1] : v[0];
+        v[0] = keep + (uint32_t)__shfl_xor((int)send, 4, 64);
+    }
+    v[0] += (uint32_t)__shfl_xor((int)v[0], 2, 64);
+    v[0] += (uint32_t)__shfl_xor((int)v[0], 1, 64);
+    return v[0];
+}
+
+__device__ __forceinline__ uint32_t gi(float g, float fp) { return (uint32_t)f2i_sat(g * fp); }
+
+struct PixelState {
+    float dCr, dCg, dCb, dD;
+    float dT_C, dT_D;
+    float sx, sy, sz;  // surface point for the 3D pull
+    bool fg;
+};
+
+// Reverse-pass body for one (pixel, splat) pair: splat_backward.wgsl:264-455
+// with the per-splat factors taken from the records.  Writes 16 i32 values.
+__device__ __forceinline__ void pair_grads(const FwdRec& f, const BwdRec& b, float px, float py, float a,
+                                           float T_prev, PixelState& st, uint32_t (&out)[16]) {
+    PairGeom g = pair_geom(f, px, py);
+    const float cr = f.r, cg = f.g, cb = f.b, opacity = f.op;
+    float Ta = T_prev * a;
+    float dColor_r = st.dCr * Ta, dColor_g = st.dCg * Ta, dColor_b = st.dCb * Ta;
+    float depth = f.depth;
+    float da_C = st.dT_C * (-T_prev) + ((st.dCr * (T_prev * cr) + st.dCg * (T_prev * cg)) + st.dCb * (T_prev * cb));
+    float da_D = st.dT_D * (-T_prev) + (st.dD * T_prev) * depth;
+    float da = da_C + da_D;
+    st.dT_C = st.dT_C * (1.0f - a) + ((st.dCr * (a * cr) + st.dCg * (a * cg)) + st.dCb * (a * cb));
+    st.dT_D = st.dT_D * (1.0f - a) + (st.dD * a) * depth;
+    float d_depth = (st.dD * T_prev) * a;
+    const float sx = b.s3[0], sy = b.s3[1], sz = b.s3[2];
+    const float shape_a = f.sa, shape_b = f.sb;
+    float power = g.power;
+    bool above = power > -15.0f;
+    float a_un = above ? exp_det(power) : 0.0f;
+    float d_opacity = above ? da * a_un : 0.0f;
+    float d_power = above ? (da * opacity) * a_un : 0.0f;
+    float r_pow_a = g.rpa;
+    float r_pow_a_m2 = exp2_det((shape_a - 2.0f) * g.log2r);
+    float log_r = g.log2r * kLn2;
+    float d_shape_a = above ? d_power * ((-shape_b * r_pow_a) * log_r) : 0.0f;
+    float d_shape_b = above ? d_power * (-r_pow_a) : 0.0f;
+    float kk = d_power * ((-shape_b * shape_a) * r_pow_a_m2);
+    float d_dx = above ? kk * g.spx : 0.0f;
+    float d_dy = above ? kk * g.spy : 0.0f;
+    float d_screen_x = -d_dx, d_screen_y = -d_dy;
+
+    float sp_dot_m0 = g.spx * b.m[0] + g.spy * b.m[1];
+    float sp_dot_m1 = g.spx * b.m[2] + g.spy * b.m[3];
+    float sp_dot_m2 = g.spx * b.m[4] + g.spy * b.m[5];
+    float d_m0x = -sp_dot_m0 * d_dx, d_m0y = -sp_dot_m0 * d_dy;
+    float d_m1x = -sp_dot_m1 * d_dx, d_m1y = -sp_dot_m1 * d_dy;
+    float d_m2x = -sp_dot_m2 * d_dx, d_m2y = -sp_dot_m2 * d_dy;
+    float d_axs_x = d_m0x * sx, d_axs_y = d_m0y * sx;
+    float d_ays_x = d_m1x * sy, d_ays_y = d_m1y * sy;
+    float d_azs_x = d_m2x * sz, d_azs_y = d_m2y * sz;
+    float d_sx = d_m0x * b.axs[0] + d_m0y * b.axs[1];
+    float d_sy = d_m1x * b.axs[2] + d_m1y * b.axs[3];
+    float d_sz = d_m2x * b.axs[4] + d_m2y * b.axs[5];
+
+    bool w_ok = b.w > 0.1f;
+    float d_pos[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        float dp = d_screen_x * b.J0[ax] + d_screen_y * b.J1[ax];
+        d_pos[ax] = dp + (w_ok ? b.vpd[ax] * d_depth : 0.0f);
+    }
+    if (st.fg) {
+        const float wgt = 0.3f * 2.0f;
+        d_pos[0] = d_pos[0] + (wgt * (b.pos[0] - st.sx)) * Ta;
+        d_pos[1] = d_pos[1] + (wgt * (b.pos[1] - st.sy)) * Ta;
+        d_pos[2] = d_pos[2] + (wgt * (b.pos[2] - st.sz)) * Ta;
+    }
+    float axw[3], ayw[3], azw[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        axw[k] = d_axs_x * b.J0[k] + d_axs_y * b.J1[k];
+        ayw[k] = d_ays_x * b.J0[k] + d_ays_y * b.J1[k];
+        azw[k] = d_azs_x * b.J0[k] + d_azs_y * b.J1[k];
+    }
+    const float qw = b.q[0], qx = b.q[1], qy = b.q[2], qz = b.q[3];
+    const float T2 = 2.0f, T4 = 4.0f;
+    float d_qw_ax = (axw[1] * T2) * qz - (axw[2] * T2) * qy;
+    float d_qw_ay = ((-ayw[0]) * T2) * qz + (ayw[2] * T2) * qx;
+    float d_qw_az = (azw[0] * T2) * qy - (azw[1] * T2) * qx;
+    float d_qx_ax = (axw[1] * T2) * qy + (axw[2] * T2) * qz;
+    float d_qx_ay = ((ayw[0] * T2) * qy - (ayw[1] * T4) * qx) + (ayw[2] * T2) * qw;
+    float d_qx_az = ((azw[0] * T2) * qz - (azw[1] * T2) * qw) - (azw[2] * T4) * qx;
+    float d_qy_ax = (((-axw[0]) * T4) * qy + (axw[1] * T2) * qx) + axw[2] * (-T2 * qw);
+    float d_qy_ay = (ayw[0] * T2) * qx + (ayw[2] * T2) * qz;
+    float d_qy_az = ((azw[0] * T2) * qw + (azw[1] * T2) * qz) - (azw[2] * T4) * qy;
+    float d_qz_ax = (((-axw[0]) * T4) * qz + (axw[1] * T2) * qw) + (axw[2] * T2) * qx;
+    float d_qz_ay = (((-ayw[0]) * T2) * qw - (ayw[1] * T4) * qz) + (ayw[2] * T2) * qy;
+    float d_qz_az = (azw[0] * T2) * qx + (azw[1] * T2) * qy;
+    float d_qw = (d_qw_ax + d_qw_ay) + d_qw_az;
+    float d_qx = (d_qx_ax + d_qx_ay) + d_qx_az;
+    float d_qy = (d_qy_ax + d_qy_ay) + d_qy_az;
+    float d_qz = (d_qz_ax + d_qz_ay) + d_qz_az;
+    const float reg = 0.005f * 2.0f;
+    d_shape_a = d_shape_a + reg * (shape_a - 6.0f);
+    d_shape_b = d_shape_b + reg * (shape_b - 0.3f);
+
+    const float FPP = 10000.0f, FPC = 100000.0f;
+    out[0] = gi(d_pos[0], FPP);
+    out[1] = gi(d_pos[1], FPP);
+    out[2] = gi(d_pos[2], FPP);
+    out[3] = gi(d_sx, FPP);
+    out[4] = gi(dColor_r, FPC);
+    out[5] = gi(dColor_g, FPC);
+    out[6] = gi(dColor_b, FPC);
+    out[7] = gi(d_opacity, FPC);
+    out[8] = gi(d_qw, FPP);
+    out[9] = gi(d_qx, FPP);
+    out[10] = gi(d_qy, FPP);
+    out[11] = gi(d_qz, FPP);
+    out[12] = gi(d_sy, FPP);
+    out[13] = gi(d_shape_a, FPP);
+    out[14] = gi(d_shape_b, FPP);
+    out[15] = gi(d_sz, FPP);
+}
+
+__device__ __forceinline__ void stage_records(const FwdRec* __restrict__ fwd, const BwdRec* __restrict__ bwd,
+                                              const uint64_t* __restrict__ keys, uint32_t base, int nb,
+                                              FwdRec* s_fwd, BwdRec* s_bwd, uint32_t* s_ids, bool with_bwd) {
+    const int t = threadIdx.x;
+    {
+        int j = t >> 2, part = t & 3;
+        if (j < nb) {
+            uint32_t id = (uint32_t)keys[base + j];
+            reinterpret_cast<float4*>(&s_fwd[j])[part] = reinterpret_cast<const float4*>(&fwd[id])[part];
+            if (part == 0) s_ids[j] = id;
+        }
+    }
+    if (with_bwd) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            int e = t + h * 256;
+            int j = e >> 3, part = e & 7;
+            if (j < nb) {
+                uint32_t id = (uint32_t)keys[base + j];
+                reinterpret_cast<float4*>(&s_bwd[j])[part] = reinterpret_cast<const float4*>(&bwd[id])[part];
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_composite_ref(ViewParams v, CompositeArgs A) {
+    __shared__ FwdRec s_fwd[kBatch];
+    __shared__ BwdRec s_bwd[kBatch];
+    __shared__ uint32_t s_ids[kBatch];
+    __shared__ uint32_t s_grad[kBatch][16];
+    __shared__ float s_a[kChunk][kTilePx];
+    __shared__ float s_T[kChunk][kTilePx];
+    __shared__ double s_red[2][4];
+
+    const uint32_t tile = xcd_remap(blockIdx.x, v.num_tiles);
+    const uint32_t tx = tile % v.tiles_x, ty = tile / v.tiles_x;
+    const uint32_t start = A.ranges[2 * tile], end = A.ranges[2 * tile + 1];
+    const uint32_t len = end - start;
+    const int count = (int)min(len, A.cap);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const uint32_t px = tx * kTile + (tid & 15), py = ty * kTile + (tid >> 4);
+    const bool valid = px < v.W && py < v.H;
+    const float pxf = px_to_x(px, v.W, v.aspect);
+    const float pyf = py_to_y(py, v.H);
+    float* ckpt = A.ckpt + (size_t)tile * A.nchunk_max * kTilePx;
+
+    // ---------------- forward (BW:216-248) ----------------
+    float T = 1.0f, Cr = 0.0f, Cg = 0.0f, Cb = 0.0f, D = 0.0f;
+    for (int b0 = 0; b0 < count; b0 += kBatch) {
+        int nb = min(kBatch, count - b0);
+        __syncthreads();
+        stage_records(A.fwd, A.bwd, A.keys, start + (uint32_t)b0, nb, s_fwd, s_bwd, s_ids, false);
+        __syncthreads();
+        for (int j = 0; j < nb; ++j) {
+            if ((j & (kChunk - 1)) == 0) ckpt[(size_t)((b0 + j) / kChunk) * kTilePx + tid] = T;
+            const FwdRec& f = s_fwd[j];
+            PairGeom g = pair_geom(f, pxf, pyf);
+            float a = pair_alpha(f, g);
+            float Ta = T * a;
+            Cr = Cr + Ta * f.r;
+            Cg = Cg + Ta * f.g;
+            Cb = Cb + Ta * f.b;
+            D = D + Ta * f.depth;
+            T = T * (1.0f - a);
+        }
+    }
+    const float bg = 0.05f;
+    Cr = Cr + T * bg;
+    Cg = Cg + T * bg;
+    Cb = Cb + T * bg;
+    D = D + T * 1.0f;
+
+    // ---------------- loss seeds (BW:250-257) ----------------
+    PixelState st;
+    float4 tg = make_float4(0.f, 0.f, 0.f, 0.f);
+    float tdepth = 1.0f;
+    if (valid) {
+        size_t pix = (size_t)py * v.W + px;
+        tg = A.tgt_rgba[pix];
+        tdepth = A.tgt_depth[pix];
+        A.c_img[3 * pix + 0] = Cr;
+        A.c_img[3 * pix + 1] = Cg;
+        A.c_img[3 * pix + 2] = Cb;
+        A.d_img[pix] = D;
+    }
+    st.dCr = 2.0f * (Cr - tg.x);
+    st.dCg = 2.0f * (Cg - tg.y);
+    st.dCb = 2.0f * (Cb - tg.z);
+    st.fg = tdepth < 0.99f;
+    st.dD = st.fg ? 2.0f * (D - tdepth) : 0.0f;
+    double l2 = 0.0, l1 = 0.0;
+    if (valid) {
+        double er = (double)Cr - (double)tg.x, eg = (double)Cg - (double)tg.y, eb = (double)Cb - (double)tg.z;
+        double ed = st.fg ? (double)D - (double)tdepth : 0.0;
+        l2 = er * er + eg * eg + eb * eb + ed * ed;
+        l1 = fabs(er) + fabs(eg) + fabs(eb);
+    }
+    st.dT_C = (st.dCr * bg + st.dCg * bg) + st.dCb * bg;
+    st.dT_D = st.dD * 1.0f;
+    st.sx = st.sy = st.sz = 0.0f;
+    if (st.fg) {
+        // reconstruct_world_pos (BW:64-75)
+        float w = 0.1f / fmax_w(1.0f - tdepth, 1e-5f);
+        float ndc_x = pxf / v.aspect;
+        float ndc_y = pyf;
+        F4 world = mat_vec(v.vp_inv, ndc_x * w, ndc_y * w, w, w);
+        st.sx = world.x / world.w;
+        st.sy = world.y / world.w;
+        st.sz = world.z / world.w;
+    }
+    // per-tile loss partial (fixed reduction tree: deterministic)
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        l2 += __shfl_xor(l2, o, 64);
+        l1 += __shfl_xor(l1, o, 64);
+    }
+    if (lane == 0) {
+        s_red[0][tid >> 6] = l2;
+        s_red[1][tid >> 6] = l1;
+    }
+
+    // ---------------- reverse pass (BW:259-455) ----------------
+    const int nbatch = (count + kBatch - 1) / kBatch;
+    for (int bi = nbatch - 1; bi >= 0; --bi) {
+        const int b0 = bi * kBatch;
+        const int nb = min(kBatch, count - b0);
+        __syncthreads();
+        stage_records(A.fwd, A.bwd, A.keys, start + (uint32_t)b0, nb, s_fwd, s_bwd, s_ids, true);
+        for (int e = tid; e < kBatch * 16; e += 256) (&s_grad[0][0])[e] = 0u;
+        __syncthreads();
+        const int nch = (nb + kChunk - 1) / kChunk;
+        for (int c = nch - 1; c >= 0; --c) {
+            const int j0 = c * kChunk, j1 = min(j0 + kChunk, nb);
+            float Tc = ckpt[(size_t)((b0 + j0) / kChunk) * kTilePx + tid];
+            for (int j = j0; j < j1; ++j) {
+                const FwdRec& f = s_fwd[j];
+                PairGeom g = pair_geom(f, pxf, pyf);
+                float a = pair_alpha(f, g);
+                s_a[j - j0][tid] = a;
+                s_T[j - j0][tid] = Tc;
+                Tc = Tc * (1.0f - a);
+            }
+            for (int j = j1 - 1; j >= j0; --j) {
+                float a = s_a[j - j0][tid];
+                float Tp = s_T[j - j0][tid];
+                bool active = valid && !(a < 0.001f);
+                if (__ballot(active) == 0ull) continue;
+                uint32_t gv[16];
+                if (active) {
+                    pair_grads(s_fwd[j], s_bwd[j], pxf, pyf, a, Tp, st, gv);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) gv[k] = 0u;
+                }
+                uint32_t tot = wave_reduce16(gv, lane);
+                if ((lane & 3) == 0 && tot != 0u) {
+                    int idx = ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);
+                    atomicAdd(&s_grad[j][idx], tot);
+                }
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < nb * 16; e += 256) {
+            uint32_t val = (&s_grad[0][0])[e];
+            if (val != 0u) atomicAdd((unsigned int*)&A.grads[16 * (size_t)s_ids[e >> 4] + (e & 15)], val);
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double a2 = (s_red[0][0] + s_red[0][1]) + (s_red[0][2] + s_red[0][3]);
+        double a1 = (s_red[1][0] + s_red[1][1]) + (s_red[1][2] + s_red[1][3]);
+        A.tile_loss[2 * tile] = a2;
+        A.tile_loss[2 * tile + 1] = a1;
+        uint32_t vw = min(v.W - tx * kTile, (uint32_t)kTile), vh = min(v.H - ty * kTile, (uint32_t)kTile);
+        if (count > 0) atomicAdd(&A.stats[0], (unsigned long long)count * (unsigned long long)(vw * vh));
+        if (len > A.cap) atomicAdd(&A.stats[1], 1ull);
+    }
+}
+
+void launch_composite_ref(hipStream_t st, const ViewParams& v, const CompositeArgs& a) {
+    if (v.num_tiles == 0) return;
+    hipLaunchKernelGGL(k_composite_ref, dim3(v.num_tiles), dim3(256), 0, st, v, a);
+}
+
+}  // namespace nps

@@ -1,0 +1,659 @@
+// nps_api.cpp -- the C ABI of libnprush_hip.so (declared in include/nprush_hip.h).
+//
+// Host orchestration of one training view, in the order of
+// GpuSplatOptimizerManager.dispatch (src/lib/gpu/GpuSplatOptimizerManager.ts:365-388):
+//   backward = preprocess -> scan of per-splat tile counts -> key emission ->
+//              LSD radix sort on the tile bits -> tile ranges -> fused composite
+//   step     = fused Adam
+//   adc      = every adc_period-th dispatch
+// All work is enqueued on the handle's stream (default priority).  Per-view
+// scratch grows on demand and is kept for the next call.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/nprush_hip.h"
+#include "nps_kernels.h"
+
+using namespace nps;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Scratch {
+    DevBuf counts, rects, offsets, keys, keys_alt, ranges, scan_tmp, sort_tmp;
+    uint64_t m = 0;
+    bool in_alt = false;
+};
+
+}  // namespace
+
+struct nps_ctx {
+    nps_config cfg;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t n = 0;
+    float* splats = nullptr;
+    int32_t* grads = nullptr;
+    float* m = nullptr;
+    float* v = nullptr;
+    float* accum = nullptr;
+    float adam_t = 0.0f;
+    int64_t step_count = 0;
+    float vp[16] = {0}, vp_inv[16] = {0};
+    int blur = 0;
+    bool has_camera = false;
+    uint32_t W = 0, H = 0;
+    bool has_target = false;
+    DevBuf tgt_rgba, tgt_depth;
+    DevBuf fwd, bwd, ckpt, c_img, d_img, tile_loss, stats;
+    Scratch sc;
+    uint32_t last_tiles = 0;
+    bool have_backward = false;
+    // ADC buffers
+    DevBuf adc_u32, adc_tmp, adc_scan;
+    int64_t stats_host[8] = {0};
+    bool timing = false;
+    hipEvent_t ev[8] = {nullptr};
+    float timing_ms[8] = {0};
+    std::string err;
+};
+
+struct npf_ctx {
+    nps_ctx* owner = nullptr;
+    float vp[16] = {0};
+    bool has_vp = false;
+    uint32_t W = 0, H = 0;
+    DevBuf color, depth, rec;
+    Scratch sc;
+};
+
+namespace {
+
+int set_err(nps_ctx* h, int code, const std::string& msg) {
+    if (h) h->err = msg;
+    return code;
+}
+
+#define NPS_HIP(h, expr)                                                                        \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) return set_err((h), NPS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int ensure(nps_ctx* h, DevBuf& b, size_t bytes) {
+    if (bytes <= b.bytes && b.p) return NPS_OK;
+    if (b.p) {
+        hipError_t e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) return set_err(h, NPS_ERR_HIP, std::string("sync: ") + hipGetErrorString(e));
+        (void)hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    size_t alloc = std::max<size_t>(bytes + bytes / 4, 256);
+    hipError_t e = hipMalloc(&b.p, alloc);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return set_err(h, NPS_ERR_HIP, "hipMalloc(" + std::to_string(alloc) + "): " + hipGetErrorString(e));
+    }
+    b.bytes = alloc;
+    return NPS_OK;
+}
+
+void free_buf(DevBuf& b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+void free_scratch(Scratch& s) {
+    free_buf(s.counts); free_buf(s.rects); free_buf(s.offsets); free_buf(s.keys);
+    free_buf(s.keys_alt); free_buf(s.ranges); free_buf(s.scan_tmp); free_buf(s.sort_tmp);
+}
+
+ViewParams make_view(const float* vp, const float* vp_inv, uint32_t W, uint32_t H) {
+    ViewParams v;
+    memset(&v, 0, sizeof(v));
+    memcpy(v.vp, vp, sizeof(v.vp));
+    if (vp_inv) memcpy(v.vp_inv, vp_inv, sizeof(v.vp_inv));
+    v.W = W;
+    v.H = H;
+    v.aspect = (float)W / (float)H;
+    v.tiles_x = (W + kTile - 1) / kTile;
+    v.tiles_y = (H + kTile - 1) / kTile;
+    v.num_tiles = v.tiles_x * v.tiles_y;
+    return v;
+}
+
+int bits_for(uint32_t x) {
+    int b = 1;
+    while (b < 32 && (x >> b) != 0u) ++b;
+    return b;
+}
+
+// keys for the counts already in sc.counts/sc.rects: scan, emit, sort, ranges.
+int build_tile_lists(nps_ctx* h, Scratch& sc, const ViewParams& v, uint32_t n, int ev_sort, int ev_ranges) {
+    int rc;
+    if ((rc = ensure(h, sc.offsets, sizeof(uint64_t) * ((size_t)n + 1))) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.scan_tmp, scan_temp_bytes(n) + 64)) != NPS_OK) return rc;
+    exclusive_scan_u32_to_u64(h->stream, (const uint32_t*)sc.counts.p, (uint64_t*)sc.offsets.p, n, sc.scan_tmp.p);
+    uint64_t m = 0;
+    NPS_HIP(h, hipMemcpyAsync(&m, (uint64_t*)sc.offsets.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost, h->stream));
+    NPS_HIP(h, hipStreamSynchronize(h->stream));
+    if (m >= 0xFFFFFFFFull) return set_err(h, NPS_ERR_CAPACITY, "tile key count exceeds 2^32");
+    sc.m = m;
+    if ((rc = ensure(h, sc.keys, sizeof(uint64_t) * std::max<uint64_t>(m, 1))) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.keys_alt, sizeof(uint64_t) * std::max<uint64_t>(m, 1))) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.sort_tmp, radix_temp_bytes(m))) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.ranges, sizeof(uint32_t) * 2 * (size_t)v.num_tiles)) != NPS_OK) return rc;
+    launch_emit_keys(h->stream, n, (const uint32_t*)sc.counts.p, (const uint64_t*)sc.offsets.p,
+                     (const uint4*)sc.rects.p, v.tiles_x, (uint64_t*)sc.keys.p);
+    if (h->timing && ev_sort >= 0) NPS_HIP(h, hipEventRecord(h->ev[ev_sort], h->stream));
+    int tile_bits = bits_for(v.num_tiles > 0 ? v.num_tiles - 1 : 0);
+    sc.in_alt = radix_sort_u64(h->stream, (uint64_t*)sc.keys.p, (uint64_t*)sc.keys_alt.p, nullptr, nullptr, m, 32,
+                               32 + tile_bits, sc.sort_tmp.p);
+    if (h->timing && ev_ranges >= 0) NPS_HIP(h, hipEventRecord(h->ev[ev_ranges], h->stream));
+    launch_fill_u32(h->stream, (uint32_t*)sc.ranges.p, 0u, 2 * (uint64_t)v.num_tiles);
+    const uint64_t* sorted = (const uint64_t*)(sc.in_alt ? sc.keys_alt.p : sc.keys.p);
+    launch_tile_ranges(h->stream, sorted, m, (uint32_t*)sc.ranges.p);
+    NPS_HIP(h, hipGetLastError());
+    return NPS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nps_config_default(nps_config* cfg) {
+    if (!cfg) return NPS_ERR_ARG;
+    memset(cfg, 0, sizeof(*cfg));
+    cfg->max_tile_splats = 1024;
+    cfg->adc_period = 25;
+    cfg->adc_enabled = 1;
+    cfg->device = 0;
+    return NPS_OK;
+}
+
+int nps_create(const nps_config* cfg, int32_t num_splats, nps_handle* out) {
+    if (!out || num_splats < 0) return NPS_ERR_ARG;
+    *out = nullptr;
+    nps_ctx* h = new nps_ctx();
+    if (cfg) h->cfg = *cfg;
+    else nps_config_default(&h->cfg);
+    if (h->cfg.max_tile_splats <= 0 || h->cfg.adc_period <= 0) {
+        delete h;
+        return NPS_ERR_ARG;
+    }
+    h->device = h->cfg.device;
+    h->n = (uint32_t)num_splats;
+    auto fail = [&](hipError_t e) {
+        (void)e;
+        if (h->splats) (void)hipFree(h->splats);
+        if (h->grads) (void)hipFree(h->grads);
+        if (h->m) (void)hipFree(h->m);
+        if (h->v) (void)hipFree(h->v);
+        if (h->accum) (void)hipFree(h->accum);
+        if (h->stream) (void)hipStreamDestroy(h->stream);
+        delete h;
+        return NPS_ERR_HIP;
+    };
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return fail(e);
+    if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e);
+    size_t n16 = 16 * (size_t)std::max<uint32_t>(h->n, 1);
+    if ((e = hipMalloc((void**)&h->splats, n16 * 4)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc((void**)&h->grads, n16 * 4)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc((void**)&h->m, n16 * 4)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc((void**)&h->v, n16 * 4)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc((void**)&h->accum, (n16 / 16) * 4)) != hipSuccess) return fail(e);
+    hipMemsetAsync(h->splats, 0, n16 * 4, h->stream);
+    hipMemsetAsync(h->grads, 0, n16 * 4, h->stream);
+    hipMemsetAsync(h->m, 0, n16 * 4, h->stream);
+    hipMemsetAsync(h->v, 0, n16 * 4, h->stream);
+    hipMemsetAsync(h->accum, 0, (n16 / 16) * 4, h->stream);
+    for (int k = 0; k < 8; ++k)
+        if ((e = hipEventCreate(&h->ev[k])) != hipSuccess) return fail(e);
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return fail(e);
+    *out = h;
+    return NPS_OK;
+}
+
+void nps_destroy(nps_handle h) {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(h->splats); (void)hipFree(h->grads); (void)hipFree(h->m); (void)hipFree(h->v);
+    (void)hipFree(h->accum);
+    free_buf(h->tgt_rgba); free_buf(h->tgt_depth); free_buf(h->fwd); free_buf(h->bwd); free_buf(h->ckpt);
+    free_buf(h->c_img); free_buf(h->d_img); free_buf(h->tile_loss); free_buf(h->stats);
+    free_buf(h->adc_u32); free_buf(h->adc_tmp); free_buf(h->adc_scan);
+    free_scratch(h->sc);
+    for (int k = 0; k < 8; ++k)
+        if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+const char* nps_last_error(nps_handle h) {
+    if (!h) return "invalid handle";
+    return h->err.c_str();
+}
+
+int32_t nps_num_splats(nps_handle h) { return h ? (int32_t)h->n : -1; }
+
+static int check_n(nps_handle h, int32_t n) {
+    if (!h) return NPS_ERR_ARG;
+    if (n != (int32_t)h->n) return set_err(h, NPS_ERR_ARG, "n must equal num_splats");
+    return NPS_OK;
+}
+
+#define NPS_COPY(h, dst, src, bytes, kind)                                                       \
+    do {                                                                                        \
+        NPS_HIP(h, hipSetDevice(h->device));                                                    \
+        NPS_HIP(h, hipMemcpyAsync(dst, src, bytes, kind, h->stream));                           \
+        NPS_HIP(h, hipStreamSynchronize(h->stream));                                            \
+    } while (0)
+
+int nps_write_splats(nps_handle h, const float* s, int32_t n) {
+    int rc = check_n(h, n);
+    if (rc) return rc;
+    if (!s && n) return set_err(h, NPS_ERR_ARG, "null pointer");
+    NPS_COPY(h, h->splats, s, 64 * (size_t)n, hipMemcpyHostToDevice);
+    return NPS_OK;
+}
+int nps_read_splats(nps_handle h, float* s, int32_t n) {
+    int rc = check_n(h, n);
+    if (rc) return rc;
+    if (!s && n) return set_err(h, NPS_ERR_ARG, "null pointer");
+    NPS_COPY(h, s, h->splats, 64 * (size_t)n, hipMemcpyDeviceToHost);
+    return NPS_OK;
+}
+int nps_write_grads(nps_handle h, const int32_t* g, int32_t n) {
+    int rc = check_n(h, n);
+    if (rc) return rc;
+    if (!g && n) return set_err(h, NPS_ERR_ARG, "null pointer");
+    NPS_COPY(h, h->grads, g, 64 * (size_t)n, hipMemcpyHostToDevice);
+    return NPS_OK;
+}
+int nps_read_grads(nps_handle h, int32_t* g, int32_t n) {
+    int rc = check_n(h, n);
+    if (rc) return rc;
+    if (!g && n) return set_err(h, NPS_ERR_ARG, "null pointer");
+    NPS_COPY(h, g, h->grads, 64 * (size_t)n, hipMemcpyDeviceToHost);
+    return NPS_OK;
+}
+int nps_write_adam(nps_handle h, const float* m, const float* v, float t, int32_t n) {
+    int rc = check_n(h, n);
+    if (rc) return rc;
+    if ((!m || !v) && n) return set_err(h, NPS_ERR_ARG, "null pointer");
+    NPS_COPY(h, h->m, m, 64 * (size_t)n, hipMemcpyHostToDevice);
+    NPS_COPY(h, h->v, v, 64 * (size_t)n, hipMemcpyHostToDevice);
+    h->adam_t = t;
+    return NPS_OK;
+}
+int nps_read_adam(nps_handle h, float* m, float* v, float* t, int32_t n) {
+    int rc = check_n(h, n);
+    if (rc) return rc;
+    if (m) NPS_COPY(h, m, h->m, 64 * (size_t)n, hipMemcpyDeviceToHost);
+    if (v) NPS_COPY(h, v, h->v, 64 * (size_t)n, hipMemcpyDeviceToHost);
+    if (t) *t = h->adam_t;
+    return NPS_OK;
+}
+int nps_write_accum(nps_handle h, const float* a, int32_t n) {
+    int rc = check_n(h, n);
+    if (rc) return rc;
+    if (!a && n) return set_err(h, NPS_ERR_ARG, "null pointer");
+    NPS_COPY(h, h->accum, a, 4 * (size_t)n, hipMemcpyHostToDevice);
+    return NPS_OK;
+}
+int nps_read_accum(nps_handle h, float* a, int32_t n) {
+    int rc = check_n(h, n);
+    if (rc) return rc;
+    if (!a && n) return set_err(h, NPS_ERR_ARG, "null pointer");
+    NPS_COPY(h, a, h->accum, 4 * (size_t)n, hipMemcpyDeviceToHost);
+    return NPS_OK;
+}
+
+int nps_set_camera(nps_handle h, const float vp[16], const float vp_inv[16], int32_t blur_enabled) {
+    if (!h || !vp || !vp_inv) return h ? set_err(h, NPS_ERR_ARG, "null matrix") : NPS_ERR_ARG;
+    memcpy(h->vp, vp, sizeof(h->vp));
+    memcpy(h->vp_inv, vp_inv, sizeof(h->vp_inv));
+    h->blur = blur_enabled;
+    h->has_camera = true;
+    return NPS_OK;
+}
+
+static int set_target_impl(nps_handle h, const float* rgba, const float* depth, int32_t w, int32_t hh,
+                           hipMemcpyKind kind) {
+    if (!h) return NPS_ERR_ARG;
+    if (w <= 0 || hh <= 0 || !rgba || !depth) return set_err(h, NPS_ERR_ARG, "bad target");
+    size_t px = (size_t)w * (size_t)hh;
+    int rc;
+    NPS_HIP(h, hipSetDevice(h->device));
+    if ((rc = ensure(h, h->tgt_rgba, 16 * px)) != NPS_OK) return rc;
+    if ((rc = ensure(h, h->tgt_depth, 4 * px)) != NPS_OK) return rc;
+    NPS_HIP(h, hipMemcpyAsync(h->tgt_rgba.p, rgba, 16 * px, kind, h->stream));
+    NPS_HIP(h, hipMemcpyAsync(h->tgt_depth.p, depth, 4 * px, kind, h->stream));
+    NPS_HIP(h, hipStreamSynchronize(h->stream));
+    h->W = (uint32_t)w;
+    h->H = (uint32_t)hh;
+    h->has_target = true;
+    return NPS_OK;
+}
+
+int nps_set_target(nps_handle h, const float* rgba, const float* depth, int32_t w, int32_t hh) {
+    return set_target_impl(h, rgba, depth, w, hh, hipMemcpyHostToDevice);
+}
+int nps_set_target_dev(nps_handle h, const float* rgba, const float* depth, int32_t w, int32_t hh) {
+    return set_target_impl(h, rgba, depth, w, hh, hipMemcpyDeviceToDevice);
+}
+
+int nps_backward(nps_handle h) {
+    if (!h) return NPS_ERR_ARG;
+    if (!h->has_target) return set_err(h, NPS_ERR_STATE, "no backward target (nps_set_target)");
+    if (!h->has_camera) return set_err(h, NPS_ERR_STATE, "no camera (nps_set_camera)");
+    NPS_HIP(h, hipSetDevice(h->device));
+    const uint32_t n = h->n;
+    ViewParams v = make_view(h->vp, h->vp_inv, h->W, h->H);
+    Scratch& sc = h->sc;
+    int rc;
+    if ((rc = ensure(h, h->fwd, sizeof(FwdRec) * (size_t)std::max<uint32_t>(n, 1))) != NPS_OK) return rc;
+    if ((rc = ensure(h, h->bwd, sizeof(BwdRec) * (size_t)std::max<uint32_t>(n, 1))) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.counts, 4 * (size_t)std::max<uint32_t>(n, 1))) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.rects, 16 * (size_t)std::max<uint32_t>(n, 1))) != NPS_OK) return rc;
+    if ((rc = ensure(h, h->stats, 64)) != NPS_OK) return rc;
+    if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[0], h->stream));
+    launch_fill_u32(h->stream, (uint32_t*)h->stats.p, 0u, 16);
+    launch_preprocess(h->stream, h->splats, n, v, (FwdRec*)h->fwd.p, (BwdRec*)h->bwd.p, (uint32_t*)sc.counts.p,
+                      (uint4*)sc.rects.p);
+    if ((rc = build_tile_lists(h, sc, v, n, 1, 2)) != NPS_OK) return rc;
+    const uint32_t cap = (uint32_t)h->cfg.max_tile_splats;
+    const uint32_t nchunk = (cap + kChunk - 1) / kChunk;
+    size_t px = (size_t)h->W * h->H;
+    if ((rc = ensure(h, h->ckpt, sizeof(float) * (size_t)v.num_tiles * nchunk * kTilePx)) != NPS_OK) return rc;
+    if ((rc = ensure(h, h->c_img, 12 * px)) != NPS_OK) return rc;
+    if ((rc = ensure(h, h->d_img, 4 * px)) != NPS_OK) return rc;
+    if ((rc = ensure(h, h->tile_loss, 16 * (size_t)std::max<uint32_t>(v.num_tiles, 1))) != NPS_OK) return rc;
+    if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[3], h->stream));
+    CompositeArgs a;
+    a.fwd = (const FwdRec*)h->fwd.p;
+    a.bwd = (const BwdRec*)h->bwd.p;
+    a.keys = (const uint64_t*)(sc.in_alt ? sc.keys_alt.p : sc.keys.p);
+    a.ranges = (const uint32_t*)sc.ranges.p;
+    a.tgt_rgba = (const float4*)h->tgt_rgba.p;
+    a.tgt_depth = (const float*)h->tgt_depth.p;
+    a.grads = h->grads;
+    a.c_img = (float*)h->c_img.p;
+    a.d_img = (float*)h->d_img.p;
+    a.tile_loss = (double*)h->tile_loss.p;
+    a.ckpt = (float*)h->ckpt.p;
+    a.stats = (unsigned long long*)h->stats.p;
+    a.cap = cap;
+    a.nchunk_max = nchunk;
+    launch_composite_ref(h->stream, v, a);
+    if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[4], h->stream));
+    NPS_HIP(h, hipGetLastError());
+    h->last_tiles = v.num_tiles;
+    h->have_backward = true;
+    h->stats_host[0] = (int64_t)sc.m;
+    if (h->timing) {
+        NPS_HIP(h, hipEventSynchronize(h->ev[4]));
+        float t;
+        hipEventElapsedTime(&t, h->ev[0], h->ev[1]); h->timing_ms[0] = t;
+        hipEventElapsedTime(&t, h->ev[1], h->ev[2]); h->timing_ms[1] = t;
+        hipEventElapsedTime(&t, h->ev[2], h->ev[3]); h->timing_ms[2] = t;
+        hipEventElapsedTime(&t, h->ev[3], h->ev[4]); h->timing_ms[3] = t;
+        hipEventElapsedTime(&t, h->ev[0], h->ev[4]); h->timing_ms[6] = t;
+    }
+    return NPS_OK;
+}
+
+int nps_step(nps_handle h) {
+    if (!h) return NPS_ERR_ARG;
+    NPS_HIP(h, hipSetDevice(h->device));
+    float t = h->adam_t + 1.0f;
+    if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[5], h->stream));
+    launch_adam_ref(h->stream, h->splats, h->grads, h->m, h->v, h->accum, h->n, t);
+    if (h->timing) {
+        NPS_HIP(h, hipEventRecord(h->ev[6], h->stream));
+        NPS_HIP(h, hipEventSynchronize(h->ev[6]));
+        hipEventElapsedTime(&h->timing_ms[4], h->ev[5], h->ev[6]);
+    }
+    NPS_HIP(h, hipGetLastError());
+    h->adam_t = t;
+    return NPS_OK;
+}
+
+int nps_adc(nps_handle h) {
+    if (!h) return NPS_ERR_ARG;
+    NPS_HIP(h, hipSetDevice(h->device));
+    const uint32_t n = h->n;
+    int rc;
+    size_t nn = (size_t)std::max<uint32_t>(n, 1);
+    // alive0 qa dead dead_rank(n+1) dead_list Q K(n+1) qual_list flag(4)
+    if ((rc = ensure(h, h->adc_u32, 4 * (8 * nn + 2 + 4 + 64))) != NPS_OK) return rc;
+    if ((rc = ensure(h, h->adc_tmp, 128 * nn)) != NPS_OK) return rc;
+    if ((rc = ensure(h, h->adc_scan, scan_temp_bytes(n) + 64)) != NPS_OK) return rc;
+    uint32_t* base = (uint32_t*)h->adc_u32.p;
+    AdcBuffers b;
+    b.alive0 = base;
+    b.qa = base + nn;
+    b.dead = base + 2 * nn;
+    b.dead_rank = base + 3 * nn;        // n + 1
+    b.dead_list = base + 4 * nn + 1;
+    b.Q = base + 5 * nn + 1;
+    b.K = base + 6 * nn + 1;            // n + 1
+    b.qual_list = base + 7 * nn + 2;
+    b.flag = base + 8 * nn + 2;
+    b.tmp = (float*)h->adc_tmp.p;
+    b.scan_tmp = h->adc_scan.p;
+    int64_t st[5];
+    if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[5], h->stream));
+    int r = run_adc_ref(h->stream, h->splats, h->m, h->v, h->accum, n, h->adam_t, b, st);
+    if (h->timing) {
+        NPS_HIP(h, hipEventRecord(h->ev[6], h->stream));
+        NPS_HIP(h, hipEventSynchronize(h->ev[6]));
+        hipEventElapsedTime(&h->timing_ms[5], h->ev[5], h->ev[6]);
+    }
+    NPS_HIP(h, hipGetLastError());
+    for (int k = 0; k < 5; ++k) h->stats_host[3 + k] = st[k];
+    if (r != 0) return set_err(h, NPS_ERR_CAPACITY, "ADC cascade chain deeper than 256");
+    return NPS_OK;
+}
+
+int nps_dispatch(nps_handle h) {
+    if (!h) return NPS_ERR_ARG;
+    int rc = nps_backward(h);
+    if (rc) return rc;
+    if ((rc = nps_step(h)) != NPS_OK) return rc;
+    h->step_count++;
+    if (h->cfg.adc_enabled && h->step_count % h->cfg.adc_period == 0) return nps_adc(h);
+    return NPS_OK;
+}
+
+int nps_sync(nps_handle h) {
+    if (!h) return NPS_ERR_ARG;
+    NPS_HIP(h, hipSetDevice(h->device));
+    NPS_HIP(h, hipStreamSynchronize(h->stream));
+    return NPS_OK;
+}
+
+int nps_read_image(nps_handle h, float* c_pred, float* d_pred) {
+    if (!h) return NPS_ERR_ARG;
+    if (!h->have_backward) return set_err(h, NPS_ERR_STATE, "no backward has run");
+    size_t px = (size_t)h->W * h->H;
+    if (c_pred) NPS_COPY(h, c_pred, h->c_img.p, 12 * px, hipMemcpyDeviceToHost);
+    if (d_pred) NPS_COPY(h, d_pred, h->d_img.p, 4 * px, hipMemcpyDeviceToHost);
+    return NPS_OK;
+}
+
+int nps_read_loss(nps_handle h, double loss[2]) {
+    if (!h || !loss) return NPS_ERR_ARG;
+    if (!h->have_backward) return set_err(h, NPS_ERR_STATE, "no backward has run");
+    std::vector<double> t(2 * (size_t)h->last_tiles);
+    NPS_COPY(h, t.data(), h->tile_loss.p, sizeof(double) * t.size(), hipMemcpyDeviceToHost);
+    double a = 0.0, b = 0.0;
+    for (size_t k = 0; k < h->last_tiles; ++k) {
+        a += t[2 * k];
+        b += t[2 * k + 1];
+    }
+    loss[0] = a;
+    loss[1] = b;
+    return NPS_OK;
+}
+
+int nps_read_tiles(nps_handle h, uint32_t* ranges, uint32_t* ids, int32_t* num_tiles, int64_t* num_ids) {
+    if (!h) return NPS_ERR_ARG;
+    if (!h->have_backward) return set_err(h, NPS_ERR_STATE, "no backward has run");
+    if (num_tiles) *num_tiles = (int32_t)h->last_tiles;
+    if (num_ids) *num_ids = (int64_t)h->sc.m;
+    if (ranges) NPS_COPY(h, ranges, h->sc.ranges.p, 8 * (size_t)h->last_tiles, hipMemcpyDeviceToHost);
+    if (ids && h->sc.m) {
+        std::vector<uint64_t> k(h->sc.m);
+        NPS_COPY(h, k.data(), h->sc.in_alt ? h->sc.keys_alt.p : h->sc.keys.p, 8 * (size_t)h->sc.m,
+                 hipMemcpyDeviceToHost);
+        for (size_t j = 0; j < k.size(); ++j) ids[j] = (uint32_t)k[j];
+    }
+    return NPS_OK;
+}
+
+int nps_read_stats(nps_handle h, int64_t stats[8]) {
+    if (!h || !stats) return NPS_ERR_ARG;
+    if (h->have_backward && h->stats.p) {
+        unsigned long long s[2] = {0, 0};
+        NPS_COPY(h, s, h->stats.p, sizeof(s), hipMemcpyDeviceToHost);
+        h->stats_host[2] = (int64_t)s[0];
+        h->stats_host[1] = (int64_t)s[1];
+    }
+    for (int k = 0; k < 8; ++k) stats[k] = h->stats_host[k];
+    return NPS_OK;
+}
+
+int nps_enable_timing(nps_handle h, int32_t on) {
+    if (!h) return NPS_ERR_ARG;
+    h->timing = on != 0;
+    return NPS_OK;
+}
+
+int nps_read_timing(nps_handle h, float ms[8]) {
+    if (!h || !ms) return NPS_ERR_ARG;
+    for (int k = 0; k < 8; ++k) ms[k] = h->timing_ms[k];
+    return NPS_OK;
+}
+
+// ---------------- forward render manager ----------------
+
+int npf_create(nps_handle owner, npf_handle* out) {
+    if (!owner || !out) return NPS_ERR_ARG;
+    npf_ctx* f = new npf_ctx();
+    f->owner = owner;
+    *out = f;
+    return NPS_OK;
+}
+
+void npf_destroy(npf_handle f) {
+    if (!f) return;
+    (void)hipSetDevice(f->owner->device);
+    (void)hipStreamSynchronize(f->owner->stream);
+    free_buf(f->color); free_buf(f->depth); free_buf(f->rec);
+    free_scratch(f->sc);
+    delete f;
+}
+
+int npf_write_vp(npf_handle f, const float vp[16]) {
+    if (!f || !vp) return NPS_ERR_ARG;
+    memcpy(f->vp, vp, sizeof(f->vp));
+    f->has_vp = true;
+    return NPS_OK;
+}
+
+int npf_set_target(npf_handle f, int32_t w, int32_t hh) {
+    if (!f) return NPS_ERR_ARG;
+    nps_ctx* h = f->owner;
+    if (w <= 0 || hh <= 0) return set_err(h, NPS_ERR_ARG, "bad render target size");
+    NPS_HIP(h, hipSetDevice(h->device));
+    size_t px = (size_t)w * hh;
+    int rc;
+    if ((rc = ensure(h, f->color, 16 * px)) != NPS_OK) return rc;
+    if ((rc = ensure(h, f->depth, 16 * px)) != NPS_OK) return rc;
+    f->W = (uint32_t)w;
+    f->H = (uint32_t)hh;
+    return NPS_OK;
+}
+
+int npf_dispatch(npf_handle f, int32_t clear, int32_t quantize) {
+    if (!f) return NPS_ERR_ARG;
+    nps_ctx* h = f->owner;
+    if (!f->has_vp || f->W == 0) return set_err(h, NPS_ERR_STATE, "render target or VP not set");
+    NPS_HIP(h, hipSetDevice(h->device));
+    const uint32_t n = h->n;
+    ViewParams v = make_view(f->vp, nullptr, f->W, f->H);
+    int rc;
+    if ((rc = ensure(h, f->rec, 64 * (size_t)std::max<uint32_t>(n, 1))) != NPS_OK) return rc;
+    if ((rc = ensure(h, f->sc.counts, 4 * (size_t)std::max<uint32_t>(n, 1))) != NPS_OK) return rc;
+    if ((rc = ensure(h, f->sc.rects, 16 * (size_t)std::max<uint32_t>(n, 1))) != NPS_OK) return rc;
+    launch_preprocess_render(h->stream, h->splats, n, v, (float4*)f->rec.p, (uint32_t*)f->sc.counts.p,
+                             (uint4*)f->sc.rects.p);
+    if ((rc = build_tile_lists(h, f->sc, v, n, -1, -1)) != NPS_OK) return rc;
+    RenderArgs a;
+    a.rec = (const float4*)f->rec.p;
+    a.keys = (const uint64_t*)(f->sc.in_alt ? f->sc.keys_alt.p : f->sc.keys.p);
+    a.ranges = (const uint32_t*)f->sc.ranges.p;
+    a.color = (float4*)f->color.p;
+    a.depth = (float4*)f->depth.p;
+    a.clear = clear != 0;
+    a.quantize = quantize != 0;
+    launch_render(h->stream, v, a);
+    NPS_HIP(h, hipGetLastError());
+    return NPS_OK;
+}
+
+int npf_read_target(npf_handle f, float* color, float* depth) {
+    if (!f) return NPS_ERR_ARG;
+    nps_ctx* h = f->owner;
+    size_t px = (size_t)f->W * f->H;
+    if (color) NPS_COPY(h, color, f->color.p, 16 * px, hipMemcpyDeviceToHost);
+    if (depth) NPS_COPY(h, depth, f->depth.p, 16 * px, hipMemcpyDeviceToHost);
+    return NPS_OK;
+}
+
+int npf_write_target(npf_handle f, const float* color, const float* depth) {
+    if (!f) return NPS_ERR_ARG;
+    nps_ctx* h = f->owner;
+    size_t px = (size_t)f->W * f->H;
+    if (color) NPS_COPY(h, f->color.p, color, 16 * px, hipMemcpyHostToDevice);
+    if (depth) NPS_COPY(h, f->depth.p, depth, 16 * px, hipMemcpyHostToDevice);
+    return NPS_OK;
+}
+
+int nps_debug_math(int32_t device, int32_t op, const float* x, const float* y, float* out, int32_t n) {
+    if (!x || !out || n < 0 || (op == 2 && !y)) return NPS_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return NPS_ERR_HIP;
+    float *dx = nullptr, *dy = nullptr, *dout = nullptr;
+    size_t bytes = 4 * (size_t)std::max(n, 1);
+    int rc = NPS_OK;
+    if (hipMalloc(&dx, bytes) != hipSuccess || hipMalloc(&dy, bytes) != hipSuccess ||
+        hipMalloc(&dout, bytes) != hipSuccess) {
+        rc = NPS_ERR_HIP;
+    } else {
+        hipMemcpy(dx, x, 4 * (size_t)n, hipMemcpyHostToDevice);
+        if (y) hipMemcpy(dy, y, 4 * (size_t)n, hipMemcpyHostToDevice);
+        launch_debug_math(nullptr, op, dx, dy, dout, (uint32_t)n);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = NPS_ERR_HIP;
+        else hipMemcpy(out, dout, 4 * (size_t)n, hipMemcpyDeviceToHost);
+    }
+    if (dx) hipFree(dx);
+    if (dy) hipFree(dy);
+    if (dout) hipFree(dout);
+    return rc;
+}
+
+}  // extern "C"

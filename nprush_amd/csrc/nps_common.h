@@ -1,0 +1,137 @@
+// nps_common.h -- shared device-side definitions of the gfx950 splat path.
+//
+// Geometry helpers restate the reference's WGSL with a fixed evaluation order
+// (left-associative operators, no contraction; built with -ffp-contract=off):
+//   M * v      = ((c0*v.x + c1*v.y) + c2*v.z) + c3*v.w   (column-major M)
+//   dot(a, b)  = (a.x*b.x + a.y*b.y) + a.z*b.z
+// so that every value a kernel produces equals the CPU restatement bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nps_math.h"
+
+namespace nps {
+
+constexpr int kTile = 16;        // reference workgroup tile, splat_backward.wgsl:136
+constexpr int kTilePx = kTile * kTile;
+constexpr int kBatch = 64;       // splat records staged in LDS per batch
+constexpr int kChunk = 16;       // transmittance checkpoint stride (entries)
+
+// Per-view constants, passed by value to every kernel.
+struct ViewParams {
+    float vp[16];      // column-major view-projection (SplatUniforms.vp)
+    float vp_inv[16];  // its inverse (SplatUniforms.vp_inv)
+    uint32_t W, H;     // target dimensions
+    float aspect;      // f32(W) / f32(H)
+    uint32_t tiles_x, tiles_y, num_tiles;
+    uint32_t pad[2];
+};
+
+// Per-splat forward record (64 B): everything the per-pair forward needs.
+struct alignas(16) FwdRec {
+    float scr_x, scr_y;   // projected centre, aspect-corrected NDC
+    float A, B, C;        // conic = inverse 2D covariance
+    float op;             // opacity
+    float r, g, b;        // colour
+    float sa, sb;         // shape a, shape b
+    float depth;          // clamp(1 - 0.1/max(w,0.1), 0, 1)
+    float pad[4];
+};
+
+// Per-splat backward record (128 B).
+struct alignas(16) BwdRec {
+    float m[6];     // m0x m0y m1x m1y m2x m2y   (projected axis * scale)
+    float axs[6];   // ax_s.xy ay_s.xy az_s.xy   (projected unit axes)
+    float s3[3];    // max(s, 1e-4) per axis
+    float J0[3];    // d screen.x / d pos_j = aspect*(vp[j][0] w - clip.x vp[j][3]) / w^2
+    float J1[3];    // d screen.y / d pos_j = (vp[j][1] w - clip.y vp[j][3]) / w^2
+    float vpd[3];   // vp[j][3] * (0.1 / (w*w))
+    float pos[3];   // splat centre
+    float q[4];     // qw qx qy qz
+    float w;        // clip.w
+};
+
+static_assert(sizeof(FwdRec) == 64, "FwdRec layout");
+static_assert(sizeof(BwdRec) == 128, "BwdRec layout");
+
+struct F4 {
+    float x, y, z, w;
+};
+struct F3 {
+    float x, y, z;
+};
+struct F2 {
+    float x, y;
+};
+
+NPS_HD F4 mat_vec(const float* m, float x, float y, float z, float w) {
+    F4 o;
+    o.x = ((m[0] * x + m[4] * y) + m[8] * z) + m[12] * w;
+    o.y = ((m[1] * x + m[5] * y) + m[9] * z) + m[13] * w;
+    o.z = ((m[2] * x + m[6] * y) + m[10] * z) + m[14] * w;
+    o.w = ((m[3] * x + m[7] * y) + m[11] * z) + m[15] * w;
+    return o;
+}
+
+// splat_backward.wgsl:34-40, one coordinate at a time.
+NPS_HD float px_to_x(uint32_t px, uint32_t W, float aspect) {
+    float u = ((float)px + 0.5f) / (float)W;
+    float x = u * 2.0f - 1.0f;
+    return x * aspect;
+}
+NPS_HD float py_to_y(uint32_t py, uint32_t H) {
+    float v = ((float)py + 0.5f) / (float)H;
+    float y = v * 2.0f - 1.0f;
+    return -y;
+}
+
+NPS_HD F3 cross3(F3 a, F3 b) {
+    return F3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// splat_backward.wgsl:42-45
+NPS_HD F3 quat_rotate(float qw, float qx, float qy, float qz, F3 v) {
+    F3 u{qx, qy, qz};
+    F3 c = cross3(u, v);
+    F3 t{2.0f * c.x, 2.0f * c.y, 2.0f * c.z};
+    F3 c2 = cross3(u, t);
+    return F3{(v.x + qw * t.x) + c2.x, (v.y + qw * t.y) + c2.y, (v.z + qw * t.z) + c2.z};
+}
+
+// splat_backward.wgsl:52-58
+NPS_HD F2 project_axis(const float* vp, F3 ax, float cx, float cy, float w, float aspect) {
+    F4 ac = mat_vec(vp, ax.x, ax.y, ax.z, 0.0f);
+    F2 o;
+    o.x = (ac.x * w - cx * ac.w) / (w * w) * aspect;
+    o.y = (ac.y * w - cy * ac.w) / (w * w);
+    return o;
+}
+
+// Per-pair geometry shared by forward and backward: d = p - centre,
+// sp = conic * d, r = sqrt(max(d.sp, 1e-4)) (splat_backward.wgsl:98,127-132).
+struct PairGeom {
+    float dx, dy, spx, spy, r, log2r, rpa, power;
+};
+
+__device__ __forceinline__ PairGeom pair_geom(const FwdRec& f, float px, float py) {
+    PairGeom g;
+    g.dx = px - f.scr_x;
+    g.dy = py - f.scr_y;
+    g.spx = f.A * g.dx + f.B * g.dy;
+    g.spy = f.B * g.dx + f.C * g.dy;
+    float r2 = g.dx * g.spx + g.dy * g.spy;
+    g.r = sqrtf(fmax_w(r2, 0.0001f));
+    g.log2r = log2_det(g.r);
+    g.rpa = exp2_det(f.sa * g.log2r);  // pow(r, shape_a)
+    g.power = -f.sb * g.rpa;
+    return g;
+}
+
+// alpha = clamp(select(0, exp(power)*opacity, power > -15), 0, 0.999)  (BW:236-237)
+__device__ __forceinline__ float pair_alpha(const FwdRec& f, const PairGeom& g) {
+    float a = (g.power > -15.0f) ? exp_det(g.power) * f.op : 0.0f;
+    return clamp_w(a, 0.0f, 0.999f);
+}
+
+}  // namespace nps

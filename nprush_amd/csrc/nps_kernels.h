@@ -1,0 +1,87 @@
+// nps_kernels.h -- host-side launchers of the gfx950 kernels (one translation
+// unit per subsystem; every launcher enqueues on the given stream).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nps_common.h"
+
+namespace nps {
+
+struct CompositeArgs {
+    const FwdRec* fwd;
+    const BwdRec* bwd;
+    const uint64_t* keys;     // sorted (tile << 32 | splat id)
+    const uint32_t* ranges;   // [2 * num_tiles]
+    const float4* tgt_rgba;   // H*W
+    const float* tgt_depth;   // H*W
+    int32_t* grads;           // 16N fixed point
+    float* c_img;             // H*W*3
+    float* d_img;             // H*W
+    double* tile_loss;        // 2 * num_tiles
+    float* ckpt;              // num_tiles * nchunk_max * 256
+    unsigned long long* stats;  // [0] pairs, [1] tiles over the cap
+    uint32_t cap;
+    uint32_t nchunk_max;
+};
+
+// k_bin.hip
+void launch_preprocess(hipStream_t st, const float* splats, uint32_t n, const ViewParams& v, FwdRec* fwd,
+                       BwdRec* bwd, uint32_t* counts, uint4* rects);
+void launch_emit_keys(hipStream_t st, uint32_t n, const uint32_t* counts, const uint64_t* offsets,
+                      const uint4* rects, uint32_t tiles_x, uint64_t* keys);
+void launch_tile_ranges(hipStream_t st, const uint64_t* keys, uint64_t m, uint32_t* ranges);
+void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint64_t n);
+
+// k_sort.hip
+size_t scan_temp_bytes(uint64_t n);
+void exclusive_scan_u32_to_u64(hipStream_t st, const uint32_t* in, uint64_t* out, uint64_t n, void* temp);
+void exclusive_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t n, void* temp);
+size_t radix_temp_bytes(uint64_t m);
+bool radix_sort_u64(hipStream_t st, uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt,
+                    uint64_t m, int bit_lo, int bit_hi, void* temp);
+
+// k_composite.hip
+void launch_composite_ref(hipStream_t st, const ViewParams& v, const CompositeArgs& a);
+
+// k_step.hip
+void launch_adam_ref(hipStream_t st, float* splats, int32_t* grads, float* m, float* v, float* accum, uint32_t n,
+                     float t);
+
+// k_adc.hip
+struct AdcBuffers {
+    uint32_t* alive0;     // n
+    uint32_t* qa;         // n
+    uint32_t* dead;       // n
+    uint32_t* dead_rank;  // n + 1
+    uint32_t* dead_list;  // n
+    uint32_t* Q;          // n
+    uint32_t* K;          // n + 1
+    uint32_t* qual_list;  // n
+    float* tmp;           // 32 * n
+    uint32_t* flag;       // 4
+    void* scan_tmp;
+};
+// Runs the whole ADC; returns 0, or -1 when a cascade chain exceeds the depth
+// bound.  stats: qualifiers, children, splits, clones, rounds.
+int run_adc_ref(hipStream_t st, float* splats, float* m, float* v, float* accum, uint32_t n, float adam_t,
+                const AdcBuffers& b, int64_t stats[5]);
+
+// k_render.hip
+struct RenderArgs {
+    const float4* rec;        // 4 float4 per splat
+    const uint64_t* keys;
+    const uint32_t* ranges;
+    float4* color;            // H*W
+    float4* depth;            // H*W
+    int clear;
+    int quantize;
+};
+void launch_preprocess_render(hipStream_t st, const float* splats, uint32_t n, const ViewParams& v, float4* rec,
+                              uint32_t* counts, uint4* rects);
+void launch_render(hipStream_t st, const ViewParams& v, const RenderArgs& a);
+
+// k_debug.hip
+void launch_debug_math(hipStream_t st, int op, const float* x, const float* y, float* out, uint32_t n);
+
+}  // namespace nps

@@ -1,0 +1,127 @@
+// nps_math.h -- deterministic FP32 transcendentals for the gfx950 kernels.
+//
+// WGSL leaves exp/pow/log/sin implementation-defined within ULP bounds
+// (pow is defined as exp2(y * log2(x))).  The kernels need a definition that
+// is identical on every device and on a CPU, so these are built only from
+// correctly rounded IEEE-754 +,-,*,/ (hipcc's default for gfx950: division and
+// sqrt are correctly rounded, fp32 denormals kept) and compiled with
+// -ffp-contract=off:
+//   exp2: Cody-Waite split x = k + f (|f| <= 1/2), degree-7 Taylor series of
+//         e^(f ln 2) in Horner form, exact scaling by 2^k; results below
+//         2^-126 flush to zero.
+//   log2: x = m 2^e, m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s),
+//         s = (m-1)/(m+1), odd series to s^11.
+//   pow(x,y) = exp2(y log2 x); log(x) = log2(x) ln 2; exp(x) = exp2(x log2 e).
+//   sin: double-precision Cody-Waite reduction by pi/2 (three-part constant,
+//        exact for |x| < 2^27) and Taylor polynomials, rounded to float.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#define NPS_HD __host__ __device__ __forceinline__
+
+namespace nps {
+
+constexpr float kLn2 = 0x1.62e430p-1f;
+constexpr float kLog2e = 0x1.715476p+0f;
+
+NPS_HD float exp2_det(float x) {
+    if (x != x) return x;
+    if (x >= 128.0f) return INFINITY;
+    if (x < -126.0f) return 0.0f;
+    float k = rintf(x);
+    float f = x - k;
+    float p = 0x1.ffcbfcp-17f;
+    p = p * f + 0x1.430912p-13f;
+    p = p * f + 0x1.5d87fep-10f;
+    p = p * f + 0x1.3b2ab6p-7f;
+    p = p * f + 0x1.c6b08ep-5f;
+    p = p * f + 0x1.ebfbe0p-3f;
+    p = p * f + 0x1.62e430p-1f;
+    p = p * f + 1.0f;
+    float r = ldexpf(p, (int)k);
+    return r < 0x1p-126f ? 0.0f : r;
+}
+
+NPS_HD float log2_det(float x) {
+    if (x != x) return x;
+    if (x < 0.0f) return NAN;
+    if (x == 0.0f) return -INFINITY;
+    if (x == INFINITY) return x;
+    int e;
+    float m = frexpf(x, &e);
+    if (m < 0x1.6a09e6p-1f) {
+        m = m * 2.0f;
+        e = e - 1;
+    }
+    float s = (m - 1.0f) / (m + 1.0f);
+    float s2 = s * s;
+    float p = 0x1.745d18p-4f;
+    p = p * s2 + 0x1.c71c72p-4f;
+    p = p * s2 + 0x1.24924ap-3f;
+    p = p * s2 + 0x1.99999ap-3f;
+    p = p * s2 + 0x1.555556p-2f;
+    float two_s = s * 2.0f;
+    float lnm = two_s + (two_s * s2) * p;
+    return (float)e + lnm * kLog2e;
+}
+
+NPS_HD float pow_det(float x, float y) { return exp2_det(y * log2_det(x)); }
+NPS_HD float log_det(float x) { return log2_det(x) * kLn2; }
+NPS_HD float exp_det(float x) { return exp2_det(x * kLog2e); }
+
+NPS_HD float sin_det(float xf) {
+    double x = (double)xf;
+    if (!(x - x == 0.0)) return NAN;
+    double k = rint(x * 0x1.45f306dc9c883p-1);
+    double r = x - k * 0x1.921fb5p+0;
+    r = r - k * 0x1.110b46p-26;
+    r = r - k * 0x1.1a62633145c07p-54;
+    int q = (int)(((int64_t)k) & 3);
+    double r2 = r * r;
+    double s = -0x1.ae7f3e733b81fp-41;
+    s = s * r2 + 0x1.6124613a86d09p-33;
+    s = s * r2 - 0x1.ae64567f544e4p-26;
+    s = s * r2 + 0x1.71de3a556c734p-19;
+    s = s * r2 - 0x1.a01a01a01a01ap-13;
+    s = s * r2 + 0x1.1111111111111p-7;
+    s = s * r2 - 0x1.5555555555555p-3;
+    double sinr = r + (r * r2) * s;
+    double c = 0x1.ae7f3e733b81fp-45;
+    c = c * r2 - 0x1.93974a8c07c9dp-37;
+    c = c * r2 + 0x1.1eed8eff8d898p-29;
+    c = c * r2 - 0x1.27e4fb7789f5cp-22;
+    c = c * r2 + 0x1.a01a01a01a01ap-16;
+    c = c * r2 - 0x1.6c16c16c16c17p-10;
+    c = c * r2 + 0x1.5555555555555p-5;
+    c = c * r2 - 0x1.0p-1;
+    double cosr = 1.0 + r2 * c;
+    double out = (q == 0) ? sinr : (q == 1) ? cosr : (q == 2) ? -sinr : -cosr;
+    return (float)out;
+}
+
+NPS_HD float fract_det(float x) { return x - floorf(x); }
+
+// WGSL i32(f32): truncation toward zero, saturating; NaN -> 0.
+NPS_HD int32_t f2i_sat(float x) {
+    if (x != x) return 0;
+    if (x >= 2147483648.0f) return INT32_MAX;
+    if (x <= -2147483648.0f) return INT32_MIN;
+    return (int32_t)x;
+}
+
+// WGSL max/min (IEEE maxNum: a NaN operand loses) and clamp = min(max(e, lo), hi).
+NPS_HD float fmax_w(float a, float b) {
+    if (a != a) return b;
+    if (b != b) return a;
+    return a > b ? a : b;
+}
+NPS_HD float fmin_w(float a, float b) {
+    if (a != a) return b;
+    if (b != b) return a;
+    return a < b ? a : b;
+}
+NPS_HD float clamp_w(float e, float lo, float hi) { return fmin_w(fmax_w(e, lo), hi); }
+
+}  // namespace nps

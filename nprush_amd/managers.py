@@ -1,0 +1,257 @@
+"""Host classes mirroring the reference's WebGPU pipeline managers.
+
+``GpuSplatOptimizerManager`` and ``GpuSplatForwardPipelineManager`` keep the
+reference's class and method names (src/lib/gpu/GpuSplatOptimizerManager.ts,
+src/lib/gpu/GpuSplatForwardPipelineManager.ts) and drive libnprush_hip.so
+through its C ABI.  WebGPU objects become host arrays: a texture view passed to
+``setBackwardTarget`` is an H x W x 4 float32 array (colour) plus an H x W
+float32 array (depth .r); ``device``/``format`` arguments are accepted and
+ignored.  Differences from the reference:
+
+* errors raise ``NpsError`` instead of WebGPU's asynchronous error scopes;
+* ``dispatch`` before ``setBackwardTarget`` raises (the reference returns
+  silently, GpuSplatOptimizerManager.ts:366);
+* splats are written explicitly (``write_splats``) instead of the reference's
+  unseeded Math.random() initialisation (GpuSplatOptimizerManager.ts:59-86);
+  ``reference_init`` reproduces that distribution with a seed.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr, fptr, iptr, lptr, uptr
+
+
+def reference_init(num_splats: int, seed: int = 0, live: int = 512) -> np.ndarray:
+    """GpuSplatOptimizerManager.ts:59-86 with a seeded generator: positions
+    U(-0.3, 0.3), scales 0.1 + U(0, 0.15), RGB U(0,1), opacity 0.3 + U(0, 0.4)
+    for the first `live` splats and 0 (dead) after, identity quaternion,
+    shape (2, 0.5)."""
+    rng = np.random.default_rng(seed)
+    s = np.zeros((num_splats, 16), dtype=np.float32)
+    for i in range(num_splats):
+        r = rng.random(11)
+        s[i, 0:3] = (r[0:3] * 2 - 1) * 0.3
+        s[i, 3] = 0.1 + r[3] * 0.15
+        s[i, 4:7] = r[4:7]
+        s[i, 7] = 0.3 + r[7] * 0.4 if i < live else 0.0
+        s[i, 8:12] = (1.0, 0.0, 0.0, 0.0)
+        s[i, 12] = 0.1 + r[8] * 0.15
+        s[i, 13] = 2.0
+        s[i, 14] = 0.5
+        s[i, 15] = 0.1 + r[9] * 0.15
+    return s
+
+
+class GpuSplatOptimizerManager:
+    """new GpuSplatOptimizerManager({device, format, numSplats}) (OPT:43-125)."""
+
+    def __init__(self, device=None, format=None, numSplats: int = 512, numBeziers=None, *,
+                 max_tile_splats: int = 1024, adc_period: int = 25, adc_enabled: bool = True, hip_device: int = 0,
+                 splats: np.ndarray | None = None):
+        lib = _lib.load()
+        cfg = _lib.NpsConfig()
+        check(lib.nps_config_default(C.byref(cfg)), None, "nps_config_default")
+        cfg.max_tile_splats = max_tile_splats
+        cfg.adc_period = adc_period
+        cfg.adc_enabled = 1 if adc_enabled else 0
+        cfg.device = hip_device
+        h = C.c_void_p()
+        check(lib.nps_create(C.byref(cfg), int(numSplats), C.byref(h)), None, "nps_create")
+        self._lib = lib
+        self._h = h
+        self.numSplats = int(numSplats)
+        self.numParams = self.numSplats * 16
+        self.dims = (0, 0)
+        if splats is not None:
+            self.write_splats(splats)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.nps_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- buffers (splatBuffer, gradBuffer, adamBuffer, adcBuffer; OPT:15-18) ----
+    def write_splats(self, splats: np.ndarray):
+        a = np.ascontiguousarray(splats, dtype=np.float32).reshape(self.numSplats, 16)
+        check(self._lib.nps_write_splats(self._h, fptr(a), self.numSplats), self._h, "write_splats")
+
+    def read_splats(self) -> np.ndarray:
+        a = np.empty((self.numSplats, 16), dtype=np.float32)
+        check(self._lib.nps_read_splats(self._h, fptr(a), self.numSplats), self._h, "read_splats")
+        return a
+
+    def write_grads(self, grads: np.ndarray):
+        a = np.ascontiguousarray(grads, dtype=np.int32).reshape(self.numSplats, 16)
+        check(self._lib.nps_write_grads(self._h, iptr(a), self.numSplats), self._h, "write_grads")
+
+    def read_grads(self) -> np.ndarray:
+        a = np.empty((self.numSplats, 16), dtype=np.int32)
+        check(self._lib.nps_read_grads(self._h, iptr(a), self.numSplats), self._h, "read_grads")
+        return a
+
+    def write_adam(self, m: np.ndarray, v: np.ndarray, t: float):
+        m = np.ascontiguousarray(m, dtype=np.float32).reshape(self.numSplats, 16)
+        v = np.ascontiguousarray(v, dtype=np.float32).reshape(self.numSplats, 16)
+        check(self._lib.nps_write_adam(self._h, fptr(m), fptr(v), float(t), self.numSplats), self._h, "write_adam")
+
+    def read_adam(self):
+        m = np.empty((self.numSplats, 16), dtype=np.float32)
+        v = np.empty((self.numSplats, 16), dtype=np.float32)
+        t = C.c_float()
+        check(self._lib.nps_read_adam(self._h, fptr(m), fptr(v), C.byref(t), self.numSplats), self._h, "read_adam")
+        return m, v, float(t.value)
+
+    def write_accum(self, accum: np.ndarray):
+        a = np.ascontiguousarray(accum, dtype=np.float32).reshape(self.numSplats)
+        check(self._lib.nps_write_accum(self._h, fptr(a), self.numSplats), self._h, "write_accum")
+
+    def read_accum(self) -> np.ndarray:
+        a = np.empty(self.numSplats, dtype=np.float32)
+        check(self._lib.nps_read_accum(self._h, fptr(a), self.numSplats), self._h, "read_accum")
+        return a
+
+    # ---- reference methods ----
+    def writeSplatVPMatrix(self, mat, invMat, blurEnabled: bool = False):
+        vp = np.ascontiguousarray(mat, dtype=np.float32).reshape(16)
+        vi = np.ascontiguousarray(invMat, dtype=np.float32).reshape(16)
+        check(self._lib.nps_set_camera(self._h, fptr(vp), fptr(vi), 1 if blurEnabled else 0), self._h,
+              "writeSplatVPMatrix")
+
+    def setBackwardTarget(self, targetTextureView: np.ndarray, targetDepthTextureView: np.ndarray, width: int,
+                          height: int):
+        rgba = np.ascontiguousarray(targetTextureView, dtype=np.float32).reshape(height, width, 4)
+        depth = np.ascontiguousarray(targetDepthTextureView, dtype=np.float32).reshape(height, width)
+        check(self._lib.nps_set_target(self._h, fptr(rgba), fptr(depth), int(width), int(height)), self._h,
+              "setBackwardTarget")
+        self.dims = (int(width), int(height))
+
+    def dispatch(self, commandEncoder=None):
+        check(self._lib.nps_dispatch(self._h), self._h, "dispatch")
+
+    # ---- the three passes dispatch() records, callable one at a time ----
+    def backward(self):
+        check(self._lib.nps_backward(self._h), self._h, "backward")
+
+    def step(self):
+        check(self._lib.nps_step(self._h), self._h, "step")
+
+    def adc(self):
+        check(self._lib.nps_adc(self._h), self._h, "adc")
+
+    def sync(self):
+        check(self._lib.nps_sync(self._h), self._h, "sync")
+
+    # ---- outputs the reference never materialises ----
+    def read_image(self):
+        w, h = self.dims
+        c = np.empty((h, w, 3), dtype=np.float32)
+        d = np.empty((h, w), dtype=np.float32)
+        check(self._lib.nps_read_image(self._h, fptr(c), fptr(d)), self._h, "read_image")
+        return c, d
+
+    def read_loss(self):
+        out = np.zeros(2, dtype=np.float64)
+        check(self._lib.nps_read_loss(self._h, dptr(out)), self._h, "read_loss")
+        return float(out[0]), float(out[1])
+
+    def read_tiles(self):
+        nt = C.c_int32()
+        nid = C.c_int64()
+        check(self._lib.nps_read_tiles(self._h, None, None, C.byref(nt), C.byref(nid)), self._h, "read_tiles")
+        ranges = np.empty((nt.value, 2), dtype=np.uint32)
+        ids = np.empty(max(nid.value, 1), dtype=np.uint32)
+        check(self._lib.nps_read_tiles(self._h, uptr(ranges), uptr(ids), None, None), self._h, "read_tiles")
+        return ranges, ids[: nid.value]
+
+    def read_stats(self) -> dict:
+        s = np.zeros(8, dtype=np.int64)
+        check(self._lib.nps_read_stats(self._h, lptr(s)), self._h, "read_stats")
+        keys = ["keys", "tiles_over_cap", "pairs", "adc_qualifiers", "adc_children", "adc_splits", "adc_clones",
+                "adc_rounds"]
+        return dict(zip(keys, (int(x) for x in s)))
+
+    def enable_timing(self, on: bool = True):
+        check(self._lib.nps_enable_timing(self._h, 1 if on else 0), self._h, "enable_timing")
+
+    def read_timing(self) -> dict:
+        t = np.zeros(8, dtype=np.float32)
+        check(self._lib.nps_read_timing(self._h, fptr(t)), self._h, "read_timing")
+        keys = ["preprocess_keys", "sort", "ranges", "composite", "step", "adc", "backward", "unused"]
+        return dict(zip(keys, (float(x) for x in t)))
+
+
+class GpuSplatForwardPipelineManager:
+    """new GpuSplatForwardPipelineManager({device, numSplats, splatBuffer}) (FWDM:17-92).
+
+    ``splatBuffer`` is the optimizer manager whose splats are borrowed."""
+
+    def __init__(self, device=None, numSplats: int | None = None, splatBuffer: GpuSplatOptimizerManager = None):
+        if splatBuffer is None:
+            raise ValueError("splatBuffer (a GpuSplatOptimizerManager) is required")
+        lib = _lib.load()
+        f = C.c_void_p()
+        check(lib.npf_create(splatBuffer.handle, C.byref(f)), splatBuffer.handle, "npf_create")
+        self._lib = lib
+        self._f = f
+        self._owner = splatBuffer
+        self.numSplats = splatBuffer.numSplats
+        self.dims = (0, 0)
+
+    def close(self):
+        if getattr(self, "_f", None):
+            self._lib.npf_destroy(self._f)
+            self._f = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def writeVPMatrix(self, mat):
+        vp = np.ascontiguousarray(mat, dtype=np.float32).reshape(16)
+        check(self._lib.npf_write_vp(self._f, fptr(vp)), self._owner.handle, "writeVPMatrix")
+
+    def setTarget(self, targetColorView=None, targetDepthView=None, width: int = 0, height: int = 0):
+        check(self._lib.npf_set_target(self._f, int(width), int(height)), self._owner.handle, "setTarget")
+        self.dims = (int(width), int(height))
+        if targetColorView is not None or targetDepthView is not None:
+            c = None if targetColorView is None else np.ascontiguousarray(targetColorView, dtype=np.float32)
+            d = None if targetDepthView is None else np.ascontiguousarray(targetDepthView, dtype=np.float32)
+            check(self._lib.npf_write_target(self._f, fptr(c) if c is not None else None,
+                                             fptr(d) if d is not None else None), self._owner.handle, "setTarget")
+
+    def dispatch(self, commandEncoder=None, clear: bool = False, quantize: bool = True):
+        check(self._lib.npf_dispatch(self._f, 1 if clear else 0, 1 if quantize else 0), self._owner.handle,
+              "dispatch")
+
+    def read_target(self):
+        w, h = self.dims
+        c = np.empty((h, w, 4), dtype=np.float32)
+        d = np.empty((h, w, 4), dtype=np.float32)
+        check(self._lib.npf_read_target(self._f, fptr(c), fptr(d)), self._owner.handle, "read_target")
+        return c, d
+
+
+def debug_math(op: int, x: np.ndarray, y: np.ndarray | None = None, device: int = 0) -> np.ndarray:
+    lib = _lib.load()
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty_like(x)
+    yy = None if y is None else np.ascontiguousarray(y, dtype=np.float32)
+    check(lib.nps_debug_math(device, op, fptr(x), fptr(yy) if yy is not None else None, fptr(out), x.size), None,
+          "nps_debug_math")
+    return out

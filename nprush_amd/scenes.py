@@ -1,0 +1,76 @@
+"""Synthetic scenes of the benchmark configurations (seeded, numpy PCG64).
+
+Splat layout (16 float32, splat_backward.wgsl:1-6):
+``[x y z sx] [r g b opacity] [qw qx qy qz] [sy shape_a shape_b sz]``.
+
+Distribution (all configurations): means uniform in [-1, 1]^3, per-axis scales
+exp(N(-4, 0.5)), uniform unit quaternions (Shoemake's method), opacity
+clamp(sigmoid(N(0, 1)), 0.01, 0.99) (the reference stores linear opacity
+clamped to [0.01, 0.99], splat_step.wgsl:85-86), RGB uniform in [0, 1],
+shape_a = 2, shape_b = 0.5 (the reference's initial shape,
+GpuSplatOptimizerManager.ts:83-84, which makes exp(-b r^a) a Gaussian).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import camera
+
+SPLAT_FLOATS = 16
+
+
+def make_splats(n: int, seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    s = np.zeros((n, SPLAT_FLOATS), dtype=np.float32)
+    means = rng.uniform(-1.0, 1.0, size=(n, 3))
+    scales = np.exp(rng.normal(-4.0, 0.5, size=(n, 3)))
+    u = rng.random((n, 3))
+    q = np.stack([
+        np.sqrt(1.0 - u[:, 0]) * np.sin(2.0 * math.pi * u[:, 1]),
+        np.sqrt(1.0 - u[:, 0]) * np.cos(2.0 * math.pi * u[:, 1]),
+        np.sqrt(u[:, 0]) * np.sin(2.0 * math.pi * u[:, 2]),
+        np.sqrt(u[:, 0]) * np.cos(2.0 * math.pi * u[:, 2]),
+    ], axis=1)
+    logit = rng.normal(0.0, 1.0, size=n)
+    opacity = np.clip(1.0 / (1.0 + np.exp(-logit)), 0.01, 0.99)
+    rgb = rng.uniform(0.0, 1.0, size=(n, 3))
+    s[:, 0:3] = means
+    s[:, 3] = scales[:, 0]
+    s[:, 4:7] = rgb
+    s[:, 7] = opacity
+    s[:, 8:12] = q
+    s[:, 12] = scales[:, 1]
+    s[:, 13] = 2.0
+    s[:, 14] = 0.5
+    s[:, 15] = scales[:, 2]
+    return s
+
+
+def with_dead_capacity(live: np.ndarray, capacity: int) -> np.ndarray:
+    """Pads a live set with dead slots (opacity 0) up to `capacity` splats, as the
+    reference's fixed-capacity pool holds them (GpuSplatOptimizerManager.ts:71-75)."""
+    out = np.zeros((capacity, SPLAT_FLOATS), dtype=np.float32)
+    out[: live.shape[0]] = live
+    dead = out[live.shape[0]:]
+    dead[:, 3] = dead[:, 12] = dead[:, 15] = 0.1
+    dead[:, 8] = 1.0
+    dead[:, 13] = 2.0
+    dead[:, 14] = 0.5
+    return out
+
+
+def quantize8(img: np.ndarray) -> np.ndarray:
+    """Values as an rgba8unorm texture returns them: k/255."""
+    return (np.rint(np.clip(img, 0.0, 1.0) * 255.0) / 255.0).astype(np.float32)
+
+
+def config0_camera():
+    """One pinhole camera at (0,0,3) looking at the origin, fov 60 deg, square."""
+    return camera.view_proj((0.0, 0.0, 3.0), aspect=1.0, fovy=math.pi / 3)
+
+
+CONFIG0 = dict(n=16384, width=256, height=256, seed=0, target_seed=1, steps=100)
+CONFIG1 = dict(n=1_000_000, capacity=2_000_000, width=1920, height=1080, views=8, radius=3.0, seed=0,
+               target_seed=1)

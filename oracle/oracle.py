@@ -1,0 +1,157 @@
+"""ctypes front end of the CPU oracle (oracle/libnps_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker or the timed CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libnps_oracle.so")
+_lib = None
+
+_F = C.POINTER(C.c_float)
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int32)
+_U = C.POINTER(C.c_uint32)
+i32 = C.c_int32
+
+NO_REG = 1
+NO_PULL = 2
+NO_SKIP = 4
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        lib = C.CDLL(LIB)
+        sig = {
+            "ora_bin": [_F, i32, _F, i32, i32, _I, i32, i32, _U, _I, _I],
+            "ora_backward_f32": [_F, i32, _F, _F, _F, _F, i32, i32, i32, _I, i32, _I, _F, _F, _D, _I],
+            "ora_backward_f64": [_D, i32, _D, _D, _F, _F, i32, i32, _I, i32, _U, _I, i32, _D, _D, _D, _D, i32],
+            "ora_step_f32": [_F, _I, _F, _F, _F, i32, _F],
+            "ora_adc_f32": [_F, _F, _F, _F, i32, C.c_float, _I],
+            "ora_render_f32": [_F, i32, _F, i32, i32, _F, _F, i32, i32],
+        }
+        for k, args in sig.items():
+            getattr(lib, k).argtypes = args
+            getattr(lib, k).restype = i32
+        for k in ["ora_exp2f", "ora_log2f", "ora_expf", "ora_logf", "ora_sinf"]:
+            getattr(lib, k).argtypes = [C.c_float]
+            getattr(lib, k).restype = C.c_float
+        lib.ora_powf.argtypes = [C.c_float, C.c_float]
+        lib.ora_powf.restype = C.c_float
+        lib.ora_f2i.argtypes = [C.c_float]
+        lib.ora_f2i.restype = C.c_int32
+        lib.ora_num_threads.restype = C.c_int
+        _lib = lib
+    return _lib
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def tiles_of(W, H):
+    return ((W + 15) // 16) * ((H + 15) // 16)
+
+
+def bin_tiles(splats, vp, W, H, tiles=None, cap=1024):
+    lib = load()
+    s = f32(splats)
+    n = s.shape[0]
+    tiles = np.arange(tiles_of(W, H), dtype=np.int32) if tiles is None else np.ascontiguousarray(tiles, np.int32)
+    ids = np.zeros((len(tiles), cap), dtype=np.uint32)
+    cnt = np.zeros(len(tiles), dtype=np.int32)
+    full = np.zeros(len(tiles), dtype=np.int32)
+    lib.ora_bin(_p(s, _F), n, _p(f32(vp), _F), W, H, _p(tiles, _I), len(tiles), cap, _p(ids, _U), _p(cnt, _I),
+                _p(full, _I))
+    return ids, cnt, full
+
+
+def backward_f32(splats, vp, vp_inv, tgt_rgba, tgt_depth, W, H, cap=1024, tiles=None, grads=None):
+    """REF backward.  Returns (grads int32 [n,16], C [H,W,3], D [H,W], loss[2], full_counts)."""
+    lib = load()
+    s = f32(splats)
+    n = s.shape[0]
+    g = np.zeros((n, 16), dtype=np.int32) if grads is None else np.ascontiguousarray(grads, dtype=np.int32)
+    c = np.zeros((H, W, 3), dtype=np.float32)
+    d = np.zeros((H, W), dtype=np.float32)
+    loss = np.zeros(2, dtype=np.float64)
+    t = None if tiles is None else np.ascontiguousarray(tiles, np.int32)
+    nt = tiles_of(W, H) if t is None else len(t)
+    full = np.zeros(nt, dtype=np.int32)
+    lib.ora_backward_f32(_p(s, _F), n, _p(f32(vp), _F), _p(f32(vp_inv), _F), _p(f32(tgt_rgba), _F),
+                         _p(f32(tgt_depth), _F), W, H, cap, _p(t, _I), 0 if t is None else len(t), _p(g, _I),
+                         _p(c, _F), _p(d, _F), _p(loss, _D), _p(full, _I))
+    return g, c, d, loss, full
+
+
+def backward_f64(splats64, vp64, vpinv64, tgt_rgba, tgt_depth, W, H, tiles, lists, counts, want_grads=True,
+                 flags=0):
+    lib = load()
+    s = np.ascontiguousarray(splats64, dtype=np.float64)
+    n = s.shape[0]
+    g = np.zeros((n, 16), dtype=np.float64) if want_grads else None
+    c = np.zeros((H, W, 3), dtype=np.float64)
+    d = np.zeros((H, W), dtype=np.float64)
+    loss = np.zeros(2, dtype=np.float64)
+    tiles = np.ascontiguousarray(tiles, np.int32)
+    lists = np.ascontiguousarray(lists, np.uint32)
+    counts = np.ascontiguousarray(counts, np.int32)
+    lib.ora_backward_f64(_p(s, _D), n, _p(np.ascontiguousarray(vp64, np.float64), _D),
+                         _p(np.ascontiguousarray(vpinv64, np.float64), _D), _p(f32(tgt_rgba), _F),
+                         _p(f32(tgt_depth), _F), W, H, _p(tiles, _I), len(tiles), _p(lists, _U), _p(counts, _I),
+                         lists.shape[1], _p(g, _D), _p(c, _D), _p(d, _D), _p(loss, _D), flags)
+    return g, c, d, loss
+
+
+def step_f32(splats, grads, m, v, accum, t):
+    lib = load()
+    s, g = f32(splats).copy(), np.ascontiguousarray(grads, np.int32).copy()
+    m, v, a = f32(m).copy(), f32(v).copy(), f32(accum).copy()
+    tt = np.array([t], dtype=np.float32)
+    lib.ora_step_f32(_p(s, _F), _p(g, _I), _p(m, _F), _p(v, _F), _p(a, _F), s.shape[0], _p(tt, _F))
+    return s, g, m, v, a, float(tt[0])
+
+
+def adc_f32(splats, m, v, accum, t):
+    lib = load()
+    s, m, v, a = f32(splats).copy(), f32(m).copy(), f32(v).copy(), f32(accum).copy()
+    st = np.zeros(4, dtype=np.int32)
+    lib.ora_adc_f32(_p(s, _F), _p(m, _F), _p(v, _F), _p(a, _F), s.shape[0], float(t), _p(st, _I))
+    return s, m, v, a, st
+
+
+def render_f32(splats, vp, W, H, color=None, depth=None, clear=True, quantize=True):
+    lib = load()
+    s = f32(splats)
+    c = np.zeros((H, W, 4), np.float32) if color is None else f32(color).copy()
+    d = np.zeros((H, W, 4), np.float32) if depth is None else f32(depth).copy()
+    lib.ora_render_f32(_p(s, _F), s.shape[0], _p(f32(vp), _F), W, H, _p(c, _F), _p(d, _F), int(clear),
+                       int(quantize))
+    return c, d
+
+
+def math_f32(op, x, y=None):
+    lib = load()
+    fn = {0: lib.ora_exp2f, 1: lib.ora_log2f, 3: lib.ora_expf, 4: lib.ora_logf, 5: lib.ora_sinf}
+    x = f32(x)
+    if op == 2:
+        return np.array([lib.ora_powf(float(a), float(b)) for a, b in zip(x, f32(y))], dtype=np.float32)
+    return np.array([fn[op](float(a)) for a in x], dtype=np.float32)

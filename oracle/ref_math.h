@@ -1,0 +1,127 @@
+/*
+ * ref_math.h -- oracle-side deterministic transcendentals (plain C, FP32).
+ *
+ * TEST INFRASTRUCTURE ONLY. Nothing in the product links this file.
+ *
+ * The reference evaluates exp/pow/log/sqrt/sin with WGSL built-ins, whose results
+ * are implementation-defined to within ULP bounds (WGSL spec: pow(x,y) is
+ * "inherited from exp2(y * log2(x))").  To compare a GPU path against the CPU
+ * bit-for-bit, both sides must evaluate the SAME function.  This header defines
+ * that function from published methods, using only IEEE-754 correctly rounded
+ * +,-,*,/ and sqrt, with no fused multiply-add (build with -ffp-contract=off):
+ *
+ *   exp2f: Cody & Waite argument reduction x = k + f, |f| <= 1/2, then the
+ *          Taylor series of e^(f ln2) to degree 7 (truncation < 2^-27 relative),
+ *          Horner, then exact scaling by 2^k.  Results below 2^-126 flush to 0.
+ *   log2f: x = m 2^e with m in [sqrt(1/2), sqrt(2)), ln m = 2 atanh(s),
+ *          s = (m-1)/(m+1), series 2s(1 + s^2/3 + ... + s^10/11).
+ *   powf(x,y) = exp2f(y * log2f(x))   (the WGSL definition).
+ *   logf(x)   = log2f(x) * ln2,  expf(x) = exp2f(x * log2 e).
+ *   sinf:     double-precision Cody-Waite reduction by pi/2 (three-part pi/2,
+ *             exact for |x| < 2^27), Taylor polynomials, rounded to float.
+ *             Used only by the clone-jitter hash (splat_adc.wgsl:70-72).
+ *
+ * The product carries its own implementation of the same methods; a GPU test
+ * sweeps both and requires identical bits.
+ */
+#ifndef NPS_REF_MATH_H
+#define NPS_REF_MATH_H
+
+#include <math.h>
+#include <stdint.h>
+
+#define RM_LN2F   0x1.62e430p-1f
+#define RM_LOG2EF 0x1.715476p+0f
+
+static inline float rm_exp2f(float x) {
+    if (x != x) return x;
+    if (x >= 128.0f) return INFINITY;
+    if (x < -126.0f) return 0.0f;
+    float k = rintf(x);
+    float f = x - k; /* exact: |f| <= 0.5 */
+    float p = 0x1.ffcbfcp-17f;  /* ln2^7/7! */
+    p = p * f + 0x1.430912p-13f; /* ln2^6/6! */
+    p = p * f + 0x1.5d87fep-10f; /* ln2^5/5! */
+    p = p * f + 0x1.3b2ab6p-7f;  /* ln2^4/4! */
+    p = p * f + 0x1.c6b08ep-5f;  /* ln2^3/3! */
+    p = p * f + 0x1.ebfbe0p-3f;  /* ln2^2/2! */
+    p = p * f + 0x1.62e430p-1f;  /* ln2      */
+    p = p * f + 1.0f;
+    float r = ldexpf(p, (int)k);
+    if (r < 0x1p-126f) return 0.0f;
+    return r;
+}
+
+static inline float rm_log2f(float x) {
+    if (x != x) return x;
+    if (x < 0.0f) return NAN;
+    if (x == 0.0f) return -INFINITY;
+    if (x == INFINITY) return x;
+    int e;
+    float m = frexpf(x, &e); /* m in [0.5, 1) */
+    if (m < 0x1.6a09e6p-1f) { m = m * 2.0f; e = e - 1; }
+    float s = (m - 1.0f) / (m + 1.0f);
+    float s2 = s * s;
+    float p = 0x1.745d18p-4f;     /* 1/11 */
+    p = p * s2 + 0x1.c71c72p-4f;  /* 1/9 */
+    p = p * s2 + 0x1.24924ap-3f;  /* 1/7 */
+    p = p * s2 + 0x1.99999ap-3f;  /* 1/5 */
+    p = p * s2 + 0x1.555556p-2f;  /* 1/3 */
+    float two_s = s * 2.0f;
+    float lnm = two_s + (two_s * s2) * p;
+    return (float)e + lnm * RM_LOG2EF;
+}
+
+static inline float rm_powf(float x, float y) { return rm_exp2f(y * rm_log2f(x)); }
+static inline float rm_logf(float x) { return rm_log2f(x) * RM_LN2F; }
+static inline float rm_expf(float x) { return rm_exp2f(x * RM_LOG2EF); }
+
+static inline float rm_sinf(float xf) {
+    double x = (double)xf;
+    if (!(x - x == 0.0)) return NAN; /* inf or nan */
+    double k = rint(x * 0x1.45f306dc9c883p-1);
+    double r = x - k * 0x1.921fb5p+0;
+    r = r - k * 0x1.110b46p-26;
+    r = r - k * 0x1.1a62633145c07p-54;
+    int64_t ki = (int64_t)k;
+    int q = (int)(ki & 3);
+    double r2 = r * r;
+    double s = -0x1.ae7f3e733b81fp-41;
+    s = s * r2 + 0x1.6124613a86d09p-33;
+    s = s * r2 - 0x1.ae64567f544e4p-26;
+    s = s * r2 + 0x1.71de3a556c734p-19;
+    s = s * r2 - 0x1.a01a01a01a01ap-13;
+    s = s * r2 + 0x1.1111111111111p-7;
+    s = s * r2 - 0x1.5555555555555p-3;
+    double sinr = r + (r * r2) * s;
+    double c = 0x1.ae7f3e733b81fp-45;
+    c = c * r2 - 0x1.93974a8c07c9dp-37;
+    c = c * r2 + 0x1.1eed8eff8d898p-29;
+    c = c * r2 - 0x1.27e4fb7789f5cp-22;
+    c = c * r2 + 0x1.a01a01a01a01ap-16;
+    c = c * r2 - 0x1.6c16c16c16c17p-10;
+    c = c * r2 + 0x1.5555555555555p-5;
+    c = c * r2 - 0x1.0p-1;
+    double cosr = 1.0 + r2 * c;
+    double out;
+    switch (q) {
+        case 0: out = sinr; break;
+        case 1: out = cosr; break;
+        case 2: out = -sinr; break;
+        default: out = -cosr; break;
+    }
+    return (float)out;
+}
+
+/* WGSL fract(e) = e - floor(e) */
+static inline float rm_fractf(float x) { return x - floorf(x); }
+
+/* WGSL i32(f32): truncate toward zero, saturate; NaN -> 0 (defined here). */
+static inline int32_t rm_f2i_sat(float x) {
+    if (x != x) return 0;
+    if (x >= 2147483648.0f) return INT32_MAX;
+    if (x <= -2147483648.0f) return INT32_MIN;
+    return (int32_t)x;
+}
+
+#endif
