@@ -1,0 +1,186 @@
+"""Benchmark: fwd+bwd ms per view at 1M splats, 1920x1080 (REF semantics).
+
+Workload (one "step" = one reference dispatch for one view, i.e.
+GpuSplatOptimizerManager.dispatch: backward (fwd+loss+bwd) + Adam step, ADC
+every 25th step):
+  1,000,000 live splats drawn with seed 0 plus 1,000,000 dead slots (capacity
+  2M, the fixed-capacity pool the reference's ADC fills), 8 cameras on a
+  radius-3 ring at 1920x1080 (16x16 tiles: 120x68 = 8,160 tiles), targets =
+  the REF forward image of a seed-1 scene of 1M splats per camera, quantised to
+  k/255, target depth 1 (background).  Targets are uploaded to HBM before the
+  timed region.  Colour is direct RGB as in the reference (SH is not part of
+  the reference path).
+
+Prints one JSON line.  value = mean GPU time of the backward (preprocess, key
+generation, sort, ranges, fused composite) per view, from HIP events on the
+library's stream; ms_per_step = wall clock per dispatch.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Algorithmic work of the fused composite kernel, counted from the reference
+# algorithm (splat_backward.wgsl:230-455) after the per-splat terms are
+# hoisted; transcendentals (sqrt, log2, exp2) weighted x4 (quarter-rate VALU).
+#   forward, every composited (pixel, splat) pair: 34 FLOP + 4 transcendentals
+#   reverse, every pair with alpha >= 0.001: 240 FLOP + 3 transcendentals
+#     (colour/opacity/depth terms, shape a/b, conic and axis gradients, screen
+#      and depth position terms, quaternion partials, fixed-point conversion)
+FWD_FLOP_PER_PAIR = 34 + 4 * 4
+BWD_FLOP_PER_ACTIVE_PAIR = 240 + 3 * 4
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def build_workload(n_live, capacity, W, H, views, radius):
+    from nprush_amd import GpuSplatOptimizerManager, camera, scenes
+
+    cams = camera.ring_cameras(views, radius, W, H)
+    tscene = scenes.make_splats(n_live, 1)
+    tmgr = GpuSplatOptimizerManager(numSplats=n_live, splats=tscene)
+    dummy = np.zeros((H, W, 4), np.float32)
+    ones = np.ones((H, W), np.float32)
+    tmgr.setBackwardTarget(dummy, ones, W, H)
+    targets = []
+    for vp, vi in cams:
+        tmgr.writeSplatVPMatrix(vp, vi)
+        tmgr.backward()
+        c, _ = tmgr.read_image()
+        rgba = np.ones((H, W, 4), np.float32)
+        rgba[..., :3] = scenes.quantize8(c)
+        targets.append(rgba)
+    tmgr.close()
+    splats = scenes.with_dead_capacity(scenes.make_splats(n_live, 0), capacity)
+    return cams, targets, ones, splats
+
+
+def cpu_baseline(splats, vp, vi, target, depth, W, H, sample_tiles, seed=0):
+    """The oracle (FP32 REF restatement, faithful O(tiles x N) binning) on a
+    random sample of tiles of one view, scaled to the whole view."""
+    from oracle import oracle
+
+    oracle.load()
+    ntiles = ((W + 15) // 16) * ((H + 15) // 16)
+    rng = np.random.default_rng(seed)
+    tiles = np.sort(rng.choice(ntiles, size=min(sample_tiles, ntiles), replace=False)).astype(np.int32)
+    t0 = time.perf_counter()
+    oracle.backward_f32(splats, vp, vi, target, depth, W, H, tiles=tiles)
+    dt = time.perf_counter() - t0
+    ms_view = dt * 1000.0 * ntiles / len(tiles)
+    return {
+        "value": round(ms_view, 1),
+        "unit": "ms/view (fwd+bwd)",
+        "cores": int(oracle.load().ora_num_threads()),
+        "kind": "port",
+        "sample": f"{len(tiles)} random tiles of view 0 ({dt:.1f} s), scaled by {ntiles}/{len(tiles)} tiles",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--small", action="store_true", help="16,384 splats at 256x256 (quick check)")
+    ap.add_argument("--cpu-tiles", type=int, default=48, help="tiles in the CPU-baseline sample (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is described in "
+              "DESIGN.md and is not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    from nprush_amd import GpuSplatOptimizerManager
+
+    if args.small:
+        n_live, capacity, W, H, views = 16384, 32768, 256, 256, 8
+    else:
+        n_live, capacity, W, H, views = 1_000_000, 2_000_000, 1920, 1080, 8
+    print(f"[bench] building workload: {n_live} live / {capacity} capacity, {W}x{H}, {views} views", file=sys.stderr,
+          flush=True)
+    cams, targets, depth, splats = build_workload(n_live, capacity, W, H, views, 3.0)
+
+    mgr = GpuSplatOptimizerManager(numSplats=capacity, splats=splats)
+    slots = [mgr.upload_target(t, depth) for t in targets]
+    mgr.enable_timing(True)
+
+    def one_step(k):
+        vp, vi = cams[k % views]
+        mgr.writeSplatVPMatrix(vp, vi)
+        mgr.set_target_slot(slots[k % views])
+        mgr.dispatch()
+        t = mgr.read_timing()
+        st = mgr.read_stats()
+        return t, st
+
+    for k in range(args.warmup):
+        one_step(k)
+    mgr.sync()
+    print(f"[bench] warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+    rows = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        rows.append(one_step(args.warmup + k))
+    mgr.sync()
+    wall = time.perf_counter() - t0
+    print(f"[bench] timed {args.steps} steps in {wall:.2f} s", file=sys.stderr, flush=True)
+
+    bwd_ms = float(np.mean([t["backward"] for t, _ in rows]))
+    comp_ms = float(np.mean([t["composite"] for t, _ in rows]))
+    step_ms = float(np.mean([t["step"] for t, _ in rows]))
+    pairs = float(np.mean([s["pairs"] for _, s in rows]))
+    active = float(np.mean([s["active_pairs"] for _, s in rows]))
+    flops = pairs * FWD_FLOP_PER_PAIR + active * BWD_FLOP_PER_ACTIVE_PAIR
+    achieved = flops / (comp_ms * 1e-3) / 1e12
+    phases = {k: round(float(np.mean([t[k] for t, _ in rows])), 3) for k in
+              ("preprocess_keys", "sort", "ranges", "composite", "step")}
+    adam_gbs = 520.0 * capacity / (step_ms * 1e-3) / 1e9
+
+    cpu = None
+    if args.cpu_tiles > 0:
+        print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
+        vp, vi = cams[0]
+        cpu = cpu_baseline(splats, vp, vi, targets[0], depth, W, H, args.cpu_tiles)
+
+    out = {
+        "metric": "fwd+bwd ms/view @1M splats 1080p",
+        "value": round(bwd_ms, 3),
+        "unit": "ms/view",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1000.0 / args.steps, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scenes, targets rendered from a seed-1 scene)",
+        "config": {"workload": f"REF splat fwd+bwd+Adam, {n_live} live / {capacity} capacity splats, {W}x{H}, "
+                               f"{views} ring views, 16x16 tiles, cap 1024/tile",
+                   "views": views, "width": W, "height": H, "splats": n_live, "capacity": capacity},
+        "roofline": {"bound": "valu", "kernel": "k_composite_ref", "achieved": round(achieved, 3),
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": None},
+        "phases_ms": phases,
+        "pairs_per_view": int(pairs),
+        "active_pairs_per_view": int(active),
+        "adam_hbm_gbs": round(adam_gbs, 1),
+        "steps_per_s": round(args.steps / wall, 2),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    for s in slots:
+        mgr.free_target(s)
+    mgr.close()
+
+
+if __name__ == "__main__":
+    main()

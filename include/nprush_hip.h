@@ -110,9 +110,10 @@ int nps_read_loss(nps_handle h, double loss[2]);
  * sizes in *num_tiles / *num_ids. */
 int nps_read_tiles(nps_handle h, uint32_t* ranges, uint32_t* ids, int32_t* num_tiles, int64_t* num_ids);
 /* Counters of the last backward/ADC: stats[0]=keys, [1]=tiles over the cap,
- * [2]=pairs composited, [3]=ADC qualifiers, [4]=ADC children, [5]=ADC splits,
- * [6]=ADC clones, [7]=ADC cascade rounds. */
-int nps_read_stats(nps_handle h, int64_t stats[8]);
+ * [2]=(pixel, splat) pairs composited, [3]=pairs taken by the reverse pass
+ * (alpha >= 0.001), [4]=ADC qualifiers, [5]=ADC children, [6]=ADC splits,
+ * [7]=ADC clones, [8]=ADC cascade rounds, [9..11]=0. */
+int nps_read_stats(nps_handle h, int64_t stats[12]);
 /* Per-phase GPU time (ms, HIP events on the handle's stream) of the last
  * backward/step/adc when timing is enabled: [0]=preprocess+keys, [1]=sort,
  * [2]=ranges, [3]=composite fwd+bwd, [4]=step, [5]=adc, [6]=whole backward. */
@@ -138,6 +139,13 @@ int npf_set_target(npf_handle f, int32_t w, int32_t hgt);
 int npf_dispatch(npf_handle f, int32_t clear, int32_t quantize);
 int npf_read_target(npf_handle f, float* color, float* depth);
 int npf_write_target(npf_handle f, const float* color, const float* depth);
+
+/* Device buffers on the handle's GPU for callers without a GPU allocator of
+ * their own (Deno, ctypes), e.g. to keep several targets resident in HBM for
+ * nps_set_target_dev.  No reference counterpart (WebGPU's createBuffer). */
+int nps_dev_alloc(nps_handle h, int64_t bytes, void** out);
+int nps_dev_free(nps_handle h, void* p);
+int nps_dev_upload(nps_handle h, void* dst, const void* src, int64_t bytes);
 
 /* Deterministic transcendentals used by the kernels, evaluated on the GPU
  * over x[n] (and y[n] for pow): op 0=exp2 1=log2 2=pow 3=exp 4=log 5=sin.

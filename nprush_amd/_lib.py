@@ -68,6 +68,9 @@ SIGNATURES = {
     "npf_dispatch": (i32, [_P, i32, i32]),
     "npf_read_target": (i32, [_P, _F, _F]),
     "npf_write_target": (i32, [_P, _F, _F]),
+    "nps_dev_alloc": (i32, [_P, C.c_int64, C.POINTER(_P)]),
+    "nps_dev_free": (i32, [_P, _P]),
+    "nps_dev_upload": (i32, [_P, _P, _P, C.c_int64]),
     "nps_debug_math": (i32, [i32, i32, _F, _F, _F, i32]),
 }
 

@@ -313,6 +313,7 @@ __global__ __launch_bounds__(256) void k_composite_ref(ViewParams v, CompositeAr
 
     // ---------------- reverse pass (BW:259-455) ----------------
     const int nbatch = (count + kBatch - 1) / kBatch;
+    uint32_t n_active = 0;  // wave-uniform count of pairs taken by the reverse pass
     for (int bi = nbatch - 1; bi >= 0; --bi) {
         const int b0 = bi * kBatch;
         const int nb = min(kBatch, count - b0);
@@ -336,7 +337,9 @@ __global__ __launch_bounds__(256) void k_composite_ref(ViewParams v, CompositeAr
                 float a = s_a[j - j0][tid];
                 float Tp = s_T[j - j0][tid];
                 bool active = valid && !(a < 0.001f);
-                if (__ballot(active) == 0ull) continue;
+                uint64_t bal = __ballot(active);
+                if (bal == 0ull) continue;
+                n_active += (uint32_t)__popcll(bal);
                 uint32_t gv[16];
                 if (active) {
                     pair_grads(s_fwd[j], s_bwd[j], pxf, pyf, a, Tp, st, gv);
@@ -357,6 +360,7 @@ __global__ __launch_bounds__(256) void k_composite_ref(ViewParams v, CompositeAr
             if (val != 0u) atomicAdd((unsigned int*)&A.grads[16 * (size_t)s_ids[e >> 4] + (e & 15)], val);
         }
     }
+    if (lane == 0 && n_active) atomicAdd(&A.stats[2], (unsigned long long)n_active);
     __syncthreads();
     if (tid == 0) {
         double a2 = (s_red[0][0] + s_red[0][1]) + (s_red[0][2] + s_red[0][3]);

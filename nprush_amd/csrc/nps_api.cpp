@@ -60,7 +60,7 @@ struct nps_ctx {
     bool have_backward = false;
     // ADC buffers
     DevBuf adc_u32, adc_tmp, adc_scan;
-    int64_t stats_host[8] = {0};
+    int64_t stats_host[12] = {0};
     bool timing = false;
     hipEvent_t ev[8] = {nullptr};
     float timing_ms[8] = {0};
@@ -464,7 +464,7 @@ int nps_adc(nps_handle h) {
         hipEventElapsedTime(&h->timing_ms[5], h->ev[5], h->ev[6]);
     }
     NPS_HIP(h, hipGetLastError());
-    for (int k = 0; k < 5; ++k) h->stats_host[3 + k] = st[k];
+    for (int k = 0; k < 5; ++k) h->stats_host[4 + k] = st[k];
     if (r != 0) return set_err(h, NPS_ERR_CAPACITY, "ADC cascade chain deeper than 256");
     return NPS_OK;
 }
@@ -525,15 +525,16 @@ int nps_read_tiles(nps_handle h, uint32_t* ranges, uint32_t* ids, int32_t* num_t
     return NPS_OK;
 }
 
-int nps_read_stats(nps_handle h, int64_t stats[8]) {
+int nps_read_stats(nps_handle h, int64_t stats[12]) {
     if (!h || !stats) return NPS_ERR_ARG;
     if (h->have_backward && h->stats.p) {
-        unsigned long long s[2] = {0, 0};
+        unsigned long long s[3] = {0, 0, 0};
         NPS_COPY(h, s, h->stats.p, sizeof(s), hipMemcpyDeviceToHost);
         h->stats_host[2] = (int64_t)s[0];
         h->stats_host[1] = (int64_t)s[1];
+        h->stats_host[3] = (int64_t)s[2];
     }
-    for (int k = 0; k < 8; ++k) stats[k] = h->stats_host[k];
+    for (int k = 0; k < 12; ++k) stats[k] = h->stats_host[k];
     return NPS_OK;
 }
 
@@ -631,6 +632,28 @@ int npf_write_target(npf_handle f, const float* color, const float* depth) {
     size_t px = (size_t)f->W * f->H;
     if (color) NPS_COPY(h, f->color.p, color, 16 * px, hipMemcpyHostToDevice);
     if (depth) NPS_COPY(h, f->depth.p, depth, 16 * px, hipMemcpyHostToDevice);
+    return NPS_OK;
+}
+
+int nps_dev_alloc(nps_handle h, int64_t bytes, void** out) {
+    if (!h || !out || bytes < 0) return NPS_ERR_ARG;
+    NPS_HIP(h, hipSetDevice(h->device));
+    *out = nullptr;
+    NPS_HIP(h, hipMalloc(out, (size_t)std::max<int64_t>(bytes, 1)));
+    return NPS_OK;
+}
+
+int nps_dev_free(nps_handle h, void* p) {
+    if (!h) return NPS_ERR_ARG;
+    NPS_HIP(h, hipSetDevice(h->device));
+    NPS_HIP(h, hipStreamSynchronize(h->stream));
+    if (p) NPS_HIP(h, hipFree(p));
+    return NPS_OK;
+}
+
+int nps_dev_upload(nps_handle h, void* dst, const void* src, int64_t bytes) {
+    if (!h || !dst || !src || bytes < 0) return NPS_ERR_ARG;
+    NPS_COPY(h, dst, src, (size_t)bytes, hipMemcpyHostToDevice);
     return NPS_OK;
 }
 

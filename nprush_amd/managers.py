@@ -142,6 +142,29 @@ class GpuSplatOptimizerManager:
     def dispatch(self, commandEncoder=None):
         check(self._lib.nps_dispatch(self._h), self._h, "dispatch")
 
+    # ---- targets resident in HBM (no reference counterpart) ----
+    def upload_target(self, rgba: np.ndarray, depth: np.ndarray):
+        """Copies a target to device memory once; returns a slot for set_target_slot."""
+        rgba = np.ascontiguousarray(rgba, dtype=np.float32)
+        depth = np.ascontiguousarray(depth, dtype=np.float32)
+        slot = []
+        for a in (rgba, depth):
+            p = C.c_void_p()
+            check(self._lib.nps_dev_alloc(self._h, a.nbytes, C.byref(p)), self._h, "nps_dev_alloc")
+            check(self._lib.nps_dev_upload(self._h, p, a.ctypes.data_as(C.c_void_p), a.nbytes), self._h,
+                  "nps_dev_upload")
+            slot.append(p)
+        return (slot[0], slot[1], rgba.shape[1], rgba.shape[0])
+
+    def set_target_slot(self, slot):
+        rgba_p, depth_p, w, h = slot
+        check(self._lib.nps_set_target_dev(self._h, rgba_p, depth_p, int(w), int(h)), self._h, "set_target_dev")
+        self.dims = (int(w), int(h))
+
+    def free_target(self, slot):
+        for p in slot[:2]:
+            self._lib.nps_dev_free(self._h, p)
+
     # ---- the three passes dispatch() records, callable one at a time ----
     def backward(self):
         check(self._lib.nps_backward(self._h), self._h, "backward")
@@ -178,10 +201,10 @@ class GpuSplatOptimizerManager:
         return ranges, ids[: nid.value]
 
     def read_stats(self) -> dict:
-        s = np.zeros(8, dtype=np.int64)
+        s = np.zeros(12, dtype=np.int64)
         check(self._lib.nps_read_stats(self._h, lptr(s)), self._h, "read_stats")
-        keys = ["keys", "tiles_over_cap", "pairs", "adc_qualifiers", "adc_children", "adc_splits", "adc_clones",
-                "adc_rounds"]
+        keys = ["keys", "tiles_over_cap", "pairs", "active_pairs", "adc_qualifiers", "adc_children", "adc_splits",
+                "adc_clones", "adc_rounds"]
         return dict(zip(keys, (int(x) for x in s)))
 
     def enable_timing(self, on: bool = True):
