@@ -92,6 +92,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--small", action="store_true", help="16,384 splats at 256x256 (quick check)")
     ap.add_argument("--cpu-tiles", type=int, default=48, help="tiles in the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run one plain streaming kernel (k_debug_math, 16M floats: 64 MiB read + 64 MiB "
+                         "written) to calibrate profiler byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is described in "
@@ -143,6 +146,21 @@ def main():
     phases = {k: round(float(np.mean([t[k] for t, _ in rows])), 3) for k in
               ("preprocess_keys", "sort", "ranges", "composite", "step")}
     adam_gbs = 520.0 * capacity / (step_ms * 1e-3) / 1e9
+    keys = float(np.mean([s["keys"] for _, s in rows]))
+    # HBM bytes per composite launch from the committed rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE runs of this workload (calibrated against a streaming kernel)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath) and not args.small:
+        with open(tpath) as fh:
+            tj = json.load(fh).get("k_composite_ref")
+        if tj:
+            traffic = int(tj["fetch_bytes"] + tj["write_bytes"])
+
+    if args.calibrate:
+        from nprush_amd import debug_math
+
+        debug_math(0, np.linspace(-8.0, 8.0, 1 << 24, dtype=np.float32))
 
     cpu = None
     if args.cpu_tiles > 0:
@@ -168,11 +186,13 @@ def main():
                    "views": views, "width": W, "height": H, "splats": n_live, "capacity": capacity},
         "roofline": {"bound": "valu", "kernel": "k_composite_ref", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": None},
+                     "traffic": traffic},
         "phases_ms": phases,
+        "tile_keys_per_view": int(keys),
         "pairs_per_view": int(pairs),
         "active_pairs_per_view": int(active),
         "adam_hbm_gbs": round(adam_gbs, 1),
+        "adam_roofline_frac": round(adam_gbs / PEAK_HBM_GBS, 3),
         "steps_per_s": round(args.steps / wall, 2),
         "cpu_baseline": cpu,
     }
