@@ -148,7 +148,8 @@ int nps_dev_free(nps_handle h, void* p);
 int nps_dev_upload(nps_handle h, void* dst, const void* src, int64_t bytes);
 
 /* Deterministic transcendentals used by the kernels, evaluated on the GPU
- * over x[n] (and y[n] for pow): op 0=exp2 1=log2 2=pow 3=exp 4=log 5=sin.
+ * over x[n] (and y[n] for pow): op 0=exp2 1=log2 2=pow 3=exp 4=log 5=sin,
+ * 6=float->int32 conversion of the gradient path (result bits in out).
  * For tests (bit-identity with the CPU oracle). */
 int nps_debug_math(int32_t device, int32_t op, const float* x, const float* y, float* out, int32_t n);
 

@@ -104,7 +104,18 @@ __global__ __launch_bounds__(256) void k_preprocess(const float4* __restrict__ s
     f.sb = s3.z;
     float lin = fmax_w(w, 0.1f);
     f.depth = clamp_w(1.0f - 0.1f / lin, 0.0f, 1.0f);
-    f.pad[0] = f.pad[1] = f.pad[2] = f.pad[3] = 0.0f;
+    // Pairs with r^2 > rcut2 have power = -b*pow(r, a) <= -15, so alpha is exactly
+    // 0 and the pair can skip pow/exp.  rcut = 1.001 * R with R = (15/b)^(1/a)
+    // as computed above; for a in [0.1, 64] and b >= 1e-4 the 0.1 % margin
+    // (>= 1e-4 relative in b*r^a) exceeds the combined error of the two pow
+    // evaluations (< 2e-5 relative), so the float test power > -15 is false.
+    float rcut2 = INFINITY;
+    if (s3.y >= 0.1f && s3.y <= 64.0f && s3.z >= 0.0001f && s3.z <= 1e30f && Rr > 0.0f && Rr <= 1e15f) {
+        float rc = Rr * 1.001f;
+        rcut2 = fmax_w(rc * rc, 0.0001f);
+    }
+    f.rcut2 = rcut2;
+    f.pad[0] = f.pad[1] = f.pad[2] = 0.0f;
     fwd[i] = f;
 
     BwdRec b;

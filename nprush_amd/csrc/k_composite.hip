@@ -2,25 +2,27 @@
 // reverse pass with fixed-point gradients (REF semantics of
 // splat_backward.wgsl:213-455).
 //
-// One 256-thread workgroup per 16x16 tile, one thread per pixel.  The tile list
-// (ascending splat ids, first `cap` of them) is walked in batches of 64 splat
-// records staged in LDS.
-//   forward : per pixel front-to-back alpha compositing; the transmittance T
-//             is written to a per-(tile, chunk, pixel) checkpoint every 16
-//             entries (coalesced, 1 KB per tile per chunk);
-//   reverse : batches and 16-entry chunks in reverse order.  For each chunk the
-//             alphas and T values are recomputed from the checkpoint into LDS
-//             (bit-identical to the forward values, same expressions), then
-//             walked backwards.  The reference keeps alphas[1024]/Ts[1025] per
-//             thread instead (8 KB of private memory per pixel).
+// One 256-thread workgroup per 16x16 tile, one thread per pixel, each wave an
+// 8x8 pixel block.  The tile list (ascending splat ids, first `cap` of them)
+// is walked in batches of 64 splat records staged in LDS.
+//   forward : per pixel front-to-back alpha compositing.  Every 8 entries the
+//             transmittance T is checkpointed ([tile][chunk][pixel], coalesced),
+//             and each wave records one bit per entry: does any of its 64 pixels
+//             reach alpha >= 0.001 (the reference's reverse-pass condition,
+//             BW:263)?  Pairs with r^2 beyond the splat's cut radius skip
+//             pow/exp: their alpha is provably exactly 0.
+//   reverse : batches and 8-entry chunks in reverse order.  Chunks without an
+//             active bit are skipped by the whole wave.  Otherwise alpha, T,
+//             log2 r and r^a are recomputed from the checkpoint into registers up
+//             to the last active entry (same expressions, same bits as the
+//             forward), and the active entries are walked backwards.  The
+//             reference keeps alphas[1024]/Ts[1025] per thread instead.
 //   grads   : per pair the 16 gradients are converted to i32 exactly as the
 //             reference's atomicAdd(i32(g * FP)) does; a wave sums them with a
 //             17-shuffle transpose reduction (integer adds, so the sum equals
 //             the reference's order-independent atomics), 16 lanes add the
 //             totals into an LDS accumulator per batch, and each batch ends with
 //             one global atomic per non-zero (splat, parameter).
-// Waves whose 64 pixels all have alpha < 0.001 for a splat skip it, which is
-// exactly the reference's per-pixel `continue` (BW:263).
 #include "nps_common.h"
 #include "nps_kernels.h"
 
@@ -75,7 +77,7 @@ __device__ __forceinline__ uint32_t wave_reduce16(uint32_t (&v)[16], int lane) {
     return v[0];
 }
 
-__device__ __forceinline__ uint32_t gi(float g, float fp) { return (uint32_t)f2i_sat(g * fp); }
+__device__ __forceinline__ uint32_t gi(float g, float fp) { return (uint32_t)f2i_hw(g * fp); }
 
 struct PixelState {
     float dCr, dCg, dCb, dD;
@@ -86,9 +88,8 @@ struct PixelState {
 
 // Reverse-pass body for one (pixel, splat) pair: splat_backward.wgsl:264-455
 // with the per-splat factors taken from the records.  Writes 16 i32 values.
-__device__ __forceinline__ void pair_grads(const FwdRec& f, const BwdRec& b, float px, float py, float a,
-                                           float T_prev, PixelState& st, uint32_t (&out)[16]) {
-    PairGeom g = pair_geom(f, px, py);
+__device__ __forceinline__ void pair_grads(const FwdRec& f, const BwdRec& b, const PairR2& g, float log2r,
+                                           float rpa, float a, float T_prev, PixelState& st, uint32_t (&out)[16]) {
     const float cr = f.r, cg = f.g, cb = f.b, opacity = f.op;
     float Ta = T_prev * a;
     float dColor_r = st.dCr * Ta, dColor_g = st.dCg * Ta, dColor_b = st.dCb * Ta;
@@ -101,14 +102,14 @@ __device__ __forceinline__ void pair_grads(const FwdRec& f, const BwdRec& b, flo
     float d_depth = (st.dD * T_prev) * a;
     const float sx = b.s3[0], sy = b.s3[1], sz = b.s3[2];
     const float shape_a = f.sa, shape_b = f.sb;
-    float power = g.power;
+    float power = -shape_b * rpa;
     bool above = power > -15.0f;
     float a_un = above ? exp_det(power) : 0.0f;
     float d_opacity = above ? da * a_un : 0.0f;
     float d_power = above ? (da * opacity) * a_un : 0.0f;
-    float r_pow_a = g.rpa;
-    float r_pow_a_m2 = exp2_det((shape_a - 2.0f) * g.log2r);
-    float log_r = g.log2r * kLn2;
+    float r_pow_a = rpa;
+    float r_pow_a_m2 = exp2_det((shape_a - 2.0f) * log2r);
+    float log_r = log2r * kLn2;
     float d_shape_a = above ? d_power * ((-shape_b * r_pow_a) * log_r) : 0.0f;
     float d_shape_b = above ? d_power * (-r_pow_a) : 0.0f;
     float kk = d_power * ((-shape_b * shape_a) * r_pow_a_m2);
@@ -190,6 +191,22 @@ __device__ __forceinline__ void pair_grads(const FwdRec& f, const BwdRec& b, flo
     out[15] = gi(d_sz, FPP);
 }
 
+// alpha of a pair from its r^2 (BW:232-237); log2 r and r^a are returned for
+// the reverse pass.  Beyond rcut2 the float power is <= -15, so alpha is 0.
+__device__ __forceinline__ float alpha_of(const FwdRec& f, float r2, float& log2r, float& rpa) {
+    if (r2 > f.rcut2) {
+        log2r = 0.0f;
+        rpa = 0.0f;
+        return 0.0f;
+    }
+    float r = sqrtf(fmax_w(r2, 0.0001f));
+    log2r = log2_det(r);
+    rpa = exp2_det(f.sa * log2r);
+    float power = -f.sb * rpa;
+    float a = (power > -15.0f) ? exp_det(power) * f.op : 0.0f;
+    return clamp_w(a, 0.0f, 0.999f);
+}
+
 __device__ __forceinline__ void stage_records(const FwdRec* __restrict__ fwd, const BwdRec* __restrict__ bwd,
                                               const uint64_t* __restrict__ keys, uint32_t base, int nb,
                                               FwdRec* s_fwd, BwdRec* s_bwd, uint32_t* s_ids, bool with_bwd) {
@@ -220,8 +237,7 @@ __global__ __launch_bounds__(256) void k_composite_ref(ViewParams v, CompositeAr
     __shared__ BwdRec s_bwd[kBatch];
     __shared__ uint32_t s_ids[kBatch];
     __shared__ uint32_t s_grad[kBatch][16];
-    __shared__ float s_a[kChunk][kTilePx];
-    __shared__ float s_T[kChunk][kTilePx];
+    __shared__ uint32_t s_act[4][kMaxCap / 32];
     __shared__ double s_red[2][4];
 
     const uint32_t tile = xcd_remap(blockIdx.x, v.num_tiles);
@@ -229,8 +245,9 @@ __global__ __launch_bounds__(256) void k_composite_ref(ViewParams v, CompositeAr
     const uint32_t start = A.ranges[2 * tile], end = A.ranges[2 * tile + 1];
     const uint32_t len = end - start;
     const int count = (int)min(len, A.cap);
-    const int tid = threadIdx.x, lane = tid & 63;
-    const uint32_t px = tx * kTile + (tid & 15), py = ty * kTile + (tid >> 4);
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const uint32_t px = tx * kTile + (wid & 1) * 8 + (lane & 7);
+    const uint32_t py = ty * kTile + (wid >> 1) * 8 + (lane >> 3);
     const bool valid = px < v.W && py < v.H;
     const float pxf = px_to_x(px, v.W, v.aspect);
     const float pyf = py_to_y(py, v.H);
@@ -238,16 +255,24 @@ __global__ __launch_bounds__(256) void k_composite_ref(ViewParams v, CompositeAr
 
     // ---------------- forward (BW:216-248) ----------------
     float T = 1.0f, Cr = 0.0f, Cg = 0.0f, Cb = 0.0f, D = 0.0f;
+    uint32_t actword = 0u;  // wave-uniform activity bits of the current 32 entries
     for (int b0 = 0; b0 < count; b0 += kBatch) {
         int nb = min(kBatch, count - b0);
         __syncthreads();
         stage_records(A.fwd, A.bwd, A.keys, start + (uint32_t)b0, nb, s_fwd, s_bwd, s_ids, false);
         __syncthreads();
         for (int j = 0; j < nb; ++j) {
-            if ((j & (kChunk - 1)) == 0) ckpt[(size_t)((b0 + j) / kChunk) * kTilePx + tid] = T;
+            const int e = b0 + j;
+            if ((e & (kChunk - 1)) == 0) ckpt[(size_t)(e / kChunk) * kTilePx + tid] = T;
             const FwdRec& f = s_fwd[j];
-            PairGeom g = pair_geom(f, pxf, pyf);
-            float a = pair_alpha(f, g);
+            PairR2 q = pair_r2(f, pxf, pyf);
+            float l2r, rpa;
+            float a = alpha_of(f, q.r2, l2r, rpa);
+            if (__ballot(valid && !(a < 0.001f)) != 0ull) actword |= 1u << (e & 31);
+            if ((e & 31) == 31 || e == count - 1) {
+                if (lane == 0) s_act[wid][e >> 5] = actword;
+                actword = 0u;
+            }
             float Ta = T * a;
             Cr = Cr + Ta * f.r;
             Cg = Cg + Ta * f.g;
@@ -307,8 +332,8 @@ __global__ __launch_bounds__(256) void k_composite_ref(ViewParams v, CompositeAr
         l1 += __shfl_xor(l1, o, 64);
     }
     if (lane == 0) {
-        s_red[0][tid >> 6] = l2;
-        s_red[1][tid >> 6] = l1;
+        s_red[0][wid] = l2;
+        s_red[1][wid] = l1;
     }
 
     // ---------------- reverse pass (BW:259-455) ----------------
@@ -323,34 +348,44 @@ __global__ __launch_bounds__(256) void k_composite_ref(ViewParams v, CompositeAr
         __syncthreads();
         const int nch = (nb + kChunk - 1) / kChunk;
         for (int c = nch - 1; c >= 0; --c) {
-            const int j0 = c * kChunk, j1 = min(j0 + kChunk, nb);
-            float Tc = ckpt[(size_t)((b0 + j0) / kChunk) * kTilePx + tid];
-            for (int j = j0; j < j1; ++j) {
-                const FwdRec& f = s_fwd[j];
-                PairGeom g = pair_geom(f, pxf, pyf);
-                float a = pair_alpha(f, g);
-                s_a[j - j0][tid] = a;
-                s_T[j - j0][tid] = Tc;
-                Tc = Tc * (1.0f - a);
-            }
-            for (int j = j1 - 1; j >= j0; --j) {
-                float a = s_a[j - j0][tid];
-                float Tp = s_T[j - j0][tid];
-                bool active = valid && !(a < 0.001f);
-                uint64_t bal = __ballot(active);
-                if (bal == 0ull) continue;
-                n_active += (uint32_t)__popcll(bal);
-                uint32_t gv[16];
-                if (active) {
-                    pair_grads(s_fwd[j], s_bwd[j], pxf, pyf, a, Tp, st, gv);
-                } else {
+            const int e0 = b0 + c * kChunk;
+            const uint32_t bits = (s_act[wid][e0 >> 5] >> (e0 & 31)) & ((1u << kChunk) - 1u);
+            if (bits == 0u) continue;
+            const int hi = 31 - __clz((int)bits);
+            float Tc = ckpt[(size_t)(e0 / kChunk) * kTilePx + tid];
+            float ra[kChunk], rT[kChunk], rl[kChunk], rp[kChunk];
 #pragma unroll
-                    for (int k = 0; k < 16; ++k) gv[k] = 0u;
+            for (int j = 0; j < kChunk; ++j) {
+                if (j <= hi) {
+                    const FwdRec& f = s_fwd[c * kChunk + j];
+                    PairR2 q = pair_r2(f, pxf, pyf);
+                    float a = alpha_of(f, q.r2, rl[j], rp[j]);
+                    ra[j] = a;
+                    rT[j] = Tc;
+                    Tc = Tc * (1.0f - a);
                 }
-                uint32_t tot = wave_reduce16(gv, lane);
-                if ((lane & 3) == 0 && tot != 0u) {
-                    int idx = ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);
-                    atomicAdd(&s_grad[j][idx], tot);
+            }
+#pragma unroll
+            for (int j = kChunk - 1; j >= 0; --j) {
+                if ((bits >> j) & 1u) {
+                    const int jj = c * kChunk + j;
+                    const bool active = valid && !(ra[j] < 0.001f);
+                    n_active += (uint32_t)__popcll(__ballot(active));
+                    uint32_t gv[16];
+                    if (active) {
+                        const FwdRec& f = s_fwd[jj];
+                        PairR2 q = pair_r2(f, pxf, pyf);
+                        pair_grads(f, s_bwd[jj], q, rl[j], rp[j], ra[j], rT[j], st, gv);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 16; ++k) gv[k] = 0u;
+                    }
+                    uint32_t tot = wave_reduce16(gv, lane);
+                    if ((lane & 3) == 0 && tot != 0u) {
+                        int idx = ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 +
+                                  ((lane >> 2) & 1);
+                        atomicAdd(&s_grad[jj][idx], tot);
+                    }
                 }
             }
         }

@@ -16,7 +16,9 @@ __global__ void k_debug_math(int op, const float* x, const float* y, float* out,
         case 2: r = pow_det(a, y[i]); break;
         case 3: r = exp_det(a); break;
         case 4: r = log_det(a); break;
-        default: r = sin_det(a); break;
+        case 6: r = __int_as_float(f2i_hw(a)); break;
+        case 5: r = sin_det(a); break;
+        default: r = 0.0f; break;
     }
     out[i] = r;
 }

@@ -376,6 +376,7 @@ int nps_backward(nps_handle h) {
                       (uint4*)sc.rects.p);
     if ((rc = build_tile_lists(h, sc, v, n, 1, 2)) != NPS_OK) return rc;
     const uint32_t cap = (uint32_t)h->cfg.max_tile_splats;
+    if (cap > (uint32_t)kMaxCap) return set_err(h, NPS_ERR_ARG, "max_tile_splats above 1024 is not supported");
     const uint32_t nchunk = (cap + kChunk - 1) / kChunk;
     size_t px = (size_t)h->W * h->H;
     if ((rc = ensure(h, h->ckpt, sizeof(float) * (size_t)v.num_tiles * nchunk * kTilePx)) != NPS_OK) return rc;

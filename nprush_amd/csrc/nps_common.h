@@ -16,7 +16,8 @@ namespace nps {
 constexpr int kTile = 16;        // reference workgroup tile, splat_backward.wgsl:136
 constexpr int kTilePx = kTile * kTile;
 constexpr int kBatch = 64;       // splat records staged in LDS per batch
-constexpr int kChunk = 16;       // transmittance checkpoint stride (entries)
+constexpr int kChunk = 8;        // transmittance checkpoint stride (entries)
+constexpr int kMaxCap = 1024;    // longest tile list the REF composite handles (the reference's cap)
 
 // Per-view constants, passed by value to every kernel.
 struct ViewParams {
@@ -36,7 +37,8 @@ struct alignas(16) FwdRec {
     float r, g, b;        // colour
     float sa, sb;         // shape a, shape b
     float depth;          // clamp(1 - 0.1/max(w,0.1), 0, 1)
-    float pad[4];
+    float rcut2;          // r^2 beyond which power <= -15 is guaranteed (alpha is exactly 0)
+    float pad[3];
 };
 
 // Per-splat backward record (128 B).
@@ -126,6 +128,29 @@ __device__ __forceinline__ PairGeom pair_geom(const FwdRec& f, float px, float p
     g.rpa = exp2_det(f.sa * g.log2r);  // pow(r, shape_a)
     g.power = -f.sb * g.rpa;
     return g;
+}
+
+// r^2 of a pair, d = p - centre, sp = conic * d (splat_backward.wgsl:98,127-131).
+struct PairR2 {
+    float dx, dy, spx, spy, r2;
+};
+__device__ __forceinline__ PairR2 pair_r2(const FwdRec& f, float px, float py) {
+    PairR2 q;
+    q.dx = px - f.scr_x;
+    q.dy = py - f.scr_y;
+    q.spx = f.A * q.dx + f.B * q.dy;
+    q.spy = f.B * q.dx + f.C * q.dy;
+    q.r2 = q.dx * q.spx + q.dy * q.spy;
+    return q;
+}
+
+// Saturating float -> int32 conversion in one instruction: gfx950's
+// v_cvt_i32_f32 truncates toward zero, clamps out-of-range values and maps
+// NaN to 0, which is exactly f2i_sat (checked against it on the GPU).
+__device__ __forceinline__ int32_t f2i_hw(float x) {
+    int32_t r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
 }
 
 // alpha = clamp(select(0, exp(power)*opacity, power > -15), 0, 0.999)  (BW:236-237)

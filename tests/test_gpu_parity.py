@@ -56,6 +56,21 @@ def test_math_bit_identity(oracle, op):
     assert np.array_equal(g.view(np.uint32), c.view(np.uint32)) or np.array_equal(g, c, equal_nan=True)
 
 
+def test_float_to_int_instruction_matches_wgsl_conversion(oracle):
+    """The gradient path converts with gfx950's v_cvt_i32_f32; it must equal the
+    reference's saturating i32(f32) (NaN -> 0) on every class of input."""
+    lib = oracle.load()
+    rng = np.random.default_rng(6)
+    x = np.concatenate([
+        np.array([0.0, -0.0, 0.5, -0.5, 1.5, -1.5, 2147483520.0, -2147483648.0, 2147483648.0, -2147483904.0,
+                  3e9, -3e9, np.inf, -np.inf, np.nan, 1e-40, -1e-40, 16777217.0], np.float32),
+        rng.normal(0, 1e4, 5000).astype(np.float32),
+        (rng.normal(0, 1, 2000) * 10.0 ** rng.uniform(0, 12, 2000)).astype(np.float32)])
+    got = debug_math(6, x).view(np.int32)
+    want = np.array([lib.ora_f2i(float(v)) for v in x], dtype=np.int32)
+    assert np.array_equal(got, want)
+
+
 def test_config0_tiles_and_backward(oracle):
     cfg = scenes.CONFIG0
     W, H = cfg["width"], cfg["height"]

@@ -412,3 +412,26 @@ def test_render_single_splat_closed_form(oracle):
     assert np.allclose(c[0, 0], [0.05, 0.05, 0.05, 1.0]) and np.allclose(d[0, 0], 1.0)
     cq, _ = oracle.render_f32(s, vp, W, H, clear=True, quantize=True)
     assert np.allclose(cq * 255, np.rint(cq * 255), atol=1e-4)
+
+
+# ---------------------------------------------------------------- cut radius
+def test_cut_radius_gives_exact_zero_alpha(oracle):
+    """The GPU skips pow/exp for pairs with r^2 > rcut2 = max((1.001 R)^2, 1e-4),
+    R = pow(15/max(b,1e-4), 1/a), when a in [0.1, 64] and b >= 1e-4.  Check with
+    the deterministic FP32 functions that every such pair has power <= -15,
+    i.e. alpha is exactly 0 in the reference's formula (BW:236-237)."""
+    lib = oracle.load()
+    rng = np.random.default_rng(12)
+    f32 = np.float32
+    for _ in range(20000):
+        a = f32(np.exp(rng.uniform(np.log(0.1), np.log(64.0))))
+        b = f32(np.exp(rng.uniform(np.log(1e-4), np.log(1e4))))
+        R = f32(lib.ora_powf(float(f32(15.0) / max(b, f32(1e-4))), float(f32(1.0) / a)))
+        if not (0 < R <= 1e15):
+            continue
+        rc = f32(R * f32(1.001))
+        rcut2 = max(f32(rc * rc), f32(1e-4))
+        for r2 in (np.nextafter(rcut2, f32(np.inf)), f32(rcut2 * f32(1.0 + rng.uniform(0, 0.01)))):
+            r = f32(np.sqrt(max(r2, f32(1e-4))))
+            power = f32(-b * f32(lib.ora_powf(float(r), float(a))))
+            assert not (power > f32(-15.0)), (a, b, r2, power)
