@@ -191,6 +191,8 @@ def main():
         "tile_keys_per_view": int(keys),
         "pairs_per_view": int(pairs),
         "active_pairs_per_view": int(active),
+        "reverse_lane_efficiency": round(active / max(1.0, 64.0 * float(np.mean([s["reverse_wave_entries"] for _, s in rows]))), 3),
+        "forward_eval_fraction": round(64.0 * float(np.mean([s["forward_wave_evals"] for _, s in rows])) / max(pairs, 1.0), 3),
         "adam_hbm_gbs": round(adam_gbs, 1),
         "adam_roofline_frac": round(adam_gbs / PEAK_HBM_GBS, 3),
         "steps_per_s": round(args.steps / wall, 2),

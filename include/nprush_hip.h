@@ -112,7 +112,8 @@ int nps_read_tiles(nps_handle h, uint32_t* ranges, uint32_t* ids, int32_t* num_t
 /* Counters of the last backward/ADC: stats[0]=keys, [1]=tiles over the cap,
  * [2]=(pixel, splat) pairs composited, [3]=pairs taken by the reverse pass
  * (alpha >= 0.001), [4]=ADC qualifiers, [5]=ADC children, [6]=ADC splits,
- * [7]=ADC clones, [8]=ADC cascade rounds, [9..11]=0. */
+ * [7]=ADC clones, [8]=ADC cascade rounds, [9]=wave-entries walked by the
+ * reverse pass, [10]=forward wave-entries that evaluate pow/exp, [11]=0. */
 int nps_read_stats(nps_handle h, int64_t stats[12]);
 /* Per-phase GPU time (ms, HIP events on the handle's stream) of the last
  * backward/step/adc when timing is enabled: [0]=preprocess+keys, [1]=sort,

@@ -199,9 +199,8 @@ __device__ __forceinline__ float alpha_of(const FwdRec& f, float r2, float& log2
         rpa = 0.0f;
         return 0.0f;
     }
-    float r = sqrtf(fmax_w(r2, 0.0001f));
-    log2r = log2_det(r);
-    rpa = exp2_det(f.sa * log2r);
+    log2r = log2_sqrt_det(fmax_w(r2, 0.0001f));  // log2 of r = sqrt(max(r2, 1e-4))
+    rpa = exp2_det(f.sa * log2r);                 // pow(r, shape_a)
     float power = -f.sb * rpa;
     float a = (power > -15.0f) ? exp_det(power) * f.op : 0.0f;
     return clamp_w(a, 0.0f, 0.999f);
@@ -256,6 +255,7 @@ __global__ __launch_bounds__(256) void k_composite_ref(ViewParams v, CompositeAr
     // ---------------- forward (BW:216-248) ----------------
     float T = 1.0f, Cr = 0.0f, Cg = 0.0f, Cb = 0.0f, D = 0.0f;
     uint32_t actword = 0u;  // wave-uniform activity bits of the current 32 entries
+    uint32_t n_feval = 0u;  // wave-entries that evaluate pow/exp in the forward
     for (int b0 = 0; b0 < count; b0 += kBatch) {
         int nb = min(kBatch, count - b0);
         __syncthreads();
@@ -267,6 +267,7 @@ __global__ __launch_bounds__(256) void k_composite_ref(ViewParams v, CompositeAr
             const FwdRec& f = s_fwd[j];
             PairR2 q = pair_r2(f, pxf, pyf);
             float l2r, rpa;
+            n_feval += (uint32_t)(__ballot(valid && !(q.r2 > f.rcut2)) != 0ull);
             float a = alpha_of(f, q.r2, l2r, rpa);
             if (__ballot(valid && !(a < 0.001f)) != 0ull) actword |= 1u << (e & 31);
             if ((e & 31) == 31 || e == count - 1) {
@@ -339,6 +340,7 @@ __global__ __launch_bounds__(256) void k_composite_ref(ViewParams v, CompositeAr
     // ---------------- reverse pass (BW:259-455) ----------------
     const int nbatch = (count + kBatch - 1) / kBatch;
     uint32_t n_active = 0;  // wave-uniform count of pairs taken by the reverse pass
+    uint32_t n_went = 0;    // wave-entries walked by the reverse pass
     for (int bi = nbatch - 1; bi >= 0; --bi) {
         const int b0 = bi * kBatch;
         const int nb = min(kBatch, count - b0);
@@ -371,6 +373,7 @@ __global__ __launch_bounds__(256) void k_composite_ref(ViewParams v, CompositeAr
                     const int jj = c * kChunk + j;
                     const bool active = valid && !(ra[j] < 0.001f);
                     n_active += (uint32_t)__popcll(__ballot(active));
+                    n_went += 1u;
                     uint32_t gv[16];
                     if (active) {
                         const FwdRec& f = s_fwd[jj];
@@ -395,7 +398,11 @@ __global__ __launch_bounds__(256) void k_composite_ref(ViewParams v, CompositeAr
             if (val != 0u) atomicAdd((unsigned int*)&A.grads[16 * (size_t)s_ids[e >> 4] + (e & 15)], val);
         }
     }
-    if (lane == 0 && n_active) atomicAdd(&A.stats[2], (unsigned long long)n_active);
+    if (lane == 0) {
+        if (n_active) atomicAdd(&A.stats[2], (unsigned long long)n_active);
+        if (n_went) atomicAdd(&A.stats[3], (unsigned long long)n_went);
+        if (n_feval) atomicAdd(&A.stats[4], (unsigned long long)n_feval);
+    }
     __syncthreads();
     if (tid == 0) {
         double a2 = (s_red[0][0] + s_red[0][1]) + (s_red[0][2] + s_red[0][3]);

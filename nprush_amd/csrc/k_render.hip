@@ -121,8 +121,8 @@ __global__ __launch_bounds__(256) void k_render(ViewParams v, RenderArgs A) {
             if (!(dx >= -r3.x && dx <= r3.x && dy >= -r3.y && dy <= r3.y)) continue;
             float Aa = r0.z, Bb = r0.w, Cc = r1.x;
             float rr2 = ((Aa * dx) * dx + ((2.0f * Bb) * dx) * dy) + (Cc * dy) * dy;
-            float r = sqrtf(fmax_w(rr2, 0.0001f));
-            float pw = -r2_.z * pow_det(r, r2_.y);
+            float lr = log2_sqrt_det(fmax_w(rr2, 0.0001f));
+            float pw = -r2_.z * exp2_det(r2_.y * lr);
             float a = (pw > -15.0f) ? exp_det(pw) * r1.y : 0.0f;
             a = clamp_w(a, 0.0f, 0.999f);
             if (a < 0.001f) continue;

@@ -529,11 +529,13 @@ int nps_read_tiles(nps_handle h, uint32_t* ranges, uint32_t* ids, int32_t* num_t
 int nps_read_stats(nps_handle h, int64_t stats[12]) {
     if (!h || !stats) return NPS_ERR_ARG;
     if (h->have_backward && h->stats.p) {
-        unsigned long long s[3] = {0, 0, 0};
+        unsigned long long s[5] = {0, 0, 0, 0, 0};
         NPS_COPY(h, s, h->stats.p, sizeof(s), hipMemcpyDeviceToHost);
         h->stats_host[2] = (int64_t)s[0];
         h->stats_host[1] = (int64_t)s[1];
         h->stats_host[3] = (int64_t)s[2];
+        h->stats_host[9] = (int64_t)s[3];
+        h->stats_host[10] = (int64_t)s[4];
     }
     for (int k = 0; k < 12; ++k) stats[k] = h->stats_host[k];
     return NPS_OK;

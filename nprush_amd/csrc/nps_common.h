@@ -110,26 +110,6 @@ NPS_HD F2 project_axis(const float* vp, F3 ax, float cx, float cy, float w, floa
     return o;
 }
 
-// Per-pair geometry shared by forward and backward: d = p - centre,
-// sp = conic * d, r = sqrt(max(d.sp, 1e-4)) (splat_backward.wgsl:98,127-132).
-struct PairGeom {
-    float dx, dy, spx, spy, r, log2r, rpa, power;
-};
-
-__device__ __forceinline__ PairGeom pair_geom(const FwdRec& f, float px, float py) {
-    PairGeom g;
-    g.dx = px - f.scr_x;
-    g.dy = py - f.scr_y;
-    g.spx = f.A * g.dx + f.B * g.dy;
-    g.spy = f.B * g.dx + f.C * g.dy;
-    float r2 = g.dx * g.spx + g.dy * g.spy;
-    g.r = sqrtf(fmax_w(r2, 0.0001f));
-    g.log2r = log2_det(g.r);
-    g.rpa = exp2_det(f.sa * g.log2r);  // pow(r, shape_a)
-    g.power = -f.sb * g.rpa;
-    return g;
-}
-
 // r^2 of a pair, d = p - centre, sp = conic * d (splat_backward.wgsl:98,127-131).
 struct PairR2 {
     float dx, dy, spx, spy, r2;
@@ -151,12 +131,6 @@ __device__ __forceinline__ int32_t f2i_hw(float x) {
     int32_t r;
     asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
     return r;
-}
-
-// alpha = clamp(select(0, exp(power)*opacity, power > -15), 0, 0.999)  (BW:236-237)
-__device__ __forceinline__ float pair_alpha(const FwdRec& f, const PairGeom& g) {
-    float a = (g.power > -15.0f) ? exp_det(g.power) * f.op : 0.0f;
-    return clamp_w(a, 0.0f, 0.999f);
 }
 
 }  // namespace nps

@@ -20,7 +20,8 @@ struct CompositeArgs {
     float* d_img;             // H*W
     double* tile_loss;        // 2 * num_tiles
     float* ckpt;              // num_tiles * nchunk_max * 256
-    unsigned long long* stats;  // [0] pairs, [1] tiles over the cap, [2] pairs in the reverse pass
+    unsigned long long* stats;  // [0] pairs, [1] tiles over the cap, [2] pairs in the reverse pass,
+                                // [3] reverse wave-entries, [4] forward wave-entries evaluating pow/exp
     uint32_t cap;
     uint32_t nchunk_max;
 };

@@ -3,14 +3,15 @@
 // WGSL leaves exp/pow/log/sin implementation-defined within ULP bounds
 // (pow is defined as exp2(y * log2(x))).  The kernels need a definition that
 // is identical on every device and on a CPU, so these are built only from
-// correctly rounded IEEE-754 +,-,*,/ (hipcc's default for gfx950: division and
-// sqrt are correctly rounded, fp32 denormals kept) and compiled with
-// -ffp-contract=off:
+// correctly rounded IEEE-754 +,-,*,/ and explicit fmaf (hipcc's default for
+// gfx950: division and sqrt are correctly rounded, fp32 denormals kept) and
+// compiled with -ffp-contract=off:
 //   exp2: Cody-Waite split x = k + f (|f| <= 1/2), degree-7 Taylor series of
-//         e^(f ln 2) in Horner form, exact scaling by 2^k; results below
+//         e^(f ln 2), fmaf Horner, exact scaling by 2^k; results below
 //         2^-126 flush to zero.
-//   log2: x = m 2^e, m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s),
-//         s = (m-1)/(m+1), odd series to s^11.
+//   log2: x = m 2^e, m in [sqrt(1/2), sqrt(2)); log2 m = f P(f), f = m - 1,
+//         P = degree-9 Chebyshev interpolant of log2(1+f)/f, fmaf Horner.
+//   log2_sqrt(x) = log2(x)/2 (log2 of a square root without rounding the root).
 //   pow(x,y) = exp2(y log2 x); log(x) = log2(x) ln 2; exp(x) = exp2(x log2 e).
 //   sin: double-precision Cody-Waite reduction by pi/2 (three-part constant,
 //        exact for |x| < 2^27) and Taylor polynomials, rounded to float.
@@ -33,13 +34,13 @@ NPS_HD float exp2_det(float x) {
     float k = rintf(x);
     float f = x - k;
     float p = 0x1.ffcbfcp-17f;
-    p = p * f + 0x1.430912p-13f;
-    p = p * f + 0x1.5d87fep-10f;
-    p = p * f + 0x1.3b2ab6p-7f;
-    p = p * f + 0x1.c6b08ep-5f;
-    p = p * f + 0x1.ebfbe0p-3f;
-    p = p * f + 0x1.62e430p-1f;
-    p = p * f + 1.0f;
+    p = fmaf(p, f, 0x1.430912p-13f);
+    p = fmaf(p, f, 0x1.5d87fep-10f);
+    p = fmaf(p, f, 0x1.3b2ab6p-7f);
+    p = fmaf(p, f, 0x1.c6b08ep-5f);
+    p = fmaf(p, f, 0x1.ebfbe0p-3f);
+    p = fmaf(p, f, 0x1.62e430p-1f);
+    p = fmaf(p, f, 1.0f);
     float r = ldexpf(p, (int)k);
     return r < 0x1p-126f ? 0.0f : r;
 }
@@ -55,17 +56,22 @@ NPS_HD float log2_det(float x) {
         m = m * 2.0f;
         e = e - 1;
     }
-    float s = (m - 1.0f) / (m + 1.0f);
-    float s2 = s * s;
-    float p = 0x1.745d18p-4f;
-    p = p * s2 + 0x1.c71c72p-4f;
-    p = p * s2 + 0x1.24924ap-3f;
-    p = p * s2 + 0x1.99999ap-3f;
-    p = p * s2 + 0x1.555556p-2f;
-    float two_s = s * 2.0f;
-    float lnm = two_s + (two_s * s2) * p;
-    return (float)e + lnm * kLog2e;
+    float f = m - 1.0f;
+    float p = -0x1.b84fb6p-4f;
+    p = fmaf(p, f, 0x1.7a63a0p-3f);
+    p = fmaf(p, f, -0x1.87f6a2p-3f);
+    p = fmaf(p, f, 0x1.a38c68p-3f);
+    p = fmaf(p, f, -0x1.eab7aap-3f);
+    p = fmaf(p, f, 0x1.277a52p-2f);
+    p = fmaf(p, f, -0x1.715a70p-2f);
+    p = fmaf(p, f, 0x1.ec70aap-2f);
+    p = fmaf(p, f, -0x1.715470p-1f);
+    p = fmaf(p, f, 0x1.715476p+0f);
+    return fmaf(f, p, (float)e);
 }
+
+// log2(sqrt(x)) as log2(x)/2: no intermediate rounding of the root.
+NPS_HD float log2_sqrt_det(float x) { return 0.5f * log2_det(x); }
 
 NPS_HD float pow_det(float x, float y) { return exp2_det(y * log2_det(x)); }
 NPS_HD float log_det(float x) { return log2_det(x) * kLn2; }

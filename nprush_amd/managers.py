@@ -204,7 +204,7 @@ class GpuSplatOptimizerManager:
         s = np.zeros(12, dtype=np.int64)
         check(self._lib.nps_read_stats(self._h, lptr(s)), self._h, "read_stats")
         keys = ["keys", "tiles_over_cap", "pairs", "active_pairs", "adc_qualifiers", "adc_children", "adc_splits",
-                "adc_clones", "adc_rounds"]
+                "adc_clones", "adc_rounds", "reverse_wave_entries", "forward_wave_evals"]
         return dict(zip(keys, (int(x) for x in s)))
 
     def enable_timing(self, on: bool = True):

@@ -30,10 +30,10 @@
 /* ---------------- FP32 instantiation (bit-level REF semantics) ---------------- */
 #define R float
 #define SFX(name) name##_f32
-#define RPOW rm_powf
+#define REXP2 rm_exp2f
 #define REXP rm_expf
-#define RLOG rm_logf
-#define RSQRT sqrtf
+#define RLOG2SQRT rm_log2_sqrtf
+#define RLN2 RM_LN2F
 #define GRAD_T uint32_t
 /* atomicAdd(&grads[k], i32(g * FP)) with wrapping two's-complement sums */
 #define GRAD_ADD(p, g, fp)                                   \
@@ -44,29 +44,29 @@
 #include "oracle_core.inc"
 #undef R
 #undef SFX
-#undef RPOW
+#undef REXP2
 #undef REXP
-#undef RLOG
-#undef RSQRT
+#undef RLOG2SQRT
+#undef RLN2
 #undef GRAD_T
 #undef GRAD_ADD
 
 /* ---------------- FP64 twin ---------------- */
 #define R double
 #define SFX(name) name##_f64
-#define RPOW pow
+#define REXP2 exp2
 #define REXP exp
-#define RLOG log
-#define RSQRT sqrt
+#define RLOG2SQRT(x) (0.5 * log2(x))
+#define RLN2 0.69314718055994530942
 #define GRAD_T double
 #define GRAD_ADD(p, g, fp) do { (void)(fp); *(p) += (g); } while (0)
 #include "oracle_core.inc"
 #undef R
 #undef SFX
-#undef RPOW
+#undef REXP2
 #undef REXP
-#undef RLOG
-#undef RSQRT
+#undef RLOG2SQRT
+#undef RLN2
 #undef GRAD_T
 #undef GRAD_ADD
 
@@ -417,8 +417,8 @@ int ora_render_f32(const float* splats, int n, const float* vp, int W, int H, fl
                 float dx = p.x - scx, dy = p.y - scy;
                 if (!(dx >= -ext_x && dx <= ext_x && dy >= -ext_y && dy <= ext_y)) continue;
                 float r2 = ((A * dx) * dx + ((2.0f * B) * dx) * dy) + (C * dy) * dy;
-                float r = sqrtf(fmax__f32(r2, 0.0001f));
-                float pw = -shape_b * rm_powf(r, shape_a);
+                float lr = rm_log2_sqrtf(fmax__f32(r2, 0.0001f));
+                float pw = -shape_b * rm_exp2f(shape_a * lr);
                 float a = (pw > -15.0f) ? rm_expf(pw) * s[7] : 0.0f;
                 a = clamp__f32(a, 0.0f, 0.999f);
                 if (a < 0.001f) continue;

@@ -8,14 +8,17 @@
  * "inherited from exp2(y * log2(x))").  To compare a GPU path against the CPU
  * bit-for-bit, both sides must evaluate the SAME function.  This header defines
  * that function from published methods, using only IEEE-754 correctly rounded
- * +,-,*,/ and sqrt, with no fused multiply-add (build with -ffp-contract=off):
+ * +,-,*,/, sqrt and explicit fmaf, with no implicit contraction (build with
+ * -ffp-contract=off):
  *
  *   exp2f: Cody & Waite argument reduction x = k + f, |f| <= 1/2, then the
  *          Taylor series of e^(f ln2) to degree 7 (truncation < 2^-27 relative),
- *          Horner, then exact scaling by 2^k.  Results below 2^-126 flush to 0.
- *   log2f: x = m 2^e with m in [sqrt(1/2), sqrt(2)), ln m = 2 atanh(s),
- *          s = (m-1)/(m+1), series 2s(1 + s^2/3 + ... + s^10/11).
+ *          Horner with fused multiply-add (fmaf, exactly specified by IEEE-754),
+ *          then exact scaling by 2^k.  Results below 2^-126 flush to 0.
+ *   log2f: x = m 2^e with m in [sqrt(1/2), sqrt(2)), log2 m = f P(f), f = m-1,
+ *          P the degree-9 Chebyshev interpolant of log2(1+f)/f, fmaf Horner.
  *   powf(x,y) = exp2f(y * log2f(x))   (the WGSL definition).
+ *   log2 of a square root: log2_sqrt(x) = log2(x)/2.
  *   logf(x)   = log2f(x) * ln2,  expf(x) = exp2f(x * log2 e).
  *   sinf:     double-precision Cody-Waite reduction by pi/2 (three-part pi/2,
  *             exact for |x| < 2^27), Taylor polynomials, rounded to float.
@@ -39,19 +42,22 @@ static inline float rm_exp2f(float x) {
     if (x < -126.0f) return 0.0f;
     float k = rintf(x);
     float f = x - k; /* exact: |f| <= 0.5 */
-    float p = 0x1.ffcbfcp-17f;  /* ln2^7/7! */
-    p = p * f + 0x1.430912p-13f; /* ln2^6/6! */
-    p = p * f + 0x1.5d87fep-10f; /* ln2^5/5! */
-    p = p * f + 0x1.3b2ab6p-7f;  /* ln2^4/4! */
-    p = p * f + 0x1.c6b08ep-5f;  /* ln2^3/3! */
-    p = p * f + 0x1.ebfbe0p-3f;  /* ln2^2/2! */
-    p = p * f + 0x1.62e430p-1f;  /* ln2      */
-    p = p * f + 1.0f;
+    float p = 0x1.ffcbfcp-17f;         /* ln2^7/7! */
+    p = fmaf(p, f, 0x1.430912p-13f);   /* ln2^6/6! */
+    p = fmaf(p, f, 0x1.5d87fep-10f);   /* ln2^5/5! */
+    p = fmaf(p, f, 0x1.3b2ab6p-7f);    /* ln2^4/4! */
+    p = fmaf(p, f, 0x1.c6b08ep-5f);    /* ln2^3/3! */
+    p = fmaf(p, f, 0x1.ebfbe0p-3f);    /* ln2^2/2! */
+    p = fmaf(p, f, 0x1.62e430p-1f);    /* ln2      */
+    p = fmaf(p, f, 1.0f);
     float r = ldexpf(p, (int)k);
     if (r < 0x1p-126f) return 0.0f;
     return r;
 }
 
+/* log2(m) = f * P(f), f = m - 1, m in [sqrt(1/2), sqrt(2)): P is the degree-9
+ * Chebyshev interpolant of log2(1+f)/f on that interval (error < 6e-9
+ * relative), coefficients rounded to float, Horner with fmaf. */
 static inline float rm_log2f(float x) {
     if (x != x) return x;
     if (x < 0.0f) return NAN;
@@ -60,19 +66,23 @@ static inline float rm_log2f(float x) {
     int e;
     float m = frexpf(x, &e); /* m in [0.5, 1) */
     if (m < 0x1.6a09e6p-1f) { m = m * 2.0f; e = e - 1; }
-    float s = (m - 1.0f) / (m + 1.0f);
-    float s2 = s * s;
-    float p = 0x1.745d18p-4f;     /* 1/11 */
-    p = p * s2 + 0x1.c71c72p-4f;  /* 1/9 */
-    p = p * s2 + 0x1.24924ap-3f;  /* 1/7 */
-    p = p * s2 + 0x1.99999ap-3f;  /* 1/5 */
-    p = p * s2 + 0x1.555556p-2f;  /* 1/3 */
-    float two_s = s * 2.0f;
-    float lnm = two_s + (two_s * s2) * p;
-    return (float)e + lnm * RM_LOG2EF;
+    float f = m - 1.0f; /* exact (Sterbenz) */
+    float p = -0x1.b84fb6p-4f;
+    p = fmaf(p, f, 0x1.7a63a0p-3f);
+    p = fmaf(p, f, -0x1.87f6a2p-3f);
+    p = fmaf(p, f, 0x1.a38c68p-3f);
+    p = fmaf(p, f, -0x1.eab7aap-3f);
+    p = fmaf(p, f, 0x1.277a52p-2f);
+    p = fmaf(p, f, -0x1.715a70p-2f);
+    p = fmaf(p, f, 0x1.ec70aap-2f);
+    p = fmaf(p, f, -0x1.715470p-1f);
+    p = fmaf(p, f, 0x1.715476p+0f);
+    return fmaf(f, p, (float)e);
 }
 
 static inline float rm_powf(float x, float y) { return rm_exp2f(y * rm_log2f(x)); }
+/* log2(sqrt(x)) evaluated as log2(x)/2 (no intermediate rounding of the root) */
+static inline float rm_log2_sqrtf(float x) { return 0.5f * rm_log2f(x); }
 static inline float rm_logf(float x) { return rm_log2f(x) * RM_LN2F; }
 static inline float rm_expf(float x) { return rm_exp2f(x * RM_LOG2EF); }
 

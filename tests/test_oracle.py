@@ -418,8 +418,9 @@ def test_render_single_splat_closed_form(oracle):
 def test_cut_radius_gives_exact_zero_alpha(oracle):
     """The GPU skips pow/exp for pairs with r^2 > rcut2 = max((1.001 R)^2, 1e-4),
     R = pow(15/max(b,1e-4), 1/a), when a in [0.1, 64] and b >= 1e-4.  Check with
-    the deterministic FP32 functions that every such pair has power <= -15,
-    i.e. alpha is exactly 0 in the reference's formula (BW:236-237)."""
+    the deterministic FP32 functions that every such pair has
+    power = -b exp2(a log2(r2)/2) <= -15, i.e. alpha is exactly 0 in the
+    reference's formula (BW:236-237)."""
     lib = oracle.load()
     rng = np.random.default_rng(12)
     f32 = np.float32
@@ -432,6 +433,6 @@ def test_cut_radius_gives_exact_zero_alpha(oracle):
         rc = f32(R * f32(1.001))
         rcut2 = max(f32(rc * rc), f32(1e-4))
         for r2 in (np.nextafter(rcut2, f32(np.inf)), f32(rcut2 * f32(1.0 + rng.uniform(0, 0.01)))):
-            r = f32(np.sqrt(max(r2, f32(1e-4))))
-            power = f32(-b * f32(lib.ora_powf(float(r), float(a))))
+            lr = f32(f32(0.5) * f32(lib.ora_log2f(float(max(r2, f32(1e-4))))))
+            power = f32(-b * f32(lib.ora_exp2f(float(f32(a * lr)))))
             assert not (power > f32(-15.0)), (a, b, r2, power)
