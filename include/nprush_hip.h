@@ -106,9 +106,14 @@ int nps_read_image(nps_handle h, float* c_pred, float* d_pred);
  * gradient the reference seeds, BW:250-257); loss[1] = sum |C-tgt| (L1). */
 int nps_read_loss(nps_handle h, double loss[2]);
 /* Tile lists of the last nps_backward: ranges[2*T] = [start, end) into the
- * sorted list (uncapped), ids[M] = splat ids in tile order.  Pass NULL to query
- * sizes in *num_tiles / *num_ids. */
-int nps_read_tiles(nps_handle h, uint32_t* ranges, uint32_t* ids, int32_t* num_tiles, int64_t* num_ids);
+ * sorted list ids[M] (splat ids, ascending within a tile).  Each tile's range
+ * holds at least its first min(full, max_tile_splats) ids -- the ones the
+ * composite uses (splat_backward.wgsl:199-210); ids of later id blocks that
+ * cannot be among them are not emitted.  full_counts[T] = number of splats whose
+ * bounding box overlaps the tile (uncapped).  Pass NULL to query sizes in
+ * *num_tiles / *num_ids. */
+int nps_read_tiles(nps_handle h, uint32_t* ranges, uint32_t* ids, uint32_t* full_counts, int32_t* num_tiles,
+                   int64_t* num_ids);
 /* Counters of the last backward/ADC: stats[0]=keys, [1]=tiles over the cap,
  * [2]=(pixel, splat) pairs composited, [3]=pairs taken by the reverse pass
  * (alpha >= 0.001), [4]=ADC qualifiers, [5]=ADC children, [6]=ADC splits,

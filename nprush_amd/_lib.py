@@ -57,7 +57,7 @@ SIGNATURES = {
     "nps_sync": (i32, [_P]),
     "nps_read_image": (i32, [_P, _F, _F]),
     "nps_read_loss": (i32, [_P, _D]),
-    "nps_read_tiles": (i32, [_P, _U, _U, _I, _L]),
+    "nps_read_tiles": (i32, [_P, _U, _U, _U, _I, _L]),
     "nps_read_stats": (i32, [_P, _L]),
     "nps_enable_timing": (i32, [_P, i32]),
     "nps_read_timing": (i32, [_P, _F]),

@@ -9,7 +9,11 @@
 //                  tiles whose rectangle passes the reference's float overlap
 //                  test (binary search with the identical float predicate), and
 //                  the forward/backward records of the splat;
-//   k_emit_keys  : one 64-bit key (tile << 32 | splat id) per (splat, tile),
+//   k_tile_cut   : per tile, the id block in which its count of covering
+//                  splats reaches the cap (2D difference arrays per block of
+//                  splat ids, prefix-summed); later blocks cannot be among the
+//                  tile's first `cap` ascending ids, so they emit no key;
+//   k_keys_wave  : one 64-bit key (tile << 32 | splat id) per kept (splat, tile),
 //                  written in splat order so a stable sort on the tile bits
 //                  leaves every tile list in ascending id order;
 //   k_tile_ranges: [start, end) of each tile in the sorted keys.
@@ -136,20 +140,201 @@ __global__ __launch_bounds__(256) void k_preprocess(const float4* __restrict__ s
     bwd[i] = b;
 }
 
-__global__ __launch_bounds__(256) void k_emit_keys(uint32_t n, const uint32_t* __restrict__ counts,
-                                                   const uint64_t* __restrict__ offsets,
-                                                   const uint4* __restrict__ rects, uint32_t tiles_x,
-                                                   uint64_t* __restrict__ keys) {
+constexpr uint32_t kStripRows = 16;  // diff-grid rows per workgroup of k_diff_build
+constexpr uint32_t kDiffSubs = 4;    // id sub-ranges per (strip, id block), summed by k_diff_cols
+
+// 2D difference array of the rectangles of one id block sb, restricted to a
+// strip of kStripRows grid rows, accumulated with LDS atomics and prefix-summed
+// along x: after k_diff_cols, cnt[sb][ty][tx] = splats of block sb covering tile
+// (tx, ty).  Grid: (strips, id blocks).
+__global__ __launch_bounds__(256) void k_diff_build(const uint4* __restrict__ rects,
+                                                    const uint32_t* __restrict__ counts, uint32_t n,
+                                                    uint32_t splat_block, uint32_t gx, uint32_t gy, uint32_t nsb,
+                                                    int32_t* __restrict__ diff) {
+    extern __shared__ int32_t s_strip[];  // kStripRows * gx
+    const uint32_t sb = blockIdx.y, y0 = blockIdx.x * kStripRows, sub = blockIdx.z;
+    for (uint32_t k = threadIdx.x; k < kStripRows * gx; k += blockDim.x) s_strip[k] = 0;
+    __syncthreads();
+    const uint32_t part = (splat_block + kDiffSubs - 1) / kDiffSubs;
+    const uint32_t b0 = sb * splat_block, b1 = min(n, b0 + splat_block);
+    const uint32_t i0 = min(b1, b0 + sub * part), i1 = min(b1, i0 + part);
+    for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        if (counts[i] == 0u) continue;
+        uint4 r = rects[i];
+        if (r.z >= y0 && r.z < y0 + kStripRows) {
+            atomicAdd(&s_strip[(r.z - y0) * gx + r.x], 1);
+            atomicAdd(&s_strip[(r.z - y0) * gx + r.y], -1);
+        }
+        if (r.w >= y0 && r.w < y0 + kStripRows) {
+            atomicAdd(&s_strip[(r.w - y0) * gx + r.x], -1);
+            atomicAdd(&s_strip[(r.w - y0) * gx + r.y], 1);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kStripRows) {
+        int32_t acc = 0;
+        int32_t* row = s_strip + threadIdx.x * gx;
+        for (uint32_t x = 0; x < gx; ++x) {
+            acc += row[x];
+            row[x] = acc;
+        }
+    }
+    __syncthreads();
+    int32_t* out = diff + (((size_t)sub * nsb + sb) * gy + y0) * gx;
+    for (uint32_t k = threadIdx.x; k < kStripRows * gx; k += blockDim.x)
+        if (y0 + k / gx < gy) out[k] = s_strip[k];
+}
+
+// Sums the kDiffSubs partial grids of each id block and prefix-sums along y;
+// the result lands in sub-grid 0.
+__global__ void k_diff_cols(int32_t* diff, uint32_t nsb, uint32_t gx, uint32_t gy) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nsb * gx) return;
+    uint32_t sb = k / gx, x = k % gx;
+    const size_t sub_stride = (size_t)nsb * gx * gy;
+    int32_t* base = diff + (size_t)sb * gx * gy + x;
+    int32_t acc = 0;
+    for (uint32_t y = 0; y < gy; ++y) {
+        int32_t v = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < kDiffSubs; ++q) v += base[q * sub_stride + (size_t)y * gx];
+        acc += v;
+        base[(size_t)y * gx] = acc;
+    }
+}
+
+// cut[t] = first id block at which the running count of splats covering tile t
+// reaches `cap` (the last block if it never does); full[t] = total count.  The
+// first `cap` ascending ids of the tile all lie in blocks <= cut[t], so keys of
+// later blocks are never composited and are not emitted.
+__global__ void k_tile_cut(const int32_t* diff, uint32_t nsb, uint32_t gx, uint32_t gy, uint32_t tiles_x,
+                           uint32_t num_tiles, uint32_t cap, uint32_t* cut, uint32_t* full,
+                           unsigned long long* over_cap) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= num_tiles) return;
+    uint32_t tx = t % tiles_x, ty = t / tiles_x;
+    uint32_t cum = 0, c = nsb - 1u;
+    bool found = false;
+    for (uint32_t sb = 0; sb < nsb; ++sb) {
+        cum += (uint32_t)diff[((size_t)sb * gy + ty) * gx + tx];
+        if (!found && cum >= cap) {
+            c = sb;
+            found = true;
+        }
+    }
+    cut[t] = c;
+    full[t] = cum;
+    if (cum > cap) atomicAdd(over_cap, 1ull);
+}
+
+// Keys of one splat per wave iteration: the 64 lanes take 64 tiles of the
+// splat's rectangle at a time, drop tiles whose cut is below the splat's id
+// block, and (emit mode) write the kept keys contiguously, row-major in tile
+// order, at the splat's offset; count mode writes the number of kept keys.  One
+// wave per 64 consecutive splats, so a large rectangle costs its wave a few
+// more iterations instead of stalling one lane.
+// P[b][y][x] = number of tiles in [0,x) x [0,y) whose cut is >= b: a 2D prefix
+// table per id block b, so the keys a splat of block b keeps in its rectangle
+// [x0,x1) x [y0,y1) are P[b][y1][x1] - P[b][y0][x1] - P[b][y1][x0] + P[b][y0][x0].
+__global__ void k_cut_rows(const uint32_t* cut, uint32_t nsb, uint32_t tiles_x, uint32_t tiles_y, int32_t* P) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t gx = tiles_x + 1u, gy = tiles_y + 1u;
+    if (k >= nsb * gy) return;
+    const uint32_t b = k / gy, y = k % gy;
+    int32_t* row = P + ((size_t)b * gy + y) * gx;
+    int32_t acc = 0;
+    row[0] = 0;
+    for (uint32_t x = 0; x < tiles_x; ++x) {
+        if (y > 0u) acc += (cut[(y - 1u) * tiles_x + x] >= b) ? 1 : 0;
+        row[x + 1u] = acc;
+    }
+}
+__global__ void k_cut_cols(uint32_t nsb, uint32_t tiles_x, uint32_t tiles_y, int32_t* P) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t gx = tiles_x + 1u, gy = tiles_y + 1u;
+    if (k >= nsb * gx) return;
+    const uint32_t b = k / gx, x = k % gx;
+    int32_t* col = P + (size_t)b * gy * gx + x;
+    int32_t acc = 0;
+    for (uint32_t y = 0; y < gy; ++y) {
+        acc += col[(size_t)y * gx];
+        col[(size_t)y * gx] = acc;
+    }
+}
+__global__ __launch_bounds__(256) void k_count_kept(uint32_t n, const uint32_t* __restrict__ counts,
+                                                    const uint4* __restrict__ rects, uint32_t splat_block,
+                                                    uint32_t tiles_x, uint32_t tiles_y, const int32_t* __restrict__ P,
+                                                    uint32_t* __restrict__ kept) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (counts[i] == 0u) return;
-    uint4 r = rects[i];
-    uint64_t o = offsets[i];
-    for (uint32_t ty = r.z; ty < r.w; ++ty)
-        for (uint32_t tx = r.x; tx < r.y; ++tx) {
-            uint64_t tile = (uint64_t)(ty * tiles_x + tx);
-            keys[o++] = (tile << 32) | (uint64_t)i;
+    if (counts[i] == 0u) {
+        kept[i] = 0u;
+        return;
+    }
+    const uint4 r = rects[i];
+    const uint32_t gx = tiles_x + 1u, gy = tiles_y + 1u;
+    const int32_t* T = P + (size_t)(i / splat_block) * gy * gx;
+    int32_t c = T[r.w * gx + r.y] - T[r.z * gx + r.y] - T[r.w * gx + r.x] + T[r.z * gx + r.x];
+    kept[i] = (uint32_t)c;
+}
+
+// The cut table is staged in LDS as bytes (id blocks < 256) when it fits, so
+// the per-splat lookups cost LDS rather than L2 latency.
+constexpr uint32_t kCutLdsMax = 48 * 1024;
+
+template <bool kEmit>
+__global__ __launch_bounds__(256) void k_keys_wave(uint32_t n, const uint32_t* __restrict__ counts,
+                                                   const uint4* __restrict__ rects, uint32_t tiles_x,
+                                                   uint32_t num_tiles, const uint32_t* __restrict__ cut,
+                                                   uint32_t splat_block, const uint64_t* __restrict__ offsets,
+                                                   uint32_t* __restrict__ kept_counts,
+                                                   uint64_t* __restrict__ keys) {
+    extern __shared__ uint8_t s_cut[];
+    const bool lds_cut = cut && num_tiles <= kCutLdsMax;
+    if (lds_cut) {
+        for (uint32_t t = threadIdx.x; t < num_tiles; t += blockDim.x) s_cut[t] = (uint8_t)cut[t];
+        __syncthreads();
+    }
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wbase = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // each lane loads one splat of the wave; the wave then walks them in order
+    const uint32_t mine = wbase + lane;
+    const uint32_t my_cnt = mine < n ? counts[mine] : 0u;
+    uint4 my_r = make_uint4(0u, 0u, 0u, 0u);
+    uint64_t my_off = 0ull;
+    if (my_cnt) {
+        my_r = rects[mine];
+        if (kEmit) my_off = offsets[mine];
+    }
+    uint32_t my_kept = 0u;
+    for (uint32_t s = 0; s < 64u; ++s) {
+        const uint32_t cnt = __shfl(my_cnt, (int)s, 64);
+        if (cnt == 0u) continue;
+        const uint32_t i = wbase + s;
+        const uint32_t rx0 = __shfl(my_r.x, (int)s, 64), rx1 = __shfl(my_r.y, (int)s, 64);
+        const uint32_t ry0 = __shfl(my_r.z, (int)s, 64), ry1 = __shfl(my_r.w, (int)s, 64);
+        const uint32_t rw = rx1 - rx0, total = rw * (ry1 - ry0);
+        const uint32_t sb = i / splat_block;
+        const uint64_t pos = kEmit ? __shfl(my_off, (int)s, 64) : 0ull;
+        uint32_t kept = 0u;
+        for (uint32_t k0 = 0; k0 < total; k0 += 64u) {
+            const uint32_t k = k0 + lane;
+            bool keep = false;
+            uint32_t tile = 0u;
+            if (k < total) {
+                const uint32_t ty = ry0 + k / rw, tx = rx0 + k % rw;
+                tile = ty * tiles_x + tx;
+                keep = !cut || sb <= (lds_cut ? (uint32_t)s_cut[tile] : cut[tile]);
+            }
+            const uint64_t bal = __ballot(keep);
+            if (kEmit && keep)
+                keys[pos + kept + (uint32_t)__popcll(bal & lt_mask)] = ((uint64_t)tile << 32) | (uint64_t)i;
+            kept += (uint32_t)__popcll(bal);
         }
+        if (lane == s) my_kept = kept;
+    }
+    if (!kEmit && mine < n) kept_counts[mine] = my_kept;
 }
 
 // ranges[2t] = start, ranges[2t+1] = end of tile t in the sorted keys.
@@ -175,11 +360,33 @@ void launch_preprocess(hipStream_t st, const float* splats, uint32_t n, const Vi
                        bwd, counts, rects);
 }
 
-void launch_emit_keys(hipStream_t st, uint32_t n, const uint32_t* counts, const uint64_t* offsets,
-                      const uint4* rects, uint32_t tiles_x, uint64_t* keys) {
+void launch_tile_cut(hipStream_t st, const ViewParams& v, uint32_t n, int32_t* diff, uint32_t splat_block,
+                     uint32_t cap, uint32_t* cut, uint32_t* full, unsigned long long* over_cap,
+                     const uint32_t* counts, const uint4* rects, uint32_t* kept_counts) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_emit_keys, dim3((n + 255) / 256), dim3(256), 0, st, n, counts, offsets, rects, tiles_x,
-                       keys);
+    const uint32_t nsb = (n + splat_block - 1) / splat_block;
+    const uint32_t gx = v.tiles_x + 1u, gy = v.tiles_y + 1u;
+    const uint32_t strips = (gy + kStripRows - 1) / kStripRows;
+    hipLaunchKernelGGL(k_diff_build, dim3(strips, nsb, kDiffSubs), dim3(256), sizeof(int32_t) * kStripRows * gx, st,
+                       rects, counts, n, splat_block, gx, gy, nsb, diff);
+    hipLaunchKernelGGL(k_diff_cols, dim3((nsb * gx + 63) / 64), dim3(64), 0, st, diff, nsb, gx, gy);
+    hipLaunchKernelGGL(k_tile_cut, dim3((v.num_tiles + 255) / 256), dim3(256), 0, st, (const int32_t*)diff, nsb,
+                       gx, gy, v.tiles_x, v.num_tiles, cap, cut, full, over_cap);
+    // the diff grids are consumed; their buffer now holds the prefix tables P
+    hipLaunchKernelGGL(k_cut_rows, dim3((nsb * gy + 63) / 64), dim3(64), 0, st, (const uint32_t*)cut, nsb, v.tiles_x,
+                       v.tiles_y, diff);
+    hipLaunchKernelGGL(k_cut_cols, dim3((nsb * gx + 63) / 64), dim3(64), 0, st, nsb, v.tiles_x, v.tiles_y, diff);
+    hipLaunchKernelGGL(k_count_kept, dim3((n + 255) / 256), dim3(256), 0, st, n, counts, rects, splat_block,
+                       v.tiles_x, v.tiles_y, (const int32_t*)diff, kept_counts);
+}
+
+void launch_emit_keys(hipStream_t st, uint32_t n, const uint32_t* counts, const uint64_t* offsets,
+                      const uint4* rects, uint32_t tiles_x, uint32_t num_tiles, const uint32_t* cut,
+                      uint32_t splat_block, uint64_t* keys) {
+    if (n == 0) return;
+    const size_t lds = (cut && num_tiles <= kCutLdsMax) ? num_tiles : 0;
+    hipLaunchKernelGGL(k_keys_wave<true>, dim3((n + 255) / 256), dim3(256), lds, st, n, counts, rects, tiles_x,
+                       num_tiles, cut, splat_block, offsets, (uint32_t*)nullptr, keys);
 }
 
 void launch_tile_ranges(hipStream_t st, const uint64_t* keys, uint64_t m, uint32_t* ranges) {

@@ -411,7 +411,6 @@ __global__ __launch_bounds__(256) void k_composite_ref(ViewParams v, CompositeAr
         A.tile_loss[2 * tile + 1] = a1;
         uint32_t vw = min(v.W - tx * kTile, (uint32_t)kTile), vh = min(v.H - ty * kTile, (uint32_t)kTile);
         if (count > 0) atomicAdd(&A.stats[0], (unsigned long long)count * (unsigned long long)(vw * vh));
-        if (len > A.cap) atomicAdd(&A.stats[1], 1ull);
     }
 }
 

@@ -30,8 +30,11 @@ struct DevBuf {
 
 struct Scratch {
     DevBuf counts, rects, offsets, keys, keys_alt, ranges, scan_tmp, sort_tmp;
+    DevBuf diff, cut, full, kept;  // per-tile id-block cut of capped (REF) lists
     uint64_t m = 0;
     bool in_alt = false;
+    bool have_full = false;
+    uint32_t splat_block = 0;
 };
 
 }  // namespace
@@ -117,6 +120,13 @@ void free_buf(DevBuf& b) {
 void free_scratch(Scratch& s) {
     free_buf(s.counts); free_buf(s.rects); free_buf(s.offsets); free_buf(s.keys);
     free_buf(s.keys_alt); free_buf(s.ranges); free_buf(s.scan_tmp); free_buf(s.sort_tmp);
+    free_buf(s.diff); free_buf(s.cut); free_buf(s.full); free_buf(s.kept);
+}
+
+// Ids per block of the per-tile cut: about 32 blocks, at least 4096 splats each.
+uint32_t splat_block_of(uint32_t n) {
+    uint32_t b = (n + 31u) / 32u;
+    return b < 4096u ? 4096u : b;
 }
 
 ViewParams make_view(const float* vp, const float* vp_inv, uint32_t W, uint32_t H) {
@@ -144,7 +154,9 @@ int build_tile_lists(nps_ctx* h, Scratch& sc, const ViewParams& v, uint32_t n, i
     int rc;
     if ((rc = ensure(h, sc.offsets, sizeof(uint64_t) * ((size_t)n + 1))) != NPS_OK) return rc;
     if ((rc = ensure(h, sc.scan_tmp, scan_temp_bytes(n) + 64)) != NPS_OK) return rc;
-    exclusive_scan_u32_to_u64(h->stream, (const uint32_t*)sc.counts.p, (uint64_t*)sc.offsets.p, n, sc.scan_tmp.p);
+    // capped lists: key counts under the per-tile cut; uncapped: the rectangle sizes
+    const uint32_t* key_counts = (const uint32_t*)(sc.have_full ? sc.kept.p : sc.counts.p);
+    exclusive_scan_u32_to_u64(h->stream, key_counts, (uint64_t*)sc.offsets.p, n, sc.scan_tmp.p);
     uint64_t m = 0;
     NPS_HIP(h, hipMemcpyAsync(&m, (uint64_t*)sc.offsets.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost, h->stream));
     NPS_HIP(h, hipStreamSynchronize(h->stream));
@@ -154,8 +166,11 @@ int build_tile_lists(nps_ctx* h, Scratch& sc, const ViewParams& v, uint32_t n, i
     if ((rc = ensure(h, sc.keys_alt, sizeof(uint64_t) * std::max<uint64_t>(m, 1))) != NPS_OK) return rc;
     if ((rc = ensure(h, sc.sort_tmp, radix_temp_bytes(m))) != NPS_OK) return rc;
     if ((rc = ensure(h, sc.ranges, sizeof(uint32_t) * 2 * (size_t)v.num_tiles)) != NPS_OK) return rc;
-    launch_emit_keys(h->stream, n, (const uint32_t*)sc.counts.p, (const uint64_t*)sc.offsets.p,
-                     (const uint4*)sc.rects.p, v.tiles_x, (uint64_t*)sc.keys.p);
+    // splats that keep no key skip emission entirely
+    launch_emit_keys(h->stream, n, key_counts, (const uint64_t*)sc.offsets.p,
+                     (const uint4*)sc.rects.p, v.tiles_x, v.num_tiles,
+                     sc.have_full ? (const uint32_t*)sc.cut.p : nullptr,
+                     sc.splat_block ? sc.splat_block : 1u, (uint64_t*)sc.keys.p);
     if (h->timing && ev_sort >= 0) NPS_HIP(h, hipEventRecord(h->ev[ev_sort], h->stream));
     int tile_bits = bits_for(v.num_tiles > 0 ? v.num_tiles - 1 : 0);
     sc.in_alt = radix_sort_u64(h->stream, (uint64_t*)sc.keys.p, (uint64_t*)sc.keys_alt.p, nullptr, nullptr, m, 32,
@@ -370,13 +385,24 @@ int nps_backward(nps_handle h) {
     if ((rc = ensure(h, sc.counts, 4 * (size_t)std::max<uint32_t>(n, 1))) != NPS_OK) return rc;
     if ((rc = ensure(h, sc.rects, 16 * (size_t)std::max<uint32_t>(n, 1))) != NPS_OK) return rc;
     if ((rc = ensure(h, h->stats, 64)) != NPS_OK) return rc;
+    const uint32_t cap = (uint32_t)h->cfg.max_tile_splats;
+    if (cap > (uint32_t)kMaxCap) return set_err(h, NPS_ERR_ARG, "max_tile_splats above 1024 is not supported");
+    sc.splat_block = splat_block_of(n);
+    const uint32_t nsb = (std::max<uint32_t>(n, 1) + sc.splat_block - 1) / sc.splat_block;
+    const size_t diff_n = 4 * (size_t)nsb * (v.tiles_x + 1) * (v.tiles_y + 1) + 16 * (v.tiles_x + 1);
+    if ((rc = ensure(h, sc.diff, 4 * diff_n)) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.cut, 4 * (size_t)std::max<uint32_t>(v.num_tiles, 1))) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.full, 4 * (size_t)std::max<uint32_t>(v.num_tiles, 1))) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.kept, 4 * (size_t)std::max<uint32_t>(n, 1))) != NPS_OK) return rc;
     if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[0], h->stream));
     launch_fill_u32(h->stream, (uint32_t*)h->stats.p, 0u, 16);
     launch_preprocess(h->stream, h->splats, n, v, (FwdRec*)h->fwd.p, (BwdRec*)h->bwd.p, (uint32_t*)sc.counts.p,
                       (uint4*)sc.rects.p);
+    launch_tile_cut(h->stream, v, n, (int32_t*)sc.diff.p, sc.splat_block, cap, (uint32_t*)sc.cut.p,
+                    (uint32_t*)sc.full.p, (unsigned long long*)h->stats.p + 1, (const uint32_t*)sc.counts.p,
+                    (const uint4*)sc.rects.p, (uint32_t*)sc.kept.p);
+    sc.have_full = true;
     if ((rc = build_tile_lists(h, sc, v, n, 1, 2)) != NPS_OK) return rc;
-    const uint32_t cap = (uint32_t)h->cfg.max_tile_splats;
-    if (cap > (uint32_t)kMaxCap) return set_err(h, NPS_ERR_ARG, "max_tile_splats above 1024 is not supported");
     const uint32_t nchunk = (cap + kChunk - 1) / kChunk;
     size_t px = (size_t)h->W * h->H;
     if ((rc = ensure(h, h->ckpt, sizeof(float) * (size_t)v.num_tiles * nchunk * kTilePx)) != NPS_OK) return rc;
@@ -511,12 +537,14 @@ int nps_read_loss(nps_handle h, double loss[2]) {
     return NPS_OK;
 }
 
-int nps_read_tiles(nps_handle h, uint32_t* ranges, uint32_t* ids, int32_t* num_tiles, int64_t* num_ids) {
+int nps_read_tiles(nps_handle h, uint32_t* ranges, uint32_t* ids, uint32_t* full_counts, int32_t* num_tiles,
+                   int64_t* num_ids) {
     if (!h) return NPS_ERR_ARG;
     if (!h->have_backward) return set_err(h, NPS_ERR_STATE, "no backward has run");
     if (num_tiles) *num_tiles = (int32_t)h->last_tiles;
     if (num_ids) *num_ids = (int64_t)h->sc.m;
     if (ranges) NPS_COPY(h, ranges, h->sc.ranges.p, 8 * (size_t)h->last_tiles, hipMemcpyDeviceToHost);
+    if (full_counts) NPS_COPY(h, full_counts, h->sc.full.p, 4 * (size_t)h->last_tiles, hipMemcpyDeviceToHost);
     if (ids && h->sc.m) {
         std::vector<uint64_t> k(h->sc.m);
         NPS_COPY(h, k.data(), h->sc.in_alt ? h->sc.keys_alt.p : h->sc.keys.p, 8 * (size_t)h->sc.m,
@@ -604,6 +632,8 @@ int npf_dispatch(npf_handle f, int32_t clear, int32_t quantize) {
     if ((rc = ensure(h, f->rec, 64 * (size_t)std::max<uint32_t>(n, 1))) != NPS_OK) return rc;
     if ((rc = ensure(h, f->sc.counts, 4 * (size_t)std::max<uint32_t>(n, 1))) != NPS_OK) return rc;
     if ((rc = ensure(h, f->sc.rects, 16 * (size_t)std::max<uint32_t>(n, 1))) != NPS_OK) return rc;
+    f->sc.have_full = false;
+    f->sc.splat_block = 0;
     launch_preprocess_render(h->stream, h->splats, n, v, (float4*)f->rec.p, (uint32_t*)f->sc.counts.p,
                              (uint4*)f->sc.rects.p);
     if ((rc = build_tile_lists(h, f->sc, v, n, -1, -1)) != NPS_OK) return rc;

@@ -29,8 +29,14 @@ struct CompositeArgs {
 // k_bin.hip
 void launch_preprocess(hipStream_t st, const float* splats, uint32_t n, const ViewParams& v, FwdRec* fwd,
                        BwdRec* bwd, uint32_t* counts, uint4* rects);
+// Per-tile id-block cut of capped lists.  diff: 4 * nsb * (tiles_x+1) * (tiles_y+1) + 16 * (tiles_x+1) i32
+// scratch (overwritten); kept_counts[n] = keys each splat emits under the cut.
+void launch_tile_cut(hipStream_t st, const ViewParams& v, uint32_t n, int32_t* diff, uint32_t splat_block,
+                     uint32_t cap, uint32_t* cut, uint32_t* full, unsigned long long* over_cap,
+                     const uint32_t* counts, const uint4* rects, uint32_t* kept_counts);
 void launch_emit_keys(hipStream_t st, uint32_t n, const uint32_t* counts, const uint64_t* offsets,
-                      const uint4* rects, uint32_t tiles_x, uint64_t* keys);
+                      const uint4* rects, uint32_t tiles_x, uint32_t num_tiles, const uint32_t* cut,
+                      uint32_t splat_block, uint64_t* keys);
 void launch_tile_ranges(hipStream_t st, const uint64_t* keys, uint64_t m, uint32_t* ranges);
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint64_t n);
 

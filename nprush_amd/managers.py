@@ -192,13 +192,16 @@ class GpuSplatOptimizerManager:
         return float(out[0]), float(out[1])
 
     def read_tiles(self):
+        """(ranges [T,2], ids [M], full_counts [T]) of the last backward."""
         nt = C.c_int32()
         nid = C.c_int64()
-        check(self._lib.nps_read_tiles(self._h, None, None, C.byref(nt), C.byref(nid)), self._h, "read_tiles")
+        check(self._lib.nps_read_tiles(self._h, None, None, None, C.byref(nt), C.byref(nid)), self._h, "read_tiles")
         ranges = np.empty((nt.value, 2), dtype=np.uint32)
         ids = np.empty(max(nid.value, 1), dtype=np.uint32)
-        check(self._lib.nps_read_tiles(self._h, uptr(ranges), uptr(ids), None, None), self._h, "read_tiles")
-        return ranges, ids[: nid.value]
+        full = np.empty(nt.value, dtype=np.uint32)
+        check(self._lib.nps_read_tiles(self._h, uptr(ranges), uptr(ids), uptr(full), None, None), self._h,
+              "read_tiles")
+        return ranges, ids[: nid.value], full
 
     def read_stats(self) -> dict:
         s = np.zeros(12, dtype=np.int64)

@@ -78,10 +78,11 @@ def test_config0_tiles_and_backward(oracle):
     vp, vi = scenes.config0_camera()
     tgt, dep = _target(H, W)
     mgr = _run_backward(splats, vp, vi, tgt, dep, W, H)
-    ranges, ids = mgr.read_tiles()
+    ranges, ids, full = mgr.read_tiles()
     o_ids, o_cnt, o_full = oracle.bin_tiles(splats, vp, W, H)
     lens = (ranges[:, 1] - ranges[:, 0]).astype(np.int64)
-    assert np.array_equal(lens, o_full), "uncapped tile lengths differ"
+    assert np.array_equal(full.astype(np.int64), o_full), "uncapped tile counts differ"
+    assert (lens >= np.minimum(o_full, 1024)).all() and (lens <= o_full).all()
     for t in range(ranges.shape[0]):
         got = ids[ranges[t, 0]: ranges[t, 0] + min(lens[t], 1024)]
         assert np.array_equal(got, o_ids[t, : o_cnt[t]]), f"tile {t} list differs"
