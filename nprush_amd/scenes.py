@@ -74,3 +74,30 @@ def config0_camera():
 CONFIG0 = dict(n=16384, width=256, height=256, seed=0, target_seed=1, steps=100)
 CONFIG1 = dict(n=1_000_000, capacity=2_000_000, width=1920, height=1080, views=8, radius=3.0, seed=0,
                target_seed=1)
+
+
+SH_C0 = 0.28209479177387814
+
+
+def make_ext_splats(n: int, seed: int, sh_degree: int = 3, higher_std: float = 0.05):
+    """EXT-layout splats (config-1 distribution): [x y z opacity_logit]
+    [log_sx log_sy log_sz 0] [qw qx qy qz] [0 0 0 0] plus SH coefficients
+    (n, 48): DC from RGB uniform in [0, 1] (c0 = (rgb - 0.5) / C0), higher
+    bands N(0, higher_std) up to sh_degree, zero beyond."""
+    rng = np.random.default_rng(seed)
+    s = np.zeros((n, SPLAT_FLOATS), dtype=np.float32)
+    s[:, 0:3] = rng.uniform(-1.0, 1.0, size=(n, 3))
+    s[:, 3] = rng.normal(0.0, 1.0, size=n)
+    s[:, 4:7] = rng.normal(-4.0, 0.5, size=(n, 3))
+    u = rng.random((n, 3))
+    s[:, 8] = np.sqrt(1.0 - u[:, 0]) * np.sin(2.0 * math.pi * u[:, 1])
+    s[:, 9] = np.sqrt(1.0 - u[:, 0]) * np.cos(2.0 * math.pi * u[:, 1])
+    s[:, 10] = np.sqrt(u[:, 0]) * np.sin(2.0 * math.pi * u[:, 2])
+    s[:, 11] = np.sqrt(u[:, 0]) * np.cos(2.0 * math.pi * u[:, 2])
+    sh = np.zeros((n, 48), dtype=np.float32)
+    rgb = rng.uniform(0.0, 1.0, size=(n, 3))
+    sh[:, 0:3] = (rgb - 0.5) / SH_C0
+    nk = (sh_degree + 1) ** 2
+    if nk > 1:
+        sh[:, 3:3 * nk] = rng.normal(0.0, higher_std, size=(n, 3 * nk - 3))
+    return s, sh
