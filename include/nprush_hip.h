@@ -44,8 +44,28 @@ typedef struct nps_config {
     int32_t adc_period;      /* SPLAT_ADC_PERIOD = 25 (constants.ts:30; OPT:381) */
     int32_t adc_enabled;     /* 1: nps_dispatch runs ADC every adc_period steps */
     int32_t device;          /* HIP device ordinal */
-    int32_t reserved[12];
+    int32_t mode;            /* NPS_MODE_REF (0) or NPS_MODE_EXT (1) */
+    int32_t sh_degree;       /* EXT: spherical-harmonic degree 0..3 */
+    int32_t reserved[10];
 } nps_config;
+
+/*
+ * Modes.  REF: the reference's splat model and loop (splat_backward.wgsl,
+ * splat_step.wgsl, splat_adc.wgsl): 16-float splats with RGB colour and shape
+ * parameters, ascending-id tile lists capped at max_tile_splats, L2 colour +
+ * depth loss, i32 fixed-point gradients.
+ * EXT: the same loop over 3D Gaussian splats with view-dependent colour:
+ *   splat (16 fp32) [x y z opacity_logit] [log_sx log_sy log_sz 0] [qw qx qy qz] [0 0 0 0]
+ *   sh (48 fp32)    16 RGB coefficients of the real SH basis (degree sh_degree)
+ * exp scales, sigmoid opacity, normalised quaternion, 0.3 px screen filter,
+ * colour max(SH(dir) + 0.5, 0), alpha = min(0.99, opacity exp(-r^2/2)) with
+ * pairs below 1/255 skipped, tile lists ordered by view depth (then id) and not
+ * capped, a pixel stops before the splat that would bring its transmittance
+ * below 1e-4, background 0.05, L1 loss, FP32 gradients of all 64 parameters.
+ * EXT has no reference function (an extension of the same loop).
+ */
+#define NPS_MODE_REF 0
+#define NPS_MODE_EXT 1
 
 typedef struct nps_ctx* nps_handle;
 
@@ -64,15 +84,27 @@ int32_t nps_num_splats(nps_handle h);
 /* queue.writeBuffer(splatBuffer) / readback (OPT:88-93). n must equal num_splats. */
 int nps_write_splats(nps_handle h, const float* splats16n, int32_t n);
 int nps_read_splats(nps_handle h, float* splats16n, int32_t n);
-/* gradBuffer (OPT:95-99): i32 fixed point, 16 per splat. */
+/* gradBuffer (OPT:95-99): i32 fixed point, 16 per splat (REF mode). */
 int nps_write_grads(nps_handle h, const int32_t* grads16n, int32_t n);
 int nps_read_grads(nps_handle h, int32_t* grads16n, int32_t n);
-/* adamBuffer (OPT:101-106): m, v (16N each) and the step counter t. */
-int nps_write_adam(nps_handle h, const float* m16n, const float* v16n, float t, int32_t n);
-int nps_read_adam(nps_handle h, float* m16n, float* v16n, float* t, int32_t n);
-/* adcBuffer (OPT:108-112): grad_accum, N floats. */
+/* adamBuffer (OPT:101-106): m, v and the step counter t.  16 floats per
+ * splat each in REF mode, 64 (splat record then SH) in EXT mode. */
+int nps_write_adam(nps_handle h, const float* m, const float* v, float t, int32_t n);
+int nps_read_adam(nps_handle h, float* m, float* v, float* t, int32_t n);
+/* adcBuffer (OPT:108-112): REF: grad_accum, N floats.  EXT: 2 floats per
+ * splat, the sum over views of |dL/d ndc| of the splat centre and the number
+ * of views in which it was on screen. */
 int nps_write_accum(nps_handle h, const float* accum, int32_t n);
 int nps_read_accum(nps_handle h, float* accum, int32_t n);
+
+/* EXT mode only (no reference counterpart): SH coefficients (48 floats per
+ * splat) and the FP32 gradients of the 64 parameters per splat (16 of the
+ * splat record, then 48 SH).  Gradients add up over nps_backward calls and
+ * nps_step clears them. */
+int nps_write_sh(nps_handle h, const float* sh48n, int32_t n);
+int nps_read_sh(nps_handle h, float* sh48n, int32_t n);
+int nps_write_grads_f32(nps_handle h, const float* grads64n, int32_t n);
+int nps_read_grads_f32(nps_handle h, float* grads64n, int32_t n);
 
 /* writeSplatVPMatrix(mat, invMat, blurEnabled) (OPT:281-301): column-major
  * view-projection and its inverse. */
@@ -88,6 +120,11 @@ int nps_set_target_dev(nps_handle h, const float* rgba_dev, const float* depth_d
 /* dispatchWorkgroups(ceil(W/16), ceil(H/16)) of splat_backward.wgsl (OPT:372-374):
  * binning, forward composite, L2 loss seeds, reverse pass, fixed-point grads. */
 int nps_backward(nps_handle h);
+/* EXT mode only: binning and the forward composite of the current camera into
+ * an image the size of the last target (its contents unused): nps_read_image
+ * and nps_read_loss then give the image and its loss against that target.  No
+ * gradients.  REF mode renders with the npf_* entry points instead. */
+int nps_forward(nps_handle h);
 /* splat_step.wgsl (OPT:376-378): fused Adam; increments t. */
 int nps_step(nps_handle h);
 /* splat_adc.wgsl (OPT:380-385): clone/split into dead slots. */
@@ -100,13 +137,15 @@ int nps_sync(nps_handle h);
 
 /* Outputs of the last nps_backward.  The reference never materialises them
  * (splat_backward.wgsl:230-248); exposed for parity checks and metrics.
- * c_pred: H*W*3, d_pred: H*W (either may be NULL). */
+ * c_pred: H*W*3, d_pred: H*W (either may be NULL; EXT mode has no depth
+ * output and fills d_pred with zeros). */
 int nps_read_image(nps_handle h, float* c_pred, float* d_pred);
 /* loss[0] = sum over pixels of |C-tgt|^2 + fg*(D-tgt_depth)^2 (the loss whose
  * gradient the reference seeds, BW:250-257); loss[1] = sum |C-tgt| (L1). */
 int nps_read_loss(nps_handle h, double loss[2]);
 /* Tile lists of the last nps_backward: ranges[2*T] = [start, end) into the
- * sorted list ids[M] (splat ids, ascending within a tile).  Each tile's range
+ * sorted list ids[M] (splat ids, ascending within a tile in REF mode; by view
+ * depth, then id, in EXT mode, where full_counts are the range lengths).  Each tile's range
  * holds at least its first min(full, max_tile_splats) ids -- the ones the
  * composite uses (splat_backward.wgsl:199-210); ids of later id blocks that
  * cannot be among them are not emitted.  full_counts[T] = number of splats whose
@@ -122,7 +161,8 @@ int nps_read_tiles(nps_handle h, uint32_t* ranges, uint32_t* ids, uint32_t* full
 int nps_read_stats(nps_handle h, int64_t stats[12]);
 /* Per-phase GPU time (ms, HIP events on the handle's stream) of the last
  * backward/step/adc when timing is enabled: [0]=preprocess+keys, [1]=sort,
- * [2]=ranges, [3]=composite fwd+bwd, [4]=step, [5]=adc, [6]=whole backward. */
+ * [2]=ranges, [3]=composite fwd+bwd, [4]=step, [5]=adc, [6]=whole backward,
+ * [7]=EXT per-splat chain rule. */
 int nps_enable_timing(nps_handle h, int32_t on);
 int nps_read_timing(nps_handle h, float ms[8]);
 

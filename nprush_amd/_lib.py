@@ -10,6 +10,9 @@ import os
 
 import numpy as np
 
+MODE_REF = 0
+MODE_EXT = 1
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libnprush_hip.so")
 
@@ -20,7 +23,9 @@ class NpsConfig(C.Structure):
         ("adc_period", C.c_int32),
         ("adc_enabled", C.c_int32),
         ("device", C.c_int32),
-        ("reserved", C.c_int32 * 12),
+        ("mode", C.c_int32),
+        ("sh_degree", C.c_int32),
+        ("reserved", C.c_int32 * 10),
     ]
 
 
@@ -47,10 +52,15 @@ SIGNATURES = {
     "nps_read_adam": (i32, [_P, _F, _F, _F, i32]),
     "nps_write_accum": (i32, [_P, _F, i32]),
     "nps_read_accum": (i32, [_P, _F, i32]),
+    "nps_write_sh": (i32, [_P, _F, i32]),
+    "nps_read_sh": (i32, [_P, _F, i32]),
+    "nps_write_grads_f32": (i32, [_P, _F, i32]),
+    "nps_read_grads_f32": (i32, [_P, _F, i32]),
     "nps_set_camera": (i32, [_P, _F, _F, i32]),
     "nps_set_target": (i32, [_P, _F, _F, i32, i32]),
     "nps_set_target_dev": (i32, [_P, _P, _P, i32, i32]),
     "nps_backward": (i32, [_P]),
+    "nps_forward": (i32, [_P]),
     "nps_step": (i32, [_P]),
     "nps_adc": (i32, [_P]),
     "nps_dispatch": (i32, [_P]),

@@ -22,19 +22,6 @@
 
 namespace nps {
 
-// First index in [0, n) whose predicate is false, for a predicate that is true
-// on a prefix and false after it (n if always true).
-template <class P>
-__device__ __forceinline__ uint32_t first_false(uint32_t n, P pred) {
-    uint32_t lo = 0, hi = n;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (pred(mid)) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
 __global__ __launch_bounds__(256) void k_preprocess(const float4* __restrict__ splats, uint32_t n, ViewParams v,
                                                     FwdRec* __restrict__ fwd, BwdRec* __restrict__ bwd,
                                                     uint32_t* __restrict__ counts, uint4* __restrict__ rects) {

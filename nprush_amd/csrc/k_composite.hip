@@ -28,14 +28,6 @@
 
 namespace nps {
 
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t i, uint32_t n) {
-    // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
-    // contiguous run of tiles so neighbouring tiles share its L2.
-    uint32_t per = n / 8u, rem = n % 8u;
-    uint32_t xcd = i % 8u, local = i / 8u;
-    return xcd < rem ? xcd * (per + 1u) + local : rem * (per + 1u) + (xcd - rem) * per + local;
-}
-
 // Wave-wide sum of 16 values; lane l ends with the total of value
 // ((l>>5)&1)*8 + ((l>>4)&1)*4 + ((l>>3)&1)*2 + ((l>>2)&1).
 // Transpose reduction: each step pairs lanes that differ in one "keep" bit, each

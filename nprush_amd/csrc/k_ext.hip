@@ -1,0 +1,657 @@
+// k_ext.hip -- EXT path: 3D Gaussian splats with SH colour, depth-ordered
+// uncapped tile lists, early termination, L1 loss and FP32 gradients.
+//
+//   k_ext_preprocess : per splat, the projection of nps_ext.h, the 3-sigma tile
+//                      rectangle (binary search with the float overlap predicate
+//                      of the REF binning, splat_backward.wgsl:148-153), the
+//                      composite record and the depth key (float bits of w > 0);
+//   k_ext_keys       : (tile << 32 | depth bits, splat id) pairs, written in
+//                      splat order, so a stable LSD sort over bits [0, 32 +
+//                      tile bits) orders every tile by (depth, id);
+//   k_composite_ext  : per 16x16 tile, front-to-back compositing with early
+//                      termination, image + L1/L2 loss, then the reverse pass;
+//   k_ext_chain      : per splat, the 2D gradients (screen centre, conic, colour,
+//                      opacity) through the projection to the 64 parameters, in
+//                      FP64, added to the FP32 gradient buffer;
+//   k_adam_ext       : Adam with the per-group learning rates.
+#include "nps_common.h"
+#include "nps_ext.h"
+#include "nps_kernels.h"
+
+namespace nps {
+
+// ---------------------------------------------------------------- preprocess
+
+template <int DEG>
+__global__ __launch_bounds__(256) void k_ext_preprocess(const float4* __restrict__ splats,
+                                                        const float4* __restrict__ sh, uint32_t n, ViewParams v,
+                                                        float cam0, float cam1, float cam2,
+                                                        ExtRec* __restrict__ rec, uint32_t* __restrict__ counts,
+                                                        uint4* __restrict__ rects, uint32_t* __restrict__ dkey) {
+    constexpr int kNk = (DEG + 1) * (DEG + 1);
+    constexpr int kSh4 = (3 * kNk + 3) / 4;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s[16];
+    float c[4 * kSh4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float4 t = splats[4 * (size_t)i + q];
+        s[4 * q + 0] = t.x; s[4 * q + 1] = t.y; s[4 * q + 2] = t.z; s[4 * q + 3] = t.w;
+    }
+#pragma unroll
+    for (int q = 0; q < kSh4; ++q) {
+        float4 t = sh[12 * (size_t)i + q];
+        c[4 * q + 0] = t.x; c[4 * q + 1] = t.y; c[4 * q + 2] = t.z; c[4 * q + 3] = t.w;
+    }
+    counts[i] = 0u;
+    const float cam[3] = {cam0, cam1, cam2};
+    ExtProj o;
+    ext_project(s, c, DEG, v.vp, cam, v.W, v.H, o);
+    if (!o.valid) return;
+    const float sminx = o.scr[0] - o.ext[0], smaxx = o.scr[0] + o.ext[0];
+    const float sminy = o.scr[1] - o.ext[1], smaxy = o.scr[1] + o.ext[1];
+    if (!(sminx == sminx && smaxx == smaxx && sminy == sminy && smaxy == smaxy)) return;
+    const uint32_t W = v.W, H = v.H, T = (uint32_t)kTile;
+    const float aspect = v.aspect;
+    uint32_t tx0 = first_false(v.tiles_x, [&](uint32_t tx) {
+        uint32_t x1 = min(tx * T + T, W);
+        return sminx > px_to_x(x1, W, aspect);
+    });
+    uint32_t tx_end = first_false(v.tiles_x, [&](uint32_t tx) { return !(smaxx < px_to_x(tx * T, W, aspect)); });
+    uint32_t ty0 = first_false(v.tiles_y, [&](uint32_t ty) {
+        uint32_t y1 = min(ty * T + T, H);
+        return smaxy < py_to_y(y1, H);
+    });
+    uint32_t ty_end = first_false(v.tiles_y, [&](uint32_t ty) { return !(sminy > py_to_y(ty * T, H)); });
+    if (tx0 >= tx_end || ty0 >= ty_end) return;
+    counts[i] = (tx_end - tx0) * (ty_end - ty0);
+    rects[i] = make_uint4(tx0, tx_end, ty0, ty_end);
+    dkey[i] = __float_as_uint(o.w);
+    ExtRec f;
+    f.scr_x = o.scr[0];
+    f.scr_y = o.scr[1];
+    f.A = o.A;
+    f.B = o.B;
+    f.C = o.C;
+    f.op = o.op;
+    f.r = o.rgb[0];
+    f.g = o.rgb[1];
+    f.b = o.rgb[2];
+    // alpha = min(op exp(-r^2/2), 0.99) < 1/255 for every r^2 > rcut2:
+    // rcut2 >= 2 ln(255 op) + 1e-3 leaves a relative margin of e^(-5e-4) in
+    // alpha, far above the rounding of exp and of the product (< 1e-6).
+    if (o.op < kExtMinAlpha * 0.999f) {
+        f.rcut2 = -INFINITY;
+    } else {
+        float l = 2.0f * log_det(o.op * 255.0f);
+        f.rcut2 = fmax_w(l, 0.0f) * 1.0001f + 1e-3f;
+    }
+    f.pad0 = f.pad1 = 0.0f;
+    rec[i] = f;
+}
+
+// One key per (splat, covered tile); the wave walks its 64 splats in order and
+// writes each splat's keys with all lanes.
+__global__ __launch_bounds__(256) void k_ext_keys(uint32_t n, const uint32_t* __restrict__ counts,
+                                                  const uint4* __restrict__ rects, const uint32_t* __restrict__ dkey,
+                                                  uint32_t tiles_x, const uint64_t* __restrict__ offsets,
+                                                  uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wbase = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
+    const uint32_t mine = wbase + lane;
+    const uint32_t my_cnt = mine < n ? counts[mine] : 0u;
+    uint4 my_r = make_uint4(0u, 0u, 0u, 0u);
+    uint64_t my_off = 0ull;
+    uint32_t my_d = 0u;
+    if (my_cnt) {
+        my_r = rects[mine];
+        my_off = offsets[mine];
+        my_d = dkey[mine];
+    }
+    for (uint32_t s = 0; s < 64u; ++s) {
+        const uint32_t cnt = __shfl(my_cnt, (int)s, 64);
+        if (cnt == 0u) continue;
+        const uint32_t i = wbase + s;
+        const uint32_t rx0 = __shfl(my_r.x, (int)s, 64), rx1 = __shfl(my_r.y, (int)s, 64);
+        const uint32_t ry0 = __shfl(my_r.z, (int)s, 64);
+        const uint32_t rw = rx1 - rx0;
+        const uint64_t pos = __shfl(my_off, (int)s, 64);
+        const uint64_t d = (uint64_t)__shfl(my_d, (int)s, 64);
+        for (uint32_t k = lane; k < cnt; k += 64u) {
+            const uint32_t ty = ry0 + k / rw, tx = rx0 + k % rw;
+            keys[pos + k] = ((uint64_t)(ty * tiles_x + tx) << 32) | d;
+            vals[pos + k] = i;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- composite
+
+__device__ __forceinline__ float dpp_f(float x, int ctrl_sel) {
+    int r = 0;
+    switch (ctrl_sel) {
+        case 0: r = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false); break;
+        case 1: r = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x4E, 0xF, 0xF, false); break;
+        case 2: r = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x141, 0xF, 0xF, false); break;
+        default: r = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x140, 0xF, 0xF, false); break;
+    }
+    return __int_as_float(r);
+}
+
+__device__ __forceinline__ float swap32_sum(float a, float b) {
+    auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float swap16_sum(float a, float b) {
+    auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+// Wave-wide sums of 9 floats.  v[0..7] by a transpose reduction: v_permlane32_swap
+// (keep lane bit 5), v_permlane16_swap (bit 4), DPP row_mirror (bit 3), then
+// sums over lane bits 0-2 (row_half_mirror, quad_perm); lanes with (lane & 7) == 0
+// return the total of value ((lane>>5)&1)*4 + ((lane>>4)&1)*2 + ((lane>>3)&1).
+// v[8] is summed over all lanes (returned in t8, the same in every lane).
+__device__ __forceinline__ float wave_reduce9(float (&v)[9], int lane, float& t8) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = swap32_sum(v[k], v[k + 4]);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) v[k] = swap16_sum(v[k], v[k + 2]);
+    {
+        const bool hi = lane & 8;
+        float send = hi ? v[0] : v[1];
+        float keep = hi ? v[1] : v[0];
+        v[0] = keep + dpp_f(send, 3);
+    }
+    float x = v[0];
+    x = x + dpp_f(x, 2);
+    x = x + dpp_f(x, 0);
+    x = x + dpp_f(x, 1);
+    float y = v[8];
+    y = y + dpp_f(y, 0);
+    y = y + dpp_f(y, 1);
+    y = y + dpp_f(y, 2);
+    y = y + dpp_f(y, 3);
+    y = swap16_sum(y, y);
+    y = swap32_sum(y, y);
+    t8 = y;
+    return x;
+}
+
+__device__ __forceinline__ void stage_ext(const ExtRec* __restrict__ rec, const uint32_t* __restrict__ ids,
+                                          uint32_t base, int nb, ExtRec* s_rec, uint32_t* s_ids) {
+    const int t = threadIdx.x;
+    if (t < 3 * kBatch) {
+        int j = t / 3, part = t - 3 * j;
+        if (j < nb) {
+            uint32_t id = ids[base + j];
+            reinterpret_cast<float4*>(&s_rec[j])[part] = reinterpret_cast<const float4*>(&rec[id])[part];
+            if (part == 0) s_ids[j] = id;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_composite_ext(ViewParams v, ExtCompositeArgs A) {
+    __shared__ ExtRec s_rec[kBatch];
+    __shared__ uint32_t s_ids[kBatch];
+    __shared__ float s_part[4][kBatch][9];
+    __shared__ double s_red[2][4];
+    __shared__ int s_max[4];
+
+    const uint32_t tile = xcd_remap(blockIdx.x, v.num_tiles);
+    const uint32_t tx = tile % v.tiles_x, ty = tile / v.tiles_x;
+    const uint32_t start = A.ranges[2 * tile], end = A.ranges[2 * tile + 1];
+    const int len = (int)(end - start);
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const uint32_t px = tx * kTile + (wid & 1) * 8 + (lane & 7);
+    const uint32_t py = ty * kTile + (wid >> 1) * 8 + (lane >> 3);
+    const bool valid = px < v.W && py < v.H;
+    const float pxf = px_to_x(px, v.W, v.aspect);
+    const float pyf = py_to_y(py, v.H);
+    // this tile's batch slots: [start/64 + tile, ...) never overlap another tile's
+    const size_t slot0 = (size_t)(start / kBatch) + tile;
+
+    // ---------------- forward ----------------
+    float T = 1.0f, Cr = 0.0f, Cg = 0.0f, Cb = 0.0f;
+    int last = 0;
+    bool alive = valid;
+    uint32_t n_eval = 0u, n_fw = 0u;
+    for (int b0 = 0, bi = 0; b0 < len; b0 += kBatch, ++bi) {
+        if (!__syncthreads_or(alive ? 1 : 0)) break;
+        const int nb = min(kBatch, len - b0);
+        stage_ext(A.rec, A.ids, start + (uint32_t)b0, nb, s_rec, s_ids);
+        __syncthreads();
+        uint64_t mask = 0ull;
+        float* ck = A.ckpt + (slot0 + bi) * (size_t)(8 * kTilePx);
+        if (__ballot(alive) != 0ull) {
+            for (int j = 0; j < nb; ++j) {
+                if (A.backward && (j & 7) == 0) ck[(j >> 3) * kTilePx + tid] = T;
+                bool inc = false;
+                if (alive) {
+                    const ExtRec& f = s_rec[j];
+                    ExtPair q = ext_pair(f, pxf, pyf);
+                    ++n_eval;
+                    if (!(q.r2 > f.rcut2)) {
+                        float G = exp_det(-0.5f * q.r2);
+                        float a = f.op * G;
+                        if (a > kExtMaxAlpha) a = kExtMaxAlpha;
+                        if (!(a < kExtMinAlpha)) {
+                            float tT = T * (1.0f - a);
+                            if (tT < kExtTMin) {
+                                alive = false;
+                            } else {
+                                Cr = Cr + (T * a) * f.r;
+                                Cg = Cg + (T * a) * f.g;
+                                Cb = Cb + (T * a) * f.b;
+                                T = tT;
+                                last = b0 + j + 1;
+                                inc = true;
+                            }
+                        }
+                    }
+                }
+                ++n_fw;
+                if (__ballot(inc) != 0ull) mask |= 1ull << j;
+                if (__ballot(alive) == 0ull) break;
+            }
+        }
+        if (A.backward && lane == 0) A.act[(slot0 + bi) * 4 + wid] = mask;
+    }
+    Cr = Cr + T * kExtBg;
+    Cg = Cg + T * kExtBg;
+    Cb = Cb + T * kExtBg;
+
+    // ---------------- image, loss, seeds ----------------
+    float4 tg = make_float4(0.f, 0.f, 0.f, 0.f);
+    double l1 = 0.0, l2 = 0.0;
+    if (valid) {
+        size_t pix = (size_t)py * v.W + px;
+        tg = A.tgt_rgba[pix];
+        A.c_img[3 * pix + 0] = Cr;
+        A.c_img[3 * pix + 1] = Cg;
+        A.c_img[3 * pix + 2] = Cb;
+        double er = (double)Cr - (double)tg.x, eg = (double)Cg - (double)tg.y, eb = (double)Cb - (double)tg.z;
+        l1 = (fabs(er) + fabs(eg)) + fabs(eb);
+        l2 = (er * er + eg * eg) + eb * eb;
+    }
+    const float dCr = Cr > tg.x ? 1.0f : (Cr < tg.x ? -1.0f : 0.0f);
+    const float dCg = Cg > tg.y ? 1.0f : (Cg < tg.y ? -1.0f : 0.0f);
+    const float dCb = Cb > tg.z ? 1.0f : (Cb < tg.z ? -1.0f : 0.0f);
+    int ml = valid ? last : 0;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        l2 += __shfl_xor(l2, o, 64);
+        l1 += __shfl_xor(l1, o, 64);
+        ml = max(ml, __shfl_xor(ml, o, 64));
+    }
+    if (lane == 0) {
+        s_red[0][wid] = l2;
+        s_red[1][wid] = l1;
+        s_max[wid] = ml;
+    }
+    __syncthreads();
+    const int maxlast = max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
+
+    // ---------------- reverse pass ----------------
+    uint32_t n_active = 0u, n_went = 0u;
+    if (A.backward) {
+        float dT = (dCr * kExtBg + dCg * kExtBg) + dCb * kExtBg;
+        const int nbr = (maxlast + kBatch - 1) / kBatch;
+        for (int bi = nbr - 1; bi >= 0; --bi) {
+            const int b0 = bi * kBatch;
+            const int nb = min(kBatch, len - b0);
+            __syncthreads();
+            stage_ext(A.rec, A.ids, start + (uint32_t)b0, nb, s_rec, s_ids);
+            for (int e = tid; e < 4 * kBatch * 9; e += 256) (&s_part[0][0][0])[e] = 0.0f;
+            __syncthreads();
+            const uint64_t mask = A.act[(slot0 + bi) * 4 + wid];
+            const float* ck = A.ckpt + (slot0 + bi) * (size_t)(8 * kTilePx);
+            for (int c = 7; c >= 0; --c) {
+                const uint32_t bits = (uint32_t)(mask >> (8 * c)) & 0xFFu;
+                if (bits == 0u) continue;
+                const int hi = 31 - __clz((int)bits);
+                float Tc = ck[c * kTilePx + tid];
+                float ra[8], rT[8], rG[8];
+                uint32_t incb = 0u;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    ra[j] = 0.0f;
+                    rG[j] = 0.0f;
+                    rT[j] = Tc;
+                    if (j <= hi) {
+                        const int e = b0 + 8 * c + j;
+                        if (e < last) {
+                            const ExtRec& f = s_rec[8 * c + j];
+                            ExtPair q = ext_pair(f, pxf, pyf);
+                            if (!(q.r2 > f.rcut2)) {
+                                float G = exp_det(-0.5f * q.r2);
+                                float a = f.op * G;
+                                if (a > kExtMaxAlpha) a = kExtMaxAlpha;
+                                if (!(a < kExtMinAlpha)) {
+                                    ra[j] = a;
+                                    rG[j] = G;
+                                    incb |= 1u << j;
+                                    Tc = Tc * (1.0f - a);
+                                }
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int j = 7; j >= 0; --j) {
+                    if ((bits >> j) & 1u) {
+                        const int jj = 8 * c + j;
+                        float gv[9];
+#pragma unroll
+                        for (int k = 0; k < 9; ++k) gv[k] = 0.0f;
+                        const bool inc = (incb >> j) & 1u;
+                        n_active += (uint32_t)__popcll(__ballot(inc));
+                        n_went += 1u;
+                        if (inc) {
+                            const ExtRec& f = s_rec[jj];
+                            ExtPair q = ext_pair(f, pxf, pyf);
+                            const float Tk = rT[j], a = ra[j], G = rG[j];
+                            float da = ((dCr * (Tk * f.r) + dCg * (Tk * f.g)) + dCb * (Tk * f.b)) + dT * (-Tk);
+                            dT = dT * (1.0f - a) + ((dCr * (a * f.r) + dCg * (a * f.g)) + dCb * (a * f.b));
+                            gv[5] = dCr * (Tk * a);
+                            gv[6] = dCg * (Tk * a);
+                            gv[7] = dCb * (Tk * a);
+                            if (!(f.op * G > kExtMaxAlpha)) {
+                                gv[8] = da * G;
+                                float gr2 = ((-0.5f * G) * f.op) * da;
+                                gv[0] = (-2.0f * gr2) * q.spx;
+                                gv[1] = (-2.0f * gr2) * q.spy;
+                                gv[2] = (gr2 * q.dx) * q.dx;
+                                gv[3] = ((gr2 * 2.0f) * q.dx) * q.dy;
+                                gv[4] = (gr2 * q.dy) * q.dy;
+                            }
+                        }
+                        float t8;
+                        float tot = wave_reduce9(gv, lane, t8);
+                        if ((lane & 7) == 0) {
+                            int idx = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+                            s_part[wid][jj][idx] = tot;
+                        }
+                        if (lane == 0) s_part[wid][jj][8] = t8;
+                    }
+                }
+            }
+            __syncthreads();
+            for (int e = tid; e < nb * 9; e += 256) {
+                const int j = e / 9, k = e - 9 * j;
+                double sum = ((double)s_part[0][j][k] + (double)s_part[1][j][k]) +
+                             ((double)s_part[2][j][k] + (double)s_part[3][j][k]);
+                if (sum != 0.0) unsafeAtomicAdd(&A.g2[9 * (size_t)s_ids[j] + k], sum);
+            }
+        }
+    }
+    // stats: [0] pairs evaluated by the forward, [2] pairs taken by the reverse
+    // pass, [3] reverse wave-entries, [4] forward wave-entries
+    uint32_t ev = n_eval;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) ev += __shfl_xor(ev, o, 64);
+    if (lane == 0) {
+        if (ev) atomicAdd(&A.stats[0], (unsigned long long)ev);
+        if (n_active) atomicAdd(&A.stats[2], (unsigned long long)n_active);
+        if (n_went) atomicAdd(&A.stats[3], (unsigned long long)n_went);
+        if (n_fw) atomicAdd(&A.stats[4], (unsigned long long)n_fw);
+    }
+    if (tid == 0) {
+        double a2 = (s_red[0][0] + s_red[0][1]) + (s_red[0][2] + s_red[0][3]);
+        double a1 = (s_red[1][0] + s_red[1][1]) + (s_red[1][2] + s_red[1][3]);
+        A.tile_loss[2 * tile] = a2;
+        A.tile_loss[2 * tile + 1] = a1;
+    }
+}
+
+// ---------------------------------------------------------------- chain rule
+
+template <int DEG>
+__global__ __launch_bounds__(256) void k_ext_chain(const float4* __restrict__ splats, const float4* __restrict__ sh,
+                                                   uint32_t n, ViewParams v, float cam0, float cam1, float cam2,
+                                                   const double* __restrict__ g2, const uint32_t* __restrict__ counts,
+                                                   float* __restrict__ grads, float2* __restrict__ accum) {
+    constexpr int kNk = (DEG + 1) * (DEG + 1);
+    constexpr int kSh4 = (3 * kNk + 3) / 4;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (counts[i] == 0u) return;  // culled or off-screen: no pair, no gradient
+    double gg[9];
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        gg[k] = g2[9 * (size_t)i + k];
+        any = any || gg[k] != 0.0;
+    }
+    float s[16];
+    float c[4 * kSh4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float4 t = splats[4 * (size_t)i + q];
+        s[4 * q + 0] = t.x; s[4 * q + 1] = t.y; s[4 * q + 2] = t.z; s[4 * q + 3] = t.w;
+    }
+#pragma unroll
+    for (int q = 0; q < kSh4; ++q) {
+        float4 t = sh[12 * (size_t)i + q];
+        c[4 * q + 0] = t.x; c[4 * q + 1] = t.y; c[4 * q + 2] = t.z; c[4 * q + 3] = t.w;
+    }
+    const float cam[3] = {cam0, cam1, cam2};
+    ExtProj o;
+    ext_project(s, c, DEG, v.vp, cam, v.W, v.H, o);
+    if (!o.valid) return;
+    // densification statistics: |d L / d ndc| of the centre and the view count
+    {
+        double gx = gg[0] * (double)v.aspect, gy = gg[1];
+        float2 acc = accum[i];
+        acc.x = acc.x + (float)sqrt(gx * gx + gy * gy);
+        acc.y = acc.y + 1.0f;
+        accum[i] = acc;
+    }
+    if (!any) return;
+    const float* vp = v.vp;
+    double G[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) G[k] = 0.0;
+    G[3] = gg[8] * (double)(o.op * (1.0f - o.op));
+    float Y[16], Yx[16], Yy[16], Yz[16];
+    sh_basis(o.dir[0], o.dir[1], o.dir[2], DEG, Y);
+    sh_basis_grad(o.dir[0], o.dir[1], o.dir[2], DEG, Yx, Yy, Yz);
+    double gdir[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        if (!(o.sh_raw[ch] > 0.0f)) continue;
+        const double drgb = gg[5 + ch];
+#pragma unroll
+        for (int k = 0; k < kNk; ++k) {
+            // one term per SH coefficient: written directly
+            grads[64 * (size_t)i + 16 + 3 * k + ch] += (float)(drgb * (double)Y[k]);
+            const double coef = (double)c[3 * k + ch];
+            gdir[0] += drgb * coef * (double)Yx[k];
+            gdir[1] += drgb * coef * (double)Yy[k];
+            gdir[2] += drgb * coef * (double)Yz[k];
+        }
+    }
+    const double dd = gdir[0] * o.dir[0] + gdir[1] * o.dir[1] + gdir[2] * o.dir[2];
+    double gpos[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) gpos[j] = (gdir[j] - dd * (double)o.dir[j]) / (double)o.dlen;
+    // conic -> covariance: dL/dSigma = -Q (dL/dQ) Q with dL/dQ symmetric
+    const double A = o.A, B = o.B, C = o.C;
+    const double gA = gg[2], gB = gg[3] * 0.5, gC = gg[4];
+    const double t00 = A * gA + B * gB, t01 = A * gB + B * gC, t10 = B * gA + C * gB, t11 = B * gB + C * gC;
+    const double s00 = -(t00 * A + t01 * B), s01 = -(t00 * B + t01 * C), s11 = -(t10 * B + t11 * C);
+    const double gcov00 = s00, gcov01 = 2.0 * s01, gcov11 = s11;
+    const double aspect = (double)v.aspect, w = o.w, cx = o.cx, cy = o.cy;
+    double gcx = gg[0] * aspect / w;
+    double gcy = gg[1] / w;
+    double gw = -gg[0] * cx * aspect / (w * w) - gg[1] * cy / (w * w);
+    double gqn[4] = {0.0, 0.0, 0.0, 0.0};
+    const double qw = o.qn[0], qx = o.qn[1], qy = o.qn[2], qz = o.qn[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double mx = o.m[k][0], my = o.m[k][1];
+        const double gmx = 2.0 * gcov00 * mx + gcov01 * my;
+        const double gmy = 2.0 * gcov11 * my + gcov01 * mx;
+        const double gs = gmx * (double)o.axs[k][0] + gmy * (double)o.axs[k][1];
+        G[4 + k] += gs * (double)o.s[k];
+        const double gax = gmx * (double)o.s[k], gay = gmy * (double)o.s[k];
+        const float* a = o.axw[k];
+        const double acx = (double)((vp[0] * a[0] + vp[4] * a[1]) + vp[8] * a[2]);
+        const double acy = (double)((vp[1] * a[0] + vp[5] * a[1]) + vp[9] * a[2]);
+        const double acw = (double)((vp[3] * a[0] + vp[7] * a[1]) + vp[11] * a[2]);
+        const double gacx = gax * aspect / w, gacy = gay / w;
+        const double gacw = -gax * aspect * cx / (w * w) - gay * cy / (w * w);
+        gcx += -gax * aspect * acw / (w * w);
+        gcy += -gay * acw / (w * w);
+        gw += gax * aspect * (-acx / (w * w) + 2.0 * cx * acw / (w * w * w)) +
+              gay * (-acy / (w * w) + 2.0 * cy * acw / (w * w * w));
+        double gaw[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            gaw[j] = gacx * (double)vp[4 * j + 0] + gacy * (double)vp[4 * j + 1] + gacw * (double)vp[4 * j + 3];
+        // partial derivatives of column k of R(q) (polynomial form)
+        double dRw[3], dRx[3], dRy[3], dRz[3];
+        if (k == 0) {
+            dRw[0] = 0; dRw[1] = 2 * qz; dRw[2] = -2 * qy;
+            dRx[0] = 0; dRx[1] = 2 * qy; dRx[2] = 2 * qz;
+            dRy[0] = -4 * qy; dRy[1] = 2 * qx; dRy[2] = -2 * qw;
+            dRz[0] = -4 * qz; dRz[1] = 2 * qw; dRz[2] = 2 * qx;
+        } else if (k == 1) {
+            dRw[0] = -2 * qz; dRw[1] = 0; dRw[2] = 2 * qx;
+            dRx[0] = 2 * qy; dRx[1] = -4 * qx; dRx[2] = 2 * qw;
+            dRy[0] = 2 * qx; dRy[1] = 0; dRy[2] = 2 * qz;
+            dRz[0] = -2 * qw; dRz[1] = -4 * qz; dRz[2] = 2 * qy;
+        } else {
+            dRw[0] = 2 * qy; dRw[1] = -2 * qx; dRw[2] = 0;
+            dRx[0] = 2 * qz; dRx[1] = -2 * qw; dRx[2] = -4 * qx;
+            dRy[0] = 2 * qw; dRy[1] = 2 * qz; dRy[2] = -4 * qy;
+            dRz[0] = 2 * qx; dRz[1] = 2 * qy; dRz[2] = 0;
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            gqn[0] += gaw[j] * dRw[j];
+            gqn[1] += gaw[j] * dRx[j];
+            gqn[2] += gaw[j] * dRy[j];
+            gqn[3] += gaw[j] * dRz[j];
+        }
+    }
+    const double dq = gqn[0] * qw + gqn[1] * qx + gqn[2] * qy + gqn[3] * qz;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) G[8 + j] += (gqn[j] - dq * (double)o.qn[j]) / (double)o.qlen;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+        G[j] += gpos[j] + gcx * (double)vp[4 * j + 0] + gcy * (double)vp[4 * j + 1] + gw * (double)vp[4 * j + 3];
+    float* out = grads + 64 * (size_t)i;
+#pragma unroll
+    for (int j = 0; j < 12; ++j)
+        if (j != 7) out[j] = out[j] + (float)G[j];
+}
+
+// ---------------------------------------------------------------- Adam
+
+// Learning rate of parameter p of the 64 (record then SH): means 1.6e-4,
+// opacity logit 5e-2, log-scales 5e-3, quaternion 1e-3, SH DC 2.5e-3, higher
+// SH bands 2.5e-3 / 20; 0 (left alone) for the pads and SH above the degree.
+__device__ __forceinline__ float ext_lr(int p, int deg) {
+    if (p < 3) return 1.6e-4f;
+    if (p == 3) return 5e-2f;
+    if (p >= 4 && p < 7) return 5e-3f;
+    if (p >= 8 && p < 12) return 1e-3f;
+    if (p >= 16) {
+        int k = (p - 16) / 3;
+        if (k >= (deg + 1) * (deg + 1)) return 0.0f;
+        return k == 0 ? 2.5e-3f : 2.5e-3f / 20.0f;
+    }
+    return 0.0f;
+}
+
+// One thread per float4 of the 64 parameters: reads parameter + grad + m + v
+// (64 B), writes parameter + m + v and a zeroed grad (64 B).
+__global__ __launch_bounds__(256) void k_adam_ext(float4* __restrict__ splats, float4* __restrict__ sh,
+                                                  float4* __restrict__ grads, float4* __restrict__ m,
+                                                  float4* __restrict__ v, uint32_t n, int deg, float bc1, float bc2) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 16 * (size_t)n) return;
+    const size_t i = idx >> 4;
+    const int part = (int)(idx & 15);
+    const float lr0 = ext_lr(4 * part, deg);
+    const float lr3 = ext_lr(4 * part + 3, deg);
+    if (lr0 == 0.0f && lr3 == 0.0f && ext_lr(4 * part + 1, deg) == 0.0f && ext_lr(4 * part + 2, deg) == 0.0f) return;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-15f;
+    float4* xp = part < 4 ? &splats[4 * i + part] : &sh[12 * i + (part - 4)];
+    float4 x4 = *xp, g4 = grads[idx], m4 = m[idx], v4 = v[idx];
+    float x[4] = {x4.x, x4.y, x4.z, x4.w}, g[4] = {g4.x, g4.y, g4.z, g4.w};
+    float mm[4] = {m4.x, m4.y, m4.z, m4.w}, vv[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+        const float lr = ext_lr(4 * part + l, deg);
+        if (lr == 0.0f) continue;
+        float m1 = b1 * mm[l] + (1.0f - b1) * g[l];
+        float v1 = b2 * vv[l] + ((1.0f - b2) * g[l]) * g[l];
+        mm[l] = m1;
+        vv[l] = v1;
+        float upd = (lr * (m1 / bc1)) / (sqrtf(v1 / bc2) + eps);
+        x[l] = x[l] - upd;
+    }
+    *xp = make_float4(x[0], x[1], x[2], x[3]);
+    m[idx] = make_float4(mm[0], mm[1], mm[2], mm[3]);
+    v[idx] = make_float4(vv[0], vv[1], vv[2], vv[3]);
+    grads[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// ---------------------------------------------------------------- launchers
+
+void launch_ext_preprocess(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg,
+                           const ViewParams& v, const float cam[3], ExtRec* rec, uint32_t* counts, uint4* rects,
+                           uint32_t* dkey) {
+    if (n == 0) return;
+    dim3 g((n + 255) / 256), b(256);
+    const float4* s4 = (const float4*)splats;
+    const float4* h4 = (const float4*)sh;
+    switch (deg) {
+        case 0: hipLaunchKernelGGL(k_ext_preprocess<0>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], rec, counts, rects, dkey); break;
+        case 1: hipLaunchKernelGGL(k_ext_preprocess<1>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], rec, counts, rects, dkey); break;
+        case 2: hipLaunchKernelGGL(k_ext_preprocess<2>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], rec, counts, rects, dkey); break;
+        default: hipLaunchKernelGGL(k_ext_preprocess<3>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], rec, counts, rects, dkey); break;
+    }
+}
+
+void launch_ext_keys(hipStream_t st, uint32_t n, const uint32_t* counts, const uint4* rects, const uint32_t* dkey,
+                     uint32_t tiles_x, const uint64_t* offsets, uint64_t* keys, uint32_t* vals) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_ext_keys, dim3((n + 255) / 256), dim3(256), 0, st, n, counts, rects, dkey, tiles_x, offsets,
+                       keys, vals);
+}
+
+void launch_composite_ext(hipStream_t st, const ViewParams& v, const ExtCompositeArgs& a) {
+    if (v.num_tiles == 0) return;
+    hipLaunchKernelGGL(k_composite_ext, dim3(v.num_tiles), dim3(256), 0, st, v, a);
+}
+
+void launch_ext_chain(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg, const ViewParams& v,
+                      const float cam[3], const double* g2, const uint32_t* counts, float* grads, float* accum) {
+    if (n == 0) return;
+    dim3 g((n + 255) / 256), b(256);
+    const float4* s4 = (const float4*)splats;
+    const float4* h4 = (const float4*)sh;
+    float2* a2 = (float2*)accum;
+    switch (deg) {
+        case 0: hipLaunchKernelGGL(k_ext_chain<0>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2); break;
+        case 1: hipLaunchKernelGGL(k_ext_chain<1>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2); break;
+        case 2: hipLaunchKernelGGL(k_ext_chain<2>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2); break;
+        default: hipLaunchKernelGGL(k_ext_chain<3>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2); break;
+    }
+}
+
+void launch_adam_ext(hipStream_t st, float* splats, float* sh, float* grads, float* m, float* v, uint32_t n, int deg,
+                     float t) {
+    if (n == 0) return;
+    const float bc1 = 1.0f - pow_det(0.9f, t);
+    const float bc2 = 1.0f - pow_det(0.999f, t);
+    const size_t threads = 16 * (size_t)n;
+    hipLaunchKernelGGL(k_adam_ext, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, (float4*)splats,
+                       (float4*)sh, (float4*)grads, (float4*)m, (float4*)v, n, deg, bc1, bc2);
+}
+
+}  // namespace nps

@@ -6,6 +6,9 @@
 //              LSD radix sort on the tile bits -> tile ranges -> fused composite
 //   step     = fused Adam
 //   adc      = every adc_period-th dispatch
+// EXT mode: preprocess (SH colour, 3-sigma rectangles, depth keys) -> scan ->
+// (tile | depth, id) pairs -> LSD radix sort with values -> tile ranges ->
+// composite with early termination -> per-splat chain rule; step = EXT Adam.
 // All work is enqueued on the handle's stream (default priority).  Per-view
 // scratch grows on demand and is kept for the next call.
 #include <hip/hip_runtime.h>
@@ -31,6 +34,7 @@ struct DevBuf {
 struct Scratch {
     DevBuf counts, rects, offsets, keys, keys_alt, ranges, scan_tmp, sort_tmp;
     DevBuf diff, cut, full, kept;  // per-tile id-block cut of capped (REF) lists
+    DevBuf vals, vals_alt, dkey;   // EXT: splat ids sorted with the keys, depth keys
     uint64_t m = 0;
     bool in_alt = false;
     bool have_full = false;
@@ -42,6 +46,8 @@ struct Scratch {
 struct nps_ctx {
     nps_config cfg;
     int device = 0;
+    bool ext = false;  // NPS_MODE_EXT
+    int deg = 0;       // EXT SH degree
     hipStream_t stream = nullptr;
     uint32_t n = 0;
     float* splats = nullptr;
@@ -49,6 +55,8 @@ struct nps_ctx {
     float* m = nullptr;
     float* v = nullptr;
     float* accum = nullptr;
+    float* sh = nullptr;     // EXT: 48 per splat
+    float* gradx = nullptr;  // EXT: 64 FP32 gradients per splat
     float adam_t = 0.0f;
     int64_t step_count = 0;
     float vp[16] = {0}, vp_inv[16] = {0};
@@ -58,6 +66,7 @@ struct nps_ctx {
     bool has_target = false;
     DevBuf tgt_rgba, tgt_depth;
     DevBuf fwd, bwd, ckpt, c_img, d_img, tile_loss, stats;
+    DevBuf act, g2;  // EXT: per-batch activity masks, per-splat 2D gradients (FP64)
     Scratch sc;
     uint32_t last_tiles = 0;
     bool have_backward = false;
@@ -121,6 +130,7 @@ void free_scratch(Scratch& s) {
     free_buf(s.counts); free_buf(s.rects); free_buf(s.offsets); free_buf(s.keys);
     free_buf(s.keys_alt); free_buf(s.ranges); free_buf(s.scan_tmp); free_buf(s.sort_tmp);
     free_buf(s.diff); free_buf(s.cut); free_buf(s.full); free_buf(s.kept);
+    free_buf(s.vals); free_buf(s.vals_alt); free_buf(s.dkey);
 }
 
 // Ids per block of the per-tile cut: about 32 blocks, at least 4096 splats each.
@@ -183,6 +193,101 @@ int build_tile_lists(nps_ctx* h, Scratch& sc, const ViewParams& v, uint32_t n, i
     return NPS_OK;
 }
 
+// EXT view: preprocess -> keys -> sort -> ranges -> composite (-> chain rule).
+int run_ext(nps_ctx* h, bool grads) {
+    NPS_HIP(h, hipSetDevice(h->device));
+    const uint32_t n = h->n;
+    const size_t n1 = std::max<uint32_t>(n, 1);
+    ViewParams v = make_view(h->vp, h->vp_inv, h->W, h->H);
+    // camera centre: VP^-1 (0, 0, 1, 0) dehomogenised, in double, then rounded
+    float cam[3];
+    for (int k = 0; k < 3; ++k) cam[k] = (float)((double)h->vp_inv[8 + k] / (double)h->vp_inv[11]);
+    Scratch& sc = h->sc;
+    int rc;
+    if ((rc = ensure(h, h->fwd, sizeof(ExtRec) * n1)) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.counts, 4 * n1)) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.rects, 16 * n1)) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.dkey, 4 * n1)) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.offsets, sizeof(uint64_t) * (n1 + 1))) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.scan_tmp, scan_temp_bytes(n) + 64)) != NPS_OK) return rc;
+    if ((rc = ensure(h, h->stats, 64)) != NPS_OK) return rc;
+    if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[0], h->stream));
+    launch_fill_u32(h->stream, (uint32_t*)h->stats.p, 0u, 16);
+    launch_ext_preprocess(h->stream, h->splats, h->sh, n, h->deg, v, cam, (ExtRec*)h->fwd.p, (uint32_t*)sc.counts.p,
+                          (uint4*)sc.rects.p, (uint32_t*)sc.dkey.p);
+    exclusive_scan_u32_to_u64(h->stream, (const uint32_t*)sc.counts.p, (uint64_t*)sc.offsets.p, n, sc.scan_tmp.p);
+    uint64_t m = 0;
+    NPS_HIP(h, hipMemcpyAsync(&m, (uint64_t*)sc.offsets.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost, h->stream));
+    NPS_HIP(h, hipStreamSynchronize(h->stream));
+    if (m >= 0xFFFFFFFFull) return set_err(h, NPS_ERR_CAPACITY, "tile key count exceeds 2^32");
+    sc.m = m;
+    sc.have_full = false;
+    const size_t m1 = std::max<uint64_t>(m, 1);
+    if ((rc = ensure(h, sc.keys, 8 * m1)) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.keys_alt, 8 * m1)) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.vals, 4 * m1)) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.vals_alt, 4 * m1)) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.sort_tmp, radix_temp_bytes(m))) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.ranges, sizeof(uint32_t) * 2 * (size_t)std::max<uint32_t>(v.num_tiles, 1))) != NPS_OK)
+        return rc;
+    // batch slots of the composite: tile t uses [start/64 + t, ...), at most ceil(len/64) of them
+    const size_t slots = (size_t)(m / kBatch) + v.num_tiles + 1;
+    if (grads) {
+        if ((rc = ensure(h, h->ckpt, sizeof(float) * slots * 8 * kTilePx)) != NPS_OK) return rc;
+        if ((rc = ensure(h, h->act, sizeof(uint64_t) * slots * 4)) != NPS_OK) return rc;
+        if ((rc = ensure(h, h->g2, sizeof(double) * 9 * n1)) != NPS_OK) return rc;
+    }
+    const size_t px = (size_t)h->W * h->H;
+    if ((rc = ensure(h, h->c_img, 12 * px)) != NPS_OK) return rc;
+    if ((rc = ensure(h, h->tile_loss, 16 * (size_t)std::max<uint32_t>(v.num_tiles, 1))) != NPS_OK) return rc;
+    launch_ext_keys(h->stream, n, (const uint32_t*)sc.counts.p, (const uint4*)sc.rects.p,
+                    (const uint32_t*)sc.dkey.p, v.tiles_x, (const uint64_t*)sc.offsets.p, (uint64_t*)sc.keys.p,
+                    (uint32_t*)sc.vals.p);
+    if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[1], h->stream));
+    const int tile_bits = bits_for(v.num_tiles > 0 ? v.num_tiles - 1 : 0);
+    sc.in_alt = radix_sort_u64(h->stream, (uint64_t*)sc.keys.p, (uint64_t*)sc.keys_alt.p, (uint32_t*)sc.vals.p,
+                               (uint32_t*)sc.vals_alt.p, m, 0, 32 + tile_bits, sc.sort_tmp.p);
+    if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[2], h->stream));
+    launch_fill_u32(h->stream, (uint32_t*)sc.ranges.p, 0u, 2 * (uint64_t)v.num_tiles);
+    launch_tile_ranges(h->stream, (const uint64_t*)(sc.in_alt ? sc.keys_alt.p : sc.keys.p), m,
+                       (uint32_t*)sc.ranges.p);
+    if (grads) launch_fill_u32(h->stream, (uint32_t*)h->g2.p, 0u, 18 * (uint64_t)n);
+    if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[3], h->stream));
+    ExtCompositeArgs a;
+    a.rec = (const ExtRec*)h->fwd.p;
+    a.ids = (const uint32_t*)(sc.in_alt ? sc.vals_alt.p : sc.vals.p);
+    a.ranges = (const uint32_t*)sc.ranges.p;
+    a.tgt_rgba = (const float4*)h->tgt_rgba.p;
+    a.g2 = grads ? (double*)h->g2.p : nullptr;
+    a.c_img = (float*)h->c_img.p;
+    a.tile_loss = (double*)h->tile_loss.p;
+    a.ckpt = grads ? (float*)h->ckpt.p : nullptr;
+    a.act = grads ? (uint64_t*)h->act.p : nullptr;
+    a.stats = (unsigned long long*)h->stats.p;
+    a.backward = grads ? 1 : 0;
+    launch_composite_ext(h->stream, v, a);
+    if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[4], h->stream));
+    if (grads)
+        launch_ext_chain(h->stream, h->splats, h->sh, n, h->deg, v, cam, (const double*)h->g2.p,
+                         (const uint32_t*)sc.counts.p, h->gradx, h->accum);
+    if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[7], h->stream));
+    NPS_HIP(h, hipGetLastError());
+    h->last_tiles = v.num_tiles;
+    h->have_backward = true;
+    h->stats_host[0] = (int64_t)m;
+    if (h->timing) {
+        NPS_HIP(h, hipEventSynchronize(h->ev[7]));
+        float t;
+        hipEventElapsedTime(&t, h->ev[0], h->ev[1]); h->timing_ms[0] = t;
+        hipEventElapsedTime(&t, h->ev[1], h->ev[2]); h->timing_ms[1] = t;
+        hipEventElapsedTime(&t, h->ev[2], h->ev[3]); h->timing_ms[2] = t;
+        hipEventElapsedTime(&t, h->ev[3], h->ev[4]); h->timing_ms[3] = t;
+        hipEventElapsedTime(&t, h->ev[4], h->ev[7]); h->timing_ms[7] = t;
+        hipEventElapsedTime(&t, h->ev[0], h->ev[7]); h->timing_ms[6] = t;
+    }
+    return NPS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -203,10 +308,14 @@ int nps_create(const nps_config* cfg, int32_t num_splats, nps_handle* out) {
     nps_ctx* h = new nps_ctx();
     if (cfg) h->cfg = *cfg;
     else nps_config_default(&h->cfg);
-    if (h->cfg.max_tile_splats <= 0 || h->cfg.adc_period <= 0) {
+    if (h->cfg.max_tile_splats <= 0 || h->cfg.adc_period <= 0 ||
+        (h->cfg.mode != NPS_MODE_REF && h->cfg.mode != NPS_MODE_EXT) || h->cfg.sh_degree < 0 ||
+        h->cfg.sh_degree > 3) {
         delete h;
         return NPS_ERR_ARG;
     }
+    h->ext = h->cfg.mode == NPS_MODE_EXT;
+    h->deg = h->cfg.sh_degree;
     h->device = h->cfg.device;
     h->n = (uint32_t)num_splats;
     auto fail = [&](hipError_t e) {
@@ -216,6 +325,8 @@ int nps_create(const nps_config* cfg, int32_t num_splats, nps_handle* out) {
         if (h->m) (void)hipFree(h->m);
         if (h->v) (void)hipFree(h->v);
         if (h->accum) (void)hipFree(h->accum);
+        if (h->sh) (void)hipFree(h->sh);
+        if (h->gradx) (void)hipFree(h->gradx);
         if (h->stream) (void)hipStreamDestroy(h->stream);
         delete h;
         return NPS_ERR_HIP;
@@ -223,17 +334,27 @@ int nps_create(const nps_config* cfg, int32_t num_splats, nps_handle* out) {
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return fail(e);
     if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e);
-    size_t n16 = 16 * (size_t)std::max<uint32_t>(h->n, 1);
+    const size_t n1 = (size_t)std::max<uint32_t>(h->n, 1);
+    const size_t n16 = 16 * n1;
+    const size_t nopt = (h->ext ? 64 : 16) * n1;  // Adam state per splat
+    const size_t nacc = (h->ext ? 2 : 1) * n1;
     if ((e = hipMalloc((void**)&h->splats, n16 * 4)) != hipSuccess) return fail(e);
-    if ((e = hipMalloc((void**)&h->grads, n16 * 4)) != hipSuccess) return fail(e);
-    if ((e = hipMalloc((void**)&h->m, n16 * 4)) != hipSuccess) return fail(e);
-    if ((e = hipMalloc((void**)&h->v, n16 * 4)) != hipSuccess) return fail(e);
-    if ((e = hipMalloc((void**)&h->accum, (n16 / 16) * 4)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc((void**)&h->m, nopt * 4)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc((void**)&h->v, nopt * 4)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc((void**)&h->accum, nacc * 4)) != hipSuccess) return fail(e);
     hipMemsetAsync(h->splats, 0, n16 * 4, h->stream);
-    hipMemsetAsync(h->grads, 0, n16 * 4, h->stream);
-    hipMemsetAsync(h->m, 0, n16 * 4, h->stream);
-    hipMemsetAsync(h->v, 0, n16 * 4, h->stream);
-    hipMemsetAsync(h->accum, 0, (n16 / 16) * 4, h->stream);
+    hipMemsetAsync(h->m, 0, nopt * 4, h->stream);
+    hipMemsetAsync(h->v, 0, nopt * 4, h->stream);
+    hipMemsetAsync(h->accum, 0, nacc * 4, h->stream);
+    if (h->ext) {
+        if ((e = hipMalloc((void**)&h->sh, 48 * n1 * 4)) != hipSuccess) return fail(e);
+        if ((e = hipMalloc((void**)&h->gradx, 64 * n1 * 4)) != hipSuccess) return fail(e);
+        hipMemsetAsync(h->sh, 0, 48 * n1 * 4, h->stream);
+        hipMemsetAsync(h->gradx, 0, 64 * n1 * 4, h->stream);
+    } else {
+        if ((e = hipMalloc((void**)&h->grads, n16 * 4)) != hipSuccess) return fail(e);
+        hipMemsetAsync(h->grads, 0, n16 * 4, h->stream);
+    }
     for (int k = 0; k < 8; ++k)
         if ((e = hipEventCreate(&h->ev[k])) != hipSuccess) return fail(e);
     if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return fail(e);
@@ -245,8 +366,11 @@ void nps_destroy(nps_handle h) {
     if (!h) return;
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
-    (void)hipFree(h->splats); (void)hipFree(h->grads); (void)hipFree(h->m); (void)hipFree(h->v);
-    (void)hipFree(h->accum);
+    (void)hipFree(h->splats); (void)hipFree(h->m); (void)hipFree(h->v); (void)hipFree(h->accum);
+    if (h->grads) (void)hipFree(h->grads);
+    if (h->sh) (void)hipFree(h->sh);
+    if (h->gradx) (void)hipFree(h->gradx);
+    free_buf(h->act); free_buf(h->g2);
     free_buf(h->tgt_rgba); free_buf(h->tgt_depth); free_buf(h->fwd); free_buf(h->bwd); free_buf(h->ckpt);
     free_buf(h->c_img); free_buf(h->d_img); free_buf(h->tile_loss); free_buf(h->stats);
     free_buf(h->adc_u32); free_buf(h->adc_tmp); free_buf(h->adc_scan);
@@ -294,6 +418,7 @@ int nps_read_splats(nps_handle h, float* s, int32_t n) {
 int nps_write_grads(nps_handle h, const int32_t* g, int32_t n) {
     int rc = check_n(h, n);
     if (rc) return rc;
+    if (h->ext) return set_err(h, NPS_ERR_STATE, "EXT mode: gradients are FP32 (nps_write_grads_f32)");
     if (!g && n) return set_err(h, NPS_ERR_ARG, "null pointer");
     NPS_COPY(h, h->grads, g, 64 * (size_t)n, hipMemcpyHostToDevice);
     return NPS_OK;
@@ -301,6 +426,7 @@ int nps_write_grads(nps_handle h, const int32_t* g, int32_t n) {
 int nps_read_grads(nps_handle h, int32_t* g, int32_t n) {
     int rc = check_n(h, n);
     if (rc) return rc;
+    if (h->ext) return set_err(h, NPS_ERR_STATE, "EXT mode: gradients are FP32 (nps_read_grads_f32)");
     if (!g && n) return set_err(h, NPS_ERR_ARG, "null pointer");
     NPS_COPY(h, g, h->grads, 64 * (size_t)n, hipMemcpyDeviceToHost);
     return NPS_OK;
@@ -309,16 +435,18 @@ int nps_write_adam(nps_handle h, const float* m, const float* v, float t, int32_
     int rc = check_n(h, n);
     if (rc) return rc;
     if ((!m || !v) && n) return set_err(h, NPS_ERR_ARG, "null pointer");
-    NPS_COPY(h, h->m, m, 64 * (size_t)n, hipMemcpyHostToDevice);
-    NPS_COPY(h, h->v, v, 64 * (size_t)n, hipMemcpyHostToDevice);
+    const size_t per = h->ext ? 256 : 64;
+    NPS_COPY(h, h->m, m, per * (size_t)n, hipMemcpyHostToDevice);
+    NPS_COPY(h, h->v, v, per * (size_t)n, hipMemcpyHostToDevice);
     h->adam_t = t;
     return NPS_OK;
 }
 int nps_read_adam(nps_handle h, float* m, float* v, float* t, int32_t n) {
     int rc = check_n(h, n);
     if (rc) return rc;
-    if (m) NPS_COPY(h, m, h->m, 64 * (size_t)n, hipMemcpyDeviceToHost);
-    if (v) NPS_COPY(h, v, h->v, 64 * (size_t)n, hipMemcpyDeviceToHost);
+    const size_t per = h->ext ? 256 : 64;
+    if (m) NPS_COPY(h, m, h->m, per * (size_t)n, hipMemcpyDeviceToHost);
+    if (v) NPS_COPY(h, v, h->v, per * (size_t)n, hipMemcpyDeviceToHost);
     if (t) *t = h->adam_t;
     return NPS_OK;
 }
@@ -326,15 +454,36 @@ int nps_write_accum(nps_handle h, const float* a, int32_t n) {
     int rc = check_n(h, n);
     if (rc) return rc;
     if (!a && n) return set_err(h, NPS_ERR_ARG, "null pointer");
-    NPS_COPY(h, h->accum, a, 4 * (size_t)n, hipMemcpyHostToDevice);
+    NPS_COPY(h, h->accum, a, (h->ext ? 8 : 4) * (size_t)n, hipMemcpyHostToDevice);
     return NPS_OK;
 }
 int nps_read_accum(nps_handle h, float* a, int32_t n) {
     int rc = check_n(h, n);
     if (rc) return rc;
     if (!a && n) return set_err(h, NPS_ERR_ARG, "null pointer");
-    NPS_COPY(h, a, h->accum, 4 * (size_t)n, hipMemcpyDeviceToHost);
+    NPS_COPY(h, a, h->accum, (h->ext ? 8 : 4) * (size_t)n, hipMemcpyDeviceToHost);
     return NPS_OK;
+}
+
+static int ext_copy(nps_handle h, int32_t n, const void* src, void* dst, size_t per, hipMemcpyKind kind) {
+    int rc = check_n(h, n);
+    if (rc) return rc;
+    if (!h->ext) return set_err(h, NPS_ERR_STATE, "EXT mode only");
+    if ((!src || !dst) && n) return set_err(h, NPS_ERR_ARG, "null pointer");
+    NPS_COPY(h, dst, src, per * (size_t)n, kind);
+    return NPS_OK;
+}
+int nps_write_sh(nps_handle h, const float* sh, int32_t n) {
+    return ext_copy(h, n, sh, h ? h->sh : nullptr, 192, hipMemcpyHostToDevice);
+}
+int nps_read_sh(nps_handle h, float* sh, int32_t n) {
+    return ext_copy(h, n, h ? h->sh : nullptr, sh, 192, hipMemcpyDeviceToHost);
+}
+int nps_write_grads_f32(nps_handle h, const float* g, int32_t n) {
+    return ext_copy(h, n, g, h ? h->gradx : nullptr, 256, hipMemcpyHostToDevice);
+}
+int nps_read_grads_f32(nps_handle h, float* g, int32_t n) {
+    return ext_copy(h, n, h ? h->gradx : nullptr, g, 256, hipMemcpyDeviceToHost);
 }
 
 int nps_set_camera(nps_handle h, const float vp[16], const float vp_inv[16], int32_t blur_enabled) {
@@ -375,6 +524,7 @@ int nps_backward(nps_handle h) {
     if (!h) return NPS_ERR_ARG;
     if (!h->has_target) return set_err(h, NPS_ERR_STATE, "no backward target (nps_set_target)");
     if (!h->has_camera) return set_err(h, NPS_ERR_STATE, "no camera (nps_set_camera)");
+    if (h->ext) return run_ext(h, true);
     NPS_HIP(h, hipSetDevice(h->device));
     const uint32_t n = h->n;
     ViewParams v = make_view(h->vp, h->vp_inv, h->W, h->H);
@@ -443,12 +593,21 @@ int nps_backward(nps_handle h) {
     return NPS_OK;
 }
 
+int nps_forward(nps_handle h) {
+    if (!h) return NPS_ERR_ARG;
+    if (!h->ext) return set_err(h, NPS_ERR_STATE, "nps_forward is EXT mode only (REF renders with npf_*)");
+    if (!h->has_target) return set_err(h, NPS_ERR_STATE, "no target size (nps_set_target)");
+    if (!h->has_camera) return set_err(h, NPS_ERR_STATE, "no camera (nps_set_camera)");
+    return run_ext(h, false);
+}
+
 int nps_step(nps_handle h) {
     if (!h) return NPS_ERR_ARG;
     NPS_HIP(h, hipSetDevice(h->device));
     float t = h->adam_t + 1.0f;
     if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[5], h->stream));
-    launch_adam_ref(h->stream, h->splats, h->grads, h->m, h->v, h->accum, h->n, t);
+    if (h->ext) launch_adam_ext(h->stream, h->splats, h->sh, h->gradx, h->m, h->v, h->n, h->deg, t);
+    else launch_adam_ref(h->stream, h->splats, h->grads, h->m, h->v, h->accum, h->n, t);
     if (h->timing) {
         NPS_HIP(h, hipEventRecord(h->ev[6], h->stream));
         NPS_HIP(h, hipEventSynchronize(h->ev[6]));
@@ -461,6 +620,7 @@ int nps_step(nps_handle h) {
 
 int nps_adc(nps_handle h) {
     if (!h) return NPS_ERR_ARG;
+    if (h->ext) return set_err(h, NPS_ERR_STATE, "EXT densification is not available");
     NPS_HIP(h, hipSetDevice(h->device));
     const uint32_t n = h->n;
     int rc;
@@ -518,7 +678,8 @@ int nps_read_image(nps_handle h, float* c_pred, float* d_pred) {
     if (!h->have_backward) return set_err(h, NPS_ERR_STATE, "no backward has run");
     size_t px = (size_t)h->W * h->H;
     if (c_pred) NPS_COPY(h, c_pred, h->c_img.p, 12 * px, hipMemcpyDeviceToHost);
-    if (d_pred) NPS_COPY(h, d_pred, h->d_img.p, 4 * px, hipMemcpyDeviceToHost);
+    if (d_pred && h->ext) memset(d_pred, 0, 4 * px);
+    else if (d_pred) NPS_COPY(h, d_pred, h->d_img.p, 4 * px, hipMemcpyDeviceToHost);
     return NPS_OK;
 }
 
@@ -543,6 +704,17 @@ int nps_read_tiles(nps_handle h, uint32_t* ranges, uint32_t* ids, uint32_t* full
     if (!h->have_backward) return set_err(h, NPS_ERR_STATE, "no backward has run");
     if (num_tiles) *num_tiles = (int32_t)h->last_tiles;
     if (num_ids) *num_ids = (int64_t)h->sc.m;
+    if (h->ext) {
+        std::vector<uint32_t> r(2 * (size_t)h->last_tiles);
+        NPS_COPY(h, r.data(), h->sc.ranges.p, 8 * (size_t)h->last_tiles, hipMemcpyDeviceToHost);
+        if (ranges) memcpy(ranges, r.data(), 8 * (size_t)h->last_tiles);
+        if (full_counts)
+            for (size_t t = 0; t < h->last_tiles; ++t) full_counts[t] = r[2 * t + 1] - r[2 * t];
+        if (ids && h->sc.m)
+            NPS_COPY(h, ids, h->sc.in_alt ? h->sc.vals_alt.p : h->sc.vals.p, 4 * (size_t)h->sc.m,
+                     hipMemcpyDeviceToHost);
+        return NPS_OK;
+    }
     if (ranges) NPS_COPY(h, ranges, h->sc.ranges.p, 8 * (size_t)h->last_tiles, hipMemcpyDeviceToHost);
     if (full_counts) NPS_COPY(h, full_counts, h->sc.full.p, 4 * (size_t)h->last_tiles, hipMemcpyDeviceToHost);
     if (ids && h->sc.m) {
@@ -585,6 +757,7 @@ int nps_read_timing(nps_handle h, float ms[8]) {
 
 int npf_create(nps_handle owner, npf_handle* out) {
     if (!owner || !out) return NPS_ERR_ARG;
+    if (owner->ext) return set_err(owner, NPS_ERR_STATE, "npf_* renders REF splats; EXT renders with nps_forward");
     npf_ctx* f = new npf_ctx();
     f->owner = owner;
     *out = f;

@@ -133,4 +133,25 @@ __device__ __forceinline__ int32_t f2i_hw(float x) {
     return r;
 }
 
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t i, uint32_t n) {
+    // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+    // contiguous run of tiles so neighbouring tiles share its L2.
+    uint32_t per = n / 8u, rem = n % 8u;
+    uint32_t xcd = i % 8u, local = i / 8u;
+    return xcd < rem ? xcd * (per + 1u) + local : rem * (per + 1u) + (xcd - rem) * per + local;
+}
+
+// First index in [0, n) whose predicate is false, for a predicate that is true
+// on a prefix and false after it (n if always true).
+template <class P>
+__device__ __forceinline__ uint32_t first_false(uint32_t n, P pred) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (pred(mid)) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 }  // namespace nps

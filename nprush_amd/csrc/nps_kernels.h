@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "nps_common.h"
+#include "nps_ext.h"
 
 namespace nps {
 
@@ -87,6 +88,32 @@ struct RenderArgs {
 void launch_preprocess_render(hipStream_t st, const float* splats, uint32_t n, const ViewParams& v, float4* rec,
                               uint32_t* counts, uint4* rects);
 void launch_render(hipStream_t st, const ViewParams& v, const RenderArgs& a);
+
+// k_ext.hip
+struct ExtCompositeArgs {
+    const ExtRec* rec;
+    const uint32_t* ids;        // splat ids in (tile, depth, id) order
+    const uint32_t* ranges;     // [2 * num_tiles]
+    const float4* tgt_rgba;     // H*W
+    double* g2;                 // 9 per splat: d scr.xy, d conic A,2B,C, d rgb, d opacity
+    float* c_img;               // H*W*3
+    double* tile_loss;          // 2 * num_tiles: L2, L1
+    float* ckpt;                // per batch slot: 8 chunks x 256 pixels of T
+    uint64_t* act;              // per batch slot: 4 waves x 64 entry bits
+    unsigned long long* stats;  // [0] forward pairs, [2] reverse pairs, [3] reverse wave-entries,
+                                // [4] forward wave-entries
+    int backward;               // 0: image and loss only
+};
+void launch_ext_preprocess(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg,
+                           const ViewParams& v, const float cam[3], ExtRec* rec, uint32_t* counts, uint4* rects,
+                           uint32_t* dkey);
+void launch_ext_keys(hipStream_t st, uint32_t n, const uint32_t* counts, const uint4* rects, const uint32_t* dkey,
+                     uint32_t tiles_x, const uint64_t* offsets, uint64_t* keys, uint32_t* vals);
+void launch_composite_ext(hipStream_t st, const ViewParams& v, const ExtCompositeArgs& a);
+void launch_ext_chain(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg, const ViewParams& v,
+                      const float cam[3], const double* g2, const uint32_t* counts, float* grads, float* accum);
+void launch_adam_ext(hipStream_t st, float* splats, float* sh, float* grads, float* m, float* v, uint32_t n, int deg,
+                     float t);
 
 // k_debug.hip
 void launch_debug_math(hipStream_t st, int op, const float* x, const float* y, float* out, uint32_t n);

@@ -14,6 +14,13 @@ ignored.  Differences from the reference:
 * splats are written explicitly (``write_splats``) instead of the reference's
   unseeded Math.random() initialisation (GpuSplatOptimizerManager.ts:59-86);
   ``reference_init`` reproduces that distribution with a seed.
+
+``mode="ext"`` selects the EXT model of the same loop (3D Gaussians with SH
+colour, depth-ordered uncapped tiles, early termination, L1 loss, FP32
+gradients; no reference counterpart): splats use the EXT record, SH
+coefficients live in their own buffer (``write_sh``), gradients are FP32
+(``read_grads_f32``, 64 per splat), Adam state has 64 entries per splat and
+``forward()`` renders without gradients.
 """
 from __future__ import annotations
 
@@ -51,7 +58,8 @@ class GpuSplatOptimizerManager:
 
     def __init__(self, device=None, format=None, numSplats: int = 512, numBeziers=None, *,
                  max_tile_splats: int = 1024, adc_period: int = 25, adc_enabled: bool = True, hip_device: int = 0,
-                 splats: np.ndarray | None = None):
+                 splats: np.ndarray | None = None, mode: str = "ref", sh_degree: int = 0,
+                 sh: np.ndarray | None = None):
         lib = _lib.load()
         cfg = _lib.NpsConfig()
         check(lib.nps_config_default(C.byref(cfg)), None, "nps_config_default")
@@ -59,15 +67,24 @@ class GpuSplatOptimizerManager:
         cfg.adc_period = adc_period
         cfg.adc_enabled = 1 if adc_enabled else 0
         cfg.device = hip_device
+        if mode not in ("ref", "ext"):
+            raise ValueError("mode must be 'ref' or 'ext'")
+        cfg.mode = _lib.MODE_EXT if mode == "ext" else _lib.MODE_REF
+        cfg.sh_degree = int(sh_degree)
         h = C.c_void_p()
         check(lib.nps_create(C.byref(cfg), int(numSplats), C.byref(h)), None, "nps_create")
         self._lib = lib
         self._h = h
         self.numSplats = int(numSplats)
-        self.numParams = self.numSplats * 16
+        self.mode = mode
+        self.sh_degree = int(sh_degree)
+        self.opt_width = 64 if mode == "ext" else 16  # Adam entries per splat
+        self.numParams = self.numSplats * self.opt_width
         self.dims = (0, 0)
         if splats is not None:
             self.write_splats(splats)
+        if sh is not None:
+            self.write_sh(sh)
 
     @property
     def handle(self):
@@ -104,24 +121,46 @@ class GpuSplatOptimizerManager:
         return a
 
     def write_adam(self, m: np.ndarray, v: np.ndarray, t: float):
-        m = np.ascontiguousarray(m, dtype=np.float32).reshape(self.numSplats, 16)
-        v = np.ascontiguousarray(v, dtype=np.float32).reshape(self.numSplats, 16)
+        m = np.ascontiguousarray(m, dtype=np.float32).reshape(self.numSplats, self.opt_width)
+        v = np.ascontiguousarray(v, dtype=np.float32).reshape(self.numSplats, self.opt_width)
         check(self._lib.nps_write_adam(self._h, fptr(m), fptr(v), float(t), self.numSplats), self._h, "write_adam")
 
     def read_adam(self):
-        m = np.empty((self.numSplats, 16), dtype=np.float32)
-        v = np.empty((self.numSplats, 16), dtype=np.float32)
+        m = np.empty((self.numSplats, self.opt_width), dtype=np.float32)
+        v = np.empty((self.numSplats, self.opt_width), dtype=np.float32)
         t = C.c_float()
         check(self._lib.nps_read_adam(self._h, fptr(m), fptr(v), C.byref(t), self.numSplats), self._h, "read_adam")
         return m, v, float(t.value)
 
+    def _accum_shape(self):
+        return (self.numSplats, 2) if self.mode == "ext" else (self.numSplats,)
+
     def write_accum(self, accum: np.ndarray):
-        a = np.ascontiguousarray(accum, dtype=np.float32).reshape(self.numSplats)
+        a = np.ascontiguousarray(accum, dtype=np.float32).reshape(self._accum_shape())
         check(self._lib.nps_write_accum(self._h, fptr(a), self.numSplats), self._h, "write_accum")
 
     def read_accum(self) -> np.ndarray:
-        a = np.empty(self.numSplats, dtype=np.float32)
+        a = np.empty(self._accum_shape(), dtype=np.float32)
         check(self._lib.nps_read_accum(self._h, fptr(a), self.numSplats), self._h, "read_accum")
+        return a
+
+    # ---- EXT buffers ----
+    def write_sh(self, sh: np.ndarray):
+        a = np.ascontiguousarray(sh, dtype=np.float32).reshape(self.numSplats, 48)
+        check(self._lib.nps_write_sh(self._h, fptr(a), self.numSplats), self._h, "write_sh")
+
+    def read_sh(self) -> np.ndarray:
+        a = np.empty((self.numSplats, 48), dtype=np.float32)
+        check(self._lib.nps_read_sh(self._h, fptr(a), self.numSplats), self._h, "read_sh")
+        return a
+
+    def write_grads_f32(self, grads: np.ndarray):
+        a = np.ascontiguousarray(grads, dtype=np.float32).reshape(self.numSplats, 64)
+        check(self._lib.nps_write_grads_f32(self._h, fptr(a), self.numSplats), self._h, "write_grads_f32")
+
+    def read_grads_f32(self) -> np.ndarray:
+        a = np.empty((self.numSplats, 64), dtype=np.float32)
+        check(self._lib.nps_read_grads_f32(self._h, fptr(a), self.numSplats), self._h, "read_grads_f32")
         return a
 
     # ---- reference methods ----
@@ -168,6 +207,10 @@ class GpuSplatOptimizerManager:
     # ---- the three passes dispatch() records, callable one at a time ----
     def backward(self):
         check(self._lib.nps_backward(self._h), self._h, "backward")
+
+    def forward(self):
+        """EXT: render the current camera at the target's size (no gradients)."""
+        check(self._lib.nps_forward(self._h), self._h, "forward")
 
     def step(self):
         check(self._lib.nps_step(self._h), self._h, "step")
@@ -216,7 +259,7 @@ class GpuSplatOptimizerManager:
     def read_timing(self) -> dict:
         t = np.zeros(8, dtype=np.float32)
         check(self._lib.nps_read_timing(self._h, fptr(t)), self._h, "read_timing")
-        keys = ["preprocess_keys", "sort", "ranges", "composite", "step", "adc", "backward", "unused"]
+        keys = ["preprocess_keys", "sort", "ranges", "composite", "step", "adc", "backward", "chain"]
         return dict(zip(keys, (float(x) for x in t)))
 
 

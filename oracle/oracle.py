@@ -215,3 +215,16 @@ def ext_backward_f64(splats64, sh64, deg, vp64, cam64, tgt, W, H, tiles, lists, 
                              len(tiles), _p(lists, _U), _p(counts, _I), lists.shape[1], _p(g, _D), _p(gs, _D),
                              _p(c, _D), _p(loss, _D))
     return g, gs, c, loss
+
+
+def ext_adam_f32(splats, sh, grads64, m, v, deg, t):
+    """EXT Adam (FP32) on copies; returns (splats, sh, m, v, t)."""
+    lib = load()
+    lib.ora_ext_adam_f32.argtypes = [_F, _F, _F, _F, _F, i32, i32, _F]
+    lib.ora_ext_adam_f32.restype = i32
+    s, h = f32(splats).copy(), f32(sh).copy()
+    mm, vv = f32(m).copy(), f32(v).copy()
+    tt = np.array([t], np.float32)
+    lib.ora_ext_adam_f32(_p(s, _F), _p(h, _F), _p(f32(grads64), _F), _p(mm, _F), _p(vv, _F), s.shape[0], deg,
+                         _p(tt, _F))
+    return s, h, mm, vv, float(tt[0])

@@ -85,3 +85,42 @@ int ora_ext_backward_f64(const double* splats, const double* sh, int n, int deg,
                       grads, gsh, c_out, loss);
     return 0;
 }
+
+/* EXT Adam (FP32): 64 parameters per splat (16 of the splat record + 48 SH),
+ * learning rates per parameter group, bias correction with t = *adam_t + 1,
+ * eps 1e-15; parameters without a learning rate (pads, SH above the degree)
+ * are left alone.  grads are FP32 [n][64] (record then SH). */
+static float ext_lr(int p, int deg) {
+    if (p < 3) return 1.6e-4f;
+    if (p == 3) return 5e-2f;
+    if (p >= 4 && p < 7) return 5e-3f;
+    if (p >= 8 && p < 12) return 1e-3f;
+    if (p >= 16) {
+        int k = (p - 16) / 3;
+        if (k >= (deg + 1) * (deg + 1)) return 0.0f;
+        return k == 0 ? 2.5e-3f : 2.5e-3f / 20.0f;
+    }
+    return 0.0f;
+}
+
+int ora_ext_adam_f32(float* splats, float* sh, const float* grads, float* m, float* v, int n, int deg, float* adam_t) {
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-15f;
+    float t = *adam_t + 1.0f;
+    float bc1 = 1.0f - rm_powf(b1, t), bc2 = 1.0f - rm_powf(b2, t);
+    for (int i = 0; i < n; ++i) {
+        for (int p = 0; p < 64; ++p) {
+            float lr = ext_lr(p, deg);
+            if (lr == 0.0f) continue;
+            size_t k = 64 * (size_t)i + p;
+            float g = grads[k];
+            float mm = b1 * m[k] + (1.0f - b1) * g;
+            float vv = b2 * v[k] + ((1.0f - b2) * g) * g;
+            m[k] = mm; v[k] = vv;
+            float upd = (lr * (mm / bc1)) / (sqrtf(vv / bc2) + eps);
+            float* x = p < 16 ? splats + 16 * (size_t)i + p : sh + 48 * (size_t)i + (p - 16);
+            *x = *x - upd;
+        }
+    }
+    *adam_t = t;
+    return 0;
+}

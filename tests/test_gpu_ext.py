@@ -1,0 +1,201 @@
+"""EXT path on the GPU against the EXT oracle (FP32 mode).
+
+Bit-exact: tile lists (depth order, ids), per-tile counts, composited images,
+Adam.  Gradients: the GPU sums the FP32 per-pair terms with FP32 wave sums and
+FP64 across waves, the oracle in FP64 sequentially, so they agree to FP32
+rounding of the per-wave sums: |g_gpu - g_ora| <= 1e-4 * max |g_ora| over the
+parameter group (means, opacity, scales, quaternion, SH) plus 1e-5 |g_ora|.
+"""
+import numpy as np
+import pytest
+
+from nprush_amd import camera, scenes
+from nprush_amd.managers import GpuSplatOptimizerManager
+
+pytestmark = pytest.mark.gpu
+
+GROUPS = {"means": [0, 1, 2], "opacity": [3], "scales": [4, 5, 6], "quat": [8, 9, 10, 11]}
+
+
+def _target(oracle, seed, n, deg, vp, vi, W, H):
+    s, sh = scenes.make_ext_splats(n, seed, sh_degree=deg)
+    tiles, lists, counts = oracle.ext_bin(s, sh, deg, vp, vi, W, H)
+    zero = np.zeros((H, W, 4), np.float32)
+    _, _, c, _ = oracle.ext_backward_f32(s, sh, deg, vp, vi, zero, W, H, tiles, lists, counts, want_grads=False)
+    tgt = np.ones((H, W, 4), np.float32)
+    tgt[..., :3] = scenes.quantize8(c)
+    return tgt
+
+
+def _run(s, sh, deg, vp, vi, tgt, W, H):
+    mgr = GpuSplatOptimizerManager(numSplats=len(s), mode="ext", sh_degree=deg, splats=s, sh=sh,
+                                   adc_enabled=False)
+    mgr.writeSplatVPMatrix(vp, vi)
+    mgr.setBackwardTarget(tgt, np.ones((H, W), np.float32), W, H)
+    mgr.backward()
+    return mgr
+
+
+def _check_tiles(mgr, oracle, s, sh, deg, vp, vi, W, H):
+    ranges, ids, full = mgr.read_tiles()
+    tiles, lists, counts = oracle.ext_bin(s, sh, deg, vp, vi, W, H)
+    assert np.array_equal(full.astype(np.int64), counts.astype(np.int64))
+    for t in range(len(tiles)):
+        a, b = ranges[t]
+        assert b - a == counts[t]
+        assert np.array_equal(ids[a:b], lists[t, :counts[t]]), t
+    return tiles, lists, counts
+
+
+def _check_grads(g_gpu, g, gs, deg):
+    errs = {}
+    for name, cols in GROUPS.items():
+        ref = g[:, cols]
+        got = g_gpu[:, cols].astype(np.float64)
+        scale = np.abs(ref).max()
+        err = np.abs(got - ref)
+        assert np.all(err <= 1e-4 * scale + 1e-5 * np.abs(ref)), (name, err.max(), scale)
+        errs[name] = float(err.max() / max(scale, 1e-30))
+    nk = (deg + 1) ** 2
+    ref = gs[:, :3 * nk]
+    got = g_gpu[:, 16:16 + 3 * nk].astype(np.float64)
+    scale = np.abs(ref).max()
+    err = np.abs(got - ref)
+    assert np.all(err <= 1e-4 * scale + 1e-5 * np.abs(ref)), ("sh", err.max(), scale)
+    errs["sh"] = float(err.max() / max(scale, 1e-30))
+    # untouched entries stay zero
+    assert np.all(g_gpu[:, [7, 12, 13, 14, 15]] == 0)
+    assert np.all(g_gpu[:, 16 + 3 * nk:] == 0)
+    return errs
+
+
+@pytest.mark.parametrize("deg", [0, 3])
+def test_ext_config0_tiles_image_grads(oracle, deg):
+    """Config-0 scene (16,384 Gaussians, 256x256, camera at (0,0,3), target from
+    seed 1) in EXT mode."""
+    W = H = 256
+    n = 16384
+    vp, vi = scenes.config0_camera()
+    s, sh = scenes.make_ext_splats(n, 0, sh_degree=deg)
+    tgt = _target(oracle, 1, n, deg, vp, vi, W, H)
+    mgr = _run(s, sh, deg, vp, vi, tgt, W, H)
+    tiles, lists, counts = _check_tiles(mgr, oracle, s, sh, deg, vp, vi, W, H)
+    g, gs, c, loss = oracle.ext_backward_f32(s, sh, deg, vp, vi, tgt, W, H, tiles, lists, counts)
+    img, _ = mgr.read_image()
+    assert np.array_equal(img, c)
+    l2, l1 = mgr.read_loss()
+    assert abs(l1 - loss[0]) <= 1e-9 * loss[0] and abs(l2 - loss[1]) <= 1e-9 * loss[1]
+    errs = _check_grads(mgr.read_grads_f32(), g, gs, deg)
+    print("max relative gradient error per group:", errs)
+    st = mgr.read_stats()
+    assert st["keys"] == int(counts.sum())
+    mgr.close()
+
+
+def test_ext_ragged_offscreen_and_empty_tiles(oracle):
+    """200x120 target (ragged tiles), splats behind the camera, beyond the image
+    and with very low opacity, and tiles that no splat covers."""
+    W, H = 200, 120
+    deg = 2
+    n = 3000
+    rng = np.random.default_rng(11)
+    s, sh = scenes.make_ext_splats(n, 4, sh_degree=deg)
+    s[:, 0] = s[:, 0] * 0.5 - 0.4           # leave the right side of the image empty
+    s[:200, 2] = 5.0                          # behind the camera (w < 0.1)
+    s[200:300, 0] += 6.0                      # far off screen
+    s[300:400, 3] = -8.0                      # opacity below 1/255
+    vp, vi = camera.view_proj((0.0, 0.0, 3.0), aspect=W / H)
+    tgt = np.ones((H, W, 4), np.float32)
+    tgt[..., :3] = rng.random((H, W, 3)).astype(np.float32)
+    mgr = _run(s, sh, deg, vp, vi, tgt, W, H)
+    tiles, lists, counts = _check_tiles(mgr, oracle, s, sh, deg, vp, vi, W, H)
+    assert (counts == 0).any()
+    g, gs, c, loss = oracle.ext_backward_f32(s, sh, deg, vp, vi, tgt, W, H, tiles, lists, counts)
+    img, _ = mgr.read_image()
+    assert np.array_equal(img, c)
+    ntx = (W + 15) // 16
+    for t in np.nonzero(counts == 0)[0]:
+        tx, ty = t % ntx, t // ntx
+        assert np.all(img[ty * 16:(ty + 1) * 16, tx * 16:(tx + 1) * 16] == np.float32(0.05))
+    _check_grads(mgr.read_grads_f32(), g, gs, deg)
+    mgr.close()
+
+
+def test_ext_dense_early_termination(oracle):
+    """Opaque, dense splats: most pixels stop early (T < 1e-4)."""
+    W = H = 64
+    deg = 1
+    n = 4000
+    s, sh = scenes.make_ext_splats(n, 7, sh_degree=deg)
+    s[:, 0:3] *= 0.35
+    s[:, 3] = 4.0                             # opacity ~0.98
+    s[:, 4:7] = np.log(0.05)
+    vp, vi = camera.view_proj((0.2, 0.1, 2.5), aspect=1.0)
+    rng = np.random.default_rng(2)
+    tgt = np.ones((H, W, 4), np.float32)
+    tgt[..., :3] = rng.random((H, W, 3)).astype(np.float32)
+    mgr = _run(s, sh, deg, vp, vi, tgt, W, H)
+    tiles, lists, counts = _check_tiles(mgr, oracle, s, sh, deg, vp, vi, W, H)
+    g, gs, c, loss = oracle.ext_backward_f32(s, sh, deg, vp, vi, tgt, W, H, tiles, lists, counts)
+    img, _ = mgr.read_image()
+    assert np.array_equal(img, c)
+    st = mgr.read_stats()
+    assert st["pairs"] < int(counts.sum()) * 256 // 4   # termination cut most pairs
+    _check_grads(mgr.read_grads_f32(), g, gs, deg)
+    mgr.close()
+
+
+def test_ext_forward_only_and_accumulation(oracle):
+    W = H = 96
+    deg = 3
+    n = 2000
+    s, sh = scenes.make_ext_splats(n, 5, sh_degree=deg)
+    vp, vi = camera.view_proj((0.5, -0.2, 2.8), aspect=1.0)
+    tgt = np.ones((H, W, 4), np.float32)
+    tgt[..., :3] = 0.3
+    mgr = _run(s, sh, deg, vp, vi, tgt, W, H)
+    img_b, _ = mgr.read_image()
+    loss_b = mgr.read_loss()
+    g1 = mgr.read_grads_f32()
+    acc1 = mgr.read_accum()
+    mgr.forward()
+    img_f, _ = mgr.read_image()
+    assert np.array_equal(img_b, img_f)
+    assert mgr.read_loss() == loss_b
+    assert np.array_equal(mgr.read_grads_f32(), g1)     # forward adds nothing
+    mgr.backward()
+    g2 = mgr.read_grads_f32()
+    assert np.allclose(g2, 2 * g1, rtol=1e-5, atol=1e-6 * np.abs(g1).max())
+    acc2 = mgr.read_accum()
+    assert np.array_equal(acc2[:, 1], 2 * acc1[:, 1])
+    _, _, counts = oracle.ext_bin(s, sh, deg, vp, vi, W, H)
+    tiles, lists, counts = oracle.ext_bin(s, sh, deg, vp, vi, W, H)
+    on_screen = np.zeros(n, bool)
+    for t in range(len(tiles)):
+        on_screen[lists[t, :counts[t]]] = True
+    assert np.array_equal(acc1[:, 1] > 0, on_screen)
+    mgr.step()
+    assert np.all(mgr.read_grads_f32() == 0)
+    mgr.close()
+
+
+def test_ext_adam_bit_exact(oracle):
+    n = 1500
+    deg = 2
+    rng = np.random.default_rng(9)
+    s, sh = scenes.make_ext_splats(n, 2, sh_degree=deg)
+    g = (rng.standard_normal((n, 64)) * 10.0 ** rng.uniform(-6, 1, (n, 64))).astype(np.float32)
+    m = (rng.standard_normal((n, 64)) * 1e-3).astype(np.float32)
+    v = np.abs(rng.standard_normal((n, 64)) * 1e-4).astype(np.float32)
+    mgr = GpuSplatOptimizerManager(numSplats=n, mode="ext", sh_degree=deg, splats=s, sh=sh, adc_enabled=False)
+    mgr.write_adam(m, v, 7.0)
+    mgr.write_grads_f32(g)
+    mgr.step()
+    es, esh, em, ev, et = oracle.ext_adam_f32(s, sh, g, m, v, deg, 7.0)
+    gm, gv, gt = mgr.read_adam()
+    assert gt == et == 8.0
+    assert np.array_equal(mgr.read_splats(), es)
+    assert np.array_equal(mgr.read_sh(), esh)
+    assert np.array_equal(gm, em) and np.array_equal(gv, ev)
+    assert np.all(mgr.read_grads_f32() == 0)
+    mgr.close()
