@@ -577,7 +577,10 @@ __global__ __launch_bounds__(256) void k_adam_ext(float4* __restrict__ splats, f
     const int part = (int)(idx & 15);
     const float lr0 = ext_lr(4 * part, deg);
     const float lr3 = ext_lr(4 * part + 3, deg);
-    if (lr0 == 0.0f && lr3 == 0.0f && ext_lr(4 * part + 1, deg) == 0.0f && ext_lr(4 * part + 2, deg) == 0.0f) return;
+    if (lr0 == 0.0f && lr3 == 0.0f && ext_lr(4 * part + 1, deg) == 0.0f && ext_lr(4 * part + 2, deg) == 0.0f) {
+        grads[idx] = make_float4(0.f, 0.f, 0.f, 0.f);  // nothing to update; the step still clears
+        return;
+    }
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-15f;
     float4* xp = part < 4 ? &splats[4 * i + part] : &sh[12 * i + (part - 4)];
     float4 x4 = *xp, g4 = grads[idx], m4 = m[idx], v4 = v[idx];

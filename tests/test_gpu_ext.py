@@ -129,7 +129,7 @@ def test_ext_dense_early_termination(oracle):
     s, sh = scenes.make_ext_splats(n, 7, sh_degree=deg)
     s[:, 0:3] *= 0.35
     s[:, 3] = 4.0                             # opacity ~0.98
-    s[:, 4:7] = np.log(0.05)
+    s[:, 4:7] = np.log(0.2)
     vp, vi = camera.view_proj((0.2, 0.1, 2.5), aspect=1.0)
     rng = np.random.default_rng(2)
     tgt = np.ones((H, W, 4), np.float32)
@@ -140,7 +140,8 @@ def test_ext_dense_early_termination(oracle):
     img, _ = mgr.read_image()
     assert np.array_equal(img, c)
     st = mgr.read_stats()
-    assert st["pairs"] < int(counts.sum()) * 256 // 4   # termination cut most pairs
+    assert st["pairs"] == int(loss[2])                  # same stopping entry per pixel
+    assert loss[2] < 0.5 * int(counts.sum()) * 256       # termination cut most pairs
     _check_grads(mgr.read_grads_f32(), g, gs, deg)
     mgr.close()
 
