@@ -1,5 +1,16 @@
-"""Benchmark: fwd+bwd ms per view at 1M splats, 1920x1080 (REF semantics).
+"""Benchmark: fwd+bwd ms per view at 1M splats, 1920x1080.
 
+--mode ext (default): the scale-up configuration with SH degree 3.  One "step"
+= one view of the optimizer loop: backward (preprocess with SH colour, depth
+keys, sort, ranges, composite fwd+bwd, per-splat chain rule) + Adam.
+  1,000,000 Gaussians drawn with seed 0 (means U[-1,1]^3, log-scales
+  N(-4,0.5), uniform quaternions, opacity logits N(0,1), SH degree 3: DC from
+  RGB U[0,1], higher bands N(0,0.05)), 8 cameras on a radius-3 ring at
+  1920x1080 (16x16 tiles), targets = the EXT forward image of a seed-1 scene
+  drawn the same way, quantised to k/255.  Densification runs every 100
+  steps, so the timed steps (fewer than 100) contain none.
+
+--mode ref: the reference's own splat model.
 Workload (one "step" = one reference dispatch for one view, i.e.
 GpuSplatOptimizerManager.dispatch: backward (fwd+loss+bwd) + Adam step, ADC
 every 25th step):
@@ -37,6 +48,14 @@ sys.path.insert(0, ROOT)
 #      and depth position terms, quaternion partials, fixed-point conversion)
 FWD_FLOP_PER_PAIR = 34 + 4 * 4
 BWD_FLOP_PER_ACTIVE_PAIR = 240 + 3 * 4
+# EXT composite (exp weighted x4 as above):
+#   forward, every (pixel, entry) pair evaluated before the pixel stops:
+#     r^2 (10) + exp (1 + 4) + alpha and transmittance (4) + colour (6) = 25
+#   reverse, every pair the pixel composited: recomputed alpha and
+#     transmittance (19) + pair geometry (10) + colour/opacity/transmittance
+#     seeds (22) + screen-centre and conic gradients (12) + colour (4) = 67
+EXT_FWD_FLOP_PER_PAIR = 25
+EXT_BWD_FLOP_PER_ACTIVE_PAIR = 67
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
@@ -61,6 +80,52 @@ def build_workload(n_live, capacity, W, H, views, radius):
     tmgr.close()
     splats = scenes.with_dead_capacity(scenes.make_splats(n_live, 0), capacity)
     return cams, targets, ones, splats
+
+
+def build_workload_ext(n, W, H, views, radius, deg=3):
+    from nprush_amd import GpuSplatOptimizerManager, camera, scenes
+
+    cams = camera.ring_cameras(views, radius, W, H)
+    ts, tsh = scenes.make_ext_splats(n, 1, sh_degree=deg)
+    tmgr = GpuSplatOptimizerManager(numSplats=n, mode="ext", sh_degree=deg, splats=ts, sh=tsh, adc_enabled=False)
+    tmgr.setBackwardTarget(np.zeros((H, W, 4), np.float32), np.ones((H, W), np.float32), W, H)
+    targets = []
+    for vp, vi in cams:
+        tmgr.writeSplatVPMatrix(vp, vi)
+        tmgr.forward()
+        c, _ = tmgr.read_image()
+        rgba = np.ones((H, W, 4), np.float32)
+        rgba[..., :3] = scenes.quantize8(c)
+        targets.append(rgba)
+    tmgr.close()
+    s, sh = scenes.make_ext_splats(n, 0, sh_degree=deg)
+    return cams, targets, s, sh
+
+
+def cpu_baseline_ext(s, sh, deg, vp, vi, target, W, H, sample_tiles, seed=0):
+    """The EXT oracle (FP32, all host cores): binning of the sampled tiles
+    against all splats (depth sort per tile) + forward + L1 + backward + chain
+    rule, scaled to the whole view by the number of tiles."""
+    from oracle import oracle
+
+    oracle.load()
+    ntiles = ((W + 15) // 16) * ((H + 15) // 16)
+    rng = np.random.default_rng(seed)
+    tiles = np.sort(rng.choice(ntiles, size=min(sample_tiles, ntiles), replace=False)).astype(np.int32)
+    t0 = time.perf_counter()
+    _, lists, counts = oracle.ext_bin(s, sh, deg, vp, vi, W, H, stride=20000, tiles=tiles)
+    if int(counts.max()) > lists.shape[1]:
+        _, lists, counts = oracle.ext_bin(s, sh, deg, vp, vi, W, H, stride=int(counts.max()), tiles=tiles)
+    oracle.ext_backward_f32(s, sh, deg, vp, vi, target, W, H, tiles, lists, counts)
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(dt * 1000.0 * ntiles / len(tiles), 1),
+        "unit": "ms/view (fwd+bwd)",
+        "cores": int(oracle.load().ora_num_threads()),
+        "kind": "port",
+        "sample": f"{len(tiles)} random tiles of view 0 ({dt:.1f} s incl. a projection of all splats), "
+                  f"scaled by {ntiles}/{len(tiles)} tiles",
+    }
 
 
 def cpu_baseline(splats, vp, vi, target, depth, W, H, sample_tiles, seed=0):
@@ -91,6 +156,7 @@ def main():
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--small", action="store_true", help="16,384 splats at 256x256 (quick check)")
+    ap.add_argument("--mode", choices=("ext", "ref"), default="ext")
     ap.add_argument("--cpu-tiles", type=int, default=48, help="tiles in the CPU-baseline sample (0 = skip)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one plain streaming kernel (k_debug_math, 16M floats: 64 MiB read + 64 MiB "
@@ -100,6 +166,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is described in "
               "DESIGN.md and is not part of this build.", file=sys.stderr)
         sys.exit(2)
+
+    if args.mode == "ext":
+        return main_ext(args)
 
     from nprush_amd import GpuSplatOptimizerManager
 
@@ -201,6 +270,108 @@ def main():
     print(json.dumps(out), flush=True)
     for s in slots:
         mgr.free_target(s)
+    mgr.close()
+
+
+def main_ext(args):
+    from nprush_amd import GpuSplatOptimizerManager
+
+    deg = 3
+    if args.small:
+        n, W, H, views = 16384, 256, 256, 8
+    else:
+        n, W, H, views = 1_000_000, 1920, 1080, 8
+    print(f"[bench] building EXT workload: {n} Gaussians SH{deg}, {W}x{H}, {views} views", file=sys.stderr, flush=True)
+    cams, targets, s, sh = build_workload_ext(n, W, H, views, 3.0, deg)
+    depth = np.ones((H, W), np.float32)
+    mgr = GpuSplatOptimizerManager(numSplats=n, mode="ext", sh_degree=deg, splats=s, sh=sh, adc_period=100,
+                                   adc_enabled=False)
+    slots = [mgr.upload_target(t, depth) for t in targets]
+    mgr.enable_timing(True)
+
+    def one_step(k):
+        vp, vi = cams[k % views]
+        mgr.writeSplatVPMatrix(vp, vi)
+        mgr.set_target_slot(slots[k % views])
+        mgr.dispatch()
+        return mgr.read_timing(), mgr.read_stats()
+
+    for k in range(args.warmup):
+        one_step(k)
+    mgr.sync()
+    print(f"[bench] warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+    rows = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        rows.append(one_step(args.warmup + k))
+    mgr.sync()
+    wall = time.perf_counter() - t0
+    print(f"[bench] timed {args.steps} steps in {wall:.2f} s", file=sys.stderr, flush=True)
+
+    def mean(key, src=0):
+        return float(np.mean([r[src][key] for r in rows]))
+
+    bwd_ms, comp_ms, step_ms = mean("backward"), mean("composite"), mean("step")
+    pairs, active = mean("pairs", 1), mean("active_pairs", 1)
+    flops = pairs * EXT_FWD_FLOP_PER_PAIR + active * EXT_BWD_FLOP_PER_ACTIVE_PAIR
+    achieved = flops / (comp_ms * 1e-3) / 1e12
+    phases = {k: round(mean(k), 3) for k in ("preprocess_keys", "sort", "ranges", "composite", "chain", "step")}
+    keys = mean("keys", 1)
+    # sort: 8-byte keys read + written per pass; 2 passes over the tile keys
+    # (13 tile bits at 1080p) and 4 over the per-splat (depth, id) keys
+    tile_bits = int(np.ceil(np.log2(((W + 15) // 16) * ((H + 15) // 16))))
+    sort_bytes = 16.0 * keys * int(np.ceil(tile_bits / 8)) + 16.0 * n * 4
+    # Adam: per splat 64 parameters x (param + grad + m + v), read and written = 2 KiB
+    adam_gbs = 2048.0 * n / (step_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath) and not args.small:
+        with open(tpath) as fh:
+            tj = json.load(fh).get("k_composite_ext")
+        if tj:
+            traffic = int(tj["fetch_bytes"] + tj["write_bytes"])
+    if args.calibrate:
+        from nprush_amd import debug_math
+
+        debug_math(0, np.linspace(-8.0, 8.0, 1 << 24, dtype=np.float32))
+    cpu = None
+    if args.cpu_tiles > 0:
+        print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
+        vp, vi = cams[0]
+        cpu = cpu_baseline_ext(s, sh, deg, vp, vi, targets[0], W, H, args.cpu_tiles)
+    out = {
+        "metric": "fwd+bwd ms/view @1M splats 1080p",
+        "value": round(bwd_ms, 3),
+        "unit": "ms/view",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1000.0 / args.steps, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scenes, targets rendered from a seed-1 scene)",
+        "config": {"workload": f"EXT 3D Gaussian fwd+bwd+Adam, {n} Gaussians, SH degree {deg}, {W}x{H}, "
+                               f"{views} ring views, 16x16 tiles, depth-ordered uncapped lists, L1",
+                   "views": views, "width": W, "height": H, "splats": n, "sh_degree": deg},
+        "roofline": {"bound": "valu", "kernel": "k_composite_ext", "achieved": round(achieved, 3),
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": traffic},
+        "phases_ms": phases,
+        "tile_keys_per_view": int(keys),
+        "pairs_per_view": int(pairs),
+        "active_pairs_per_view": int(active),
+        "reverse_lane_efficiency": round(active / max(1.0, 64.0 * mean("reverse_wave_entries", 1)), 3),
+        "sort_hbm_gbs": round(sort_bytes / (mean("sort") * 1e-3) / 1e9, 1),
+        "adam_hbm_gbs": round(adam_gbs, 1),
+        "adam_roofline_frac": round(adam_gbs / PEAK_HBM_GBS, 3),
+        "steps_per_s": round(args.steps / wall, 2),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    for sl in slots:
+        mgr.free_target(sl)
     mgr.close()
 
 

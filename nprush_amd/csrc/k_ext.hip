@@ -4,10 +4,14 @@
 //   k_ext_preprocess : per splat, the projection of nps_ext.h, the 3-sigma tile
 //                      rectangle (binary search with the float overlap predicate
 //                      of the REF binning, splat_backward.wgsl:148-153), the
-//                      composite record and the depth key (float bits of w > 0);
-//   k_ext_keys       : (tile << 32 | depth bits, splat id) pairs, written in
-//                      splat order, so a stable LSD sort over bits [0, 32 +
-//                      tile bits) orders every tile by (depth, id);
+//                      composite record and the depth key (float bits of w > 0,
+//                      monotone in w) << 32 | id;
+//   depth order      : the per-splat keys sorted on their high 32 bits (stable,
+//                      so equal depths stay in id order) give every splat its
+//                      rank in (depth, id) order;
+//   k_ext_keys       : one key tile << 32 | id per (splat, covered tile),
+//                      written in rank order, so a stable LSD sort on the tile
+//                      bits alone leaves every tile list in (depth, id) order;
 //   k_composite_ext  : per 16x16 tile, front-to-back compositing with early
 //                      termination, image + L1/L2 loss, then the reverse pass;
 //   k_ext_chain      : per splat, the 2D gradients (screen centre, conic, colour,
@@ -27,7 +31,7 @@ __global__ __launch_bounds__(256) void k_ext_preprocess(const float4* __restrict
                                                         const float4* __restrict__ sh, uint32_t n, ViewParams v,
                                                         float cam0, float cam1, float cam2,
                                                         ExtRec* __restrict__ rec, uint32_t* __restrict__ counts,
-                                                        uint4* __restrict__ rects, uint32_t* __restrict__ dkey) {
+                                                        uint4* __restrict__ rects, uint64_t* __restrict__ dkey) {
     constexpr int kNk = (DEG + 1) * (DEG + 1);
     constexpr int kSh4 = (3 * kNk + 3) / 4;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -45,6 +49,7 @@ __global__ __launch_bounds__(256) void k_ext_preprocess(const float4* __restrict
         c[4 * q + 0] = t.x; c[4 * q + 1] = t.y; c[4 * q + 2] = t.z; c[4 * q + 3] = t.w;
     }
     counts[i] = 0u;
+    dkey[i] = 0xFFFFFFFF00000000ull | i;  // not drawn: after every drawn splat
     const float cam[3] = {cam0, cam1, cam2};
     ExtProj o;
     ext_project(s, c, DEG, v.vp, cam, v.W, v.H, o);
@@ -67,7 +72,7 @@ __global__ __launch_bounds__(256) void k_ext_preprocess(const float4* __restrict
     if (tx0 >= tx_end || ty0 >= ty_end) return;
     counts[i] = (tx_end - tx0) * (ty_end - ty0);
     rects[i] = make_uint4(tx0, tx_end, ty0, ty_end);
-    dkey[i] = __float_as_uint(o.w);
+    dkey[i] = ((uint64_t)__float_as_uint(o.w) << 32) | i;
     ExtRec f;
     f.scr_x = o.scr[0];
     f.scr_y = o.scr[1];
@@ -91,37 +96,44 @@ __global__ __launch_bounds__(256) void k_ext_preprocess(const float4* __restrict
     rec[i] = f;
 }
 
-// One key per (splat, covered tile); the wave walks its 64 splats in order and
-// writes each splat's keys with all lanes.
-__global__ __launch_bounds__(256) void k_ext_keys(uint32_t n, const uint32_t* __restrict__ counts,
-                                                  const uint4* __restrict__ rects, const uint32_t* __restrict__ dkey,
-                                                  uint32_t tiles_x, const uint64_t* __restrict__ offsets,
-                                                  uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+// Tile counts in rank order: rank_counts[r] = counts[id of rank r].
+__global__ __launch_bounds__(256) void k_ext_rank_counts(uint32_t n, const uint64_t* __restrict__ order,
+                                                         const uint32_t* __restrict__ counts,
+                                                         uint32_t* __restrict__ rank_counts) {
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) rank_counts[r] = counts[(uint32_t)order[r]];
+}
+
+// One key tile << 32 | id per (splat, covered tile), splats taken in rank
+// order; the wave walks its 64 ranks in order and writes each splat's keys
+// with all lanes.
+__global__ __launch_bounds__(256) void k_ext_keys(uint32_t n, const uint64_t* __restrict__ order,
+                                                  const uint32_t* __restrict__ rank_counts,
+                                                  const uint4* __restrict__ rects, uint32_t tiles_x,
+                                                  const uint64_t* __restrict__ offsets, uint64_t* __restrict__ keys) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wbase = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
     const uint32_t mine = wbase + lane;
-    const uint32_t my_cnt = mine < n ? counts[mine] : 0u;
+    const uint32_t my_cnt = mine < n ? rank_counts[mine] : 0u;
     uint4 my_r = make_uint4(0u, 0u, 0u, 0u);
     uint64_t my_off = 0ull;
-    uint32_t my_d = 0u;
+    uint32_t my_id = 0u;
     if (my_cnt) {
-        my_r = rects[mine];
+        my_id = (uint32_t)order[mine];
+        my_r = rects[my_id];
         my_off = offsets[mine];
-        my_d = dkey[mine];
     }
     for (uint32_t s = 0; s < 64u; ++s) {
         const uint32_t cnt = __shfl(my_cnt, (int)s, 64);
         if (cnt == 0u) continue;
-        const uint32_t i = wbase + s;
+        const uint64_t id = (uint64_t)__shfl(my_id, (int)s, 64);
         const uint32_t rx0 = __shfl(my_r.x, (int)s, 64), rx1 = __shfl(my_r.y, (int)s, 64);
         const uint32_t ry0 = __shfl(my_r.z, (int)s, 64);
         const uint32_t rw = rx1 - rx0;
         const uint64_t pos = __shfl(my_off, (int)s, 64);
-        const uint64_t d = (uint64_t)__shfl(my_d, (int)s, 64);
         for (uint32_t k = lane; k < cnt; k += 64u) {
             const uint32_t ty = ry0 + k / rw, tx = rx0 + k % rw;
-            keys[pos + k] = ((uint64_t)(ty * tiles_x + tx) << 32) | d;
-            vals[pos + k] = i;
+            keys[pos + k] = ((uint64_t)(ty * tiles_x + tx) << 32) | id;
         }
     }
 }
@@ -179,13 +191,13 @@ __device__ __forceinline__ float wave_reduce9(float (&v)[9], int lane, float& t8
     return x;
 }
 
-__device__ __forceinline__ void stage_ext(const ExtRec* __restrict__ rec, const uint32_t* __restrict__ ids,
+__device__ __forceinline__ void stage_ext(const ExtRec* __restrict__ rec, const uint64_t* __restrict__ keys,
                                           uint32_t base, int nb, ExtRec* s_rec, uint32_t* s_ids) {
     const int t = threadIdx.x;
     if (t < 3 * kBatch) {
         int j = t / 3, part = t - 3 * j;
         if (j < nb) {
-            uint32_t id = ids[base + j];
+            uint32_t id = (uint32_t)keys[base + j];
             reinterpret_cast<float4*>(&s_rec[j])[part] = reinterpret_cast<const float4*>(&rec[id])[part];
             if (part == 0) s_ids[j] = id;
         }
@@ -220,7 +232,7 @@ __global__ __launch_bounds__(256) void k_composite_ext(ViewParams v, ExtComposit
     for (int b0 = 0, bi = 0; b0 < len; b0 += kBatch, ++bi) {
         if (!__syncthreads_or(alive ? 1 : 0)) break;
         const int nb = min(kBatch, len - b0);
-        stage_ext(A.rec, A.ids, start + (uint32_t)b0, nb, s_rec, s_ids);
+        stage_ext(A.rec, A.keys, start + (uint32_t)b0, nb, s_rec, s_ids);
         __syncthreads();
         uint64_t mask = 0ull;
         float* ck = A.ckpt + (slot0 + bi) * (size_t)(8 * kTilePx);
@@ -302,7 +314,7 @@ __global__ __launch_bounds__(256) void k_composite_ext(ViewParams v, ExtComposit
             const int b0 = bi * kBatch;
             const int nb = min(kBatch, len - b0);
             __syncthreads();
-            stage_ext(A.rec, A.ids, start + (uint32_t)b0, nb, s_rec, s_ids);
+            stage_ext(A.rec, A.keys, start + (uint32_t)b0, nb, s_rec, s_ids);
             for (int e = tid; e < 4 * kBatch * 9; e += 256) (&s_part[0][0][0])[e] = 0.0f;
             __syncthreads();
             const uint64_t mask = A.act[(slot0 + bi) * 4 + wid];
@@ -607,7 +619,7 @@ __global__ __launch_bounds__(256) void k_adam_ext(float4* __restrict__ splats, f
 
 void launch_ext_preprocess(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg,
                            const ViewParams& v, const float cam[3], ExtRec* rec, uint32_t* counts, uint4* rects,
-                           uint32_t* dkey) {
+                           uint64_t* dkey) {
     if (n == 0) return;
     dim3 g((n + 255) / 256), b(256);
     const float4* s4 = (const float4*)splats;
@@ -620,11 +632,17 @@ void launch_ext_preprocess(hipStream_t st, const float* splats, const float* sh,
     }
 }
 
-void launch_ext_keys(hipStream_t st, uint32_t n, const uint32_t* counts, const uint4* rects, const uint32_t* dkey,
-                     uint32_t tiles_x, const uint64_t* offsets, uint64_t* keys, uint32_t* vals) {
+void launch_ext_rank_counts(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* counts,
+                            uint32_t* rank_counts) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_ext_keys, dim3((n + 255) / 256), dim3(256), 0, st, n, counts, rects, dkey, tiles_x, offsets,
-                       keys, vals);
+    hipLaunchKernelGGL(k_ext_rank_counts, dim3((n + 255) / 256), dim3(256), 0, st, n, order, counts, rank_counts);
+}
+
+void launch_ext_keys(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts,
+                     const uint4* rects, uint32_t tiles_x, const uint64_t* offsets, uint64_t* keys) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_ext_keys, dim3((n + 255) / 256), dim3(256), 0, st, n, order, rank_counts, rects, tiles_x,
+                       offsets, keys);
 }
 
 void launch_composite_ext(hipStream_t st, const ViewParams& v, const ExtCompositeArgs& a) {

@@ -34,7 +34,7 @@ struct DevBuf {
 struct Scratch {
     DevBuf counts, rects, offsets, keys, keys_alt, ranges, scan_tmp, sort_tmp;
     DevBuf diff, cut, full, kept;  // per-tile id-block cut of capped (REF) lists
-    DevBuf vals, vals_alt, dkey;   // EXT: splat ids sorted with the keys, depth keys
+    DevBuf dkey, dkey_alt, rank_counts;  // EXT: (depth bits << 32 | id) per splat, tile counts by rank
     uint64_t m = 0;
     bool in_alt = false;
     bool have_full = false;
@@ -130,7 +130,7 @@ void free_scratch(Scratch& s) {
     free_buf(s.counts); free_buf(s.rects); free_buf(s.offsets); free_buf(s.keys);
     free_buf(s.keys_alt); free_buf(s.ranges); free_buf(s.scan_tmp); free_buf(s.sort_tmp);
     free_buf(s.diff); free_buf(s.cut); free_buf(s.full); free_buf(s.kept);
-    free_buf(s.vals); free_buf(s.vals_alt); free_buf(s.dkey);
+    free_buf(s.dkey); free_buf(s.dkey_alt); free_buf(s.rank_counts);
 }
 
 // Ids per block of the per-tile cut: about 32 blocks, at least 4096 splats each.
@@ -193,7 +193,8 @@ int build_tile_lists(nps_ctx* h, Scratch& sc, const ViewParams& v, uint32_t n, i
     return NPS_OK;
 }
 
-// EXT view: preprocess -> keys -> sort -> ranges -> composite (-> chain rule).
+// EXT view: preprocess -> depth order of the splats -> keys in that order ->
+// stable sort on the tile bits -> ranges -> composite (-> chain rule).
 int run_ext(nps_ctx* h, bool grads) {
     NPS_HIP(h, hipSetDevice(h->device));
     const uint32_t n = h->n;
@@ -207,15 +208,24 @@ int run_ext(nps_ctx* h, bool grads) {
     if ((rc = ensure(h, h->fwd, sizeof(ExtRec) * n1)) != NPS_OK) return rc;
     if ((rc = ensure(h, sc.counts, 4 * n1)) != NPS_OK) return rc;
     if ((rc = ensure(h, sc.rects, 16 * n1)) != NPS_OK) return rc;
-    if ((rc = ensure(h, sc.dkey, 4 * n1)) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.dkey, 8 * n1)) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.dkey_alt, 8 * n1)) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.rank_counts, 4 * n1)) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.sort_tmp, radix_temp_bytes(n1))) != NPS_OK) return rc;
     if ((rc = ensure(h, sc.offsets, sizeof(uint64_t) * (n1 + 1))) != NPS_OK) return rc;
     if ((rc = ensure(h, sc.scan_tmp, scan_temp_bytes(n) + 64)) != NPS_OK) return rc;
     if ((rc = ensure(h, h->stats, 64)) != NPS_OK) return rc;
     if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[0], h->stream));
     launch_fill_u32(h->stream, (uint32_t*)h->stats.p, 0u, 16);
     launch_ext_preprocess(h->stream, h->splats, h->sh, n, h->deg, v, cam, (ExtRec*)h->fwd.p, (uint32_t*)sc.counts.p,
-                          (uint4*)sc.rects.p, (uint32_t*)sc.dkey.p);
-    exclusive_scan_u32_to_u64(h->stream, (const uint32_t*)sc.counts.p, (uint64_t*)sc.offsets.p, n, sc.scan_tmp.p);
+                          (uint4*)sc.rects.p, (uint64_t*)sc.dkey.p);
+    // rank of every splat in (depth, id) order: stable sort of depth << 32 | id on the depth bits
+    const bool dalt = radix_sort_u64(h->stream, (uint64_t*)sc.dkey.p, (uint64_t*)sc.dkey_alt.p, nullptr, nullptr, n,
+                                     32, 64, sc.sort_tmp.p);
+    const uint64_t* order = (const uint64_t*)(dalt ? sc.dkey_alt.p : sc.dkey.p);
+    launch_ext_rank_counts(h->stream, n, order, (const uint32_t*)sc.counts.p, (uint32_t*)sc.rank_counts.p);
+    exclusive_scan_u32_to_u64(h->stream, (const uint32_t*)sc.rank_counts.p, (uint64_t*)sc.offsets.p, n,
+                              sc.scan_tmp.p);
     uint64_t m = 0;
     NPS_HIP(h, hipMemcpyAsync(&m, (uint64_t*)sc.offsets.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost, h->stream));
     NPS_HIP(h, hipStreamSynchronize(h->stream));
@@ -225,9 +235,7 @@ int run_ext(nps_ctx* h, bool grads) {
     const size_t m1 = std::max<uint64_t>(m, 1);
     if ((rc = ensure(h, sc.keys, 8 * m1)) != NPS_OK) return rc;
     if ((rc = ensure(h, sc.keys_alt, 8 * m1)) != NPS_OK) return rc;
-    if ((rc = ensure(h, sc.vals, 4 * m1)) != NPS_OK) return rc;
-    if ((rc = ensure(h, sc.vals_alt, 4 * m1)) != NPS_OK) return rc;
-    if ((rc = ensure(h, sc.sort_tmp, radix_temp_bytes(m))) != NPS_OK) return rc;
+    if ((rc = ensure(h, sc.sort_tmp, radix_temp_bytes(std::max<uint64_t>(m, n1)))) != NPS_OK) return rc;
     if ((rc = ensure(h, sc.ranges, sizeof(uint32_t) * 2 * (size_t)std::max<uint32_t>(v.num_tiles, 1))) != NPS_OK)
         return rc;
     // batch slots of the composite: tile t uses [start/64 + t, ...), at most ceil(len/64) of them
@@ -240,13 +248,12 @@ int run_ext(nps_ctx* h, bool grads) {
     const size_t px = (size_t)h->W * h->H;
     if ((rc = ensure(h, h->c_img, 12 * px)) != NPS_OK) return rc;
     if ((rc = ensure(h, h->tile_loss, 16 * (size_t)std::max<uint32_t>(v.num_tiles, 1))) != NPS_OK) return rc;
-    launch_ext_keys(h->stream, n, (const uint32_t*)sc.counts.p, (const uint4*)sc.rects.p,
-                    (const uint32_t*)sc.dkey.p, v.tiles_x, (const uint64_t*)sc.offsets.p, (uint64_t*)sc.keys.p,
-                    (uint32_t*)sc.vals.p);
+    launch_ext_keys(h->stream, n, order, (const uint32_t*)sc.rank_counts.p, (const uint4*)sc.rects.p, v.tiles_x,
+                    (const uint64_t*)sc.offsets.p, (uint64_t*)sc.keys.p);
     if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[1], h->stream));
     const int tile_bits = bits_for(v.num_tiles > 0 ? v.num_tiles - 1 : 0);
-    sc.in_alt = radix_sort_u64(h->stream, (uint64_t*)sc.keys.p, (uint64_t*)sc.keys_alt.p, (uint32_t*)sc.vals.p,
-                               (uint32_t*)sc.vals_alt.p, m, 0, 32 + tile_bits, sc.sort_tmp.p);
+    sc.in_alt = radix_sort_u64(h->stream, (uint64_t*)sc.keys.p, (uint64_t*)sc.keys_alt.p, nullptr, nullptr, m, 32,
+                               32 + tile_bits, sc.sort_tmp.p);
     if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[2], h->stream));
     launch_fill_u32(h->stream, (uint32_t*)sc.ranges.p, 0u, 2 * (uint64_t)v.num_tiles);
     launch_tile_ranges(h->stream, (const uint64_t*)(sc.in_alt ? sc.keys_alt.p : sc.keys.p), m,
@@ -255,7 +262,7 @@ int run_ext(nps_ctx* h, bool grads) {
     if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[3], h->stream));
     ExtCompositeArgs a;
     a.rec = (const ExtRec*)h->fwd.p;
-    a.ids = (const uint32_t*)(sc.in_alt ? sc.vals_alt.p : sc.vals.p);
+    a.keys = (const uint64_t*)(sc.in_alt ? sc.keys_alt.p : sc.keys.p);
     a.ranges = (const uint32_t*)sc.ranges.p;
     a.tgt_rgba = (const float4*)h->tgt_rgba.p;
     a.g2 = grads ? (double*)h->g2.p : nullptr;
@@ -710,13 +717,11 @@ int nps_read_tiles(nps_handle h, uint32_t* ranges, uint32_t* ids, uint32_t* full
         if (ranges) memcpy(ranges, r.data(), 8 * (size_t)h->last_tiles);
         if (full_counts)
             for (size_t t = 0; t < h->last_tiles; ++t) full_counts[t] = r[2 * t + 1] - r[2 * t];
-        if (ids && h->sc.m)
-            NPS_COPY(h, ids, h->sc.in_alt ? h->sc.vals_alt.p : h->sc.vals.p, 4 * (size_t)h->sc.m,
-                     hipMemcpyDeviceToHost);
-        return NPS_OK;
+    } else {
+        if (ranges) NPS_COPY(h, ranges, h->sc.ranges.p, 8 * (size_t)h->last_tiles, hipMemcpyDeviceToHost);
+        if (full_counts)
+            NPS_COPY(h, full_counts, h->sc.full.p, 4 * (size_t)h->last_tiles, hipMemcpyDeviceToHost);
     }
-    if (ranges) NPS_COPY(h, ranges, h->sc.ranges.p, 8 * (size_t)h->last_tiles, hipMemcpyDeviceToHost);
-    if (full_counts) NPS_COPY(h, full_counts, h->sc.full.p, 4 * (size_t)h->last_tiles, hipMemcpyDeviceToHost);
     if (ids && h->sc.m) {
         std::vector<uint64_t> k(h->sc.m);
         NPS_COPY(h, k.data(), h->sc.in_alt ? h->sc.keys_alt.p : h->sc.keys.p, 8 * (size_t)h->sc.m,

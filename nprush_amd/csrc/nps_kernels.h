@@ -92,7 +92,7 @@ void launch_render(hipStream_t st, const ViewParams& v, const RenderArgs& a);
 // k_ext.hip
 struct ExtCompositeArgs {
     const ExtRec* rec;
-    const uint32_t* ids;        // splat ids in (tile, depth, id) order
+    const uint64_t* keys;       // tile << 32 | splat id, in (tile, depth, id) order
     const uint32_t* ranges;     // [2 * num_tiles]
     const float4* tgt_rgba;     // H*W
     double* g2;                 // 9 per splat: d scr.xy, d conic A,2B,C, d rgb, d opacity
@@ -106,9 +106,11 @@ struct ExtCompositeArgs {
 };
 void launch_ext_preprocess(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg,
                            const ViewParams& v, const float cam[3], ExtRec* rec, uint32_t* counts, uint4* rects,
-                           uint32_t* dkey);
-void launch_ext_keys(hipStream_t st, uint32_t n, const uint32_t* counts, const uint4* rects, const uint32_t* dkey,
-                     uint32_t tiles_x, const uint64_t* offsets, uint64_t* keys, uint32_t* vals);
+                           uint64_t* dkey);
+void launch_ext_rank_counts(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* counts,
+                            uint32_t* rank_counts);
+void launch_ext_keys(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts,
+                     const uint4* rects, uint32_t tiles_x, const uint64_t* offsets, uint64_t* keys);
 void launch_composite_ext(hipStream_t st, const ViewParams& v, const ExtCompositeArgs& a);
 void launch_ext_chain(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg, const ViewParams& v,
                       const float cam[3], const double* g2, const uint32_t* counts, float* grads, float* accum);
