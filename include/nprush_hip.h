@@ -159,6 +159,11 @@ int nps_read_tiles(nps_handle h, uint32_t* ranges, uint32_t* ids, uint32_t* full
  * [7]=ADC clones, [8]=ADC cascade rounds, [9]=wave-entries walked by the
  * reverse pass, [10]=forward wave-entries that evaluate pow/exp, [11]=0. */
 int nps_read_stats(nps_handle h, int64_t stats[12]);
+/* EXT mode: per-tile work of the last nps_backward/nps_forward, 2 per tile:
+ * entries of the tile list the forward staged before every pixel had stopped,
+ * and the reverse pass's extent (last entry any pixel composited + 1).  No
+ * reference counterpart (the cost model of the composite's work order). */
+int nps_read_tile_work(nps_handle h, uint32_t* work2t);
 /* Per-phase GPU time (ms, HIP events on the handle's stream) of the last
  * backward/step/adc when timing is enabled: [0]=preprocess+keys, [1]=sort,
  * [2]=ranges, [3]=composite fwd+bwd, [4]=step, [5]=adc, [6]=whole backward,
@@ -185,6 +190,28 @@ int npf_set_target(npf_handle f, int32_t w, int32_t hgt);
 int npf_dispatch(npf_handle f, int32_t clear, int32_t quantize);
 int npf_read_target(npf_handle f, float* color, float* depth);
 int npf_write_target(npf_handle f, const float* color, const float* depth);
+
+/*
+ * new GpuBlurPipelineManager(device) (src/lib/gpu/GpuBlurPipelineManager.ts:9-49).
+ * blur(encoder, srcView, dstView, tempView, w, h, radius, sigma, isSrgb)
+ * (GpuBlurPipelineManager.ts:51-99, blur.wgsl:13-48): separable Gaussian over
+ * i in [-radius, radius], weights exp(-i^2 / (2 sigma^2)), clamp to edge.  The
+ * horizontal pass decodes sRGB when is_srgb and writes an 8-bit linear temp,
+ * the vertical pass encodes sRGB when is_srgb and writes the 8-bit output
+ * (alpha 1).  Images are H*W*4 floats holding rgba8unorm texel values (k/255).
+ * The caller of the reference uses radius 16, sigma 8 (ViewerState.svelte.ts:28,
+ * GpuRunner.svelte.ts:644-663): sRGB for colour, linear for depth.
+ */
+typedef struct npb_ctx* npb_handle;
+int npb_create(int32_t device, npb_handle* out);
+void npb_destroy(npb_handle b);
+const char* npb_last_error(npb_handle b);
+/* host pointers */
+int npb_blur(npb_handle b, const float* src_rgba, float* dst_rgba, int32_t w, int32_t hgt, int32_t radius,
+             float sigma, int32_t is_srgb);
+/* device pointers on the handle's GPU (the output can feed nps_set_target_dev) */
+int npb_blur_dev(npb_handle b, const float* src_dev, float* dst_dev, int32_t w, int32_t hgt, int32_t radius,
+                 float sigma, int32_t is_srgb);
 
 /* Device buffers on the handle's GPU for callers without a GPU allocator of
  * their own (Deno, ctypes), e.g. to keep several targets resident in HBM for

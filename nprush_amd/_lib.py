@@ -69,6 +69,7 @@ SIGNATURES = {
     "nps_read_loss": (i32, [_P, _D]),
     "nps_read_tiles": (i32, [_P, _U, _U, _U, _I, _L]),
     "nps_read_stats": (i32, [_P, _L]),
+    "nps_read_tile_work": (i32, [_P, _U]),
     "nps_enable_timing": (i32, [_P, i32]),
     "nps_read_timing": (i32, [_P, _F]),
     "npf_create": (i32, [_P, C.POINTER(_P)]),
@@ -78,6 +79,11 @@ SIGNATURES = {
     "npf_dispatch": (i32, [_P, i32, i32]),
     "npf_read_target": (i32, [_P, _F, _F]),
     "npf_write_target": (i32, [_P, _F, _F]),
+    "npb_create": (i32, [i32, C.POINTER(_P)]),
+    "npb_destroy": (None, [_P]),
+    "npb_last_error": (C.c_char_p, [_P]),
+    "npb_blur": (i32, [_P, _F, _F, i32, i32, i32, C.c_float, i32]),
+    "npb_blur_dev": (i32, [_P, _P, _P, i32, i32, i32, C.c_float, i32]),
     "nps_dev_alloc": (i32, [_P, C.c_int64, C.POINTER(_P)]),
     "nps_dev_free": (i32, [_P, _P]),
     "nps_dev_upload": (i32, [_P, _P, _P, C.c_int64]),

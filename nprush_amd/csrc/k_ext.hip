@@ -191,15 +191,68 @@ __device__ __forceinline__ float wave_reduce9(float (&v)[9], int lane, float& t8
     return x;
 }
 
-__device__ __forceinline__ void stage_ext(const ExtRec* __restrict__ rec, const uint64_t* __restrict__ keys,
-                                          uint32_t base, int nb, ExtRec* s_rec, uint32_t* s_ids) {
-    const int t = threadIdx.x;
-    if (t < 3 * kBatch) {
-        int j = t / 3, part = t - 3 * j;
-        if (j < nb) {
-            uint32_t id = (uint32_t)keys[base + j];
-            reinterpret_cast<float4*>(&s_rec[j])[part] = reinterpret_cast<const float4*>(&rec[id])[part];
-            if (part == 0) s_ids[j] = id;
+// Workgroup order of the composite.  Super-tiles of 2x2 tiles are ranked by
+// their total list length (longest first, ties by index) with a bitonic sort in
+// LDS and dealt round-robin to the 8 XCDs (workgroup b runs on XCD b % 8); the
+// 4 tiles of a super-tile go to the same XCD, where they share splat records in
+// its L2.  order[8 * (4 q + j) + x] = tile j of the super-tile of rank 8 q + x,
+// or ~0 (no tile).  Above kOrderMax super-tiles the ranking is skipped.
+constexpr uint32_t kOrderMax = 8192;
+
+__global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict__ ranges, uint32_t tiles_x,
+                                                     uint32_t tiles_y, uint32_t* __restrict__ order) {
+    __shared__ uint64_t s_key[kOrderMax];
+    const uint32_t sx = (tiles_x + 1u) / 2u, sy = (tiles_y + 1u) / 2u, S = sx * sy;
+    const uint32_t tid = threadIdx.x;
+    const bool rank = S <= kOrderMax;
+    uint32_t P = 1u;
+    while (P < S) P <<= 1;
+    if (rank) {
+        for (uint32_t sidx = tid; sidx < P; sidx += blockDim.x) {
+            uint64_t key = ~0ull;
+            if (sidx < S) {
+                const uint32_t stx = sidx % sx, sty = sidx / sx;
+                uint32_t cost = 0u;
+                for (uint32_t j = 0; j < 4u; ++j) {
+                    const uint32_t tx = 2u * stx + (j & 1u), ty = 2u * sty + (j >> 1);
+                    if (tx < tiles_x && ty < tiles_y) {
+                        const uint32_t t = ty * tiles_x + tx;
+                        cost += ranges[2 * t + 1] - ranges[2 * t];
+                    }
+                }
+                key = ((uint64_t)(~cost) << 32) | sidx;
+            }
+            s_key[sidx] = key;
+        }
+        __syncthreads();
+        for (uint32_t k = 2u; k <= P; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0u; j >>= 1) {
+                for (uint32_t i = tid; i < P; i += blockDim.x) {
+                    const uint32_t ixj = i ^ j;
+                    if (ixj > i) {
+                        const uint64_t a = s_key[i], b = s_key[ixj];
+                        const bool up = (i & k) == 0u;
+                        if ((a > b) == up) {
+                            s_key[i] = b;
+                            s_key[ixj] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    const uint32_t R = (S + 7u) / 8u * 8u;
+    for (uint32_t r = tid; r < R; r += blockDim.x) {
+        const uint32_t q = r / 8u, x = r % 8u;
+        const uint32_t sidx = r < S ? (rank ? (uint32_t)s_key[r] : r) : ~0u;
+        for (uint32_t j = 0; j < 4u; ++j) {
+            uint32_t t = ~0u;
+            if (sidx != ~0u) {
+                const uint32_t tx = 2u * (sidx % sx) + (j & 1u), ty = 2u * (sidx / sx) + (j >> 1);
+                if (tx < tiles_x && ty < tiles_y) t = ty * tiles_x + tx;
+            }
+            order[8u * (4u * q + j) + x] = t;
         }
     }
 }
@@ -211,7 +264,8 @@ __global__ __launch_bounds__(256) void k_composite_ext(ViewParams v, ExtComposit
     __shared__ double s_red[2][4];
     __shared__ int s_max[4];
 
-    const uint32_t tile = xcd_remap(blockIdx.x, v.num_tiles);
+    const uint32_t tile = A.order[blockIdx.x];
+    if (tile >= v.num_tiles) return;  // padding slot of the order (uniform per workgroup)
     const uint32_t tx = tile % v.tiles_x, ty = tile / v.tiles_x;
     const uint32_t start = A.ranges[2 * tile], end = A.ranges[2 * tile + 1];
     const int len = (int)(end - start);
@@ -224,47 +278,91 @@ __global__ __launch_bounds__(256) void k_composite_ext(ViewParams v, ExtComposit
     // this tile's batch slots: [start/64 + tile, ...) never overlap another tile's
     const size_t slot0 = (size_t)(start / kBatch) + tile;
 
+    // Batch staging: thread t < 192 carries float4 (t % 3) of record t / 3.  The
+    // records of the next batch are loaded into registers while the current
+    // batch is composited, and the id of the batch after that as well.
+    const int sj = tid / 3, spart = tid - 3 * sj;
+    const bool stager = tid < 3 * kBatch;
+    auto load_id = [&](int b0) -> uint32_t {
+        return (stager && b0 >= 0 && b0 + sj < len) ? (uint32_t)A.keys[start + (uint32_t)(b0 + sj)] : 0u;
+    };
+    auto load_rec = [&](uint32_t id, int b0) -> float4 {
+        return (stager && b0 >= 0 && b0 + sj < len) ? reinterpret_cast<const float4*>(&A.rec[id])[spart]
+                                                    : make_float4(0.f, 0.f, 0.f, 0.f);
+    };
+    auto put = [&](uint32_t id, float4 r, int nb) {
+        if (stager && sj < nb) {
+            reinterpret_cast<float4*>(&s_rec[sj])[spart] = r;
+            if (spart == 0) s_ids[sj] = id;
+        }
+    };
+
     // ---------------- forward ----------------
     float T = 1.0f, Cr = 0.0f, Cg = 0.0f, Cb = 0.0f;
     int last = 0;
     bool alive = valid;
     uint32_t n_eval = 0u, n_fw = 0u;
+    int fw_extent = 0;  // entries staged by the forward (uniform)
+    uint32_t id_cur = load_id(0);
+    float4 rec_cur = load_rec(id_cur, 0);
+    uint32_t id_next = load_id(kBatch);
     for (int b0 = 0, bi = 0; b0 < len; b0 += kBatch, ++bi) {
         if (!__syncthreads_or(alive ? 1 : 0)) break;
         const int nb = min(kBatch, len - b0);
-        stage_ext(A.rec, A.keys, start + (uint32_t)b0, nb, s_rec, s_ids);
+        fw_extent = b0 + nb;
+        put(id_cur, rec_cur, nb);
         __syncthreads();
+        id_cur = id_next;
+        rec_cur = load_rec(id_cur, b0 + kBatch);
+        id_next = load_id(b0 + 2 * kBatch);
         uint64_t mask = 0ull;
         float* ck = A.ckpt + (slot0 + bi) * (size_t)(8 * kTilePx);
         if (__ballot(alive) != 0ull) {
-            for (int j = 0; j < nb; ++j) {
-                if (A.backward && (j & 7) == 0) ck[(j >> 3) * kTilePx + tid] = T;
-                bool inc = false;
-                if (alive) {
-                    const ExtRec& f = s_rec[j];
-                    ExtPair q = ext_pair(f, pxf, pyf);
-                    ++n_eval;
-                    if (!(q.r2 > f.rcut2)) {
-                        float G = exp_det(-0.5f * q.r2);
-                        float a = f.op * G;
-                        if (a > kExtMaxAlpha) a = kExtMaxAlpha;
-                        if (!(a < kExtMinAlpha)) {
-                            float tT = T * (1.0f - a);
-                            if (tT < kExtTMin) {
-                                alive = false;
-                            } else {
-                                Cr = Cr + (T * a) * f.r;
-                                Cg = Cg + (T * a) * f.g;
-                                Cb = Cb + (T * a) * f.b;
-                                T = tT;
-                                last = b0 + j + 1;
-                                inc = true;
-                            }
+            for (int c0 = 0; c0 < nb; c0 += 8) {
+                if (A.backward) ck[(c0 >> 3) * kTilePx + tid] = T;
+                // alphas of the chunk's 8 entries are independent: computed first
+                float ca[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    ca[j] = 0.0f;
+                    if (c0 + j < nb) {
+                        const ExtRec& f = s_rec[c0 + j];
+                        ExtPair q = ext_pair(f, pxf, pyf);
+                        if (!(q.r2 > f.rcut2)) {
+                            float G = exp2_det_in_range((-0.5f * q.r2) * kLog2e);  // r2 <= rcut2 < 12
+                            float a = f.op * G;
+                            if (a > kExtMaxAlpha) a = kExtMaxAlpha;
+                            ca[j] = a;
                         }
                     }
                 }
-                ++n_fw;
-                if (__ballot(inc) != 0ull) mask |= 1ull << j;
+                // then the front-to-back chain, entry by entry
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    if (c0 + j < nb) {
+                        bool inc = false;
+                        if (alive) {
+                            ++n_eval;
+                            const float a = ca[j];
+                            if (!(a < kExtMinAlpha)) {
+                                float tT = T * (1.0f - a);
+                                if (tT < kExtTMin) {
+                                    alive = false;
+                                } else {
+                                    const ExtRec& f = s_rec[c0 + j];
+                                    Cr = Cr + (T * a) * f.r;
+                                    Cg = Cg + (T * a) * f.g;
+                                    Cb = Cb + (T * a) * f.b;
+                                    T = tT;
+                                    last = b0 + c0 + j + 1;
+                                    inc = true;
+                                }
+                            }
+                        }
+                        ++n_fw;
+                        if (__ballot(inc) != 0ull) mask |= 1ull << (c0 + j);
+                    }
+                }
                 if (__ballot(alive) == 0ull) break;
             }
         }
@@ -310,13 +408,19 @@ __global__ __launch_bounds__(256) void k_composite_ext(ViewParams v, ExtComposit
     if (A.backward) {
         float dT = (dCr * kExtBg + dCg * kExtBg) + dCb * kExtBg;
         const int nbr = (maxlast + kBatch - 1) / kBatch;
+        id_cur = load_id((nbr - 1) * kBatch);
+        rec_cur = load_rec(id_cur, (nbr - 1) * kBatch);
+        id_next = load_id((nbr - 2) * kBatch);
         for (int bi = nbr - 1; bi >= 0; --bi) {
             const int b0 = bi * kBatch;
             const int nb = min(kBatch, len - b0);
             __syncthreads();
-            stage_ext(A.rec, A.keys, start + (uint32_t)b0, nb, s_rec, s_ids);
+            put(id_cur, rec_cur, nb);
             for (int e = tid; e < 4 * kBatch * 9; e += 256) (&s_part[0][0][0])[e] = 0.0f;
             __syncthreads();
+            id_cur = id_next;
+            rec_cur = load_rec(id_cur, b0 - kBatch);
+            id_next = load_id(b0 - 2 * kBatch);
             const uint64_t mask = A.act[(slot0 + bi) * 4 + wid];
             const float* ck = A.ckpt + (slot0 + bi) * (size_t)(8 * kTilePx);
             for (int c = 7; c >= 0; --c) {
@@ -337,7 +441,7 @@ __global__ __launch_bounds__(256) void k_composite_ext(ViewParams v, ExtComposit
                             const ExtRec& f = s_rec[8 * c + j];
                             ExtPair q = ext_pair(f, pxf, pyf);
                             if (!(q.r2 > f.rcut2)) {
-                                float G = exp_det(-0.5f * q.r2);
+                                float G = exp2_det_in_range((-0.5f * q.r2) * kLog2e);  // r2 <= rcut2 < 12
                                 float a = f.op * G;
                                 if (a > kExtMaxAlpha) a = kExtMaxAlpha;
                                 if (!(a < kExtMinAlpha)) {
@@ -414,6 +518,8 @@ __global__ __launch_bounds__(256) void k_composite_ext(ViewParams v, ExtComposit
         double a1 = (s_red[1][0] + s_red[1][1]) + (s_red[1][2] + s_red[1][3]);
         A.tile_loss[2 * tile] = a2;
         A.tile_loss[2 * tile + 1] = a1;
+        A.tile_work[2 * tile] = (uint32_t)fw_extent;
+        A.tile_work[2 * tile + 1] = (uint32_t)maxlast;
     }
 }
 
@@ -645,9 +751,19 @@ void launch_ext_keys(hipStream_t st, uint32_t n, const uint64_t* order, const ui
                        offsets, keys);
 }
 
+uint32_t ext_order_slots(uint32_t tiles_x, uint32_t tiles_y) {
+    const uint32_t S = ((tiles_x + 1u) / 2u) * ((tiles_y + 1u) / 2u);
+    return 4u * ((S + 7u) / 8u * 8u);
+}
+
+void launch_tile_order(hipStream_t st, const uint32_t* ranges, uint32_t tiles_x, uint32_t tiles_y, uint32_t* order) {
+    if (tiles_x == 0 || tiles_y == 0) return;
+    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, st, ranges, tiles_x, tiles_y, order);
+}
+
 void launch_composite_ext(hipStream_t st, const ViewParams& v, const ExtCompositeArgs& a) {
     if (v.num_tiles == 0) return;
-    hipLaunchKernelGGL(k_composite_ext, dim3(v.num_tiles), dim3(256), 0, st, v, a);
+    hipLaunchKernelGGL(k_composite_ext, dim3(ext_order_slots(v.tiles_x, v.tiles_y)), dim3(256), 0, st, v, a);
 }
 
 void launch_ext_chain(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg, const ViewParams& v,

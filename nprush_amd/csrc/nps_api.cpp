@@ -66,7 +66,7 @@ struct nps_ctx {
     bool has_target = false;
     DevBuf tgt_rgba, tgt_depth;
     DevBuf fwd, bwd, ckpt, c_img, d_img, tile_loss, stats;
-    DevBuf act, g2;  // EXT: per-batch activity masks, per-splat 2D gradients (FP64)
+    DevBuf act, g2, order, tile_work;  // EXT: activity masks, 2D gradients (FP64), tile order, per-tile work
     Scratch sc;
     uint32_t last_tiles = 0;
     bool have_backward = false;
@@ -86,6 +86,13 @@ struct npf_ctx {
     uint32_t W = 0, H = 0;
     DevBuf color, depth, rec;
     Scratch sc;
+};
+
+struct npb_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevBuf src, dst, temp;
+    std::string err;
 };
 
 namespace {
@@ -248,6 +255,8 @@ int run_ext(nps_ctx* h, bool grads) {
     const size_t px = (size_t)h->W * h->H;
     if ((rc = ensure(h, h->c_img, 12 * px)) != NPS_OK) return rc;
     if ((rc = ensure(h, h->tile_loss, 16 * (size_t)std::max<uint32_t>(v.num_tiles, 1))) != NPS_OK) return rc;
+    if ((rc = ensure(h, h->order, 4 * (size_t)ext_order_slots(v.tiles_x, v.tiles_y))) != NPS_OK) return rc;
+    if ((rc = ensure(h, h->tile_work, 8 * (size_t)std::max<uint32_t>(v.num_tiles, 1))) != NPS_OK) return rc;
     launch_ext_keys(h->stream, n, order, (const uint32_t*)sc.rank_counts.p, (const uint4*)sc.rects.p, v.tiles_x,
                     (const uint64_t*)sc.offsets.p, (uint64_t*)sc.keys.p);
     if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[1], h->stream));
@@ -259,6 +268,7 @@ int run_ext(nps_ctx* h, bool grads) {
     launch_tile_ranges(h->stream, (const uint64_t*)(sc.in_alt ? sc.keys_alt.p : sc.keys.p), m,
                        (uint32_t*)sc.ranges.p);
     if (grads) launch_fill_u32(h->stream, (uint32_t*)h->g2.p, 0u, 18 * (uint64_t)n);
+    launch_tile_order(h->stream, (const uint32_t*)sc.ranges.p, v.tiles_x, v.tiles_y, (uint32_t*)h->order.p);
     if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[3], h->stream));
     ExtCompositeArgs a;
     a.rec = (const ExtRec*)h->fwd.p;
@@ -271,6 +281,8 @@ int run_ext(nps_ctx* h, bool grads) {
     a.ckpt = grads ? (float*)h->ckpt.p : nullptr;
     a.act = grads ? (uint64_t*)h->act.p : nullptr;
     a.stats = (unsigned long long*)h->stats.p;
+    a.order = (const uint32_t*)h->order.p;
+    a.tile_work = (uint32_t*)h->tile_work.p;
     a.backward = grads ? 1 : 0;
     launch_composite_ext(h->stream, v, a);
     if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[4], h->stream));
@@ -377,7 +389,7 @@ void nps_destroy(nps_handle h) {
     if (h->grads) (void)hipFree(h->grads);
     if (h->sh) (void)hipFree(h->sh);
     if (h->gradx) (void)hipFree(h->gradx);
-    free_buf(h->act); free_buf(h->g2);
+    free_buf(h->act); free_buf(h->g2); free_buf(h->order); free_buf(h->tile_work);
     free_buf(h->tgt_rgba); free_buf(h->tgt_depth); free_buf(h->fwd); free_buf(h->bwd); free_buf(h->ckpt);
     free_buf(h->c_img); free_buf(h->d_img); free_buf(h->tile_loss); free_buf(h->stats);
     free_buf(h->adc_u32); free_buf(h->adc_tmp); free_buf(h->adc_scan);
@@ -731,6 +743,13 @@ int nps_read_tiles(nps_handle h, uint32_t* ranges, uint32_t* ids, uint32_t* full
     return NPS_OK;
 }
 
+int nps_read_tile_work(nps_handle h, uint32_t* work) {
+    if (!h || !work) return NPS_ERR_ARG;
+    if (!h->ext || !h->have_backward) return set_err(h, NPS_ERR_STATE, "no EXT backward/forward has run");
+    NPS_COPY(h, work, h->tile_work.p, 8 * (size_t)h->last_tiles, hipMemcpyDeviceToHost);
+    return NPS_OK;
+}
+
 int nps_read_stats(nps_handle h, int64_t stats[12]) {
     if (!h || !stats) return NPS_ERR_ARG;
     if (h->have_backward && h->stats.p) {
@@ -844,6 +863,92 @@ int npf_write_target(npf_handle f, const float* color, const float* depth) {
     if (color) NPS_COPY(h, f->color.p, color, 16 * px, hipMemcpyHostToDevice);
     if (depth) NPS_COPY(h, f->depth.p, depth, 16 * px, hipMemcpyHostToDevice);
     return NPS_OK;
+}
+
+// ---------------- target blur ----------------
+
+static int npb_fail(npb_handle b, int code, const std::string& msg) {
+    b->err = msg;
+    return code;
+}
+
+static int npb_ensure(npb_handle b, DevBuf& d, size_t bytes) {
+    if (bytes <= d.bytes && d.p) return NPS_OK;
+    if (d.p) {
+        (void)hipStreamSynchronize(b->stream);
+        (void)hipFree(d.p);
+        d.p = nullptr;
+        d.bytes = 0;
+    }
+    hipError_t e = hipMalloc(&d.p, bytes);
+    if (e != hipSuccess) {
+        d.p = nullptr;
+        return npb_fail(b, NPS_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    d.bytes = bytes;
+    return NPS_OK;
+}
+
+int npb_create(int32_t device, npb_handle* out) {
+    if (!out) return NPS_ERR_ARG;
+    *out = nullptr;
+    if (hipSetDevice(device) != hipSuccess) return NPS_ERR_HIP;
+    npb_ctx* b = new npb_ctx();
+    b->device = device;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return NPS_ERR_HIP;
+    }
+    *out = b;
+    return NPS_OK;
+}
+
+void npb_destroy(npb_handle b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    free_buf(b->src); free_buf(b->dst); free_buf(b->temp);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+const char* npb_last_error(npb_handle b) { return b ? b->err.c_str() : "invalid handle"; }
+
+static int npb_run(npb_handle b, const float* src, float* dst, int32_t w, int32_t hh, int32_t radius, float sigma,
+                   int32_t is_srgb, bool dev) {
+    if (!b) return NPS_ERR_ARG;
+    if (!src || !dst || w <= 0 || hh <= 0 || radius < 0) return npb_fail(b, NPS_ERR_ARG, "bad blur arguments");
+    hipError_t e = hipSetDevice(b->device);
+    if (e != hipSuccess) return npb_fail(b, NPS_ERR_HIP, hipGetErrorString(e));
+    const size_t bytes = 16 * (size_t)w * hh;
+    int rc;
+    if ((rc = npb_ensure(b, b->temp, bytes)) != NPS_OK) return rc;
+    const float4* s = (const float4*)src;
+    float4* d = (float4*)dst;
+    if (!dev) {
+        if ((rc = npb_ensure(b, b->src, bytes)) != NPS_OK) return rc;
+        if ((rc = npb_ensure(b, b->dst, bytes)) != NPS_OK) return rc;
+        if ((e = hipMemcpyAsync(b->src.p, src, bytes, hipMemcpyHostToDevice, b->stream)) != hipSuccess)
+            return npb_fail(b, NPS_ERR_HIP, hipGetErrorString(e));
+        s = (const float4*)b->src.p;
+        d = (float4*)b->dst.p;
+    }
+    launch_blur(b->stream, s, d, (float4*)b->temp.p, w, hh, radius, sigma, is_srgb ? 1 : 0);
+    if ((e = hipGetLastError()) != hipSuccess) return npb_fail(b, NPS_ERR_HIP, hipGetErrorString(e));
+    if (!dev && (e = hipMemcpyAsync(dst, b->dst.p, bytes, hipMemcpyDeviceToHost, b->stream)) != hipSuccess)
+        return npb_fail(b, NPS_ERR_HIP, hipGetErrorString(e));
+    if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) return npb_fail(b, NPS_ERR_HIP, hipGetErrorString(e));
+    return NPS_OK;
+}
+
+int npb_blur(npb_handle b, const float* src, float* dst, int32_t w, int32_t hh, int32_t radius, float sigma,
+             int32_t is_srgb) {
+    return npb_run(b, src, dst, w, hh, radius, sigma, is_srgb, false);
+}
+
+int npb_blur_dev(npb_handle b, const float* src, float* dst, int32_t w, int32_t hh, int32_t radius, float sigma,
+                 int32_t is_srgb) {
+    return npb_run(b, src, dst, w, hh, radius, sigma, is_srgb, true);
 }
 
 int nps_dev_alloc(nps_handle h, int64_t bytes, void** out) {

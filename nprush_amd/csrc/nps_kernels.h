@@ -102,8 +102,13 @@ struct ExtCompositeArgs {
     uint64_t* act;              // per batch slot: 4 waves x 64 entry bits
     unsigned long long* stats;  // [0] forward pairs, [2] reverse pairs, [3] reverse wave-entries,
                                 // [4] forward wave-entries
+    const uint32_t* order;      // workgroup -> tile (launch_tile_order), ~0 = none
+    uint32_t* tile_work;        // 2 per tile: entries staged by the forward, reverse extent
     int backward;               // 0: image and loss only
 };
+// Entries of the composite's workgroup order (= its grid size).
+uint32_t ext_order_slots(uint32_t tiles_x, uint32_t tiles_y);
+void launch_tile_order(hipStream_t st, const uint32_t* ranges, uint32_t tiles_x, uint32_t tiles_y, uint32_t* order);
 void launch_ext_preprocess(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg,
                            const ViewParams& v, const float cam[3], ExtRec* rec, uint32_t* counts, uint4* rects,
                            uint64_t* dkey);
@@ -116,6 +121,10 @@ void launch_ext_chain(hipStream_t st, const float* splats, const float* sh, uint
                       const float cam[3], const double* g2, const uint32_t* counts, float* grads, float* accum);
 void launch_adam_ext(hipStream_t st, float* splats, float* sh, float* grads, float* m, float* v, uint32_t n, int deg,
                      float t);
+
+// k_blur.hip: both passes; temp is an H*W float4 scratch image
+void launch_blur(hipStream_t st, const float4* src, float4* dst, float4* temp, int W, int H, int radius, float sigma,
+                 int is_srgb);
 
 // k_debug.hip
 void launch_debug_math(hipStream_t st, int op, const float* x, const float* y, float* out, uint32_t n);

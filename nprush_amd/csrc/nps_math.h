@@ -45,6 +45,23 @@ NPS_HD float exp2_det(float x) {
     return r < 0x1p-126f ? 0.0f : r;
 }
 
+// exp2_det for x in [-125, 127], where none of its special cases applies
+// (no NaN, no overflow, results >= 2^-125.5, never flushed): the same
+// operations, so the same bits.  Callers guarantee the range.
+NPS_HD float exp2_det_in_range(float x) {
+    float k = rintf(x);
+    float f = x - k;
+    float p = 0x1.ffcbfcp-17f;
+    p = fmaf(p, f, 0x1.430912p-13f);
+    p = fmaf(p, f, 0x1.5d87fep-10f);
+    p = fmaf(p, f, 0x1.3b2ab6p-7f);
+    p = fmaf(p, f, 0x1.c6b08ep-5f);
+    p = fmaf(p, f, 0x1.ebfbe0p-3f);
+    p = fmaf(p, f, 0x1.62e430p-1f);
+    p = fmaf(p, f, 1.0f);
+    return ldexpf(p, (int)k);
+}
+
 NPS_HD float log2_det(float x) {
     if (x != x) return x;
     if (x < 0.0f) return NAN;

@@ -246,6 +246,14 @@ class GpuSplatOptimizerManager:
               "read_tiles")
         return ranges, ids[: nid.value], full
 
+    def read_tile_work(self) -> np.ndarray:
+        """EXT: [T, 2] entries staged by the forward, reverse extent (last backward/forward)."""
+        nt = C.c_int32()
+        check(self._lib.nps_read_tiles(self._h, None, None, None, C.byref(nt), None), self._h, "read_tiles")
+        w = np.empty((nt.value, 2), dtype=np.uint32)
+        check(self._lib.nps_read_tile_work(self._h, uptr(w)), self._h, "read_tile_work")
+        return w
+
     def read_stats(self) -> dict:
         s = np.zeros(12, dtype=np.int64)
         check(self._lib.nps_read_stats(self._h, lptr(s)), self._h, "read_stats")
@@ -314,6 +322,45 @@ class GpuSplatForwardPipelineManager:
         d = np.empty((h, w, 4), dtype=np.float32)
         check(self._lib.npf_read_target(self._f, fptr(c), fptr(d)), self._owner.handle, "read_target")
         return c, d
+
+
+class GpuBlurPipelineManager:
+    """new GpuBlurPipelineManager(device) (src/lib/gpu/GpuBlurPipelineManager.ts:9-49)."""
+
+    def __init__(self, device=None, *, hip_device: int = 0):
+        lib = _lib.load()
+        b = C.c_void_p()
+        check(lib.npb_create(int(hip_device), C.byref(b)), None, "npb_create")
+        self._lib = lib
+        self._b = b
+
+    def close(self):
+        if getattr(self, "_b", None):
+            self._lib.npb_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def blur(self, commandEncoder=None, srcView: np.ndarray = None, dstView=None, tempView=None, width: int = 0,
+             height: int = 0, radius: int = 16, sigma: float | None = None, isSrgb: bool = False) -> np.ndarray:
+        """GpuBlurPipelineManager.blur (GpuBlurPipelineManager.ts:51-99).  srcView is an
+        H x W x 4 float32 array of texel values; returns the blurred texels (and
+        writes them into dstView when given).  tempView is accepted and unused."""
+        src = np.ascontiguousarray(srcView, dtype=np.float32).reshape(height, width, 4)
+        out = np.empty((height, width, 4), dtype=np.float32)
+        sigma = radius / 2 if sigma is None else sigma
+        rc = self._lib.npb_blur(self._b, fptr(src), fptr(out), int(width), int(height), int(radius), float(sigma),
+                                1 if isSrgb else 0)
+        if rc != 0:
+            m = self._lib.npb_last_error(self._b)
+            raise _lib.NpsError(f"blur failed ({rc}): {m.decode() if m else ''}")
+        if dstView is not None:
+            dstView[...] = out
+        return out
 
 
 def debug_math(op: int, x: np.ndarray, y: np.ndarray | None = None, device: int = 0) -> np.ndarray:

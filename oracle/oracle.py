@@ -228,3 +228,17 @@ def ext_adam_f32(splats, sh, grads64, m, v, deg, t):
     lib.ora_ext_adam_f32(_p(s, _F), _p(h, _F), _p(f32(grads64), _F), _p(mm, _F), _p(vv, _F), s.shape[0], deg,
                          _p(tt, _F))
     return s, h, mm, vv, float(tt[0])
+
+
+def blur_f32(src_rgba, W, H, radius=16, sigma=None, is_srgb=True):
+    """GpuBlurPipelineManager.blur on the CPU: returns the H x W x 4 output."""
+    lib = load()
+    lib.ora_blur_f32.argtypes = [_F, _F, _F, i32, i32, i32, C.c_float, i32]
+    lib.ora_blur_f32.restype = i32
+    sigma = radius / 2 if sigma is None else sigma
+    src = f32(src_rgba)
+    dst = np.zeros((H, W, 4), np.float32)
+    tmp = np.zeros((H, W, 4), np.float32)
+    rc = lib.ora_blur_f32(_p(src, _F), _p(dst, _F), _p(tmp, _F), W, H, int(radius), float(sigma), 1 if is_srgb else 0)
+    assert rc == 0
+    return dst

@@ -10,7 +10,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def _declared():
     src = open(os.path.join(ROOT, "include", "nprush_hip.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b((?:nps|npf)_[a-z0-9_]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b((?:nps|npf|npb)_[a-z0-9_]+)\s*\(", src)))
 
 
 def test_header_symbols_exported():
@@ -38,3 +38,6 @@ def test_manager_method_names_match_reference():
         assert callable(getattr(GpuSplatOptimizerManager, m))
     for m in ("writeVPMatrix", "setTarget", "dispatch"):
         assert callable(getattr(GpuSplatForwardPipelineManager, m))
+    from nprush_amd import GpuBlurPipelineManager   # GpuBlurPipelineManager.ts:51
+
+    assert callable(GpuBlurPipelineManager.blur)
