@@ -129,6 +129,18 @@ int nps_forward(nps_handle h);
 int nps_step(nps_handle h);
 /* splat_adc.wgsl (OPT:380-385): clone/split into dead slots. */
 int nps_adc(nps_handle h);
+/* EXT mode: densification parameters (defaults 2e-4, 1.0, 0.005, 0.01).  No
+ * reference function; nps_adc in EXT mode then runs 3D Gaussian splatting's
+ * clone/split/prune, deterministic: a slot is dead if sigmoid(logit) <
+ * prune_opacity; a live slot qualifies if summed |dL/d ndc| / views >=
+ * grad_threshold; the k-th qualifier (ascending index) takes the k-th dead slot;
+ * split (largest scale > percent_dense * scene_extent) into two children (one
+ * in the parent's slot) at centre + R (z * s), z standard normal from a counter
+ * hash seeded by the ADC call count, log-scales - ln 1.6; otherwise clone.
+ * Children get zero Adam state, unfilled dead slots are parked at logit -1e4,
+ * accumulators are reset.  stats[4..8] = qualifiers, children, splits, clones,
+ * dead slots. */
+int nps_set_adc_ext(nps_handle h, float grad_threshold, float scene_extent, float prune_opacity, float percent_dense);
 /* GpuSplatOptimizerManager.dispatch (OPT:365-388): backward, step, and ADC on
  * every adc_period-th call.  Returns NPS_ERR_STATE when no target is set
  * (the reference returns silently, OPT:366). */
@@ -155,9 +167,10 @@ int nps_read_tiles(nps_handle h, uint32_t* ranges, uint32_t* ids, uint32_t* full
                    int64_t* num_ids);
 /* Counters of the last backward/ADC: stats[0]=keys, [1]=tiles over the cap,
  * [2]=(pixel, splat) pairs composited, [3]=pairs taken by the reverse pass
- * (alpha >= 0.001), [4]=ADC qualifiers, [5]=ADC children, [6]=ADC splits,
- * [7]=ADC clones, [8]=ADC cascade rounds, [9]=wave-entries walked by the
- * reverse pass, [10]=forward wave-entries that evaluate pow/exp, [11]=0. */
+ * (REF: alpha >= 0.001; EXT: composited), [4]=ADC qualifiers, [5]=ADC
+ * children, [6]=ADC splits, [7]=ADC clones, [8]=ADC cascade rounds (REF) or
+ * dead slots (EXT), [9]=wave-entries walked by the reverse pass,
+ * [10]=forward wave-entries (REF: those evaluating pow/exp), [11]=0. */
 int nps_read_stats(nps_handle h, int64_t stats[12]);
 /* EXT mode: per-tile work of the last nps_backward/nps_forward, 2 per tile:
  * entries of the tile list the forward staged before every pixel had stopped,

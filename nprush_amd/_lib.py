@@ -63,6 +63,7 @@ SIGNATURES = {
     "nps_forward": (i32, [_P]),
     "nps_step": (i32, [_P]),
     "nps_adc": (i32, [_P]),
+    "nps_set_adc_ext": (i32, [_P, C.c_float, C.c_float, C.c_float, C.c_float]),
     "nps_dispatch": (i32, [_P]),
     "nps_sync": (i32, [_P]),
     "nps_read_image": (i32, [_P, _F, _F]),

@@ -721,6 +721,165 @@ __global__ __launch_bounds__(256) void k_adam_ext(float4* __restrict__ splats, f
     grads[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// ---------------------------------------------------------------- densification
+
+namespace {
+constexpr float kExtLn16 = 0.47000363f;     // ln 1.6: log-scale reduction of a split
+constexpr float kExtDeadLogit = -10000.0f;  // parked slot: opacity 0
+
+// Wellons' lowbias32 integer hash
+__device__ __forceinline__ uint32_t lowbias32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+// standard normal sample k of child c of a split landing in `slot`: Box-Muller
+// on two hashed uniforms, deterministic functions only
+__device__ __forceinline__ float adc_normal(uint32_t seed, uint32_t slot, uint32_t child, uint32_t k) {
+    uint32_t h1 = lowbias32(seed ^ lowbias32(slot * 8u + child * 4u + k));
+    uint32_t h2 = lowbias32(h1 ^ 0x9e3779b9u);
+    float u1 = (float)((h1 >> 8) + 1u) * 0x1p-24f;
+    float u2 = (float)(h2 >> 8) * 0x1p-24f;
+    float r = sqrtf(-2.0f * log_det(u1));
+    return r * sin_det(6.2831855f * u2 + 1.5707964f);
+}
+}  // namespace
+
+// dead[i] = sigmoid(logit) < prune; qual[i] = live, seen (count > 0) and
+// summed |dL/d ndc| / count >= threshold
+__global__ __launch_bounds__(256) void k_adc_ext_mark(const float* __restrict__ splats,
+                                                      const float2* __restrict__ accum, uint32_t n, float threshold,
+                                                      float prune, uint32_t* __restrict__ dead,
+                                                      uint32_t* __restrict__ qual) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float op = 1.0f / (1.0f + exp_det(-splats[16 * (size_t)i + 3]));
+    uint32_t d = op < prune ? 1u : 0u;
+    uint32_t q = 0u;
+    if (!d) {
+        float2 a = accum[i];
+        q = (a.y > 0.0f && a.x / a.y >= threshold) ? 1u : 0u;
+    }
+    dead[i] = d;
+    qual[i] = q;
+}
+
+// dead_list[rank] = slot, qual_list[rank] = slot; cnt[0] = qualifiers, cnt[1] = dead slots
+__global__ __launch_bounds__(256) void k_adc_ext_lists(uint32_t n, const uint32_t* __restrict__ dead,
+                                                       const uint32_t* __restrict__ dead_rank,
+                                                       const uint32_t* __restrict__ qual,
+                                                       const uint32_t* __restrict__ qual_rank,
+                                                       uint32_t* __restrict__ dead_list,
+                                                       uint32_t* __restrict__ qual_list, uint32_t* __restrict__ cnt) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (dead[i]) dead_list[dead_rank[i]] = i;
+    if (qual[i]) qual_list[qual_rank[i]] = i;
+    if (i == n - 1) {
+        cnt[0] = qual_rank[i] + qual[i];
+        cnt[1] = dead_rank[i] + dead[i];
+    }
+}
+
+// Thread k < min(qualifiers, dead): the k-th placement; min <= k < dead: parks
+// the k-th dead slot.  Parents are read whole before any write; a parent slot is
+// written only by its own thread and dead slots are never parents.
+__global__ __launch_bounds__(256) void k_adc_ext_apply(uint32_t n, float4* __restrict__ splats,
+                                                       float4* __restrict__ sh, float4* __restrict__ m,
+                                                       float4* __restrict__ v, const uint32_t* __restrict__ dead_list,
+                                                       const uint32_t* __restrict__ qual_list,
+                                                       uint32_t* __restrict__ cnt, float split_scale, uint32_t seed) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nq = cnt[0], nd = cnt[1];
+    const uint32_t P = nq < nd ? nq : nd;
+    if (k >= nd) return;
+    const uint32_t d = dead_list[k];
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (k >= P) {
+        float4 r0 = splats[4 * (size_t)d];
+        r0.w = kExtDeadLogit;
+        splats[4 * (size_t)d] = r0;
+        for (int j = 0; j < 16; ++j) {
+            m[16 * (size_t)d + j] = z4;
+            v[16 * (size_t)d + j] = z4;
+        }
+        return;
+    }
+    const uint32_t q = qual_list[k];
+    float p[16];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        float4 t = splats[4 * (size_t)q + j];
+        p[4 * j] = t.x; p[4 * j + 1] = t.y; p[4 * j + 2] = t.z; p[4 * j + 3] = t.w;
+    }
+    for (int j = 0; j < 12; ++j) sh[12 * (size_t)d + j] = sh[12 * (size_t)q + j];
+    float sc[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) sc[a] = exp_det(p[4 + a]);
+    float smax = sc[0] > sc[1] ? sc[0] : sc[1];
+    smax = smax > sc[2] ? smax : sc[2];
+    if (smax > split_scale) {
+        float qq = ((p[8] * p[8] + p[9] * p[9]) + p[10] * p[10]) + p[11] * p[11];
+        float ql = sqrtf(qq > 1e-24f ? qq : 1e-24f);
+        float qw = p[8] / ql, qx = p[9] / ql, qy = p[10] / ql, qz = p[11] / ql;
+        const float ax[3][3] = {
+            {1.0f - 2.0f * (qy * qy + qz * qz), 2.0f * (qx * qy + qw * qz), 2.0f * (qx * qz - qw * qy)},
+            {2.0f * (qx * qy - qw * qz), 1.0f - 2.0f * (qx * qx + qz * qz), 2.0f * (qy * qz + qw * qx)},
+            {2.0f * (qx * qz + qw * qy), 2.0f * (qy * qz - qw * qx), 1.0f - 2.0f * (qx * qx + qy * qy)}};
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const uint32_t slot = c == 0 ? q : d;
+            float o[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) o[a] = adc_normal(seed, slot, (uint32_t)c, (uint32_t)a) * sc[a];
+            float out[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) out[j] = p[j];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) out[j] = p[j] + ((ax[0][j] * o[0] + ax[1][j] * o[1]) + ax[2][j] * o[2]);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) out[4 + a] = p[4 + a] - kExtLn16;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                splats[4 * (size_t)slot + j] = make_float4(out[4 * j], out[4 * j + 1], out[4 * j + 2], out[4 * j + 3]);
+        }
+        for (int j = 0; j < 16; ++j) {
+            m[16 * (size_t)q + j] = z4;
+            v[16 * (size_t)q + j] = z4;
+        }
+        atomicAdd(&cnt[2], 1u);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            splats[4 * (size_t)d + j] = make_float4(p[4 * j], p[4 * j + 1], p[4 * j + 2], p[4 * j + 3]);
+        atomicAdd(&cnt[3], 1u);
+    }
+    for (int j = 0; j < 16; ++j) {
+        m[16 * (size_t)d + j] = z4;
+        v[16 * (size_t)d + j] = z4;
+    }
+}
+
+void run_adc_ext(hipStream_t st, float* splats, float* sh, float* m, float* v, float* accum, uint32_t n,
+                 const AdcExtParams& prm, uint32_t seed, const AdcExtBuffers& b) {
+    launch_fill_u32(st, b.cnt, 0u, 4);
+    if (n == 0) return;
+    const dim3 g((n + 255) / 256), blk(256);
+    hipLaunchKernelGGL(k_adc_ext_mark, g, blk, 0, st, (const float*)splats, (const float2*)accum, n, prm.threshold,
+                       prm.prune_opacity, b.dead, b.qual);
+    exclusive_scan_u32(st, b.dead, b.dead_rank, n, b.scan_tmp);
+    exclusive_scan_u32(st, b.qual, b.qual_rank, n, b.scan_tmp);
+    hipLaunchKernelGGL(k_adc_ext_lists, g, blk, 0, st, n, (const uint32_t*)b.dead, (const uint32_t*)b.dead_rank,
+                       (const uint32_t*)b.qual, (const uint32_t*)b.qual_rank, b.dead_list, b.qual_list, b.cnt);
+    hipLaunchKernelGGL(k_adc_ext_apply, g, blk, 0, st, n, (float4*)splats, (float4*)sh, (float4*)m, (float4*)v,
+                       (const uint32_t*)b.dead_list, (const uint32_t*)b.qual_list, b.cnt,
+                       prm.percent_dense * prm.extent, seed);
+    launch_fill_u32(st, (uint32_t*)accum, 0u, 2 * (uint64_t)n);
+}
+
 // ---------------------------------------------------------------- launchers
 
 void launch_ext_preprocess(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg,

@@ -59,6 +59,8 @@ struct nps_ctx {
     float* gradx = nullptr;  // EXT: 64 FP32 gradients per splat
     float adam_t = 0.0f;
     int64_t step_count = 0;
+    AdcExtParams adcx = {2e-4f, 1.0f, 0.005f, 0.01f};  // EXT densification
+    uint32_t adc_calls = 0;                             // seed of the EXT split samples
     float vp[16] = {0}, vp_inv[16] = {0};
     int blur = 0;
     bool has_camera = false;
@@ -637,9 +639,53 @@ int nps_step(nps_handle h) {
     return NPS_OK;
 }
 
+static int adc_ext(nps_handle h) {
+    NPS_HIP(h, hipSetDevice(h->device));
+    const uint32_t n = h->n;
+    const size_t nn = (size_t)std::max<uint32_t>(n, 1);
+    int rc;
+    if ((rc = ensure(h, h->adc_u32, 4 * (6 * nn + 8))) != NPS_OK) return rc;
+    if ((rc = ensure(h, h->adc_scan, scan_temp_bytes(n) + 64)) != NPS_OK) return rc;
+    uint32_t* base = (uint32_t*)h->adc_u32.p;
+    AdcExtBuffers b;
+    b.dead = base;
+    b.qual = base + nn;
+    b.dead_rank = base + 2 * nn;
+    b.qual_rank = base + 3 * nn;
+    b.dead_list = base + 4 * nn;
+    b.qual_list = base + 5 * nn;
+    b.cnt = base + 6 * nn;
+    b.scan_tmp = h->adc_scan.p;
+    if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[5], h->stream));
+    run_adc_ext(h->stream, h->splats, h->sh, h->m, h->v, h->accum, n, h->adcx, h->adc_calls, b);
+    if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[6], h->stream));
+    NPS_HIP(h, hipGetLastError());
+    uint32_t cnt[4] = {0, 0, 0, 0};
+    NPS_HIP(h, hipMemcpyAsync(cnt, b.cnt, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
+    NPS_HIP(h, hipStreamSynchronize(h->stream));
+    if (h->timing) hipEventElapsedTime(&h->timing_ms[5], h->ev[5], h->ev[6]);
+    h->stats_host[4] = cnt[0];
+    h->stats_host[5] = std::min(cnt[0], cnt[1]);
+    h->stats_host[6] = cnt[2];
+    h->stats_host[7] = cnt[3];
+    h->stats_host[8] = cnt[1];
+    h->adc_calls++;
+    return NPS_OK;
+}
+
+int nps_set_adc_ext(nps_handle h, float grad_threshold, float scene_extent, float prune_opacity,
+                    float percent_dense) {
+    if (!h) return NPS_ERR_ARG;
+    if (!h->ext) return set_err(h, NPS_ERR_STATE, "EXT mode only");
+    if (!(grad_threshold >= 0.0f) || !(scene_extent > 0.0f) || !(prune_opacity >= 0.0f) || !(percent_dense > 0.0f))
+        return set_err(h, NPS_ERR_ARG, "bad densification parameters");
+    h->adcx = AdcExtParams{grad_threshold, scene_extent, prune_opacity, percent_dense};
+    return NPS_OK;
+}
+
 int nps_adc(nps_handle h) {
     if (!h) return NPS_ERR_ARG;
-    if (h->ext) return set_err(h, NPS_ERR_STATE, "EXT densification is not available");
+    if (h->ext) return adc_ext(h);
     NPS_HIP(h, hipSetDevice(h->device));
     const uint32_t n = h->n;
     int rc;

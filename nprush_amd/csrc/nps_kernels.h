@@ -122,6 +122,21 @@ void launch_ext_chain(hipStream_t st, const float* splats, const float* sh, uint
 void launch_adam_ext(hipStream_t st, float* splats, float* sh, float* grads, float* m, float* v, uint32_t n, int deg,
                      float t);
 
+// EXT densification (k_ext.hip)
+struct AdcExtParams {
+    float threshold;      // mean |dL/d ndc| per view that qualifies a splat
+    float extent;         // scene extent
+    float prune_opacity;  // sigmoid(logit) below: dead slot
+    float percent_dense;  // split when the largest scale exceeds percent_dense * extent
+};
+struct AdcExtBuffers {
+    uint32_t *dead, *qual, *dead_rank, *qual_rank, *dead_list, *qual_list;  // n each
+    uint32_t* cnt;  // 4: qualifiers, dead slots, splits, clones
+    void* scan_tmp;
+};
+void run_adc_ext(hipStream_t st, float* splats, float* sh, float* m, float* v, float* accum, uint32_t n,
+                 const AdcExtParams& prm, uint32_t seed, const AdcExtBuffers& b);
+
 // k_blur.hip: both passes; temp is an H*W float4 scratch image
 void launch_blur(hipStream_t st, const float4* src, float4* dst, float4* temp, int W, int H, int radius, float sigma,
                  int is_srgb);

@@ -218,6 +218,12 @@ class GpuSplatOptimizerManager:
     def adc(self):
         check(self._lib.nps_adc(self._h), self._h, "adc")
 
+    def set_adc_ext(self, grad_threshold: float = 2e-4, scene_extent: float = 1.0, prune_opacity: float = 0.005,
+                    percent_dense: float = 0.01):
+        """EXT densification parameters (see nps_set_adc_ext)."""
+        check(self._lib.nps_set_adc_ext(self._h, grad_threshold, scene_extent, prune_opacity, percent_dense),
+              self._h, "set_adc_ext")
+
     def sync(self):
         check(self._lib.nps_sync(self._h), self._h, "sync")
 

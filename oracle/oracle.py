@@ -19,6 +19,7 @@ _F = C.POINTER(C.c_float)
 _D = C.POINTER(C.c_double)
 _I = C.POINTER(C.c_int32)
 _U = C.POINTER(C.c_uint32)
+_L = C.POINTER(C.c_int64)
 i32 = C.c_int32
 
 NO_REG = 1
@@ -242,3 +243,25 @@ def blur_f32(src_rgba, W, H, radius=16, sigma=None, is_srgb=True):
     rc = lib.ora_blur_f32(_p(src, _F), _p(dst, _F), _p(tmp, _F), W, H, int(radius), float(sigma), 1 if is_srgb else 0)
     assert rc == 0
     return dst
+
+
+def ext_adc_f32(splats, sh, m, v, accum, threshold=2e-4, extent=3.3, prune_opacity=0.005, percent_dense=0.01,
+                seed=0):
+    """EXT densification on copies; returns (splats, sh, m, v, accum, stats[5])."""
+    lib = load()
+    lib.ora_ext_adc_f32.argtypes = [_F, _F, _F, _F, _F, i32, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32,
+                                    _L]
+    lib.ora_ext_adc_f32.restype = i32
+    s, h = f32(splats).copy(), f32(sh).copy()
+    mm, vv, acc = f32(m).copy(), f32(v).copy(), f32(accum).copy()
+    st = np.zeros(5, np.int64)
+    lib.ora_ext_adc_f32(_p(s, _F), _p(h, _F), _p(mm, _F), _p(vv, _F), _p(acc, _F), s.shape[0], threshold, extent,
+                        prune_opacity, percent_dense, seed, _p(st, _L))
+    return s, h, mm, vv, acc, st
+
+
+def ext_adc_normal(seed, slot, child, k):
+    lib = load()
+    lib.ora_ext_adc_normal.argtypes = [C.c_uint32] * 4
+    lib.ora_ext_adc_normal.restype = C.c_float
+    return float(lib.ora_ext_adc_normal(seed, slot, child, k))

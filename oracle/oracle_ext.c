@@ -124,3 +124,105 @@ int ora_ext_adam_f32(float* splats, float* sh, const float* grads, float* m, flo
     *adam_t = t;
     return 0;
 }
+
+/* ---------------------------------------------------------------- EXT ADC
+ * Densification of the EXT model (no reference function; 3D Gaussian
+ * splatting's clone/split/prune made deterministic):
+ *   op = sigmoid(logit); a slot is dead if op < prune_opacity;
+ *   a live slot qualifies if its view count c > 0 and g / c >= threshold
+ *   (g = summed |dL/d ndc| of its centre since the last ADC);
+ *   the k-th qualifier (ascending index) takes the k-th dead slot (ascending);
+ *   qualifiers without a dead slot are left alone;
+ *   split if max_k exp(log_s_k) > percent_dense * extent, else clone;
+ *   split: two children replace the parent (child 0 in the parent's slot,
+ *     child 1 in the dead slot): centre + sum_k axis_k * (z_k * s_k), z_k
+ *     standard normal (Box-Muller on a counter hash of (seed, slot, child, k)),
+ *     log-scales reduced by ln 1.6, every other parameter and the SH copied;
+ *   clone: an exact copy in the dead slot;
+ *   Adam m, v of every child slot are zeroed (the clone's parent keeps its own);
+ *   dead slots that receive nothing get logit -1e4 and zero m, v;
+ *   accumulators are reset.
+ * stats[5] = qualifiers, children placed, splits, clones, dead slots. */
+static uint32_t lowbias32(uint32_t x) {   /* Wellons' lowbias32 integer hash */
+    x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+    return x;
+}
+static float adc_normal(uint32_t seed, uint32_t slot, uint32_t child, uint32_t k) {
+    uint32_t h1 = lowbias32(seed ^ lowbias32(slot * 8u + child * 4u + k));
+    uint32_t h2 = lowbias32(h1 ^ 0x9e3779b9u);
+    float u1 = (float)((h1 >> 8) + 1u) * 0x1p-24f;   /* (0, 1] */
+    float u2 = (float)(h2 >> 8) * 0x1p-24f;          /* [0, 1) */
+    float r = sqrtf(-2.0f * rm_logf(u1));
+    return r * rm_sinf(6.2831855f * u2 + 1.5707964f);
+}
+#define EXT_LN_1_6 0.47000363f
+#define EXT_DEAD_LOGIT -10000.0f
+
+int ora_ext_adc_f32(float* splats, float* sh, float* m, float* v, float* accum, int n, float threshold, float extent,
+                    float prune_opacity, float percent_dense, uint32_t seed, int64_t* stats) {
+    int32_t* dead = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    int32_t* qual = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    int nd = 0, nq = 0;
+    for (int i = 0; i < n; ++i) {
+        float op = 1.0f / (1.0f + rm_expf(-splats[16 * (size_t)i + 3]));
+        if (op < prune_opacity) { dead[nd++] = i; continue; }
+        float c = accum[2 * (size_t)i + 1];
+        if (c > 0.0f && accum[2 * (size_t)i] / c >= threshold) qual[nq++] = i;
+    }
+    int P = nq < nd ? nq : nd;
+    int64_t nsplit = 0, nclone = 0;
+    for (int k = 0; k < P; ++k) {
+        int q = qual[k], d = dead[k];
+        float* sp = splats + 16 * (size_t)q;
+        float* sd = splats + 16 * (size_t)d;
+        float s[3];
+        for (int a = 0; a < 3; ++a) s[a] = rm_expf(sp[4 + a]);
+        float smax = s[0] > s[1] ? s[0] : s[1];
+        smax = smax > s[2] ? smax : s[2];
+        memcpy(sh + 48 * (size_t)d, sh + 48 * (size_t)q, 48 * sizeof(float));
+        if (smax > percent_dense * extent) {
+            float qq = ((sp[8] * sp[8] + sp[9] * sp[9]) + sp[10] * sp[10]) + sp[11] * sp[11];
+            float ql = sqrtf(qq > 1e-24f ? qq : 1e-24f);
+            float qw = sp[8] / ql, qx = sp[9] / ql, qy = sp[10] / ql, qz = sp[11] / ql;
+            float ax[3][3] = {
+                { 1.0f - 2.0f * (qy * qy + qz * qz), 2.0f * (qx * qy + qw * qz), 2.0f * (qx * qz - qw * qy) },
+                { 2.0f * (qx * qy - qw * qz), 1.0f - 2.0f * (qx * qx + qz * qz), 2.0f * (qy * qz + qw * qx) },
+                { 2.0f * (qx * qz + qw * qy), 2.0f * (qy * qz - qw * qx), 1.0f - 2.0f * (qx * qx + qy * qy) } };
+            float parent[16];
+            memcpy(parent, sp, sizeof(parent));
+            for (int c = 0; c < 2; ++c) {
+                float* out = c == 0 ? sp : sd;
+                uint32_t slot = (uint32_t)(c == 0 ? q : d);
+                float o[3];
+                for (int a = 0; a < 3; ++a) o[a] = adc_normal(seed, slot, (uint32_t)c, (uint32_t)a) * s[a];
+                memcpy(out, parent, sizeof(parent));
+                for (int j = 0; j < 3; ++j)
+                    out[j] = parent[j] + ((ax[0][j] * o[0] + ax[1][j] * o[1]) + ax[2][j] * o[2]);
+                for (int a = 0; a < 3; ++a) out[4 + a] = parent[4 + a] - EXT_LN_1_6;
+            }
+            memset(m + 64 * (size_t)q, 0, 64 * sizeof(float));
+            memset(v + 64 * (size_t)q, 0, 64 * sizeof(float));
+            nsplit++;
+        } else {
+            memcpy(sd, sp, 16 * sizeof(float));
+            nclone++;
+        }
+        memset(m + 64 * (size_t)d, 0, 64 * sizeof(float));
+        memset(v + 64 * (size_t)d, 0, 64 * sizeof(float));
+    }
+    for (int k = P; k < nd; ++k) {
+        int d = dead[k];
+        splats[16 * (size_t)d + 3] = EXT_DEAD_LOGIT;
+        memset(m + 64 * (size_t)d, 0, 64 * sizeof(float));
+        memset(v + 64 * (size_t)d, 0, 64 * sizeof(float));
+    }
+    memset(accum, 0, 2 * (size_t)n * sizeof(float));
+    if (stats) { stats[0] = nq; stats[1] = P; stats[2] = nsplit; stats[3] = nclone; stats[4] = nd; }
+    free(dead); free(qual);
+    return 0;
+}
+
+/* the normal samples of a split, for tests of their distribution */
+float ora_ext_adc_normal(uint32_t seed, uint32_t slot, uint32_t child, uint32_t k) {
+    return adc_normal(seed, slot, child, k);
+}

@@ -200,3 +200,66 @@ def test_ext_adam_bit_exact(oracle):
     assert np.array_equal(gm, em) and np.array_equal(gv, ev)
     assert np.all(mgr.read_grads_f32() == 0)
     mgr.close()
+
+
+@pytest.mark.parametrize("n,dead_frac,seed", [(5000, 0.3, 0), (3000, 0.02, 1), (64, 0.5, 2)])
+def test_ext_adc_bit_exact(oracle, n, dead_frac, seed):
+    """EXT densification on the GPU against the oracle, two calls in a row
+    (the second uses the next seed), with more or fewer dead slots than
+    qualifiers and a mix of splits and clones."""
+    rng = np.random.default_rng(seed)
+    s, sh = scenes.make_ext_splats(n, seed, sh_degree=3)
+    s[rng.random(n) < dead_frac, 3] = -9.0
+    s[:, 4:7] += rng.normal(0.0, 1.0, (n, 3)).astype(np.float32)   # scales spread over the split threshold
+    m = rng.standard_normal((n, 64)).astype(np.float32)
+    v = np.abs(rng.standard_normal((n, 64))).astype(np.float32)
+    mgr = GpuSplatOptimizerManager(numSplats=n, mode="ext", sh_degree=3, splats=s, sh=sh, adc_enabled=False)
+    mgr.set_adc_ext(2e-4, 3.3, 0.005, 0.01)
+    mgr.write_adam(m, v, 3.0)
+    for call in range(2):
+        acc = np.zeros((n, 2), np.float32)
+        acc[:, 1] = rng.integers(0, 4, n)
+        acc[:, 0] = (rng.random(n) * 6e-4 * acc[:, 1]).astype(np.float32)
+        mgr.write_accum(acc)
+        mgr.adc()
+        s, sh, m, v, acc_o, st = oracle.ext_adc_f32(s, sh, m, v, acc, 2e-4, 3.3, 0.005, 0.01, seed=call)
+        assert np.array_equal(mgr.read_splats(), s)
+        assert np.array_equal(mgr.read_sh(), sh)
+        gm, gv, _ = mgr.read_adam()
+        assert np.array_equal(gm, m) and np.array_equal(gv, v)
+        assert np.all(mgr.read_accum() == 0)
+        g = mgr.read_stats()
+        assert [g["adc_qualifiers"], g["adc_children"], g["adc_splits"], g["adc_clones"], g["adc_rounds"]] == \
+            [int(st[0]), int(st[1]), int(st[2]), int(st[3]), int(st[4])]
+    mgr.close()
+
+
+def test_ext_training_reduces_l1(oracle):
+    """Config-0 scene in EXT mode (SH degree 0, 256x256, target from seed 1):
+    60 dispatches (backward + Adam, densification every 25 with a dead pool)
+    lower the L1 loss."""
+    W = H = 256
+    n_live, cap = 16384, 20480
+    vp, vi = scenes.config0_camera()
+    tgt = _target(oracle, 1, n_live, 0, vp, vi, W, H)
+    s, sh = scenes.make_ext_splats(n_live, 0, sh_degree=0)
+    pool = np.zeros((cap, 16), np.float32)
+    pool[:n_live] = s
+    pool[n_live:, 3] = -10000.0
+    pool[n_live:, 4:7] = -4.0
+    pool[n_live:, 8] = 1.0
+    shp = np.zeros((cap, 48), np.float32)
+    shp[:n_live] = sh
+    mgr = GpuSplatOptimizerManager(numSplats=cap, mode="ext", sh_degree=0, splats=pool, sh=shp, adc_period=25)
+    mgr.set_adc_ext(2e-4, 3.3, 0.005, 0.01)
+    mgr.writeSplatVPMatrix(vp, vi)
+    mgr.setBackwardTarget(tgt, np.ones((H, W), np.float32), W, H)
+    losses = []
+    for k in range(60):
+        mgr.dispatch()
+        losses.append(mgr.read_loss()[1])
+        if k == 24:
+            st = mgr.read_stats()
+            assert st["adc_rounds"] >= cap - n_live        # the dead pool was seen
+    assert losses[-1] < 0.9 * losses[0], (losses[0], losses[-1])
+    mgr.close()
