@@ -205,6 +205,34 @@ int npf_read_target(npf_handle f, float* color, float* depth);
 int npf_write_target(npf_handle f, const float* color, const float* depth);
 
 /*
+ * Multi-view training step over several EXT scenes (no reference counterpart:
+ * the reference trains one view per frame, OPT:365-388).  A step composites
+ * every registered view of every scene -- a scene's gradients add up over its
+ * views in a fixed order (most pixels first, then registration order) -- then
+ * runs Adam once per scene and densification on each scene's nps_config
+ * cadence.  Scenes must be EXT handles on one device.
+ *   mode 0: view after view, each with its own launches and key-count
+ *           readback (the per-view path of nps_backward);
+ *   mode 1: the whole step -- views spread over `lanes` concurrent graph
+ *           branches, then the Adam updates -- as one hipGraph, captured on
+ *           the first call after the view set changes and replayed after
+ *           that.  Key buffers are sized from a measurement at capture time
+ *           (+25 %); a view that outgrows them makes that step fall back to
+ *           mode 0 and the next one re-capture.
+ * Both modes give bit-identical results.
+ */
+typedef struct npt_ctx* npt_handle;
+int npt_create(nps_handle* scenes, int32_t n_scenes, int32_t lanes, npt_handle* out);
+void npt_destroy(npt_handle t);
+const char* npt_last_error(npt_handle t);
+/* rgba_dev: the view's target, H*W*4 floats on the device (kept by pointer). */
+int npt_add_view(npt_handle t, int32_t scene, const float vp[16], const float vp_inv[16], const float* rgba_dev,
+                 int32_t w, int32_t hgt);
+int npt_step(npt_handle t, int32_t mode);
+/* Losses of every view in the last step: loss2v[2 v] = L2, loss2v[2 v + 1] = L1. */
+int npt_read_losses(npt_handle t, double* loss2v);
+
+/*
  * new GpuBlurPipelineManager(device) (src/lib/gpu/GpuBlurPipelineManager.ts:9-49).
  * blur(encoder, srcView, dstView, tempView, w, h, radius, sigma, isSrgb)
  * (GpuBlurPipelineManager.ts:51-99, blur.wgsl:13-48): separable Gaussian over

@@ -80,6 +80,12 @@ SIGNATURES = {
     "npf_dispatch": (i32, [_P, i32, i32]),
     "npf_read_target": (i32, [_P, _F, _F]),
     "npf_write_target": (i32, [_P, _F, _F]),
+    "npt_create": (i32, [C.POINTER(_P), i32, i32, C.POINTER(_P)]),
+    "npt_destroy": (None, [_P]),
+    "npt_last_error": (C.c_char_p, [_P]),
+    "npt_add_view": (i32, [_P, i32, _F, _F, _P, i32, i32]),
+    "npt_step": (i32, [_P, i32]),
+    "npt_read_losses": (i32, [_P, _D]),
     "npb_create": (i32, [i32, C.POINTER(_P)]),
     "npb_destroy": (None, [_P]),
     "npb_last_error": (C.c_char_p, [_P]),

@@ -327,7 +327,9 @@ __global__ __launch_bounds__(256) void k_keys_wave(uint32_t n, const uint32_t* _
 // ranges[2t] = start, ranges[2t+1] = end of tile t in the sorted keys.
 // Tiles without keys keep the [0,0) the caller cleared.
 __global__ __launch_bounds__(256) void k_tile_ranges(const uint64_t* __restrict__ keys, uint64_t m,
+                                                     const uint64_t* __restrict__ m_dev,
                                                      uint32_t* __restrict__ ranges) {
+    if (m_dev) m = min(*m_dev, m);
     uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= m) return;
     uint32_t t = (uint32_t)(keys[j] >> 32);
@@ -376,9 +378,9 @@ void launch_emit_keys(hipStream_t st, uint32_t n, const uint32_t* counts, const 
                        num_tiles, cut, splat_block, offsets, (uint32_t*)nullptr, keys);
 }
 
-void launch_tile_ranges(hipStream_t st, const uint64_t* keys, uint64_t m, uint32_t* ranges) {
+void launch_tile_ranges(hipStream_t st, const uint64_t* keys, uint64_t m, uint32_t* ranges, const uint64_t* m_dev) {
     if (m == 0) return;
-    hipLaunchKernelGGL(k_tile_ranges, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, keys, m, ranges);
+    hipLaunchKernelGGL(k_tile_ranges, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, keys, m, m_dev, ranges);
 }
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint64_t n) {

@@ -110,7 +110,8 @@ __global__ __launch_bounds__(256) void k_ext_rank_counts(uint32_t n, const uint6
 __global__ __launch_bounds__(256) void k_ext_keys(uint32_t n, const uint64_t* __restrict__ order,
                                                   const uint32_t* __restrict__ rank_counts,
                                                   const uint4* __restrict__ rects, uint32_t tiles_x,
-                                                  const uint64_t* __restrict__ offsets, uint64_t* __restrict__ keys) {
+                                                  const uint64_t* __restrict__ offsets, uint64_t* __restrict__ keys,
+                                                  uint64_t cap, uint32_t* __restrict__ overflow) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wbase = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
     const uint32_t mine = wbase + lane;
@@ -131,9 +132,12 @@ __global__ __launch_bounds__(256) void k_ext_keys(uint32_t n, const uint64_t* __
         const uint32_t ry0 = __shfl(my_r.z, (int)s, 64);
         const uint32_t rw = rx1 - rx0;
         const uint64_t pos = __shfl(my_off, (int)s, 64);
+        if (pos + cnt > cap) {
+            if (lane == 0) *overflow = 1u;
+        }
         for (uint32_t k = lane; k < cnt; k += 64u) {
             const uint32_t ty = ry0 + k / rw, tx = rx0 + k % rw;
-            keys[pos + k] = ((uint64_t)(ty * tiles_x + tx) << 32) | id;
+            if (pos + k < cap) keys[pos + k] = ((uint64_t)(ty * tiles_x + tx) << 32) | id;
         }
     }
 }
@@ -684,13 +688,34 @@ __device__ __forceinline__ float ext_lr(int p, int deg) {
     return 0.0f;
 }
 
+// Adam's step counter lives on the device: ctl = {t, 1 - b1^t, 1 - b2^t, skip}.
+// One thread: unless a flag (one per 16 words) reports a dropped key, t += 1
+// and the bias corrections are recomputed; otherwise the step is skipped.
+__global__ void k_adam_ext_prep(float* __restrict__ ctl, const uint32_t* __restrict__ flags, uint32_t nflags) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t bad = 0u;
+    for (uint32_t k = 0; k < nflags; ++k) bad |= flags[16 * k];
+    if (bad) {
+        ctl[3] = 1.0f;
+        return;
+    }
+    const float t = ctl[0] + 1.0f;
+    ctl[0] = t;
+    ctl[1] = 1.0f - pow_det(0.9f, t);
+    ctl[2] = 1.0f - pow_det(0.999f, t);
+    ctl[3] = 0.0f;
+}
+
 // One thread per float4 of the 64 parameters: reads parameter + grad + m + v
 // (64 B), writes parameter + m + v and a zeroed grad (64 B).
 __global__ __launch_bounds__(256) void k_adam_ext(float4* __restrict__ splats, float4* __restrict__ sh,
                                                   float4* __restrict__ grads, float4* __restrict__ m,
-                                                  float4* __restrict__ v, uint32_t n, int deg, float bc1, float bc2) {
+                                                  float4* __restrict__ v, uint32_t n, int deg,
+                                                  const float* __restrict__ ctl) {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= 16 * (size_t)n) return;
+    if (ctl[3] != 0.0f) return;
+    const float bc1 = ctl[1], bc2 = ctl[2];
     const size_t i = idx >> 4;
     const int part = (int)(idx & 15);
     const float lr0 = ext_lr(4 * part, deg);
@@ -880,6 +905,35 @@ void run_adc_ext(hipStream_t st, float* splats, float* sh, float* m, float* v, f
     launch_fill_u32(st, (uint32_t*)accum, 0u, 2 * (uint64_t)n);
 }
 
+// out[0..1] = sums of the per-tile (L2, L1) pairs, in a fixed order (one workgroup).
+__global__ __launch_bounds__(256) void k_sum_tile_loss(const double* __restrict__ tl, uint32_t ntiles,
+                                                       double* __restrict__ out) {
+    __shared__ double s[2][256];
+    double a = 0.0, b = 0.0;
+    for (uint32_t t = threadIdx.x; t < ntiles; t += 256u) {
+        a += tl[2 * t];
+        b += tl[2 * t + 1];
+    }
+    s[0][threadIdx.x] = a;
+    s[1][threadIdx.x] = b;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s[0][threadIdx.x] += s[0][threadIdx.x + w];
+            s[1][threadIdx.x] += s[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s[0][0];
+        out[1] = s[1][0];
+    }
+}
+
+void launch_sum_tile_loss(hipStream_t st, const double* tile_loss, uint32_t ntiles, double* out2) {
+    hipLaunchKernelGGL(k_sum_tile_loss, dim3(1), dim3(256), 0, st, tile_loss, ntiles, out2);
+}
+
 // ---------------------------------------------------------------- launchers
 
 void launch_ext_preprocess(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg,
@@ -904,10 +958,11 @@ void launch_ext_rank_counts(hipStream_t st, uint32_t n, const uint64_t* order, c
 }
 
 void launch_ext_keys(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts,
-                     const uint4* rects, uint32_t tiles_x, const uint64_t* offsets, uint64_t* keys) {
+                     const uint4* rects, uint32_t tiles_x, const uint64_t* offsets, uint64_t* keys, uint64_t cap,
+                     uint32_t* overflow) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_ext_keys, dim3((n + 255) / 256), dim3(256), 0, st, n, order, rank_counts, rects, tiles_x,
-                       offsets, keys);
+                       offsets, keys, cap, overflow);
 }
 
 uint32_t ext_order_slots(uint32_t tiles_x, uint32_t tiles_y) {
@@ -941,13 +996,12 @@ void launch_ext_chain(hipStream_t st, const float* splats, const float* sh, uint
 }
 
 void launch_adam_ext(hipStream_t st, float* splats, float* sh, float* grads, float* m, float* v, uint32_t n, int deg,
-                     float t) {
+                     float* ctl, const uint32_t* flags, uint32_t nflags) {
+    hipLaunchKernelGGL(k_adam_ext_prep, dim3(1), dim3(64), 0, st, ctl, flags, nflags);
     if (n == 0) return;
-    const float bc1 = 1.0f - pow_det(0.9f, t);
-    const float bc2 = 1.0f - pow_det(0.999f, t);
     const size_t threads = 16 * (size_t)n;
     hipLaunchKernelGGL(k_adam_ext, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, (float4*)splats,
-                       (float4*)sh, (float4*)grads, (float4*)m, (float4*)v, n, deg, bc1, bc2);
+                       (float4*)sh, (float4*)grads, (float4*)m, (float4*)v, n, deg, (const float*)ctl);
 }
 
 }  // namespace nps

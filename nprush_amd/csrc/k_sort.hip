@@ -149,10 +149,14 @@ void exclusive_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint6
     exclusive_scan<uint32_t, uint32_t>(st, in, out, n, temp);
 }
 
+// m_dev (optional): the key count lives in device memory (capacity m); the
+// grid covers the capacity and blocks past the count contribute nothing.
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t* __restrict__ keys, uint64_t m,
-                                                             int shift, uint32_t mask, uint32_t nblk,
+                                                             const uint64_t* __restrict__ m_dev, int shift,
+                                                             uint32_t mask, uint32_t nblk,
                                                              uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[kRadix];
+    if (m_dev) m = min(*m_dev, m);
     h[threadIdx.x] = 0u;
     __syncthreads();
     uint64_t base = (uint64_t)blockIdx.x * kSortBlock;
@@ -171,11 +175,13 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t* __r
 template <bool kHasVals>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t* __restrict__ keys_in,
                                                                 const uint32_t* __restrict__ vals_in, uint64_t m,
+                                                                const uint64_t* __restrict__ m_dev,
                                                                 int shift, uint32_t mask, int bits, uint32_t nblk,
                                                                 const uint32_t* __restrict__ offsets,
                                                                 uint64_t* __restrict__ keys_out,
                                                                 uint32_t* __restrict__ vals_out) {
     __shared__ uint32_t wcnt[4][kRadix];
+    if (m_dev) m = min(*m_dev, m);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int k = threadIdx.x; k < 4 * kRadix; k += kSortThreads) (&wcnt[0][0])[k] = 0u;
     __syncthreads();
@@ -235,9 +241,11 @@ size_t radix_temp_bytes(uint64_t m) {
 }
 
 // Sorts keys[0..m) (and vals when non-null) by bits [bit_lo, bit_hi).  Returns
-// true when the result is in the *_alt buffers.
+// true when the result is in the *_alt buffers.  With m_dev, the count is
+// min(*m_dev, m) read on the device (m is then the capacity): no host sync,
+// and the launches are the same for every count (graph capture).
 bool radix_sort_u64(hipStream_t st, uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt,
-                    uint64_t m, int bit_lo, int bit_hi, void* temp) {
+                    uint64_t m, int bit_lo, int bit_hi, void* temp, const uint64_t* m_dev) {
     if (m == 0 || bit_hi <= bit_lo) return false;
     uint32_t nblk = (uint32_t)((m + kSortBlock - 1) / kSortBlock);
     uint64_t hist_n = (uint64_t)kRadix * nblk;
@@ -256,16 +264,17 @@ bool radix_sort_u64(hipStream_t st, uint64_t* keys, uint64_t* keys_alt, uint32_t
         int shift = bit_lo + p * per;
         int bits = (shift + per <= bit_hi) ? per : (bit_hi - shift);
         uint32_t mask = (1u << bits) - 1u;
-        hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(kSortThreads), 0, st, (const uint64_t*)kin, m, shift, mask,
-                           nblk, hist);
+        hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(kSortThreads), 0, st, (const uint64_t*)kin, m, m_dev, shift,
+                           mask, nblk, hist);
         exclusive_scan_u32(st, hist, offs, hist_n, scan_tmp);
         if (vals)
             hipLaunchKernelGGL(k_radix_scatter<true>, dim3(nblk), dim3(kSortThreads), 0, st, (const uint64_t*)kin,
-                               (const uint32_t*)vin, m, shift, mask, bits, nblk, (const uint32_t*)offs, kout, vout);
+                               (const uint32_t*)vin, m, m_dev, shift, mask, bits, nblk, (const uint32_t*)offs, kout,
+                               vout);
         else
             hipLaunchKernelGGL(k_radix_scatter<false>, dim3(nblk), dim3(kSortThreads), 0, st, (const uint64_t*)kin,
-                               (const uint32_t*)nullptr, m, shift, mask, bits, nblk, (const uint32_t*)offs, kout,
-                               (uint32_t*)nullptr);
+                               (const uint32_t*)nullptr, m, m_dev, shift, mask, bits, nblk, (const uint32_t*)offs,
+                               kout, (uint32_t*)nullptr);
         uint64_t* tk = kin; kin = kout; kout = tk;
         uint32_t* tv = vin; vin = vout; vout = tv;
         alt = !alt;

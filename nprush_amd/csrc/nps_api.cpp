@@ -34,11 +34,34 @@ struct DevBuf {
 struct Scratch {
     DevBuf counts, rects, offsets, keys, keys_alt, ranges, scan_tmp, sort_tmp;
     DevBuf diff, cut, full, kept;  // per-tile id-block cut of capped (REF) lists
-    DevBuf dkey, dkey_alt, rank_counts;  // EXT: (depth bits << 32 | id) per splat, tile counts by rank
     uint64_t m = 0;
     bool in_alt = false;
     bool have_full = false;
     uint32_t splat_block = 0;
+};
+
+// Buffers of one EXT view pipeline: the handle's own (nps_backward,
+// nps_forward) and one set per lane of a multi-view step.
+struct ExtView {
+    DevBuf rec, counts, rects, dkey, dkey_alt, rank_counts, offsets, scan_tmp, keys, keys_alt, sort_tmp, ranges;
+    DevBuf order, tile_work, ckpt, act, g2, c_img, tile_loss, stats, flags;  // flags: [0] key overflow
+    uint64_t m = 0;    // key count of the last eager run
+    uint64_t cap = 0;  // key capacity of a captured run
+    bool in_alt = false;
+    uint32_t num_tiles = 0;
+    uint32_t* flags_at = nullptr;  // flags in a shared buffer instead of `flags` (16 words)
+};
+
+void free_ext_view(ExtView& x);
+
+// The splats an EXT view pipeline reads and the gradients it adds into.
+struct SceneRef {
+    const float* splats;
+    const float* sh;
+    uint32_t n;
+    int deg;
+    float* gradx;
+    float* accum;
 };
 
 }  // namespace
@@ -68,7 +91,8 @@ struct nps_ctx {
     bool has_target = false;
     DevBuf tgt_rgba, tgt_depth;
     DevBuf fwd, bwd, ckpt, c_img, d_img, tile_loss, stats;
-    DevBuf act, g2, order, tile_work;  // EXT: activity masks, 2D gradients (FP64), tile order, per-tile work
+    ExtView xv;      // EXT: buffers of nps_backward / nps_forward
+    DevBuf adam_dev;  // EXT: Adam step counter and bias corrections on the device (t, 1-b1^t, 1-b2^t, skip)
     Scratch sc;
     uint32_t last_tiles = 0;
     bool have_backward = false;
@@ -110,11 +134,16 @@ int set_err(nps_ctx* h, int code, const std::string& msg) {
         if (e_ != hipSuccess) return set_err((h), NPS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-int ensure(nps_ctx* h, DevBuf& b, size_t bytes) {
+// Grows b to at least `bytes` (25 % headroom); a live buffer is freed only
+// after `st` has drained.
+int ensure_dev(DevBuf& b, size_t bytes, hipStream_t st, std::string& err) {
     if (bytes <= b.bytes && b.p) return NPS_OK;
     if (b.p) {
-        hipError_t e = hipStreamSynchronize(h->stream);
-        if (e != hipSuccess) return set_err(h, NPS_ERR_HIP, std::string("sync: ") + hipGetErrorString(e));
+        hipError_t e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            err = std::string("sync: ") + hipGetErrorString(e);
+            return NPS_ERR_HIP;
+        }
         (void)hipFree(b.p);
         b.p = nullptr;
         b.bytes = 0;
@@ -123,11 +152,14 @@ int ensure(nps_ctx* h, DevBuf& b, size_t bytes) {
     hipError_t e = hipMalloc(&b.p, alloc);
     if (e != hipSuccess) {
         b.p = nullptr;
-        return set_err(h, NPS_ERR_HIP, "hipMalloc(" + std::to_string(alloc) + "): " + hipGetErrorString(e));
+        err = "hipMalloc(" + std::to_string(alloc) + "): " + hipGetErrorString(e);
+        return NPS_ERR_HIP;
     }
     b.bytes = alloc;
     return NPS_OK;
 }
+
+int ensure(nps_ctx* h, DevBuf& b, size_t bytes) { return ensure_dev(b, bytes, h->stream, h->err); }
 
 void free_buf(DevBuf& b) {
     if (b.p) (void)hipFree(b.p);
@@ -139,7 +171,13 @@ void free_scratch(Scratch& s) {
     free_buf(s.counts); free_buf(s.rects); free_buf(s.offsets); free_buf(s.keys);
     free_buf(s.keys_alt); free_buf(s.ranges); free_buf(s.scan_tmp); free_buf(s.sort_tmp);
     free_buf(s.diff); free_buf(s.cut); free_buf(s.full); free_buf(s.kept);
-    free_buf(s.dkey); free_buf(s.dkey_alt); free_buf(s.rank_counts);
+}
+
+void free_ext_view(ExtView& x) {
+    DevBuf* all[] = {&x.rec, &x.counts, &x.rects, &x.dkey, &x.dkey_alt, &x.rank_counts, &x.offsets, &x.scan_tmp,
+                     &x.keys, &x.keys_alt, &x.sort_tmp, &x.ranges, &x.order, &x.tile_work, &x.ckpt, &x.act,
+                     &x.g2, &x.c_img, &x.tile_loss, &x.stats, &x.flags};
+    for (DevBuf* b : all) free_buf(*b);
 }
 
 // Ids per block of the per-tile cut: about 32 blocks, at least 4096 splats each.
@@ -202,100 +240,158 @@ int build_tile_lists(nps_ctx* h, Scratch& sc, const ViewParams& v, uint32_t n, i
     return NPS_OK;
 }
 
-// EXT view: preprocess -> depth order of the splats -> keys in that order ->
-// stable sort on the tile bits -> ranges -> composite (-> chain rule).
-int run_ext(nps_ctx* h, bool grads) {
-    NPS_HIP(h, hipSetDevice(h->device));
-    const uint32_t n = h->n;
+// Ordering hooks of a captured multi-view step: the chain-rule kernel of a
+// view waits for `chain_wait` (the previous view of the same scene) and
+// records `chain_done`, so gradients add up in a fixed view order.
+struct ChainHooks {
+    hipEvent_t chain_wait = nullptr;
+    hipEvent_t chain_done = nullptr;
+};
+
+// Key-buffer capacity of a captured view with `m` keys measured: 25 % + 64K headroom.
+uint64_t ext_key_capacity(uint64_t m) { return m + m / 4 + 65536; }
+
+// Allocates every buffer of view x for `n` splats, a W x H view and `cap` keys
+// (capture mode needs them all in place before the capture starts).
+int ext_view_reserve(ExtView& x, uint32_t n, uint32_t W, uint32_t H, uint64_t cap, bool grads, hipStream_t st,
+                     std::string& err) {
     const size_t n1 = std::max<uint32_t>(n, 1);
-    ViewParams v = make_view(h->vp, h->vp_inv, h->W, h->H);
-    // camera centre: VP^-1 (0, 0, 1, 0) dehomogenised, in double, then rounded
-    float cam[3];
-    for (int k = 0; k < 3; ++k) cam[k] = (float)((double)h->vp_inv[8 + k] / (double)h->vp_inv[11]);
-    Scratch& sc = h->sc;
+    const uint32_t tx = (W + kTile - 1) / kTile, ty = (H + kTile - 1) / kTile, nt = std::max<uint32_t>(tx * ty, 1);
+    const size_t c1 = std::max<uint64_t>(cap, 1);
+    const size_t slots = (size_t)(cap / kBatch) + nt + 1;
+    struct {
+        DevBuf* b;
+        size_t bytes;
+    } need[] = {
+        {&x.rec, sizeof(ExtRec) * n1}, {&x.counts, 4 * n1}, {&x.rects, 16 * n1}, {&x.dkey, 8 * n1},
+        {&x.dkey_alt, 8 * n1}, {&x.rank_counts, 4 * n1}, {&x.offsets, 8 * (n1 + 1)},
+        {&x.scan_tmp, scan_temp_bytes(n1) + 64}, {&x.stats, 64}, {&x.flags, 64},
+        {&x.sort_tmp, radix_temp_bytes(std::max<uint64_t>(c1, n1))}, {&x.keys, 8 * c1}, {&x.keys_alt, 8 * c1},
+        {&x.ranges, 8 * (size_t)nt}, {&x.order, 4 * (size_t)ext_order_slots(tx, ty)}, {&x.tile_work, 8 * (size_t)nt},
+        {&x.c_img, 12 * (size_t)W * H}, {&x.tile_loss, 16 * (size_t)nt},
+    };
     int rc;
-    if ((rc = ensure(h, h->fwd, sizeof(ExtRec) * n1)) != NPS_OK) return rc;
-    if ((rc = ensure(h, sc.counts, 4 * n1)) != NPS_OK) return rc;
-    if ((rc = ensure(h, sc.rects, 16 * n1)) != NPS_OK) return rc;
-    if ((rc = ensure(h, sc.dkey, 8 * n1)) != NPS_OK) return rc;
-    if ((rc = ensure(h, sc.dkey_alt, 8 * n1)) != NPS_OK) return rc;
-    if ((rc = ensure(h, sc.rank_counts, 4 * n1)) != NPS_OK) return rc;
-    if ((rc = ensure(h, sc.sort_tmp, radix_temp_bytes(n1))) != NPS_OK) return rc;
-    if ((rc = ensure(h, sc.offsets, sizeof(uint64_t) * (n1 + 1))) != NPS_OK) return rc;
-    if ((rc = ensure(h, sc.scan_tmp, scan_temp_bytes(n) + 64)) != NPS_OK) return rc;
-    if ((rc = ensure(h, h->stats, 64)) != NPS_OK) return rc;
-    if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[0], h->stream));
-    launch_fill_u32(h->stream, (uint32_t*)h->stats.p, 0u, 16);
-    launch_ext_preprocess(h->stream, h->splats, h->sh, n, h->deg, v, cam, (ExtRec*)h->fwd.p, (uint32_t*)sc.counts.p,
-                          (uint4*)sc.rects.p, (uint64_t*)sc.dkey.p);
-    // rank of every splat in (depth, id) order: stable sort of depth << 32 | id on the depth bits
-    const bool dalt = radix_sort_u64(h->stream, (uint64_t*)sc.dkey.p, (uint64_t*)sc.dkey_alt.p, nullptr, nullptr, n,
-                                     32, 64, sc.sort_tmp.p);
-    const uint64_t* order = (const uint64_t*)(dalt ? sc.dkey_alt.p : sc.dkey.p);
-    launch_ext_rank_counts(h->stream, n, order, (const uint32_t*)sc.counts.p, (uint32_t*)sc.rank_counts.p);
-    exclusive_scan_u32_to_u64(h->stream, (const uint32_t*)sc.rank_counts.p, (uint64_t*)sc.offsets.p, n,
-                              sc.scan_tmp.p);
-    uint64_t m = 0;
-    NPS_HIP(h, hipMemcpyAsync(&m, (uint64_t*)sc.offsets.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost, h->stream));
-    NPS_HIP(h, hipStreamSynchronize(h->stream));
-    if (m >= 0xFFFFFFFFull) return set_err(h, NPS_ERR_CAPACITY, "tile key count exceeds 2^32");
-    sc.m = m;
-    sc.have_full = false;
-    const size_t m1 = std::max<uint64_t>(m, 1);
-    if ((rc = ensure(h, sc.keys, 8 * m1)) != NPS_OK) return rc;
-    if ((rc = ensure(h, sc.keys_alt, 8 * m1)) != NPS_OK) return rc;
-    if ((rc = ensure(h, sc.sort_tmp, radix_temp_bytes(std::max<uint64_t>(m, n1)))) != NPS_OK) return rc;
-    if ((rc = ensure(h, sc.ranges, sizeof(uint32_t) * 2 * (size_t)std::max<uint32_t>(v.num_tiles, 1))) != NPS_OK)
-        return rc;
-    // batch slots of the composite: tile t uses [start/64 + t, ...), at most ceil(len/64) of them
-    const size_t slots = (size_t)(m / kBatch) + v.num_tiles + 1;
+    for (auto& q : need)
+        if ((rc = ensure_dev(*q.b, q.bytes, st, err)) != NPS_OK) return rc;
     if (grads) {
-        if ((rc = ensure(h, h->ckpt, sizeof(float) * slots * 8 * kTilePx)) != NPS_OK) return rc;
-        if ((rc = ensure(h, h->act, sizeof(uint64_t) * slots * 4)) != NPS_OK) return rc;
-        if ((rc = ensure(h, h->g2, sizeof(double) * 9 * n1)) != NPS_OK) return rc;
+        if ((rc = ensure_dev(x.ckpt, sizeof(float) * slots * 8 * kTilePx, st, err)) != NPS_OK) return rc;
+        if ((rc = ensure_dev(x.act, sizeof(uint64_t) * slots * 4, st, err)) != NPS_OK) return rc;
+        if ((rc = ensure_dev(x.g2, sizeof(double) * 9 * n1, st, err)) != NPS_OK) return rc;
     }
-    const size_t px = (size_t)h->W * h->H;
-    if ((rc = ensure(h, h->c_img, 12 * px)) != NPS_OK) return rc;
-    if ((rc = ensure(h, h->tile_loss, 16 * (size_t)std::max<uint32_t>(v.num_tiles, 1))) != NPS_OK) return rc;
-    if ((rc = ensure(h, h->order, 4 * (size_t)ext_order_slots(v.tiles_x, v.tiles_y))) != NPS_OK) return rc;
-    if ((rc = ensure(h, h->tile_work, 8 * (size_t)std::max<uint32_t>(v.num_tiles, 1))) != NPS_OK) return rc;
-    launch_ext_keys(h->stream, n, order, (const uint32_t*)sc.rank_counts.p, (const uint4*)sc.rects.p, v.tiles_x,
-                    (const uint64_t*)sc.offsets.p, (uint64_t*)sc.keys.p);
-    if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[1], h->stream));
+    return NPS_OK;
+}
+
+// One EXT view: preprocess -> depth order of the splats -> keys in that order
+// -> stable sort on the tile bits -> ranges -> composite (-> chain rule).
+// Eager: the key count is read back (one host sync) and buffers grow to fit.
+// Capture: nothing is allocated or read back; the count stays on the device
+// and keys beyond x.cap are dropped with flags[0] set.
+// ev: optional 8 timing events (eager only).
+int run_ext_view(const SceneRef& sc, const ViewParams& v, const float cam[3], const float4* tgt, ExtView& x,
+                 hipStream_t st, bool grads, bool capture, std::string& err, const ChainHooks* hooks,
+                 hipEvent_t* ev) {
+    const uint32_t n = sc.n;
+    int rc;
+    auto hip = [&](hipError_t e, const char* what) {
+        if (e == hipSuccess) return NPS_OK;
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return NPS_ERR_HIP;
+    };
+    if (!capture && (rc = ext_view_reserve(x, n, v.W, v.H, 0, grads, st, err)) != NPS_OK) return rc;
+    if (ev && (rc = hip(hipEventRecord(ev[0], st), "event")) != NPS_OK) return rc;
+    launch_fill_u32(st, (uint32_t*)x.stats.p, 0u, 16);
+    // flags: cleared per view when eager; a captured step clears them once, so a
+    // dropped key anywhere in a lane survives to the Adam update
+    uint32_t* flags = x.flags_at ? x.flags_at : (uint32_t*)x.flags.p;
+    if (!capture) launch_fill_u32(st, flags, 0u, 16);
+    launch_ext_preprocess(st, sc.splats, sc.sh, n, sc.deg, v, cam, (ExtRec*)x.rec.p, (uint32_t*)x.counts.p,
+                          (uint4*)x.rects.p, (uint64_t*)x.dkey.p);
+    // rank of every splat in (depth, id) order: stable sort of depth << 32 | id on the depth bits
+    const bool dalt = radix_sort_u64(st, (uint64_t*)x.dkey.p, (uint64_t*)x.dkey_alt.p, nullptr, nullptr, n, 32, 64,
+                                     x.sort_tmp.p);
+    const uint64_t* order = (const uint64_t*)(dalt ? x.dkey_alt.p : x.dkey.p);
+    launch_ext_rank_counts(st, n, order, (const uint32_t*)x.counts.p, (uint32_t*)x.rank_counts.p);
+    exclusive_scan_u32_to_u64(st, (const uint32_t*)x.rank_counts.p, (uint64_t*)x.offsets.p, n, x.scan_tmp.p);
+    uint64_t cap;
+    const uint64_t* m_dev = nullptr;
+    if (!capture) {
+        uint64_t m = 0;
+        if ((rc = hip(hipMemcpyAsync(&m, (uint64_t*)x.offsets.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost, st),
+                      "count readback")) != NPS_OK)
+            return rc;
+        if ((rc = hip(hipStreamSynchronize(st), "sync")) != NPS_OK) return rc;
+        if (m >= 0xFFFFFFFFull) {
+            err = "tile key count exceeds 2^32";
+            return NPS_ERR_CAPACITY;
+        }
+        x.m = m;
+        cap = m;
+        if ((rc = ext_view_reserve(x, n, v.W, v.H, m, grads, st, err)) != NPS_OK) return rc;
+    } else {
+        cap = x.cap;
+        m_dev = (const uint64_t*)x.offsets.p + n;
+    }
+    x.num_tiles = v.num_tiles;
+    launch_ext_keys(st, n, order, (const uint32_t*)x.rank_counts.p, (const uint4*)x.rects.p, v.tiles_x,
+                    (const uint64_t*)x.offsets.p, (uint64_t*)x.keys.p, cap, flags);
+    if (ev && (rc = hip(hipEventRecord(ev[1], st), "event")) != NPS_OK) return rc;
     const int tile_bits = bits_for(v.num_tiles > 0 ? v.num_tiles - 1 : 0);
-    sc.in_alt = radix_sort_u64(h->stream, (uint64_t*)sc.keys.p, (uint64_t*)sc.keys_alt.p, nullptr, nullptr, m, 32,
-                               32 + tile_bits, sc.sort_tmp.p);
-    if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[2], h->stream));
-    launch_fill_u32(h->stream, (uint32_t*)sc.ranges.p, 0u, 2 * (uint64_t)v.num_tiles);
-    launch_tile_ranges(h->stream, (const uint64_t*)(sc.in_alt ? sc.keys_alt.p : sc.keys.p), m,
-                       (uint32_t*)sc.ranges.p);
-    if (grads) launch_fill_u32(h->stream, (uint32_t*)h->g2.p, 0u, 18 * (uint64_t)n);
-    launch_tile_order(h->stream, (const uint32_t*)sc.ranges.p, v.tiles_x, v.tiles_y, (uint32_t*)h->order.p);
-    if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[3], h->stream));
+    x.in_alt = radix_sort_u64(st, (uint64_t*)x.keys.p, (uint64_t*)x.keys_alt.p, nullptr, nullptr, cap, 32,
+                              32 + tile_bits, x.sort_tmp.p, m_dev);
+    if (ev && (rc = hip(hipEventRecord(ev[2], st), "event")) != NPS_OK) return rc;
+    const uint64_t* sorted = (const uint64_t*)(x.in_alt ? x.keys_alt.p : x.keys.p);
+    launch_fill_u32(st, (uint32_t*)x.ranges.p, 0u, 2 * (uint64_t)v.num_tiles);
+    launch_tile_ranges(st, sorted, cap, (uint32_t*)x.ranges.p, m_dev);
+    if (grads) launch_fill_u32(st, (uint32_t*)x.g2.p, 0u, 18 * (uint64_t)n);
+    launch_tile_order(st, (const uint32_t*)x.ranges.p, v.tiles_x, v.tiles_y, (uint32_t*)x.order.p);
+    if (ev && (rc = hip(hipEventRecord(ev[3], st), "event")) != NPS_OK) return rc;
     ExtCompositeArgs a;
-    a.rec = (const ExtRec*)h->fwd.p;
-    a.keys = (const uint64_t*)(sc.in_alt ? sc.keys_alt.p : sc.keys.p);
-    a.ranges = (const uint32_t*)sc.ranges.p;
-    a.tgt_rgba = (const float4*)h->tgt_rgba.p;
-    a.g2 = grads ? (double*)h->g2.p : nullptr;
-    a.c_img = (float*)h->c_img.p;
-    a.tile_loss = (double*)h->tile_loss.p;
-    a.ckpt = grads ? (float*)h->ckpt.p : nullptr;
-    a.act = grads ? (uint64_t*)h->act.p : nullptr;
-    a.stats = (unsigned long long*)h->stats.p;
-    a.order = (const uint32_t*)h->order.p;
-    a.tile_work = (uint32_t*)h->tile_work.p;
+    a.rec = (const ExtRec*)x.rec.p;
+    a.keys = sorted;
+    a.ranges = (const uint32_t*)x.ranges.p;
+    a.tgt_rgba = tgt;
+    a.g2 = grads ? (double*)x.g2.p : nullptr;
+    a.c_img = (float*)x.c_img.p;
+    a.tile_loss = (double*)x.tile_loss.p;
+    a.ckpt = grads ? (float*)x.ckpt.p : nullptr;
+    a.act = grads ? (uint64_t*)x.act.p : nullptr;
+    a.stats = (unsigned long long*)x.stats.p;
+    a.order = (const uint32_t*)x.order.p;
+    a.tile_work = (uint32_t*)x.tile_work.p;
     a.backward = grads ? 1 : 0;
-    launch_composite_ext(h->stream, v, a);
-    if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[4], h->stream));
-    if (grads)
-        launch_ext_chain(h->stream, h->splats, h->sh, n, h->deg, v, cam, (const double*)h->g2.p,
-                         (const uint32_t*)sc.counts.p, h->gradx, h->accum);
-    if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[7], h->stream));
-    NPS_HIP(h, hipGetLastError());
+    launch_composite_ext(st, v, a);
+    if (ev && (rc = hip(hipEventRecord(ev[4], st), "event")) != NPS_OK) return rc;
+    if (grads) {
+        if (hooks && hooks->chain_wait && (rc = hip(hipStreamWaitEvent(st, hooks->chain_wait, 0), "wait")) != NPS_OK)
+            return rc;
+        launch_ext_chain(st, sc.splats, sc.sh, n, sc.deg, v, cam, (const double*)x.g2.p,
+                         (const uint32_t*)x.counts.p, sc.gradx, sc.accum);
+        if (hooks && hooks->chain_done && (rc = hip(hipEventRecord(hooks->chain_done, st), "event")) != NPS_OK)
+            return rc;
+    }
+    if (ev && (rc = hip(hipEventRecord(ev[7], st), "event")) != NPS_OK) return rc;
+    return hip(hipGetLastError(), "launch");
+}
+
+// Camera centre of a view: VP^-1 (0, 0, 1, 0) dehomogenised, in double, then rounded.
+void ext_camera_centre(const float* vp_inv, float cam[3]) {
+    for (int k = 0; k < 3; ++k) cam[k] = (float)((double)vp_inv[8 + k] / (double)vp_inv[11]);
+}
+
+SceneRef scene_of(nps_ctx* h) { return SceneRef{h->splats, h->sh, h->n, h->deg, h->gradx, h->accum}; }
+
+// nps_backward / nps_forward in EXT mode: the handle's camera and target.
+int run_ext(nps_ctx* h, bool grads) {
+    if (hipSetDevice(h->device) != hipSuccess) return set_err(h, NPS_ERR_HIP, "hipSetDevice");
+    ViewParams v = make_view(h->vp, h->vp_inv, h->W, h->H);
+    float cam[3];
+    ext_camera_centre(h->vp_inv, cam);
+    int rc = run_ext_view(scene_of(h), v, cam, (const float4*)h->tgt_rgba.p, h->xv, h->stream, grads, false, h->err,
+                          nullptr, h->timing ? h->ev : nullptr);
+    if (rc != NPS_OK) return rc;
     h->last_tiles = v.num_tiles;
     h->have_backward = true;
-    h->stats_host[0] = (int64_t)m;
+    h->stats_host[0] = (int64_t)h->xv.m;
     if (h->timing) {
         NPS_HIP(h, hipEventSynchronize(h->ev[7]));
         float t;
@@ -348,6 +444,7 @@ int nps_create(const nps_config* cfg, int32_t num_splats, nps_handle* out) {
         if (h->accum) (void)hipFree(h->accum);
         if (h->sh) (void)hipFree(h->sh);
         if (h->gradx) (void)hipFree(h->gradx);
+        if (h->adam_dev.p) (void)hipFree(h->adam_dev.p);
         if (h->stream) (void)hipStreamDestroy(h->stream);
         delete h;
         return NPS_ERR_HIP;
@@ -368,6 +465,9 @@ int nps_create(const nps_config* cfg, int32_t num_splats, nps_handle* out) {
     hipMemsetAsync(h->v, 0, nopt * 4, h->stream);
     hipMemsetAsync(h->accum, 0, nacc * 4, h->stream);
     if (h->ext) {
+        if ((e = hipMalloc(&h->adam_dev.p, 16)) != hipSuccess) return fail(e);
+        h->adam_dev.bytes = 16;
+        hipMemsetAsync(h->adam_dev.p, 0, 16, h->stream);
         if ((e = hipMalloc((void**)&h->sh, 48 * n1 * 4)) != hipSuccess) return fail(e);
         if ((e = hipMalloc((void**)&h->gradx, 64 * n1 * 4)) != hipSuccess) return fail(e);
         hipMemsetAsync(h->sh, 0, 48 * n1 * 4, h->stream);
@@ -391,7 +491,8 @@ void nps_destroy(nps_handle h) {
     if (h->grads) (void)hipFree(h->grads);
     if (h->sh) (void)hipFree(h->sh);
     if (h->gradx) (void)hipFree(h->gradx);
-    free_buf(h->act); free_buf(h->g2); free_buf(h->order); free_buf(h->tile_work);
+    free_ext_view(h->xv);
+    free_buf(h->adam_dev);
     free_buf(h->tgt_rgba); free_buf(h->tgt_depth); free_buf(h->fwd); free_buf(h->bwd); free_buf(h->ckpt);
     free_buf(h->c_img); free_buf(h->d_img); free_buf(h->tile_loss); free_buf(h->stats);
     free_buf(h->adc_u32); free_buf(h->adc_tmp); free_buf(h->adc_scan);
@@ -460,6 +561,10 @@ int nps_write_adam(nps_handle h, const float* m, const float* v, float t, int32_
     NPS_COPY(h, h->m, m, per * (size_t)n, hipMemcpyHostToDevice);
     NPS_COPY(h, h->v, v, per * (size_t)n, hipMemcpyHostToDevice);
     h->adam_t = t;
+    if (h->ext) {
+        const float ctl[4] = {t, 0.0f, 0.0f, 0.0f};
+        NPS_COPY(h, h->adam_dev.p, ctl, sizeof(ctl), hipMemcpyHostToDevice);
+    }
     return NPS_OK;
 }
 int nps_read_adam(nps_handle h, float* m, float* v, float* t, int32_t n) {
@@ -627,7 +732,9 @@ int nps_step(nps_handle h) {
     NPS_HIP(h, hipSetDevice(h->device));
     float t = h->adam_t + 1.0f;
     if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[5], h->stream));
-    if (h->ext) launch_adam_ext(h->stream, h->splats, h->sh, h->gradx, h->m, h->v, h->n, h->deg, t);
+    if (h->ext)
+        launch_adam_ext(h->stream, h->splats, h->sh, h->gradx, h->m, h->v, h->n, h->deg, (float*)h->adam_dev.p,
+                        nullptr, 0);
     else launch_adam_ref(h->stream, h->splats, h->grads, h->m, h->v, h->accum, h->n, t);
     if (h->timing) {
         NPS_HIP(h, hipEventRecord(h->ev[6], h->stream));
@@ -742,7 +849,7 @@ int nps_read_image(nps_handle h, float* c_pred, float* d_pred) {
     if (!h) return NPS_ERR_ARG;
     if (!h->have_backward) return set_err(h, NPS_ERR_STATE, "no backward has run");
     size_t px = (size_t)h->W * h->H;
-    if (c_pred) NPS_COPY(h, c_pred, h->c_img.p, 12 * px, hipMemcpyDeviceToHost);
+    if (c_pred) NPS_COPY(h, c_pred, h->ext ? h->xv.c_img.p : h->c_img.p, 12 * px, hipMemcpyDeviceToHost);
     if (d_pred && h->ext) memset(d_pred, 0, 4 * px);
     else if (d_pred) NPS_COPY(h, d_pred, h->d_img.p, 4 * px, hipMemcpyDeviceToHost);
     return NPS_OK;
@@ -752,7 +859,8 @@ int nps_read_loss(nps_handle h, double loss[2]) {
     if (!h || !loss) return NPS_ERR_ARG;
     if (!h->have_backward) return set_err(h, NPS_ERR_STATE, "no backward has run");
     std::vector<double> t(2 * (size_t)h->last_tiles);
-    NPS_COPY(h, t.data(), h->tile_loss.p, sizeof(double) * t.size(), hipMemcpyDeviceToHost);
+    NPS_COPY(h, t.data(), h->ext ? h->xv.tile_loss.p : h->tile_loss.p, sizeof(double) * t.size(),
+             hipMemcpyDeviceToHost);
     double a = 0.0, b = 0.0;
     for (size_t k = 0; k < h->last_tiles; ++k) {
         a += t[2 * k];
@@ -768,13 +876,20 @@ int nps_read_tiles(nps_handle h, uint32_t* ranges, uint32_t* ids, uint32_t* full
     if (!h) return NPS_ERR_ARG;
     if (!h->have_backward) return set_err(h, NPS_ERR_STATE, "no backward has run");
     if (num_tiles) *num_tiles = (int32_t)h->last_tiles;
-    if (num_ids) *num_ids = (int64_t)h->sc.m;
+    if (num_ids) *num_ids = (int64_t)(h->ext ? h->xv.m : h->sc.m);
     if (h->ext) {
         std::vector<uint32_t> r(2 * (size_t)h->last_tiles);
-        NPS_COPY(h, r.data(), h->sc.ranges.p, 8 * (size_t)h->last_tiles, hipMemcpyDeviceToHost);
+        NPS_COPY(h, r.data(), h->xv.ranges.p, 8 * (size_t)h->last_tiles, hipMemcpyDeviceToHost);
         if (ranges) memcpy(ranges, r.data(), 8 * (size_t)h->last_tiles);
         if (full_counts)
             for (size_t t = 0; t < h->last_tiles; ++t) full_counts[t] = r[2 * t + 1] - r[2 * t];
+        if (ids && h->xv.m) {
+            std::vector<uint64_t> k(h->xv.m);
+            NPS_COPY(h, k.data(), h->xv.in_alt ? h->xv.keys_alt.p : h->xv.keys.p, 8 * (size_t)h->xv.m,
+                     hipMemcpyDeviceToHost);
+            for (size_t j = 0; j < k.size(); ++j) ids[j] = (uint32_t)k[j];
+        }
+        return NPS_OK;
     } else {
         if (ranges) NPS_COPY(h, ranges, h->sc.ranges.p, 8 * (size_t)h->last_tiles, hipMemcpyDeviceToHost);
         if (full_counts)
@@ -792,15 +907,16 @@ int nps_read_tiles(nps_handle h, uint32_t* ranges, uint32_t* ids, uint32_t* full
 int nps_read_tile_work(nps_handle h, uint32_t* work) {
     if (!h || !work) return NPS_ERR_ARG;
     if (!h->ext || !h->have_backward) return set_err(h, NPS_ERR_STATE, "no EXT backward/forward has run");
-    NPS_COPY(h, work, h->tile_work.p, 8 * (size_t)h->last_tiles, hipMemcpyDeviceToHost);
+    NPS_COPY(h, work, h->xv.tile_work.p, 8 * (size_t)h->last_tiles, hipMemcpyDeviceToHost);
     return NPS_OK;
 }
 
 int nps_read_stats(nps_handle h, int64_t stats[12]) {
     if (!h || !stats) return NPS_ERR_ARG;
-    if (h->have_backward && h->stats.p) {
+    const DevBuf& sb = h->ext ? h->xv.stats : h->stats;
+    if (h->have_backward && sb.p) {
         unsigned long long s[5] = {0, 0, 0, 0, 0};
-        NPS_COPY(h, s, h->stats.p, sizeof(s), hipMemcpyDeviceToHost);
+        NPS_COPY(h, s, sb.p, sizeof(s), hipMemcpyDeviceToHost);
         h->stats_host[2] = (int64_t)s[0];
         h->stats_host[1] = (int64_t)s[1];
         h->stats_host[3] = (int64_t)s[2];
@@ -908,6 +1024,320 @@ int npf_write_target(npf_handle f, const float* color, const float* depth) {
     size_t px = (size_t)f->W * f->H;
     if (color) NPS_COPY(h, f->color.p, color, 16 * px, hipMemcpyHostToDevice);
     if (depth) NPS_COPY(h, f->depth.p, depth, 16 * px, hipMemcpyHostToDevice);
+    return NPS_OK;
+}
+
+// ---------------- multi-view step ----------------
+
+}  // extern "C"
+
+struct npt_view {
+    int scene;
+    float vp[16], vp_inv[16];
+    const float* tgt;
+    uint32_t W, H;
+};
+
+struct npt_lane {
+    hipStream_t st = nullptr;
+    ExtView x;
+};
+
+struct npt_ctx {
+    std::vector<nps_handle> scenes;
+    std::vector<npt_view> views;
+    std::vector<int> order;    // step order of the views (largest first)
+    std::vector<int> lane_of;  // lane of each view
+    std::vector<npt_lane> lanes;
+    int device = 0;
+    hipStream_t main = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> lane_done, chain_done;
+    DevBuf view_loss, flags;  // 2 doubles per view; 16 words per lane
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool planned = false;
+    double cap_scale = 1.0;
+    std::string err;
+};
+
+namespace {
+
+int npt_fail(npt_handle t, int code, const std::string& msg) {
+    t->err = msg;
+    return code;
+}
+
+#define NPT_HIP(t, expr)                                                                                 \
+    do {                                                                                                 \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess) return npt_fail((t), NPS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+ViewParams npt_view_params(const npt_view& w) { return make_view(w.vp, w.vp_inv, w.W, w.H); }
+
+void npt_drop_graph(npt_handle t) {
+    if (t->exec) (void)hipGraphExecDestroy(t->exec);
+    if (t->graph) (void)hipGraphDestroy(t->graph);
+    t->exec = nullptr;
+    t->graph = nullptr;
+    t->planned = false;
+}
+
+// Adam (and densification on its cadence) for every scene after the views.
+int npt_finish_scenes(npt_handle t, bool adam_done) {
+    for (nps_handle h : t->scenes) {
+        if (!adam_done) {
+            launch_adam_ext(h->stream, h->splats, h->sh, h->gradx, h->m, h->v, h->n, h->deg, (float*)h->adam_dev.p,
+                            nullptr, 0);
+            NPT_HIP(t, hipGetLastError());
+        }
+        h->adam_t += 1.0f;
+        h->step_count++;
+        if (h->cfg.adc_enabled && h->step_count % h->cfg.adc_period == 0) {
+            NPT_HIP(t, hipStreamSynchronize(h->stream));
+            int rc = nps_adc(h);
+            if (rc != NPS_OK) return npt_fail(t, rc, h->err);
+        }
+        NPT_HIP(t, hipStreamSynchronize(h->stream));
+    }
+    return NPS_OK;
+}
+
+// Views one after another, each with its own launches and key-count readback.
+int npt_step_sequential(npt_handle t) {
+    for (nps_handle h : t->scenes) NPT_HIP(t, hipStreamSynchronize(h->stream));
+    for (int vi : t->order) {
+        const npt_view& w = t->views[vi];
+        nps_handle h = t->scenes[w.scene];
+        ViewParams v = npt_view_params(w);
+        float cam[3];
+        ext_camera_centre(w.vp_inv, cam);
+        int rc = run_ext_view(scene_of(h), v, cam, (const float4*)w.tgt, h->xv, h->stream, true, false, t->err,
+                              nullptr, nullptr);
+        if (rc != NPS_OK) return rc;
+        launch_sum_tile_loss(h->stream, (const double*)h->xv.tile_loss.p, v.num_tiles,
+                             (double*)t->view_loss.p + 2 * vi);
+    }
+    return npt_finish_scenes(t, false);
+}
+
+// Sizes every lane for its views (key counts measured by a forward-only eager
+// run) and captures the whole step as one graph.
+int npt_plan(npt_handle t) {
+    npt_drop_graph(t);
+    const int L = (int)t->lanes.size();
+    std::vector<uint32_t> nmax(L, 1), wmax(L, 1), hmax(L, 1);
+    std::vector<uint64_t> mmax(L, 0);
+    for (int vi : t->order) {
+        const npt_view& w = t->views[vi];
+        nps_handle h = t->scenes[w.scene];
+        npt_lane& ln = t->lanes[t->lane_of[vi]];
+        ViewParams v = npt_view_params(w);
+        float cam[3];
+        ext_camera_centre(w.vp_inv, cam);
+        int rc = run_ext_view(scene_of(h), v, cam, (const float4*)w.tgt, ln.x, ln.st, false, false, t->err, nullptr,
+                              nullptr);
+        if (rc != NPS_OK) return rc;
+        const int l = t->lane_of[vi];
+        nmax[l] = std::max(nmax[l], h->n);
+        wmax[l] = std::max(wmax[l], w.W);
+        hmax[l] = std::max(hmax[l], w.H);
+        mmax[l] = std::max(mmax[l], ln.x.m);
+    }
+    for (int l = 0; l < L; ++l) {
+        npt_lane& ln = t->lanes[l];
+        ln.x.cap = (uint64_t)((double)ext_key_capacity(mmax[l]) * t->cap_scale);
+        ln.x.flags_at = (uint32_t*)t->flags.p + 16 * l;
+        int rc = ext_view_reserve(ln.x, nmax[l], wmax[l], hmax[l], ln.x.cap, true, ln.st, t->err);
+        if (rc != NPS_OK) return rc;
+        NPT_HIP(t, hipStreamSynchronize(ln.st));
+    }
+    for (nps_handle h : t->scenes) NPT_HIP(t, hipStreamSynchronize(h->stream));
+
+    NPT_HIP(t, hipStreamBeginCapture(t->main, hipStreamCaptureModeThreadLocal));
+    int rc = NPS_OK;
+    [&]() {
+        launch_fill_u32(t->main, (uint32_t*)t->flags.p, 0u, 16 * (uint64_t)t->lanes.size());
+        if (hipEventRecord(t->fork, t->main) != hipSuccess) { rc = npt_fail(t, NPS_ERR_HIP, "fork"); return; }
+        for (auto& ln : t->lanes)
+            if (hipStreamWaitEvent(ln.st, t->fork, 0) != hipSuccess) { rc = npt_fail(t, NPS_ERR_HIP, "join"); return; }
+        std::vector<int> last_chain(t->scenes.size(), -1);
+        for (int vi : t->order) {
+            const npt_view& w = t->views[vi];
+            nps_handle h = t->scenes[w.scene];
+            npt_lane& ln = t->lanes[t->lane_of[vi]];
+            ViewParams v = npt_view_params(w);
+            float cam[3];
+            ext_camera_centre(w.vp_inv, cam);
+            ChainHooks hooks;
+            hooks.chain_wait = last_chain[w.scene] >= 0 ? t->chain_done[last_chain[w.scene]] : nullptr;
+            hooks.chain_done = t->chain_done[vi];
+            rc = run_ext_view(scene_of(h), v, cam, (const float4*)w.tgt, ln.x, ln.st, true, true, t->err, &hooks,
+                              nullptr);
+            if (rc != NPS_OK) return;
+            launch_sum_tile_loss(ln.st, (const double*)ln.x.tile_loss.p, v.num_tiles,
+                                 (double*)t->view_loss.p + 2 * vi);
+            last_chain[w.scene] = vi;
+        }
+        for (size_t l = 0; l < t->lanes.size(); ++l) {
+            if (hipEventRecord(t->lane_done[l], t->lanes[l].st) != hipSuccess ||
+                hipStreamWaitEvent(t->main, t->lane_done[l], 0) != hipSuccess) {
+                rc = npt_fail(t, NPS_ERR_HIP, "join");
+                return;
+            }
+        }
+        for (nps_handle h : t->scenes)
+            launch_adam_ext(t->main, h->splats, h->sh, h->gradx, h->m, h->v, h->n, h->deg, (float*)h->adam_dev.p,
+                            (const uint32_t*)t->flags.p, (uint32_t)t->lanes.size());
+    }();
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(t->main, &g);
+    if (rc != NPS_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return npt_fail(t, NPS_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+    t->graph = g;
+    NPT_HIP(t, hipGraphInstantiate(&t->exec, t->graph, nullptr, nullptr, 0));
+    t->planned = true;
+    return NPS_OK;
+}
+
+int npt_step_graph(npt_handle t) {
+    if (!t->planned) {
+        int rc = npt_plan(t);
+        if (rc != NPS_OK) return rc;
+    }
+    for (nps_handle h : t->scenes) NPT_HIP(t, hipStreamSynchronize(h->stream));
+    NPT_HIP(t, hipGraphLaunch(t->exec, t->main));
+    std::vector<uint32_t> fl(16 * t->lanes.size());
+    NPT_HIP(t, hipMemcpyAsync(fl.data(), t->flags.p, 4 * fl.size(), hipMemcpyDeviceToHost, t->main));
+    NPT_HIP(t, hipStreamSynchronize(t->main));
+    bool overflow = false;
+    for (size_t l = 0; l < t->lanes.size(); ++l) overflow = overflow || fl[16 * l] != 0;
+    if (overflow) {
+        // a view had more keys than its lane holds: Adam was skipped on the
+        // device; clear the partial gradients, grow the lanes, redo the step
+        // view by view and capture again next time
+        for (nps_handle h : t->scenes) {
+            launch_fill_u32(h->stream, (uint32_t*)h->gradx, 0u, 64 * (uint64_t)h->n);
+            NPT_HIP(t, hipStreamSynchronize(h->stream));
+        }
+        t->cap_scale *= 1.5;
+        npt_drop_graph(t);
+        return npt_step_sequential(t);
+    }
+    return npt_finish_scenes(t, true);
+}
+
+}  // namespace
+
+extern "C" {
+
+int npt_create(nps_handle* scenes, int32_t n_scenes, int32_t lanes, npt_handle* out) {
+    if (!out || !scenes || n_scenes <= 0 || lanes <= 0 || lanes > 16) return NPS_ERR_ARG;
+    *out = nullptr;
+    for (int i = 0; i < n_scenes; ++i)
+        if (!scenes[i] || !scenes[i]->ext || scenes[i]->device != scenes[0]->device) return NPS_ERR_ARG;
+    npt_ctx* t = new npt_ctx();
+    t->scenes.assign(scenes, scenes + n_scenes);
+    t->device = scenes[0]->device;
+    auto fail = [&]() {
+        npt_destroy(t);
+        return NPS_ERR_HIP;
+    };
+    if (hipSetDevice(t->device) != hipSuccess) return fail();
+    if (hipStreamCreateWithFlags(&t->main, hipStreamNonBlocking) != hipSuccess) return fail();
+    if (hipEventCreateWithFlags(&t->fork, hipEventDisableTiming) != hipSuccess) return fail();
+    t->lanes.resize(lanes);
+    t->lane_done.assign(lanes, nullptr);
+    for (int l = 0; l < lanes; ++l) {
+        if (hipStreamCreateWithFlags(&t->lanes[l].st, hipStreamNonBlocking) != hipSuccess) return fail();
+        if (hipEventCreateWithFlags(&t->lane_done[l], hipEventDisableTiming) != hipSuccess) return fail();
+    }
+    if (ensure_dev(t->flags, 64 * (size_t)lanes, t->main, t->err) != NPS_OK) return fail();
+    launch_fill_u32(t->main, (uint32_t*)t->flags.p, 0u, 16 * (uint64_t)lanes);
+    if (hipStreamSynchronize(t->main) != hipSuccess) return fail();
+    *out = t;
+    return NPS_OK;
+}
+
+void npt_destroy(npt_handle t) {
+    if (!t) return;
+    (void)hipSetDevice(t->device);
+    if (t->main) (void)hipStreamSynchronize(t->main);
+    npt_drop_graph(t);
+    for (auto& ln : t->lanes) {
+        if (ln.st) {
+            (void)hipStreamSynchronize(ln.st);
+            (void)hipStreamDestroy(ln.st);
+        }
+        free_ext_view(ln.x);
+    }
+    for (hipEvent_t e : t->lane_done)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : t->chain_done)
+        if (e) (void)hipEventDestroy(e);
+    if (t->fork) (void)hipEventDestroy(t->fork);
+    free_buf(t->view_loss);
+    free_buf(t->flags);
+    if (t->main) (void)hipStreamDestroy(t->main);
+    delete t;
+}
+
+const char* npt_last_error(npt_handle t) { return t ? t->err.c_str() : "invalid handle"; }
+
+int npt_add_view(npt_handle t, int32_t scene, const float vp[16], const float vp_inv[16], const float* rgba_dev,
+                 int32_t w, int32_t hgt) {
+    if (!t) return NPS_ERR_ARG;
+    if (scene < 0 || scene >= (int)t->scenes.size() || !vp || !vp_inv || !rgba_dev || w <= 0 || hgt <= 0)
+        return npt_fail(t, NPS_ERR_ARG, "bad view");
+    npt_view v;
+    v.scene = scene;
+    memcpy(v.vp, vp, sizeof(v.vp));
+    memcpy(v.vp_inv, vp_inv, sizeof(v.vp_inv));
+    v.tgt = rgba_dev;
+    v.W = (uint32_t)w;
+    v.H = (uint32_t)hgt;
+    t->views.push_back(v);
+    hipEvent_t e = nullptr;
+    NPT_HIP(t, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    t->chain_done.push_back(e);
+    // step order: most pixels first (longest-processing-time order), then index;
+    // lanes are filled greedily with the least-loaded lane first
+    const int V = (int)t->views.size();
+    t->order.resize(V);
+    for (int i = 0; i < V; ++i) t->order[i] = i;
+    std::stable_sort(t->order.begin(), t->order.end(), [&](int a, int b) {
+        return (uint64_t)t->views[a].W * t->views[a].H > (uint64_t)t->views[b].W * t->views[b].H;
+    });
+    t->lane_of.assign(V, 0);
+    std::vector<uint64_t> load(t->lanes.size(), 0);
+    for (int vi : t->order) {
+        int best = 0;
+        for (int l = 1; l < (int)t->lanes.size(); ++l)
+            if (load[l] < load[best]) best = l;
+        t->lane_of[vi] = best;
+        load[best] += (uint64_t)t->views[vi].W * t->views[vi].H;
+    }
+    if (ensure_dev(t->view_loss, 16 * (size_t)V, t->main, t->err) != NPS_OK) return NPS_ERR_HIP;
+    npt_drop_graph(t);
+    return NPS_OK;
+}
+
+int npt_step(npt_handle t, int32_t mode) {
+    if (!t) return NPS_ERR_ARG;
+    if (t->views.empty()) return npt_fail(t, NPS_ERR_STATE, "no views");
+    NPT_HIP(t, hipSetDevice(t->device));
+    if (mode == 0) return npt_step_sequential(t);
+    if (mode == 1) return npt_step_graph(t);
+    return npt_fail(t, NPS_ERR_ARG, "mode must be 0 (per view) or 1 (graph)");
+}
+
+int npt_read_losses(npt_handle t, double* loss2v) {
+    if (!t || !loss2v) return NPS_ERR_ARG;
+    NPT_HIP(t, hipMemcpy(loss2v, t->view_loss.p, 16 * t->views.size(), hipMemcpyDeviceToHost));
     return NPS_OK;
 }
 

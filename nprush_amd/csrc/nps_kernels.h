@@ -38,7 +38,8 @@ void launch_tile_cut(hipStream_t st, const ViewParams& v, uint32_t n, int32_t* d
 void launch_emit_keys(hipStream_t st, uint32_t n, const uint32_t* counts, const uint64_t* offsets,
                       const uint4* rects, uint32_t tiles_x, uint32_t num_tiles, const uint32_t* cut,
                       uint32_t splat_block, uint64_t* keys);
-void launch_tile_ranges(hipStream_t st, const uint64_t* keys, uint64_t m, uint32_t* ranges);
+void launch_tile_ranges(hipStream_t st, const uint64_t* keys, uint64_t m, uint32_t* ranges,
+                        const uint64_t* m_dev = nullptr);
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint64_t n);
 
 // k_sort.hip
@@ -47,7 +48,7 @@ void exclusive_scan_u32_to_u64(hipStream_t st, const uint32_t* in, uint64_t* out
 void exclusive_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t n, void* temp);
 size_t radix_temp_bytes(uint64_t m);
 bool radix_sort_u64(hipStream_t st, uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt,
-                    uint64_t m, int bit_lo, int bit_hi, void* temp);
+                    uint64_t m, int bit_lo, int bit_hi, void* temp, const uint64_t* m_dev = nullptr);
 
 // k_composite.hip
 void launch_composite_ref(hipStream_t st, const ViewParams& v, const CompositeArgs& a);
@@ -114,13 +115,19 @@ void launch_ext_preprocess(hipStream_t st, const float* splats, const float* sh,
                            uint64_t* dkey);
 void launch_ext_rank_counts(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* counts,
                             uint32_t* rank_counts);
+// keys beyond `cap` are not written; *overflow is then set to 1
 void launch_ext_keys(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts,
-                     const uint4* rects, uint32_t tiles_x, const uint64_t* offsets, uint64_t* keys);
+                     const uint4* rects, uint32_t tiles_x, const uint64_t* offsets, uint64_t* keys, uint64_t cap,
+                     uint32_t* overflow);
 void launch_composite_ext(hipStream_t st, const ViewParams& v, const ExtCompositeArgs& a);
 void launch_ext_chain(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg, const ViewParams& v,
                       const float cam[3], const double* g2, const uint32_t* counts, float* grads, float* accum);
+// ctl = {t, 1 - 0.9^t, 1 - 0.999^t, skip} on the device: t advances by one
+// unless one of the nflags flags (16 words apart) is set, then nothing changes.
 void launch_adam_ext(hipStream_t st, float* splats, float* sh, float* grads, float* m, float* v, uint32_t n, int deg,
-                     float t);
+                     float* ctl, const uint32_t* flags, uint32_t nflags);
+
+void launch_sum_tile_loss(hipStream_t st, const double* tile_loss, uint32_t ntiles, double* out2);
 
 // EXT densification (k_ext.hip)
 struct AdcExtParams {

@@ -330,6 +330,67 @@ class GpuSplatForwardPipelineManager:
         return c, d
 
 
+class GpuMultiViewStep:
+    """Multi-view training step over several EXT scenes (npt_* of the C ABI; no
+    reference counterpart).  ``step(graph=True)`` replays one captured hipGraph
+    for all views + Adam; ``step(graph=False)`` runs view after view."""
+
+    def __init__(self, scenes, lanes: int = 4):
+        lib = _lib.load()
+        self._scenes = list(scenes)
+        arr = (C.c_void_p * len(self._scenes))(*[m.handle for m in self._scenes])
+        t = C.c_void_p()
+        rc = lib.npt_create(arr, len(self._scenes), int(lanes), C.byref(t))
+        if rc != 0:
+            raise _lib.NpsError(f"npt_create failed ({rc})")
+        self._lib = lib
+        self._t = t
+        self._targets = []
+        self.num_views = 0
+
+    def _check(self, rc, what):
+        if rc != 0:
+            m = self._lib.npt_last_error(self._t)
+            raise _lib.NpsError(f"{what} failed ({rc}): {m.decode() if m else ''}")
+
+    def add_view(self, scene: int, vp, vp_inv, target_rgba: np.ndarray):
+        """Uploads the target (H x W x 4) to the scene's GPU and registers the view."""
+        owner = self._scenes[scene]
+        rgba = np.ascontiguousarray(target_rgba, dtype=np.float32)
+        h, w = rgba.shape[:2]
+        p = C.c_void_p()
+        check(self._lib.nps_dev_alloc(owner.handle, rgba.nbytes, C.byref(p)), owner.handle, "nps_dev_alloc")
+        check(self._lib.nps_dev_upload(owner.handle, p, rgba.ctypes.data_as(C.c_void_p), rgba.nbytes), owner.handle,
+              "nps_dev_upload")
+        self._targets.append((owner, p))
+        a = np.ascontiguousarray(vp, dtype=np.float32).reshape(16)
+        b = np.ascontiguousarray(vp_inv, dtype=np.float32).reshape(16)
+        self._check(self._lib.npt_add_view(self._t, int(scene), fptr(a), fptr(b), p, int(w), int(h)), "add_view")
+        self.num_views += 1
+
+    def step(self, graph: bool = True):
+        self._check(self._lib.npt_step(self._t, 1 if graph else 0), "step")
+
+    def read_losses(self) -> np.ndarray:
+        out = np.zeros((self.num_views, 2), dtype=np.float64)
+        self._check(self._lib.npt_read_losses(self._t, dptr(out)), "read_losses")
+        return out
+
+    def close(self):
+        if getattr(self, "_t", None):
+            self._lib.npt_destroy(self._t)
+            self._t = None
+            for owner, p in self._targets:
+                self._lib.nps_dev_free(owner.handle, p)
+            self._targets = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class GpuBlurPipelineManager:
     """new GpuBlurPipelineManager(device) (src/lib/gpu/GpuBlurPipelineManager.ts:9-49)."""
 
