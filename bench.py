@@ -102,6 +102,77 @@ def build_workload_ext(n, W, H, views, radius, deg=3):
     return cams, targets, s, sh
 
 
+def batched_config2(steps=3, warmup=1, lanes=4):
+    """The uneven multi-view workload: 4 scenes of 50k/150k/300k/500k Gaussians
+    (seeds 0-3, SH degree 3), 64 views with resolutions drawn uniformly (seed 4)
+    from {256^2, 512^2, 1024x768, 1920x1080} on radius-3 rings at random angles
+    and heights, targets = the forward image of a second scene per scene (seed
+    100 + s), quantised to k/255.  One step = all 64 views fwd+bwd + Adam per
+    scene.  Returns per-step ms of the captured graph and of view-after-view
+    launches."""
+    from nprush_amd import GpuMultiViewStep, GpuSplatOptimizerManager, camera, scenes
+
+    sizes = [(256, 256), (512, 512), (1024, 768), (1920, 1080)]
+    counts = [50_000, 150_000, 300_000, 500_000]
+    rng = np.random.default_rng(4)
+    views = []
+    for k in range(64):
+        w, h = sizes[int(rng.integers(0, 4))]
+        ang = rng.uniform(0.0, 2.0 * np.pi)
+        eye = (3.0 * np.cos(ang), rng.uniform(-1.0, 1.0), 3.0 * np.sin(ang))
+        views.append((k % 4, w, h) + camera.view_proj(eye, aspect=w / h))
+    targets = []
+    for sidx, n in enumerate(counts):
+        ts, tsh = scenes.make_ext_splats(n, 100 + sidx, sh_degree=3)
+        tm = GpuSplatOptimizerManager(numSplats=n, mode="ext", sh_degree=3, splats=ts, sh=tsh, adc_enabled=False)
+        for (sc, w, h, vp, vi) in views:
+            if sc != sidx:
+                continue
+            tm.setBackwardTarget(np.zeros((h, w, 4), np.float32), np.ones((h, w), np.float32), w, h)
+            tm.writeSplatVPMatrix(vp, vi)
+            tm.forward()
+            c, _ = tm.read_image()
+            rgba = np.ones((h, w, 4), np.float32)
+            rgba[..., :3] = scenes.quantize8(c)
+            targets.append(rgba)
+        tm.close()
+    tgt_of = {}
+    it = iter(targets)
+    for sidx in range(4):
+        for k, v in enumerate(views):
+            if v[0] == sidx:
+                tgt_of[k] = next(it)
+    out = {}
+    for graph in (False, True):
+        mgrs = []
+        for sidx, n in enumerate(counts):
+            s, sh = scenes.make_ext_splats(n, sidx, sh_degree=3)
+            mgrs.append(GpuSplatOptimizerManager(numSplats=n, mode="ext", sh_degree=3, splats=s, sh=sh,
+                                                 adc_enabled=False))
+        step = GpuMultiViewStep(mgrs, lanes=lanes)
+        for k, (sc, w, h, vp, vi) in enumerate(views):
+            step.add_view(sc, vp, vi, tgt_of[k])
+        for _ in range(warmup):
+            step.step(graph=graph)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step.step(graph=graph)
+        out["graph" if graph else "per_view"] = (time.perf_counter() - t0) * 1000.0 / steps
+        losses = step.read_losses()
+        out["l1_" + ("graph" if graph else "per_view")] = float(losses[:, 1].sum())
+        step.close()
+        for m in mgrs:
+            m.close()
+    return {
+        "workload": "4 scenes 50k/150k/300k/500k SH3, 64 views of 256^2/512^2/1024x768/1920x1080 (seed 4)",
+        "lanes": lanes,
+        "graph_ms_per_step": round(out["graph"], 3),
+        "per_view_ms_per_step": round(out["per_view"], 3),
+        "speedup": round(out["per_view"] / out["graph"], 3),
+        "same_losses": out["l1_graph"] == out["l1_per_view"],
+    }
+
+
 def cpu_baseline_ext(s, sh, deg, vp, vi, target, W, H, sample_tiles, seed=0):
     """The EXT oracle (FP32, all host cores): binning of the sampled tiles
     against all splats (depth sort per tile) + forward + L1 + backward + chain
@@ -157,6 +228,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--small", action="store_true", help="16,384 splats at 256x256 (quick check)")
     ap.add_argument("--mode", choices=("ext", "ref"), default="ext")
+    ap.add_argument("--no-batched", action="store_true", help="skip the multi-view (64 views, 4 scenes) comparison")
     ap.add_argument("--cpu-tiles", type=int, default=48, help="tiles in the CPU-baseline sample (0 = skip)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one plain streaming kernel (k_debug_math, 16M floats: 64 MiB read + 64 MiB "
@@ -334,6 +406,10 @@ def main_ext(args):
         from nprush_amd import debug_math
 
         debug_math(0, np.linspace(-8.0, 8.0, 1 << 24, dtype=np.float32))
+    batched = None
+    if not args.small and not args.no_batched:
+        print("[bench] multi-view step: graph vs per-view launches ...", file=sys.stderr, flush=True)
+        batched = batched_config2()
     cpu = None
     if args.cpu_tiles > 0:
         print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
@@ -367,6 +443,7 @@ def main_ext(args):
         "adam_hbm_gbs": round(adam_gbs, 1),
         "adam_roofline_frac": round(adam_gbs / PEAK_HBM_GBS, 3),
         "steps_per_s": round(args.steps / wall, 2),
+        "batched_multiview": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -240,12 +240,14 @@ int build_tile_lists(nps_ctx* h, Scratch& sc, const ViewParams& v, uint32_t n, i
     return NPS_OK;
 }
 
-// Ordering hooks of a captured multi-view step: the chain-rule kernel of a
-// view waits for `chain_wait` (the previous view of the same scene) and
-// records `chain_done`, so gradients add up in a fixed view order.
-struct ChainHooks {
-    hipEvent_t chain_wait = nullptr;
-    hipEvent_t chain_done = nullptr;
+// Where a view's per-splat tile counts and 2D gradients go, and whether its
+// chain-rule kernel runs right away.  A captured multi-view step keeps them
+// per view and runs every scene's chain kernels after the composites, in the
+// step order of its views, on one stream per scene.
+struct ViewOut {
+    uint32_t* counts = nullptr;  // n (default: the view buffers')
+    double* g2 = nullptr;        // 9n (default: the view buffers')
+    bool defer_chain = false;
 };
 
 // Key-buffer capacity of a captured view with `m` keys measured: 25 % + 64K headroom.
@@ -288,8 +290,7 @@ int ext_view_reserve(ExtView& x, uint32_t n, uint32_t W, uint32_t H, uint64_t ca
 // and keys beyond x.cap are dropped with flags[0] set.
 // ev: optional 8 timing events (eager only).
 int run_ext_view(const SceneRef& sc, const ViewParams& v, const float cam[3], const float4* tgt, ExtView& x,
-                 hipStream_t st, bool grads, bool capture, std::string& err, const ChainHooks* hooks,
-                 hipEvent_t* ev) {
+                 hipStream_t st, bool grads, bool capture, std::string& err, const ViewOut* out, hipEvent_t* ev) {
     const uint32_t n = sc.n;
     int rc;
     auto hip = [&](hipError_t e, const char* what) {
@@ -304,13 +305,15 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, const float cam[3], co
     // dropped key anywhere in a lane survives to the Adam update
     uint32_t* flags = x.flags_at ? x.flags_at : (uint32_t*)x.flags.p;
     if (!capture) launch_fill_u32(st, flags, 0u, 16);
-    launch_ext_preprocess(st, sc.splats, sc.sh, n, sc.deg, v, cam, (ExtRec*)x.rec.p, (uint32_t*)x.counts.p,
-                          (uint4*)x.rects.p, (uint64_t*)x.dkey.p);
+    uint32_t* counts = (out && out->counts) ? out->counts : (uint32_t*)x.counts.p;
+    double* g2 = (out && out->g2) ? out->g2 : (double*)x.g2.p;
+    launch_ext_preprocess(st, sc.splats, sc.sh, n, sc.deg, v, cam, (ExtRec*)x.rec.p, counts, (uint4*)x.rects.p,
+                          (uint64_t*)x.dkey.p);
     // rank of every splat in (depth, id) order: stable sort of depth << 32 | id on the depth bits
     const bool dalt = radix_sort_u64(st, (uint64_t*)x.dkey.p, (uint64_t*)x.dkey_alt.p, nullptr, nullptr, n, 32, 64,
                                      x.sort_tmp.p);
     const uint64_t* order = (const uint64_t*)(dalt ? x.dkey_alt.p : x.dkey.p);
-    launch_ext_rank_counts(st, n, order, (const uint32_t*)x.counts.p, (uint32_t*)x.rank_counts.p);
+    launch_ext_rank_counts(st, n, order, (const uint32_t*)counts, (uint32_t*)x.rank_counts.p);
     exclusive_scan_u32_to_u64(st, (const uint32_t*)x.rank_counts.p, (uint64_t*)x.offsets.p, n, x.scan_tmp.p);
     uint64_t cap;
     const uint64_t* m_dev = nullptr;
@@ -342,7 +345,7 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, const float cam[3], co
     const uint64_t* sorted = (const uint64_t*)(x.in_alt ? x.keys_alt.p : x.keys.p);
     launch_fill_u32(st, (uint32_t*)x.ranges.p, 0u, 2 * (uint64_t)v.num_tiles);
     launch_tile_ranges(st, sorted, cap, (uint32_t*)x.ranges.p, m_dev);
-    if (grads) launch_fill_u32(st, (uint32_t*)x.g2.p, 0u, 18 * (uint64_t)n);
+    if (grads) launch_fill_u32(st, (uint32_t*)g2, 0u, 18 * (uint64_t)n);
     launch_tile_order(st, (const uint32_t*)x.ranges.p, v.tiles_x, v.tiles_y, (uint32_t*)x.order.p);
     if (ev && (rc = hip(hipEventRecord(ev[3], st), "event")) != NPS_OK) return rc;
     ExtCompositeArgs a;
@@ -350,7 +353,7 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, const float cam[3], co
     a.keys = sorted;
     a.ranges = (const uint32_t*)x.ranges.p;
     a.tgt_rgba = tgt;
-    a.g2 = grads ? (double*)x.g2.p : nullptr;
+    a.g2 = grads ? g2 : nullptr;
     a.c_img = (float*)x.c_img.p;
     a.tile_loss = (double*)x.tile_loss.p;
     a.ckpt = grads ? (float*)x.ckpt.p : nullptr;
@@ -361,14 +364,9 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, const float cam[3], co
     a.backward = grads ? 1 : 0;
     launch_composite_ext(st, v, a);
     if (ev && (rc = hip(hipEventRecord(ev[4], st), "event")) != NPS_OK) return rc;
-    if (grads) {
-        if (hooks && hooks->chain_wait && (rc = hip(hipStreamWaitEvent(st, hooks->chain_wait, 0), "wait")) != NPS_OK)
-            return rc;
-        launch_ext_chain(st, sc.splats, sc.sh, n, sc.deg, v, cam, (const double*)x.g2.p,
-                         (const uint32_t*)x.counts.p, sc.gradx, sc.accum);
-        if (hooks && hooks->chain_done && (rc = hip(hipEventRecord(hooks->chain_done, st), "event")) != NPS_OK)
-            return rc;
-    }
+    if (grads && !(out && out->defer_chain))
+        launch_ext_chain(st, sc.splats, sc.sh, n, sc.deg, v, cam, (const double*)g2, (const uint32_t*)counts, sc.gradx,
+                         sc.accum);
     if (ev && (rc = hip(hipEventRecord(ev[7], st), "event")) != NPS_OK) return rc;
     return hip(hipGetLastError(), "launch");
 }
@@ -1052,8 +1050,11 @@ struct npt_ctx {
     int device = 0;
     hipStream_t main = nullptr;
     hipEvent_t fork = nullptr;
-    std::vector<hipEvent_t> lane_done, chain_done;
+    std::vector<hipEvent_t> lane_done, chain_done;  // chain_done: one per scene
+    hipEvent_t joined = nullptr;
     DevBuf view_loss, flags;  // 2 doubles per view; 16 words per lane
+    DevBuf view_g2, view_counts;    // per view: 9n doubles, n words (graph mode)
+    std::vector<size_t> view_base;  // splat offset of each view in view_g2 / view_counts
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool planned = false;
@@ -1124,6 +1125,7 @@ int npt_step_sequential(npt_handle t) {
 
 // Sizes every lane for its views (key counts measured by a forward-only eager
 // run) and captures the whole step as one graph.
+
 int npt_plan(npt_handle t) {
     npt_drop_graph(t);
     const int L = (int)t->lanes.size();
@@ -1153,6 +1155,14 @@ int npt_plan(npt_handle t) {
         if (rc != NPS_OK) return rc;
         NPT_HIP(t, hipStreamSynchronize(ln.st));
     }
+    t->view_base.assign(t->views.size(), 0);
+    size_t total = 0;
+    for (size_t vi = 0; vi < t->views.size(); ++vi) {
+        t->view_base[vi] = total;
+        total += std::max<uint32_t>(t->scenes[t->views[vi].scene]->n, 1);
+    }
+    if (ensure_dev(t->view_g2, sizeof(double) * 9 * total, t->main, t->err) != NPS_OK) return NPS_ERR_HIP;
+    if (ensure_dev(t->view_counts, sizeof(uint32_t) * total, t->main, t->err) != NPS_OK) return NPS_ERR_HIP;
     for (nps_handle h : t->scenes) NPT_HIP(t, hipStreamSynchronize(h->stream));
 
     NPT_HIP(t, hipStreamBeginCapture(t->main, hipStreamCaptureModeThreadLocal));
@@ -1162,7 +1172,6 @@ int npt_plan(npt_handle t) {
         if (hipEventRecord(t->fork, t->main) != hipSuccess) { rc = npt_fail(t, NPS_ERR_HIP, "fork"); return; }
         for (auto& ln : t->lanes)
             if (hipStreamWaitEvent(ln.st, t->fork, 0) != hipSuccess) { rc = npt_fail(t, NPS_ERR_HIP, "join"); return; }
-        std::vector<int> last_chain(t->scenes.size(), -1);
         for (int vi : t->order) {
             const npt_view& w = t->views[vi];
             nps_handle h = t->scenes[w.scene];
@@ -1170,19 +1179,45 @@ int npt_plan(npt_handle t) {
             ViewParams v = npt_view_params(w);
             float cam[3];
             ext_camera_centre(w.vp_inv, cam);
-            ChainHooks hooks;
-            hooks.chain_wait = last_chain[w.scene] >= 0 ? t->chain_done[last_chain[w.scene]] : nullptr;
-            hooks.chain_done = t->chain_done[vi];
-            rc = run_ext_view(scene_of(h), v, cam, (const float4*)w.tgt, ln.x, ln.st, true, true, t->err, &hooks,
+            ViewOut out;
+            out.counts = (uint32_t*)t->view_counts.p + t->view_base[vi];
+            out.g2 = (double*)t->view_g2.p + 9 * t->view_base[vi];
+            out.defer_chain = true;
+            rc = run_ext_view(scene_of(h), v, cam, (const float4*)w.tgt, ln.x, ln.st, true, true, t->err, &out,
                               nullptr);
             if (rc != NPS_OK) return;
             launch_sum_tile_loss(ln.st, (const double*)ln.x.tile_loss.p, v.num_tiles,
                                  (double*)t->view_loss.p + 2 * vi);
-            last_chain[w.scene] = vi;
         }
         for (size_t l = 0; l < t->lanes.size(); ++l) {
             if (hipEventRecord(t->lane_done[l], t->lanes[l].st) != hipSuccess ||
                 hipStreamWaitEvent(t->main, t->lane_done[l], 0) != hipSuccess) {
+                rc = npt_fail(t, NPS_ERR_HIP, "join");
+                return;
+            }
+        }
+        // chain rule: each scene's views in step order on one stream, scenes side by side
+        if (hipEventRecord(t->joined, t->main) != hipSuccess) { rc = npt_fail(t, NPS_ERR_HIP, "fork"); return; }
+        const size_t S = t->scenes.size(), L = t->lanes.size();
+        for (size_t l = 0; l < std::min(S, L); ++l)
+            if (hipStreamWaitEvent(t->lanes[l].st, t->joined, 0) != hipSuccess) { rc = npt_fail(t, NPS_ERR_HIP, "fork"); return; }
+        for (size_t sidx = 0; sidx < S; ++sidx) {
+            nps_handle h = t->scenes[sidx];
+            hipStream_t st = t->lanes[sidx % L].st;
+            for (int vi : t->order) {
+                const npt_view& w = t->views[vi];
+                if (w.scene != (int)sidx) continue;
+                ViewParams v = npt_view_params(w);
+                float cam[3];
+                ext_camera_centre(w.vp_inv, cam);
+                launch_ext_chain(st, h->splats, h->sh, h->n, h->deg, v, cam,
+                                 (const double*)t->view_g2.p + 9 * t->view_base[vi],
+                                 (const uint32_t*)t->view_counts.p + t->view_base[vi], h->gradx, h->accum);
+            }
+        }
+        for (size_t l = 0; l < std::min(S, L); ++l) {
+            if (hipEventRecord(t->chain_done[l], t->lanes[l].st) != hipSuccess ||
+                hipStreamWaitEvent(t->main, t->chain_done[l], 0) != hipSuccess) {
                 rc = npt_fail(t, NPS_ERR_HIP, "join");
                 return;
             }
@@ -1250,11 +1285,14 @@ int npt_create(nps_handle* scenes, int32_t n_scenes, int32_t lanes, npt_handle* 
     if (hipSetDevice(t->device) != hipSuccess) return fail();
     if (hipStreamCreateWithFlags(&t->main, hipStreamNonBlocking) != hipSuccess) return fail();
     if (hipEventCreateWithFlags(&t->fork, hipEventDisableTiming) != hipSuccess) return fail();
+    if (hipEventCreateWithFlags(&t->joined, hipEventDisableTiming) != hipSuccess) return fail();
     t->lanes.resize(lanes);
     t->lane_done.assign(lanes, nullptr);
+    t->chain_done.assign(lanes, nullptr);
     for (int l = 0; l < lanes; ++l) {
         if (hipStreamCreateWithFlags(&t->lanes[l].st, hipStreamNonBlocking) != hipSuccess) return fail();
         if (hipEventCreateWithFlags(&t->lane_done[l], hipEventDisableTiming) != hipSuccess) return fail();
+        if (hipEventCreateWithFlags(&t->chain_done[l], hipEventDisableTiming) != hipSuccess) return fail();
     }
     if (ensure_dev(t->flags, 64 * (size_t)lanes, t->main, t->err) != NPS_OK) return fail();
     launch_fill_u32(t->main, (uint32_t*)t->flags.p, 0u, 16 * (uint64_t)lanes);
@@ -1280,7 +1318,10 @@ void npt_destroy(npt_handle t) {
     for (hipEvent_t e : t->chain_done)
         if (e) (void)hipEventDestroy(e);
     if (t->fork) (void)hipEventDestroy(t->fork);
+    if (t->joined) (void)hipEventDestroy(t->joined);
     free_buf(t->view_loss);
+    free_buf(t->view_g2);
+    free_buf(t->view_counts);
     free_buf(t->flags);
     if (t->main) (void)hipStreamDestroy(t->main);
     delete t;
@@ -1301,9 +1342,6 @@ int npt_add_view(npt_handle t, int32_t scene, const float vp[16], const float vp
     v.W = (uint32_t)w;
     v.H = (uint32_t)hgt;
     t->views.push_back(v);
-    hipEvent_t e = nullptr;
-    NPT_HIP(t, hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    t->chain_done.push_back(e);
     // step order: most pixels first (longest-processing-time order), then index;
     // lanes are filled greedily with the least-loaded lane first
     const int V = (int)t->views.size();
