@@ -46,7 +46,9 @@ typedef struct nps_config {
     int32_t device;          /* HIP device ordinal */
     int32_t mode;            /* NPS_MODE_REF (0) or NPS_MODE_EXT (1) */
     int32_t sh_degree;       /* EXT: spherical-harmonic degree 0..3 */
-    int32_t reserved[10];
+    int32_t ext_tile;        /* EXT: composite tile edge 8, 16 (default) or 32 pixels */
+    int32_t ext_batch;       /* EXT: records per LDS stage 32, 64 (default) or 128 */
+    int32_t reserved[8];
 } nps_config;
 
 /*
@@ -60,8 +62,10 @@ typedef struct nps_config {
  * exp scales, sigmoid opacity, normalised quaternion, 0.3 px screen filter,
  * colour max(SH(dir) + 0.5, 0), alpha = min(0.99, opacity exp(-r^2/2)) with
  * pairs below 1/255 skipped, tile lists ordered by view depth (then id) and not
- * capped, a pixel stops before the splat that would bring its transmittance
- * below 1e-4, background 0.05, L1 loss, FP32 gradients of all 64 parameters.
+ * capped, a pixel sees a splat only inside its 3-sigma box (so results do not
+ * depend on the tile size), a pixel stops before the splat that would bring its
+ * transmittance below 1e-4, background 0.05, L1 loss, FP32 gradients of all
+ * 64 parameters.
  * EXT has no reference function (an extension of the same loop).
  */
 #define NPS_MODE_REF 0
@@ -125,6 +129,12 @@ int nps_backward(nps_handle h);
  * and nps_read_loss then give the image and its loss against that target.  No
  * gradients.  REF mode renders with the npf_* entry points instead. */
 int nps_forward(nps_handle h);
+/* EXT mode (no reference counterpart): times the composite for tile edges
+ * {8, 16, 32} x records per LDS stage {32, 64, 128} on the current camera and
+ * target (binning, sort, work order and composite fwd+bwd; the chain rule is not
+ * run, gradients stay untouched), keeps the fastest for this handle and returns
+ * it; ms9[3 * tile_index + batch_index] receives the times (may be NULL). */
+int nps_autotune(nps_handle h, int32_t* tile, int32_t* batch, float* ms9);
 /* splat_step.wgsl (OPT:376-378): fused Adam; increments t. */
 int nps_step(nps_handle h);
 /* splat_adc.wgsl (OPT:380-385): clone/split into dead slots. */
@@ -229,6 +239,11 @@ const char* npt_last_error(npt_handle t);
 int npt_add_view(npt_handle t, int32_t scene, const float vp[16], const float vp_inv[16], const float* rgba_dev,
                  int32_t w, int32_t hgt);
 int npt_step(npt_handle t, int32_t mode);
+/* Autotunes (as nps_autotune) one view of every (scene, resolution) class and
+ * uses its choice for the whole class; views start with their scene's setting.
+ * tile_batch2v[2 v], [2 v + 1]: the choice of view v. */
+int npt_autotune(npt_handle t);
+int npt_read_tune(npt_handle t, int32_t* tile_batch2v);
 /* Losses of every view in the last step: loss2v[2 v] = L2, loss2v[2 v + 1] = L1. */
 int npt_read_losses(npt_handle t, double* loss2v);
 

@@ -25,7 +25,9 @@ class NpsConfig(C.Structure):
         ("device", C.c_int32),
         ("mode", C.c_int32),
         ("sh_degree", C.c_int32),
-        ("reserved", C.c_int32 * 10),
+        ("ext_tile", C.c_int32),
+        ("ext_batch", C.c_int32),
+        ("reserved", C.c_int32 * 8),
     ]
 
 
@@ -61,6 +63,7 @@ SIGNATURES = {
     "nps_set_target_dev": (i32, [_P, _P, _P, i32, i32]),
     "nps_backward": (i32, [_P]),
     "nps_forward": (i32, [_P]),
+    "nps_autotune": (i32, [_P, _I, _I, _F]),
     "nps_step": (i32, [_P]),
     "nps_adc": (i32, [_P]),
     "nps_set_adc_ext": (i32, [_P, C.c_float, C.c_float, C.c_float, C.c_float]),
@@ -85,6 +88,8 @@ SIGNATURES = {
     "npt_last_error": (C.c_char_p, [_P]),
     "npt_add_view": (i32, [_P, i32, _F, _F, _P, i32, i32]),
     "npt_step": (i32, [_P, i32]),
+    "npt_autotune": (i32, [_P]),
+    "npt_read_tune": (i32, [_P, _I]),
     "npt_read_losses": (i32, [_P, _D]),
     "npb_create": (i32, [i32, C.POINTER(_P)]),
     "npb_destroy": (None, [_P]),

@@ -54,10 +54,11 @@ __global__ __launch_bounds__(256) void k_ext_preprocess(const float4* __restrict
     ExtProj o;
     ext_project(s, c, DEG, v.vp, cam, v.W, v.H, o);
     if (!o.valid) return;
-    const float sminx = o.scr[0] - o.ext[0], smaxx = o.scr[0] + o.ext[0];
-    const float sminy = o.scr[1] - o.ext[1], smaxy = o.scr[1] + o.ext[1];
+    const float ebx = o.ext[0] * kExtBinScale + kExtBinPad, eby = o.ext[1] * kExtBinScale + kExtBinPad;
+    const float sminx = o.scr[0] - ebx, smaxx = o.scr[0] + ebx;
+    const float sminy = o.scr[1] - eby, smaxy = o.scr[1] + eby;
     if (!(sminx == sminx && smaxx == smaxx && sminy == sminy && smaxy == smaxy)) return;
-    const uint32_t W = v.W, H = v.H, T = (uint32_t)kTile;
+    const uint32_t W = v.W, H = v.H, T = v.tile;
     const float aspect = v.aspect;
     uint32_t tx0 = first_false(v.tiles_x, [&](uint32_t tx) {
         uint32_t x1 = min(tx * T + T, W);
@@ -92,7 +93,8 @@ __global__ __launch_bounds__(256) void k_ext_preprocess(const float4* __restrict
         float l = 2.0f * log_det(o.op * 255.0f);
         f.rcut2 = fmax_w(l, 0.0f) * 1.0001f + 1e-3f;
     }
-    f.pad0 = f.pad1 = 0.0f;
+    f.ext_x = o.ext[0];
+    f.ext_y = o.ext[1];
     rec[i] = f;
 }
 
@@ -261,12 +263,29 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict_
     }
 }
 
-__global__ __launch_bounds__(256) void k_composite_ext(ViewParams v, ExtCompositeArgs A) {
-    __shared__ ExtRec s_rec[kBatch];
-    __shared__ uint32_t s_ids[kBatch];
-    __shared__ float s_part[4][kBatch][9];
-    __shared__ double s_red[2][4];
-    __shared__ int s_max[4];
+// Composite of one TILE x TILE tile (one workgroup, one thread per pixel, each
+// wave an 8x8 pixel block), records staged in LDS BATCH at a time.
+//   forward : front to back, alphas of 8 entries at a time (independent), then
+//             the transmittance chain with early termination; T checkpointed
+//             every 8 entries ([slot][chunk][pixel]); per wave one bit per entry:
+//             did any of its pixels composite it;
+//   reverse : batches and 8-entry chunks backwards, chunks without a bit
+//             skipped, alpha and T recomputed from the checkpoint, 9 FP32
+//             per-pair gradients summed over the wave, the 4..16 wave partials
+//             of an entry summed in FP64 and added to the splat's 2D gradients.
+template <int TILE, int BATCH>
+__global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, ExtCompositeArgs A) {
+    constexpr int NT = TILE * TILE;            // threads = pixels
+    constexpr int NW = NT / 64;                // waves
+    constexpr int BW = TILE / 8;               // wave blocks per tile row
+    constexpr int MW = (BATCH + 63) / 64;      // 64-bit mask words per batch
+    constexpr int NCH = BATCH / 8;             // chunks per batch
+    constexpr int NSTAGE = (3 * BATCH + NT - 1) / NT;  // float4 record pieces per thread
+    __shared__ ExtRec s_rec[BATCH];
+    __shared__ uint32_t s_ids[BATCH];
+    __shared__ float s_part[NW][BATCH][9];
+    __shared__ double s_red[2][NW];
+    __shared__ int s_max[NW];
 
     const uint32_t tile = A.order[blockIdx.x];
     if (tile >= v.num_tiles) return;  // padding slot of the order (uniform per workgroup)
@@ -274,30 +293,52 @@ __global__ __launch_bounds__(256) void k_composite_ext(ViewParams v, ExtComposit
     const uint32_t start = A.ranges[2 * tile], end = A.ranges[2 * tile + 1];
     const int len = (int)(end - start);
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const uint32_t px = tx * kTile + (wid & 1) * 8 + (lane & 7);
-    const uint32_t py = ty * kTile + (wid >> 1) * 8 + (lane >> 3);
+    const uint32_t px = tx * TILE + (wid % BW) * 8 + (lane & 7);
+    const uint32_t py = ty * TILE + (wid / BW) * 8 + (lane >> 3);
     const bool valid = px < v.W && py < v.H;
     const float pxf = px_to_x(px, v.W, v.aspect);
     const float pyf = py_to_y(py, v.H);
-    // this tile's batch slots: [start/64 + tile, ...) never overlap another tile's
-    const size_t slot0 = (size_t)(start / kBatch) + tile;
+    // this tile's batch slots: [start/BATCH + tile, ...) never overlap another tile's
+    const size_t slot0 = (size_t)(start / BATCH) + tile;
 
-    // Batch staging: thread t < 192 carries float4 (t % 3) of record t / 3.  The
-    // records of the next batch are loaded into registers while the current
-    // batch is composited, and the id of the batch after that as well.
-    const int sj = tid / 3, spart = tid - 3 * sj;
-    const bool stager = tid < 3 * kBatch;
-    auto load_id = [&](int b0) -> uint32_t {
-        return (stager && b0 >= 0 && b0 + sj < len) ? (uint32_t)A.keys[start + (uint32_t)(b0 + sj)] : 0u;
+    // Batch staging: piece q = tid + k*NT (< 3*BATCH) is float4 (q % 3) of record
+    // q / 3.  The next batch's pieces are loaded into registers while the current
+    // batch is composited, with the ids of the batch after that.
+    auto load_id = [&](int b0, int k) -> uint32_t {
+        const int q = tid + k * NT, j = q / 3;
+        return (q < 3 * BATCH && b0 >= 0 && b0 + j < len) ? (uint32_t)A.keys[start + (uint32_t)(b0 + j)] : 0u;
     };
-    auto load_rec = [&](uint32_t id, int b0) -> float4 {
-        return (stager && b0 >= 0 && b0 + sj < len) ? reinterpret_cast<const float4*>(&A.rec[id])[spart]
-                                                    : make_float4(0.f, 0.f, 0.f, 0.f);
+    auto load_rec = [&](uint32_t id, int b0, int k) -> float4 {
+        const int q = tid + k * NT, j = q / 3;
+        return (q < 3 * BATCH && b0 >= 0 && b0 + j < len) ? reinterpret_cast<const float4*>(&A.rec[id])[q - 3 * j]
+                                                          : make_float4(0.f, 0.f, 0.f, 0.f);
     };
-    auto put = [&](uint32_t id, float4 r, int nb) {
-        if (stager && sj < nb) {
-            reinterpret_cast<float4*>(&s_rec[sj])[spart] = r;
-            if (spart == 0) s_ids[sj] = id;
+    uint32_t id_cur[NSTAGE], id_next[NSTAGE];
+    float4 rec_cur[NSTAGE];
+    auto put = [&](int nb) {
+#pragma unroll
+        for (int k = 0; k < NSTAGE; ++k) {
+            const int q = tid + k * NT, j = q / 3;
+            if (q < 3 * BATCH && j < nb) {
+                reinterpret_cast<float4*>(&s_rec[j])[q - 3 * j] = rec_cur[k];
+                if (q - 3 * j == 0) s_ids[j] = id_cur[k];
+            }
+        }
+    };
+    auto prefetch = [&](int b_next, int b_after) {
+#pragma unroll
+        for (int k = 0; k < NSTAGE; ++k) {
+            id_cur[k] = id_next[k];
+            rec_cur[k] = load_rec(id_cur[k], b_next, k);
+            id_next[k] = load_id(b_after, k);
+        }
+    };
+    auto prime = [&](int b_first, int b_second) {
+#pragma unroll
+        for (int k = 0; k < NSTAGE; ++k) {
+            id_cur[k] = load_id(b_first, k);
+            rec_cur[k] = load_rec(id_cur[k], b_first, k);
+            id_next[k] = load_id(b_second, k);
         }
     };
 
@@ -307,23 +348,21 @@ __global__ __launch_bounds__(256) void k_composite_ext(ViewParams v, ExtComposit
     bool alive = valid;
     uint32_t n_eval = 0u, n_fw = 0u;
     int fw_extent = 0;  // entries staged by the forward (uniform)
-    uint32_t id_cur = load_id(0);
-    float4 rec_cur = load_rec(id_cur, 0);
-    uint32_t id_next = load_id(kBatch);
-    for (int b0 = 0, bi = 0; b0 < len; b0 += kBatch, ++bi) {
+    prime(0, BATCH);
+    for (int b0 = 0, bi = 0; b0 < len; b0 += BATCH, ++bi) {
         if (!__syncthreads_or(alive ? 1 : 0)) break;
-        const int nb = min(kBatch, len - b0);
+        const int nb = min(BATCH, len - b0);
         fw_extent = b0 + nb;
-        put(id_cur, rec_cur, nb);
+        put(nb);
         __syncthreads();
-        id_cur = id_next;
-        rec_cur = load_rec(id_cur, b0 + kBatch);
-        id_next = load_id(b0 + 2 * kBatch);
-        uint64_t mask = 0ull;
-        float* ck = A.ckpt + (slot0 + bi) * (size_t)(8 * kTilePx);
+        prefetch(b0 + BATCH, b0 + 2 * BATCH);
+        uint64_t mask[MW];
+#pragma unroll
+        for (int w = 0; w < MW; ++w) mask[w] = 0ull;
+        float* ck = A.ckpt + (slot0 + bi) * (size_t)(NCH * NT);
         if (__ballot(alive) != 0ull) {
             for (int c0 = 0; c0 < nb; c0 += 8) {
-                if (A.backward) ck[(c0 >> 3) * kTilePx + tid] = T;
+                if (A.backward) ck[(c0 >> 3) * NT + tid] = T;
                 // alphas of the chunk's 8 entries are independent: computed first
                 float ca[8];
 #pragma unroll
@@ -332,7 +371,7 @@ __global__ __launch_bounds__(256) void k_composite_ext(ViewParams v, ExtComposit
                     if (c0 + j < nb) {
                         const ExtRec& f = s_rec[c0 + j];
                         ExtPair q = ext_pair(f, pxf, pyf);
-                        if (!(q.r2 > f.rcut2)) {
+                        if (in_box(q.dx, q.dy, f.ext_x, f.ext_y) && !(q.r2 > f.rcut2)) {
                             float G = exp2_det_in_range((-0.5f * q.r2) * kLog2e);  // r2 <= rcut2 < 12
                             float a = f.op * G;
                             if (a > kExtMaxAlpha) a = kExtMaxAlpha;
@@ -364,13 +403,17 @@ __global__ __launch_bounds__(256) void k_composite_ext(ViewParams v, ExtComposit
                             }
                         }
                         ++n_fw;
-                        if (__ballot(inc) != 0ull) mask |= 1ull << (c0 + j);
+                        if (__ballot(inc) != 0ull) mask[(c0 + j) >> 6] |= 1ull << ((c0 + j) & 63);
                     }
                 }
                 if (__ballot(alive) == 0ull) break;
             }
         }
-        if (A.backward && lane == 0) A.act[(slot0 + bi) * 4 + wid] = mask;
+        if (A.backward && lane < MW) {
+#pragma unroll
+            for (int w = 0; w < MW; ++w)
+                if (lane == w) A.act[((slot0 + bi) * NW + wid) * MW + w] = mask[w];
+        }
     }
     Cr = Cr + T * kExtBg;
     Cg = Cg + T * kExtBg;
@@ -405,33 +448,33 @@ __global__ __launch_bounds__(256) void k_composite_ext(ViewParams v, ExtComposit
         s_max[wid] = ml;
     }
     __syncthreads();
-    const int maxlast = max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
+    int maxlast = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) maxlast = max(maxlast, s_max[w]);
 
     // ---------------- reverse pass ----------------
     uint32_t n_active = 0u, n_went = 0u;
     if (A.backward) {
         float dT = (dCr * kExtBg + dCg * kExtBg) + dCb * kExtBg;
-        const int nbr = (maxlast + kBatch - 1) / kBatch;
-        id_cur = load_id((nbr - 1) * kBatch);
-        rec_cur = load_rec(id_cur, (nbr - 1) * kBatch);
-        id_next = load_id((nbr - 2) * kBatch);
+        const int nbr = (maxlast + BATCH - 1) / BATCH;
+        prime((nbr - 1) * BATCH, (nbr - 2) * BATCH);
         for (int bi = nbr - 1; bi >= 0; --bi) {
-            const int b0 = bi * kBatch;
-            const int nb = min(kBatch, len - b0);
+            const int b0 = bi * BATCH;
+            const int nb = min(BATCH, len - b0);
             __syncthreads();
-            put(id_cur, rec_cur, nb);
-            for (int e = tid; e < 4 * kBatch * 9; e += 256) (&s_part[0][0][0])[e] = 0.0f;
+            put(nb);
+            for (int e = tid; e < NW * BATCH * 9; e += NT) (&s_part[0][0][0])[e] = 0.0f;
             __syncthreads();
-            id_cur = id_next;
-            rec_cur = load_rec(id_cur, b0 - kBatch);
-            id_next = load_id(b0 - 2 * kBatch);
-            const uint64_t mask = A.act[(slot0 + bi) * 4 + wid];
-            const float* ck = A.ckpt + (slot0 + bi) * (size_t)(8 * kTilePx);
-            for (int c = 7; c >= 0; --c) {
-                const uint32_t bits = (uint32_t)(mask >> (8 * c)) & 0xFFu;
+            prefetch(b0 - BATCH, b0 - 2 * BATCH);
+            uint64_t mask[MW];
+#pragma unroll
+            for (int w = 0; w < MW; ++w) mask[w] = A.act[((slot0 + bi) * NW + wid) * MW + w];
+            const float* ck = A.ckpt + (slot0 + bi) * (size_t)(NCH * NT);
+            for (int c = NCH - 1; c >= 0; --c) {
+                const uint32_t bits = (uint32_t)(mask[c >> 3] >> (8 * (c & 7))) & 0xFFu;
                 if (bits == 0u) continue;
                 const int hi = 31 - __clz((int)bits);
-                float Tc = ck[c * kTilePx + tid];
+                float Tc = ck[c * NT + tid];
                 float ra[8], rT[8], rG[8];
                 uint32_t incb = 0u;
 #pragma unroll
@@ -444,7 +487,7 @@ __global__ __launch_bounds__(256) void k_composite_ext(ViewParams v, ExtComposit
                         if (e < last) {
                             const ExtRec& f = s_rec[8 * c + j];
                             ExtPair q = ext_pair(f, pxf, pyf);
-                            if (!(q.r2 > f.rcut2)) {
+                            if (in_box(q.dx, q.dy, f.ext_x, f.ext_y) && !(q.r2 > f.rcut2)) {
                                 float G = exp2_det_in_range((-0.5f * q.r2) * kLog2e);  // r2 <= rcut2 < 12
                                 float a = f.op * G;
                                 if (a > kExtMaxAlpha) a = kExtMaxAlpha;
@@ -498,10 +541,11 @@ __global__ __launch_bounds__(256) void k_composite_ext(ViewParams v, ExtComposit
                 }
             }
             __syncthreads();
-            for (int e = tid; e < nb * 9; e += 256) {
+            for (int e = tid; e < nb * 9; e += NT) {
                 const int j = e / 9, k = e - 9 * j;
-                double sum = ((double)s_part[0][j][k] + (double)s_part[1][j][k]) +
-                             ((double)s_part[2][j][k] + (double)s_part[3][j][k]);
+                double sum = 0.0;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) sum += (double)s_part[w][j][k];
                 if (sum != 0.0) unsafeAtomicAdd(&A.g2[9 * (size_t)s_ids[j] + k], sum);
             }
         }
@@ -518,8 +562,12 @@ __global__ __launch_bounds__(256) void k_composite_ext(ViewParams v, ExtComposit
         if (n_fw) atomicAdd(&A.stats[4], (unsigned long long)n_fw);
     }
     if (tid == 0) {
-        double a2 = (s_red[0][0] + s_red[0][1]) + (s_red[0][2] + s_red[0][3]);
-        double a1 = (s_red[1][0] + s_red[1][1]) + (s_red[1][2] + s_red[1][3]);
+        double a2 = 0.0, a1 = 0.0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            a2 += s_red[0][w];
+            a1 += s_red[1][w];
+        }
         A.tile_loss[2 * tile] = a2;
         A.tile_loss[2 * tile + 1] = a1;
         A.tile_work[2 * tile] = (uint32_t)fw_extent;
@@ -975,9 +1023,30 @@ void launch_tile_order(hipStream_t st, const uint32_t* ranges, uint32_t tiles_x,
     hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, st, ranges, tiles_x, tiles_y, order);
 }
 
-void launch_composite_ext(hipStream_t st, const ViewParams& v, const ExtCompositeArgs& a) {
+template <int TILE, int BATCH>
+static void launch_composite_t(hipStream_t st, const ViewParams& v, const ExtCompositeArgs& a) {
+    hipLaunchKernelGGL((k_composite_ext<TILE, BATCH>), dim3(ext_order_slots(v.tiles_x, v.tiles_y)),
+                       dim3(TILE * TILE), 0, st, v, a);
+}
+
+bool ext_tune_valid(int tile, int batch) {
+    return (tile == 8 || tile == 16 || tile == 32) && (batch == 32 || batch == 64 || batch == 128);
+}
+
+void launch_composite_ext(hipStream_t st, const ViewParams& v, const ExtCompositeArgs& a, int batch) {
     if (v.num_tiles == 0) return;
-    hipLaunchKernelGGL(k_composite_ext, dim3(ext_order_slots(v.tiles_x, v.tiles_y)), dim3(256), 0, st, v, a);
+    const int tile = (int)v.tile;
+#define NPS_COMPOSITE_CASE(T, B) \
+    if (tile == T && batch == B) return launch_composite_t<T, B>(st, v, a);
+    NPS_COMPOSITE_CASE(8, 32) NPS_COMPOSITE_CASE(8, 64) NPS_COMPOSITE_CASE(8, 128)
+    NPS_COMPOSITE_CASE(16, 32) NPS_COMPOSITE_CASE(16, 64) NPS_COMPOSITE_CASE(16, 128)
+    NPS_COMPOSITE_CASE(32, 32) NPS_COMPOSITE_CASE(32, 64) NPS_COMPOSITE_CASE(32, 128)
+#undef NPS_COMPOSITE_CASE
+}
+
+size_t ext_ckpt_floats(size_t slots, int tile, int batch) { return slots * (size_t)(batch / 8) * tile * tile; }
+size_t ext_act_words(size_t slots, int tile, int batch) {
+    return slots * (size_t)(tile * tile / 64) * (size_t)((batch + 63) / 64);
 }
 
 void launch_ext_chain(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg, const ViewParams& v,

@@ -54,6 +54,12 @@ struct ExtView {
 
 void free_ext_view(ExtView& x);
 
+// Tile edge and records per LDS stage of the EXT composite.
+struct ExtTune {
+    int tile = kTile;
+    int batch = kBatch;
+};
+
 // The splats an EXT view pipeline reads and the gradients it adds into.
 struct SceneRef {
     const float* splats;
@@ -71,6 +77,7 @@ struct nps_ctx {
     int device = 0;
     bool ext = false;  // NPS_MODE_EXT
     int deg = 0;       // EXT SH degree
+    ExtTune tune;      // EXT composite tile size and batch
     hipStream_t stream = nullptr;
     uint32_t n = 0;
     float* splats = nullptr;
@@ -186,7 +193,7 @@ uint32_t splat_block_of(uint32_t n) {
     return b < 4096u ? 4096u : b;
 }
 
-ViewParams make_view(const float* vp, const float* vp_inv, uint32_t W, uint32_t H) {
+ViewParams make_view(const float* vp, const float* vp_inv, uint32_t W, uint32_t H, uint32_t tile = kTile) {
     ViewParams v;
     memset(&v, 0, sizeof(v));
     memcpy(v.vp, vp, sizeof(v.vp));
@@ -194,8 +201,9 @@ ViewParams make_view(const float* vp, const float* vp_inv, uint32_t W, uint32_t 
     v.W = W;
     v.H = H;
     v.aspect = (float)W / (float)H;
-    v.tiles_x = (W + kTile - 1) / kTile;
-    v.tiles_y = (H + kTile - 1) / kTile;
+    v.tile = tile;
+    v.tiles_x = (W + tile - 1) / tile;
+    v.tiles_y = (H + tile - 1) / tile;
     v.num_tiles = v.tiles_x * v.tiles_y;
     return v;
 }
@@ -255,12 +263,13 @@ uint64_t ext_key_capacity(uint64_t m) { return m + m / 4 + 65536; }
 
 // Allocates every buffer of view x for `n` splats, a W x H view and `cap` keys
 // (capture mode needs them all in place before the capture starts).
-int ext_view_reserve(ExtView& x, uint32_t n, uint32_t W, uint32_t H, uint64_t cap, bool grads, hipStream_t st,
-                     std::string& err) {
+int ext_view_reserve(ExtView& x, uint32_t n, uint32_t W, uint32_t H, uint64_t cap, bool grads, const ExtTune& tu,
+                     hipStream_t st, std::string& err) {
     const size_t n1 = std::max<uint32_t>(n, 1);
-    const uint32_t tx = (W + kTile - 1) / kTile, ty = (H + kTile - 1) / kTile, nt = std::max<uint32_t>(tx * ty, 1);
+    const uint32_t T = (uint32_t)tu.tile;
+    const uint32_t tx = (W + T - 1) / T, ty = (H + T - 1) / T, nt = std::max<uint32_t>(tx * ty, 1);
     const size_t c1 = std::max<uint64_t>(cap, 1);
-    const size_t slots = (size_t)(cap / kBatch) + nt + 1;
+    const size_t slots = (size_t)(cap / (uint64_t)tu.batch) + nt + 1;
     struct {
         DevBuf* b;
         size_t bytes;
@@ -276,8 +285,10 @@ int ext_view_reserve(ExtView& x, uint32_t n, uint32_t W, uint32_t H, uint64_t ca
     for (auto& q : need)
         if ((rc = ensure_dev(*q.b, q.bytes, st, err)) != NPS_OK) return rc;
     if (grads) {
-        if ((rc = ensure_dev(x.ckpt, sizeof(float) * slots * 8 * kTilePx, st, err)) != NPS_OK) return rc;
-        if ((rc = ensure_dev(x.act, sizeof(uint64_t) * slots * 4, st, err)) != NPS_OK) return rc;
+        if ((rc = ensure_dev(x.ckpt, sizeof(float) * ext_ckpt_floats(slots, tu.tile, tu.batch), st, err)) != NPS_OK)
+            return rc;
+        if ((rc = ensure_dev(x.act, sizeof(uint64_t) * ext_act_words(slots, tu.tile, tu.batch), st, err)) != NPS_OK)
+            return rc;
         if ((rc = ensure_dev(x.g2, sizeof(double) * 9 * n1, st, err)) != NPS_OK) return rc;
     }
     return NPS_OK;
@@ -289,8 +300,10 @@ int ext_view_reserve(ExtView& x, uint32_t n, uint32_t W, uint32_t H, uint64_t ca
 // Capture: nothing is allocated or read back; the count stays on the device
 // and keys beyond x.cap are dropped with flags[0] set.
 // ev: optional 8 timing events (eager only).
-int run_ext_view(const SceneRef& sc, const ViewParams& v, const float cam[3], const float4* tgt, ExtView& x,
-                 hipStream_t st, bool grads, bool capture, std::string& err, const ViewOut* out, hipEvent_t* ev) {
+int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float cam[3], const float4* tgt,
+                 ExtView& x, hipStream_t st, bool grads, bool capture, std::string& err, const ViewOut* out,
+                 hipEvent_t* ev) {
+    const ExtTune tu{(int)v.tile, batch};
     const uint32_t n = sc.n;
     int rc;
     auto hip = [&](hipError_t e, const char* what) {
@@ -298,7 +311,7 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, const float cam[3], co
         err = std::string(what) + ": " + hipGetErrorString(e);
         return NPS_ERR_HIP;
     };
-    if (!capture && (rc = ext_view_reserve(x, n, v.W, v.H, 0, grads, st, err)) != NPS_OK) return rc;
+    if (!capture && (rc = ext_view_reserve(x, n, v.W, v.H, 0, grads, tu, st, err)) != NPS_OK) return rc;
     if (ev && (rc = hip(hipEventRecord(ev[0], st), "event")) != NPS_OK) return rc;
     launch_fill_u32(st, (uint32_t*)x.stats.p, 0u, 16);
     // flags: cleared per view when eager; a captured step clears them once, so a
@@ -329,7 +342,7 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, const float cam[3], co
         }
         x.m = m;
         cap = m;
-        if ((rc = ext_view_reserve(x, n, v.W, v.H, m, grads, st, err)) != NPS_OK) return rc;
+        if ((rc = ext_view_reserve(x, n, v.W, v.H, m, grads, tu, st, err)) != NPS_OK) return rc;
     } else {
         cap = x.cap;
         m_dev = (const uint64_t*)x.offsets.p + n;
@@ -362,7 +375,7 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, const float cam[3], co
     a.order = (const uint32_t*)x.order.p;
     a.tile_work = (uint32_t*)x.tile_work.p;
     a.backward = grads ? 1 : 0;
-    launch_composite_ext(st, v, a);
+    launch_composite_ext(st, v, a, batch);
     if (ev && (rc = hip(hipEventRecord(ev[4], st), "event")) != NPS_OK) return rc;
     if (grads && !(out && out->defer_chain))
         launch_ext_chain(st, sc.splats, sc.sh, n, sc.deg, v, cam, (const double*)g2, (const uint32_t*)counts, sc.gradx,
@@ -378,14 +391,52 @@ void ext_camera_centre(const float* vp_inv, float cam[3]) {
 
 SceneRef scene_of(nps_ctx* h) { return SceneRef{h->splats, h->sh, h->n, h->deg, h->gradx, h->accum}; }
 
+const int kTuneTiles[3] = {8, 16, 32};
+const int kTuneBatches[3] = {32, 64, 128};
+
+// Times the 9 (tile, batch) variants of one view: sort + ranges + work order +
+// composite fwd+bwd (HIP events, best of 3 after a warm-up; the chain rule is
+// not run, so gradients and accumulators are untouched).  ms9[3 * ti + bi].
+int ext_time_variants(const SceneRef& sc, const float* vp, const float* vp_inv, uint32_t W, uint32_t H,
+                      const float4* tgt, ExtView& x, hipStream_t st, hipEvent_t* ev, std::string& err, float ms9[9],
+                      ExtTune* best) {
+    float cam[3];
+    ext_camera_centre(vp_inv, cam);
+    float best_ms = 1e30f;
+    for (int ti = 0; ti < 3; ++ti)
+        for (int bi = 0; bi < 3; ++bi) {
+            ViewParams v = make_view(vp, vp_inv, W, H, (uint32_t)kTuneTiles[ti]);
+            ViewOut out;
+            out.defer_chain = true;
+            float tmin = 1e30f;
+            for (int rep = 0; rep < 4; ++rep) {
+                int rc = run_ext_view(sc, v, kTuneBatches[bi], cam, tgt, x, st, true, false, err, &out, ev);
+                if (rc != NPS_OK) return rc;
+                if (hipEventSynchronize(ev[4]) != hipSuccess) {
+                    err = "event sync";
+                    return NPS_ERR_HIP;
+                }
+                float t = 0.0f;
+                hipEventElapsedTime(&t, ev[1], ev[4]);
+                if (rep > 0) tmin = std::min(tmin, t);
+            }
+            ms9[3 * ti + bi] = tmin;
+            if (tmin < best_ms) {
+                best_ms = tmin;
+                *best = ExtTune{kTuneTiles[ti], kTuneBatches[bi]};
+            }
+        }
+    return NPS_OK;
+}
+
 // nps_backward / nps_forward in EXT mode: the handle's camera and target.
 int run_ext(nps_ctx* h, bool grads) {
     if (hipSetDevice(h->device) != hipSuccess) return set_err(h, NPS_ERR_HIP, "hipSetDevice");
-    ViewParams v = make_view(h->vp, h->vp_inv, h->W, h->H);
+    ViewParams v = make_view(h->vp, h->vp_inv, h->W, h->H, (uint32_t)h->tune.tile);
     float cam[3];
     ext_camera_centre(h->vp_inv, cam);
-    int rc = run_ext_view(scene_of(h), v, cam, (const float4*)h->tgt_rgba.p, h->xv, h->stream, grads, false, h->err,
-                          nullptr, h->timing ? h->ev : nullptr);
+    int rc = run_ext_view(scene_of(h), v, h->tune.batch, cam, (const float4*)h->tgt_rgba.p, h->xv, h->stream, grads,
+                          false, h->err, nullptr, h->timing ? h->ev : nullptr);
     if (rc != NPS_OK) return rc;
     h->last_tiles = v.num_tiles;
     h->have_backward = true;
@@ -414,6 +465,8 @@ int nps_config_default(nps_config* cfg) {
     cfg->adc_period = 25;
     cfg->adc_enabled = 1;
     cfg->device = 0;
+    cfg->ext_tile = 16;
+    cfg->ext_batch = 64;
     return NPS_OK;
 }
 
@@ -431,6 +484,13 @@ int nps_create(const nps_config* cfg, int32_t num_splats, nps_handle* out) {
     }
     h->ext = h->cfg.mode == NPS_MODE_EXT;
     h->deg = h->cfg.sh_degree;
+    if (h->cfg.ext_tile || h->cfg.ext_batch) {
+        if (!ext_tune_valid(h->cfg.ext_tile, h->cfg.ext_batch)) {
+            delete h;
+            return NPS_ERR_ARG;
+        }
+        h->tune = ExtTune{h->cfg.ext_tile, h->cfg.ext_batch};
+    }
     h->device = h->cfg.device;
     h->n = (uint32_t)num_splats;
     auto fail = [&](hipError_t e) {
@@ -723,6 +783,25 @@ int nps_forward(nps_handle h) {
     if (!h->has_target) return set_err(h, NPS_ERR_STATE, "no target size (nps_set_target)");
     if (!h->has_camera) return set_err(h, NPS_ERR_STATE, "no camera (nps_set_camera)");
     return run_ext(h, false);
+}
+
+int nps_autotune(nps_handle h, int32_t* tile, int32_t* batch, float* ms9) {
+    if (!h) return NPS_ERR_ARG;
+    if (!h->ext) return set_err(h, NPS_ERR_STATE, "EXT mode only");
+    if (!h->has_target) return set_err(h, NPS_ERR_STATE, "no target (nps_set_target)");
+    if (!h->has_camera) return set_err(h, NPS_ERR_STATE, "no camera (nps_set_camera)");
+    NPS_HIP(h, hipSetDevice(h->device));
+    float t9[9];
+    ExtTune best = h->tune;
+    int rc = ext_time_variants(scene_of(h), h->vp, h->vp_inv, h->W, h->H, (const float4*)h->tgt_rgba.p, h->xv,
+                               h->stream, h->ev, h->err, t9, &best);
+    if (rc != NPS_OK) return rc;
+    h->tune = best;
+    h->have_backward = false;  // the view buffers hold the last variant timed
+    if (tile) *tile = best.tile;
+    if (batch) *batch = best.batch;
+    if (ms9) memcpy(ms9, t9, sizeof(t9));
+    return NPS_OK;
 }
 
 int nps_step(nps_handle h) {
@@ -1034,6 +1113,7 @@ struct npt_view {
     float vp[16], vp_inv[16];
     const float* tgt;
     uint32_t W, H;
+    ExtTune tune;
 };
 
 struct npt_lane {
@@ -1075,7 +1155,7 @@ int npt_fail(npt_handle t, int code, const std::string& msg) {
         if (e_ != hipSuccess) return npt_fail((t), NPS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-ViewParams npt_view_params(const npt_view& w) { return make_view(w.vp, w.vp_inv, w.W, w.H); }
+ViewParams npt_view_params(const npt_view& w) { return make_view(w.vp, w.vp_inv, w.W, w.H, (uint32_t)w.tune.tile); }
 
 void npt_drop_graph(npt_handle t) {
     if (t->exec) (void)hipGraphExecDestroy(t->exec);
@@ -1114,8 +1194,8 @@ int npt_step_sequential(npt_handle t) {
         ViewParams v = npt_view_params(w);
         float cam[3];
         ext_camera_centre(w.vp_inv, cam);
-        int rc = run_ext_view(scene_of(h), v, cam, (const float4*)w.tgt, h->xv, h->stream, true, false, t->err,
-                              nullptr, nullptr);
+        int rc = run_ext_view(scene_of(h), v, w.tune.batch, cam, (const float4*)w.tgt, h->xv, h->stream, true, false,
+                              t->err, nullptr, nullptr);
         if (rc != NPS_OK) return rc;
         launch_sum_tile_loss(h->stream, (const double*)h->xv.tile_loss.p, v.num_tiles,
                              (double*)t->view_loss.p + 2 * vi);
@@ -1129,7 +1209,6 @@ int npt_step_sequential(npt_handle t) {
 int npt_plan(npt_handle t) {
     npt_drop_graph(t);
     const int L = (int)t->lanes.size();
-    std::vector<uint32_t> nmax(L, 1), wmax(L, 1), hmax(L, 1);
     std::vector<uint64_t> mmax(L, 0);
     for (int vi : t->order) {
         const npt_view& w = t->views[vi];
@@ -1138,23 +1217,25 @@ int npt_plan(npt_handle t) {
         ViewParams v = npt_view_params(w);
         float cam[3];
         ext_camera_centre(w.vp_inv, cam);
-        int rc = run_ext_view(scene_of(h), v, cam, (const float4*)w.tgt, ln.x, ln.st, false, false, t->err, nullptr,
-                              nullptr);
+        int rc = run_ext_view(scene_of(h), v, w.tune.batch, cam, (const float4*)w.tgt, ln.x, ln.st, false, false,
+                              t->err, nullptr, nullptr);
         if (rc != NPS_OK) return rc;
         const int l = t->lane_of[vi];
-        nmax[l] = std::max(nmax[l], h->n);
-        wmax[l] = std::max(wmax[l], w.W);
-        hmax[l] = std::max(hmax[l], w.H);
         mmax[l] = std::max(mmax[l], ln.x.m);
     }
     for (int l = 0; l < L; ++l) {
         npt_lane& ln = t->lanes[l];
         ln.x.cap = (uint64_t)((double)ext_key_capacity(mmax[l]) * t->cap_scale);
         ln.x.flags_at = (uint32_t*)t->flags.p + 16 * l;
-        int rc = ext_view_reserve(ln.x, nmax[l], wmax[l], hmax[l], ln.x.cap, true, ln.st, t->err);
-        if (rc != NPS_OK) return rc;
-        NPT_HIP(t, hipStreamSynchronize(ln.st));
     }
+    // every buffer of a lane large enough for each of its views (tile and batch differ by view)
+    for (int vi : t->order) {
+        const npt_view& w = t->views[vi];
+        npt_lane& ln = t->lanes[t->lane_of[vi]];
+        int rc = ext_view_reserve(ln.x, t->scenes[w.scene]->n, w.W, w.H, ln.x.cap, true, w.tune, ln.st, t->err);
+        if (rc != NPS_OK) return rc;
+    }
+    for (auto& ln : t->lanes) NPT_HIP(t, hipStreamSynchronize(ln.st));
     t->view_base.assign(t->views.size(), 0);
     size_t total = 0;
     for (size_t vi = 0; vi < t->views.size(); ++vi) {
@@ -1183,8 +1264,8 @@ int npt_plan(npt_handle t) {
             out.counts = (uint32_t*)t->view_counts.p + t->view_base[vi];
             out.g2 = (double*)t->view_g2.p + 9 * t->view_base[vi];
             out.defer_chain = true;
-            rc = run_ext_view(scene_of(h), v, cam, (const float4*)w.tgt, ln.x, ln.st, true, true, t->err, &out,
-                              nullptr);
+            rc = run_ext_view(scene_of(h), v, w.tune.batch, cam, (const float4*)w.tgt, ln.x, ln.st, true, true, t->err,
+                              &out, nullptr);
             if (rc != NPS_OK) return;
             launch_sum_tile_loss(ln.st, (const double*)ln.x.tile_loss.p, v.num_tiles,
                                  (double*)t->view_loss.p + 2 * vi);
@@ -1341,6 +1422,7 @@ int npt_add_view(npt_handle t, int32_t scene, const float vp[16], const float vp
     v.tgt = rgba_dev;
     v.W = (uint32_t)w;
     v.H = (uint32_t)hgt;
+    v.tune = t->scenes[scene]->tune;
     t->views.push_back(v);
     // step order: most pixels first (longest-processing-time order), then index;
     // lanes are filled greedily with the least-loaded lane first
@@ -1371,6 +1453,45 @@ int npt_step(npt_handle t, int32_t mode) {
     if (mode == 0) return npt_step_sequential(t);
     if (mode == 1) return npt_step_graph(t);
     return npt_fail(t, NPS_ERR_ARG, "mode must be 0 (per view) or 1 (graph)");
+}
+
+int npt_autotune(npt_handle t) {
+    if (!t) return NPS_ERR_ARG;
+    if (t->views.empty()) return npt_fail(t, NPS_ERR_STATE, "no views");
+    NPT_HIP(t, hipSetDevice(t->device));
+    std::vector<hipEvent_t> ev(8, nullptr);
+    for (auto& e : ev) NPT_HIP(t, hipEventCreate(&e));
+    int rc = NPS_OK;
+    std::vector<bool> done(t->views.size(), false);
+    for (int vi : t->order) {
+        if (done[vi]) continue;
+        const npt_view& w = t->views[vi];
+        float t9[9];
+        ExtTune best = w.tune;
+        rc = ext_time_variants(scene_of(t->scenes[w.scene]), w.vp, w.vp_inv, w.W, w.H, (const float4*)w.tgt,
+                               t->lanes[0].x, t->lanes[0].st, ev.data(), t->err, t9, &best);
+        if (rc != NPS_OK) break;
+        // the same choice for every view of this scene at this resolution
+        for (size_t k = 0; k < t->views.size(); ++k) {
+            npt_view& o = t->views[k];
+            if (o.scene == w.scene && o.W == w.W && o.H == w.H) {
+                o.tune = best;
+                done[k] = true;
+            }
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    npt_drop_graph(t);
+    return rc;
+}
+
+int npt_read_tune(npt_handle t, int32_t* tile_batch2v) {
+    if (!t || !tile_batch2v) return NPS_ERR_ARG;
+    for (size_t k = 0; k < t->views.size(); ++k) {
+        tile_batch2v[2 * k] = t->views[k].tune.tile;
+        tile_batch2v[2 * k + 1] = t->views[k].tune.batch;
+    }
+    return NPS_OK;
 }
 
 int npt_read_losses(npt_handle t, double* loss2v) {

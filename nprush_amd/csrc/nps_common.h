@@ -26,7 +26,8 @@ struct ViewParams {
     uint32_t W, H;     // target dimensions
     float aspect;      // f32(W) / f32(H)
     uint32_t tiles_x, tiles_y, num_tiles;
-    uint32_t pad[2];
+    uint32_t tile;     // tile edge in pixels (16 in REF mode)
+    uint32_t pad;
 };
 
 // Per-splat forward record (64 B): everything the per-pair forward needs.

@@ -39,15 +39,26 @@ constexpr float kExtMaxAlpha = 0.99f;
 constexpr float kExtTMin = 0.0001f;                    // early termination threshold
 constexpr float kExtBg = 0.05f;
 
+// The pixel test of the 3-sigma box, on the same dx, dy as r^2.
+__device__ __forceinline__ bool in_box(float dx, float dy, float ex, float ey) {
+    return fabsf(dx) <= ex && fabsf(dy) <= ey;
+}
+
 // Per-splat record of the EXT composite (48 B).
 struct alignas(16) ExtRec {
     float scr_x, scr_y;  // projected centre (aspect-corrected NDC)
     float A, B, C;       // conic
     float op;            // sigmoid(logit)
     float r, g, b;       // max(SH + 0.5, 0)
-    float rcut2;         // r^2 beyond which alpha < 1/255 is guaranteed (-1: never drawn)
-    float pad0, pad1;
+    float rcut2;         // r^2 beyond which alpha < 1/255 is guaranteed (-inf: never drawn)
+    float ext_x, ext_y;  // 3-sigma box half-extents: a pixel sees the splat only inside it
 };
+
+// Binning box = 3-sigma box * kExtBinScale + kExtBinPad: wide enough that a
+// tile list holds every splat whose exact box contains one of the tile's pixel
+// centres (the pixel test then makes images independent of the tile size).
+constexpr float kExtBinScale = 1.00001f;
+constexpr float kExtBinPad = 1e-6f;
 static_assert(sizeof(ExtRec) == 48, "ExtRec layout");
 
 struct ExtProj {

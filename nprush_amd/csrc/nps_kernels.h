@@ -99,8 +99,8 @@ struct ExtCompositeArgs {
     double* g2;                 // 9 per splat: d scr.xy, d conic A,2B,C, d rgb, d opacity
     float* c_img;               // H*W*3
     double* tile_loss;          // 2 * num_tiles: L2, L1
-    float* ckpt;                // per batch slot: 8 chunks x 256 pixels of T
-    uint64_t* act;              // per batch slot: 4 waves x 64 entry bits
+    float* ckpt;                // per batch slot: batch/8 chunks x tile^2 pixels of T
+    uint64_t* act;              // per batch slot and wave: batch entry bits
     unsigned long long* stats;  // [0] forward pairs, [2] reverse pairs, [3] reverse wave-entries,
                                 // [4] forward wave-entries
     const uint32_t* order;      // workgroup -> tile (launch_tile_order), ~0 = none
@@ -119,7 +119,12 @@ void launch_ext_rank_counts(hipStream_t st, uint32_t n, const uint64_t* order, c
 void launch_ext_keys(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts,
                      const uint4* rects, uint32_t tiles_x, const uint64_t* offsets, uint64_t* keys, uint64_t cap,
                      uint32_t* overflow);
-void launch_composite_ext(hipStream_t st, const ViewParams& v, const ExtCompositeArgs& a);
+// tile edge v.tile in {8, 16, 32}, records staged `batch` in {32, 64, 128} at a time
+bool ext_tune_valid(int tile, int batch);
+void launch_composite_ext(hipStream_t st, const ViewParams& v, const ExtCompositeArgs& a, int batch);
+// batch slots of a view: keys/batch + tiles + 1; checkpoint floats and mask words per slot set
+size_t ext_ckpt_floats(size_t slots, int tile, int batch);
+size_t ext_act_words(size_t slots, int tile, int batch);
 void launch_ext_chain(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg, const ViewParams& v,
                       const float cam[3], const double* g2, const uint32_t* counts, float* grads, float* accum);
 // ctl = {t, 1 - 0.9^t, 1 - 0.999^t, skip} on the device: t advances by one

@@ -59,7 +59,7 @@ class GpuSplatOptimizerManager:
     def __init__(self, device=None, format=None, numSplats: int = 512, numBeziers=None, *,
                  max_tile_splats: int = 1024, adc_period: int = 25, adc_enabled: bool = True, hip_device: int = 0,
                  splats: np.ndarray | None = None, mode: str = "ref", sh_degree: int = 0,
-                 sh: np.ndarray | None = None):
+                 sh: np.ndarray | None = None, ext_tile: int = 16, ext_batch: int = 64):
         lib = _lib.load()
         cfg = _lib.NpsConfig()
         check(lib.nps_config_default(C.byref(cfg)), None, "nps_config_default")
@@ -71,6 +71,8 @@ class GpuSplatOptimizerManager:
             raise ValueError("mode must be 'ref' or 'ext'")
         cfg.mode = _lib.MODE_EXT if mode == "ext" else _lib.MODE_REF
         cfg.sh_degree = int(sh_degree)
+        cfg.ext_tile = int(ext_tile)
+        cfg.ext_batch = int(ext_batch)
         h = C.c_void_p()
         check(lib.nps_create(C.byref(cfg), int(numSplats), C.byref(h)), None, "nps_create")
         self._lib = lib
@@ -207,6 +209,14 @@ class GpuSplatOptimizerManager:
     # ---- the three passes dispatch() records, callable one at a time ----
     def backward(self):
         check(self._lib.nps_backward(self._h), self._h, "backward")
+
+    def autotune(self):
+        """EXT: picks the fastest composite tile size / LDS batch for the current
+        camera and target; returns (tile, batch, ms[3, 3])."""
+        t, b = C.c_int32(), C.c_int32()
+        ms = np.zeros(9, dtype=np.float32)
+        check(self._lib.nps_autotune(self._h, C.byref(t), C.byref(b), fptr(ms)), self._h, "autotune")
+        return int(t.value), int(b.value), ms.reshape(3, 3)
 
     def forward(self):
         """EXT: render the current camera at the target's size (no gradients)."""
@@ -370,6 +380,13 @@ class GpuMultiViewStep:
 
     def step(self, graph: bool = True):
         self._check(self._lib.npt_step(self._t, 1 if graph else 0), "step")
+
+    def autotune(self) -> np.ndarray:
+        """Chooses tile size and LDS batch per (scene, resolution); returns [V, 2]."""
+        self._check(self._lib.npt_autotune(self._t), "autotune")
+        out = np.zeros((self.num_views, 2), dtype=np.int32)
+        self._check(self._lib.npt_read_tune(self._t, iptr(out)), "read_tune")
+        return out
 
     def read_losses(self) -> np.ndarray:
         out = np.zeros((self.num_views, 2), dtype=np.float64)

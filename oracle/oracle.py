@@ -160,9 +160,11 @@ def math_f32(op, x, y=None):
 
 # ---------------------------------------------------------------- EXT path
 def _ext_sig(lib):
-    lib.ora_ext_bin.argtypes = [_F, _F, i32, i32, _F, _F, i32, i32, _I, i32, i32, _U, _I]
-    lib.ora_ext_backward_f32.argtypes = [_F, _F, i32, i32, _F, _F, _F, i32, i32, _I, i32, _U, _I, i32, _D, _D, _F, _D]
-    lib.ora_ext_backward_f64.argtypes = [_D, _D, i32, i32, _D, _D, _F, i32, i32, _I, i32, _U, _I, i32, _D, _D, _D, _D]
+    lib.ora_ext_bin.argtypes = [_F, _F, i32, i32, _F, _F, i32, i32, i32, _I, i32, i32, _U, _I]
+    lib.ora_ext_backward_f32.argtypes = [_F, _F, i32, i32, _F, _F, _F, i32, i32, i32, _I, i32, _U, _I, i32, _D, _D, _F,
+                                         _D]
+    lib.ora_ext_backward_f64.argtypes = [_D, _D, i32, i32, _D, _D, _F, i32, i32, i32, _I, i32, _U, _I, i32, _D, _D, _D,
+                                         _D]
     for k in ("ora_ext_bin", "ora_ext_backward_f32", "ora_ext_backward_f64"):
         getattr(lib, k).restype = i32
 
@@ -172,21 +174,22 @@ def cam_from_inv(vp_inv):
     return np.array([vi[8] / vi[11], vi[9] / vi[11], vi[10] / vi[11]], np.float64)
 
 
-def ext_bin(splats, sh, deg, vp, vp_inv, W, H, stride=None, tiles=None):
+def ext_bin(splats, sh, deg, vp, vp_inv, W, H, stride=None, tiles=None, tile=16):
     lib = load()
     _ext_sig(lib)
     s, h = f32(splats), f32(sh)
     n = s.shape[0]
-    tiles = np.arange(tiles_of(W, H), dtype=np.int32) if tiles is None else np.ascontiguousarray(tiles, np.int32)
+    nt = ((W + tile - 1) // tile) * ((H + tile - 1) // tile)
+    tiles = np.arange(nt, dtype=np.int32) if tiles is None else np.ascontiguousarray(tiles, np.int32)
     stride = n if stride is None else stride
     lists = np.zeros((len(tiles), max(stride, 1)), np.uint32)
     counts = np.zeros(len(tiles), np.int32)
-    lib.ora_ext_bin(_p(s, _F), _p(h, _F), n, deg, _p(f32(vp), _F), _p(f32(vp_inv), _F), W, H, _p(tiles, _I),
+    lib.ora_ext_bin(_p(s, _F), _p(h, _F), n, deg, _p(f32(vp), _F), _p(f32(vp_inv), _F), W, H, tile, _p(tiles, _I),
                     len(tiles), max(stride, 1), _p(lists, _U), _p(counts, _I))
     return tiles, lists, counts
 
 
-def ext_backward_f32(splats, sh, deg, vp, vp_inv, tgt, W, H, tiles, lists, counts, want_grads=True):
+def ext_backward_f32(splats, sh, deg, vp, vp_inv, tgt, W, H, tiles, lists, counts, want_grads=True, tile=16):
     lib = load()
     _ext_sig(lib)
     s, h = f32(splats), f32(sh)
@@ -196,12 +199,12 @@ def ext_backward_f32(splats, sh, deg, vp, vp_inv, tgt, W, H, tiles, lists, count
     c = np.zeros((H, W, 3), np.float32)
     loss = np.zeros(3, np.float64)
     lib.ora_ext_backward_f32(_p(s, _F), _p(h, _F), n, deg, _p(f32(vp), _F), _p(f32(vp_inv), _F), _p(f32(tgt), _F), W,
-                             H, _p(tiles, _I), len(tiles), _p(lists, _U), _p(counts, _I), lists.shape[1], _p(g, _D),
+                             H, tile, _p(tiles, _I), len(tiles), _p(lists, _U), _p(counts, _I), lists.shape[1], _p(g, _D),
                              _p(gs, _D), _p(c, _F), _p(loss, _D))
     return g, gs, c, loss
 
 
-def ext_backward_f64(splats64, sh64, deg, vp64, cam64, tgt, W, H, tiles, lists, counts, want_grads=True):
+def ext_backward_f64(splats64, sh64, deg, vp64, cam64, tgt, W, H, tiles, lists, counts, want_grads=True, tile=16):
     lib = load()
     _ext_sig(lib)
     s = np.ascontiguousarray(splats64, np.float64)
@@ -212,7 +215,7 @@ def ext_backward_f64(splats64, sh64, deg, vp64, cam64, tgt, W, H, tiles, lists, 
     c = np.zeros((H, W, 3), np.float64)
     loss = np.zeros(3, np.float64)
     lib.ora_ext_backward_f64(_p(s, _D), _p(h, _D), n, deg, _p(np.ascontiguousarray(vp64, np.float64), _D),
-                             _p(np.ascontiguousarray(cam64, np.float64), _D), _p(f32(tgt), _F), W, H, _p(tiles, _I),
+                             _p(np.ascontiguousarray(cam64, np.float64), _D), _p(f32(tgt), _F), W, H, tile, _p(tiles, _I),
                              len(tiles), _p(lists, _U), _p(counts, _I), lists.shape[1], _p(g, _D), _p(gs, _D),
                              _p(c, _D), _p(loss, _D))
     return g, gs, c, loss

@@ -12,6 +12,7 @@
  *   M M^T + (0.3 px)^2 I with conic (BW:109-126); cull w < 0.1;
  *   colour = max(SH(d) + 0.5, 0), d = normalise(mean - camera centre), real SH
  *   basis to degree 3;  alpha = min(0.99, opacity exp(-r^2/2)), r^2 = d^T Sigma^-1 d;
+ *   a pixel sees a splat only if its centre lies in the splat's 3-sigma box;
  *   pairs with alpha < 1/255 skipped; front-to-back in increasing view depth w
  *   (ties by id); a pixel stops before the splat that would bring T below 1e-4;
  *   background 0.05; loss L1 = sum |C - target|;
@@ -58,31 +59,32 @@ static void cam_from_inv(const float* vi, double* c) {
 }
 
 int ora_ext_bin(const float* splats, const float* sh, int n, int deg, const float* vp, const float* vp_inv, int W,
-                int H, const int32_t* tiles, int ntiles, int stride, uint32_t* lists, int32_t* counts) {
+                int H, int tile, const int32_t* tiles, int ntiles, int stride, uint32_t* lists, int32_t* counts) {
     double c[3];
     cam_from_inv(vp_inv, c);
     float cam[3] = { (float)c[0], (float)c[1], (float)c[2] };
-    ext_bin_xf32(splats, sh, n, deg, vp, cam, (uint32_t)W, (uint32_t)H, tiles, ntiles, stride, lists, counts);
+    ext_bin_xf32(splats, sh, n, deg, vp, cam, (uint32_t)W, (uint32_t)H, (uint32_t)tile, tiles, ntiles, stride, lists,
+                 counts);
     return 0;
 }
 
 int ora_ext_backward_f32(const float* splats, const float* sh, int n, int deg, const float* vp, const float* vp_inv,
-                         const float* tgt, int W, int H, const int32_t* tiles, int ntiles, const uint32_t* lists,
+                         const float* tgt, int W, int H, int tile, const int32_t* tiles, int ntiles, const uint32_t* lists,
                          const int32_t* counts, int stride, double* grads, double* gsh, float* c_out, double* loss) {
     double c[3];
     cam_from_inv(vp_inv, c);
     float cam[3] = { (float)c[0], (float)c[1], (float)c[2] };
-    ext_backward_xf32(splats, sh, n, deg, vp, cam, tgt, (uint32_t)W, (uint32_t)H, tiles, ntiles, lists, counts, stride,
-                      grads, gsh, c_out, loss);
+    ext_backward_xf32(splats, sh, n, deg, vp, cam, tgt, (uint32_t)W, (uint32_t)H, (uint32_t)tile, tiles, ntiles, lists,
+                      counts, stride, grads, gsh, c_out, loss);
     return 0;
 }
 
 int ora_ext_backward_f64(const double* splats, const double* sh, int n, int deg, const double* vp,
-                         const double* cam, const float* tgt, int W, int H, const int32_t* tiles, int ntiles,
+                         const double* cam, const float* tgt, int W, int H, int tile, const int32_t* tiles, int ntiles,
                          const uint32_t* lists, const int32_t* counts, int stride, double* grads, double* gsh,
                          double* c_out, double* loss) {
-    ext_backward_xf64(splats, sh, n, deg, vp, cam, tgt, (uint32_t)W, (uint32_t)H, tiles, ntiles, lists, counts, stride,
-                      grads, gsh, c_out, loss);
+    ext_backward_xf64(splats, sh, n, deg, vp, cam, tgt, (uint32_t)W, (uint32_t)H, (uint32_t)tile, tiles, ntiles, lists,
+                      counts, stride, grads, gsh, c_out, loss);
     return 0;
 }
 

@@ -263,3 +263,56 @@ def test_ext_training_reduces_l1(oracle):
             assert st["adc_rounds"] >= cap - n_live        # the dead pool was seen
     assert losses[-1] < 0.9 * losses[0], (losses[0], losses[-1])
     mgr.close()
+
+
+@pytest.mark.parametrize("tile,batch", [(8, 32), (8, 128), (16, 32), (16, 128), (32, 32), (32, 64), (32, 128)])
+def test_ext_tile_and_batch_variants(oracle, tile, batch):
+    """Every composite variant the autotuner can choose: tile lists bit-identical
+    to the oracle's at that tile size, images bit-identical to the 16-pixel
+    oracle image (a pixel's result does not depend on the tiling), gradients
+    within tolerance."""
+    W, H = 200, 136
+    deg = 3
+    n = 6000
+    s, sh = scenes.make_ext_splats(n, 8, sh_degree=deg)
+    vp, vi = camera.view_proj((0.4, -0.3, 2.7), aspect=W / H)
+    rng = np.random.default_rng(5)
+    tgt = np.ones((H, W, 4), np.float32)
+    tgt[..., :3] = rng.random((H, W, 3)).astype(np.float32)
+    mgr = GpuSplatOptimizerManager(numSplats=n, mode="ext", sh_degree=deg, splats=s, sh=sh, adc_enabled=False,
+                                   ext_tile=tile, ext_batch=batch)
+    mgr.writeSplatVPMatrix(vp, vi)
+    mgr.setBackwardTarget(tgt, np.ones((H, W), np.float32), W, H)
+    mgr.backward()
+    ranges, ids, full = mgr.read_tiles()
+    tiles, lists, counts = oracle.ext_bin(s, sh, deg, vp, vi, W, H, tile=tile)
+    assert np.array_equal(full.astype(np.int64), counts.astype(np.int64))
+    for t in range(len(tiles)):
+        a, b = ranges[t]
+        assert np.array_equal(ids[a:b], lists[t, :counts[t]]), t
+    t16, l16, c16 = oracle.ext_bin(s, sh, deg, vp, vi, W, H)
+    g, gs, c, loss = oracle.ext_backward_f32(s, sh, deg, vp, vi, tgt, W, H, t16, l16, c16)
+    img, _ = mgr.read_image()
+    assert np.array_equal(img, c)
+    _check_grads(mgr.read_grads_f32(), g, gs, deg)
+    mgr.close()
+
+
+def test_ext_autotune(oracle):
+    W, H = 320, 200
+    n = 8000
+    s, sh = scenes.make_ext_splats(n, 9, sh_degree=1)
+    vp, vi = camera.view_proj((0.0, 0.5, 3.0), aspect=W / H)
+    tgt = np.ones((H, W, 4), np.float32)
+    tgt[..., :3] = 0.4
+    mgr = _run(s, sh, 1, vp, vi, tgt, W, H)
+    img0, _ = mgr.read_image()
+    g0 = mgr.read_grads_f32()
+    tile, batch, ms = mgr.autotune()
+    assert tile in (8, 16, 32) and batch in (32, 64, 128)
+    assert np.all(ms > 0) and ms[(8, 16, 32).index(tile), (32, 64, 128).index(batch)] == ms.min()
+    assert np.array_equal(mgr.read_grads_f32(), g0)      # timing runs add no gradients
+    mgr.backward()
+    img1, _ = mgr.read_image()
+    assert np.array_equal(img1, img0)
+    mgr.close()

@@ -73,3 +73,25 @@ def test_graph_step_equals_per_view_step():
     sa.close(); sb.close()
     for mgr in ma + mb:
         mgr.close()
+
+
+def test_autotuned_graph_step_matches_per_view():
+    data = [_scene(2500, 2, 3000), _scene(4000, 3, 4500)]
+    views = _views(7)
+    ma, sa = _build(views, data)
+    mb, sb = _build(views, data)
+    tune = sb.autotune()
+    assert tune.shape == (len(views), 2)
+    assert set(tune[:, 0]) <= {8, 16, 32} and set(tune[:, 1]) <= {32, 64, 128}
+    for k in range(2):
+        sa.step(graph=False)
+        sb.step(graph=True)
+        la, lb = sa.read_losses(), sb.read_losses()
+        # same images; the FP64 loss sums group pixels by tile, so only their rounding may differ
+        assert np.allclose(la, lb, rtol=1e-12, atol=0), k
+        for x, y in zip(ma, mb):
+            gx, gy = x.read_splats(), y.read_splats()
+            assert np.allclose(gx, gy, rtol=1e-5, atol=1e-6), k
+    sa.close(); sb.close()
+    for mgr in ma + mb:
+        mgr.close()

@@ -123,3 +123,30 @@ def test_ext_depth_order_and_early_stop(oracle):
     px = c[8, 8].astype(np.float64)
     assert abs(px[2] - (0.99 + 0.05 * 0.01)) < 1e-6
     assert abs(px[1] - 0.05 * 0.01) < 1e-7 and abs(px[0] - 0.05 * 0.01) < 1e-7
+
+
+def test_images_do_not_depend_on_tile_size(oracle):
+    """Each pixel applies the splat's 3-sigma box itself and the binning box is
+    wider, so tile lists of any size are supersets of what a pixel can see:
+    images are identical for 8, 16 and 32 pixel tiles (gradients up to the FP64
+    summation order)."""
+    W, H = 100, 72
+    s, sh = scenes.make_ext_splats(1500, 6, sh_degree=1)
+    s[:, 0:3] *= 0.6
+    vp, vi = camera.view_proj((0.3, 0.2, 2.6), aspect=W / H)
+    rng = np.random.default_rng(3)
+    tgt = np.ones((H, W, 4), np.float32)
+    tgt[..., :3] = rng.random((H, W, 3))
+    out = {}
+    for tile in (8, 16, 32):
+        tiles, lists, counts = oracle.ext_bin(s, sh, 1, vp, vi, W, H, tile=tile)
+        out[tile] = oracle.ext_backward_f32(s, sh, 1, vp, vi, tgt, W, H, tiles, lists, counts, tile=tile)
+    g16, gs16, c16, l16 = out[16]
+    for tile in (8, 32):
+        g, gs, c, l = out[tile]
+        assert np.array_equal(c, c16)
+        assert abs(l[0] - l16[0]) <= 1e-12 * l16[0]
+        assert np.allclose(g, g16, rtol=1e-12, atol=1e-12 * np.abs(g16).max())
+        assert np.allclose(gs, gs16, rtol=1e-12, atol=1e-12 * np.abs(gs16).max())
+    # a box that ignored the pixel test would see more splats with bigger tiles
+    assert out[32][3][2] > out[8][3][2]
