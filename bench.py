@@ -143,7 +143,8 @@ def batched_config2(steps=3, warmup=1, lanes=4):
             if v[0] == sidx:
                 tgt_of[k] = next(it)
     out = {}
-    for graph in (False, True):
+    for graph, tuned in ((False, False), (True, False), (True, True)):
+        key = ("graph" if graph else "per_view") + ("_tuned" if tuned else "")
         mgrs = []
         for sidx, n in enumerate(counts):
             s, sh = scenes.make_ext_splats(n, sidx, sh_degree=3)
@@ -152,50 +153,52 @@ def batched_config2(steps=3, warmup=1, lanes=4):
         step = GpuMultiViewStep(mgrs, lanes=lanes)
         for k, (sc, w, h, vp, vi) in enumerate(views):
             step.add_view(sc, vp, vi, tgt_of[k])
+        if tuned:
+            tune = step.autotune()
+            out["tune"] = {f"{sz[0]}x{sz[1]}": [int(x) for x in tune[[i for i, v in enumerate(views)
+                                                                       if (v[1], v[2]) == sz][0]]]
+                           for sz in sizes}
         for _ in range(warmup):
             step.step(graph=graph)
         t0 = time.perf_counter()
         for _ in range(steps):
             step.step(graph=graph)
-        out["graph" if graph else "per_view"] = (time.perf_counter() - t0) * 1000.0 / steps
+        out[key] = (time.perf_counter() - t0) * 1000.0 / steps
         losses = step.read_losses()
-        out["l1_" + ("graph" if graph else "per_view")] = float(losses[:, 1].sum())
+        out["l1_" + key] = float(losses[:, 1].sum())
         step.close()
         for m in mgrs:
             m.close()
     return {
         "workload": "4 scenes 50k/150k/300k/500k SH3, 64 views of 256^2/512^2/1024x768/1920x1080 (seed 4)",
         "lanes": lanes,
-        "graph_ms_per_step": round(out["graph"], 3),
         "per_view_ms_per_step": round(out["per_view"], 3),
-        "speedup": round(out["per_view"] / out["graph"], 3),
+        "graph_ms_per_step": round(out["graph"], 3),
+        "graph_autotuned_ms_per_step": round(out["graph_tuned"], 3),
+        "autotuned_tile_batch_by_resolution (scene 0)": out["tune"],
+        "speedup_graph": round(out["per_view"] / out["graph"], 3),
+        "speedup_graph_autotuned": round(out["per_view"] / out["graph_tuned"], 3),
         "same_losses": out["l1_graph"] == out["l1_per_view"],
     }
 
 
-def cpu_baseline_ext(s, sh, deg, vp, vi, target, W, H, sample_tiles, seed=0):
-    """The EXT oracle (FP32, all host cores): binning of the sampled tiles
-    against all splats (depth sort per tile) + forward + L1 + backward + chain
-    rule, scaled to the whole view by the number of tiles."""
+def cpu_baseline_ext(s, sh, deg, vp, vi, target, W, H, sample_tiles=0, seed=0):
+    """The EXT oracle (FP32, OpenMP on all host cores) on one whole view:
+    projection, binning in O(keys) with the per-tile depth sort, forward + L1 +
+    backward over every tile, and the per-splat chain rule."""
     from oracle import oracle
 
     oracle.load()
-    ntiles = ((W + 15) // 16) * ((H + 15) // 16)
-    rng = np.random.default_rng(seed)
-    tiles = np.sort(rng.choice(ntiles, size=min(sample_tiles, ntiles), replace=False)).astype(np.int32)
     t0 = time.perf_counter()
-    _, lists, counts = oracle.ext_bin(s, sh, deg, vp, vi, W, H, stride=20000, tiles=tiles)
-    if int(counts.max()) > lists.shape[1]:
-        _, lists, counts = oracle.ext_bin(s, sh, deg, vp, vi, W, H, stride=int(counts.max()), tiles=tiles)
-    oracle.ext_backward_f32(s, sh, deg, vp, vi, target, W, H, tiles, lists, counts)
+    offs, ids = oracle.ext_bin_all(s, sh, deg, vp, vi, W, H)
+    oracle.ext_backward_csr_f32(s, sh, deg, vp, vi, target, W, H, offs, ids)
     dt = time.perf_counter() - t0
     return {
-        "value": round(dt * 1000.0 * ntiles / len(tiles), 1),
+        "value": round(dt * 1000.0, 1),
         "unit": "ms/view (fwd+bwd)",
         "cores": int(oracle.load().ora_num_threads()),
         "kind": "port",
-        "sample": f"{len(tiles)} random tiles of view 0 ({dt:.1f} s incl. a projection of all splats), "
-                  f"scaled by {ntiles}/{len(tiles)} tiles",
+        "sample": f"one whole view (view 0, {len(ids)} tile keys), binning + fwd + L1 + bwd + chain rule",
     }
 
 
@@ -359,6 +362,11 @@ def main_ext(args):
     mgr = GpuSplatOptimizerManager(numSplats=n, mode="ext", sh_degree=deg, splats=s, sh=sh, adc_period=100,
                                    adc_enabled=False)
     slots = [mgr.upload_target(t, depth) for t in targets]
+    # composite tile size and LDS batch chosen by the autotuner on view 0
+    mgr.writeSplatVPMatrix(*cams[0])
+    mgr.set_target_slot(slots[0])
+    tile, batch, tune_ms = mgr.autotune()
+    print(f"[bench] autotune: tile {tile}, batch {batch}", file=sys.stderr, flush=True)
     mgr.enable_timing(True)
 
     def one_step(k):
@@ -430,7 +438,10 @@ def main_ext(args):
         "data": "synthetic (seeded scenes, targets rendered from a seed-1 scene)",
         "config": {"workload": f"EXT 3D Gaussian fwd+bwd+Adam, {n} Gaussians, SH degree {deg}, {W}x{H}, "
                                f"{views} ring views, 16x16 tiles, depth-ordered uncapped lists, L1",
-                   "views": views, "width": W, "height": H, "splats": n, "sh_degree": deg},
+                   "views": views, "width": W, "height": H, "splats": n, "sh_degree": deg,
+                   "tile": tile, "lds_batch": batch},
+        "autotune_ms": {f"tile{t}_batch{b}": round(float(tune_ms[i, j]), 3) for i, t in enumerate((8, 16, 32))
+                        for j, b in enumerate((32, 64, 128))},
         "roofline": {"bound": "valu", "kernel": "k_composite_ext", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": traffic},

@@ -268,3 +268,39 @@ def ext_adc_normal(seed, slot, child, k):
     lib.ora_ext_adc_normal.argtypes = [C.c_uint32] * 4
     lib.ora_ext_adc_normal.restype = C.c_float
     return float(lib.ora_ext_adc_normal(seed, slot, child, k))
+
+
+def ext_bin_all(splats, sh, deg, vp, vp_inv, W, H, tile=16):
+    """Whole-view tile lists in O(keys), CSR: (offs [T+1] int64, ids [M] uint32)."""
+    lib = load()
+    lib.ora_ext_bin_all.argtypes = [_F, _F, i32, i32, _F, _F, i32, i32, i32, _L, _U]
+    lib.ora_ext_bin_all.restype = C.c_int64
+    s, h = f32(splats), f32(sh)
+    T = ((W + tile - 1) // tile) * ((H + tile - 1) // tile)
+    offs = np.zeros(T + 1, np.int64)
+    m = lib.ora_ext_bin_all(_p(s, _F), _p(h, _F), s.shape[0], deg, _p(f32(vp), _F), _p(f32(vp_inv), _F), W, H, tile,
+                            _p(offs, _L), None)
+    ids = np.zeros(max(int(m), 1), np.uint32)
+    lib.ora_ext_bin_all(_p(s, _F), _p(h, _F), s.shape[0], deg, _p(f32(vp), _F), _p(f32(vp_inv), _F), W, H, tile,
+                        _p(offs, _L), _p(ids, _U))
+    return offs, ids[:m]
+
+
+def ext_backward_csr_f32(splats, sh, deg, vp, vp_inv, tgt, W, H, offs, ids, want_grads=True, tile=16):
+    T = ((W + tile - 1) // tile) * ((H + tile - 1) // tile)
+    if len(offs) != T + 1:
+        raise ValueError(f"offs has {len(offs)} entries, {T + 1} expected for {tile}-pixel tiles")
+    lib = load()
+    lib.ora_ext_backward_csr_f32.argtypes = [_F, _F, i32, i32, _F, _F, _F, i32, i32, i32, _U, _L, _D, _D, _F, _D]
+    lib.ora_ext_backward_csr_f32.restype = i32
+    s, h = f32(splats), f32(sh)
+    n = s.shape[0]
+    g = np.zeros((n, 16), np.float64) if want_grads else None
+    gs = np.zeros((n, 48), np.float64) if want_grads else None
+    c = np.zeros((H, W, 3), np.float32)
+    loss = np.zeros(3, np.float64)
+    idsc = np.ascontiguousarray(ids, np.uint32) if len(ids) else np.zeros(1, np.uint32)
+    lib.ora_ext_backward_csr_f32(_p(s, _F), _p(h, _F), n, deg, _p(f32(vp), _F), _p(f32(vp_inv), _F), _p(f32(tgt), _F),
+                                 W, H, tile, _p(idsc, _U), _p(np.ascontiguousarray(offs, np.int64), _L), _p(g, _D),
+                                 _p(gs, _D), _p(c, _F), _p(loss, _D))
+    return g, gs, c, loss

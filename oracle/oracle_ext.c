@@ -75,7 +75,34 @@ int ora_ext_backward_f32(const float* splats, const float* sh, int n, int deg, c
     cam_from_inv(vp_inv, c);
     float cam[3] = { (float)c[0], (float)c[1], (float)c[2] };
     ext_backward_xf32(splats, sh, n, deg, vp, cam, tgt, (uint32_t)W, (uint32_t)H, (uint32_t)tile, tiles, ntiles, lists,
-                      counts, stride, grads, gsh, c_out, loss);
+                      counts, stride, NULL, grads, gsh, c_out, loss);
+    return 0;
+}
+
+/* Whole-view binning in O(keys) (CSR): returns the key count; with ids == NULL
+ * only offs[T + 1] is filled. */
+int64_t ora_ext_bin_all(const float* splats, const float* sh, int n, int deg, const float* vp, const float* vp_inv,
+                        int W, int H, int tile, int64_t* offs, uint32_t* ids) {
+    double c[3];
+    cam_from_inv(vp_inv, c);
+    float cam[3] = { (float)c[0], (float)c[1], (float)c[2] };
+    return ext_bin_all_xf32(splats, sh, n, deg, vp, cam, (uint32_t)W, (uint32_t)H, (uint32_t)tile, offs, ids);
+}
+
+/* ora_ext_backward_f32 over CSR lists (from ora_ext_bin_all): every tile of the view. */
+int ora_ext_backward_csr_f32(const float* splats, const float* sh, int n, int deg, const float* vp,
+                             const float* vp_inv, const float* tgt, int W, int H, int tile, const uint32_t* ids,
+                             const int64_t* offs, double* grads, double* gsh, float* c_out, double* loss) {
+    double c[3];
+    cam_from_inv(vp_inv, c);
+    float cam[3] = { (float)c[0], (float)c[1], (float)c[2] };
+    const int ntx = (W + tile - 1) / tile, nty = (H + tile - 1) / tile, T = ntx * nty;
+    int32_t* tiles = (int32_t*)malloc(sizeof(int32_t) * (size_t)T);
+    int32_t* counts = (int32_t*)malloc(sizeof(int32_t) * (size_t)T);
+    for (int t = 0; t < T; ++t) { tiles[t] = t; counts[t] = (int32_t)(offs[t + 1] - offs[t]); }
+    ext_backward_xf32(splats, sh, n, deg, vp, cam, tgt, (uint32_t)W, (uint32_t)H, (uint32_t)tile, tiles, T, ids,
+                      counts, 0, offs, grads, gsh, c_out, loss);
+    free(tiles); free(counts);
     return 0;
 }
 
@@ -84,7 +111,7 @@ int ora_ext_backward_f64(const double* splats, const double* sh, int n, int deg,
                          const uint32_t* lists, const int32_t* counts, int stride, double* grads, double* gsh,
                          double* c_out, double* loss) {
     ext_backward_xf64(splats, sh, n, deg, vp, cam, tgt, (uint32_t)W, (uint32_t)H, (uint32_t)tile, tiles, ntiles, lists,
-                      counts, stride, grads, gsh, c_out, loss);
+                      counts, stride, NULL, grads, gsh, c_out, loss);
     return 0;
 }
 

@@ -150,3 +150,22 @@ def test_images_do_not_depend_on_tile_size(oracle):
         assert np.allclose(gs, gs16, rtol=1e-12, atol=1e-12 * np.abs(gs16).max())
     # a box that ignored the pixel test would see more splats with bigger tiles
     assert out[32][3][2] > out[8][3][2]
+
+
+def test_whole_view_binning_matches_per_tile_binning(oracle):
+    """The O(keys) whole-view binning (used to time the CPU path) gives the same
+    lists as the per-tile brute force, and the CSR backward the same image."""
+    W, H = 120, 90
+    s, sh = scenes.make_ext_splats(2000, 12, sh_degree=2)
+    vp, vi = camera.view_proj((0.5, 0.1, 2.9), aspect=W / H)
+    for tile in (8, 16):
+        tiles, lists, counts = oracle.ext_bin(s, sh, 2, vp, vi, W, H, tile=tile)
+        offs, ids = oracle.ext_bin_all(s, sh, 2, vp, vi, W, H, tile=tile)
+        assert np.array_equal(np.diff(offs), counts.astype(np.int64))
+        for t in range(len(tiles)):
+            assert np.array_equal(ids[offs[t]:offs[t + 1]], lists[t, :counts[t]])
+    tgt = np.full((H, W, 4), 0.5, np.float32)
+    a = oracle.ext_backward_f32(s, sh, 2, vp, vi, tgt, W, H, tiles, lists, counts, tile=16)
+    offs8, ids8 = oracle.ext_bin_all(s, sh, 2, vp, vi, W, H, tile=8)
+    b = oracle.ext_backward_csr_f32(s, sh, 2, vp, vi, tgt, W, H, offs8, ids8, tile=8)
+    assert np.array_equal(a[2], b[2])
