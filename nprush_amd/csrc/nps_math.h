@@ -124,6 +124,37 @@ NPS_HD float sin_det(float xf) {
     return (float)out;
 }
 
+// cos with the reduction and polynomials of sin_det, quadrant shifted by one.
+NPS_HD float cos_det(float xf) {
+    double x = (double)xf;
+    if (!(x - x == 0.0)) return NAN;
+    double k = rint(x * 0x1.45f306dc9c883p-1);
+    double r = x - k * 0x1.921fb5p+0;
+    r = r - k * 0x1.110b46p-26;
+    r = r - k * 0x1.1a62633145c07p-54;
+    int q = (int)((((int64_t)k) + 1) & 3);
+    double r2 = r * r;
+    double s = -0x1.ae7f3e733b81fp-41;
+    s = s * r2 + 0x1.6124613a86d09p-33;
+    s = s * r2 - 0x1.ae64567f544e4p-26;
+    s = s * r2 + 0x1.71de3a556c734p-19;
+    s = s * r2 - 0x1.a01a01a01a01ap-13;
+    s = s * r2 + 0x1.1111111111111p-7;
+    s = s * r2 - 0x1.5555555555555p-3;
+    double sinr = r + (r * r2) * s;
+    double c = 0x1.ae7f3e733b81fp-45;
+    c = c * r2 - 0x1.93974a8c07c9dp-37;
+    c = c * r2 + 0x1.1eed8eff8d898p-29;
+    c = c * r2 - 0x1.27e4fb7789f5cp-22;
+    c = c * r2 + 0x1.a01a01a01a01ap-16;
+    c = c * r2 - 0x1.6c16c16c16c17p-10;
+    c = c * r2 + 0x1.5555555555555p-5;
+    c = c * r2 - 0x1.0p-1;
+    double cosr = 1.0 + r2 * c;
+    double out = (q == 0) ? sinr : (q == 1) ? cosr : (q == 2) ? -sinr : -cosr;
+    return (float)out;
+}
+
 NPS_HD float fract_det(float x) { return x - floorf(x); }
 
 // WGSL i32(f32): truncation toward zero, saturating; NaN -> 0.

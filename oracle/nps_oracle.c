@@ -452,6 +452,7 @@ float ora_powf(float x, float y) { return rm_powf(x, y); }
 float ora_expf(float x) { return rm_expf(x); }
 float ora_logf(float x) { return rm_logf(x); }
 float ora_sinf(float x) { return rm_sinf(x); }
+float ora_cosf(float x) { return rm_cosf(x); }
 int32_t ora_f2i(float x) { return rm_f2i_sat(x); }
 
 int ora_num_threads(void) {

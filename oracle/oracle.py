@@ -48,7 +48,7 @@ def load():
         for k, args in sig.items():
             getattr(lib, k).argtypes = args
             getattr(lib, k).restype = i32
-        for k in ["ora_exp2f", "ora_log2f", "ora_expf", "ora_logf", "ora_sinf"]:
+        for k in ["ora_exp2f", "ora_log2f", "ora_expf", "ora_logf", "ora_sinf", "ora_cosf"]:
             getattr(lib, k).argtypes = [C.c_float]
             getattr(lib, k).restype = C.c_float
         lib.ora_powf.argtypes = [C.c_float, C.c_float]
@@ -151,7 +151,7 @@ def render_f32(splats, vp, W, H, color=None, depth=None, clear=True, quantize=Tr
 
 def math_f32(op, x, y=None):
     lib = load()
-    fn = {0: lib.ora_exp2f, 1: lib.ora_log2f, 3: lib.ora_expf, 4: lib.ora_logf, 5: lib.ora_sinf}
+    fn = {0: lib.ora_exp2f, 1: lib.ora_log2f, 3: lib.ora_expf, 4: lib.ora_logf, 5: lib.ora_sinf, 7: lib.ora_cosf}
     x = f32(x)
     if op == 2:
         return np.array([lib.ora_powf(float(a), float(b)) for a, b in zip(x, f32(y))], dtype=np.float32)
@@ -304,3 +304,101 @@ def ext_backward_csr_f32(splats, sh, deg, vp, vp_inv, tgt, W, H, offs, ids, want
                                  W, H, tile, _p(idsc, _U), _p(np.ascontiguousarray(offs, np.int64), _L), _p(g, _D),
                                  _p(gs, _D), _p(c, _F), _p(loss, _D))
     return g, gs, c, loss
+
+
+# ---------------------------------------------------------------- Bezier path
+BZ_CAP = 1024
+
+
+def bz_uniforms(vp, vp_inv, mode=1.0, max_width=0.0, prune_alpha=0.0, prune_width=0.0, bg_penalty=0.0):
+    """BezierUniforms (bezier_backward.wgsl:20-32) as the oracle's flat struct."""
+    u = np.zeros(37, np.float32)
+    u[:16] = f32(vp).reshape(16)
+    u[16:21] = [mode, max_width, prune_alpha, prune_width, bg_penalty]
+    u[21:37] = f32(vp_inv).reshape(16)
+    return u
+
+
+def _tex4(a, H, W):
+    a = f32(a)
+    if a.ndim == 2:
+        out = np.zeros((H, W, 4), np.float32)
+        out[..., 0] = a
+        return out
+    return np.ascontiguousarray(a.reshape(H, W, 4))
+
+
+def bz_bin(bz, uni, W, H, tiles=None):
+    """Tile lists (first 1024 ascending ids).  Returns (lists [T,1024], counts, full counts)."""
+    lib = load()
+    lib.ora_bz_bin.argtypes = [_F, i32, _F, i32, i32, _I, i32, _U, _I, _I]
+    lib.ora_bz_bin.restype = i32
+    b = f32(bz)
+    tiles = np.arange(tiles_of(W, H), dtype=np.int32) if tiles is None else np.ascontiguousarray(tiles, np.int32)
+    lists = np.zeros((len(tiles), BZ_CAP), np.uint32)
+    cnt = np.zeros(len(tiles), np.int32)
+    full = np.zeros(len(tiles), np.int32)
+    rc = lib.ora_bz_bin(_p(b, _F), b.shape[0], _p(f32(uni), _F), W, H, _p(tiles, _I), len(tiles), _p(lists, _U),
+                        _p(cnt, _I), _p(full, _I))
+    assert rc == 0
+    return lists, cnt, full
+
+
+def bz_backward_f32(bz, uni, tgt, tgt_depth, bg, bg_depth, normal, W, H, tiles=None, lists=None, counts=None,
+                    grads=None, loss_acc=None, pixel_loss=None):
+    """bezier_backward over the given tiles (all by default).  Accumulates into copies
+    of grads [n,18] i32, loss_acc [n] i64 (2^-32 fixed point) and pixel_loss [H*W] i32.
+    Returns a dict with grads, loss_acc, pixel_loss, C [H,W,3], D [H,W], T [H,W], loss[2]."""
+    lib = load()
+    lib.ora_bz_backward_f32.argtypes = [_F, i32, _F, _F, _F, _F, _F, _F, i32, i32, _I, i32, _U, _I, _I, _L, _I, _F,
+                                        _F, _F, _D]
+    lib.ora_bz_backward_f32.restype = i32
+    b = f32(bz)
+    n = b.shape[0]
+    tiles = np.arange(tiles_of(W, H), dtype=np.int32) if tiles is None else np.ascontiguousarray(tiles, np.int32)
+    if lists is None:
+        lists, counts, _ = bz_bin(b, uni, W, H, tiles)
+    g = np.zeros((n, 18), np.int32) if grads is None else np.ascontiguousarray(grads, np.int32).copy()
+    la = np.zeros(n, np.int64) if loss_acc is None else np.ascontiguousarray(loss_acc, np.int64).copy()
+    pl = np.zeros(H * W, np.int32) if pixel_loss is None else np.ascontiguousarray(pixel_loss, np.int32).copy()
+    c = np.zeros((H, W, 3), np.float32)
+    d = np.zeros((H, W), np.float32)
+    t = np.zeros((H, W), np.float32)
+    loss = np.zeros(2, np.float64)
+    texs = [_tex4(x, H, W) for x in (tgt, tgt_depth, bg, bg_depth, normal)]
+    rc = lib.ora_bz_backward_f32(_p(b, _F), n, _p(f32(uni), _F), *[_p(x, _F) for x in texs], W, H, _p(tiles, _I),
+                                 len(tiles), _p(np.ascontiguousarray(lists, np.uint32), _U),
+                                 _p(np.ascontiguousarray(counts, np.int32), _I), _p(g, _I), _p(la, _L), _p(pl, _I),
+                                 _p(c, _F), _p(d, _F), _p(t, _F), _p(loss, _D))
+    assert rc == 0
+    return dict(grads=g, loss_acc=la, pixel_loss=pl, C=c, D=d, T=t, loss=loss)
+
+
+def bz_step_f32(bz, grads, m, v, grad_accum, uni, t, pixel_count, no_kill=0.0):
+    """bezier_step on copies; returns (bz, grads (drained), m, v, grad_accum)."""
+    lib = load()
+    lib.ora_bz_step_f32.argtypes = [_F, i32, _I, _F, _F, _F, _F, C.c_float, C.c_float, C.c_float]
+    lib.ora_bz_step_f32.restype = i32
+    b = f32(bz).copy()
+    g = np.ascontiguousarray(grads, np.int32).copy()
+    mm, vv, ga = f32(m).copy(), f32(v).copy(), f32(grad_accum).copy()
+    rc = lib.ora_bz_step_f32(_p(b, _F), b.shape[0], _p(g, _I), _p(mm, _F), _p(vv, _F), _p(ga, _F), _p(f32(uni), _F),
+                             float(t), float(pixel_count), float(no_kill))
+    assert rc == 0
+    return b, g, mm, vv, ga
+
+
+def bz_adc_f32(bz, m, v, grad_accum, loss_acc, pixel_loss, W, H, uni, adam_t, no_kill=0.0):
+    """bezier_adc on copies; returns (bz, m, v, grad_accum, loss_acc, pixel_loss, stats[6])."""
+    lib = load()
+    lib.ora_bz_adc_f32.argtypes = [_F, i32, _F, _F, _F, _L, _I, i32, i32, _F, C.c_float, C.c_float, _L]
+    lib.ora_bz_adc_f32.restype = i32
+    b = f32(bz).copy()
+    mm, vv, ga = f32(m).copy(), f32(v).copy(), f32(grad_accum).copy()
+    la = np.ascontiguousarray(loss_acc, np.int64).copy()
+    pl = np.ascontiguousarray(pixel_loss, np.int32).copy()
+    st = np.zeros(6, np.int64)
+    rc = lib.ora_bz_adc_f32(_p(b, _F), b.shape[0], _p(mm, _F), _p(vv, _F), _p(ga, _F), _p(la, _L), _p(pl, _I), W, H,
+                            _p(f32(uni), _F), float(adam_t), float(no_kill), _p(st, _L))
+    assert rc == 0
+    return b, mm, vv, ga, la, pl, st

@@ -23,6 +23,7 @@
  *   sinf:     double-precision Cody-Waite reduction by pi/2 (three-part pi/2,
  *             exact for |x| < 2^27), Taylor polynomials, rounded to float.
  *             Used only by the clone-jitter hash (splat_adc.wgsl:70-72).
+ *   cosf:     the same reduction, quadrant + 1 (Bezier spawn orientation).
  *
  * The product carries its own implementation of the same methods; a GPU test
  * sweeps both and requires identical bits.
@@ -95,6 +96,46 @@ static inline float rm_sinf(float xf) {
     r = r - k * 0x1.1a62633145c07p-54;
     int64_t ki = (int64_t)k;
     int q = (int)(ki & 3);
+    double r2 = r * r;
+    double s = -0x1.ae7f3e733b81fp-41;
+    s = s * r2 + 0x1.6124613a86d09p-33;
+    s = s * r2 - 0x1.ae64567f544e4p-26;
+    s = s * r2 + 0x1.71de3a556c734p-19;
+    s = s * r2 - 0x1.a01a01a01a01ap-13;
+    s = s * r2 + 0x1.1111111111111p-7;
+    s = s * r2 - 0x1.5555555555555p-3;
+    double sinr = r + (r * r2) * s;
+    double c = 0x1.ae7f3e733b81fp-45;
+    c = c * r2 - 0x1.93974a8c07c9dp-37;
+    c = c * r2 + 0x1.1eed8eff8d898p-29;
+    c = c * r2 - 0x1.27e4fb7789f5cp-22;
+    c = c * r2 + 0x1.a01a01a01a01ap-16;
+    c = c * r2 - 0x1.6c16c16c16c17p-10;
+    c = c * r2 + 0x1.5555555555555p-5;
+    c = c * r2 - 0x1.0p-1;
+    double cosr = 1.0 + r2 * c;
+    double out;
+    switch (q) {
+        case 0: out = sinr; break;
+        case 1: out = cosr; break;
+        case 2: out = -sinr; break;
+        default: out = -cosr; break;
+    }
+    return (float)out;
+}
+
+/* cos with the same reduction and polynomials as rm_sinf, quadrant shifted by
+ * one (cos x = sin(x + pi/2) in exact arithmetic).  Used only by the spawn
+ * orientation of the Bezier densification (bezier_adc.wgsl:226-228). */
+static inline float rm_cosf(float xf) {
+    double x = (double)xf;
+    if (!(x - x == 0.0)) return NAN;
+    double k = rint(x * 0x1.45f306dc9c883p-1);
+    double r = x - k * 0x1.921fb5p+0;
+    r = r - k * 0x1.110b46p-26;
+    r = r - k * 0x1.1a62633145c07p-54;
+    int64_t ki = (int64_t)k;
+    int q = (int)((ki + 1) & 3);
     double r2 = r * r;
     double s = -0x1.ae7f3e733b81fp-41;
     s = s * r2 + 0x1.6124613a86d09p-33;

@@ -34,7 +34,7 @@ def _run_backward(splats, vp, vi, tgt, dep, W, H, cap=1024):
     return mgr
 
 
-@pytest.mark.parametrize("op", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("op", [0, 1, 2, 3, 4, 5, 7])
 def test_math_bit_identity(oracle, op):
     rng = np.random.default_rng(op)
     if op in (0, 3):

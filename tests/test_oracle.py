@@ -45,6 +45,10 @@ def test_pow_exp_log_sin_accuracy(oracle):
     s = rng.uniform(-1e6, 1e6, 3000).astype(np.float32)
     sref = np.sin(s.astype(np.float64))
     assert np.abs(oracle.math_f32(5, s) - sref).max() < 1e-7
+    cref = np.cos(s.astype(np.float64))
+    assert np.abs(oracle.math_f32(7, s) - cref).max() < 1e-7
+    q = np.float32(np.pi / 2) * np.arange(-8, 9, dtype=np.float32)
+    assert np.abs(oracle.math_f32(7, q) - np.cos(q.astype(np.float64))).max() < 1e-7
 
 
 def test_math_special_values(oracle):
