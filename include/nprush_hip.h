@@ -269,6 +269,78 @@ int npb_blur(npb_handle b, const float* src_rgba, float* dst_rgba, int32_t w, in
 int npb_blur_dev(npb_handle b, const float* src_dev, float* dst_dev, int32_t w, int32_t hgt, int32_t radius,
                  float sigma, int32_t is_srgb);
 
+/* ---------------------------------------------------------------- Bezier
+ * Stroke optimizer: replaces GpuBezierOptimizerManager
+ * (src/lib/gpu/GpuBezierOptimizerManager.ts:17-406) and its three compute
+ * pipelines (bezier_backward.wgsl, bezier_step.wgsl, bezier_adc.wgsl).
+ * Curves: 20 floats each (p0.xyz width | p1.xyz softness | p2.xyz pad |
+ * p3.xyz pad | rgba, bezier_backward.wgsl:4-10).  Gradients: 18 i32 per curve
+ * (positions, rgb, a, width, softness; fixed point 1e4 / 1e5).  Textures are
+ * H*W float arrays: rgba (4 floats per pixel) for target, background and
+ * normal, one float per pixel for the two depths.
+ * Defined where the reference is not: loss_accum is an int64 per curve in
+ * 2^-32 fixed point (the reference's non-atomic float += races), adam.t is a
+ * host counter, and pixel_loss and the densification's pixel grid have the
+ * size of the backward target (the reference fixes 512^2 and 256^2). */
+typedef struct nbz_ctx* nbz_handle;
+/* new GpuBezierOptimizerManager({device, numBeziers}) (:46-55); curves start at
+ * zero until nbz_write_beziers (the reference draws random squiggles, :59-95). */
+int nbz_create(int32_t device, int32_t num_beziers, nbz_handle* out);
+void nbz_destroy(nbz_handle h);
+const char* nbz_last_error(nbz_handle h);
+int nbz_write_beziers(nbz_handle h, const float* curves20n, int32_t n);   /* bezierBuffer */
+int nbz_read_beziers(nbz_handle h, float* curves20n, int32_t n);
+int nbz_write_grads(nbz_handle h, const int32_t* grads18n, int32_t n);   /* gradBuffer */
+int nbz_read_grads(nbz_handle h, int32_t* grads18n, int32_t n);
+/* adamBuffer: m, v (18 per curve), t */
+int nbz_write_adam(nbz_handle h, const float* m, const float* v, float t, int32_t n);
+int nbz_read_adam(nbz_handle h, float* m, float* v, float* t, int32_t n);
+/* adcBuffer: grad_accum (float) and loss_accum (int64, 2^-32) per curve */
+int nbz_write_adc(nbz_handle h, const float* grad_accum, const int64_t* loss_acc, int32_t n);
+int nbz_read_adc(nbz_handle h, float* grad_accum, int64_t* loss_acc, int32_t n);
+/* pixel loss map of the current target size (W*H i32, fixed point 1e4) */
+int nbz_write_pixel_loss(nbz_handle h, const int32_t* pl);
+int nbz_read_pixel_loss(nbz_handle h, int32_t* pl);
+int nbz_write_vp_matrix(nbz_handle h, const float vp[16]);         /* writeVPMatrix (:269-277) */
+int nbz_write_vp_inv_matrix(nbz_handle h, const float vp_inv[16]); /* writeVPInvMatrix (:279-288) */
+int nbz_write_mode(nbz_handle h, float mode);                      /* writeMode (:290-296) */
+int nbz_write_bg_penalty(nbz_handle h, float weight);              /* writeBgPenalty (:298-307) */
+int nbz_write_max_width(nbz_handle h, float max_width);            /* writeMaxWidth (:309-316) */
+int nbz_write_kill_thresholds(nbz_handle h, float alpha, float width); /* writeKillThresholds (:318-326) */
+int nbz_set_adc_period(nbz_handle h, int32_t period);              /* setAdcPeriod (:328-330) */
+int nbz_write_no_kill(nbz_handle h, int32_t no_kill);              /* writeNoKill (:332-339) */
+/* setBackwardTarget (:341-368): host images are copied; the _dev variant keeps
+ * the device pointers, which must stay valid while the target is in use. */
+int nbz_set_backward_target(nbz_handle h, const float* target_rgba, const float* target_depth, const float* bg_rgba,
+                            const float* bg_depth, const float* normal_rgba, int32_t w, int32_t hgt);
+int nbz_set_backward_target_dev(nbz_handle h, const float* target_rgba, const float* target_depth,
+                                const float* bg_rgba, const float* bg_depth, const float* normal_rgba, int32_t w,
+                                int32_t hgt);
+/* dispatch (:370-406): backward, step, and every adc_period-th call the ADC */
+int nbz_dispatch(nbz_handle h);
+/* the three passes on their own */
+int nbz_backward(nbz_handle h);
+int nbz_step(nbz_handle h);
+int nbz_adc(nbz_handle h);
+/* image only (no gradients): the backward's forward half */
+int nbz_forward(nbz_handle h);
+int nbz_sync(nbz_handle h);
+/* Outputs of the last nbz_backward / nbz_forward: colour (H*W*3), encoded
+ * depth (H*W) and final transmittance (H*W); loss = sum |C - target|^2 and
+ * sum |C - target| over the colour channels. */
+int nbz_read_image(nbz_handle h, float* color, float* depth, float* trans);
+int nbz_read_loss(nbz_handle h, double loss[2]);
+/* tile lists of the last backward (ascending curve ids, first 1024 per tile) */
+int nbz_read_tiles(nbz_handle h, uint32_t* ranges, uint32_t* ids, int32_t* num_tiles, int64_t* num_keys);
+/* [0] keys, [1] tiles over the cap, [2] pairs, [3] reverse pairs, [4] reverse
+ * wave-entries, [5] evaluated wave-entries, [6..11] last ADC: dead slots found,
+ * killed, split/clone pass 1, spawned, split/clone pass 3, dead slots left */
+int nbz_read_stats(nbz_handle h, int64_t stats[12]);
+/* ms of [0] preprocess+cut, [1] keys, [2] sort+ranges, [3] composite, [4] step,
+ * [5] adc, [6] backward total of the last calls */
+int nbz_enable_timing(nbz_handle h, int32_t on);
+int nbz_read_timing(nbz_handle h, float ms[8]);
+
 /* Device buffers on the handle's GPU for callers without a GPU allocator of
  * their own (Deno, ctypes), e.g. to keep several targets resident in HBM for
  * nps_set_target_dev.  No reference counterpart (WebGPU's createBuffer). */
@@ -278,7 +350,7 @@ int nps_dev_upload(nps_handle h, void* dst, const void* src, int64_t bytes);
 
 /* Deterministic transcendentals used by the kernels, evaluated on the GPU
  * over x[n] (and y[n] for pow): op 0=exp2 1=log2 2=pow 3=exp 4=log 5=sin,
- * 6=float->int32 conversion of the gradient path (result bits in out).
+ * 6=float->int32 conversion of the gradient path (result bits in out), 7=cos.
  * For tests (bit-identity with the CPU oracle). */
 int nps_debug_math(int32_t device, int32_t op, const float* x, const float* y, float* out, int32_t n);
 

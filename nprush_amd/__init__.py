@@ -6,10 +6,12 @@ reference's GpuSplatOptimizerManager / GpuSplatForwardPipelineManager.
 """
 from . import camera, scenes  # noqa: F401
 from .managers import (  # noqa: F401
+    GpuBezierOptimizerManager,
     GpuBlurPipelineManager,
     GpuMultiViewStep,
     GpuSplatForwardPipelineManager,
     GpuSplatOptimizerManager,
     debug_math,
+    reference_bezier_init,
     reference_init,
 )

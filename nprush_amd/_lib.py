@@ -100,6 +100,41 @@ SIGNATURES = {
     "nps_dev_free": (i32, [_P, _P]),
     "nps_dev_upload": (i32, [_P, _P, _P, C.c_int64]),
     "nps_debug_math": (i32, [i32, i32, _F, _F, _F, i32]),
+    "nbz_create": (i32, [i32, i32, C.POINTER(_P)]),
+    "nbz_destroy": (None, [_P]),
+    "nbz_last_error": (C.c_char_p, [_P]),
+    "nbz_write_beziers": (i32, [_P, _F, i32]),
+    "nbz_read_beziers": (i32, [_P, _F, i32]),
+    "nbz_write_grads": (i32, [_P, _I, i32]),
+    "nbz_read_grads": (i32, [_P, _I, i32]),
+    "nbz_write_adam": (i32, [_P, _F, _F, C.c_float, i32]),
+    "nbz_read_adam": (i32, [_P, _F, _F, _F, i32]),
+    "nbz_write_adc": (i32, [_P, _F, _L, i32]),
+    "nbz_read_adc": (i32, [_P, _F, _L, i32]),
+    "nbz_write_pixel_loss": (i32, [_P, _I]),
+    "nbz_read_pixel_loss": (i32, [_P, _I]),
+    "nbz_write_vp_matrix": (i32, [_P, _F]),
+    "nbz_write_vp_inv_matrix": (i32, [_P, _F]),
+    "nbz_write_mode": (i32, [_P, C.c_float]),
+    "nbz_write_bg_penalty": (i32, [_P, C.c_float]),
+    "nbz_write_max_width": (i32, [_P, C.c_float]),
+    "nbz_write_kill_thresholds": (i32, [_P, C.c_float, C.c_float]),
+    "nbz_set_adc_period": (i32, [_P, i32]),
+    "nbz_write_no_kill": (i32, [_P, i32]),
+    "nbz_set_backward_target": (i32, [_P, _F, _F, _F, _F, _F, i32, i32]),
+    "nbz_set_backward_target_dev": (i32, [_P, _P, _P, _P, _P, _P, i32, i32]),
+    "nbz_dispatch": (i32, [_P]),
+    "nbz_backward": (i32, [_P]),
+    "nbz_step": (i32, [_P]),
+    "nbz_adc": (i32, [_P]),
+    "nbz_forward": (i32, [_P]),
+    "nbz_sync": (i32, [_P]),
+    "nbz_read_image": (i32, [_P, _F, _F, _F]),
+    "nbz_read_loss": (i32, [_P, _D]),
+    "nbz_read_tiles": (i32, [_P, _U, _U, _I, _L]),
+    "nbz_read_stats": (i32, [_P, _L]),
+    "nbz_enable_timing": (i32, [_P, i32]),
+    "nbz_read_timing": (i32, [_P, _F]),
 }
 
 _lib = None

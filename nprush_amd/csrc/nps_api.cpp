@@ -20,25 +20,12 @@
 #include <vector>
 
 #include "../../include/nprush_hip.h"
+#include "nps_host.h"
 #include "nps_kernels.h"
 
 using namespace nps;
 
 namespace {
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-};
-
-struct Scratch {
-    DevBuf counts, rects, offsets, keys, keys_alt, ranges, scan_tmp, sort_tmp;
-    DevBuf diff, cut, full, kept;  // per-tile id-block cut of capped (REF) lists
-    uint64_t m = 0;
-    bool in_alt = false;
-    bool have_full = false;
-    uint32_t splat_block = 0;
-};
 
 // Buffers of one EXT view pipeline: the handle's own (nps_backward,
 // nps_forward) and one set per lane of a multi-view step.
@@ -141,44 +128,7 @@ int set_err(nps_ctx* h, int code, const std::string& msg) {
         if (e_ != hipSuccess) return set_err((h), NPS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-// Grows b to at least `bytes` (25 % headroom); a live buffer is freed only
-// after `st` has drained.
-int ensure_dev(DevBuf& b, size_t bytes, hipStream_t st, std::string& err) {
-    if (bytes <= b.bytes && b.p) return NPS_OK;
-    if (b.p) {
-        hipError_t e = hipStreamSynchronize(st);
-        if (e != hipSuccess) {
-            err = std::string("sync: ") + hipGetErrorString(e);
-            return NPS_ERR_HIP;
-        }
-        (void)hipFree(b.p);
-        b.p = nullptr;
-        b.bytes = 0;
-    }
-    size_t alloc = std::max<size_t>(bytes + bytes / 4, 256);
-    hipError_t e = hipMalloc(&b.p, alloc);
-    if (e != hipSuccess) {
-        b.p = nullptr;
-        err = "hipMalloc(" + std::to_string(alloc) + "): " + hipGetErrorString(e);
-        return NPS_ERR_HIP;
-    }
-    b.bytes = alloc;
-    return NPS_OK;
-}
-
 int ensure(nps_ctx* h, DevBuf& b, size_t bytes) { return ensure_dev(b, bytes, h->stream, h->err); }
-
-void free_buf(DevBuf& b) {
-    if (b.p) (void)hipFree(b.p);
-    b.p = nullptr;
-    b.bytes = 0;
-}
-
-void free_scratch(Scratch& s) {
-    free_buf(s.counts); free_buf(s.rects); free_buf(s.offsets); free_buf(s.keys);
-    free_buf(s.keys_alt); free_buf(s.ranges); free_buf(s.scan_tmp); free_buf(s.sort_tmp);
-    free_buf(s.diff); free_buf(s.cut); free_buf(s.full); free_buf(s.kept);
-}
 
 void free_ext_view(ExtView& x) {
     DevBuf* all[] = {&x.rec, &x.counts, &x.rects, &x.dkey, &x.dkey_alt, &x.rank_counts, &x.offsets, &x.scan_tmp,
@@ -187,65 +137,10 @@ void free_ext_view(ExtView& x) {
     for (DevBuf* b : all) free_buf(*b);
 }
 
-// Ids per block of the per-tile cut: about 32 blocks, at least 4096 splats each.
-uint32_t splat_block_of(uint32_t n) {
-    uint32_t b = (n + 31u) / 32u;
-    return b < 4096u ? 4096u : b;
-}
-
-ViewParams make_view(const float* vp, const float* vp_inv, uint32_t W, uint32_t H, uint32_t tile = kTile) {
-    ViewParams v;
-    memset(&v, 0, sizeof(v));
-    memcpy(v.vp, vp, sizeof(v.vp));
-    if (vp_inv) memcpy(v.vp_inv, vp_inv, sizeof(v.vp_inv));
-    v.W = W;
-    v.H = H;
-    v.aspect = (float)W / (float)H;
-    v.tile = tile;
-    v.tiles_x = (W + tile - 1) / tile;
-    v.tiles_y = (H + tile - 1) / tile;
-    v.num_tiles = v.tiles_x * v.tiles_y;
-    return v;
-}
-
-int bits_for(uint32_t x) {
-    int b = 1;
-    while (b < 32 && (x >> b) != 0u) ++b;
-    return b;
-}
-
-// keys for the counts already in sc.counts/sc.rects: scan, emit, sort, ranges.
 int build_tile_lists(nps_ctx* h, Scratch& sc, const ViewParams& v, uint32_t n, int ev_sort, int ev_ranges) {
-    int rc;
-    if ((rc = ensure(h, sc.offsets, sizeof(uint64_t) * ((size_t)n + 1))) != NPS_OK) return rc;
-    if ((rc = ensure(h, sc.scan_tmp, scan_temp_bytes(n) + 64)) != NPS_OK) return rc;
-    // capped lists: key counts under the per-tile cut; uncapped: the rectangle sizes
-    const uint32_t* key_counts = (const uint32_t*)(sc.have_full ? sc.kept.p : sc.counts.p);
-    exclusive_scan_u32_to_u64(h->stream, key_counts, (uint64_t*)sc.offsets.p, n, sc.scan_tmp.p);
-    uint64_t m = 0;
-    NPS_HIP(h, hipMemcpyAsync(&m, (uint64_t*)sc.offsets.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost, h->stream));
-    NPS_HIP(h, hipStreamSynchronize(h->stream));
-    if (m >= 0xFFFFFFFFull) return set_err(h, NPS_ERR_CAPACITY, "tile key count exceeds 2^32");
-    sc.m = m;
-    if ((rc = ensure(h, sc.keys, sizeof(uint64_t) * std::max<uint64_t>(m, 1))) != NPS_OK) return rc;
-    if ((rc = ensure(h, sc.keys_alt, sizeof(uint64_t) * std::max<uint64_t>(m, 1))) != NPS_OK) return rc;
-    if ((rc = ensure(h, sc.sort_tmp, radix_temp_bytes(m))) != NPS_OK) return rc;
-    if ((rc = ensure(h, sc.ranges, sizeof(uint32_t) * 2 * (size_t)v.num_tiles)) != NPS_OK) return rc;
-    // splats that keep no key skip emission entirely
-    launch_emit_keys(h->stream, n, key_counts, (const uint64_t*)sc.offsets.p,
-                     (const uint4*)sc.rects.p, v.tiles_x, v.num_tiles,
-                     sc.have_full ? (const uint32_t*)sc.cut.p : nullptr,
-                     sc.splat_block ? sc.splat_block : 1u, (uint64_t*)sc.keys.p);
-    if (h->timing && ev_sort >= 0) NPS_HIP(h, hipEventRecord(h->ev[ev_sort], h->stream));
-    int tile_bits = bits_for(v.num_tiles > 0 ? v.num_tiles - 1 : 0);
-    sc.in_alt = radix_sort_u64(h->stream, (uint64_t*)sc.keys.p, (uint64_t*)sc.keys_alt.p, nullptr, nullptr, m, 32,
-                               32 + tile_bits, sc.sort_tmp.p);
-    if (h->timing && ev_ranges >= 0) NPS_HIP(h, hipEventRecord(h->ev[ev_ranges], h->stream));
-    launch_fill_u32(h->stream, (uint32_t*)sc.ranges.p, 0u, 2 * (uint64_t)v.num_tiles);
-    const uint64_t* sorted = (const uint64_t*)(sc.in_alt ? sc.keys_alt.p : sc.keys.p);
-    launch_tile_ranges(h->stream, sorted, m, (uint32_t*)sc.ranges.p);
-    NPS_HIP(h, hipGetLastError());
-    return NPS_OK;
+    hipEvent_t es = (h->timing && ev_sort >= 0) ? h->ev[ev_sort] : nullptr;
+    hipEvent_t er = (h->timing && ev_ranges >= 0) ? h->ev[ev_ranges] : nullptr;
+    return build_tile_lists_on(h->stream, h->err, sc, v, n, es, er);
 }
 
 // Where a view's per-splat tile counts and 2D gradients go, and whether its

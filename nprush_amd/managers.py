@@ -447,6 +447,237 @@ class GpuBlurPipelineManager:
         return out
 
 
+def reference_bezier_init(num_beziers: int, seed: int = 0) -> np.ndarray:
+    """The reference's initial curves (GpuBezierOptimizerManager.ts:56-95: short
+    randomly oriented squiggles near the origin, width 0.02, softness 0.005,
+    alpha 0.5) drawn from a seeded generator instead of Math.random()."""
+    rng = np.random.default_rng(seed)
+    n = int(num_beziers)
+    d = np.zeros((n, 20), np.float32)
+    cx = rng.random(n) * 2 - 1
+    cy = rng.random(n) * 2 - 1
+    ln = 0.2 + rng.random(n) * 0.2
+    ang = rng.random(n) * np.pi * 2
+    dx, dy = np.cos(ang) * ln, np.sin(ang) * ln
+
+    def jit():
+        return (rng.random(n) - 0.5) * 0.08
+    d[:, 0], d[:, 1], d[:, 2], d[:, 3] = cx - dx * 0.5, cy - dy * 0.5, (rng.random(n) * 2 - 1) * 0.3, 0.02
+    d[:, 4], d[:, 5], d[:, 6], d[:, 7] = cx - dx * 0.15 + jit(), cy - dy * 0.15 + jit(), (rng.random(n) * 2 - 1) * 0.3, 0.005
+    d[:, 8], d[:, 9], d[:, 10] = cx + dx * 0.15 + jit(), cy + dy * 0.15 + jit(), (rng.random(n) * 2 - 1) * 0.3
+    d[:, 12], d[:, 13], d[:, 14] = cx + dx * 0.5, cy + dy * 0.5, (rng.random(n) * 2 - 1) * 0.3
+    d[:, 16], d[:, 17], d[:, 18] = rng.random(n), rng.random(n), rng.random(n)
+    d[:, 19] = 0.5
+    return d
+
+
+class GpuBezierOptimizerManager:
+    """new GpuBezierOptimizerManager({device, numBeziers}) (src/lib/gpu/GpuBezierOptimizerManager.ts:17-406).
+
+    Texture views are host arrays: H x W x 4 float32 for the target, background
+    and normal images, H x W float32 (or the .r of H x W x 4) for the two depths.
+    The constructor writes the reference's initial curve distribution from a
+    seeded generator (``reference_bezier_init``)."""
+
+    def __init__(self, device=None, numBeziers: int = 16, *, hip_device: int = 0, seed: int = 0):
+        lib = _lib.load()
+        h = C.c_void_p()
+        rc = lib.nbz_create(int(hip_device), int(numBeziers), C.byref(h))
+        if rc != 0:
+            raise _lib.NpsError(f"nbz_create failed ({rc})")
+        self._lib = lib
+        self._h = h
+        self.numBeziers = int(numBeziers)
+        self.numParams = self.numBeziers * 18
+        self.dims = (0, 0)
+        self.write_beziers(reference_bezier_init(self.numBeziers, seed))
+
+    def _check(self, rc, what):
+        if rc != 0:
+            m = self._lib.nbz_last_error(self._h)
+            raise _lib.NpsError(f"{what} failed ({rc}): {m.decode() if m else ''}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.nbz_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- buffers
+    def write_beziers(self, curves: np.ndarray):
+        c = np.ascontiguousarray(curves, dtype=np.float32).reshape(self.numBeziers, 20)
+        self._check(self._lib.nbz_write_beziers(self._h, fptr(c), self.numBeziers), "nbz_write_beziers")
+
+    def read_beziers(self) -> np.ndarray:
+        c = np.empty((self.numBeziers, 20), np.float32)
+        self._check(self._lib.nbz_read_beziers(self._h, fptr(c), self.numBeziers), "nbz_read_beziers")
+        return c
+
+    def write_grads(self, grads: np.ndarray):
+        g = np.ascontiguousarray(grads, dtype=np.int32).reshape(self.numBeziers, 18)
+        self._check(self._lib.nbz_write_grads(self._h, iptr(g), self.numBeziers), "nbz_write_grads")
+
+    def read_grads(self) -> np.ndarray:
+        g = np.empty((self.numBeziers, 18), np.int32)
+        self._check(self._lib.nbz_read_grads(self._h, iptr(g), self.numBeziers), "nbz_read_grads")
+        return g
+
+    def write_adam(self, m: np.ndarray, v: np.ndarray, t: float):
+        mm = np.ascontiguousarray(m, dtype=np.float32).reshape(self.numBeziers, 18)
+        vv = np.ascontiguousarray(v, dtype=np.float32).reshape(self.numBeziers, 18)
+        self._check(self._lib.nbz_write_adam(self._h, fptr(mm), fptr(vv), float(t), self.numBeziers),
+                    "nbz_write_adam")
+
+    def read_adam(self):
+        m = np.empty((self.numBeziers, 18), np.float32)
+        v = np.empty((self.numBeziers, 18), np.float32)
+        t = C.c_float()
+        self._check(self._lib.nbz_read_adam(self._h, fptr(m), fptr(v), C.byref(t), self.numBeziers),
+                    "nbz_read_adam")
+        return m, v, float(t.value)
+
+    def write_adc(self, grad_accum: np.ndarray, loss_accum: np.ndarray):
+        ga = np.ascontiguousarray(grad_accum, dtype=np.float32).reshape(self.numBeziers)
+        la = np.ascontiguousarray(loss_accum, dtype=np.int64).reshape(self.numBeziers)
+        self._check(self._lib.nbz_write_adc(self._h, fptr(ga), lptr(la), self.numBeziers), "nbz_write_adc")
+
+    def read_adc(self):
+        """(grad_accum float32 [n], loss_accum int64 [n] in 2^-32 fixed point)."""
+        ga = np.empty(self.numBeziers, np.float32)
+        la = np.empty(self.numBeziers, np.int64)
+        self._check(self._lib.nbz_read_adc(self._h, fptr(ga), lptr(la), self.numBeziers), "nbz_read_adc")
+        return ga, la
+
+    def write_pixel_loss(self, pl: np.ndarray):
+        p = np.ascontiguousarray(pl, dtype=np.int32).reshape(-1)
+        self._check(self._lib.nbz_write_pixel_loss(self._h, iptr(p)), "nbz_write_pixel_loss")
+
+    def read_pixel_loss(self) -> np.ndarray:
+        w, h = self.dims
+        p = np.empty(w * h, np.int32)
+        self._check(self._lib.nbz_read_pixel_loss(self._h, iptr(p)), "nbz_read_pixel_loss")
+        return p
+
+    # ---- the reference's methods
+    def writeVPMatrix(self, mat):
+        self._check(self._lib.nbz_write_vp_matrix(self._h, fptr(np.ascontiguousarray(mat, np.float32).reshape(16))),
+                    "writeVPMatrix")
+
+    def writeVPInvMatrix(self, mat):
+        self._check(self._lib.nbz_write_vp_inv_matrix(self._h,
+                                                      fptr(np.ascontiguousarray(mat, np.float32).reshape(16))),
+                    "writeVPInvMatrix")
+
+    def writeMode(self, mode: float = 0):
+        self._check(self._lib.nbz_write_mode(self._h, float(mode)), "writeMode")
+
+    def writeBgPenalty(self, weight: float = 0):
+        self._check(self._lib.nbz_write_bg_penalty(self._h, float(weight)), "writeBgPenalty")
+
+    def writeMaxWidth(self, maxWidth: float = 0):
+        self._check(self._lib.nbz_write_max_width(self._h, float(maxWidth)), "writeMaxWidth")
+
+    def writeKillThresholds(self, alphaThresh: float = 0, widthThresh: float = 0):
+        self._check(self._lib.nbz_write_kill_thresholds(self._h, float(alphaThresh), float(widthThresh)),
+                    "writeKillThresholds")
+
+    def setAdcPeriod(self, period: int):
+        self._check(self._lib.nbz_set_adc_period(self._h, int(period)), "setAdcPeriod")
+
+    def writeNoKill(self, noKill: bool):
+        self._check(self._lib.nbz_write_no_kill(self._h, 1 if noKill else 0), "writeNoKill")
+
+    def setBackwardTarget(self, targetTextureView, targetDepthTextureView, bgColorTextureView, bgDepthTextureView,
+                          normalTextureView, width: int, height: int):
+        W, H = int(width), int(height)
+
+        def rgba(a):
+            return np.ascontiguousarray(np.asarray(a, np.float32).reshape(H, W, 4))
+
+        def red(a):
+            a = np.asarray(a, np.float32)
+            return np.ascontiguousarray(a[..., 0] if a.ndim == 3 else a.reshape(H, W))
+        t, td, b, bd, nr = (rgba(targetTextureView), red(targetDepthTextureView), rgba(bgColorTextureView),
+                            red(bgDepthTextureView), rgba(normalTextureView))
+        self._check(self._lib.nbz_set_backward_target(self._h, fptr(t), fptr(td), fptr(b), fptr(bd), fptr(nr), W, H),
+                    "setBackwardTarget")
+        self.dims = (W, H)
+
+    def set_backward_target_dev(self, tgt_ptr, tdepth_ptr, bg_ptr, bgdepth_ptr, normal_ptr, width: int,
+                                height: int):
+        """setBackwardTarget with device pointers (e.g. torch tensors' data_ptr()) kept by reference."""
+        self._check(self._lib.nbz_set_backward_target_dev(self._h, C.c_void_p(tgt_ptr), C.c_void_p(tdepth_ptr),
+                                                          C.c_void_p(bg_ptr), C.c_void_p(bgdepth_ptr),
+                                                          C.c_void_p(normal_ptr), int(width), int(height)),
+                    "setBackwardTarget")
+        self.dims = (int(width), int(height))
+
+    def dispatch(self, commandEncoder=None):
+        self._check(self._lib.nbz_dispatch(self._h), "dispatch")
+
+    # ---- passes and readback
+    def backward(self):
+        self._check(self._lib.nbz_backward(self._h), "nbz_backward")
+
+    def forward(self):
+        self._check(self._lib.nbz_forward(self._h), "nbz_forward")
+
+    def step(self):
+        self._check(self._lib.nbz_step(self._h), "nbz_step")
+
+    def adc(self):
+        self._check(self._lib.nbz_adc(self._h), "nbz_adc")
+
+    def sync(self):
+        self._check(self._lib.nbz_sync(self._h), "nbz_sync")
+
+    def read_image(self):
+        """(colour [H,W,3], encoded depth [H,W], final transmittance [H,W])."""
+        w, h = self.dims
+        c = np.empty((h, w, 3), np.float32)
+        d = np.empty((h, w), np.float32)
+        t = np.empty((h, w), np.float32)
+        self._check(self._lib.nbz_read_image(self._h, fptr(c), fptr(d), fptr(t)), "nbz_read_image")
+        return c, d, t
+
+    def read_loss(self) -> np.ndarray:
+        out = np.zeros(2, np.float64)
+        self._check(self._lib.nbz_read_loss(self._h, dptr(out)), "nbz_read_loss")
+        return out
+
+    def read_tiles(self):
+        """(ranges [T,2], ids [M]) of the last backward / forward."""
+        nt = C.c_int32()
+        nk = C.c_int64()
+        self._check(self._lib.nbz_read_tiles(self._h, None, None, C.byref(nt), C.byref(nk)), "nbz_read_tiles")
+        ranges = np.empty((nt.value, 2), np.uint32)
+        ids = np.empty(max(nk.value, 1), np.uint32)
+        self._check(self._lib.nbz_read_tiles(self._h, uptr(ranges), uptr(ids), C.byref(nt), C.byref(nk)),
+                    "nbz_read_tiles")
+        return ranges, ids[:nk.value]
+
+    def read_stats(self) -> dict:
+        s = np.zeros(12, np.int64)
+        self._check(self._lib.nbz_read_stats(self._h, lptr(s)), "nbz_read_stats")
+        keys = ["keys", "tiles_over_cap", "pairs", "reverse_pairs", "reverse_wave_entries", "evaluated_wave_entries",
+                "adc_dead", "adc_killed", "adc_split1", "adc_spawned", "adc_split3", "adc_dead_left"]
+        return {k: int(v) for k, v in zip(keys, s)}
+
+    def enable_timing(self, on: bool = True):
+        self._check(self._lib.nbz_enable_timing(self._h, 1 if on else 0), "nbz_enable_timing")
+
+    def read_timing(self) -> dict:
+        ms = np.zeros(8, np.float32)
+        self._check(self._lib.nbz_read_timing(self._h, fptr(ms)), "nbz_read_timing")
+        return {"preprocess": float(ms[0]), "keys": float(ms[1]), "sort": float(ms[2]), "composite": float(ms[3]),
+                "step": float(ms[4]), "adc": float(ms[5]), "backward": float(ms[6])}
+
+
 def debug_math(op: int, x: np.ndarray, y: np.ndarray | None = None, device: int = 0) -> np.ndarray:
     lib = _lib.load()
     x = np.ascontiguousarray(x, dtype=np.float32)

@@ -101,3 +101,30 @@ def make_ext_splats(n: int, seed: int, sh_degree: int = 3, higher_std: float = 0
     if nk > 1:
         sh[:, 3:3 * nk] = rng.normal(0.0, higher_std, size=(n, 3 * nk - 3))
     return s, sh
+
+
+BEZIER_FLOATS = 20
+
+
+def make_bezier_curves(n: int, seed: int, radius_median: float = 0.004, radius_sigma: float = 0.3,
+                       walk_std: float = 0.05, soft_ratio: float = 0.25) -> np.ndarray:
+    """Cubic Bezier curves of the stroke benchmark (layout: p0.xyz width |
+    p1.xyz softness | p2.xyz 0 | p3.xyz 0 | rgba, bezier_backward.wgsl:4-10).
+
+    p0 uniform in [-1, 1]^3, p1..p3 a random walk with N(0, walk_std) steps,
+    width log-normal(ln radius_median, radius_sigma), softness = soft_ratio *
+    width (the reference's initial 0.02 / 0.005), opacity sigmoid(N(1, 0.5))
+    capped at 0.99 (the step's clamp), RGB uniform in [0, 1]."""
+    rng = np.random.default_rng(seed)
+    c = np.zeros((n, BEZIER_FLOATS), dtype=np.float32)
+    p = rng.uniform(-1.0, 1.0, size=(n, 3))
+    for k in range(4):
+        if k:
+            p = p + rng.normal(0.0, walk_std, size=(n, 3))
+        c[:, 4 * k:4 * k + 3] = p
+    width = np.exp(rng.normal(math.log(radius_median), radius_sigma, size=n))
+    c[:, 3] = width
+    c[:, 7] = width * soft_ratio
+    c[:, 16:19] = rng.uniform(0.0, 1.0, size=(n, 3))
+    c[:, 19] = np.minimum(1.0 / (1.0 + np.exp(-rng.normal(1.0, 0.5, size=n))), 0.99)
+    return c
