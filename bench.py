@@ -56,6 +56,16 @@ BWD_FLOP_PER_ACTIVE_PAIR = 240 + 3 * 4
 #     seeds (22) + screen-centre and conic gradients (12) + colour (4) = 67
 EXT_FWD_FLOP_PER_PAIR = 25
 EXT_BWD_FLOP_PER_ACTIVE_PAIR = 67
+# Bezier composite (bezier_backward.wgsl:196-563), div and sqrt counted once:
+#   every (pixel, curve) evaluation of the distance loop (forward, and again in
+#     the reverse recompute): 16 segments x 26 (segment, projection parameter,
+#     clamp, projection, squared distance, sqrt, strict minimum) + 40 (t,
+#     pressure, smoothstep, Bernstein depth) + 10 (compositing) = 466;
+#   every reverse pair with alpha >= 0.001: 300 (alpha and transmittance
+#     seeds, segment recompute, smoothstep derivatives, depth term, Bernstein
+#     weights, 12 backprojections, fixed-point conversion, loss term).
+BZ_FLOP_PER_EVAL = 466
+BZ_FLOP_PER_REV_PAIR = 300
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
@@ -224,14 +234,156 @@ def cpu_baseline(splats, vp, vi, target, depth, W, H, sample_tiles, seed=0):
     }
 
 
+def bezier_config3(steps=20, warmup=3, cpu_tiles=16, n=100_000, W=1024, H=1024, views=4, small=False):
+    """Bezier stroke optimizer (GpuBezierOptimizerManager.dispatch: backward +
+    Adam, densification every 50 steps): 100,000 curves (p0 uniform in
+    [-1, 1]^3, random-walk control points with N(0, 0.05) steps, width
+    log-normal(ln 0.004, 0.3), softness width / 4, opacity sigmoid(N(1, 0.5)),
+    RGB uniform), 4 ring views at 1024 x 1024, colour mode, targets = the
+    forward image and depth of a seed-1 curve set, black background at depth 1.
+    Targets stay in HBM (torch tensors) for the whole run."""
+    import torch
+
+    from nprush_amd import GpuBezierOptimizerManager, camera, scenes
+
+    if small:
+        n, W, H = 10_000, 256, 256
+    cams = camera.ring_cameras(views, 3.0, W, H)
+    curves = scenes.make_bezier_curves(n, 0)
+    tcurves = scenes.make_bezier_curves(n, 1)
+    m = GpuBezierOptimizerManager(numBeziers=n)
+    m.writeMode(1.0)
+    dev = torch.device("cuda:0")
+    zeros4 = torch.zeros((H, W, 4), dtype=torch.float32, device=dev)
+    ones1 = torch.ones((H, W), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    p = lambda t: t.data_ptr()  # noqa: E731
+    m.write_beziers(tcurves)
+    targets = []
+    for vp, vi in cams:
+        m.writeVPMatrix(vp)
+        m.writeVPInvMatrix(vi)
+        m.set_backward_target_dev(p(zeros4), p(ones1), p(zeros4), p(ones1), p(zeros4), W, H)
+        m.forward()
+        C, D, _ = m.read_image()
+        t = np.zeros((H, W, 4), np.float32)
+        t[..., :3] = C
+        t[..., 3] = 1.0
+        targets.append((torch.from_numpy(t).to(dev), torch.from_numpy(np.ascontiguousarray(D)).to(dev)))
+    torch.cuda.synchronize()
+    m.write_beziers(curves)
+    m.enable_timing(True)
+
+    def one(k):
+        vp, vi = cams[k % views]
+        tg, td = targets[k % views]
+        m.writeVPMatrix(vp)
+        m.writeVPInvMatrix(vi)
+        m.set_backward_target_dev(p(tg), p(td), p(zeros4), p(ones1), p(zeros4), W, H)
+        m.dispatch()
+        return m.read_timing(), m.read_stats()
+
+    for k in range(warmup):
+        one(k)
+    m.sync()
+    rows = []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        rows.append(one(warmup + k))
+    m.sync()
+    wall = time.perf_counter() - t0
+    m.adc()  # one densification pass (every 50th dispatch), timed on its own
+    adc_ms = m.read_timing()["adc"]
+    st_adc = m.read_stats()
+
+    def mean(key, src=0):
+        return float(np.mean([r[src][key] for r in rows]))
+    bwd, comp, step = mean("backward"), mean("composite"), mean("step")
+    fwd_eval, rev_eval = mean("evaluated_wave_entries", 1), mean("reverse_evaluated_wave_entries", 1)
+    rev_pairs = mean("reverse_pairs", 1)
+    flops = 64.0 * (fwd_eval + rev_eval) * BZ_FLOP_PER_EVAL + rev_pairs * BZ_FLOP_PER_REV_PAIR
+    achieved = flops / (comp * 1e-3) / 1e12
+    # Adam: 18 parameters x (grad i32, m, v) + 20-float curve + grad_accum, read and written
+    adam_gbs = (18 * 12 + 80 + 4) * 2.0 * n / (step * 1e-3) / 1e9
+    cpu = None
+    if cpu_tiles > 0:
+        from oracle import oracle
+
+        oracle.load()
+        vp, vi = cams[0]
+        tg, td = targets[0]
+        uni = oracle.bz_uniforms(vp, vi, mode=1.0)
+        tex = (tg.cpu().numpy(), td.cpu().numpy(), np.zeros((H, W, 4), np.float32), np.ones((H, W), np.float32),
+               np.zeros((H, W, 4), np.float32))
+        ntiles = ((W + 15) // 16) * ((H + 15) // 16)
+        rng = np.random.default_rng(0)
+        tiles = np.sort(rng.choice(ntiles, size=min(cpu_tiles, ntiles), replace=False)).astype(np.int32)
+        c1 = time.perf_counter()
+        lists, cnt, _ = oracle.bz_bin(curves, uni, W, H, tiles)
+        oracle.bz_backward_f32(curves, uni, *tex, W, H, tiles=tiles, lists=lists, counts=cnt)
+        dt = time.perf_counter() - c1
+        cpu = {"value": round(dt * 1000.0 * ntiles / len(tiles), 1), "unit": "ms/view (backward)",
+               "cores": int(oracle.load().ora_num_threads()), "kind": "port",
+               "sample": f"{len(tiles)} random tiles of view 0 ({dt:.1f} s): binning + fwd + loss + bwd, "
+                         f"scaled by {ntiles}/{len(tiles)} tiles"}
+    m.close()
+    return {
+        "workload": f"Bezier strokes: {n} curves, {views} ring views at {W}x{H}, colour mode, dispatch = "
+                    "backward + Adam (densification every 50 steps)",
+        "backward_ms_per_view": round(bwd, 3),
+        "step_ms": round(step, 3),
+        "ms_per_step_wall": round(wall * 1000.0 / steps, 3),
+        "steps_per_s": round(steps / wall, 2),
+        "densification_ms": round(adc_ms, 3),
+        "densification_ms_per_step": round(adc_ms / 50.0, 3),
+        "phases_ms": {k: round(mean(k), 3) for k in ("preprocess", "keys", "sort", "composite", "step")},
+        "tile_keys_per_view": int(mean("keys", 1)),
+        "pairs_per_view": int(mean("pairs", 1)),
+        "evaluated_pair_fraction": round(64.0 * fwd_eval / max(1.0, mean("pairs", 1)), 4),
+        "reverse_pairs_per_view": int(rev_pairs),
+        "roofline": {"bound": "mfma", "compute": "fp32 VALU (no matrix work on this path)",
+                     "kernel": "k_bz_composite", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None},
+        "adam_hbm_gbs": round(adam_gbs, 1),
+        "adc_stats": {k: st_adc[k] for k in ("adc_dead", "adc_killed", "adc_split1", "adc_spawned", "adc_split3")},
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": round(cpu["value"] / bwd, 1) if cpu else None,
+    }
+
+
+def main_bezier(args):
+    print("[bench] Bezier strokes: 100k curves, 4 views at 1024x1024 ...", file=sys.stderr, flush=True)
+    r = bezier_config3(steps=args.steps, warmup=args.warmup, cpu_tiles=args.cpu_tiles, small=args.small)
+    out = {
+        "metric": "Bezier backward ms/view @100k curves 1024x1024",
+        "value": r["backward_ms_per_view"],
+        "unit": "ms/view",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step_wall"],
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded curves, targets rendered from a seed-1 curve set)",
+        "config": {"workload": r["workload"]},
+        "roofline": r["roofline"],
+        "cpu_baseline": r["cpu_baseline"],
+        "bezier": r,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--small", action="store_true", help="16,384 splats at 256x256 (quick check)")
-    ap.add_argument("--mode", choices=("ext", "ref"), default="ext")
+    ap.add_argument("--mode", choices=("ext", "ref", "bezier"), default="ext")
     ap.add_argument("--no-batched", action="store_true", help="skip the multi-view (64 views, 4 scenes) comparison")
+    ap.add_argument("--no-bezier", action="store_true", help="skip the Bezier stroke workload")
     ap.add_argument("--cpu-tiles", type=int, default=48, help="tiles in the CPU-baseline sample (0 = skip)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one plain streaming kernel (k_debug_math, 16M floats: 64 MiB read + 64 MiB "
@@ -244,6 +396,8 @@ def main():
 
     if args.mode == "ext":
         return main_ext(args)
+    if args.mode == "bezier":
+        return main_bezier(args)
 
     from nprush_amd import GpuSplatOptimizerManager
 
@@ -328,7 +482,7 @@ def main():
         "config": {"workload": f"REF splat fwd+bwd+Adam, {n_live} live / {capacity} capacity splats, {W}x{H}, "
                                f"{views} ring views, 16x16 tiles, cap 1024/tile",
                    "views": views, "width": W, "height": H, "splats": n_live, "capacity": capacity},
-        "roofline": {"bound": "valu", "kernel": "k_composite_ref", "achieved": round(achieved, 3),
+        "roofline": {"bound": "mfma", "compute": "fp32 VALU (no matrix work on this path)", "kernel": "k_composite_ref", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": traffic},
         "phases_ms": phases,
@@ -418,6 +572,10 @@ def main_ext(args):
     if not args.small and not args.no_batched:
         print("[bench] multi-view step: graph vs per-view launches ...", file=sys.stderr, flush=True)
         batched = batched_config2()
+    bezier = None
+    if not args.small and not args.no_bezier:
+        print("[bench] Bezier strokes: 100k curves, 4 views at 1024x1024 ...", file=sys.stderr, flush=True)
+        bezier = bezier_config3(cpu_tiles=1024 if args.cpu_tiles > 0 else 0)
     cpu = None
     if args.cpu_tiles > 0:
         print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
@@ -442,7 +600,7 @@ def main_ext(args):
                    "tile": tile, "lds_batch": batch},
         "autotune_ms": {f"tile{t}_batch{b}": round(float(tune_ms[i, j]), 3) for i, t in enumerate((8, 16, 32))
                         for j, b in enumerate((32, 64, 128))},
-        "roofline": {"bound": "valu", "kernel": "k_composite_ext", "achieved": round(achieved, 3),
+        "roofline": {"bound": "mfma", "compute": "fp32 VALU (no matrix work on this path)", "kernel": "k_composite_ext", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": traffic},
         "phases_ms": phases,
@@ -455,6 +613,7 @@ def main_ext(args):
         "adam_roofline_frac": round(adam_gbs / PEAK_HBM_GBS, 3),
         "steps_per_s": round(args.steps / wall, 2),
         "batched_multiview": batched,
+        "bezier_strokes": bezier,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -333,9 +333,11 @@ int nbz_read_loss(nbz_handle h, double loss[2]);
 /* tile lists of the last backward (ascending curve ids, first 1024 per tile) */
 int nbz_read_tiles(nbz_handle h, uint32_t* ranges, uint32_t* ids, int32_t* num_tiles, int64_t* num_keys);
 /* [0] keys, [1] tiles over the cap, [2] pairs, [3] reverse pairs, [4] reverse
- * wave-entries, [5] evaluated wave-entries, [6..11] last ADC: dead slots found,
- * killed, split/clone pass 1, spawned, split/clone pass 3, dead slots left */
-int nbz_read_stats(nbz_handle h, int64_t stats[12]);
+ * wave-entries, [5] forward wave-entries that evaluate the distance loop (the
+ * rest are provably alpha 0), [6..11] last ADC: dead slots found, killed,
+ * split/clone pass 1, spawned, split/clone pass 3, dead slots left, [12]
+ * reverse-recompute wave-entries that evaluate the distance loop */
+int nbz_read_stats(nbz_handle h, int64_t stats[16]);
 /* ms of [0] preprocess+cut, [1] keys, [2] sort+ranges, [3] composite, [4] step,
  * [5] adc, [6] backward total of the last calls */
 int nbz_enable_timing(nbz_handle h, int32_t on);

@@ -151,6 +151,21 @@ __global__ __launch_bounds__(256) void k_bz_preprocess(const float4* __restrict_
     const float M = fmax_w(fmax_w(fabsf(bx0), fabsf(bx1)), fmax_w(fabsf(by0), fabsf(by1)));
     f.rcull = (width + soft) * 1.0001f + 1e-5f * (4.0f + M);
     f.pad[0] = f.pad[1] = f.pad[2] = 0.0f;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        float gx0 = f.x[4 * g], gx1 = f.x[4 * g], gy0 = f.y[4 * g], gy1 = f.y[4 * g];
+#pragma unroll
+        for (int j = 1; j <= 4; ++j) {
+            gx0 = fmin_w(gx0, f.x[4 * g + j]);
+            gx1 = fmax_w(gx1, f.x[4 * g + j]);
+            gy0 = fmin_w(gy0, f.y[4 * g + j]);
+            gy1 = fmax_w(gy1, f.y[4 * g + j]);
+        }
+        f.gb[4 * g + 0] = gx0;
+        f.gb[4 * g + 1] = gy0;
+        f.gb[4 * g + 2] = gx1;
+        f.gb[4 * g + 3] = gy1;
+    }
     fwd[i] = f;
 
     BzBwd b;
@@ -184,18 +199,41 @@ struct BzEval {
     uint32_t k;
 };
 
+// The wave's 8x8 pixel block in screen space and the squared distance beyond
+// which every alpha of the entry is exactly 0 (rcull plus the block's margin).
+struct BzBlock {
+    float xa, xb, ya, yb;
+};
+
 // Forward of one (pixel, curve) pair, BW:196-256.  The minimum over the 16
-// segments is taken with the reference's strict `<` on the distances; the
-// square root is only taken where d^2 < (d^2 of the current minimum), since
-// otherwise sqrt(d^2) >= the current minimum (sqrt is monotone and correctly
-// rounded), and u = clamp(num / len2, 0, 1) is 0 for num <= 0 and 1 for
-// num >= len2 (finite len2) without the division.
-__device__ __forceinline__ BzEval bz_eval(const BzFwd& f, float px, float py) {
+// segments is taken with the reference's strict `<` on the distances.  Exact
+// shortcuts, each leaving alpha (and, where alpha > 0, the segment, its
+// parameter and the depth) unchanged:
+//  - a group of 4 segments whose box is farther than the cull radius from the
+//    wave's block is skipped: all their distances exceed width + softness, so
+//    none is the minimum of a pixel with alpha > 0, and a pixel whose minimum
+//    lies there has alpha 0 either way (the test is conservative, so it uses
+//    the native min/max);
+//  - the square root is only taken where d^2 < (d^2 of the current minimum),
+//    since otherwise sqrt(d^2) >= the current minimum (sqrt is monotone and
+//    correctly rounded);
+//  - u = clamp(num / len2, 0, 1) is 0 for num <= 0 and 1 for num >= len2
+//    (finite len2) without the division;
+//  - a minimum distance >= fl(width + softness) >= the outer radius at any
+//    pressure gives alpha 0 without the taper and depth terms.
+__device__ __forceinline__ BzEval bz_eval(const BzFwd& f, float px, float py, const BzBlock& bk, float thr2) {
     float min_d = 1e9f, min_d2 = __builtin_inff(), min_u = 0.0f;
     uint32_t min_k = 1u;
-    float prevx = f.x[0], prevy = f.y[0];
 #pragma unroll
-    for (int k = 1; k <= kBzSeg; ++k) {
+    for (int g = 0; g < 4; ++g) {
+    const float gx = __builtin_fmaxf(__builtin_fmaxf(f.gb[4 * g + 0] - bk.xb, bk.xa - f.gb[4 * g + 2]), 0.0f);
+    const float gy = __builtin_fmaxf(__builtin_fmaxf(f.gb[4 * g + 1] - bk.yb, bk.ya - f.gb[4 * g + 3]), 0.0f);
+    // the operands are the same on every lane (LDS broadcast, the wave's
+    // block): a scalar branch
+    if (__builtin_amdgcn_readfirstlane((int)(gx * gx + gy * gy > thr2))) continue;
+    float prevx = f.x[4 * g], prevy = f.y[4 * g];
+#pragma unroll
+    for (int k = 4 * g + 1; k <= 4 * g + 4; ++k) {
         const float cx = f.x[k], cy = f.y[k];
         const float sx = cx - prevx, sy = cy - prevy;
         const float len2 = fmax_w(sx * sx + sy * sy, 1e-8f);
@@ -220,12 +258,19 @@ __device__ __forceinline__ BzEval bz_eval(const BzFwd& f, float px, float py) {
         prevx = cx;
         prevy = cy;
     }
+    }
+    BzEval e;
+    if (min_d >= f.width + f.soft) {
+        e.a = 0.0f;
+        e.dval = 0.0f;
+        e.k = min_k;
+        return e;
+    }
     const float t = ((float)(min_k - 1u) + min_u) / (float)kBzSeg;
     const float dt = t - 0.5f;
     const float pressure = 1.0f - 4.0f * dt * dt;
     const float lw = f.width * pressure, ls = f.soft * pressure, lo = f.op * pressure;
     const float a_geom = 1.0f - bz_smoothstep(lw - ls, lw + ls, min_d);
-    BzEval e;
     e.a = clamp_w(a_geom * lo, 0.0f, 0.999f);
     float B[4];
     bz_bern(t, B);
@@ -403,15 +448,19 @@ __global__ __launch_bounds__(256) void k_bz_composite(ViewParams v, BzCompositeA
     const float xa = px_to_x(bx, v.W, v.aspect), xb = px_to_x(bx + 7, v.W, v.aspect);
     const float ya = py_to_y(by + 7, v.H), yb = py_to_y(by, v.H);
     const float blk_margin = 1e-5f * (((fabsf(xa) + fabsf(xb)) + fabsf(ya)) + fabsf(yb));
+    const BzBlock blk = {xa, xb, ya, yb};
     float* ckpt = A.ckpt + (size_t)tile * A.nchunk_max * kTilePx;
     const bool is_fine = A.prm.max_width > 0.0f;
     const bool color_mode = A.prm.mode > 0.5f;
 
-    auto culled = [&](const BzFwd& f) {
-        const float gx = fmax_w(fmax_w(f.cx0 - xb, xa - f.cx1), 0.0f);
-        const float gy = fmax_w(fmax_w(f.cy0 - yb, ya - f.cy1), 0.0f);
+    auto thr2_of = [&](const BzFwd& f) {
         const float thr = f.rcull + blk_margin;
-        return gx * gx + gy * gy > thr * thr;
+        return thr * thr;
+    };
+    auto culled = [&](const BzFwd& f, float thr2) {
+        const float gx = __builtin_fmaxf(__builtin_fmaxf(f.cx0 - xb, xa - f.cx1), 0.0f);
+        const float gy = __builtin_fmaxf(__builtin_fmaxf(f.cy0 - yb, ya - f.cy1), 0.0f);
+        return __builtin_amdgcn_readfirstlane((int)(gx * gx + gy * gy > thr2)) != 0;
     };
 
     // ---------------- forward (BW:188-257) ----------------
@@ -428,9 +477,10 @@ __global__ __launch_bounds__(256) void k_bz_composite(ViewParams v, BzCompositeA
             if ((e & (kChunk - 1)) == 0) ckpt[(size_t)(e / kChunk) * kTilePx + tid] = T;
             const BzFwd& f = s_f[j];
             float a = 0.0f, dval = 0.0f;
-            if (!culled(f)) {
+            const float thr2 = thr2_of(f);
+            if (!culled(f, thr2)) {
                 n_feval += 1u;
-                BzEval ev = bz_eval(f, pxf, pyf);
+                BzEval ev = bz_eval(f, pxf, pyf, blk, thr2);
                 a = ev.a;
                 dval = ev.dval;
             }
@@ -489,7 +539,7 @@ __global__ __launch_bounds__(256) void k_bz_composite(ViewParams v, BzCompositeA
         s_red[0][wid] = l2;
         s_red[1][wid] = l1;
     }
-    unsigned long long n_active = 0, n_went = 0;
+    unsigned long long n_active = 0, n_went = 0, n_reval = 0;
     if (A.backward) {
         const float L0 = 0.2126f, L1 = 0.7152f, L2 = 0.0722f;
         const float dr0 = 2.0f * (C0 - tg.x), dr1 = 2.0f * (C1 - tg.y), dr2 = 2.0f * (C2 - tg.z);
@@ -556,7 +606,7 @@ __global__ __launch_bounds__(256) void k_bz_composite(ViewParams v, BzCompositeA
                 float Tc = ckpt[(size_t)(e0 / kChunk) * kTilePx + tid];
                 float ra[kChunk], rT[kChunk], rd[kChunk];
                 uint32_t rk[kChunk];
-#pragma unroll
+#pragma unroll 1
                 for (int j = 0; j < kChunk; ++j) {
                     ra[j] = 0.0f;
                     rd[j] = 0.0f;
@@ -564,8 +614,10 @@ __global__ __launch_bounds__(256) void k_bz_composite(ViewParams v, BzCompositeA
                     rT[j] = Tc;
                     if (j <= hi) {
                         const BzFwd& f = s_f[c * kChunk + j];
-                        if (!culled(f)) {
-                            BzEval ev = bz_eval(f, pxf, pyf);
+                        const float thr2 = thr2_of(f);
+                        if (!culled(f, thr2)) {
+                            n_reval += 1ull;
+                            BzEval ev = bz_eval(f, pxf, pyf, blk, thr2);
                             ra[j] = ev.a;
                             rd[j] = ev.dval;
                             rk[j] = ev.k;
@@ -573,7 +625,7 @@ __global__ __launch_bounds__(256) void k_bz_composite(ViewParams v, BzCompositeA
                         Tc = Tc * (1.0f - ra[j]);
                     }
                 }
-#pragma unroll
+#pragma unroll 1
                 for (int j = kChunk - 1; j >= 0; --j) {
                     if ((bits >> j) & 1u) {
                         const int jj = c * kChunk + j;
@@ -624,6 +676,7 @@ __global__ __launch_bounds__(256) void k_bz_composite(ViewParams v, BzCompositeA
         if (n_active) atomicAdd(&A.stats[2], n_active);
         if (n_went) atomicAdd(&A.stats[3], n_went);
         if (n_feval) atomicAdd(&A.stats[4], (unsigned long long)n_feval);
+        if (n_reval) atomicAdd(&A.stats[5], n_reval);
     }
     __syncthreads();
     if (tid == 0) {

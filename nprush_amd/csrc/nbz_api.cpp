@@ -495,9 +495,9 @@ int nbz_read_tiles(nbz_handle h, uint32_t* ranges, uint32_t* ids, int32_t* num_t
     }
     return NPS_OK;
 }
-int nbz_read_stats(nbz_handle h, int64_t stats[12]) {
+int nbz_read_stats(nbz_handle h, int64_t stats[16]) {
     if (!h || !stats) return NPS_ERR_ARG;
-    for (int k = 0; k < 12; ++k) stats[k] = 0;
+    for (int k = 0; k < 16; ++k) stats[k] = 0;
     int rc;
     if (h->have_image) {
         uint64_t d[8];
@@ -508,6 +508,7 @@ int nbz_read_stats(nbz_handle h, int64_t stats[12]) {
         stats[3] = (int64_t)d[2];
         stats[4] = (int64_t)d[3];
         stats[5] = (int64_t)d[4];
+        stats[12] = (int64_t)d[5];
     }
     if (h->have_adc_stats) {
         int64_t a[6];

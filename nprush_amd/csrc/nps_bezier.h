@@ -29,6 +29,7 @@ struct BzFwd {
     float cx0, cy0, cx1, cy1;            // box of P0..P16
     float rcull;                         // beyond this distance from the box every alpha is exactly 0
     float pad[3];
+    float gb[16];                        // boxes (x0, y0, x1, y1) of the points of segments 4g+1..4g+4
 };
 struct BzBwd {
     float q0x, q0y;   // bezier_at(0): the reverse pass's first segment start
@@ -36,7 +37,7 @@ struct BzBwd {
     float J1[12];     // d screen.y / d world
     float pad[2];
 };
-static_assert(sizeof(BzFwd) == 208, "BzFwd layout");
+static_assert(sizeof(BzFwd) == 272, "BzFwd layout");
 static_assert(sizeof(BzBwd) == 112, "BzBwd layout");
 
 // BezierUniforms scalars (bezier_backward.wgsl:20-32) and AdamState.no_kill.
@@ -67,7 +68,8 @@ struct BzCompositeArgs {
     float* t_img;              // H*W final transmittance
     double* tile_loss;         // 2 * num_tiles: sum |C - tgt|^2, sum |C - tgt|
     float* ckpt;               // num_tiles * nchunk_max * 256
-    unsigned long long* stats; // [0] pairs, [2] reverse pairs, [3] reverse wave-entries, [4] evaluated wave-entries
+    unsigned long long* stats; // [0] pairs, [2] reverse pairs, [3] reverse wave-entries, [4] forward and
+                               // [5] reverse-recompute wave-entries that evaluate the distance loop
     BzParams prm;
     uint32_t cap;
     uint32_t nchunk_max;

@@ -166,15 +166,15 @@ NPS_HD int32_t f2i_sat(float x) {
 }
 
 // WGSL max/min (IEEE maxNum: a NaN operand loses) and clamp = min(max(e, lo), hi).
+// Written as selects (no branches): a NaN `a` falls through the compare to b,
+// a NaN `b` alone gives a, two NaNs give b.
 NPS_HD float fmax_w(float a, float b) {
-    if (a != a) return b;
-    if (b != b) return a;
-    return a > b ? a : b;
+    const float r = a > b ? a : b;
+    return ((b != b) & (a == a)) ? a : r;
 }
 NPS_HD float fmin_w(float a, float b) {
-    if (a != a) return b;
-    if (b != b) return a;
-    return a < b ? a : b;
+    const float r = a < b ? a : b;
+    return ((b != b) & (a == a)) ? a : r;
 }
 NPS_HD float clamp_w(float e, float lo, float hi) { return fmin_w(fmax_w(e, lo), hi); }
 

@@ -662,10 +662,11 @@ class GpuBezierOptimizerManager:
         return ranges, ids[:nk.value]
 
     def read_stats(self) -> dict:
-        s = np.zeros(12, np.int64)
+        s = np.zeros(16, np.int64)
         self._check(self._lib.nbz_read_stats(self._h, lptr(s)), "nbz_read_stats")
         keys = ["keys", "tiles_over_cap", "pairs", "reverse_pairs", "reverse_wave_entries", "evaluated_wave_entries",
-                "adc_dead", "adc_killed", "adc_split1", "adc_spawned", "adc_split3", "adc_dead_left"]
+                "adc_dead", "adc_killed", "adc_split1", "adc_spawned", "adc_split3", "adc_dead_left",
+                "reverse_evaluated_wave_entries"]
         return {k: int(v) for k, v in zip(keys, s)}
 
     def enable_timing(self, on: bool = True):
