@@ -292,6 +292,16 @@ def bezier_config3(steps=20, warmup=3, cpu_tiles=16, n=100_000, W=1024, H=1024, 
         rows.append(one(warmup + k))
     m.sync()
     wall = time.perf_counter() - t0
+    # image only (the composite's forward half) on the current state, view 0
+    vp, vi = cams[0]
+    tg, td = targets[0]
+    m.writeVPMatrix(vp)
+    m.writeVPInvMatrix(vi)
+    m.set_backward_target_dev(p(tg), p(td), p(zeros4), p(ones1), p(zeros4), W, H)
+    fwd_ms = []
+    for _ in range(3):
+        m.forward()
+        fwd_ms.append(m.read_timing()["composite"])
     m.adc()  # one densification pass (every 50th dispatch), timed on its own
     adc_ms = m.read_timing()["adc"]
     st_adc = m.read_stats()
@@ -301,7 +311,10 @@ def bezier_config3(steps=20, warmup=3, cpu_tiles=16, n=100_000, W=1024, H=1024, 
     bwd, comp, step = mean("backward"), mean("composite"), mean("step")
     fwd_eval, rev_eval = mean("evaluated_wave_entries", 1), mean("reverse_evaluated_wave_entries", 1)
     rev_pairs = mean("reverse_pairs", 1)
-    flops = 64.0 * (fwd_eval + rev_eval) * BZ_FLOP_PER_EVAL + rev_pairs * BZ_FLOP_PER_REV_PAIR
+    # algorithmic work: every (pixel, curve) pair of the capped tile lists is
+    # evaluated once (the reference's forward), every active pair once in reverse
+    pairs = mean("pairs", 1)
+    flops = pairs * BZ_FLOP_PER_EVAL + rev_pairs * BZ_FLOP_PER_REV_PAIR
     achieved = flops / (comp * 1e-3) / 1e12
     # Adam: 18 parameters x (grad i32, m, v) + 20-float curve + grad_accum, read and written
     adam_gbs = (18 * 12 + 80 + 4) * 2.0 * n / (step * 1e-3) / 1e9
@@ -340,6 +353,8 @@ def bezier_config3(steps=20, warmup=3, cpu_tiles=16, n=100_000, W=1024, H=1024, 
         "tile_keys_per_view": int(mean("keys", 1)),
         "pairs_per_view": int(mean("pairs", 1)),
         "evaluated_pair_fraction": round(64.0 * fwd_eval / max(1.0, mean("pairs", 1)), 4),
+        "reverse_recompute_pair_fraction": round(64.0 * rev_eval / max(1.0, mean("pairs", 1)), 4),
+        "forward_only_composite_ms": round(float(np.median(fwd_ms)), 3),
         "reverse_pairs_per_view": int(rev_pairs),
         "roofline": {"bound": "mfma", "compute": "fp32 VALU (no matrix work on this path)",
                      "kernel": "k_bz_composite", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,

@@ -432,6 +432,7 @@ __global__ __launch_bounds__(256) void k_bz_composite(ViewParams v, BzCompositeA
     __shared__ uint32_t s_grad[kBatch][kBzParams];
     __shared__ uint32_t s_lossp[kBatch][4];
     __shared__ uint32_t s_act[4][kBzCap / 32];
+    __shared__ uint32_t s_nz[4][kBzCap / 32];
     __shared__ double s_red[2][4];
 
     const uint32_t tile = xcd_remap(blockIdx.x, v.num_tiles);
@@ -465,7 +466,7 @@ __global__ __launch_bounds__(256) void k_bz_composite(ViewParams v, BzCompositeA
 
     // ---------------- forward (BW:188-257) ----------------
     float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, D = 0.0f;
-    uint32_t actword = 0u;
+    uint32_t actword = 0u, nzword = 0u;
     uint32_t n_feval = 0u;
     for (int b0 = 0; b0 < count; b0 += kBatch) {
         const int nb = min(kBatch, count - b0);
@@ -483,11 +484,24 @@ __global__ __launch_bounds__(256) void k_bz_composite(ViewParams v, BzCompositeA
                 BzEval ev = bz_eval(f, pxf, pyf, blk, thr2);
                 a = ev.a;
                 dval = ev.dval;
+                // the reverse reloads the pairs of entries where some lane has alpha > 0
+                // (all-zero entries leave the transmittance unchanged)
+                if (A.backward && __ballot(a != 0.0f) != 0ull) {
+                    nzword |= 1u << (e & 31);
+                    const size_t q = (size_t)(start + (uint32_t)e) * kTilePx + tid;
+                    A.pa[q] = a;
+                    A.pd[q] = dval;
+                    A.pk[q] = (uint8_t)ev.k;
+                }
             }
             if (__ballot(valid && !(a < 0.001f)) != 0ull) actword |= 1u << (e & 31);
             if ((e & 31) == 31 || e == count - 1) {
-                if (lane == 0) s_act[wid][e >> 5] = actword;
+                if (lane == 0) {
+                    s_act[wid][e >> 5] = actword;
+                    s_nz[wid][e >> 5] = nzword;
+                }
                 actword = 0u;
+                nzword = 0u;
             }
             const float Ta = T * a;
             C0 = C0 + Ta * f.r;
@@ -603,27 +617,29 @@ __global__ __launch_bounds__(256) void k_bz_composite(ViewParams v, BzCompositeA
                 const uint32_t bits = (s_act[wid][e0 >> 5] >> (e0 & 31)) & ((1u << kChunk) - 1u);
                 if (bits == 0u) continue;
                 const int hi = 31 - __clz((int)bits);
+                const uint32_t nzb = (s_nz[wid][e0 >> 5] >> (e0 & 31)) & ((1u << kChunk) - 1u);
                 float Tc = ckpt[(size_t)(e0 / kChunk) * kTilePx + tid];
                 float ra[kChunk], rT[kChunk], rd[kChunk];
                 uint32_t rk[kChunk];
-#pragma unroll 1
+                // alphas, depths and segments stored by the forward; the
+                // transmittance chain is recomputed from the checkpoint
+#pragma unroll
                 for (int j = 0; j < kChunk; ++j) {
                     ra[j] = 0.0f;
                     rd[j] = 0.0f;
                     rk[j] = 1u;
-                    rT[j] = Tc;
-                    if (j <= hi) {
-                        const BzFwd& f = s_f[c * kChunk + j];
-                        const float thr2 = thr2_of(f);
-                        if (!culled(f, thr2)) {
-                            n_reval += 1ull;
-                            BzEval ev = bz_eval(f, pxf, pyf, blk, thr2);
-                            ra[j] = ev.a;
-                            rd[j] = ev.dval;
-                            rk[j] = ev.k;
-                        }
-                        Tc = Tc * (1.0f - ra[j]);
+                    if (j <= hi && ((nzb >> j) & 1u)) {
+                        const size_t q = (size_t)(start + (uint32_t)(e0 + j)) * kTilePx + tid;
+                        ra[j] = A.pa[q];
+                        rd[j] = A.pd[q];
+                        rk[j] = A.pk[q];
+                        n_reval += 1ull;
                     }
+                }
+#pragma unroll
+                for (int j = 0; j < kChunk; ++j) {
+                    rT[j] = Tc;
+                    if (j <= hi) Tc = Tc * (1.0f - ra[j]);
                 }
 #pragma unroll 1
                 for (int j = kChunk - 1; j >= 0; --j) {

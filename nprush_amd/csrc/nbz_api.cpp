@@ -43,7 +43,7 @@ struct nbz_ctx {
     const float *tgt = nullptr, *tdepth = nullptr, *bg = nullptr, *bgd = nullptr, *normal = nullptr;
     DevBuf pixel_loss;
     uint32_t pl_w = 0, pl_h = 0;
-    DevBuf fwd, bwd, ckpt, c_img, d_img, t_img, tile_loss, stats;
+    DevBuf fwd, bwd, ckpt, c_img, d_img, t_img, tile_loss, stats, pa, pd, pk;
     Scratch sc;
     uint32_t last_tiles = 0;
     uint32_t img_w = 0, img_h = 0;
@@ -130,6 +130,12 @@ int bz_run(nbz_ctx* h, bool backward) {
     if ((rc = bz_ensure(h, h->d_img, 4 * std::max<size_t>(px, 1))) != NPS_OK) return rc;
     if ((rc = bz_ensure(h, h->t_img, 4 * std::max<size_t>(px, 1))) != NPS_OK) return rc;
     if ((rc = bz_ensure(h, h->tile_loss, 16 * t1)) != NPS_OK) return rc;
+    const size_t pslots = (size_t)std::max<uint64_t>(sc.m, 1) * kTilePx;
+    if (backward) {
+        if ((rc = bz_ensure(h, h->pa, 4 * pslots)) != NPS_OK) return rc;
+        if ((rc = bz_ensure(h, h->pd, 4 * pslots)) != NPS_OK) return rc;
+        if ((rc = bz_ensure(h, h->pk, pslots)) != NPS_OK) return rc;
+    }
     if (h->timing) NBZ_HIP(h, hipEventRecord(h->ev[3], h->stream));
     BzCompositeArgs a;
     a.fwd = (const BzFwd*)h->fwd.p;
@@ -149,6 +155,9 @@ int bz_run(nbz_ctx* h, bool backward) {
     a.t_img = (float*)h->t_img.p;
     a.tile_loss = (double*)h->tile_loss.p;
     a.ckpt = (float*)h->ckpt.p;
+    a.pa = (float*)h->pa.p;
+    a.pd = (float*)h->pd.p;
+    a.pk = (uint8_t*)h->pk.p;
     a.stats = (unsigned long long*)h->stats.p;
     a.prm = h->prm;
     a.cap = cap;
@@ -267,6 +276,7 @@ void nbz_destroy(nbz_handle h) {
         if (p) (void)hipFree(p);
     DevBuf* bs[] = {&h->own_tgt, &h->own_tdepth, &h->own_bg, &h->own_bgd, &h->own_normal, &h->pixel_loss,
                     &h->fwd, &h->bwd, &h->ckpt, &h->c_img, &h->d_img, &h->t_img, &h->tile_loss, &h->stats,
+                    &h->pa, &h->pd, &h->pk,
                     &h->adc_dead, &h->adc_cand, &h->adc_stats};
     for (DevBuf* b : bs) free_buf(*b);
     free_scratch(h->sc);

@@ -68,8 +68,11 @@ struct BzCompositeArgs {
     float* t_img;              // H*W final transmittance
     double* tile_loss;         // 2 * num_tiles: sum |C - tgt|^2, sum |C - tgt|
     float* ckpt;               // num_tiles * nchunk_max * 256
+    float* pa;                 // per (key, pixel of its tile): alpha, depth and segment of the forward,
+    float* pd;                 //   written for entries where some lane of the wave has alpha > 0
+    uint8_t* pk;
     unsigned long long* stats; // [0] pairs, [2] reverse pairs, [3] reverse wave-entries, [4] forward and
-                               // [5] reverse-recompute wave-entries that evaluate the distance loop
+                               // [5] reverse wave-entries reloaded from the forward's pair buffers
     BzParams prm;
     uint32_t cap;
     uint32_t nchunk_max;
