@@ -425,14 +425,13 @@ __device__ __forceinline__ void bz_stage(const BzFwd* __restrict__ fwd, const Bz
     }
 }
 
-__global__ __launch_bounds__(256) void k_bz_composite(ViewParams v, BzCompositeArgs A) {
+// Forward half: per tile the front-to-back composite (BW:188-257), images,
+// loss, transmittance checkpoints, the pair buffers and activity masks of the
+// reverse, and the per-pixel loss seeds (BW:259-320), which the reverse kernel
+// reads back (seeds, masks and pair values are written only with `backward`).
+__global__ __launch_bounds__(256) void k_bz_forward(ViewParams v, BzCompositeArgs A) {
     __shared__ BzFwd s_f[kBatch];
-    __shared__ BzBwd s_b[kBatch];
     __shared__ uint32_t s_ids[kBatch];
-    __shared__ uint32_t s_grad[kBatch][kBzParams];
-    __shared__ uint32_t s_lossp[kBatch][4];
-    __shared__ uint32_t s_act[4][kBzCap / 32];
-    __shared__ uint32_t s_nz[4][kBzCap / 32];
     __shared__ double s_red[2][4];
 
     const uint32_t tile = xcd_remap(blockIdx.x, v.num_tiles);
@@ -471,7 +470,7 @@ __global__ __launch_bounds__(256) void k_bz_composite(ViewParams v, BzCompositeA
     for (int b0 = 0; b0 < count; b0 += kBatch) {
         const int nb = min(kBatch, count - b0);
         __syncthreads();
-        bz_stage(A.fwd, A.bwd, A.keys, start + (uint32_t)b0, nb, s_f, s_b, s_ids, false);
+        bz_stage(A.fwd, A.bwd, A.keys, start + (uint32_t)b0, nb, s_f, nullptr, s_ids, false);
         __syncthreads();
         for (int j = 0; j < nb; ++j) {
             const int e = b0 + j;
@@ -496,9 +495,10 @@ __global__ __launch_bounds__(256) void k_bz_composite(ViewParams v, BzCompositeA
             }
             if (__ballot(valid && !(a < 0.001f)) != 0ull) actword |= 1u << (e & 31);
             if ((e & 31) == 31 || e == count - 1) {
-                if (lane == 0) {
-                    s_act[wid][e >> 5] = actword;
-                    s_nz[wid][e >> 5] = nzword;
+                if (lane == 0 && A.backward) {
+                    const size_t mo = ((size_t)tile * 4 + wid) * (kBzCap / 32) + (e >> 5);
+                    A.act[mo] = actword;
+                    A.nzm[mo] = nzword;
                 }
                 actword = 0u;
                 nzword = 0u;
@@ -553,7 +553,6 @@ __global__ __launch_bounds__(256) void k_bz_composite(ViewParams v, BzCompositeA
         s_red[0][wid] = l2;
         s_red[1][wid] = l1;
     }
-    unsigned long long n_active = 0, n_went = 0, n_reval = 0;
     if (A.backward) {
         const float L0 = 0.2126f, L1 = 0.7152f, L2 = 0.0722f;
         const float dr0 = 2.0f * (C0 - tg.x), dr1 = 2.0f * (C1 - tg.y), dr2 = 2.0f * (C2 - tg.z);
@@ -601,6 +600,76 @@ __global__ __launch_bounds__(256) void k_bz_composite(ViewParams v, BzCompositeA
             A.pixel_loss[pix] = (int32_t)((uint32_t)A.pixel_loss[pix] + (uint32_t)f2i_hw(residual * 10000.0f));
         }
 
+        const size_t so = (size_t)tile * kTilePx + tid, ss = (size_t)v.num_tiles * kTilePx;
+        A.seeds[so + 0 * ss] = s.dC[0];
+        A.seeds[so + 1 * ss] = s.dC[1];
+        A.seeds[so + 2 * ss] = s.dC[2];
+        A.seeds[so + 3 * ss] = s.dT;
+        A.seeds[so + 4 * ss] = s.dD;
+        A.seeds[so + 5 * ss] = s.is_bg;
+        A.seeds[so + 6 * ss] = s.fx;
+        A.seeds[so + 7 * ss] = s.fy;
+        A.seeds[so + 8 * ss] = s.have_flow ? 1.0f : 0.0f;
+    }
+    if (lane == 0 && n_feval) atomicAdd(&A.stats[4], (unsigned long long)n_feval);
+    __syncthreads();
+    if (tid == 0) {
+        double a2 = (s_red[0][0] + s_red[0][1]) + (s_red[0][2] + s_red[0][3]);
+        double a1 = (s_red[1][0] + s_red[1][1]) + (s_red[1][2] + s_red[1][3]);
+        A.tile_loss[2 * tile] = a2;
+        A.tile_loss[2 * tile + 1] = a1;
+        uint32_t vw = min(v.W - tx * kTile, (uint32_t)kTile), vh = min(v.H - ty * kTile, (uint32_t)kTile);
+        if (count > 0) atomicAdd(&A.stats[0], (unsigned long long)count * (unsigned long long)(vw * vh));
+    }
+}
+
+// Reverse half (BW:325-563) from the forward's seeds, checkpoints, masks and
+// pair buffers: gradients and loss contributions of every active pair.
+__global__ __launch_bounds__(256) void k_bz_reverse(ViewParams v, BzCompositeArgs A) {
+    __shared__ BzFwd s_f[kBatch];
+    __shared__ BzBwd s_b[kBatch];
+    __shared__ uint32_t s_ids[kBatch];
+    __shared__ uint32_t s_grad[kBatch][kBzParams];
+    __shared__ uint32_t s_lossp[kBatch][4];
+    __shared__ uint32_t s_act[4][kBzCap / 32];
+    __shared__ uint32_t s_nz[4][kBzCap / 32];
+
+    const uint32_t tile = xcd_remap(blockIdx.x, v.num_tiles);
+    const uint32_t tx = tile % v.tiles_x, ty = tile / v.tiles_x;
+    const uint32_t start = A.ranges[2 * tile], end = A.ranges[2 * tile + 1];
+    const int count = (int)min(end - start, A.cap);
+    if (count == 0) return;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const uint32_t bx = tx * kTile + (wid & 1) * 8, by = ty * kTile + (wid >> 1) * 8;
+    const uint32_t px = bx + (lane & 7), py = by + (lane >> 3);
+    const bool valid = px < v.W && py < v.H;
+    const float pxf = px_to_x(px, v.W, v.aspect);
+    const float pyf = py_to_y(py, v.H);
+    float* ckpt = A.ckpt + (size_t)tile * A.nchunk_max * kTilePx;
+    const bool is_fine = A.prm.max_width > 0.0f;
+    const int nwords = (count + 31) >> 5;
+    for (int e = tid; e < 4 * (kBzCap / 32); e += 256) {
+        const int w = e / (kBzCap / 32), k = e - w * (kBzCap / 32);
+        if (k < nwords) {
+            const size_t mo = ((size_t)tile * 4 + w) * (kBzCap / 32) + k;
+            s_act[w][k] = A.act[mo];
+            s_nz[w][k] = A.nzm[mo];
+        }
+    }
+    BzSeeds s;
+    {
+        const size_t so = (size_t)tile * kTilePx + tid, ss = (size_t)v.num_tiles * kTilePx;
+        s.dC[0] = A.seeds[so + 0 * ss];
+        s.dC[1] = A.seeds[so + 1 * ss];
+        s.dC[2] = A.seeds[so + 2 * ss];
+        s.dT = A.seeds[so + 3 * ss];
+        s.dD = A.seeds[so + 4 * ss];
+        s.is_bg = A.seeds[so + 5 * ss];
+        s.fx = A.seeds[so + 6 * ss];
+        s.fy = A.seeds[so + 7 * ss];
+        s.have_flow = A.seeds[so + 8 * ss] != 0.0f;
+    }
+    unsigned long long n_active = 0, n_went = 0, n_reval = 0;
         // ---------------- reverse pass (BW:325-563) ----------------
         const int nbatch = (count + kBatch - 1) / kBatch;
         for (int bi = nbatch - 1; bi >= 0; --bi) {
@@ -687,27 +756,17 @@ __global__ __launch_bounds__(256) void k_bz_composite(ViewParams v, BzCompositeA
                 if (l != 0ull) atomicAdd(&A.loss_acc[s_ids[j]], l);
             }
         }
-    }
     if (lane == 0) {
         if (n_active) atomicAdd(&A.stats[2], n_active);
         if (n_went) atomicAdd(&A.stats[3], n_went);
-        if (n_feval) atomicAdd(&A.stats[4], (unsigned long long)n_feval);
         if (n_reval) atomicAdd(&A.stats[5], n_reval);
-    }
-    __syncthreads();
-    if (tid == 0) {
-        double a2 = (s_red[0][0] + s_red[0][1]) + (s_red[0][2] + s_red[0][3]);
-        double a1 = (s_red[1][0] + s_red[1][1]) + (s_red[1][2] + s_red[1][3]);
-        A.tile_loss[2 * tile] = a2;
-        A.tile_loss[2 * tile + 1] = a1;
-        uint32_t vw = min(v.W - tx * kTile, (uint32_t)kTile), vh = min(v.H - ty * kTile, (uint32_t)kTile);
-        if (count > 0) atomicAdd(&A.stats[0], (unsigned long long)count * (unsigned long long)(vw * vh));
     }
 }
 
 void launch_bz_composite(hipStream_t st, const ViewParams& v, const BzCompositeArgs& a) {
     if (v.num_tiles == 0) return;
-    hipLaunchKernelGGL(k_bz_composite, dim3(v.num_tiles), dim3(256), 0, st, v, a);
+    hipLaunchKernelGGL(k_bz_forward, dim3(v.num_tiles), dim3(256), 0, st, v, a);
+    if (a.backward) hipLaunchKernelGGL(k_bz_reverse, dim3(v.num_tiles), dim3(256), 0, st, v, a);
 }
 
 // ------------------------------------------------------------------- step

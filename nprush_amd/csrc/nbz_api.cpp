@@ -43,7 +43,7 @@ struct nbz_ctx {
     const float *tgt = nullptr, *tdepth = nullptr, *bg = nullptr, *bgd = nullptr, *normal = nullptr;
     DevBuf pixel_loss;
     uint32_t pl_w = 0, pl_h = 0;
-    DevBuf fwd, bwd, ckpt, c_img, d_img, t_img, tile_loss, stats, pa, pd, pk;
+    DevBuf fwd, bwd, ckpt, c_img, d_img, t_img, tile_loss, stats, pa, pd, pk, act, nzm, seeds;
     Scratch sc;
     uint32_t last_tiles = 0;
     uint32_t img_w = 0, img_h = 0;
@@ -135,6 +135,9 @@ int bz_run(nbz_ctx* h, bool backward) {
         if ((rc = bz_ensure(h, h->pa, 4 * pslots)) != NPS_OK) return rc;
         if ((rc = bz_ensure(h, h->pd, 4 * pslots)) != NPS_OK) return rc;
         if ((rc = bz_ensure(h, h->pk, pslots)) != NPS_OK) return rc;
+        if ((rc = bz_ensure(h, h->act, 4 * t1 * 4 * (kBzCap / 32))) != NPS_OK) return rc;
+        if ((rc = bz_ensure(h, h->nzm, 4 * t1 * 4 * (kBzCap / 32))) != NPS_OK) return rc;
+        if ((rc = bz_ensure(h, h->seeds, 4 * 9 * t1 * kTilePx)) != NPS_OK) return rc;
     }
     if (h->timing) NBZ_HIP(h, hipEventRecord(h->ev[3], h->stream));
     BzCompositeArgs a;
@@ -158,6 +161,9 @@ int bz_run(nbz_ctx* h, bool backward) {
     a.pa = (float*)h->pa.p;
     a.pd = (float*)h->pd.p;
     a.pk = (uint8_t*)h->pk.p;
+    a.act = (uint32_t*)h->act.p;
+    a.nzm = (uint32_t*)h->nzm.p;
+    a.seeds = (float*)h->seeds.p;
     a.stats = (unsigned long long*)h->stats.p;
     a.prm = h->prm;
     a.cap = cap;
@@ -276,7 +282,7 @@ void nbz_destroy(nbz_handle h) {
         if (p) (void)hipFree(p);
     DevBuf* bs[] = {&h->own_tgt, &h->own_tdepth, &h->own_bg, &h->own_bgd, &h->own_normal, &h->pixel_loss,
                     &h->fwd, &h->bwd, &h->ckpt, &h->c_img, &h->d_img, &h->t_img, &h->tile_loss, &h->stats,
-                    &h->pa, &h->pd, &h->pk,
+                    &h->pa, &h->pd, &h->pk, &h->act, &h->nzm, &h->seeds,
                     &h->adc_dead, &h->adc_cand, &h->adc_stats};
     for (DevBuf* b : bs) free_buf(*b);
     free_scratch(h->sc);

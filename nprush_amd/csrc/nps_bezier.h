@@ -71,6 +71,9 @@ struct BzCompositeArgs {
     float* pa;                 // per (key, pixel of its tile): alpha, depth and segment of the forward,
     float* pd;                 //   written for entries where some lane of the wave has alpha > 0
     uint8_t* pk;
+    uint32_t* act;             // per tile, wave and 32 entries: some pixel has alpha >= 0.001
+    uint32_t* nzm;             // per tile, wave and 32 entries: pair values stored (some alpha > 0)
+    float* seeds;              // 9 planes of num_tiles * 256: dC rgb, dT, dD, is_bg, flow xy, has flow
     unsigned long long* stats; // [0] pairs, [2] reverse pairs, [3] reverse wave-entries, [4] forward and
                                // [5] reverse wave-entries reloaded from the forward's pair buffers
     BzParams prm;
