@@ -343,6 +343,34 @@ int nbz_read_stats(nbz_handle h, int64_t stats[16]);
 int nbz_enable_timing(nbz_handle h, int32_t on);
 int nbz_read_timing(nbz_handle h, float ms[8]);
 
+/* ------------------------------------------------------ densification MLP
+ * No reference counterpart (the reference's densification is heuristic only).
+ * MLP 32 -> 256 -> 256 -> 256 -> 14 with GELU (tanh form) proposing the two
+ * children of a densified parent as deltas (3 position, 3 log-scale, 1 opacity
+ * logit each) on top of the heuristic children.  BF16 matrix-core products with
+ * FP32 accumulation; FP32 master weights and Adam (beta 0.9 / 0.999, eps 1e-8).
+ * Parameters (nml_num_params() = 143,630 floats), layer by layer: W [out][in]
+ * row-major, then b [out].  Inputs are standardised with (in_mean, in_std) and
+ * outputs de-standardised with (out_mean, out_std) (nml_set_norm; default 0/1);
+ * the training loss is the MSE of the standardised outputs. */
+typedef struct nml_ctx* nml_handle;
+int nml_create(int32_t device, int32_t max_batch, nml_handle* out);
+void nml_destroy(nml_handle h);
+const char* nml_last_error(nml_handle h);
+int32_t nml_num_params(void);
+int nml_write_params(nml_handle h, const float* params);  /* also resets the Adam state */
+int nml_read_params(nml_handle h, float* params);
+int nml_set_norm(nml_handle h, const float* in_mean32, const float* in_std32, const float* out_mean14,
+                 const float* out_std14);
+/* x: n x 32 features, y: n x 14 deltas (host or device pointers) */
+int nml_forward(nml_handle h, const float* x, int32_t n, float* y);
+int nml_forward_dev(nml_handle h, const float* x_dev, int32_t n, float* y_dev);
+/* gradient of the standardised MSE over one batch (n <= max_batch), host pointers */
+int nml_grads(nml_handle h, const float* x, const float* t14, int32_t n, float* grads, double* loss);
+/* one Adam step on one batch (device or host pointers) */
+int nml_train_dev(nml_handle h, const float* x_dev, const float* t14_dev, int32_t n, float lr, double* loss);
+int nml_train(nml_handle h, const float* x, const float* t14, int32_t n, float lr, double* loss);
+
 /* Device buffers on the handle's GPU for callers without a GPU allocator of
  * their own (Deno, ctypes), e.g. to keep several targets resident in HBM for
  * nps_set_target_dev.  No reference counterpart (WebGPU's createBuffer). */

@@ -6,6 +6,7 @@ reference's GpuSplatOptimizerManager / GpuSplatForwardPipelineManager.
 """
 from . import camera, scenes  # noqa: F401
 from .managers import (  # noqa: F401
+    DensifierMLP,
     GpuBezierOptimizerManager,
     GpuBlurPipelineManager,
     GpuMultiViewStep,

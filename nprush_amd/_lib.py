@@ -100,6 +100,18 @@ SIGNATURES = {
     "nps_dev_free": (i32, [_P, _P]),
     "nps_dev_upload": (i32, [_P, _P, _P, C.c_int64]),
     "nps_debug_math": (i32, [i32, i32, _F, _F, _F, i32]),
+    "nml_create": (i32, [i32, i32, C.POINTER(_P)]),
+    "nml_destroy": (None, [_P]),
+    "nml_last_error": (C.c_char_p, [_P]),
+    "nml_num_params": (i32, []),
+    "nml_write_params": (i32, [_P, _F]),
+    "nml_read_params": (i32, [_P, _F]),
+    "nml_set_norm": (i32, [_P, _F, _F, _F, _F]),
+    "nml_forward": (i32, [_P, _F, i32, _F]),
+    "nml_forward_dev": (i32, [_P, _P, i32, _P]),
+    "nml_grads": (i32, [_P, _F, _F, i32, _F, _D]),
+    "nml_train_dev": (i32, [_P, _P, _P, i32, C.c_float, _D]),
+    "nml_train": (i32, [_P, _F, _F, i32, C.c_float, _D]),
     "nbz_create": (i32, [i32, i32, C.POINTER(_P)]),
     "nbz_destroy": (None, [_P]),
     "nbz_last_error": (C.c_char_p, [_P]),

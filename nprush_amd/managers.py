@@ -679,6 +679,89 @@ class GpuBezierOptimizerManager:
                 "step": float(ms[4]), "adc": float(ms[5]), "backward": float(ms[6])}
 
 
+class DensifierMLP:
+    """The densification MLP (nml_*): 32 parent features -> 14 child deltas,
+    BF16 matrix-core products with FP32 accumulation, trained with Adam."""
+
+    NUM_PARAMS = 143630
+
+    def __init__(self, max_batch: int = 8192, *, hip_device: int = 0, params: np.ndarray | None = None):
+        lib = _lib.load()
+        h = C.c_void_p()
+        rc = lib.nml_create(int(hip_device), int(max_batch), C.byref(h))
+        if rc != 0:
+            raise _lib.NpsError(f"nml_create failed ({rc})")
+        self._lib = lib
+        self._h = h
+        self.max_batch = int(max_batch)
+        if params is not None:
+            self.write_params(params)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            m = self._lib.nml_last_error(self._h)
+            raise _lib.NpsError(f"{what} failed ({rc}): {m.decode() if m else ''}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.nml_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def write_params(self, p: np.ndarray):
+        q = np.ascontiguousarray(p, dtype=np.float32).reshape(self.NUM_PARAMS)
+        self._check(self._lib.nml_write_params(self._h, fptr(q)), "nml_write_params")
+
+    def read_params(self) -> np.ndarray:
+        q = np.empty(self.NUM_PARAMS, np.float32)
+        self._check(self._lib.nml_read_params(self._h, fptr(q)), "nml_read_params")
+        return q
+
+    def set_norm(self, in_mean, in_std, out_mean, out_std):
+        a = [np.ascontiguousarray(v, dtype=np.float32).reshape(-1) for v in (in_mean, in_std, out_mean, out_std)]
+        self._check(self._lib.nml_set_norm(self._h, *[fptr(v) for v in a]), "nml_set_norm")
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 32)
+        y = np.empty((xx.shape[0], 14), np.float32)
+        self._check(self._lib.nml_forward(self._h, fptr(xx), xx.shape[0], fptr(y)), "nml_forward")
+        return y
+
+    def forward_dev(self, x_ptr: int, n: int, y_ptr: int):
+        self._check(self._lib.nml_forward_dev(self._h, C.c_void_p(x_ptr), int(n), C.c_void_p(y_ptr)),
+                    "nml_forward_dev")
+
+    def grads(self, x: np.ndarray, t: np.ndarray):
+        """(loss, gradient [143630]) of the standardised MSE over one batch."""
+        xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 32)
+        tt = np.ascontiguousarray(t, dtype=np.float32).reshape(-1, 14)
+        g = np.empty(self.NUM_PARAMS, np.float32)
+        loss = C.c_double()
+        self._check(self._lib.nml_grads(self._h, fptr(xx), fptr(tt), xx.shape[0], fptr(g), C.byref(loss)),
+                    "nml_grads")
+        return float(loss.value), g
+
+    def train(self, x: np.ndarray, t: np.ndarray, lr: float) -> float:
+        """One Adam step on one batch of host arrays; returns the batch loss."""
+        xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 32)
+        tt = np.ascontiguousarray(t, dtype=np.float32).reshape(-1, 14)
+        loss = C.c_double()
+        self._check(self._lib.nml_train(self._h, fptr(xx), fptr(tt), xx.shape[0], float(lr), C.byref(loss)),
+                    "nml_train")
+        return float(loss.value)
+
+    def train_dev(self, x_ptr: int, t_ptr: int, n: int, lr: float) -> float:
+        loss = C.c_double()
+        self._check(self._lib.nml_train_dev(self._h, C.c_void_p(x_ptr), C.c_void_p(t_ptr), int(n), float(lr),
+                                            C.byref(loss)), "nml_train_dev")
+        return float(loss.value)
+
+
 def debug_math(op: int, x: np.ndarray, y: np.ndarray | None = None, device: int = 0) -> np.ndarray:
     lib = _lib.load()
     x = np.ascontiguousarray(x, dtype=np.float32)
