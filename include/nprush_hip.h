@@ -151,6 +151,25 @@ int nps_adc(nps_handle h);
  * accumulators are reset.  stats[4..8] = qualifiers, children, splits, clones,
  * dead slots. */
 int nps_set_adc_ext(nps_handle h, float grad_threshold, float scene_extent, float prune_opacity, float percent_dense);
+/* Learned densification support (EXT mode, no reference counterpart), used
+ * around nps_adc by a caller that proposes the children with the MLP (nml_*):
+ * nps_adc_features - 32 features per parent the next nps_adc will densify, in
+ *   its placement order (first max_rows rows copied; *nq = qualifiers);
+ * nps_adc_children - (parent slot, dead slot) of each placement of the last
+ *   nps_adc (*count = min(qualifiers, dead slots));
+ * nps_adc_apply_deltas - adds 14 deltas per placement (child in the parent
+ *   slot: position xyz, log-scale xyz, opacity logit; then the child in the dead
+ *   slot) to the children of the last nps_adc;
+ * nps_snapshot - restore = 0 saves splats, SH, Adam m/v and accumulators on the
+ *   device, restore = 1 copies them back. */
+int nps_adc_features(nps_handle h, float* feats32, int32_t max_rows, int32_t* nq);
+int nps_adc_children(nps_handle h, int32_t* parents, int32_t* children, int32_t max_rows, int32_t* count);
+int nps_adc_apply_deltas(nps_handle h, const float* deltas14, int32_t rows);
+int nps_snapshot(nps_handle h, int32_t restore);
+/* the same on the device: the features stay in a handle buffer valid until the
+ * next nps_adc_features call; the deltas are a device array (e.g. nml_forward_dev's output) */
+int nps_adc_features_dev(nps_handle h, const float** feats32_dev, int32_t* nq);
+int nps_adc_apply_deltas_dev(nps_handle h, const float* deltas14_dev, int32_t rows);
 /* GpuSplatOptimizerManager.dispatch (OPT:365-388): backward, step, and ADC on
  * every adc_period-th call.  Returns NPS_ERR_STATE when no target is set
  * (the reference returns silently, OPT:366). */
@@ -364,7 +383,7 @@ int nml_set_norm(nml_handle h, const float* in_mean32, const float* in_std32, co
                  const float* out_std14);
 /* x: n x 32 features, y: n x 14 deltas (host or device pointers) */
 int nml_forward(nml_handle h, const float* x, int32_t n, float* y);
-int nml_forward_dev(nml_handle h, const float* x_dev, int32_t n, float* y_dev);
+int nml_forward_dev(nml_handle h, const float* x_dev, int32_t n, float* y_dev);  /* returns when y_dev is written */
 /* gradient of the standardised MSE over one batch (n <= max_batch), host pointers */
 int nml_grads(nml_handle h, const float* x, const float* t14, int32_t n, float* grads, double* loss);
 /* one Adam step on one batch (device or host pointers) */

@@ -100,6 +100,12 @@ SIGNATURES = {
     "nps_dev_free": (i32, [_P, _P]),
     "nps_dev_upload": (i32, [_P, _P, _P, C.c_int64]),
     "nps_debug_math": (i32, [i32, i32, _F, _F, _F, i32]),
+    "nps_adc_features": (i32, [_P, _F, i32, _I]),
+    "nps_adc_children": (i32, [_P, _I, _I, i32, _I]),
+    "nps_adc_apply_deltas": (i32, [_P, _F, i32]),
+    "nps_snapshot": (i32, [_P, i32]),
+    "nps_adc_features_dev": (i32, [_P, C.POINTER(_P), _I]),
+    "nps_adc_apply_deltas_dev": (i32, [_P, _P, i32]),
     "nml_create": (i32, [i32, i32, C.POINTER(_P)]),
     "nml_destroy": (None, [_P]),
     "nml_last_error": (C.c_char_p, [_P]),

@@ -953,6 +953,109 @@ void run_adc_ext(hipStream_t st, float* splats, float* sh, float* m, float* v, f
     launch_fill_u32(st, (uint32_t*)accum, 0u, 2 * (uint64_t)n);
 }
 
+// Features of the qualifying parents, row qual_rank[i] (the densification's
+// placement order): mean, world covariance (xx xy xz yy yz zz), opacity, DC
+// colour, mean |dL/d ndc|, view count, log-scales, unit quaternion, largest
+// scale, largest scale / split scale, |mean|, split flag, opacity logit,
+// summed |dL/d ndc|, distance to the current camera and its inverse, 0, 0.
+__global__ __launch_bounds__(256) void k_adc_ext_features(const float* __restrict__ splats,
+                                                          const float* __restrict__ sh,
+                                                          const float2* __restrict__ accum, uint32_t n,
+                                                          const uint32_t* __restrict__ qual,
+                                                          const uint32_t* __restrict__ qual_rank, float split_scale,
+                                                          float cx, float cy, float cz, float* __restrict__ feat) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !qual[i]) return;
+    const float* p = splats + 16 * (size_t)i;
+    float f[32];
+    f[0] = p[0];
+    f[1] = p[1];
+    f[2] = p[2];
+    float qq = ((p[8] * p[8] + p[9] * p[9]) + p[10] * p[10]) + p[11] * p[11];
+    float ql = sqrtf(qq > 1e-24f ? qq : 1e-24f);
+    const float qw = p[8] / ql, qx = p[9] / ql, qy = p[10] / ql, qz = p[11] / ql;
+    const float R[3][3] = {{1.0f - 2.0f * (qy * qy + qz * qz), 2.0f * (qx * qy - qw * qz), 2.0f * (qx * qz + qw * qy)},
+                           {2.0f * (qx * qy + qw * qz), 1.0f - 2.0f * (qx * qx + qz * qz), 2.0f * (qy * qz - qw * qx)},
+                           {2.0f * (qx * qz - qw * qy), 2.0f * (qy * qz + qw * qx), 1.0f - 2.0f * (qx * qx + qy * qy)}};
+    float s2[3], sc[3];
+    for (int a = 0; a < 3; ++a) {
+        sc[a] = __expf(p[4 + a]);
+        s2[a] = sc[a] * sc[a];
+    }
+    int idx = 3;
+    for (int r = 0; r < 3; ++r)
+        for (int c = r; c < 3; ++c) f[idx++] = (R[r][0] * R[c][0] * s2[0] + R[r][1] * R[c][1] * s2[1]) + R[r][2] * R[c][2] * s2[2];
+    f[9] = 1.0f / (1.0f + __expf(-p[3]));
+    for (int c = 0; c < 3; ++c) f[10 + c] = sh[48 * (size_t)i + c] * 0.28209479177387814f + 0.5f;
+    const float2 a = accum[i];
+    f[13] = a.y > 0.0f ? a.x / a.y : 0.0f;
+    f[14] = a.y;
+    for (int c = 0; c < 3; ++c) f[15 + c] = p[4 + c];
+    f[18] = qw;
+    f[19] = qx;
+    f[20] = qy;
+    f[21] = qz;
+    const float smax = fmaxf(fmaxf(sc[0], sc[1]), sc[2]);
+    f[22] = smax;
+    f[23] = smax / split_scale;
+    f[24] = sqrtf(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    f[25] = smax > split_scale ? 1.0f : 0.0f;
+    f[26] = p[3];
+    f[27] = a.x;
+    const float dx = p[0] - cx, dy = p[1] - cy, dz = p[2] - cz;
+    const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+    f[28] = dist;
+    f[29] = 1.0f / fmaxf(dist, 1e-6f);
+    f[30] = 0.0f;
+    f[31] = 0.0f;
+    float* o = feat + 32 * (size_t)qual_rank[i];
+    for (int c = 0; c < 32; ++c) o[c] = f[c];
+}
+
+void launch_adc_ext_features(hipStream_t st, const float* splats, const float* sh, const float* accum, uint32_t n,
+                             const AdcExtParams& prm, const float cam[3], const AdcExtBuffers& b, float* feat) {
+    launch_fill_u32(st, b.cnt, 0u, 4);
+    if (n == 0) return;
+    const dim3 g((n + 255) / 256), blk(256);
+    hipLaunchKernelGGL(k_adc_ext_mark, g, blk, 0, st, splats, (const float2*)accum, n, prm.threshold,
+                       prm.prune_opacity, b.dead, b.qual);
+    exclusive_scan_u32(st, b.qual, b.qual_rank, n, b.scan_tmp);
+    exclusive_scan_u32(st, b.dead, b.dead_rank, n, b.scan_tmp);
+    hipLaunchKernelGGL(k_adc_ext_lists, g, blk, 0, st, n, (const uint32_t*)b.dead, (const uint32_t*)b.dead_rank,
+                       (const uint32_t*)b.qual, (const uint32_t*)b.qual_rank, b.dead_list, b.qual_list, b.cnt);
+    if (feat)
+        hipLaunchKernelGGL(k_adc_ext_features, g, blk, 0, st, splats, sh, (const float2*)accum, n,
+                           (const uint32_t*)b.qual, (const uint32_t*)b.qual_rank, prm.percent_dense * prm.extent,
+                           cam[0], cam[1], cam[2], feat);
+}
+
+// Child k < min(qualifiers, dead) of the last densification: child 0 (parent
+// slot) and child 1 (dead slot) get position, log-scale and opacity-logit deltas.
+__global__ __launch_bounds__(256) void k_adc_ext_deltas(float* __restrict__ splats,
+                                                        const uint32_t* __restrict__ dead_list,
+                                                        const uint32_t* __restrict__ qual_list,
+                                                        const uint32_t* __restrict__ cnt,
+                                                        const float* __restrict__ delta, uint32_t nrows) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t P = cnt[0] < cnt[1] ? cnt[0] : cnt[1];
+    if (k >= P || k >= nrows) return;
+    const uint32_t slots[2] = {qual_list[k], dead_list[k]};
+    const float* dk = delta + 14 * (size_t)k;
+    for (int c = 0; c < 2; ++c) {
+        float* s = splats + 16 * (size_t)slots[c];
+        for (int j = 0; j < 3; ++j) s[j] += dk[7 * c + j];
+        for (int j = 0; j < 3; ++j) s[4 + j] += dk[7 * c + 3 + j];
+        s[3] += dk[7 * c + 6];
+    }
+}
+
+void launch_adc_ext_deltas(hipStream_t st, float* splats, const AdcExtBuffers& b, const float* delta, uint32_t nrows) {
+    if (nrows == 0) return;
+    hipLaunchKernelGGL(k_adc_ext_deltas, dim3((nrows + 255) / 256), dim3(256), 0, st, splats,
+                       (const uint32_t*)b.dead_list, (const uint32_t*)b.qual_list, (const uint32_t*)b.cnt, delta,
+                       nrows);
+}
+
 // out[0..1] = sums of the per-tile (L2, L1) pairs, in a fixed order (one workgroup).
 __global__ __launch_bounds__(256) void k_sum_tile_loss(const double* __restrict__ tl, uint32_t ntiles,
                                                        double* __restrict__ out) {

@@ -288,6 +288,8 @@ int nml_forward_dev(nml_handle h, const float* x_dev, int32_t n, float* y_dev) {
         launch_mlp_output(h->stream, h->y, h->norm + 64, h->norm + 64 + kMlpOut, nb, y_dev + (size_t)b * kMlpOut);
     }
     NML_HIP(h, hipGetLastError());
+    // the caller's next use of y_dev may be on another stream
+    NML_HIP(h, hipStreamSynchronize(h->stream));
     return NPS_OK;
 }
 

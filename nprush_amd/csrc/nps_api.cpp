@@ -92,6 +92,8 @@ struct nps_ctx {
     bool have_backward = false;
     // ADC buffers
     DevBuf adc_u32, adc_tmp, adc_scan;
+    DevBuf adc_feat, adc_delta, snap;  // learned densification: features, deltas, state snapshot
+    bool have_snap = false;
     int64_t stats_host[12] = {0};
     bool timing = false;
     hipEvent_t ev[8] = {nullptr};
@@ -449,6 +451,7 @@ void nps_destroy(nps_handle h) {
     free_buf(h->tgt_rgba); free_buf(h->tgt_depth); free_buf(h->fwd); free_buf(h->bwd); free_buf(h->ckpt);
     free_buf(h->c_img); free_buf(h->d_img); free_buf(h->tile_loss); free_buf(h->stats);
     free_buf(h->adc_u32); free_buf(h->adc_tmp); free_buf(h->adc_scan);
+    free_buf(h->adc_feat); free_buf(h->adc_delta); free_buf(h->snap);
     free_scratch(h->sc);
     for (int k = 0; k < 8; ++k)
         if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
@@ -718,23 +721,32 @@ int nps_step(nps_handle h) {
     return NPS_OK;
 }
 
-static int adc_ext(nps_handle h) {
-    NPS_HIP(h, hipSetDevice(h->device));
+// The EXT densification's lists in the handle's ADC buffers.  The rank arrays
+// hold n + 1 entries: the exclusive scan also writes the total at [n].
+static int adc_ext_buffers(nps_handle h, AdcExtBuffers& b) {
     const uint32_t n = h->n;
     const size_t nn = (size_t)std::max<uint32_t>(n, 1);
     int rc;
-    if ((rc = ensure(h, h->adc_u32, 4 * (6 * nn + 8))) != NPS_OK) return rc;
+    if ((rc = ensure(h, h->adc_u32, 4 * (6 * nn + 10))) != NPS_OK) return rc;
     if ((rc = ensure(h, h->adc_scan, scan_temp_bytes(n) + 64)) != NPS_OK) return rc;
     uint32_t* base = (uint32_t*)h->adc_u32.p;
-    AdcExtBuffers b;
     b.dead = base;
     b.qual = base + nn;
-    b.dead_rank = base + 2 * nn;
-    b.qual_rank = base + 3 * nn;
-    b.dead_list = base + 4 * nn;
-    b.qual_list = base + 5 * nn;
-    b.cnt = base + 6 * nn;
+    b.dead_rank = base + 2 * nn;      // n + 1
+    b.qual_rank = base + 3 * nn + 1;  // n + 1
+    b.dead_list = base + 4 * nn + 2;
+    b.qual_list = base + 5 * nn + 2;
+    b.cnt = base + 6 * nn + 2;
     b.scan_tmp = h->adc_scan.p;
+    return NPS_OK;
+}
+
+static int adc_ext(nps_handle h) {
+    NPS_HIP(h, hipSetDevice(h->device));
+    const uint32_t n = h->n;
+    int rc;
+    AdcExtBuffers b;
+    if ((rc = adc_ext_buffers(h, b)) != NPS_OK) return rc;
     if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[5], h->stream));
     run_adc_ext(h->stream, h->splats, h->sh, h->m, h->v, h->accum, n, h->adcx, h->adc_calls, b);
     if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[6], h->stream));
@@ -749,6 +761,117 @@ static int adc_ext(nps_handle h) {
     h->stats_host[7] = cnt[3];
     h->stats_host[8] = cnt[1];
     h->adc_calls++;
+    return NPS_OK;
+}
+
+int nps_adc_features(nps_handle h, float* feats, int32_t max_rows, int32_t* nq) {
+    if (!h || !nq || max_rows < 0) return NPS_ERR_ARG;
+    if (!h->ext) return set_err(h, NPS_ERR_STATE, "EXT mode only");
+    NPS_HIP(h, hipSetDevice(h->device));
+    int rc;
+    AdcExtBuffers b;
+    if ((rc = adc_ext_buffers(h, b)) != NPS_OK) return rc;
+    if ((rc = ensure(h, h->adc_feat, 128 * (size_t)std::max<uint32_t>(h->n, 1))) != NPS_OK) return rc;
+    float cam[3];
+    ext_camera_centre(h->vp_inv, cam);
+    launch_adc_ext_features(h->stream, h->splats, h->sh, h->accum, h->n, h->adcx, cam, b, (float*)h->adc_feat.p);
+    NPS_HIP(h, hipGetLastError());
+    uint32_t cnt[2] = {0, 0};
+    NPS_HIP(h, hipMemcpyAsync(cnt, b.cnt, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
+    NPS_HIP(h, hipStreamSynchronize(h->stream));
+    *nq = (int32_t)cnt[0];
+    const size_t rows = std::min<size_t>(cnt[0], (size_t)max_rows);
+    if (feats && rows) {
+        NPS_HIP(h, hipMemcpyAsync(feats, h->adc_feat.p, 128 * rows, hipMemcpyDeviceToHost, h->stream));
+        NPS_HIP(h, hipStreamSynchronize(h->stream));
+    }
+    return NPS_OK;
+}
+
+int nps_adc_features_dev(nps_handle h, const float** feats_dev, int32_t* nq) {
+    if (!h || !feats_dev || !nq) return NPS_ERR_ARG;
+    int rc = nps_adc_features(h, nullptr, 0, nq);
+    if (rc != NPS_OK) return rc;
+    *feats_dev = (const float*)h->adc_feat.p;
+    return NPS_OK;
+}
+
+int nps_adc_apply_deltas_dev(nps_handle h, const float* deltas_dev, int32_t rows) {
+    if (!h || rows < 0 || (rows > 0 && !deltas_dev)) return NPS_ERR_ARG;
+    if (!h->ext) return set_err(h, NPS_ERR_STATE, "EXT mode only");
+    if (!h->adc_u32.p) return set_err(h, NPS_ERR_STATE, "no densification yet");
+    if (rows == 0) return NPS_OK;
+    NPS_HIP(h, hipSetDevice(h->device));
+    AdcExtBuffers b;
+    int rc;
+    if ((rc = adc_ext_buffers(h, b)) != NPS_OK) return rc;
+    launch_adc_ext_deltas(h->stream, h->splats, b, deltas_dev, (uint32_t)rows);
+    NPS_HIP(h, hipGetLastError());
+    NPS_HIP(h, hipStreamSynchronize(h->stream));
+    return NPS_OK;
+}
+
+int nps_adc_children(nps_handle h, int32_t* parents, int32_t* children, int32_t max_rows, int32_t* count) {
+    if (!h || !count || max_rows < 0) return NPS_ERR_ARG;
+    if (!h->ext) return set_err(h, NPS_ERR_STATE, "EXT mode only");
+    if (!h->adc_u32.p) return set_err(h, NPS_ERR_STATE, "no densification yet");
+    NPS_HIP(h, hipSetDevice(h->device));
+    AdcExtBuffers b;
+    int rc;
+    if ((rc = adc_ext_buffers(h, b)) != NPS_OK) return rc;
+    uint32_t cnt[2] = {0, 0};
+    NPS_HIP(h, hipMemcpyAsync(cnt, b.cnt, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
+    NPS_HIP(h, hipStreamSynchronize(h->stream));
+    const uint32_t P = std::min(cnt[0], cnt[1]);
+    *count = (int32_t)P;
+    const size_t rows = std::min<size_t>(P, (size_t)max_rows);
+    if (rows && parents) NPS_HIP(h, hipMemcpyAsync(parents, b.qual_list, 4 * rows, hipMemcpyDeviceToHost, h->stream));
+    if (rows && children) NPS_HIP(h, hipMemcpyAsync(children, b.dead_list, 4 * rows, hipMemcpyDeviceToHost, h->stream));
+    NPS_HIP(h, hipStreamSynchronize(h->stream));
+    return NPS_OK;
+}
+
+int nps_adc_apply_deltas(nps_handle h, const float* deltas, int32_t rows) {
+    if (!h || rows < 0 || (rows > 0 && !deltas)) return NPS_ERR_ARG;
+    if (!h->ext) return set_err(h, NPS_ERR_STATE, "EXT mode only");
+    if (!h->adc_u32.p) return set_err(h, NPS_ERR_STATE, "no densification yet");
+    if (rows == 0) return NPS_OK;
+    NPS_HIP(h, hipSetDevice(h->device));
+    AdcExtBuffers b;
+    int rc;
+    if ((rc = adc_ext_buffers(h, b)) != NPS_OK) return rc;
+    if ((rc = ensure(h, h->adc_delta, 56 * (size_t)rows)) != NPS_OK) return rc;
+    NPS_HIP(h, hipMemcpyAsync(h->adc_delta.p, deltas, 56 * (size_t)rows, hipMemcpyHostToDevice, h->stream));
+    launch_adc_ext_deltas(h->stream, h->splats, b, (const float*)h->adc_delta.p, (uint32_t)rows);
+    NPS_HIP(h, hipGetLastError());
+    NPS_HIP(h, hipStreamSynchronize(h->stream));
+    return NPS_OK;
+}
+
+int nps_snapshot(nps_handle h, int32_t restore) {
+    if (!h) return NPS_ERR_ARG;
+    if (!h->ext) return set_err(h, NPS_ERR_STATE, "EXT mode only");
+    NPS_HIP(h, hipSetDevice(h->device));
+    const size_t n = h->n;
+    const size_t sz[5] = {64 * n, 192 * n, 256 * n, 256 * n, 8 * n};
+    void* live[5] = {h->splats, h->sh, h->m, h->v, h->accum};
+    size_t tot = 0;
+    for (size_t s : sz) tot += s;
+    if (restore) {
+        if (h->snap.bytes < tot || !h->have_snap) return set_err(h, NPS_ERR_STATE, "no snapshot");
+    } else {
+        int rc;
+        if ((rc = ensure(h, h->snap, tot)) != NPS_OK) return rc;
+    }
+    size_t off = 0;
+    for (int k = 0; k < 5; ++k) {
+        char* s = (char*)h->snap.p + off;
+        if (restore) NPS_HIP(h, hipMemcpyAsync(live[k], s, sz[k], hipMemcpyDeviceToDevice, h->stream));
+        else NPS_HIP(h, hipMemcpyAsync(s, live[k], sz[k], hipMemcpyDeviceToDevice, h->stream));
+        off += sz[k];
+    }
+    NPS_HIP(h, hipStreamSynchronize(h->stream));
+    if (!restore) h->have_snap = true;
     return NPS_OK;
 }
 

@@ -148,6 +148,12 @@ struct AdcExtBuffers {
 };
 void run_adc_ext(hipStream_t st, float* splats, float* sh, float* m, float* v, float* accum, uint32_t n,
                  const AdcExtParams& prm, uint32_t seed, const AdcExtBuffers& b);
+// qualifier/dead lists as run_adc_ext builds them, and (feat != null) 32 features
+// per qualifier in placement order
+void launch_adc_ext_features(hipStream_t st, const float* splats, const float* sh, const float* accum, uint32_t n,
+                             const AdcExtParams& prm, const float cam[3], const AdcExtBuffers& b, float* feat);
+// adds 14 deltas per placement k < nrows to the two children of the last densification
+void launch_adc_ext_deltas(hipStream_t st, float* splats, const AdcExtBuffers& b, const float* delta, uint32_t nrows);
 
 // k_blur.hip: both passes; temp is an H*W float4 scratch image
 void launch_blur(hipStream_t st, const float4* src, float4* dst, float4* temp, int W, int H, int radius, float sigma,

@@ -228,6 +228,60 @@ class GpuSplatOptimizerManager:
     def adc(self):
         check(self._lib.nps_adc(self._h), self._h, "adc")
 
+    # ---- learned densification support (EXT; no reference counterpart) ----
+    def adc_features(self) -> np.ndarray:
+        """32 features per parent the next adc() densifies, in its placement order."""
+        nq = C.c_int32()
+        check(self._lib.nps_adc_features(self._h, None, 0, C.byref(nq)), self._h, "adc_features")
+        f = np.empty((max(nq.value, 1), 32), np.float32)
+        check(self._lib.nps_adc_features(self._h, fptr(f), nq.value, C.byref(nq)), self._h, "adc_features")
+        return f[:nq.value]
+
+    def adc_children(self):
+        """(parent slots, dead slots) of the last adc()'s placements."""
+        cnt = C.c_int32()
+        check(self._lib.nps_adc_children(self._h, None, None, 0, C.byref(cnt)), self._h, "adc_children")
+        p = np.empty(max(cnt.value, 1), np.int32)
+        c = np.empty(max(cnt.value, 1), np.int32)
+        check(self._lib.nps_adc_children(self._h, iptr(p), iptr(c), cnt.value, C.byref(cnt)), self._h,
+              "adc_children")
+        return p[:cnt.value], c[:cnt.value]
+
+    def adc_apply_deltas(self, deltas: np.ndarray):
+        d = np.ascontiguousarray(deltas, dtype=np.float32).reshape(-1, 14)
+        check(self._lib.nps_adc_apply_deltas(self._h, fptr(d), d.shape[0]), self._h, "adc_apply_deltas")
+
+    def adc_features_dev(self):
+        """(device pointer, rows) of the features adc_features() would return."""
+        p = C.c_void_p()
+        nq = C.c_int32()
+        check(self._lib.nps_adc_features_dev(self._h, C.byref(p), C.byref(nq)), self._h, "adc_features_dev")
+        return p.value, nq.value
+
+    def adc_apply_deltas_dev(self, ptr: int, rows: int):
+        check(self._lib.nps_adc_apply_deltas_dev(self._h, C.c_void_p(ptr), int(rows)), self._h,
+              "adc_apply_deltas_dev")
+
+    def dev_alloc(self, nbytes: int) -> int:
+        """A device buffer on the handle's GPU (nps_dev_alloc); free with dev_free."""
+        p = C.c_void_p()
+        check(self._lib.nps_dev_alloc(self._h, int(nbytes), C.byref(p)), self._h, "nps_dev_alloc")
+        return p.value
+
+    def dev_free(self, ptr: int):
+        self._lib.nps_dev_free(self._h, C.c_void_p(ptr))
+
+    def dev_upload(self, ptr: int, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        check(self._lib.nps_dev_upload(self._h, C.c_void_p(ptr), a.ctypes.data_as(C.c_void_p), a.nbytes), self._h,
+              "nps_dev_upload")
+
+    def snapshot(self):
+        check(self._lib.nps_snapshot(self._h, 0), self._h, "snapshot")
+
+    def restore(self):
+        check(self._lib.nps_snapshot(self._h, 1), self._h, "restore")
+
     def set_adc_ext(self, grad_threshold: float = 2e-4, scene_extent: float = 1.0, prune_opacity: float = 0.005,
                     percent_dense: float = 0.01):
         """EXT densification parameters (see nps_set_adc_ext)."""
