@@ -241,9 +241,7 @@ def bezier_config3(steps=20, warmup=3, cpu_tiles=16, n=100_000, W=1024, H=1024, 
     log-normal(ln 0.004, 0.3), softness width / 4, opacity sigmoid(N(1, 0.5)),
     RGB uniform), 4 ring views at 1024 x 1024, colour mode, targets = the
     forward image and depth of a seed-1 curve set, black background at depth 1.
-    Targets stay in HBM (torch tensors) for the whole run."""
-    import torch
-
+    Targets stay in HBM (device buffers of the handle) for the whole run."""
     from nprush_amd import GpuBezierOptimizerManager, camera, scenes
 
     if small:
@@ -253,24 +251,29 @@ def bezier_config3(steps=20, warmup=3, cpu_tiles=16, n=100_000, W=1024, H=1024, 
     tcurves = scenes.make_bezier_curves(n, 1)
     m = GpuBezierOptimizerManager(numBeziers=n)
     m.writeMode(1.0)
-    dev = torch.device("cuda:0")
-    zeros4 = torch.zeros((H, W, 4), dtype=torch.float32, device=dev)
-    ones1 = torch.ones((H, W), dtype=torch.float32, device=dev)
-    torch.cuda.synchronize()
-    p = lambda t: t.data_ptr()  # noqa: E731
+    bufs = []
+
+    def resident(a):
+        ptr = m.dev_alloc(a.nbytes)
+        m.dev_upload(ptr, a)
+        bufs.append(ptr)
+        return ptr
+    zeros4 = resident(np.zeros((H, W, 4), np.float32))
+    ones1 = resident(np.ones((H, W), np.float32))
     m.write_beziers(tcurves)
-    targets = []
+    targets, host_targets = [], []
     for vp, vi in cams:
         m.writeVPMatrix(vp)
         m.writeVPInvMatrix(vi)
-        m.set_backward_target_dev(p(zeros4), p(ones1), p(zeros4), p(ones1), p(zeros4), W, H)
+        m.set_backward_target_dev(zeros4, ones1, zeros4, ones1, zeros4, W, H)
         m.forward()
         C, D, _ = m.read_image()
         t = np.zeros((H, W, 4), np.float32)
         t[..., :3] = C
         t[..., 3] = 1.0
-        targets.append((torch.from_numpy(t).to(dev), torch.from_numpy(np.ascontiguousarray(D)).to(dev)))
-    torch.cuda.synchronize()
+        D = np.ascontiguousarray(D, np.float32)
+        host_targets.append((t, D))
+        targets.append((resident(t), resident(D)))
     m.write_beziers(curves)
     m.enable_timing(True)
 
@@ -279,7 +282,7 @@ def bezier_config3(steps=20, warmup=3, cpu_tiles=16, n=100_000, W=1024, H=1024, 
         tg, td = targets[k % views]
         m.writeVPMatrix(vp)
         m.writeVPInvMatrix(vi)
-        m.set_backward_target_dev(p(tg), p(td), p(zeros4), p(ones1), p(zeros4), W, H)
+        m.set_backward_target_dev(tg, td, zeros4, ones1, zeros4, W, H)
         m.dispatch()
         return m.read_timing(), m.read_stats()
 
@@ -297,7 +300,7 @@ def bezier_config3(steps=20, warmup=3, cpu_tiles=16, n=100_000, W=1024, H=1024, 
     tg, td = targets[0]
     m.writeVPMatrix(vp)
     m.writeVPInvMatrix(vi)
-    m.set_backward_target_dev(p(tg), p(td), p(zeros4), p(ones1), p(zeros4), W, H)
+    m.set_backward_target_dev(tg, td, zeros4, ones1, zeros4, W, H)
     fwd_ms = []
     for _ in range(3):
         m.forward()
@@ -318,15 +321,24 @@ def bezier_config3(steps=20, warmup=3, cpu_tiles=16, n=100_000, W=1024, H=1024, 
     achieved = flops / (comp * 1e-3) / 1e12
     # Adam: 18 parameters x (grad i32, m, v) + 20-float curve + grad_accum, read and written
     adam_gbs = (18 * 12 + 80 + 4) * 2.0 * n / (step * 1e-3) / 1e9
+    # HBM bytes per view of forward + reverse from the committed FETCH_SIZE /
+    # WRITE_SIZE runs of this workload (calibrated against a streaming kernel)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath) and not small:
+        with open(tpath) as fh:
+            tj = json.load(fh).get("k_bz_composite")
+        if tj:
+            traffic = int(tj["fetch_bytes"] + tj["write_bytes"])
     cpu = None
     if cpu_tiles > 0:
         from oracle import oracle
 
         oracle.load()
         vp, vi = cams[0]
-        tg, td = targets[0]
+        tg, td = host_targets[0]
         uni = oracle.bz_uniforms(vp, vi, mode=1.0)
-        tex = (tg.cpu().numpy(), td.cpu().numpy(), np.zeros((H, W, 4), np.float32), np.ones((H, W), np.float32),
+        tex = (tg, td, np.zeros((H, W, 4), np.float32), np.ones((H, W), np.float32),
                np.zeros((H, W, 4), np.float32))
         ntiles = ((W + 15) // 16) * ((H + 15) // 16)
         rng = np.random.default_rng(0)
@@ -339,6 +351,8 @@ def bezier_config3(steps=20, warmup=3, cpu_tiles=16, n=100_000, W=1024, H=1024, 
                "cores": int(oracle.load().ora_num_threads()), "kind": "port",
                "sample": f"{len(tiles)} random tiles of view 0 ({dt:.1f} s): binning + fwd + loss + bwd, "
                          f"scaled by {ntiles}/{len(tiles)} tiles"}
+    for ptr in bufs:
+        m.dev_free(ptr)
     m.close()
     return {
         "workload": f"Bezier strokes: {n} curves, {views} ring views at {W}x{H}, colour mode, dispatch = "
@@ -357,8 +371,9 @@ def bezier_config3(steps=20, warmup=3, cpu_tiles=16, n=100_000, W=1024, H=1024, 
         "forward_only_composite_ms": round(float(np.median(fwd_ms)), 3),
         "reverse_pairs_per_view": int(rev_pairs),
         "roofline": {"bound": "mfma", "compute": "fp32 VALU (no matrix work on this path)",
-                     "kernel": "k_bz_composite", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None},
+                     "kernel": "k_bz_forward + k_bz_reverse", "achieved": round(achieved, 3),
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": traffic},
         "adam_hbm_gbs": round(adam_gbs, 1),
         "adc_stats": {k: st_adc[k] for k in ("adc_dead", "adc_killed", "adc_split1", "adc_spawned", "adc_split3")},
         "cpu_baseline": cpu,
@@ -442,15 +457,17 @@ def learned_config4(small=False, events=3, warm=100, window=200, eval_steps=800,
                 yp = m.dev_alloc(56 * nx)
                 m.dev_upload(xp, X[:nx].astype(np.float32))
                 net.forward_dev(xp, nx, yp)
-                reps = 5
-                c0 = time.perf_counter()
-                for _ in range(reps):
+                net.enable_timing(True)
+                ms = []
+                for _ in range(5):
                     net.forward_dev(xp, nx, yp)
-                dt = (time.perf_counter() - c0) / reps
+                    ms.append(net.read_timing()["forward"])
+                net.enable_timing(False)
+                dt = float(np.mean(ms)) * 1e-3
                 tf = nx * MLP_FWD_FLOP / dt / 1e12
                 mlp_fwd = {"samples": nx, "ms": round(dt * 1e3, 3),
                            "roofline": {"bound": "mfma", "compute": "bf16 matrix cores, fp32 accumulation",
-                                        "kernel": "k_gemm_nt (4 layers + epilogues)", "achieved": round(tf, 2),
+                                        "kernel": "k_mlp_infer", "achieved": round(tf, 2),
                                         "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
                                         "frac": round(tf / PEAK_BF16_TFLOPS, 4), "traffic": None}}
                 m.dev_free(xp)
@@ -482,6 +499,75 @@ def learned_config4(small=False, events=3, warm=100, window=200, eval_steps=800,
         "time_to_target_ratio": None if t_learned is None else round(t_learned / t_plain, 3),
         "final_l1_plain_learned": [round(res["plain"][-1][1], 1), round(res["learned"][-1][1], 1)],
     }
+
+
+def mlp_bench(n=757_052, steps=20, warmup=3, batch=4096, train_steps=20):
+    """The densification MLP alone: inference over n synthetic parent-feature rows
+    (the pair count of the learned-densification recording) resident in HBM, and
+    Adam training steps at the recording's batch size; HIP-event times."""
+    from nprush_amd import DensifierMLP, densify
+
+    rng = np.random.default_rng(0)
+    x = rng.normal(0.0, 1.0, (n, 32)).astype(np.float32)
+    t = rng.normal(0.0, 1.0, (batch, 14)).astype(np.float32)
+    net = DensifierMLP(max_batch=batch, params=densify.init_params(0, out_scale=0.5))
+    xp = net.dev_alloc(x.nbytes)
+    yp = net.dev_alloc(56 * n)
+    tp = net.dev_alloc(t.nbytes)
+    net.dev_upload(xp, x)
+    net.dev_upload(tp, t)
+    net.enable_timing(True)
+    for _ in range(warmup):
+        net.forward_dev(xp, n, yp)
+    fwd = []
+    c0 = time.perf_counter()
+    for _ in range(steps):
+        net.forward_dev(xp, n, yp)
+        fwd.append(net.read_timing()["forward"])
+    wall = time.perf_counter() - c0
+    for _ in range(warmup):
+        net.train_dev(xp, tp, batch, 1e-4)
+    tr = []
+    for _ in range(train_steps):
+        net.train_dev(xp, tp, batch, 1e-4)
+        tr.append(net.read_timing()["train_step"])
+    for p_ in (xp, yp, tp):
+        net.dev_free(p_)
+    net.close()
+    fwd_ms = float(np.mean(fwd))
+    tr_ms = float(np.mean(tr))
+    # HBM bytes per inference launch from the committed FETCH_SIZE / WRITE_SIZE runs
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tj = json.load(fh).get("k_mlp_infer")
+        if tj and tj.get("rows") == n:
+            traffic = int(tj["fetch_bytes"] + tj["write_bytes"])
+    tf = n * MLP_FWD_FLOP / (fwd_ms * 1e-3) / 1e12
+    tf_train = batch * 3 * MLP_FWD_FLOP / (tr_ms * 1e-3) / 1e12
+    return {
+        "workload": f"densification MLP 32-256-256-256-14 GELU, bf16 x bf16 -> fp32: inference over {n} rows, "
+                    f"Adam training steps at batch {batch}",
+        "rows": n, "forward_ms": round(fwd_ms, 4), "forward_wall_ms": round(wall * 1e3 / steps, 4),
+        "rows_per_s": round(n / (fwd_ms * 1e-3), 1),
+        "train_step_ms": round(tr_ms, 4), "train_batch": batch, "train_tflops": round(tf_train, 2),
+        "roofline": {"bound": "mfma", "compute": "bf16 matrix cores, fp32 accumulation", "kernel": "k_mlp_infer",
+                     "achieved": round(tf, 2), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(tf / PEAK_BF16_TFLOPS, 4), "traffic": traffic},
+    }
+
+
+def main_mlp(args):
+    print("[bench] densification MLP ...", file=sys.stderr, flush=True)
+    r = mlp_bench(steps=args.steps, warmup=args.warmup, n=40_000 if args.small else 757_052)
+    out = {
+        "metric": "MLP inference rows/s", "value": r["rows_per_s"], "unit": "rows/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["forward_ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (fp32 accumulation)", "data": "synthetic",
+        "config": {"workload": r["workload"]}, "roofline": r["roofline"], "cpu_baseline": None, "mlp": r,
+    }
+    print(json.dumps(out), flush=True)
 
 
 def main_learned(args):
@@ -538,7 +624,7 @@ def main():
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--small", action="store_true", help="16,384 splats at 256x256 (quick check)")
-    ap.add_argument("--mode", choices=("ext", "ref", "bezier", "learned"), default="ext")
+    ap.add_argument("--mode", choices=("ext", "ref", "bezier", "learned", "mlp"), default="ext")
     ap.add_argument("--no-batched", action="store_true", help="skip the multi-view (64 views, 4 scenes) comparison")
     ap.add_argument("--no-bezier", action="store_true", help="skip the Bezier stroke workload")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-densification comparison")
@@ -558,6 +644,8 @@ def main():
         return main_bezier(args)
     if args.mode == "learned":
         return main_learned(args)
+    if args.mode == "mlp":
+        return main_mlp(args)
 
     from nprush_amd import GpuSplatOptimizerManager
 
@@ -740,6 +828,7 @@ def main_ext(args):
     if not args.small and not args.no_learned:
         print("[bench] learned densification vs plain ...", file=sys.stderr, flush=True)
         learned = learned_config4()
+        learned["mlp_alone"] = mlp_bench()
     cpu = None
     if args.cpu_tiles > 0:
         print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)

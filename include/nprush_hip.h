@@ -154,7 +154,12 @@ int nps_set_adc_ext(nps_handle h, float grad_threshold, float scene_extent, floa
 /* Learned densification support (EXT mode, no reference counterpart), used
  * around nps_adc by a caller that proposes the children with the MLP (nml_*):
  * nps_adc_features - 32 features per parent the next nps_adc will densify, in
- *   its placement order (first max_rows rows copied; *nq = qualifiers);
+ *   its placement order (first max_rows rows copied; *nq = qualifiers): mean,
+ *   world covariance (xx xy xz yy yz zz), opacity, DC colour, mean and summed
+ *   |dL/d ndc| and view count, log-scales, largest scale, largest scale / split
+ *   scale, split flag, opacity logit, distance to the current camera and its
+ *   inverse, and m / (sqrt(v) + 1e-12) of the mean, log-scales and opacity
+ *   logit (the direction of the parent's next Adam steps);
  * nps_adc_children - (parent slot, dead slot) of each placement of the last
  *   nps_adc (*count = min(qualifiers, dead slots));
  * nps_adc_apply_deltas - adds 14 deltas per placement (child in the parent
@@ -361,6 +366,12 @@ int nbz_read_stats(nbz_handle h, int64_t stats[16]);
  * [5] adc, [6] backward total of the last calls */
 int nbz_enable_timing(nbz_handle h, int32_t on);
 int nbz_read_timing(nbz_handle h, float ms[8]);
+/* Device buffers on the handle's GPU (targets kept resident for
+ * nbz_set_backward_target_dev), as nps_dev_alloc / nps_dev_free /
+ * nps_dev_upload.  No reference counterpart (WebGPU's createBuffer). */
+int nbz_dev_alloc(nbz_handle h, int64_t bytes, void** out);
+int nbz_dev_free(nbz_handle h, void* p);
+int nbz_dev_upload(nbz_handle h, void* dst, const void* src, int64_t bytes);
 
 /* ------------------------------------------------------ densification MLP
  * No reference counterpart (the reference's densification is heuristic only).
@@ -389,6 +400,14 @@ int nml_grads(nml_handle h, const float* x, const float* t14, int32_t n, float* 
 /* one Adam step on one batch (device or host pointers) */
 int nml_train_dev(nml_handle h, const float* x_dev, const float* t14_dev, int32_t n, float lr, double* loss);
 int nml_train(nml_handle h, const float* x, const float* t14, int32_t n, float lr, double* loss);
+/* ms (HIP events on the handle's stream) of [0] the last nml_forward_dev launch,
+ * [1] the last training step (input through Adam), while timing is on */
+int nml_enable_timing(nml_handle h, int32_t on);
+int nml_read_timing(nml_handle h, float ms[2]);
+/* device buffers on the handle's GPU, as nps_dev_alloc / nps_dev_free / nps_dev_upload */
+int nml_dev_alloc(nml_handle h, int64_t bytes, void** out);
+int nml_dev_free(nml_handle h, void* p);
+int nml_dev_upload(nml_handle h, void* dst, const void* src, int64_t bytes);
 
 /* Device buffers on the handle's GPU for callers without a GPU allocator of
  * their own (Deno, ctypes), e.g. to keep several targets resident in HBM for

@@ -118,6 +118,11 @@ SIGNATURES = {
     "nml_grads": (i32, [_P, _F, _F, i32, _F, _D]),
     "nml_train_dev": (i32, [_P, _P, _P, i32, C.c_float, _D]),
     "nml_train": (i32, [_P, _F, _F, i32, C.c_float, _D]),
+    "nml_enable_timing": (i32, [_P, i32]),
+    "nml_read_timing": (i32, [_P, _F]),
+    "nml_dev_alloc": (i32, [_P, C.c_int64, C.POINTER(_P)]),
+    "nml_dev_free": (i32, [_P, _P]),
+    "nml_dev_upload": (i32, [_P, _P, _P, C.c_int64]),
     "nbz_create": (i32, [i32, i32, C.POINTER(_P)]),
     "nbz_destroy": (None, [_P]),
     "nbz_last_error": (C.c_char_p, [_P]),
@@ -151,6 +156,9 @@ SIGNATURES = {
     "nbz_read_loss": (i32, [_P, _D]),
     "nbz_read_tiles": (i32, [_P, _U, _U, _I, _L]),
     "nbz_read_stats": (i32, [_P, _L]),
+    "nbz_dev_alloc": (i32, [_P, C.c_int64, C.POINTER(_P)]),
+    "nbz_dev_free": (i32, [_P, _P]),
+    "nbz_dev_upload": (i32, [_P, _P, _P, C.c_int64]),
     "nbz_enable_timing": (i32, [_P, i32]),
     "nbz_read_timing": (i32, [_P, _F]),
 }

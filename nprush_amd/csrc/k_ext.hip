@@ -954,14 +954,18 @@ void run_adc_ext(hipStream_t st, float* splats, float* sh, float* m, float* v, f
 }
 
 // Features of the qualifying parents, row qual_rank[i] (the densification's
-// placement order): mean, world covariance (xx xy xz yy yz zz), opacity, DC
-// colour, mean |dL/d ndc|, view count, log-scales, unit quaternion, largest
-// scale, largest scale / split scale, |mean|, split flag, opacity logit,
-// summed |dL/d ndc|, distance to the current camera and its inverse, 0, 0.
+// placement order): mean (0-2), world covariance xx xy xz yy yz zz (3-8),
+// opacity (9), DC colour (10-12), mean |dL/d ndc| (13), view count (14),
+// log-scales (15-17), largest scale (18), largest scale / split scale (19),
+// split flag (20), opacity logit (21), summed |dL/d ndc| (22), distance to the
+// current camera and its inverse (23, 24), and the direction of the parent's
+// next Adam steps, m / (sqrt(v) + 1e-12), for the mean (25-27), the log-scales
+// (28-30) and the opacity logit (31).
 __global__ __launch_bounds__(256) void k_adc_ext_features(const float* __restrict__ splats,
                                                           const float* __restrict__ sh,
-                                                          const float2* __restrict__ accum, uint32_t n,
-                                                          const uint32_t* __restrict__ qual,
+                                                          const float2* __restrict__ accum,
+                                                          const float* __restrict__ m, const float* __restrict__ v,
+                                                          uint32_t n, const uint32_t* __restrict__ qual,
                                                           const uint32_t* __restrict__ qual_rank, float split_scale,
                                                           float cx, float cy, float cz, float* __restrict__ feat) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -991,29 +995,28 @@ __global__ __launch_bounds__(256) void k_adc_ext_features(const float* __restric
     f[13] = a.y > 0.0f ? a.x / a.y : 0.0f;
     f[14] = a.y;
     for (int c = 0; c < 3; ++c) f[15 + c] = p[4 + c];
-    f[18] = qw;
-    f[19] = qx;
-    f[20] = qy;
-    f[21] = qz;
     const float smax = fmaxf(fmaxf(sc[0], sc[1]), sc[2]);
-    f[22] = smax;
-    f[23] = smax / split_scale;
-    f[24] = sqrtf(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
-    f[25] = smax > split_scale ? 1.0f : 0.0f;
-    f[26] = p[3];
-    f[27] = a.x;
+    f[18] = smax;
+    f[19] = smax / split_scale;
+    f[20] = smax > split_scale ? 1.0f : 0.0f;
+    f[21] = p[3];
+    f[22] = a.x;
     const float dx = p[0] - cx, dy = p[1] - cy, dz = p[2] - cz;
     const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
-    f[28] = dist;
-    f[29] = 1.0f / fmaxf(dist, 1e-6f);
-    f[30] = 0.0f;
-    f[31] = 0.0f;
+    f[23] = dist;
+    f[24] = 1.0f / fmaxf(dist, 1e-6f);
+    const int cols[7] = {0, 1, 2, 4, 5, 6, 3};
+    for (int c = 0; c < 7; ++c) {
+        const size_t k = 64 * (size_t)i + cols[c];
+        f[25 + c] = m[k] / (sqrtf(v[k]) + 1e-12f);
+    }
     float* o = feat + 32 * (size_t)qual_rank[i];
     for (int c = 0; c < 32; ++c) o[c] = f[c];
 }
 
-void launch_adc_ext_features(hipStream_t st, const float* splats, const float* sh, const float* accum, uint32_t n,
-                             const AdcExtParams& prm, const float cam[3], const AdcExtBuffers& b, float* feat) {
+void launch_adc_ext_features(hipStream_t st, const float* splats, const float* sh, const float* accum,
+                             const float* m, const float* v, uint32_t n, const AdcExtParams& prm, const float cam[3],
+                             const AdcExtBuffers& b, float* feat) {
     launch_fill_u32(st, b.cnt, 0u, 4);
     if (n == 0) return;
     const dim3 g((n + 255) / 256), blk(256);
@@ -1024,7 +1027,7 @@ void launch_adc_ext_features(hipStream_t st, const float* splats, const float* s
     hipLaunchKernelGGL(k_adc_ext_lists, g, blk, 0, st, n, (const uint32_t*)b.dead, (const uint32_t*)b.dead_rank,
                        (const uint32_t*)b.qual, (const uint32_t*)b.qual_rank, b.dead_list, b.qual_list, b.cnt);
     if (feat)
-        hipLaunchKernelGGL(k_adc_ext_features, g, blk, 0, st, splats, sh, (const float2*)accum, n,
+        hipLaunchKernelGGL(k_adc_ext_features, g, blk, 0, st, splats, sh, (const float2*)accum, m, v, n,
                            (const uint32_t*)b.qual, (const uint32_t*)b.qual_rank, prm.percent_dense * prm.extent,
                            cam[0], cam[1], cam[2], feat);
 }

@@ -34,16 +34,30 @@ __device__ __forceinline__ uint16_t f2bf(float x) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// GELU, tanh form: 0.5 z (1 + tanh(c (z + 0.044715 z^3))), c = sqrt(2 / pi)
+// GELU, tanh form 0.5 z (1 + tanh(u)), u = c (z + 0.044715 z^3), c = sqrt(2 / pi),
+// evaluated as z * sigmoid(2u) = z / (1 + 2^(z (k0 + k1 z^2))) with
+// k0 = -2 c log2(e), k1 = 0.044715 k0: one exp2 and one reciprocal, saturating
+// without a branch (the power overflows to inf or flushes to 0).
+constexpr float kGeluK0 = -2.302208198144325f;
+constexpr float kGeluK1 = -0.1029432395800235f;
 __device__ __forceinline__ float gelu_f(float z) {
-    const float c = 0.7978845608028654f;
-    float t = tanhf(c * (z + 0.044715f * z * z * z));
-    return 0.5f * z * (1.0f + t);
+    const float e = exp2f(z * fmaf(kGeluK1, z * z, kGeluK0));
+    return __fdividef(z, 1.0f + e);
 }
+// d/dz: s + z s (1 - s) 2c (1 + 3 * 0.044715 z^2), s = sigmoid(2u)
 __device__ __forceinline__ float gelu_d(float z) {
-    const float c = 0.7978845608028654f;
-    float t = tanhf(c * (z + 0.044715f * z * z * z));
-    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * z * z);
+    const float z2 = z * z;
+    const float e = exp2f(z * fmaf(kGeluK1, z2, kGeluK0));
+    const float s = __fdividef(1.0f, 1.0f + e);
+    return s + z * s * (1.0f - s) * (1.5957691216057308f * fmaf(0.134145f, z2, 1.0f));
+}
+
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+// two floats to bf16 (round to nearest even, v_cvt_pk_bf16_f32), a in the low half
+__device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
+    const f32x2 v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
 }
 
 __device__ __forceinline__ bf16x8 ld8(const uint16_t* p) {
@@ -233,6 +247,101 @@ __global__ void k_mlp_adam(float* __restrict__ p, const float* __restrict__ g, f
 void launch_mlp_adam(hipStream_t st, float* p, const float* g, float* m, float* v, int total, float lr, float bc1,
                      float bc2, const MlpLayout& L) {
     hipLaunchKernelGGL(k_mlp_adam, dim3((total + 255) / 256), dim3(256), 0, st, p, g, m, v, total, lr, bc1, bc2, L);
+}
+
+
+// Inference: all four layers in one launch.  A workgroup takes 64 rows; their
+// activations stay in LDS (two bf16 buffers, rows padded to 264 elements so
+// the 16 rows an MFMA operand reads fall on different banks) and the bf16
+// out-major weights stream from L2.  Each product is computed transposed,
+// D[n][m] = sum_k W[n][k] act[m][k], so a lane's four results are consecutive
+// output features of one row: one 8-byte LDS store after bias + GELU.  Wave w
+// owns output features [64w, 64w + 64) of the hidden layers (4 x 4 MFMA tiles)
+// and rows [16w, 16w + 16) of the output layer.
+constexpr int kInfRows = 64;
+constexpr int kInfStride = kMlpHidden + 8;
+
+__device__ __forceinline__ bf16x8 lds8(const uint16_t* p) {
+    uint4 q = *reinterpret_cast<const uint4*>(p);
+    return __builtin_bit_cast(bf16x8, q);
+}
+
+template <int K>
+__device__ __forceinline__ void infer_hidden(const uint16_t* __restrict__ W, const float* __restrict__ bias,
+                                             const uint16_t* src, uint16_t* dst, int w, int l) {
+    const int r = l & 15, kq = (l >> 4) * 8;
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const uint16_t* wp = W + (size_t)(64 * w + r) * K + kq;
+#pragma unroll 2
+    for (int kb = 0; kb < K; kb += 32) {
+        bf16x8 wa[4], xb[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) wa[t] = ld8(wp + (size_t)16 * t * K + kb);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) xb[t] = lds8(src + (16 * t + r) * kInfStride + kb + kq);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[a], xb[b], acc[a][b], 0, 0, 0);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int n = 64 * w + 16 * a + 4 * (l >> 4);
+        const float4 bv = *reinterpret_cast<const float4*>(bias + n);
+        const float bb[4] = {bv.x, bv.y, bv.z, bv.w};
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            uint2 pk;
+            pk.x = pack_bf16(gelu_f(acc[a][b][0] + bb[0]), gelu_f(acc[a][b][1] + bb[1]));
+            pk.y = pack_bf16(gelu_f(acc[a][b][2] + bb[2]), gelu_f(acc[a][b][3] + bb[3]));
+            *reinterpret_cast<uint2*>(dst + (16 * b + r) * kInfStride + n) = pk;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ x, int n, const float* __restrict__ norm,
+                                                   MlpInfer P, float* __restrict__ out) {
+    __shared__ __attribute__((aligned(16))) uint16_t act[2][kInfRows * kInfStride];
+    const int row0 = blockIdx.x * kInfRows;
+    for (int i = threadIdx.x; i < kInfRows * kMlpIn; i += 256) {
+        const int m = i / kMlpIn, f = i - m * kMlpIn;
+        float v = 0.0f;
+        if (row0 + m < n) v = (x[(size_t)(row0 + m) * kMlpIn + f] - norm[f]) / norm[32 + f];
+        act[0][m * kInfStride + f] = f2bf(v);
+    }
+    __syncthreads();
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    infer_hidden<kMlpIn>(P.wo[0], P.bias[0], act[0], act[1], w, l);
+    __syncthreads();
+    infer_hidden<kMlpHidden>(P.wo[1], P.bias[1], act[1], act[0], w, l);
+    __syncthreads();
+    infer_hidden<kMlpHidden>(P.wo[2], P.bias[2], act[0], act[1], w, l);
+    __syncthreads();
+    // output layer: 16 (of 32 padded) output rows of W3 x the wave's 16 rows
+    const int r = l & 15, kq = (l >> 4) * 8;
+    f32x4 c = {0.f, 0.f, 0.f, 0.f};
+    const uint16_t* src = act[1] + (16 * w + r) * kInfStride + kq;
+    const uint16_t* wp = P.wo[3] + (size_t)r * kMlpHidden + kq;
+#pragma unroll
+    for (int kb = 0; kb < kMlpHidden; kb += 32)
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ld8(wp + kb), lds8(src + kb), c, 0, 0, 0);
+    const int m = row0 + 16 * w + r;
+    if (m < n) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int j = 4 * (l >> 4) + e;
+            if (j < kMlpOut) out[(size_t)m * kMlpOut + j] = (c[e] + P.bias[3][j]) * norm[64 + kMlpOut + j] + norm[64 + j];
+        }
+    }
+}
+
+void launch_mlp_infer(hipStream_t st, const float* x, int n, const float* norm, const MlpInfer& P, float* out) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_mlp_infer, dim3((n + kInfRows - 1) / kInfRows), dim3(256), 0, st, x, n, norm, P, out);
 }
 
 }  // namespace nps

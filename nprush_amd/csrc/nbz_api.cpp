@@ -544,4 +544,23 @@ int nbz_read_timing(nbz_handle h, float ms[8]) {
     return NPS_OK;
 }
 
+int nbz_dev_alloc(nbz_handle h, int64_t bytes, void** out) {
+    if (!h || !out || bytes < 0) return NPS_ERR_ARG;
+    NBZ_HIP(h, hipSetDevice(h->device));
+    *out = nullptr;
+    NBZ_HIP(h, hipMalloc(out, (size_t)std::max<int64_t>(bytes, 1)));
+    return NPS_OK;
+}
+int nbz_dev_free(nbz_handle h, void* p) {
+    if (!h) return NPS_ERR_ARG;
+    NBZ_HIP(h, hipSetDevice(h->device));
+    NBZ_HIP(h, hipStreamSynchronize(h->stream));
+    if (p) NBZ_HIP(h, hipFree(p));
+    return NPS_OK;
+}
+int nbz_dev_upload(nbz_handle h, void* dst, const void* src, int64_t bytes) {
+    if (!h || !dst || !src || bytes < 0) return NPS_ERR_ARG;
+    return bz_copy(h, dst, src, (size_t)bytes, hipMemcpyHostToDevice);
+}
+
 }  // extern "C"

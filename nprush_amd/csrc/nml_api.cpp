@@ -58,6 +58,9 @@ struct nml_ctx {
     double* partial = nullptr;
     float adam_t = 0.0f;
     MlpLayout L;
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    float timing_ms[2] = {0.0f, 0.0f};  // last inference launch, last training step
     std::string err;
 };
 
@@ -200,6 +203,8 @@ int nml_create(int32_t device, int32_t max_batch, nml_handle* out) {
     for (int l = 0; l < kMlpLayers && ok; ++l)
         ok = al((void**)&h->wo[l], 2 * (size_t)kOutPad[l] * kIn[l]) && al((void**)&h->wi[l], 2 * (size_t)kIn[l] * kOutPad[l]);
     if (!ok) return fail(NPS_ERR_HIP);
+    for (hipEvent_t& e : h->ev)
+        if (hipEventCreate(&e) != hipSuccess) return fail(NPS_ERR_HIP);
     for (int l = 0; l < kMlpLayers; ++l) {
         h->L.w_off[l] = w_offset(l);
         h->L.in[l] = kIn[l];
@@ -235,6 +240,8 @@ void nml_destroy(nml_handle h) {
         if (h->wo[l]) (void)hipFree(h->wo[l]);
         if (h->wi[l]) (void)hipFree(h->wi[l]);
     }
+    for (hipEvent_t e : h->ev)
+        if (e) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -280,16 +287,20 @@ int nml_set_norm(nml_handle h, const float* in_mean, const float* in_std, const 
 int nml_forward_dev(nml_handle h, const float* x_dev, int32_t n, float* y_dev) {
     if (!h || n < 0 || (n > 0 && (!x_dev || !y_dev))) return NPS_ERR_ARG;
     NML_HIP(h, hipSetDevice(h->device));
-    for (int32_t b = 0; b < n; b += h->max_m) {
-        const int nb = std::min<int32_t>(h->max_m, n - b);
-        const int M = (nb + 31) / 32 * 32;
-        launch_mlp_input(h->stream, x_dev + (size_t)b * kMlpIn, h->norm, h->norm + 32, nb, M, h->x_rm, h->x_fm);
-        forward_core(h, nb, M);
-        launch_mlp_output(h->stream, h->y, h->norm + 64, h->norm + 64 + kMlpOut, nb, y_dev + (size_t)b * kMlpOut);
+    // one fused launch over all n rows: activations live in LDS, so the batch
+    // capacity does not apply
+    MlpInfer P;
+    for (int l = 0; l < kMlpLayers; ++l) {
+        P.wo[l] = h->wo[l];
+        P.bias[l] = l < 3 ? h->params + w_offset(l) + kIn[l] * kOut[l] : h->bias4;
     }
+    if (h->timing) NML_HIP(h, hipEventRecord(h->ev[0], h->stream));
+    launch_mlp_infer(h->stream, x_dev, n, h->norm, P, y_dev);
+    if (h->timing) NML_HIP(h, hipEventRecord(h->ev[1], h->stream));
     NML_HIP(h, hipGetLastError());
     // the caller's next use of y_dev may be on another stream
     NML_HIP(h, hipStreamSynchronize(h->stream));
+    if (h->timing) NML_HIP(h, hipEventElapsedTime(&h->timing_ms[0], h->ev[0], h->ev[1]));
     return NPS_OK;
 }
 
@@ -329,6 +340,7 @@ int nml_train_dev(nml_handle h, const float* x_dev, const float* t_dev, int32_t 
     if (n > h->max_m) return ml_err(h, NPS_ERR_ARG, "batch above the handle's capacity");
     NML_HIP(h, hipSetDevice(h->device));
     const int M = (n + 31) / 32 * 32;
+    if (h->timing) NML_HIP(h, hipEventRecord(h->ev[2], h->stream));
     launch_mlp_input(h->stream, x_dev, h->norm, h->norm + 32, n, M, h->x_rm, h->x_fm);
     forward_core(h, n, M);
     int np = backward_core(h, t_dev, n, M);
@@ -338,8 +350,11 @@ int nml_train_dev(nml_handle h, const float* x_dev, const float* t_dev, int32_t 
     launch_mlp_adam(h->stream, h->params, h->grads, h->am, h->av, kMlpParams, lr, bc1, bc2, h->L);
     NML_HIP(h, hipMemcpyAsync(h->bias4, h->params + w_offset(3) + kMlpOut * kMlpHidden, 4 * kMlpOut,
                               hipMemcpyDeviceToDevice, h->stream));
+    if (h->timing) NML_HIP(h, hipEventRecord(h->ev[3], h->stream));
     NML_HIP(h, hipGetLastError());
-    return read_loss(h, np, n, loss);
+    const int rc = read_loss(h, np, n, loss);
+    if (rc == NPS_OK && h->timing) NML_HIP(h, hipEventElapsedTime(&h->timing_ms[1], h->ev[2], h->ev[3]));
+    return rc;
 }
 
 int nml_train(nml_handle h, const float* x, const float* t, int32_t n, float lr, double* loss) {
@@ -349,6 +364,43 @@ int nml_train(nml_handle h, const float* x, const float* t, int32_t n, float lr,
     NML_HIP(h, hipMemcpyAsync(h->x_tmp, x, 4 * (size_t)n * kMlpIn, hipMemcpyHostToDevice, h->stream));
     NML_HIP(h, hipMemcpyAsync(h->t_tmp, t, 4 * (size_t)n * kMlpOut, hipMemcpyHostToDevice, h->stream));
     return nml_train_dev(h, h->x_tmp, h->t_tmp, n, lr, loss);
+}
+
+int nml_enable_timing(nml_handle h, int32_t on) {
+    if (!h) return NPS_ERR_ARG;
+    h->timing = on != 0;
+    return NPS_OK;
+}
+
+int nml_read_timing(nml_handle h, float ms[2]) {
+    if (!h || !ms) return NPS_ERR_ARG;
+    ms[0] = h->timing_ms[0];
+    ms[1] = h->timing_ms[1];
+    return NPS_OK;
+}
+
+int nml_dev_alloc(nml_handle h, int64_t bytes, void** out) {
+    if (!h || !out || bytes < 0) return NPS_ERR_ARG;
+    NML_HIP(h, hipSetDevice(h->device));
+    *out = nullptr;
+    NML_HIP(h, hipMalloc(out, (size_t)std::max<int64_t>(bytes, 1)));
+    return NPS_OK;
+}
+
+int nml_dev_free(nml_handle h, void* p) {
+    if (!h) return NPS_ERR_ARG;
+    NML_HIP(h, hipSetDevice(h->device));
+    NML_HIP(h, hipStreamSynchronize(h->stream));
+    if (p) NML_HIP(h, hipFree(p));
+    return NPS_OK;
+}
+
+int nml_dev_upload(nml_handle h, void* dst, const void* src, int64_t bytes) {
+    if (!h || !dst || !src || bytes < 0) return NPS_ERR_ARG;
+    NML_HIP(h, hipSetDevice(h->device));
+    NML_HIP(h, hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, h->stream));
+    NML_HIP(h, hipStreamSynchronize(h->stream));
+    return NPS_OK;
 }
 
 }  // extern "C"

@@ -774,7 +774,8 @@ int nps_adc_features(nps_handle h, float* feats, int32_t max_rows, int32_t* nq) 
     if ((rc = ensure(h, h->adc_feat, 128 * (size_t)std::max<uint32_t>(h->n, 1))) != NPS_OK) return rc;
     float cam[3];
     ext_camera_centre(h->vp_inv, cam);
-    launch_adc_ext_features(h->stream, h->splats, h->sh, h->accum, h->n, h->adcx, cam, b, (float*)h->adc_feat.p);
+    launch_adc_ext_features(h->stream, h->splats, h->sh, h->accum, h->m, h->v, h->n, h->adcx, cam, b,
+                            (float*)h->adc_feat.p);
     NPS_HIP(h, hipGetLastError());
     uint32_t cnt[2] = {0, 0};
     NPS_HIP(h, hipMemcpyAsync(cnt, b.cnt, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));

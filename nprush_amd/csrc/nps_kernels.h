@@ -142,7 +142,8 @@ struct AdcExtParams {
     float percent_dense;  // split when the largest scale exceeds percent_dense * extent
 };
 struct AdcExtBuffers {
-    uint32_t *dead, *qual, *dead_rank, *qual_rank, *dead_list, *qual_list;  // n each
+    uint32_t *dead, *qual, *dead_list, *qual_list;  // n each
+    uint32_t *dead_rank, *qual_rank;                // n + 1 each (the scan writes the total at [n])
     uint32_t* cnt;  // 4: qualifiers, dead slots, splits, clones
     void* scan_tmp;
 };
@@ -150,8 +151,9 @@ void run_adc_ext(hipStream_t st, float* splats, float* sh, float* m, float* v, f
                  const AdcExtParams& prm, uint32_t seed, const AdcExtBuffers& b);
 // qualifier/dead lists as run_adc_ext builds them, and (feat != null) 32 features
 // per qualifier in placement order
-void launch_adc_ext_features(hipStream_t st, const float* splats, const float* sh, const float* accum, uint32_t n,
-                             const AdcExtParams& prm, const float cam[3], const AdcExtBuffers& b, float* feat);
+void launch_adc_ext_features(hipStream_t st, const float* splats, const float* sh, const float* accum,
+                             const float* m, const float* v, uint32_t n, const AdcExtParams& prm, const float cam[3],
+                             const AdcExtBuffers& b, float* feat);
 // adds 14 deltas per placement k < nrows to the two children of the last densification
 void launch_adc_ext_deltas(hipStream_t st, float* splats, const AdcExtBuffers& b, const float* delta, uint32_t nrows);
 

@@ -33,6 +33,13 @@ struct MlpLayout {
     uint16_t* wi[kMlpLayers];  // bf16 [in][out_pad]
 };
 
+// Weights and biases the fused inference reads: wo[l] bf16 [out_pad][in], bias[l]
+// FP32 (bias[3] padded to 32 entries, 16-byte aligned).
+struct MlpInfer {
+    const uint16_t* wo[kMlpLayers];
+    const float* bias[kMlpLayers];
+};
+
 // C[M][N] = sum_k A[m][k] Bt[n][k]; M, N, K multiples of 32.
 void launch_gemm_nt(hipStream_t st, const uint16_t* A, const uint16_t* Bt, int M, int N, int K, const MlpEpi& e);
 void launch_mlp_input(hipStream_t st, const float* x, const float* mean, const float* stdv, int n, int M,
@@ -42,6 +49,8 @@ void launch_mlp_output(hipStream_t st, const float* y, const float* mean, const 
 int launch_mlp_loss(hipStream_t st, const float* y, const float* t, const float* mean, const float* stdv, int n, int M,
                     uint16_t* d_rm, uint16_t* d_fm, double* partial);
 void launch_mlp_bias_grad(hipStream_t st, const uint16_t* d_fm, int M, int N, float* db);
+// De-standardised outputs [n][14] of n feature rows [n][32] in one launch.
+void launch_mlp_infer(hipStream_t st, const float* x, int n, const float* norm, const MlpInfer& P, float* out);
 void launch_mlp_adam(hipStream_t st, float* p, const float* g, float* m, float* v, int total, float lr, float bc1,
                      float bc2, const MlpLayout& L);
 

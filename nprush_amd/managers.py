@@ -732,6 +732,20 @@ class GpuBezierOptimizerManager:
         return {"preprocess": float(ms[0]), "keys": float(ms[1]), "sort": float(ms[2]), "composite": float(ms[3]),
                 "step": float(ms[4]), "adc": float(ms[5]), "backward": float(ms[6])}
 
+    def dev_alloc(self, nbytes: int) -> int:
+        """A device buffer on the handle's GPU (nbz_dev_alloc); free with dev_free."""
+        p = C.c_void_p()
+        self._check(self._lib.nbz_dev_alloc(self._h, int(nbytes), C.byref(p)), "nbz_dev_alloc")
+        return p.value
+
+    def dev_free(self, ptr: int):
+        self._lib.nbz_dev_free(self._h, C.c_void_p(ptr))
+
+    def dev_upload(self, ptr: int, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        self._check(self._lib.nbz_dev_upload(self._h, C.c_void_p(ptr), a.ctypes.data_as(C.c_void_p), a.nbytes),
+                    "nbz_dev_upload")
+
 
 class DensifierMLP:
     """The densification MLP (nml_*): 32 parent features -> 14 child deltas,
@@ -814,6 +828,29 @@ class DensifierMLP:
         self._check(self._lib.nml_train_dev(self._h, C.c_void_p(x_ptr), C.c_void_p(t_ptr), int(n), float(lr),
                                             C.byref(loss)), "nml_train_dev")
         return float(loss.value)
+
+    def enable_timing(self, on: bool = True):
+        self._check(self._lib.nml_enable_timing(self._h, 1 if on else 0), "nml_enable_timing")
+
+    def read_timing(self) -> dict:
+        """HIP-event ms of the last inference launch and the last training step."""
+        ms = np.zeros(2, np.float32)
+        self._check(self._lib.nml_read_timing(self._h, fptr(ms)), "nml_read_timing")
+        return {"forward": float(ms[0]), "train_step": float(ms[1])}
+
+    def dev_alloc(self, nbytes: int) -> int:
+        """A device buffer on the handle's GPU (nml_dev_alloc); free with dev_free."""
+        p = C.c_void_p()
+        self._check(self._lib.nml_dev_alloc(self._h, int(nbytes), C.byref(p)), "nml_dev_alloc")
+        return p.value
+
+    def dev_free(self, ptr: int):
+        self._lib.nml_dev_free(self._h, C.c_void_p(ptr))
+
+    def dev_upload(self, ptr: int, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        self._check(self._lib.nml_dev_upload(self._h, C.c_void_p(ptr), a.ctypes.data_as(C.c_void_p), a.nbytes),
+                    "nml_dev_upload")
 
 
 def debug_math(op: int, x: np.ndarray, y: np.ndarray | None = None, device: int = 0) -> np.ndarray:

@@ -35,7 +35,7 @@ def _scene(n_live=3000, cap=4000, seed=0):
     return m, views
 
 
-def _host_features(s, sh, acc, idx, split_scale, cam):
+def _host_features(s, sh, acc, m, v, idx, split_scale, cam):
     f = np.zeros((len(idx), 32))
     p = s[idx].astype(np.float64)
     f[:, 0:3] = p[:, 0:3]
@@ -53,17 +53,18 @@ def _host_features(s, sh, acc, idx, split_scale, cam):
     f[:, 13] = np.where(a[:, 1] > 0, a[:, 0] / np.maximum(a[:, 1], 1e-30), 0)
     f[:, 14] = a[:, 1]
     f[:, 15:18] = p[:, 4:7]
-    f[:, 18:22] = q
     smax = sc.max(1)
-    f[:, 22] = smax
-    f[:, 23] = smax / split_scale
-    f[:, 24] = np.linalg.norm(p[:, 0:3], axis=1)
-    f[:, 25] = smax > split_scale
-    f[:, 26] = p[:, 3]
-    f[:, 27] = a[:, 0]
+    f[:, 18] = smax
+    f[:, 19] = smax / split_scale
+    f[:, 20] = smax > split_scale
+    f[:, 21] = p[:, 3]
+    f[:, 22] = a[:, 0]
     d = np.linalg.norm(p[:, 0:3] - cam, axis=1)
-    f[:, 28] = d
-    f[:, 29] = 1 / np.maximum(d, 1e-6)
+    f[:, 23] = d
+    f[:, 24] = 1 / np.maximum(d, 1e-6)
+    cols = densify.CHILD_COLS
+    mm, vv = m[idx][:, cols].astype(np.float64), v[idx][:, cols].astype(np.float64)
+    f[:, 25:32] = mm / (np.sqrt(vv) + 1e-12)
     return f
 
 
@@ -71,6 +72,7 @@ def test_features_placements_deltas_snapshot():
     m, views = _scene()
     densify.train_steps(m, views, 0, 12)
     s, sh, acc = m.read_splats(), m.read_sh(), m.read_accum()
+    am, av, _ = m.read_adam()
     feats = m.adc_features()
     with np.errstate(over="ignore"):  # parked slots hold logit -1e4
         op = 1 / (1 + np.exp(-s[:, 3].astype(np.float64)))
@@ -79,7 +81,7 @@ def test_features_placements_deltas_snapshot():
     assert len(feats) == len(qual) > 0
     vi = np.asarray(views[-1][1], np.float64)
     cam = np.array([vi[8] / vi[11], vi[9] / vi[11], vi[10] / vi[11]])
-    want = _host_features(s, sh, acc, qual, 0.01 * 3.3, cam)
+    want = _host_features(s, sh, acc, am, av, qual, 0.01 * 3.3, cam)
     err = np.abs(feats - want) / (np.abs(want) * 2e-5 + 1e-6)
     bad = np.argwhere(err > 1.0)
     rows = sorted(set(int(r) for r, _ in bad))
