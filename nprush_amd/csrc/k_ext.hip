@@ -107,8 +107,10 @@ __global__ __launch_bounds__(256) void k_ext_rank_counts(uint32_t n, const uint6
 }
 
 // One key tile << 32 | id per (splat, covered tile), splats taken in rank
-// order; the wave walks its 64 ranks in order and writes each splat's keys
-// with all lanes.
+// order, each splat's tiles row by row.  A wave owns 64 consecutive ranks, whose
+// keys are contiguous from offsets[first rank]: the wave scans its 64 tile
+// counts, then its lanes write consecutive keys of that range, each finding its
+// splat by binary search over the inclusive counts (fully coalesced stores).
 __global__ __launch_bounds__(256) void k_ext_keys(uint32_t n, const uint64_t* __restrict__ order,
                                                   const uint32_t* __restrict__ rank_counts,
                                                   const uint4* __restrict__ rects, uint32_t tiles_x,
@@ -116,31 +118,55 @@ __global__ __launch_bounds__(256) void k_ext_keys(uint32_t n, const uint64_t* __
                                                   uint64_t cap, uint32_t* __restrict__ overflow) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wbase = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
+    if (wbase >= n) return;  // whole wave
     const uint32_t mine = wbase + lane;
     const uint32_t my_cnt = mine < n ? rank_counts[mine] : 0u;
-    uint4 my_r = make_uint4(0u, 0u, 0u, 0u);
-    uint64_t my_off = 0ull;
-    uint32_t my_id = 0u;
+    uint32_t my_id = 0u, my_x0 = 0u, my_w = 1u, my_y0 = 0u;
+    float my_rcp = 1.0f;
     if (my_cnt) {
         my_id = (uint32_t)order[mine];
-        my_r = rects[my_id];
-        my_off = offsets[mine];
+        const uint4 r = rects[my_id];
+        my_x0 = r.x;
+        my_w = r.y - r.x;
+        my_y0 = r.z;
+        my_rcp = 1.0f / (float)my_w;
     }
-    for (uint32_t s = 0; s < 64u; ++s) {
-        const uint32_t cnt = __shfl(my_cnt, (int)s, 64);
-        if (cnt == 0u) continue;
-        const uint64_t id = (uint64_t)__shfl(my_id, (int)s, 64);
-        const uint32_t rx0 = __shfl(my_r.x, (int)s, 64), rx1 = __shfl(my_r.y, (int)s, 64);
-        const uint32_t ry0 = __shfl(my_r.z, (int)s, 64);
-        const uint32_t rw = rx1 - rx0;
-        const uint64_t pos = __shfl(my_off, (int)s, 64);
-        if (pos + cnt > cap) {
-            if (lane == 0) *overflow = 1u;
+    uint32_t incl = my_cnt;  // inclusive scan of the counts over the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o, 64);
+        if ((int)lane >= o) incl += y;
+    }
+    const uint32_t total = __shfl(incl, 63, 64);
+    const uint64_t pos0 = offsets[wbase];
+    if (pos0 + total > cap && lane == 0) *overflow = 1u;
+    // every lane runs every iteration: a cross-lane read from an inactive lane returns 0
+    for (uint32_t g0 = 0u; g0 < total; g0 += 64u) {
+        const uint32_t g = g0 + lane;
+        // s = first lane whose inclusive count exceeds g (63 when g >= total)
+        uint32_t s = 0u;
+#pragma unroll
+        for (uint32_t b = 32u; b > 0u; b >>= 1) {
+            const uint32_t v = (uint32_t)__shfl((int)incl, (int)(s + b - 1u), 64);
+            if (v <= g) s += b;
         }
-        for (uint32_t k = lane; k < cnt; k += 64u) {
-            const uint32_t ty = ry0 + k / rw, tx = rx0 + k % rw;
-            if (pos + k < cap) keys[pos + k] = ((uint64_t)(ty * tiles_x + tx) << 32) | id;
+        const uint32_t k = g - ((uint32_t)__shfl((int)incl, (int)s, 64) - (uint32_t)__shfl((int)my_cnt, (int)s, 64));
+        const uint32_t w = (uint32_t)__shfl((int)my_w, (int)s, 64);
+        const float rcp = __shfl(my_rcp, (int)s, 64);
+        // k / w from the float reciprocal (within one of the quotient for k < 2^24), corrected
+        uint32_t q = (uint32_t)((float)k * rcp);
+        int32_t rem = (int32_t)k - (int32_t)(q * w);
+        if (rem < 0) {
+            --q;
+            rem += (int32_t)w;
+        } else if (rem >= (int32_t)w) {
+            ++q;
+            rem -= (int32_t)w;
         }
+        const uint32_t ty = (uint32_t)__shfl((int)my_y0, (int)s, 64) + q;
+        const uint32_t tx = (uint32_t)__shfl((int)my_x0, (int)s, 64) + (uint32_t)rem;
+        const uint64_t id = (uint64_t)(uint32_t)__shfl((int)my_id, (int)s, 64);
+        if (g < total && pos0 + g < cap) keys[pos0 + g] = ((uint64_t)(ty * tiles_x + tx) << 32) | id;
     }
 }
 
@@ -298,6 +324,11 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
     const bool valid = px < v.W && py < v.H;
     const float pxf = px_to_x(px, v.W, v.aspect);
     const float pyf = py_to_y(py, v.H);
+    // the wave's 8x8 block: pixel-centre coordinates of its extreme columns and rows
+    // (px_to_x grows with px, py_to_y falls with py)
+    const uint32_t bx0 = tx * TILE + (wid % BW) * 8, by0 = ty * TILE + (wid / BW) * 8;
+    const float blk_x0 = px_to_x(bx0, v.W, v.aspect), blk_x1 = px_to_x(bx0 + 7u, v.W, v.aspect);
+    const float blk_y0 = py_to_y(by0 + 7u, v.H), blk_y1 = py_to_y(by0, v.H);
     // this tile's batch slots: [start/BATCH + tile, ...) never overlap another tile's
     const size_t slot0 = (size_t)(start / BATCH) + tile;
 
@@ -356,6 +387,26 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
         put(nb);
         __syncthreads();
         prefetch(b0 + BATCH, b0 + 2 * BATCH);
+        // Entries whose box cannot reach the wave's block: lane j of the wave tests
+        // entry 64w + j.  The margin (1e-5 of the coordinates' magnitude) is far
+        // above the rounding of the pixel test |px - centre| <= extent, so an entry
+        // left out has alpha 0 on every pixel of the block.
+        uint32_t hit_lo[MW], hit_hi[MW];
+#pragma unroll
+        for (int w = 0; w < MW; ++w) {
+            const int j = 64 * w + lane;
+            bool h = false;
+            if (j < nb) {
+                const ExtRec& f = s_rec[j];
+                const float mx = 1e-5f * (fabsf(f.scr_x) + f.ext_x + 4.0f);
+                const float my = 1e-5f * (fabsf(f.scr_y) + f.ext_y + 4.0f);
+                h = (f.scr_x - f.ext_x <= blk_x1 + mx) && (f.scr_x + f.ext_x >= blk_x0 - mx) &&
+                    (f.scr_y - f.ext_y <= blk_y1 + my) && (f.scr_y + f.ext_y >= blk_y0 - my);
+            }
+            const uint64_t bal = __ballot(h);
+            hit_lo[w] = __builtin_amdgcn_readfirstlane((uint32_t)bal);
+            hit_hi[w] = __builtin_amdgcn_readfirstlane((uint32_t)(bal >> 32));
+        }
         uint64_t mask[MW];
 #pragma unroll
         for (int w = 0; w < MW; ++w) mask[w] = 0ull;
@@ -363,12 +414,19 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
         if (__ballot(alive) != 0ull) {
             for (int c0 = 0; c0 < nb; c0 += 8) {
                 if (A.backward) ck[(c0 >> 3) * NT + tid] = T;
+                const uint32_t hw = (c0 & 32) ? hit_hi[c0 >> 6] : hit_lo[c0 >> 6];
+                const uint32_t hits = (hw >> (c0 & 31)) & 0xFFu;  // scalar: bit j = entry c0 + j may hit
+                n_fw += (uint32_t)__popc(hits);
+                if (hits == 0u) {  // no alpha anywhere in the block: T unchanged
+                    if (alive) n_eval += (uint32_t)min(8, nb - c0);
+                    continue;
+                }
                 // alphas of the chunk's 8 entries are independent: computed first
                 float ca[8];
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     ca[j] = 0.0f;
-                    if (c0 + j < nb) {
+                    if ((hits >> j) & 1u) {
                         const ExtRec& f = s_rec[c0 + j];
                         ExtPair q = ext_pair(f, pxf, pyf);
                         if (in_box(q.dx, q.dy, f.ext_x, f.ext_y) && !(q.r2 > f.rcut2)) {
@@ -382,10 +440,10 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
                 // then the front-to-back chain, entry by entry
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
-                    if (c0 + j < nb) {
+                    if (c0 + j < nb && alive) ++n_eval;
+                    if ((hits >> j) & 1u) {
                         bool inc = false;
                         if (alive) {
-                            ++n_eval;
                             const float a = ca[j];
                             if (!(a < kExtMinAlpha)) {
                                 float tT = T * (1.0f - a);
@@ -402,7 +460,6 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
                                 }
                             }
                         }
-                        ++n_fw;
                         if (__ballot(inc) != 0ull) mask[(c0 + j) >> 6] |= 1ull << ((c0 + j) & 63);
                     }
                 }
@@ -468,21 +525,27 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
             prefetch(b0 - BATCH, b0 - 2 * BATCH);
             uint64_t mask[MW];
 #pragma unroll
-            for (int w = 0; w < MW; ++w) mask[w] = A.act[((slot0 + bi) * NW + wid) * MW + w];
+            for (int w = 0; w < MW; ++w) {
+                const uint64_t m = A.act[((slot0 + bi) * NW + wid) * MW + w];  // the same word in every lane
+                mask[w] = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(m >> 32)) << 32) |
+                          __builtin_amdgcn_readfirstlane((uint32_t)m);
+            }
             const float* ck = A.ckpt + (slot0 + bi) * (size_t)(NCH * NT);
             for (int c = NCH - 1; c >= 0; --c) {
                 const uint32_t bits = (uint32_t)(mask[c >> 3] >> (8 * (c & 7))) & 0xFFu;
                 if (bits == 0u) continue;
-                const int hi = 31 - __clz((int)bits);
                 float Tc = ck[c * NT + tid];
                 float ra[8], rT[8], rG[8];
                 uint32_t incb = 0u;
+                // an entry whose bit is 0 changed no lane's transmittance before
+                // that lane's last composited entry (it would have been composited):
+                // skipped
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     ra[j] = 0.0f;
                     rG[j] = 0.0f;
                     rT[j] = Tc;
-                    if (j <= hi) {
+                    if ((bits >> j) & 1u) {
                         const int e = b0 + 8 * c + j;
                         if (e < last) {
                             const ExtRec& f = s_rec[8 * c + j];

@@ -37,6 +37,8 @@ struct ExtView {
     bool in_alt = false;
     uint32_t num_tiles = 0;
     uint32_t* flags_at = nullptr;  // flags in a shared buffer instead of `flags` (16 words)
+    hipEvent_t ev_count = nullptr;  // eager: recorded after the key-count copy
+    uint64_t* count_host = nullptr; // eager: pinned landing slot of the key count
 };
 
 void free_ext_view(ExtView& x);
@@ -137,6 +139,14 @@ void free_ext_view(ExtView& x) {
                      &x.keys, &x.keys_alt, &x.sort_tmp, &x.ranges, &x.order, &x.tile_work, &x.ckpt, &x.act,
                      &x.g2, &x.c_img, &x.tile_loss, &x.stats, &x.flags};
     for (DevBuf* b : all) free_buf(*b);
+    if (x.ev_count) {
+        (void)hipEventDestroy(x.ev_count);
+        x.ev_count = nullptr;
+    }
+    if (x.count_host) {
+        (void)hipHostFree(x.count_host);
+        x.count_host = nullptr;
+    }
 }
 
 int build_tile_lists(nps_ctx* h, Scratch& sc, const ViewParams& v, uint32_t n, int ev_sort, int ev_ranges) {
@@ -228,11 +238,20 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
     uint64_t cap;
     const uint64_t* m_dev = nullptr;
     if (!capture) {
-        uint64_t m = 0;
-        if ((rc = hip(hipMemcpyAsync(&m, (uint64_t*)x.offsets.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost, st),
-                      "count readback")) != NPS_OK)
+        // The host waits for the key count only; the fills queued behind the copy
+        // (they do not depend on it) keep the GPU busy meanwhile.
+        if (!x.ev_count && (rc = hip(hipEventCreateWithFlags(&x.ev_count, hipEventDisableTiming), "event")) != NPS_OK)
             return rc;
-        if ((rc = hip(hipStreamSynchronize(st), "sync")) != NPS_OK) return rc;
+        if (!x.count_host && (rc = hip(hipHostMalloc((void**)&x.count_host, 64, hipHostMallocDefault), "pinned")) != NPS_OK)
+            return rc;
+        if ((rc = hip(hipMemcpyAsync(x.count_host, (uint64_t*)x.offsets.p + n, sizeof(uint64_t),
+                                     hipMemcpyDeviceToHost, st), "count readback")) != NPS_OK)
+            return rc;
+        if ((rc = hip(hipEventRecord(x.ev_count, st), "event")) != NPS_OK) return rc;
+        launch_fill_u32(st, (uint32_t*)x.ranges.p, 0u, 2 * (uint64_t)v.num_tiles);
+        if (grads) launch_fill_u32(st, (uint32_t*)g2, 0u, 18 * (uint64_t)n);
+        if ((rc = hip(hipEventSynchronize(x.ev_count), "sync")) != NPS_OK) return rc;
+        const uint64_t m = *x.count_host;
         if (m >= 0xFFFFFFFFull) {
             err = "tile key count exceeds 2^32";
             return NPS_ERR_CAPACITY;
@@ -253,9 +272,11 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
                               32 + tile_bits, x.sort_tmp.p, m_dev);
     if (ev && (rc = hip(hipEventRecord(ev[2], st), "event")) != NPS_OK) return rc;
     const uint64_t* sorted = (const uint64_t*)(x.in_alt ? x.keys_alt.p : x.keys.p);
-    launch_fill_u32(st, (uint32_t*)x.ranges.p, 0u, 2 * (uint64_t)v.num_tiles);
+    if (capture) {
+        launch_fill_u32(st, (uint32_t*)x.ranges.p, 0u, 2 * (uint64_t)v.num_tiles);
+        if (grads) launch_fill_u32(st, (uint32_t*)g2, 0u, 18 * (uint64_t)n);
+    }
     launch_tile_ranges(st, sorted, cap, (uint32_t*)x.ranges.p, m_dev);
-    if (grads) launch_fill_u32(st, (uint32_t*)g2, 0u, 18 * (uint64_t)n);
     launch_tile_order(st, (const uint32_t*)x.ranges.p, v.tiles_x, v.tiles_y, (uint32_t*)x.order.p);
     if (ev && (rc = hip(hipEventRecord(ev[3], st), "event")) != NPS_OK) return rc;
     ExtCompositeArgs a;
