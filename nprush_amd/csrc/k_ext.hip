@@ -309,6 +309,7 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
     constexpr int NSTAGE = (3 * BATCH + NT - 1) / NT;  // float4 record pieces per thread
     __shared__ ExtRec s_rec[BATCH];
     __shared__ uint32_t s_ids[BATCH];
+    const float4* rec4 = reinterpret_cast<const float4*>(s_rec);  // 3 float4 per record
     __shared__ float s_part[NW][BATCH][9];
     __shared__ double s_red[2][NW];
     __shared__ int s_max[NW];
@@ -421,47 +422,46 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
                     if (alive) n_eval += (uint32_t)min(8, nb - c0);
                     continue;
                 }
-                // alphas of the chunk's 8 entries are independent: computed first
+                // alphas of the chunk's 8 entries are independent: computed first,
+                // branch-free (selects), so their LDS reads and arithmetic overlap
                 float ca[8];
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
-                    ca[j] = 0.0f;
-                    if ((hits >> j) & 1u) {
-                        const ExtRec& f = s_rec[c0 + j];
-                        ExtPair q = ext_pair(f, pxf, pyf);
-                        if (in_box(q.dx, q.dy, f.ext_x, f.ext_y) && !(q.r2 > f.rcut2)) {
-                            float G = exp2_det_in_range((-0.5f * q.r2) * kLog2e);  // r2 <= rcut2 < 12
-                            float a = f.op * G;
-                            if (a > kExtMaxAlpha) a = kExtMaxAlpha;
-                            ca[j] = a;
-                        }
-                    }
+                    const float4 p0 = rec4[3 * (c0 + j)];      // scr_x scr_y A B
+                    const float4 p1 = rec4[3 * (c0 + j) + 1];  // C op r g
+                    const float4 p2 = rec4[3 * (c0 + j) + 2];  // b rcut2 ext_x ext_y
+                    const float dx = pxf - p0.x, dy = pyf - p0.y;
+                    const float spx = p0.z * dx + p0.w * dy;
+                    const float spy = p0.w * dx + p1.x * dy;
+                    const float r2 = dx * spx + dy * spy;
+                    // non-short-circuit &: every operand is evaluated, no branch
+                    const bool ok = (((hits >> j) & 1u) != 0u) & (fabsf(dx) <= p2.z) & (fabsf(dy) <= p2.w) &
+                                    !(r2 > p2.y);
+                    const float G = exp2_det_in_range(ok ? (-0.5f * r2) * kLog2e : 0.0f);  // r2 <= rcut2 < 12
+                    float a = p1.y * G;
+                    a = a > kExtMaxAlpha ? kExtMaxAlpha : a;
+                    ca[j] = ok ? a : 0.0f;
                 }
-                // then the front-to-back chain, entry by entry
+                // then the front-to-back chain, entry by entry (an entry with
+                // alpha 0, e.g. one past the batch, changes nothing)
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     if (c0 + j < nb && alive) ++n_eval;
-                    if ((hits >> j) & 1u) {
-                        bool inc = false;
-                        if (alive) {
-                            const float a = ca[j];
-                            if (!(a < kExtMinAlpha)) {
-                                float tT = T * (1.0f - a);
-                                if (tT < kExtTMin) {
-                                    alive = false;
-                                } else {
-                                    const ExtRec& f = s_rec[c0 + j];
-                                    Cr = Cr + (T * a) * f.r;
-                                    Cg = Cg + (T * a) * f.g;
-                                    Cb = Cb + (T * a) * f.b;
-                                    T = tT;
-                                    last = b0 + c0 + j + 1;
-                                    inc = true;
-                                }
-                            }
-                        }
-                        if (__ballot(inc) != 0ull) mask[(c0 + j) >> 6] |= 1ull << ((c0 + j) & 63);
-                    }
+                    const float a = ca[j];
+                    const bool take = alive & !(a < kExtMinAlpha);
+                    const float tT = T * (1.0f - a);
+                    const bool dies = take & (tT < kExtTMin);
+                    const bool inc = take & !dies;
+                    const float4 p1 = rec4[3 * (c0 + j) + 1];
+                    const float cb = rec4[3 * (c0 + j) + 2].x;
+                    Cr = inc ? Cr + (T * a) * p1.z : Cr;
+                    Cg = inc ? Cg + (T * a) * p1.w : Cg;
+                    Cb = inc ? Cb + (T * a) * cb : Cb;
+                    T = inc ? tT : T;
+                    last = inc ? b0 + c0 + j + 1 : last;
+                    alive = alive & !dies;
+                    const uint64_t any = __ballot(inc) != 0ull ? 1ull : 0ull;
+                    mask[(c0 + j) >> 6] |= any << ((c0 + j) & 63);
                 }
                 if (__ballot(alive) == 0ull) break;
             }
@@ -539,7 +539,7 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
                 uint32_t incb = 0u;
                 // an entry whose bit is 0 changed no lane's transmittance before
                 // that lane's last composited entry (it would have been composited):
-                // skipped
+                // skipped.  Branch-free inside, as in the forward.
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     ra[j] = 0.0f;
@@ -547,52 +547,54 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
                     rT[j] = Tc;
                     if ((bits >> j) & 1u) {
                         const int e = b0 + 8 * c + j;
-                        if (e < last) {
-                            const ExtRec& f = s_rec[8 * c + j];
-                            ExtPair q = ext_pair(f, pxf, pyf);
-                            if (in_box(q.dx, q.dy, f.ext_x, f.ext_y) && !(q.r2 > f.rcut2)) {
-                                float G = exp2_det_in_range((-0.5f * q.r2) * kLog2e);  // r2 <= rcut2 < 12
-                                float a = f.op * G;
-                                if (a > kExtMaxAlpha) a = kExtMaxAlpha;
-                                if (!(a < kExtMinAlpha)) {
-                                    ra[j] = a;
-                                    rG[j] = G;
-                                    incb |= 1u << j;
-                                    Tc = Tc * (1.0f - a);
-                                }
-                            }
-                        }
+                        const float4 p0 = rec4[3 * (8 * c + j)];
+                        const float4 p1 = rec4[3 * (8 * c + j) + 1];
+                        const float4 p2 = rec4[3 * (8 * c + j) + 2];
+                        const float dx = pxf - p0.x, dy = pyf - p0.y;
+                        const float spx = p0.z * dx + p0.w * dy;
+                        const float spy = p0.w * dx + p1.x * dy;
+                        const float r2 = dx * spx + dy * spy;
+                        const bool in = (e < last) & (fabsf(dx) <= p2.z) & (fabsf(dy) <= p2.w) & !(r2 > p2.y);
+                        const float G = exp2_det_in_range(in ? (-0.5f * r2) * kLog2e : 0.0f);  // r2 <= rcut2 < 12
+                        float a = p1.y * G;
+                        a = a > kExtMaxAlpha ? kExtMaxAlpha : a;
+                        const bool ok = in & !(a < kExtMinAlpha);
+                        ra[j] = ok ? a : 0.0f;
+                        rG[j] = ok ? G : 0.0f;
+                        incb |= (ok ? 1u : 0u) << j;
+                        Tc = ok ? Tc * (1.0f - a) : Tc;
                     }
                 }
 #pragma unroll
                 for (int j = 7; j >= 0; --j) {
                     if ((bits >> j) & 1u) {
                         const int jj = 8 * c + j;
-                        float gv[9];
-#pragma unroll
-                        for (int k = 0; k < 9; ++k) gv[k] = 0.0f;
                         const bool inc = (incb >> j) & 1u;
                         n_active += (uint32_t)__popcll(__ballot(inc));
                         n_went += 1u;
-                        if (inc) {
-                            const ExtRec& f = s_rec[jj];
-                            ExtPair q = ext_pair(f, pxf, pyf);
-                            const float Tk = rT[j], a = ra[j], G = rG[j];
-                            float da = ((dCr * (Tk * f.r) + dCg * (Tk * f.g)) + dCb * (Tk * f.b)) + dT * (-Tk);
-                            dT = dT * (1.0f - a) + ((dCr * (a * f.r) + dCg * (a * f.g)) + dCb * (a * f.b));
-                            gv[5] = dCr * (Tk * a);
-                            gv[6] = dCg * (Tk * a);
-                            gv[7] = dCb * (Tk * a);
-                            if (!(f.op * G > kExtMaxAlpha)) {
-                                gv[8] = da * G;
-                                float gr2 = ((-0.5f * G) * f.op) * da;
-                                gv[0] = (-2.0f * gr2) * q.spx;
-                                gv[1] = (-2.0f * gr2) * q.spy;
-                                gv[2] = (gr2 * q.dx) * q.dx;
-                                gv[3] = ((gr2 * 2.0f) * q.dx) * q.dy;
-                                gv[4] = (gr2 * q.dy) * q.dy;
-                            }
-                        }
+                        const float4 p0 = rec4[3 * jj];
+                        const float4 p1 = rec4[3 * jj + 1];
+                        const float4 p2 = rec4[3 * jj + 2];
+                        const float fr = p1.z, fg = p1.w, fb = p2.x, fop = p1.y;
+                        const float dx = pxf - p0.x, dy = pyf - p0.y;
+                        const float spx = p0.z * dx + p0.w * dy;
+                        const float spy = p0.w * dx + p1.x * dy;
+                        const float Tk = rT[j], a = ra[j], G = rG[j];
+                        const float da = ((dCr * (Tk * fr) + dCg * (Tk * fg)) + dCb * (Tk * fb)) + dT * (-Tk);
+                        const float dTn = dT * (1.0f - a) + ((dCr * (a * fr) + dCg * (a * fg)) + dCb * (a * fb));
+                        dT = inc ? dTn : dT;
+                        const bool geo = inc & !(fop * G > kExtMaxAlpha);  // alpha not clamped
+                        const float gr2 = ((-0.5f * G) * fop) * da;
+                        float gv[9];
+                        gv[5] = inc ? dCr * (Tk * a) : 0.0f;
+                        gv[6] = inc ? dCg * (Tk * a) : 0.0f;
+                        gv[7] = inc ? dCb * (Tk * a) : 0.0f;
+                        gv[8] = geo ? da * G : 0.0f;
+                        gv[0] = geo ? (-2.0f * gr2) * spx : 0.0f;
+                        gv[1] = geo ? (-2.0f * gr2) * spy : 0.0f;
+                        gv[2] = geo ? (gr2 * dx) * dx : 0.0f;
+                        gv[3] = geo ? ((gr2 * 2.0f) * dx) * dy : 0.0f;
+                        gv[4] = geo ? (gr2 * dy) * dy : 0.0f;
                         float t8;
                         float tot = wave_reduce9(gv, lane, t8);
                         if ((lane & 7) == 0) {
