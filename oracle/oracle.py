@@ -197,7 +197,7 @@ def ext_backward_f32(splats, sh, deg, vp, vp_inv, tgt, W, H, tiles, lists, count
     g = np.zeros((n, 16), np.float64) if want_grads else None
     gs = np.zeros((n, 48), np.float64) if want_grads else None
     c = np.zeros((H, W, 3), np.float32)
-    loss = np.zeros(3, np.float64)
+    loss = np.zeros(5, np.float64)
     lib.ora_ext_backward_f32(_p(s, _F), _p(h, _F), n, deg, _p(f32(vp), _F), _p(f32(vp_inv), _F), _p(f32(tgt), _F), W,
                              H, tile, _p(tiles, _I), len(tiles), _p(lists, _U), _p(counts, _I), lists.shape[1], _p(g, _D),
                              _p(gs, _D), _p(c, _F), _p(loss, _D))
@@ -213,7 +213,7 @@ def ext_backward_f64(splats64, sh64, deg, vp64, cam64, tgt, W, H, tiles, lists, 
     g = np.zeros((n, 16), np.float64) if want_grads else None
     gs = np.zeros((n, 48), np.float64) if want_grads else None
     c = np.zeros((H, W, 3), np.float64)
-    loss = np.zeros(3, np.float64)
+    loss = np.zeros(5, np.float64)
     lib.ora_ext_backward_f64(_p(s, _D), _p(h, _D), n, deg, _p(np.ascontiguousarray(vp64, np.float64), _D),
                              _p(np.ascontiguousarray(cam64, np.float64), _D), _p(f32(tgt), _F), W, H, tile, _p(tiles, _I),
                              len(tiles), _p(lists, _U), _p(counts, _I), lists.shape[1], _p(g, _D), _p(gs, _D),
@@ -298,7 +298,7 @@ def ext_backward_csr_f32(splats, sh, deg, vp, vp_inv, tgt, W, H, offs, ids, want
     g = np.zeros((n, 16), np.float64) if want_grads else None
     gs = np.zeros((n, 48), np.float64) if want_grads else None
     c = np.zeros((H, W, 3), np.float32)
-    loss = np.zeros(3, np.float64)
+    loss = np.zeros(5, np.float64)
     idsc = np.ascontiguousarray(ids, np.uint32) if len(ids) else np.zeros(1, np.uint32)
     lib.ora_ext_backward_csr_f32(_p(s, _F), _p(h, _F), n, deg, _p(f32(vp), _F), _p(f32(vp_inv), _F), _p(f32(tgt), _F),
                                  W, H, tile, _p(idsc, _U), _p(np.ascontiguousarray(offs, np.int64), _L), _p(g, _D),
