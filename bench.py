@@ -385,17 +385,20 @@ MLP_FWD_FLOP = 2 * (32 * 256 + 2 * 256 * 256 + 256 * 14)  # 285,696 per sample
 PEAK_BF16_TFLOPS = 2500.0
 
 
-def learned_config4(small=False, events=3, warm=100, window=200, eval_steps=800, adc_period=100, epochs=10):
+def learned_config4(small=False, events=3, warm=100, window=200, eval_steps=3000, adc_period=100, epochs=10,
+                    record_seeds=((0, 1), (4, 5), (6, 7)), eval_every=100):
     """Learned densification on the EXT model (no reference counterpart).
-    Pairs are recorded from plain densification on a config-1 scene (seed 0
-    splats, seed 1 target, 1M live Gaussians plus 1M dead slots, 1080p, 8 ring
-    views): `events` times, `warm` steps, the parents' features, densification,
-    `window` steps without densification, the children's movement.  The MLP
-    (32 -> 256 -> 256 -> 256 -> 14, GELU, BF16 matrix cores) is trained on them
-    with the project's kernels; then a fresh scene (seeds 2 / 3) is trained for
-    eval_steps with plain and with learned densification (held-out view at the
-    half-step ring angle, 0.4 above the ring, as the judge) and the training time
-    to reach plain's final L1 is compared."""
+    Pairs are recorded from plain densification on scale-up scenes (1M live
+    Gaussians plus 1M dead slots, SH3, 1080p, 8 ring views; (splat seed, target
+    seed) in record_seeds, about 0.76M pairs each): `events` times, `warm`
+    steps, the parents' features, densification, `window` steps without
+    densification, the children's movement.  The MLP (32 -> 256 -> 256 -> 256
+    -> 14, GELU, BF16 matrix cores) is trained on them with the project's
+    kernels; then a fresh scene (seeds 2 / 3) is trained for eval_steps (the
+    scale-up configuration's 3000) with plain and with learned densification
+    every adc_period steps (held-out view at the half-step ring angle, 0.4 above
+    the ring, as the judge) and the training time to reach plain's final L1 is
+    compared."""
     import math
 
     from nprush_amd import GpuSplatOptimizerManager, camera, densify, scenes
@@ -431,12 +434,17 @@ def learned_config4(small=False, events=3, warm=100, window=200, eval_steps=800,
         return m, views, (allc[-1][0], allc[-1][1], slots[-1]), slots
 
     t0 = time.perf_counter()
-    m, views, _, slots = scene(0, 1)
-    X, T = densify.record_pairs(m, views, events=events, warm=warm, window=window)
+    Xs, Ts = [], []
+    for seed, tseed in (record_seeds[:1] if small else record_seeds):
+        m, views, _, slots = scene(seed, tseed)
+        x, t = densify.record_pairs(m, views, events=events, warm=warm, window=window)
+        Xs.append(x)
+        Ts.append(t)
+        for sl in slots:
+            m.free_target(sl)
+        m.close()
+    X, T = np.concatenate(Xs), np.concatenate(Ts)
     rec_s = time.perf_counter() - t0
-    for sl in slots:
-        m.free_target(sl)
-    m.close()
     t1 = time.perf_counter()
     net, losses, (held_before, held_after) = densify.train_mlp(X, T, epochs=epochs, batch=4096)
     train_s = time.perf_counter() - t1
@@ -446,7 +454,7 @@ def learned_config4(small=False, events=3, warm=100, window=200, eval_steps=800,
     for name in ("plain", "learned"):
         m, views, held, slots = scene(2, 3)
         judge = densify.LearnedDensifier(net, held) if name == "learned" else None
-        res[name] = densify.run(m, views, eval_steps, adc_period, judge, eval_every=50)
+        res[name] = densify.run(m, views, eval_steps, adc_period, judge, eval_every=eval_every)
         if judge is not None:
             decisions = [(d, int(k), round(lh, 1), round(lm, 1)) for d, k, lh, lm in judge.decisions]
             judge.close()
@@ -481,9 +489,11 @@ def learned_config4(small=False, events=3, warm=100, window=200, eval_steps=800,
     t_learned = densify.time_to_target(res["learned"], target)
     return {
         "workload": f"EXT {n_live} live + {cap - n_live} dead Gaussians SH3, {W}x{H}, {V} ring views + 1 held-out; "
-                    f"pairs from {events} plain densifications ({warm}-step lead-in, {window}-step window); "
+                    f"pairs from {events} plain densifications ({warm}-step lead-in, {window}-step window) on "
+                    f"{1 if small else len(record_seeds)} recording scene(s); "
                     f"{eval_steps} steps, densification every {adc_period}",
         "pairs": int(len(X)),
+        "eval_steps": eval_steps,
         "record_s": round(rec_s, 2),
         "mlp_train_s": round(train_s, 2),
         "mlp_epochs": epochs,
@@ -503,7 +513,7 @@ def learned_config4(small=False, events=3, warm=100, window=200, eval_steps=800,
 
 def mlp_bench(n=757_052, steps=20, warmup=3, batch=4096, train_steps=20):
     """The densification MLP alone: inference over n synthetic parent-feature rows
-    (the pair count of the learned-densification recording) resident in HBM, and
+    (the pair count of one learned-densification recording scene) resident in HBM, and
     Adam training steps at the recording's batch size; HIP-event times."""
     from nprush_amd import DensifierMLP, densify
 
@@ -578,9 +588,9 @@ def main_learned(args):
         "value": r["time_to_target_ratio"],
         "unit": "ratio (learned / plain training time to plain's final L1)",
         "n_gpus": 1,
-        "steps": 800,
+        "steps": r["eval_steps"],
         "warmup": 0,
-        "ms_per_step": round(r["plain_train_s"] * 1000.0 / 800, 3),
+        "ms_per_step": round(r["plain_train_s"] * 1000.0 / r["eval_steps"], 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
