@@ -289,6 +289,47 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict_
     }
 }
 
+// A 64-bit value that is the same in every lane, as a scalar.  readfirstlane
+// returns int: each half is taken back to uint32_t before widening, so the low
+// half does not sign-extend into the high one.
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t x) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Minimum over y in [lo, hi] of a x^2 + 2 b x y + c y^2 (c > 0, a c >= b^2) for
+// fixed x: the interior minimum (ac - b^2) x^2 / c when -b x / c lies in the
+// interval, else the value at the nearer end.  ic ~ 1/c.
+__device__ __forceinline__ float conic_edge_min(float a, float b, float c, float det, float ic, float x, float lo,
+                                                float hi) {
+    const float t = -b * x;
+    const bool below = t < c * lo, above = t > c * hi;
+    const float y = below ? lo : hi;
+    const float fe = ((a * x) * x + ((2.0f * b) * x) * y) + (c * y) * y;
+    const float fi = ((det * x) * x) * ic;
+    return (below | above) ? fe : fi;
+}
+
+// Whether the conic's level set r^2 <= rc reaches the rectangle [x0,x1]x[y0,y1]
+// (offsets from the centre): the exact minimum of r^2 over the rectangle (0 when
+// it holds the centre, else the least edge minimum) against rc.  The margin
+// 1e-5 (A X^2 + 2|B| X Y + C Y^2), X, Y the largest offsets, exceeds by far the
+// rounding of this minimum and of the per-pixel r^2 (a few FP32 ulps of the
+// same terms), so false means every pixel centre of the rectangle has
+// r^2 > rc as the pixel test computes it.
+__device__ __forceinline__ bool conic_reaches_rect(float A, float B, float C, float x0, float x1, float y0, float y1,
+                                                   float rc) {
+    const float det = A * C - B * B;
+    const float iA = __builtin_amdgcn_rcpf(A), iC = __builtin_amdgcn_rcpf(C);
+    const float m = fminf(fminf(conic_edge_min(A, B, C, det, iC, x0, y0, y1), conic_edge_min(A, B, C, det, iC, x1, y0, y1)),
+                          fminf(conic_edge_min(C, B, A, det, iA, y0, x0, x1), conic_edge_min(C, B, A, det, iA, y1, x0, x1)));
+    const bool inside = (x0 <= 0.0f) & (x1 >= 0.0f) & (y0 <= 0.0f) & (y1 >= 0.0f);
+    const float X = fmaxf(fabsf(x0), fabsf(x1)), Y = fmaxf(fabsf(y0), fabsf(y1));
+    const float S = ((A * X) * X + ((2.0f * fabsf(B)) * X) * Y) + (C * Y) * Y;
+    return inside | !(m > rc + 1e-5f * S);
+}
+
 // Composite of one TILE x TILE tile (one workgroup, one thread per pixel, each
 // wave an 8x8 pixel block), records staged in LDS BATCH at a time.
 //   forward : front to back, alphas of 8 entries at a time (independent), then
@@ -378,8 +419,9 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
     float T = 1.0f, Cr = 0.0f, Cg = 0.0f, Cb = 0.0f;
     int last = 0;
     bool alive = valid;
-    uint32_t n_eval = 0u, n_fw = 0u;
-    int fw_extent = 0;  // entries staged by the forward (uniform)
+    uint32_t n_fw = 0u;  // entries the wave evaluated (uniform)
+    int stop = 0;        // 1 + index of the entry that stopped this pixel (0: not stopped)
+    int fw_extent = 0;   // entries staged by the forward (uniform)
     prime(0, BATCH);
     for (int b0 = 0, bi = 0; b0 < len; b0 += BATCH, ++bi) {
         if (!__syncthreads_or(alive ? 1 : 0)) break;
@@ -388,11 +430,13 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
         put(nb);
         __syncthreads();
         prefetch(b0 + BATCH, b0 + 2 * BATCH);
-        // Entries whose box cannot reach the wave's block: lane j of the wave tests
-        // entry 64w + j.  The margin (1e-5 of the coordinates' magnitude) is far
-        // above the rounding of the pixel test |px - centre| <= extent, so an entry
+        // Entries that cannot reach the wave's block: lane j of the wave tests
+        // entry 64w + j against the block's pixel-centre rectangle, first the
+        // 3-sigma box (the margin, 1e-5 of the coordinates' magnitude, is far
+        // above the rounding of the pixel test |px - centre| <= extent), then the
+        // level set r^2 <= rcut2 outside which the pixel test fails.  An entry
         // left out has alpha 0 on every pixel of the block.
-        uint32_t hit_lo[MW], hit_hi[MW];
+        uint64_t hit[MW];
 #pragma unroll
         for (int w = 0; w < MW; ++w) {
             const int j = 64 * w + lane;
@@ -403,67 +447,78 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
                 const float my = 1e-5f * (fabsf(f.scr_y) + f.ext_y + 4.0f);
                 h = (f.scr_x - f.ext_x <= blk_x1 + mx) && (f.scr_x + f.ext_x >= blk_x0 - mx) &&
                     (f.scr_y - f.ext_y <= blk_y1 + my) && (f.scr_y + f.ext_y >= blk_y0 - my);
+                h = h && conic_reaches_rect(f.A, f.B, f.C, blk_x0 - f.scr_x, blk_x1 - f.scr_x, blk_y0 - f.scr_y,
+                                            blk_y1 - f.scr_y, f.rcut2);
             }
             const uint64_t bal = __ballot(h);
-            hit_lo[w] = __builtin_amdgcn_readfirstlane((uint32_t)bal);
-            hit_hi[w] = __builtin_amdgcn_readfirstlane((uint32_t)(bal >> 32));
+            hit[w] = uniform_u64(bal);
         }
         uint64_t mask[MW];
 #pragma unroll
         for (int w = 0; w < MW; ++w) mask[w] = 0ull;
         float* ck = A.ckpt + (slot0 + bi) * (size_t)(NCH * NT);
-        if (__ballot(alive) != 0ull) {
-            for (int c0 = 0; c0 < nb; c0 += 8) {
-                if (A.backward) ck[(c0 >> 3) * NT + tid] = T;
-                const uint32_t hw = (c0 & 32) ? hit_hi[c0 >> 6] : hit_lo[c0 >> 6];
-                const uint32_t hits = (hw >> (c0 & 31)) & 0xFFu;  // scalar: bit j = entry c0 + j may hit
-                n_fw += (uint32_t)__popc(hits);
-                if (hits == 0u) {  // no alpha anywhere in the block: T unchanged
-                    if (alive) n_eval += (uint32_t)min(8, nb - c0);
-                    continue;
+        // Only the entries that may hit are evaluated, 8 at a time in ascending
+        // order (an entry left out changes no lane's state).  The transmittance
+        // checkpoint of chunk c (entries 8c..8c+7) is T before the first entry
+        // of the chunk, written when the walk first reaches the chunk; chunks
+        // the walk passes over without an entry are written with the same T.
+        int ck_next = 0;  // first chunk of the batch whose checkpoint is not written (uniform)
+        bool wave_alive = __ballot(alive) != 0ull;
+#pragma unroll
+        for (int w = 0; w < MW; ++w) {
+            uint64_t m = hit[w];
+            while (m != 0ull && wave_alive) {
+                int e[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    e[j] = m != 0ull ? 64 * w + (int)__builtin_ctzll(m) : -1;
+                    m &= m - 1ull;
                 }
-                // alphas of the chunk's 8 entries are independent: computed first,
+                // alphas of the group's entries are independent: computed first,
                 // branch-free (selects), so their LDS reads and arithmetic overlap
                 float ca[8];
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
-                    const float4 p0 = rec4[3 * (c0 + j)];      // scr_x scr_y A B
-                    const float4 p1 = rec4[3 * (c0 + j) + 1];  // C op r g
-                    const float4 p2 = rec4[3 * (c0 + j) + 2];  // b rcut2 ext_x ext_y
+                    const int q = e[j] < 0 ? 0 : e[j];
+                    const float4 p0 = rec4[3 * q];      // scr_x scr_y A B
+                    const float4 p1 = rec4[3 * q + 1];  // C op r g
+                    const float4 p2 = rec4[3 * q + 2];  // b rcut2 ext_x ext_y
                     const float dx = pxf - p0.x, dy = pyf - p0.y;
                     const float spx = p0.z * dx + p0.w * dy;
                     const float spy = p0.w * dx + p1.x * dy;
                     const float r2 = dx * spx + dy * spy;
                     // non-short-circuit &: every operand is evaluated, no branch
-                    const bool ok = (((hits >> j) & 1u) != 0u) & (fabsf(dx) <= p2.z) & (fabsf(dy) <= p2.w) &
-                                    !(r2 > p2.y);
+                    const bool ok = (e[j] >= 0) & (fabsf(dx) <= p2.z) & (fabsf(dy) <= p2.w) & !(r2 > p2.y);
                     const float G = exp2_det_in_range(ok ? (-0.5f * r2) * kLog2e : 0.0f);  // r2 <= rcut2 < 12
                     float a = p1.y * G;
                     a = a > kExtMaxAlpha ? kExtMaxAlpha : a;
                     ca[j] = ok ? a : 0.0f;
                 }
-                // then the front-to-back chain, entry by entry (an entry with
-                // alpha 0, e.g. one past the batch, changes nothing)
+                // then the front-to-back chain, entry by entry
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
-                    if (c0 + j < nb && alive) ++n_eval;
+                    if (e[j] < 0) break;  // uniform
+                    ++n_fw;
+                    if (A.backward)
+                        for (; ck_next <= (e[j] >> 3); ++ck_next) ck[ck_next * NT + tid] = T;
                     const float a = ca[j];
                     const bool take = alive & !(a < kExtMinAlpha);
                     const float tT = T * (1.0f - a);
                     const bool dies = take & (tT < kExtTMin);
                     const bool inc = take & !dies;
-                    const float4 p1 = rec4[3 * (c0 + j) + 1];
-                    const float cb = rec4[3 * (c0 + j) + 2].x;
+                    const float4 p1 = rec4[3 * e[j] + 1];
+                    const float cb = rec4[3 * e[j] + 2].x;
                     Cr = inc ? Cr + (T * a) * p1.z : Cr;
                     Cg = inc ? Cg + (T * a) * p1.w : Cg;
                     Cb = inc ? Cb + (T * a) * cb : Cb;
                     T = inc ? tT : T;
-                    last = inc ? b0 + c0 + j + 1 : last;
+                    last = inc ? b0 + e[j] + 1 : last;
+                    stop = dies ? b0 + e[j] + 1 : stop;
                     alive = alive & !dies;
                     const uint64_t any = __ballot(inc) != 0ull ? 1ull : 0ull;
-                    mask[(c0 + j) >> 6] |= any << ((c0 + j) & 63);
+                    mask[e[j] >> 6] |= any << (e[j] & 63);
                 }
-                if (__ballot(alive) == 0ull) break;
+                wave_alive = __ballot(alive) != 0ull;
             }
         }
         if (A.backward && lane < MW) {
@@ -515,26 +570,42 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
         float dT = (dCr * kExtBg + dCg * kExtBg) + dCb * kExtBg;
         const int nbr = (maxlast + BATCH - 1) / BATCH;
         prime((nbr - 1) * BATCH, (nbr - 2) * BATCH);
+        // Global loads the walk waits on are issued ahead: the activity words of a
+        // batch while the previous batch is processed, the checkpoint of the
+        // batch's first active chunk before its records are staged, and that of
+        // each next active chunk while the current one is processed.
+        auto load_act = [&](int b, uint64_t (&m)[MW]) {
+#pragma unroll
+            for (int w = 0; w < MW; ++w) m[w] = b >= 0 ? A.act[((slot0 + b) * NW + wid) * MW + w] : 0ull;
+        };
+        uint64_t act_next[MW];
+        load_act(nbr - 1, act_next);
         for (int bi = nbr - 1; bi >= 0; --bi) {
             const int b0 = bi * BATCH;
             const int nb = min(BATCH, len - b0);
+            uint64_t mask[MW];
+#pragma unroll
+            for (int w = 0; w < MW; ++w)  // the same word in every lane
+                mask[w] = uniform_u64(act_next[w]);
+            load_act(bi - 1, act_next);
+            uint32_t chunks = 0u;  // bit c: chunk c has an active entry (uniform)
+#pragma unroll
+            for (int c = 0; c < NCH; ++c)
+                chunks |= (((uint32_t)(mask[c >> 3] >> (8 * (c & 7))) & 0xFFu) != 0u ? 1u : 0u) << c;
+            const float* ck = A.ckpt + (slot0 + bi) * (size_t)(NCH * NT);
+            int c = chunks ? 31 - __builtin_clz(chunks) : -1;
+            float T_next = c >= 0 ? ck[c * NT + tid] : 1.0f;
             __syncthreads();
             put(nb);
             for (int e = tid; e < NW * BATCH * 9; e += NT) (&s_part[0][0][0])[e] = 0.0f;
             __syncthreads();
             prefetch(b0 - BATCH, b0 - 2 * BATCH);
-            uint64_t mask[MW];
-#pragma unroll
-            for (int w = 0; w < MW; ++w) {
-                const uint64_t m = A.act[((slot0 + bi) * NW + wid) * MW + w];  // the same word in every lane
-                mask[w] = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(m >> 32)) << 32) |
-                          __builtin_amdgcn_readfirstlane((uint32_t)m);
-            }
-            const float* ck = A.ckpt + (slot0 + bi) * (size_t)(NCH * NT);
-            for (int c = NCH - 1; c >= 0; --c) {
+            while (c >= 0) {  // active chunks, last first (uniform)
+                chunks &= ~(1u << c);
+                const int c_next = chunks ? 31 - __builtin_clz(chunks) : -1;
+                float Tc = T_next;
+                if (c_next >= 0) T_next = ck[c_next * NT + tid];
                 const uint32_t bits = (uint32_t)(mask[c >> 3] >> (8 * (c & 7))) & 0xFFu;
-                if (bits == 0u) continue;
-                float Tc = ck[c * NT + tid];
                 float ra[8], rT[8], rG[8];
                 uint32_t incb = 0u;
                 // an entry whose bit is 0 changed no lane's transmittance before
@@ -604,6 +675,7 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
                         if (lane == 0) s_part[wid][jj][8] = t8;
                     }
                 }
+                c = c_next;
             }
             __syncthreads();
             for (int e = tid; e < nb * 9; e += NT) {
@@ -617,7 +689,8 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
     }
     // stats: [0] pairs evaluated by the forward, [2] pairs taken by the reverse
     // pass, [3] reverse wave-entries, [4] forward wave-entries
-    uint32_t ev = n_eval;
+    // pairs evaluated before the pixel stopped (the stopping entry included)
+    uint32_t ev = valid ? (uint32_t)(stop > 0 ? stop : len) : 0u;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) ev += __shfl_xor(ev, o, 64);
     if (lane == 0) {
