@@ -639,6 +639,9 @@ def main():
     ap.add_argument("--no-bezier", action="store_true", help="skip the Bezier stroke workload")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-densification comparison")
     ap.add_argument("--cpu-tiles", type=int, default=48, help="tiles in the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--tile", type=int, default=0,
+                    help="EXT: composite tile edge {8,16,32} instead of the autotuner's choice (with --batch)")
+    ap.add_argument("--batch", type=int, default=0, help="EXT: composite LDS batch {32,64,128} (with --tile)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one plain streaming kernel (k_debug_math, 16M floats: 64 MiB read + 64 MiB "
                          "written) to calibrate profiler byte counters")
@@ -771,14 +774,20 @@ def main_ext(args):
     print(f"[bench] building EXT workload: {n} Gaussians SH{deg}, {W}x{H}, {views} views", file=sys.stderr, flush=True)
     cams, targets, s, sh = build_workload_ext(n, W, H, views, 3.0, deg)
     depth = np.ones((H, W), np.float32)
+    pinned = args.tile > 0 and args.batch > 0
     mgr = GpuSplatOptimizerManager(numSplats=n, mode="ext", sh_degree=deg, splats=s, sh=sh, adc_period=100,
-                                   adc_enabled=False)
+                                   adc_enabled=False, ext_tile=args.tile if pinned else 16,
+                                   ext_batch=args.batch if pinned else 64)
     slots = [mgr.upload_target(t, depth) for t in targets]
-    # composite tile size and LDS batch chosen by the autotuner on view 0
     mgr.writeSplatVPMatrix(*cams[0])
     mgr.set_target_slot(slots[0])
-    tile, batch, tune_ms = mgr.autotune()
-    print(f"[bench] autotune: tile {tile}, batch {batch}", file=sys.stderr, flush=True)
+    if pinned:
+        tile, batch, tune_ms = args.tile, args.batch, None
+        print(f"[bench] composite variant pinned: tile {tile}, batch {batch}", file=sys.stderr, flush=True)
+    else:
+        # composite tile size and LDS batch chosen by the autotuner on view 0
+        tile, batch, tune_ms = mgr.autotune()
+        print(f"[bench] autotune: tile {tile}, batch {batch}", file=sys.stderr, flush=True)
     mgr.enable_timing(True)
 
     def one_step(k):
@@ -858,11 +867,12 @@ def main_ext(args):
         "dtype": "f32",
         "data": "synthetic (seeded scenes, targets rendered from a seed-1 scene)",
         "config": {"workload": f"EXT 3D Gaussian fwd+bwd+Adam, {n} Gaussians, SH degree {deg}, {W}x{H}, "
-                               f"{views} ring views, 16x16 tiles, depth-ordered uncapped lists, L1",
+                               f"{views} ring views, {tile}x{tile} tiles, depth-ordered uncapped lists, L1",
                    "views": views, "width": W, "height": H, "splats": n, "sh_degree": deg,
                    "tile": tile, "lds_batch": batch},
-        "autotune_ms": {f"tile{t}_batch{b}": round(float(tune_ms[i, j]), 3) for i, t in enumerate((8, 16, 32))
-                        for j, b in enumerate((32, 64, 128))},
+        "autotune_ms": None if tune_ms is None else {f"tile{t}_batch{b}": round(float(tune_ms[i, j]), 3)
+                                                     for i, t in enumerate((8, 16, 32))
+                                                     for j, b in enumerate((32, 64, 128))},
         "roofline": {"bound": "mfma", "compute": "fp32 VALU (no matrix work on this path)", "kernel": "k_composite_ext", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": traffic},
