@@ -224,60 +224,84 @@ __device__ __forceinline__ float wave_reduce9(float (&v)[9], int lane, float& t8
 }
 
 // Workgroup order of the composite.  Super-tiles of 2x2 tiles are ranked by
-// their total list length (longest first, ties by index) with a bitonic sort in
-// LDS and dealt round-robin to the 8 XCDs (workgroup b runs on XCD b % 8); the
-// 4 tiles of a super-tile go to the same XCD, where they share splat records in
-// its L2.  order[8 * (4 q + j) + x] = tile j of the super-tile of rank 8 q + x,
-// or ~0 (no tile).  Above kOrderMax super-tiles the ranking is skipped.
+// their total list length, longest first, and dealt round-robin to the 8 XCDs
+// (workgroup b runs on XCD b % 8); the 4 tiles of a super-tile go to the same
+// XCD, where they share splat records in its L2.  order[8 * (4 q + j) + x] =
+// tile j of the super-tile of rank 8 q + x, or ~0 (no tile).  The ranking is a
+// counting sort on the length quantised to 1024 levels (shifted so the longest
+// fits): super-tiles of one level take their places in arrival order, which
+// changes only which workgroup runs which tile, never a result.  Above
+// kOrderMax super-tiles the ranking is skipped.
 constexpr uint32_t kOrderMax = 8192;
+constexpr uint32_t kOrderLevels = 1024;
 
 __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict__ ranges, uint32_t tiles_x,
                                                      uint32_t tiles_y, uint32_t* __restrict__ order) {
-    __shared__ uint64_t s_key[kOrderMax];
+    __shared__ uint32_t s_rank[kOrderMax];  // super-tile of each rank
+    __shared__ uint32_t s_cnt[kOrderLevels];
+    __shared__ uint32_t s_wsum[16];
+    __shared__ uint32_t s_max;
     const uint32_t sx = (tiles_x + 1u) / 2u, sy = (tiles_y + 1u) / 2u, S = sx * sy;
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
     const bool rank = S <= kOrderMax;
-    uint32_t P = 1u;
-    while (P < S) P <<= 1;
     if (rank) {
-        for (uint32_t sidx = tid; sidx < P; sidx += blockDim.x) {
-            uint64_t key = ~0ull;
+        constexpr uint32_t kPer = kOrderMax / 1024u;
+        uint32_t cost[kPer];
+        uint32_t cmax = 0u;
+#pragma unroll
+        for (uint32_t q = 0; q < kPer; ++q) {
+            const uint32_t sidx = tid + 1024u * q;
+            uint32_t c = 0u;
             if (sidx < S) {
                 const uint32_t stx = sidx % sx, sty = sidx / sx;
-                uint32_t cost = 0u;
                 for (uint32_t j = 0; j < 4u; ++j) {
                     const uint32_t tx = 2u * stx + (j & 1u), ty = 2u * sty + (j >> 1);
                     if (tx < tiles_x && ty < tiles_y) {
                         const uint32_t t = ty * tiles_x + tx;
-                        cost += ranges[2 * t + 1] - ranges[2 * t];
+                        c += ranges[2 * t + 1] - ranges[2 * t];
                     }
                 }
-                key = ((uint64_t)(~cost) << 32) | sidx;
             }
-            s_key[sidx] = key;
+            cost[q] = c;
+            cmax = max(cmax, c);
+        }
+        s_cnt[tid] = 0u;
+        if (tid == 0) s_max = 0u;
+        __syncthreads();
+        atomicMax(&s_max, cmax);
+        __syncthreads();
+        uint32_t shift = 0u;
+        while ((s_max >> shift) >= kOrderLevels) ++shift;
+        uint32_t level[kPer];  // 0 = longest
+#pragma unroll
+        for (uint32_t q = 0; q < kPer; ++q) {
+            level[q] = kOrderLevels - 1u - (cost[q] >> shift);
+            if (tid + 1024u * q < S) atomicAdd(&s_cnt[level[q]], 1u);
         }
         __syncthreads();
-        for (uint32_t k = 2u; k <= P; k <<= 1) {
-            for (uint32_t j = k >> 1; j > 0u; j >>= 1) {
-                for (uint32_t i = tid; i < P; i += blockDim.x) {
-                    const uint32_t ixj = i ^ j;
-                    if (ixj > i) {
-                        const uint64_t a = s_key[i], b = s_key[ixj];
-                        const bool up = (i & k) == 0u;
-                        if ((a > b) == up) {
-                            s_key[i] = b;
-                            s_key[ixj] = a;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
+        // exclusive scan of the 1024 level counts (one per thread)
+        const uint32_t v = s_cnt[tid];
+        uint32_t x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if ((int)lane >= o) x += y;
         }
+        if (lane == 63u) s_wsum[wid] = x;
+        __syncthreads();
+        uint32_t base = 0u;
+        for (uint32_t w = 0; w < wid; ++w) base += s_wsum[w];
+        s_cnt[tid] = base + x - v;
+        __syncthreads();
+#pragma unroll
+        for (uint32_t q = 0; q < kPer; ++q)
+            if (tid + 1024u * q < S) s_rank[atomicAdd(&s_cnt[level[q]], 1u)] = tid + 1024u * q;
+        __syncthreads();
     }
     const uint32_t R = (S + 7u) / 8u * 8u;
     for (uint32_t r = tid; r < R; r += blockDim.x) {
         const uint32_t q = r / 8u, x = r % 8u;
-        const uint32_t sidx = r < S ? (rank ? (uint32_t)s_key[r] : r) : ~0u;
+        const uint32_t sidx = r < S ? (rank ? s_rank[r] : r) : ~0u;
         for (uint32_t j = 0; j < 4u; ++j) {
             uint32_t t = ~0u;
             if (sidx != ~0u) {
