@@ -107,10 +107,8 @@ __global__ __launch_bounds__(256) void k_bz_preprocess(const float4* __restrict_
     // tiles whose rectangle passes BW:150-151, as index ranges (the rectangle
     // edges are monotone in the tile index)
     const uint32_t W = v.W, H = v.H, T = (uint32_t)kTile;
-    uint32_t tx0 = first_false(v.tiles_x, [&](uint32_t tx) { return mnx > px_to_x(min(tx * T + T, W), W, aspect); });
-    uint32_t tx_end = first_false(v.tiles_x, [&](uint32_t tx) { return !(mxx < px_to_x(tx * T, W, aspect)); });
-    uint32_t ty0 = first_false(v.tiles_y, [&](uint32_t ty) { return mxy < py_to_y(min(ty * T + T, H), H); });
-    uint32_t ty_end = first_false(v.tiles_y, [&](uint32_t ty) { return !(mny > py_to_y(ty * T, H)); });
+    const uint4 tr = tile_rect(mnx, mxx, mny, mxy, W, H, T, v.tiles_x, v.tiles_y, aspect);
+    const uint32_t tx0 = tr.x, tx_end = tr.y, ty0 = tr.z, ty_end = tr.w;
     if (tx0 >= tx_end || ty0 >= ty_end) return;
     counts[i] = (tx_end - tx0) * (ty_end - ty0);
     rects[i] = make_uint4(tx0, tx_end, ty0, ty_end);

@@ -60,16 +60,8 @@ __global__ __launch_bounds__(256) void k_ext_preprocess(const float4* __restrict
     if (!(sminx == sminx && smaxx == smaxx && sminy == sminy && smaxy == smaxy)) return;
     const uint32_t W = v.W, H = v.H, T = v.tile;
     const float aspect = v.aspect;
-    uint32_t tx0 = first_false(v.tiles_x, [&](uint32_t tx) {
-        uint32_t x1 = min(tx * T + T, W);
-        return sminx > px_to_x(x1, W, aspect);
-    });
-    uint32_t tx_end = first_false(v.tiles_x, [&](uint32_t tx) { return !(smaxx < px_to_x(tx * T, W, aspect)); });
-    uint32_t ty0 = first_false(v.tiles_y, [&](uint32_t ty) {
-        uint32_t y1 = min(ty * T + T, H);
-        return smaxy < py_to_y(y1, H);
-    });
-    uint32_t ty_end = first_false(v.tiles_y, [&](uint32_t ty) { return !(sminy > py_to_y(ty * T, H)); });
+    const uint4 tr = tile_rect(sminx, smaxx, sminy, smaxy, W, H, T, v.tiles_x, v.tiles_y, aspect);
+    const uint32_t tx0 = tr.x, tx_end = tr.y, ty0 = tr.z, ty_end = tr.w;
     if (tx0 >= tx_end || ty0 >= ty_end) return;
     counts[i] = (tx_end - tx0) * (ty_end - ty0);
     rects[i] = make_uint4(tx0, tx_end, ty0, ty_end);

@@ -155,4 +155,74 @@ __device__ __forceinline__ uint32_t first_false(uint32_t n, P pred) {
     return lo;
 }
 
+// first_false(n, pred) started from a guess g: pred at g, then at up to two
+// neighbours towards the answer, narrows [lo, hi] (pred true below lo, false at
+// hi or hi = n); the binary search covers what is left.  Exact for any g; at
+// most three evaluations when g is within one of the answer.
+template <class P>
+__device__ __forceinline__ uint32_t first_false_near(uint32_t n, uint32_t g, P pred) {
+    uint32_t lo = 0, hi = n;
+    g = min(g, n);
+    if (g < n && pred(g)) {
+        lo = g + 1;
+        for (int k = 0; k < 2 && lo < hi; ++k) {
+            if (pred(lo)) {
+                ++lo;
+            } else {
+                hi = lo;
+                break;
+            }
+        }
+    } else {
+        hi = g;
+        for (int k = 0; k < 2 && hi > lo; ++k) {
+            if (pred(hi - 1)) {
+                lo = hi;
+                break;
+            }
+            --hi;
+        }
+    }
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (pred(mid)) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// A tile index guess from a position in tile units: floor(t) + off clamped to
+// [0, n] (NaN and infinities clamp too).
+__device__ __forceinline__ uint32_t tile_guess(float t, int off, uint32_t n) {
+    const float f = fminf(fmaxf(floorf(t), -1.0f), (float)n) + (float)off;
+    return f <= 0.0f ? 0u : (f >= (float)n ? n : (uint32_t)f);
+}
+
+// Tiles [tx0, tx_end) x [ty0, ty_end) whose pixel-centre rectangle the screen box
+// [minx, maxx] x [miny, maxy] overlaps, with the float overlap test of
+// splat_backward.wgsl:148-153: tile tx spans x in [px_to_x(T tx),
+// px_to_x(min(T tx + T, W))], tile ty spans y in [py_to_y(min(T ty + T, H)),
+// py_to_y(T ty)].  Each predicate is monotone in the tile index, so each bound
+// is the first index where it fails; the search starts from that index
+// estimated by inverting px_to_x / py_to_y.
+__device__ __forceinline__ uint4 tile_rect(float minx, float maxx, float miny, float maxy, uint32_t W, uint32_t H,
+                                           uint32_t T, uint32_t tiles_x, uint32_t tiles_y, float aspect) {
+    const float fW = (float)W, fH = (float)H, fT = (float)T;
+    const float gx0 = (((minx / aspect + 1.0f) * 0.5f) * fW - 0.5f) / fT;
+    const float gx1 = (((maxx / aspect + 1.0f) * 0.5f) * fW - 0.5f) / fT;
+    const float gy0 = (((1.0f - maxy) * 0.5f) * fH - 0.5f) / fT;
+    const float gy1 = (((1.0f - miny) * 0.5f) * fH - 0.5f) / fT;
+    const uint32_t tx0 = first_false_near(tiles_x, tile_guess(gx0, 0, tiles_x), [&](uint32_t tx) {
+        return minx > px_to_x(min(tx * T + T, W), W, aspect);
+    });
+    const uint32_t tx_end = first_false_near(tiles_x, tile_guess(gx1, 1, tiles_x),
+                                             [&](uint32_t tx) { return !(maxx < px_to_x(tx * T, W, aspect)); });
+    const uint32_t ty0 = first_false_near(tiles_y, tile_guess(gy0, 0, tiles_y), [&](uint32_t ty) {
+        return maxy < py_to_y(min(ty * T + T, H), H);
+    });
+    const uint32_t ty_end = first_false_near(tiles_y, tile_guess(gy1, 1, tiles_y),
+                                             [&](uint32_t ty) { return !(miny > py_to_y(ty * T, H)); });
+    return make_uint4(tx0, tx_end, ty0, ty_end);
+}
+
 }  // namespace nps
