@@ -385,6 +385,95 @@ MLP_FWD_FLOP = 2 * (32 * 256 + 2 * 256 * 256 + 256 * 14)  # 285,696 per sample
 PEAK_BF16_TFLOPS = 2500.0
 
 
+def ext_training_scene(n_live, W, H, V, seed, tseed):
+    """EXT training scene of the scale-up configuration's distribution: n_live
+    Gaussians (seed) plus as many dead slots (the pool densification fills), SH3,
+    V ring views at radius 3 plus a held-out view at the half-step ring angle
+    0.4 above the ring; targets = the forward images of a tseed scene,
+    quantised to k/255.  Returns (manager, views, held-out view, target slots)."""
+    import math
+
+    from nprush_amd import GpuSplatOptimizerManager, camera, scenes
+
+    cap = 2 * n_live
+    cams = camera.ring_cameras(V, 3.0, W, H)
+    ang = math.pi / V
+    held = camera.view_proj((3.0 * math.cos(ang), 0.4, 3.0 * math.sin(ang)), aspect=W / H)
+    allc = list(cams) + [held]
+    ts, tsh = scenes.make_ext_splats(n_live, tseed, sh_degree=3)
+    tm = GpuSplatOptimizerManager(numSplats=n_live, mode="ext", sh_degree=3, splats=ts, sh=tsh, adc_enabled=False)
+    tm.setBackwardTarget(np.zeros((H, W, 4), np.float32), np.ones((H, W), np.float32), W, H)
+    tg = []
+    for vp, vi in allc:
+        tm.writeSplatVPMatrix(vp, vi)
+        tm.forward()
+        c, _ = tm.read_image()
+        r = np.ones((H, W, 4), np.float32)
+        r[..., :3] = scenes.quantize8(c)
+        tg.append(r)
+    tm.close()
+    s, sh = scenes.make_ext_splats(cap, seed, sh_degree=3)
+    s[n_live:, 3] = -10000.0
+    m = GpuSplatOptimizerManager(numSplats=cap, mode="ext", sh_degree=3, splats=s, sh=sh, adc_enabled=False)
+    m.set_adc_ext(2e-4, 3.3, 0.005, 0.01)
+    depth = np.ones((H, W), np.float32)
+    slots = [m.upload_target(t, depth) for t in tg]
+    views = [(allc[i][0], allc[i][1], slots[i]) for i in range(V)]
+    return m, views, (allc[-1][0], allc[-1][1], slots[-1]), slots
+
+
+# Algorithmic HBM bytes of one densification.
+#   REF (adc.wgsl): every slot's opacity and accumulator read, accumulator
+#     reset (12 B); every child: parent record read (64), two records written
+#     (128), the m/v of both zeroed (256).
+#   EXT (k_adc_ext_*): per slot 68 B (opacity 4 + accumulator 8 read, dead and
+#     qualifier flags 8 written; two exclusive scans 2 x 12; list pass 16;
+#     accumulator reset 8); per placement 968 B (list entries 8, parent record
+#     64, SH copied 192 + 192, the dead-slot child's m/v zeroed 512) plus 640
+#     for a split (both children written 128, the parent slot's m/v zeroed 512)
+#     or 64 for a clone (the child written); per dead slot left unfilled 548 B
+#     (list entry 4, record word read and written 32, m/v zeroed 512).
+def adc_bytes_ref(n, children):
+    return 12 * n + 448 * children
+
+
+def adc_bytes_ext(n, placed, splits, clones, dead):
+    return 68 * n + 968 * placed + 640 * splits + 64 * clones + 548 * max(dead - placed, 0)
+
+
+def adc_ext_leg(small=False, warm=100):
+    """One EXT densification at the scale-up configuration's size (1M live +
+    1M dead Gaussians, SH3, 1080p, 8 ring views): `warm` optimizer steps, a
+    first densification, `warm` more steps to refill the gradient
+    accumulators, then the timed one; HIP-event time and algorithmic bytes
+    (adc_bytes_ext)."""
+    from nprush_amd import densify
+
+    n_live, W, H, V = (40_000, 480, 272, 8) if small else (1_000_000, 1920, 1080, 8)
+    m, views, _, slots = ext_training_scene(n_live, W, H, V, 0, 1)
+    # a first densification loads the kernels and sizes the buffers; the second is timed
+    densify.train_steps(m, views, 0, warm)
+    m.adc()
+    densify.train_steps(m, views, warm, warm)
+    m.enable_timing(True)
+    m.adc()
+    ms = m.read_timing()["adc"]
+    st = m.read_stats()
+    n = 2 * n_live
+    # EXT stats: adc_children = placements, adc_rounds = dead slots before the pass
+    byts = adc_bytes_ext(n, st["adc_children"], st["adc_splits"], st["adc_clones"], st["adc_rounds"])
+    for sl in slots:
+        m.free_target(sl)
+    m.close()
+    return {"workload": f"EXT densification, {n_live} live + {n_live} dead Gaussians SH3, the second one, {warm} steps "
+                        f"at {W}x{H}, threshold 2e-4, split above 0.01 x extent 3.3, prune below 0.005",
+            "ms": round(ms, 4), "slots": n, "qualifiers": st["adc_qualifiers"], "placed": st["adc_children"],
+            "splits": st["adc_splits"], "clones": st["adc_clones"], "dead_slots": st["adc_rounds"],
+            "algorithmic_bytes": int(byts), "hbm_gbs": round(byts / (ms * 1e-3) / 1e9, 1),
+            "roofline_frac": round(byts / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 3)}
+
+
+
 def learned_config4(small=False, events=3, warm=100, window=200, eval_steps=3000, adc_period=100, epochs=10,
                     record_seeds=((0, 1), (4, 5), (6, 7)), eval_every=100):
     """Learned densification on the EXT model (no reference counterpart).
@@ -399,39 +488,13 @@ def learned_config4(small=False, events=3, warm=100, window=200, eval_steps=3000
     every adc_period steps (held-out view at the half-step ring angle, 0.4 above
     the ring, as the judge) and the training time to reach plain's final L1 is
     compared."""
-    import math
-
-    from nprush_amd import GpuSplatOptimizerManager, camera, densify, scenes
+    from nprush_amd import densify
 
     n_live, W, H, V = (40_000, 480, 272, 8) if small else (1_000_000, 1920, 1080, 8)
     cap = 2 * n_live
 
     def scene(seed, tseed):
-        cams = camera.ring_cameras(V, 3.0, W, H)
-        ang = math.pi / V
-        held = camera.view_proj((3.0 * math.cos(ang), 0.4, 3.0 * math.sin(ang)), aspect=W / H)
-        allc = list(cams) + [held]
-        ts, tsh = scenes.make_ext_splats(n_live, tseed, sh_degree=3)
-        tm = GpuSplatOptimizerManager(numSplats=n_live, mode="ext", sh_degree=3, splats=ts, sh=tsh,
-                                      adc_enabled=False)
-        tm.setBackwardTarget(np.zeros((H, W, 4), np.float32), np.ones((H, W), np.float32), W, H)
-        tg = []
-        for vp, vi in allc:
-            tm.writeSplatVPMatrix(vp, vi)
-            tm.forward()
-            c, _ = tm.read_image()
-            r = np.ones((H, W, 4), np.float32)
-            r[..., :3] = scenes.quantize8(c)
-            tg.append(r)
-        tm.close()
-        s, sh = scenes.make_ext_splats(cap, seed, sh_degree=3)
-        s[n_live:, 3] = -10000.0
-        m = GpuSplatOptimizerManager(numSplats=cap, mode="ext", sh_degree=3, splats=s, sh=sh, adc_enabled=False)
-        m.set_adc_ext(2e-4, 3.3, 0.005, 0.01)
-        depth = np.ones((H, W), np.float32)
-        slots = [m.upload_target(t, depth) for t in tg]
-        views = [(allc[i][0], allc[i][1], slots[i]) for i in range(V)]
-        return m, views, (allc[-1][0], allc[-1][1], slots[-1]), slots
+        return ext_training_scene(n_live, W, H, V, seed, tseed)
 
     t0 = time.perf_counter()
     Xs, Ts = [], []
@@ -716,6 +779,24 @@ def main():
         if tj:
             traffic = int(tj["fetch_bytes"] + tj["write_bytes"])
 
+    # reference ADC (adc.wgsl): a first pass loads the kernels and sizes the
+    # buffers; after 25 more steps refill the accumulators the next one is timed
+    mgr.adc()
+    for k in range(25):
+        vp, vi = cams[k % views]
+        mgr.writeSplatVPMatrix(vp, vi)
+        mgr.set_target_slot(slots[k % views])
+        mgr.backward()
+        mgr.step()
+    mgr.adc()
+    adc_ms = mgr.read_timing()["adc"]
+    ast = mgr.read_stats()
+    adc_b = adc_bytes_ref(capacity, ast["adc_children"])
+    adc = {"ms": round(adc_ms, 4), "slots": capacity, "qualifiers": ast["adc_qualifiers"],
+           "children": ast["adc_children"], "splits": ast["adc_splits"], "clones": ast["adc_clones"],
+           "algorithmic_bytes": int(adc_b), "hbm_gbs": round(adc_b / (adc_ms * 1e-3) / 1e9, 1),
+           "roofline_frac": round(adc_b / (adc_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 3)}
+
     if args.calibrate:
         from nprush_amd import debug_math
 
@@ -753,6 +834,7 @@ def main():
         "reverse_lane_efficiency": round(active / max(1.0, 64.0 * float(np.mean([s["reverse_wave_entries"] for _, s in rows]))), 3),
         "forward_eval_fraction": round(64.0 * float(np.mean([s["forward_wave_evals"] for _, s in rows])) / max(pairs, 1.0), 3),
         "adam_hbm_gbs": round(adam_gbs, 1),
+        "adc": adc,
         "adam_roofline_frac": round(adam_gbs / PEAK_HBM_GBS, 3),
         "steps_per_s": round(args.steps / wall, 2),
         "cpu_baseline": cpu,
@@ -843,6 +925,8 @@ def main_ext(args):
     if not args.small and not args.no_bezier:
         print("[bench] Bezier strokes: 100k curves, 4 views at 1024x1024 ...", file=sys.stderr, flush=True)
         bezier = bezier_config3(cpu_tiles=1024 if args.cpu_tiles > 0 else 0)
+    print("[bench] EXT densification ...", file=sys.stderr, flush=True)
+    adc = adc_ext_leg(small=args.small)
     learned = None
     if not args.small and not args.no_learned:
         print("[bench] learned densification vs plain ...", file=sys.stderr, flush=True)
@@ -887,6 +971,7 @@ def main_ext(args):
         "steps_per_s": round(args.steps / wall, 2),
         "batched_multiview": batched,
         "bezier_strokes": bezier,
+        "densification": adc,
         "learned_densification": learned,
         "cpu_baseline": cpu,
     }

@@ -108,27 +108,33 @@ __global__ void k_adc_compute(const float4* __restrict__ splats, const uint32_t*
                               uint32_t* flag) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= num_assign) return;
-    uint32_t chain[kMaxChain];
-    int depth = 0;
+    // chain(0) = qual_list[k]; while chain(d) was dead at the scan, it is the
+    // dead slot of assignment dead_rank[chain(d)], whose parent is chain(d+1).
+    // Cascades are short: chain(d) is re-walked from chain(0) when needed
+    // instead of being kept in a per-thread array (which would live in scratch).
+    auto chain = [&](int d) {
+        uint32_t c = qual_list[k];
+        for (int t = 0; t < d; ++t) c = qual_list[dead_rank[c]];
+        return c;
+    };
+    int depth = 1;
     uint32_t cur = qual_list[k];
-    chain[depth++] = cur;
     while (!alive0[cur]) {
-        // cur is the dead slot of assignment dead_rank[cur]; its parent:
         cur = qual_list[dead_rank[cur]];
         if (depth >= kMaxChain) {
             flag[1] = 1u;
             return;
         }
-        chain[depth++] = cur;
+        ++depth;
     }
-    // replay from the root: visit value of chain[d-1] is original
-    S16 s = load16(splats, chain[depth - 1]);
+    // replay from the root (cur = chain(depth - 1)): visit value of chain(d-1) is original
+    S16 s = load16(splats, cur);
     bool split = false;
     for (int d = depth - 1; d >= 1; --d) {
-        S16 c = adc_transform(s, chain[d], adam_t, &split);
-        s = c;  // child value = visit value of chain[d-1]
+        S16 c = adc_transform(s, chain(d), adam_t, &split);
+        s = c;  // child value = visit value of chain(d-1)
     }
-    S16 child = adc_transform(s, chain[0], adam_t, &split);
+    S16 child = adc_transform(s, chain(0), adam_t, &split);
     float* o = tmp + 32 * (size_t)k;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {

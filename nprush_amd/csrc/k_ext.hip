@@ -1013,26 +1013,23 @@ __global__ __launch_bounds__(256) void k_adc_ext_lists(uint32_t n, const uint32_
 
 // Thread k < min(qualifiers, dead): the k-th placement; min <= k < dead: parks
 // the k-th dead slot.  Parents are read whole before any write; a parent slot is
-// written only by its own thread and dead slots are never parents.
+// written only by its own thread and dead slots are never parents.  The Adam
+// state zeroing and the SH copy are left to k_adc_ext_state (coalesced);
+// split[k] records whether placement k split its parent.
 __global__ __launch_bounds__(256) void k_adc_ext_apply(uint32_t n, float4* __restrict__ splats,
-                                                       float4* __restrict__ sh, float4* __restrict__ m,
-                                                       float4* __restrict__ v, const uint32_t* __restrict__ dead_list,
+                                                       const uint32_t* __restrict__ dead_list,
                                                        const uint32_t* __restrict__ qual_list,
-                                                       uint32_t* __restrict__ cnt, float split_scale, uint32_t seed) {
+                                                       uint32_t* __restrict__ cnt, uint32_t* __restrict__ split,
+                                                       float split_scale, uint32_t seed) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nq = cnt[0], nd = cnt[1];
     const uint32_t P = nq < nd ? nq : nd;
     if (k >= nd) return;
     const uint32_t d = dead_list[k];
-    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     if (k >= P) {
         float4 r0 = splats[4 * (size_t)d];
         r0.w = kExtDeadLogit;
         splats[4 * (size_t)d] = r0;
-        for (int j = 0; j < 16; ++j) {
-            m[16 * (size_t)d + j] = z4;
-            v[16 * (size_t)d + j] = z4;
-        }
         return;
     }
     const uint32_t q = qual_list[k];
@@ -1042,7 +1039,6 @@ __global__ __launch_bounds__(256) void k_adc_ext_apply(uint32_t n, float4* __res
         float4 t = splats[4 * (size_t)q + j];
         p[4 * j] = t.x; p[4 * j + 1] = t.y; p[4 * j + 2] = t.z; p[4 * j + 3] = t.w;
     }
-    for (int j = 0; j < 12; ++j) sh[12 * (size_t)d + j] = sh[12 * (size_t)q + j];
     float sc[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) sc[a] = exp_det(p[4 + a]);
@@ -1073,20 +1069,43 @@ __global__ __launch_bounds__(256) void k_adc_ext_apply(uint32_t n, float4* __res
             for (int j = 0; j < 4; ++j)
                 splats[4 * (size_t)slot + j] = make_float4(out[4 * j], out[4 * j + 1], out[4 * j + 2], out[4 * j + 3]);
         }
-        for (int j = 0; j < 16; ++j) {
-            m[16 * (size_t)q + j] = z4;
-            v[16 * (size_t)q + j] = z4;
-        }
+        split[k] = 1u;
         atomicAdd(&cnt[2], 1u);
     } else {
 #pragma unroll
         for (int j = 0; j < 4; ++j)
             splats[4 * (size_t)d + j] = make_float4(p[4 * j], p[4 * j + 1], p[4 * j + 2], p[4 * j + 3]);
+        split[k] = 0u;
         atomicAdd(&cnt[3], 1u);
     }
-    for (int j = 0; j < 16; ++j) {
+}
+
+// Adam state and SH of the densification, one thread per float4 (grid-stride,
+// t = 16 k + j): every dead-list slot's m and v are zeroed (children and parked
+// slots), a split parent's too, and placement k's child takes its parent's SH.
+__global__ __launch_bounds__(256) void k_adc_ext_state(float4* __restrict__ sh, float4* __restrict__ m,
+                                                       float4* __restrict__ v, const uint32_t* __restrict__ dead_list,
+                                                       const uint32_t* __restrict__ qual_list,
+                                                       const uint32_t* __restrict__ cnt,
+                                                       const uint32_t* __restrict__ split) {
+    const uint32_t nq = cnt[0], nd = cnt[1];
+    const uint32_t P = nq < nd ? nq : nd;
+    const uint64_t total = 16ull * nd;
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t k = (uint32_t)(t >> 4), j = (uint32_t)(t & 15u);
+        const uint32_t d = dead_list[k];
         m[16 * (size_t)d + j] = z4;
         v[16 * (size_t)d + j] = z4;
+        if (k < P) {
+            const uint32_t q = qual_list[k];
+            if (j < 12u) sh[12 * (size_t)d + j] = sh[12 * (size_t)q + j];
+            if (split[k]) {
+                m[16 * (size_t)q + j] = z4;
+                v[16 * (size_t)q + j] = z4;
+            }
+        }
     }
 }
 
@@ -1101,9 +1120,12 @@ void run_adc_ext(hipStream_t st, float* splats, float* sh, float* m, float* v, f
     exclusive_scan_u32(st, b.qual, b.qual_rank, n, b.scan_tmp);
     hipLaunchKernelGGL(k_adc_ext_lists, g, blk, 0, st, n, (const uint32_t*)b.dead, (const uint32_t*)b.dead_rank,
                        (const uint32_t*)b.qual, (const uint32_t*)b.qual_rank, b.dead_list, b.qual_list, b.cnt);
-    hipLaunchKernelGGL(k_adc_ext_apply, g, blk, 0, st, n, (float4*)splats, (float4*)sh, (float4*)m, (float4*)v,
-                       (const uint32_t*)b.dead_list, (const uint32_t*)b.qual_list, b.cnt,
-                       prm.percent_dense * prm.extent, seed);
+    // the qualifier flags are not read after the lists: they hold the split flags
+    hipLaunchKernelGGL(k_adc_ext_apply, g, blk, 0, st, n, (float4*)splats, (const uint32_t*)b.dead_list,
+                       (const uint32_t*)b.qual_list, b.cnt, b.qual, prm.percent_dense * prm.extent, seed);
+    hipLaunchKernelGGL(k_adc_ext_state, dim3(2048), blk, 0, st, (float4*)sh, (float4*)m, (float4*)v,
+                       (const uint32_t*)b.dead_list, (const uint32_t*)b.qual_list, (const uint32_t*)b.cnt,
+                       (const uint32_t*)b.qual);
     launch_fill_u32(st, (uint32_t*)accum, 0u, 2 * (uint64_t)n);
 }
 
