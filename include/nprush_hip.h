@@ -202,10 +202,11 @@ int nps_read_tiles(nps_handle h, uint32_t* ranges, uint32_t* ids, uint32_t* full
 /* Counters of the last backward/ADC: stats[0]=keys, [1]=tiles over the cap,
  * [2]=(pixel, splat) pairs composited, [3]=pairs taken by the reverse pass
  * (REF: alpha >= 0.001; EXT: composited), [4]=ADC qualifiers, [5]=ADC
- * children, [6]=ADC splits, [7]=ADC clones, [8]=ADC cascade rounds (REF) or
- * dead slots (EXT), [9]=wave-entries walked by the reverse pass,
- * [10]=forward wave-entries that evaluate alphas (REF: pow/exp; EXT: entries whose
- * box reaches the wave's 8x8 block), [11]=0. */
+ * children, [6]=ADC splits, [7]=ADC clones, [8]=ADC cascade rounds run (REF;
+ * in groups of 4, the last group's final round changing nothing) or dead slots
+ * (EXT), [9]=wave-entries walked by the reverse pass, [10]=forward wave-entries
+ * that evaluate alphas (REF: pow/exp; EXT: entries that can reach the wave's 8x8
+ * block), [11]=0. */
 int nps_read_stats(nps_handle h, int64_t stats[12]);
 /* EXT mode: per-tile work of the last nps_backward/nps_forward, 2 per tile:
  * entries of the tile list the forward staged before every pixel had stopped,

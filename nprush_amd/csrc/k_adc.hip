@@ -103,28 +103,35 @@ __device__ __forceinline__ S16 adc_transform(S16& s, uint32_t i, float adam_t, b
 }
 
 // One thread per assignment k: parent qual_list[k] -> slot dead_list[k].
-__global__ void k_adc_compute(const float4* __restrict__ splats, const uint32_t* alive0, const uint32_t* dead_rank,
-                              const uint32_t* qual_list, uint32_t num_assign, float adam_t, float* tmp,
-                              uint32_t* flag) {
+constexpr int kChainLds = 32;
+
+__global__ __launch_bounds__(256) void k_adc_compute(const float4* __restrict__ splats, const uint32_t* alive0,
+                                                     const uint32_t* dead_rank, const uint32_t* qual_list,
+                                                     uint32_t num_assign, float adam_t, float* tmp, uint32_t* flag) {
+    __shared__ uint32_t s_chain[kChainLds][256];
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= num_assign) return;
     // chain(0) = qual_list[k]; while chain(d) was dead at the scan, it is the
     // dead slot of assignment dead_rank[chain(d)], whose parent is chain(d+1).
-    // Cascades are short: chain(d) is re-walked from chain(0) when needed
-    // instead of being kept in a per-thread array (which would live in scratch).
+    // The first kChainLds entries are kept in LDS ([d][thread]: no bank
+    // conflicts), deeper ones re-walked from there (a per-thread array indexed
+    // at run time would live in scratch).
     auto chain = [&](int d) {
-        uint32_t c = qual_list[k];
-        for (int t = 0; t < d; ++t) c = qual_list[dead_rank[c]];
+        if (d < kChainLds) return s_chain[d][threadIdx.x];
+        uint32_t c = s_chain[kChainLds - 1][threadIdx.x];
+        for (int t = kChainLds - 1; t < d; ++t) c = qual_list[dead_rank[c]];
         return c;
     };
     int depth = 1;
     uint32_t cur = qual_list[k];
+    s_chain[0][threadIdx.x] = cur;
     while (!alive0[cur]) {
         cur = qual_list[dead_rank[cur]];
         if (depth >= kMaxChain) {
             flag[1] = 1u;
             return;
         }
+        if (depth < kChainLds) s_chain[depth][threadIdx.x] = cur;
         ++depth;
     }
     // replay from the root (cur = chain(depth - 1)): visit value of chain(d-1) is original
@@ -197,16 +204,22 @@ int run_adc_ref(hipStream_t st, float* splats, float* m, float* v, float* accum,
                        b.dead_list);
     uint32_t num_dead = 0;
     hipMemcpyAsync(&num_dead, b.dead_rank + n, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+    // Cascade rounds until one changes nothing.  A round only turns qualifier
+    // flags on, so a round after convergence is a no-op: rounds run in groups
+    // of kCascadeGroup per host readback of the last round's flag.
+    constexpr int kCascadeGroup = 4;
     int rounds = 0;
     for (;;) {
-        exclusive_scan_u32(st, b.Q, b.K, n, b.scan_tmp);
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, b.flag, 1u);
-        hipLaunchKernelGGL(k_adc_cascade, grid, blk, 0, st, (const uint32_t*)b.dead, (const uint32_t*)b.qa,
-                           (const uint32_t*)b.dead_rank, (const uint32_t*)b.K, n, b.Q, b.flag);
+        for (int g = 0; g < kCascadeGroup; ++g) {
+            exclusive_scan_u32(st, b.Q, b.K, n, b.scan_tmp);
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, b.flag, 1u);
+            hipLaunchKernelGGL(k_adc_cascade, grid, blk, 0, st, (const uint32_t*)b.dead, (const uint32_t*)b.qa,
+                               (const uint32_t*)b.dead_rank, (const uint32_t*)b.K, n, b.Q, b.flag);
+        }
         uint32_t changed = 0;
         hipMemcpyAsync(&changed, b.flag, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
         hipStreamSynchronize(st);
-        rounds++;
+        rounds += kCascadeGroup;
         if (!changed) break;
     }
     uint32_t total_q = 0;
