@@ -900,10 +900,11 @@ def main_ext(args):
     achieved = flops / (comp_ms * 1e-3) / 1e12
     phases = {k: round(mean(k), 3) for k in ("preprocess_keys", "sort", "ranges", "composite", "chain", "step")}
     keys = mean("keys", 1)
-    # sort: 8-byte keys read + written per pass; 2 passes over the tile keys
-    # (13 tile bits at 1080p) and 4 over the per-splat (depth, id) keys
-    tile_bits = int(np.ceil(np.log2(((W + 15) // 16) * ((H + 15) // 16))))
-    sort_bytes = 16.0 * keys * int(np.ceil(tile_bits / 8)) + 16.0 * n * 4
+    # tile sort (the "sort" phase): 8-byte keys read + written per pass, 2
+    # passes over the tile keys (13 tile bits at 1080p); the 4 passes of the
+    # per-splat (depth, id) keys run in the preprocess phase and are not counted
+    tile_bits = int(np.ceil(np.log2(((W + tile - 1) // tile) * ((H + tile - 1) // tile))))
+    sort_bytes = 16.0 * keys * int(np.ceil(tile_bits / 8))
     # Adam: per splat 64 parameters x (param + grad + m + v), read and written = 2 KiB
     adam_gbs = 2048.0 * n / (step_ms * 1e-3) / 1e9
     traffic = None
