@@ -316,3 +316,43 @@ def test_ext_autotune(oracle):
     img1, _ = mgr.read_image()
     assert np.array_equal(img1, img0)
     mgr.close()
+
+
+@pytest.mark.parametrize("tile,batch", [(8, 32), (16, 64), (32, 128)])
+def test_ext_block_cull_shapes(oracle, tile, batch):
+    """Shapes that probe the forward's per-wave block test (3-sigma box, then a
+    lower bound of r^2 over the 8x8 block against the alpha cut): needles (axis
+    ratios up to e^4), sub-pixel and tens-of-pixel splats, opacities from just
+    above the 1/255 cut to 0.99, centres spread over sub-pixel offsets.  A block
+    left out wrongly would drop a composited pair: images, the evaluated-pair
+    count and gradients against the oracle."""
+    W, H = 176, 144
+    deg = 1
+    n = 5000
+    rng = np.random.default_rng(23)
+    s, sh = scenes.make_ext_splats(n, 9, sh_degree=deg)
+    s[:, 0:3] *= 0.6
+    ls = rng.uniform(-7.0, -2.5, (n, 3))
+    needle = rng.random(n) < 0.5
+    ls[needle, 1] = ls[needle, 0] - rng.uniform(2.0, 4.0, needle.sum())
+    s[:, 4:7] = ls
+    op = rng.uniform(1.05 / 255.0, 0.99, n)
+    s[:, 3] = np.log(op / (1.0 - op))
+    vp, vi = camera.view_proj((0.3, 0.2, 2.6), aspect=W / H)
+    tgt = np.ones((H, W, 4), np.float32)
+    tgt[..., :3] = rng.random((H, W, 3)).astype(np.float32)
+    mgr = GpuSplatOptimizerManager(numSplats=n, mode="ext", sh_degree=deg, splats=s, sh=sh, adc_enabled=False,
+                                   ext_tile=tile, ext_batch=batch)
+    mgr.writeSplatVPMatrix(vp, vi)
+    mgr.setBackwardTarget(tgt, np.ones((H, W), np.float32), W, H)
+    mgr.backward()
+    # the oracle at the same tile size: evaluated pairs depend on the lists
+    tl, ll, cl = oracle.ext_bin(s, sh, deg, vp, vi, W, H, tile=tile)
+    g, gs, c, loss = oracle.ext_backward_f32(s, sh, deg, vp, vi, tgt, W, H, tl, ll, cl, tile=tile)
+    img, _ = mgr.read_image()
+    assert np.array_equal(img, c)
+    st = mgr.read_stats()
+    assert st["pairs"] == int(loss[2])
+    assert st["active_pairs"] == int(loss[3])       # composited pairs, as the reverse pass takes them
+    _check_grads(mgr.read_grads_f32(), g, gs, deg)
+    mgr.close()
