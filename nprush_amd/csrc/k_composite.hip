@@ -205,27 +205,45 @@ __global__ __launch_bounds__(256) void k_composite_ref(ViewParams v, CompositeAr
     float* ckpt = A.ckpt + (size_t)tile * A.nchunk_max * kTilePx;
 
     // ---------------- forward (BW:216-248) ----------------
+    // An entry whose cut radius misses every pixel centre of the wave's 8x8
+    // block has alpha exactly 0 there (T * (1 - 0) = T, the sums gain +0): lane j
+    // of each wave tests entry j of the batch (conic_reaches_rect against rcut2)
+    // and the wave walks only the entries that can reach it, in order.  The T
+    // checkpoint of chunk c (entries 8c..8c+7) is written when the walk first
+    // reaches the chunk (T is unchanged over the skipped entries); activity words
+    // (alpha >= 0.001 on some pixel of the wave) per 32 entries as before.
     float T = 1.0f, Cr = 0.0f, Cg = 0.0f, Cb = 0.0f, D = 0.0f;
-    uint32_t actword = 0u;  // wave-uniform activity bits of the current 32 entries
     uint32_t n_feval = 0u;  // wave-entries that evaluate pow/exp in the forward
+    int ck_next = 0;        // first chunk whose checkpoint is not written (uniform)
+    const uint32_t bx0 = tx * kTile + (wid & 1) * 8, by0 = ty * kTile + (wid >> 1) * 8;
+    const float blk_x0 = px_to_x(bx0, v.W, v.aspect), blk_x1 = px_to_x(bx0 + 7u, v.W, v.aspect);
+    const float blk_y0 = py_to_y(by0 + 7u, v.H), blk_y1 = py_to_y(by0, v.H);
+    const int ngroups = (count + 31) >> 5;
+    static_assert(kBatch == 64, "one 64-bit hit mask per batch");
     for (int b0 = 0; b0 < count; b0 += kBatch) {
         int nb = min(kBatch, count - b0);
         __syncthreads();
         stage_records(A.fwd, A.bwd, A.keys, start + (uint32_t)b0, nb, s_fwd, s_bwd, s_ids, false);
         __syncthreads();
-        for (int j = 0; j < nb; ++j) {
+        bool h = false;
+        if (lane < nb) {
+            const FwdRec& f = s_fwd[lane];
+            h = conic_reaches_rect(f.A, f.B, f.C, blk_x0 - f.scr_x, blk_x1 - f.scr_x, blk_y0 - f.scr_y,
+                                   blk_y1 - f.scr_y, f.rcut2);
+        }
+        uint64_t hits = uniform_u64(__ballot(h));
+        uint64_t act = 0ull;
+        while (hits != 0ull) {
+            const int j = (int)__builtin_ctzll(hits);
+            hits &= hits - 1ull;
             const int e = b0 + j;
-            if ((e & (kChunk - 1)) == 0) ckpt[(size_t)(e / kChunk) * kTilePx + tid] = T;
+            for (; ck_next <= (e >> 3); ++ck_next) ckpt[(size_t)ck_next * kTilePx + tid] = T;
             const FwdRec& f = s_fwd[j];
             PairR2 q = pair_r2(f, pxf, pyf);
             float l2r, rpa;
             n_feval += (uint32_t)(__ballot(valid && !(q.r2 > f.rcut2)) != 0ull);
             float a = alpha_of(f, q.r2, l2r, rpa);
-            if (__ballot(valid && !(a < 0.001f)) != 0ull) actword |= 1u << (e & 31);
-            if ((e & 31) == 31 || e == count - 1) {
-                if (lane == 0) s_act[wid][e >> 5] = actword;
-                actword = 0u;
-            }
+            act |= (__ballot(valid && !(a < 0.001f)) != 0ull ? 1ull : 0ull) << j;
             float Ta = T * a;
             Cr = Cr + Ta * f.r;
             Cg = Cg + Ta * f.g;
@@ -233,7 +251,12 @@ __global__ __launch_bounds__(256) void k_composite_ref(ViewParams v, CompositeAr
             D = D + Ta * f.depth;
             T = T * (1.0f - a);
         }
+        if (lane == 0) {
+            s_act[wid][b0 >> 5] = (uint32_t)act;
+            if ((b0 >> 5) + 1 < ngroups) s_act[wid][(b0 >> 5) + 1] = (uint32_t)(act >> 32);
+        }
     }
+    for (; ck_next < (count + kChunk - 1) / kChunk; ++ck_next) ckpt[(size_t)ck_next * kTilePx + tid] = T;
     const float bg = 0.05f;
     Cr = Cr + T * bg;
     Cg = Cg + T * bg;

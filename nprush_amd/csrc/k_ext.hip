@@ -305,47 +305,6 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict_
     }
 }
 
-// A 64-bit value that is the same in every lane, as a scalar.  readfirstlane
-// returns int: each half is taken back to uint32_t before widening, so the low
-// half does not sign-extend into the high one.
-__device__ __forceinline__ uint64_t uniform_u64(uint64_t x) {
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(x >> 32));
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// Minimum over y in [lo, hi] of a x^2 + 2 b x y + c y^2 (c > 0, a c >= b^2) for
-// fixed x: the interior minimum (ac - b^2) x^2 / c when -b x / c lies in the
-// interval, else the value at the nearer end.  ic ~ 1/c.
-__device__ __forceinline__ float conic_edge_min(float a, float b, float c, float det, float ic, float x, float lo,
-                                                float hi) {
-    const float t = -b * x;
-    const bool below = t < c * lo, above = t > c * hi;
-    const float y = below ? lo : hi;
-    const float fe = ((a * x) * x + ((2.0f * b) * x) * y) + (c * y) * y;
-    const float fi = ((det * x) * x) * ic;
-    return (below | above) ? fe : fi;
-}
-
-// Whether the conic's level set r^2 <= rc reaches the rectangle [x0,x1]x[y0,y1]
-// (offsets from the centre): the exact minimum of r^2 over the rectangle (0 when
-// it holds the centre, else the least edge minimum) against rc.  The margin
-// 1e-5 (A X^2 + 2|B| X Y + C Y^2), X, Y the largest offsets, exceeds by far the
-// rounding of this minimum and of the per-pixel r^2 (a few FP32 ulps of the
-// same terms), so false means every pixel centre of the rectangle has
-// r^2 > rc as the pixel test computes it.
-__device__ __forceinline__ bool conic_reaches_rect(float A, float B, float C, float x0, float x1, float y0, float y1,
-                                                   float rc) {
-    const float det = A * C - B * B;
-    const float iA = __builtin_amdgcn_rcpf(A), iC = __builtin_amdgcn_rcpf(C);
-    const float m = fminf(fminf(conic_edge_min(A, B, C, det, iC, x0, y0, y1), conic_edge_min(A, B, C, det, iC, x1, y0, y1)),
-                          fminf(conic_edge_min(C, B, A, det, iA, y0, x0, x1), conic_edge_min(C, B, A, det, iA, y1, x0, x1)));
-    const bool inside = (x0 <= 0.0f) & (x1 >= 0.0f) & (y0 <= 0.0f) & (y1 >= 0.0f);
-    const float X = fmaxf(fabsf(x0), fabsf(x1)), Y = fmaxf(fabsf(y0), fabsf(y1));
-    const float S = ((A * X) * X + ((2.0f * fabsf(B)) * X) * Y) + (C * Y) * Y;
-    return inside | !(m > rc + 1e-5f * S);
-}
-
 // Composite of one TILE x TILE tile (one workgroup, one thread per pixel, each
 // wave an 8x8 pixel block), records staged in LDS BATCH at a time.
 //   forward : front to back, alphas of 8 entries at a time (independent), then
