@@ -690,7 +690,9 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
 
 // ---------------------------------------------------------------- chain rule
 
-template <int DEG>
+// kStore: the gradient buffer is known to be all zero (the last Adam step
+// cleared it): the splat's gradients are stored, not read and added.
+template <int DEG, bool kStore>
 __global__ __launch_bounds__(256) void k_ext_chain(const float4* __restrict__ splats, const float4* __restrict__ sh,
                                                    uint32_t n, ViewParams v, float cam0, float cam1, float cam2,
                                                    const double* __restrict__ g2, const uint32_t* __restrict__ counts,
@@ -748,7 +750,9 @@ __global__ __launch_bounds__(256) void k_ext_chain(const float4* __restrict__ sp
 #pragma unroll
         for (int k = 0; k < kNk; ++k) {
             // one term per SH coefficient: written directly
-            grads[64 * (size_t)i + 16 + 3 * k + ch] += (float)(drgb * (double)Y[k]);
+            float& gk = grads[64 * (size_t)i + 16 + 3 * k + ch];
+            if (kStore) gk = (float)(drgb * (double)Y[k]);
+            else gk += (float)(drgb * (double)Y[k]);
             const double coef = (double)c[3 * k + ch];
             gdir[0] += drgb * coef * (double)Yx[k];
             gdir[1] += drgb * coef * (double)Yy[k];
@@ -828,7 +832,7 @@ __global__ __launch_bounds__(256) void k_ext_chain(const float4* __restrict__ sp
     float* out = grads + 64 * (size_t)i;
 #pragma unroll
     for (int j = 0; j < 12; ++j)
-        if (j != 7) out[j] = out[j] + (float)G[j];
+        if (j != 7) out[j] = kStore ? (float)G[j] : out[j] + (float)G[j];
 }
 
 // ---------------------------------------------------------------- Adam
@@ -1290,19 +1294,28 @@ size_t ext_act_words(size_t slots, int tile, int batch) {
     return slots * (size_t)(tile * tile / 64) * (size_t)((batch + 63) / 64);
 }
 
-void launch_ext_chain(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg, const ViewParams& v,
-                      const float cam[3], const double* g2, const uint32_t* counts, float* grads, float* accum) {
-    if (n == 0) return;
+template <bool kStore>
+static void launch_ext_chain_t(hipStream_t st, const float4* s4, const float4* h4, uint32_t n, int deg,
+                               const ViewParams& v, const float cam[3], const double* g2, const uint32_t* counts,
+                               float* grads, float2* a2) {
     dim3 g((n + 255) / 256), b(256);
+    switch (deg) {
+        case 0: hipLaunchKernelGGL((k_ext_chain<0, kStore>), g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2); break;
+        case 1: hipLaunchKernelGGL((k_ext_chain<1, kStore>), g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2); break;
+        case 2: hipLaunchKernelGGL((k_ext_chain<2, kStore>), g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2); break;
+        default: hipLaunchKernelGGL((k_ext_chain<3, kStore>), g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2); break;
+    }
+}
+
+void launch_ext_chain(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg, const ViewParams& v,
+                      const float cam[3], const double* g2, const uint32_t* counts, float* grads, float* accum,
+                      bool grads_zero) {
+    if (n == 0) return;
     const float4* s4 = (const float4*)splats;
     const float4* h4 = (const float4*)sh;
     float2* a2 = (float2*)accum;
-    switch (deg) {
-        case 0: hipLaunchKernelGGL(k_ext_chain<0>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2); break;
-        case 1: hipLaunchKernelGGL(k_ext_chain<1>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2); break;
-        case 2: hipLaunchKernelGGL(k_ext_chain<2>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2); break;
-        default: hipLaunchKernelGGL(k_ext_chain<3>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2); break;
-    }
+    if (grads_zero) launch_ext_chain_t<true>(st, s4, h4, n, deg, v, cam, g2, counts, grads, a2);
+    else launch_ext_chain_t<false>(st, s4, h4, n, deg, v, cam, g2, counts, grads, a2);
 }
 
 void launch_adam_ext(hipStream_t st, float* splats, float* sh, float* grads, float* m, float* v, uint32_t n, int deg,

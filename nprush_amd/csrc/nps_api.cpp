@@ -76,6 +76,7 @@ struct nps_ctx {
     float* accum = nullptr;
     float* sh = nullptr;     // EXT: 48 per splat
     float* gradx = nullptr;  // EXT: 64 FP32 gradients per splat
+    bool gradx_zero = true;  // EXT: gradx is all zero (created cleared, or cleared by the last Adam step)
     float adam_t = 0.0f;
     int64_t step_count = 0;
     AdcExtParams adcx = {2e-4f, 1.0f, 0.005f, 0.01f};  // EXT densification
@@ -163,6 +164,7 @@ struct ViewOut {
     uint32_t* counts = nullptr;  // n (default: the view buffers')
     double* g2 = nullptr;        // 9n (default: the view buffers')
     bool defer_chain = false;
+    bool grads_zero = false;     // the scene's FP32 gradients are known to be zero: the chain stores
 };
 
 // Key-buffer capacity of a captured view with `m` keys measured: 25 % + 64K headroom.
@@ -297,7 +299,7 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
     if (ev && (rc = hip(hipEventRecord(ev[4], st), "event")) != NPS_OK) return rc;
     if (grads && !(out && out->defer_chain))
         launch_ext_chain(st, sc.splats, sc.sh, n, sc.deg, v, cam, (const double*)g2, (const uint32_t*)counts, sc.gradx,
-                         sc.accum);
+                         sc.accum, out && out->grads_zero);
     if (ev && (rc = hip(hipEventRecord(ev[7], st), "event")) != NPS_OK) return rc;
     return hip(hipGetLastError(), "launch");
 }
@@ -353,8 +355,11 @@ int run_ext(nps_ctx* h, bool grads) {
     ViewParams v = make_view(h->vp, h->vp_inv, h->W, h->H, (uint32_t)h->tune.tile);
     float cam[3];
     ext_camera_centre(h->vp_inv, cam);
+    ViewOut out;
+    out.grads_zero = grads && h->gradx_zero;
     int rc = run_ext_view(scene_of(h), v, h->tune.batch, cam, (const float4*)h->tgt_rgba.p, h->xv, h->stream, grads,
-                          false, h->err, nullptr, h->timing ? h->ev : nullptr);
+                          false, h->err, &out, h->timing ? h->ev : nullptr);
+    if (grads) h->gradx_zero = false;
     if (rc != NPS_OK) return rc;
     h->last_tiles = v.num_tiles;
     h->have_backward = true;
@@ -448,6 +453,7 @@ int nps_create(const nps_config* cfg, int32_t num_splats, nps_handle* out) {
         if ((e = hipMalloc((void**)&h->gradx, 64 * n1 * 4)) != hipSuccess) return fail(e);
         hipMemsetAsync(h->sh, 0, 48 * n1 * 4, h->stream);
         hipMemsetAsync(h->gradx, 0, 64 * n1 * 4, h->stream);
+        h->gradx_zero = true;
     } else {
         if ((e = hipMalloc((void**)&h->grads, n16 * 4)) != hipSuccess) return fail(e);
         hipMemsetAsync(h->grads, 0, n16 * 4, h->stream);
@@ -583,6 +589,7 @@ int nps_read_sh(nps_handle h, float* sh, int32_t n) {
     return ext_copy(h, n, h ? h->sh : nullptr, sh, 192, hipMemcpyDeviceToHost);
 }
 int nps_write_grads_f32(nps_handle h, const float* g, int32_t n) {
+    if (h) h->gradx_zero = false;
     return ext_copy(h, n, g, h ? h->gradx : nullptr, 256, hipMemcpyHostToDevice);
 }
 int nps_read_grads_f32(nps_handle h, float* g, int32_t n) {
@@ -728,10 +735,11 @@ int nps_step(nps_handle h) {
     NPS_HIP(h, hipSetDevice(h->device));
     float t = h->adam_t + 1.0f;
     if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[5], h->stream));
-    if (h->ext)
+    if (h->ext) {
         launch_adam_ext(h->stream, h->splats, h->sh, h->gradx, h->m, h->v, h->n, h->deg, (float*)h->adam_dev.p,
                         nullptr, 0);
-    else launch_adam_ref(h->stream, h->splats, h->grads, h->m, h->v, h->accum, h->n, t);
+        h->gradx_zero = true;  // Adam clears every gradient it reads
+    } else launch_adam_ref(h->stream, h->splats, h->grads, h->m, h->v, h->accum, h->n, t);
     if (h->timing) {
         NPS_HIP(h, hipEventRecord(h->ev[6], h->stream));
         NPS_HIP(h, hipEventSynchronize(h->ev[6]));
@@ -1490,6 +1498,8 @@ int npt_step(npt_handle t, int32_t mode) {
     if (!t) return NPS_ERR_ARG;
     if (t->views.empty()) return npt_fail(t, NPS_ERR_STATE, "no views");
     NPT_HIP(t, hipSetDevice(t->device));
+    // the step's chain kernels add into the scenes' gradients (never store)
+    for (nps_handle h : t->scenes) h->gradx_zero = false;
     if (mode == 0) return npt_step_sequential(t);
     if (mode == 1) return npt_step_graph(t);
     return npt_fail(t, NPS_ERR_ARG, "mode must be 0 (per view) or 1 (graph)");
