@@ -622,24 +622,31 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
                         const float4 p1 = rec4[3 * jj + 1];
                         const float4 p2 = rec4[3 * jj + 2];
                         const float fr = p1.z, fg = p1.w, fb = p2.x, fop = p1.y;
+                        // per-pair gradient terms (alpha and T above are the forward's
+                        // values bit for bit; these terms only need FP32 accuracy, so
+                        // they use fused multiply-adds and shared subexpressions):
+                        // with s = dC . rgb, dL/dalpha = T (s - dT) and the seed of
+                        // the transmittance before the entry is dT + alpha (s - dT)
                         const float dx = pxf - p0.x, dy = pyf - p0.y;
-                        const float spx = p0.z * dx + p0.w * dy;
-                        const float spy = p0.w * dx + p1.x * dy;
+                        const float spx = fmaf(p0.z, dx, p0.w * dy);
+                        const float spy = fmaf(p0.w, dx, p1.x * dy);
                         const float Tk = rT[j], a = ra[j], G = rG[j];
-                        const float da = ((dCr * (Tk * fr) + dCg * (Tk * fg)) + dCb * (Tk * fb)) + dT * (-Tk);
-                        const float dTn = dT * (1.0f - a) + ((dCr * (a * fr) + dCg * (a * fg)) + dCb * (a * fb));
-                        dT = inc ? dTn : dT;
+                        const float u = fmaf(dCb, fb, fmaf(dCg, fg, dCr * fr)) - dT;
+                        const float da = Tk * u;
+                        dT = inc ? fmaf(a, u, dT) : dT;
                         const bool geo = inc & !(fop * G > kExtMaxAlpha);  // alpha not clamped
-                        const float gr2 = ((-0.5f * G) * fop) * da;
+                        const float gr2 = ((-0.5f * fop) * G) * da;       // dL / dr^2
+                        const float Tka = Tk * a;
+                        const float m2 = -2.0f * gr2, gx = gr2 * dx;
                         float gv[9];
-                        gv[5] = inc ? dCr * (Tk * a) : 0.0f;
-                        gv[6] = inc ? dCg * (Tk * a) : 0.0f;
-                        gv[7] = inc ? dCb * (Tk * a) : 0.0f;
+                        gv[5] = inc ? dCr * Tka : 0.0f;
+                        gv[6] = inc ? dCg * Tka : 0.0f;
+                        gv[7] = inc ? dCb * Tka : 0.0f;
                         gv[8] = geo ? da * G : 0.0f;
-                        gv[0] = geo ? (-2.0f * gr2) * spx : 0.0f;
-                        gv[1] = geo ? (-2.0f * gr2) * spy : 0.0f;
-                        gv[2] = geo ? (gr2 * dx) * dx : 0.0f;
-                        gv[3] = geo ? ((gr2 * 2.0f) * dx) * dy : 0.0f;
+                        gv[0] = geo ? m2 * spx : 0.0f;
+                        gv[1] = geo ? m2 * spy : 0.0f;
+                        gv[2] = geo ? gx * dx : 0.0f;
+                        gv[3] = geo ? (2.0f * gx) * dy : 0.0f;
                         gv[4] = geo ? (gr2 * dy) * dy : 0.0f;
                         float t8;
                         float tot = wave_reduce9(gv, lane, t8);
