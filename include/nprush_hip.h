@@ -264,6 +264,11 @@ const char* npt_last_error(npt_handle t);
 /* rgba_dev: the view's target, H*W*4 floats on the device (kept by pointer). */
 int npt_add_view(npt_handle t, int32_t scene, const float vp[16], const float vp_inv[16], const float* rgba_dev,
                  int32_t w, int32_t hgt);
+/* One step of every view (fwd+bwd) and Adam per scene.  mode 1: the whole step
+ * as one captured graph over the lanes; mode 0: view by view, each launched on
+ * its scene's stream after its key count is read back (views of different
+ * scenes overlap on the GPU); mode 2: view by view, each completed before the
+ * next is launched (strictly sequential).  Same results in every mode. */
 int npt_step(npt_handle t, int32_t mode);
 /* Autotunes (as nps_autotune) one view of every (scene, resolution) class and
  * uses its choice for the whole class; views start with their scene's setting.

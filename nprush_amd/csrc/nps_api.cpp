@@ -1234,7 +1234,8 @@ int npt_finish_scenes(npt_handle t, bool adam_done) {
 }
 
 // Views one after another, each with its own launches and key-count readback.
-int npt_step_sequential(npt_handle t) {
+// serial: each view completes before the next one is launched.
+int npt_step_sequential(npt_handle t, bool serial) {
     for (nps_handle h : t->scenes) NPT_HIP(t, hipStreamSynchronize(h->stream));
     for (int vi : t->order) {
         const npt_view& w = t->views[vi];
@@ -1247,6 +1248,7 @@ int npt_step_sequential(npt_handle t) {
         if (rc != NPS_OK) return rc;
         launch_sum_tile_loss(h->stream, (const double*)h->xv.tile_loss.p, v.num_tiles,
                              (double*)t->view_loss.p + 2 * vi);
+        if (serial) NPT_HIP(t, hipStreamSynchronize(h->stream));
     }
     return npt_finish_scenes(t, false);
 }
@@ -1390,7 +1392,7 @@ int npt_step_graph(npt_handle t) {
         }
         t->cap_scale *= 1.5;
         npt_drop_graph(t);
-        return npt_step_sequential(t);
+        return npt_step_sequential(t, false);
     }
     return npt_finish_scenes(t, true);
 }
@@ -1500,9 +1502,9 @@ int npt_step(npt_handle t, int32_t mode) {
     NPT_HIP(t, hipSetDevice(t->device));
     // the step's chain kernels add into the scenes' gradients (never store)
     for (nps_handle h : t->scenes) h->gradx_zero = false;
-    if (mode == 0) return npt_step_sequential(t);
+    if (mode == 0 || mode == 2) return npt_step_sequential(t, mode == 2);
     if (mode == 1) return npt_step_graph(t);
-    return npt_fail(t, NPS_ERR_ARG, "mode must be 0 (per view) or 1 (graph)");
+    return npt_fail(t, NPS_ERR_ARG, "mode must be 0 (per view), 1 (graph) or 2 (per view, serial)");
 }
 
 int npt_autotune(npt_handle t) {

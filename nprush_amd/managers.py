@@ -432,8 +432,10 @@ class GpuMultiViewStep:
         self._check(self._lib.npt_add_view(self._t, int(scene), fptr(a), fptr(b), p, int(w), int(h)), "add_view")
         self.num_views += 1
 
-    def step(self, graph: bool = True):
-        self._check(self._lib.npt_step(self._t, 1 if graph else 0), "step")
+    def step(self, graph: bool = True, serial: bool = False):
+        """graph: the whole step as one captured graph; otherwise view by view,
+        overlapping across scenes, or (serial) each view completed before the next."""
+        self._check(self._lib.npt_step(self._t, 1 if graph else (2 if serial else 0)), "step")
 
     def autotune(self) -> np.ndarray:
         """Chooses tile size and LDS batch per (scene, resolution); returns [V, 2]."""

@@ -95,3 +95,26 @@ def test_autotuned_graph_step_matches_per_view():
     sa.close(); sb.close()
     for mgr in ma + mb:
         mgr.close()
+
+
+def test_serial_per_view_step_equals_per_view_step():
+    """Mode 2 (each view completed before the next is launched) gives the same
+    losses, splats, SH and Adam state as mode 0 (views of different scenes
+    overlapping), bit for bit, over 3 steps with densification every 2."""
+    data = [_scene(3000, 4, 3600), _scene(5000, 5, 6000)]
+    views = _views(5)
+    ma, sa = _build(views, data)
+    mb, sb = _build(views, data)
+    for k in range(3):
+        sa.step(graph=False)
+        sb.step(graph=False, serial=True)
+        assert np.array_equal(sa.read_losses(), sb.read_losses()), k
+        for x, y in zip(ma, mb):
+            assert np.array_equal(x.read_splats(), y.read_splats()), k
+            assert np.array_equal(x.read_sh(), y.read_sh()), k
+            xm, xv, xt = x.read_adam()
+            ym, yv, yt = y.read_adam()
+            assert np.array_equal(xm, ym) and np.array_equal(xv, yv) and xt == yt, k
+    sa.close(); sb.close()
+    for mgr in ma + mb:
+        mgr.close()
