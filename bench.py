@@ -313,6 +313,23 @@ def bezier_config3(steps=20, warmup=3, cpu_tiles=16, n=100_000, W=1024, H=1024, 
     m.adc()  # one densification pass (every 50th dispatch), timed on its own
     adc_ms = m.read_timing()["adc"]
     st_adc = m.read_stats()
+    # display raster of the trained curves (GpuBezierForwardPipelineManager.dispatch,
+    # 8-bit target), view 0, synthetic 256 x 256 brush; wall clock per dispatch,
+    # the key-count readback included
+    from nprush_amd import GpuBezierForwardPipelineManager
+
+    fm = GpuBezierForwardPipelineManager(numBeziers=n, bezierBuffer=m, brushTexture=scenes.make_brush())
+    fm.writeVPMatrix(vp)
+    fm.setTarget(None, W, H)
+    for _ in range(2):
+        fm.dispatch(clear=True, quantize=True)
+    m.sync()
+    t1 = time.perf_counter()
+    for _ in range(5):
+        fm.dispatch(clear=True, quantize=True)
+    m.sync()
+    draw_ms = (time.perf_counter() - t1) * 1000.0 / 5
+    fm.close()
 
     def mean(key, src=0):
         return float(np.mean([r[src][key] for r in rows]))
@@ -368,6 +385,7 @@ def bezier_config3(steps=20, warmup=3, cpu_tiles=16, n=100_000, W=1024, H=1024, 
         "steps_per_s": round(steps / wall, 2),
         "densification_ms": round(adc_ms, 3),
         "densification_ms_per_step": round(adc_ms / 50.0, 3),
+        "display_raster_ms": round(draw_ms, 3),
         "phases_ms": {k: round(mean(k), 3) for k in ("preprocess", "keys", "sort", "composite", "step")},
         "tile_keys_per_view": int(mean("keys", 1)),
         "pairs_per_view": int(mean("pairs", 1)),

@@ -380,6 +380,29 @@ int nbz_dev_alloc(nbz_handle h, int64_t bytes, void** out);
 int nbz_dev_free(nbz_handle h, void* p);
 int nbz_dev_upload(nbz_handle h, void* dst, const void* src, int64_t bytes);
 
+/*
+ * new GpuBezierForwardPipelineManager({device, numBeziers, bezierBuffer,
+ * brushTexture}) (GpuBezierForwardPipelineManager.ts:17-98): the display raster
+ * of bezier_forward.wgsl, drawn from the optimizer handle's curve buffer (the
+ * reference's manager borrows bezierBuffer).  brush_rgba8: bh rows of bw rgba8
+ * texels (the brush texture; only its red channel is sampled), copied at
+ * creation.  Errors are reported through nbz_last_error(owner).
+ */
+typedef struct nbf_ctx* nbf_handle;
+int nbf_create(nbz_handle curves_owner, const uint8_t* brush_rgba8, int32_t bw, int32_t bh, nbf_handle* out);
+void nbf_destroy(nbf_handle f);
+/* writeVPMatrix(mat) (:100-108) */
+int nbf_write_vp(nbf_handle f, const float vp[16]);
+/* setTarget(targetView, width, height) (:110-129): the rgba8 target lives in the
+ * handle (H*W*4 floats). */
+int nbf_set_target(nbf_handle f, int32_t w, int32_t hgt);
+/* dispatch(encoder, clear = true) (:131-148): clear sets the target to 0 first;
+ * quantize != 0 rounds every channel to k/255 after each blend, as the
+ * rgba8unorm target does. */
+int nbf_dispatch(nbf_handle f, int32_t clear, int32_t quantize);
+int nbf_read_target(nbf_handle f, float* rgba);
+int nbf_write_target(nbf_handle f, const float* rgba);
+
 /* ------------------------------------------------------ densification MLP
  * No reference counterpart (the reference's densification is heuristic only).
  * MLP 32 -> 256 -> 256 -> 256 -> 14 with GELU (tanh form) proposing the two

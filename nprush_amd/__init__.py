@@ -7,6 +7,7 @@ reference's GpuSplatOptimizerManager / GpuSplatForwardPipelineManager.
 from . import camera, scenes  # noqa: F401
 from .managers import (  # noqa: F401
     DensifierMLP,
+    GpuBezierForwardPipelineManager,
     GpuBezierOptimizerManager,
     GpuBlurPipelineManager,
     GpuMultiViewStep,

@@ -159,6 +159,13 @@ SIGNATURES = {
     "nbz_dev_alloc": (i32, [_P, C.c_int64, C.POINTER(_P)]),
     "nbz_dev_free": (i32, [_P, _P]),
     "nbz_dev_upload": (i32, [_P, _P, _P, C.c_int64]),
+    "nbf_create": (i32, [_P, _P, i32, i32, C.POINTER(_P)]),
+    "nbf_destroy": (None, [_P]),
+    "nbf_write_vp": (i32, [_P, _F]),
+    "nbf_set_target": (i32, [_P, i32, i32]),
+    "nbf_dispatch": (i32, [_P, i32, i32]),
+    "nbf_read_target": (i32, [_P, _F]),
+    "nbf_write_target": (i32, [_P, _F]),
     "nbz_enable_timing": (i32, [_P, i32]),
     "nbz_read_timing": (i32, [_P, _F]),
 }

@@ -1144,4 +1144,246 @@ void launch_bz_adc(hipStream_t st, float* curves, float* m, float* v, float* gra
                        W, H, A, Bi, prm, adam_t, b);
 }
 
+// ------------------------------------------------------------ display raster
+// bezier_forward.wgsl per curve (:47-112): projection, the near cull (any control
+// point at w <= 0, :74-81), the quad (hull of the control points -+ width +
+// softness, clamped to [-4, 4], :88-96) and its tiles, and the polyline points
+// the fragment shader recomputes per pixel (:133-136).  A curve with opacity
+// below 0.0009 is dropped: its alpha, a_geom (<= 1) * brush (<= 1 up to a few
+// ulps) * opacity * pressure (<= 1), stays below 0.001 on every pixel, where
+// the fragment is discarded.
+__global__ __launch_bounds__(256) void k_bz_draw_preprocess(const float4* __restrict__ curves, uint32_t n,
+                                                            ViewParams v, BzDraw* __restrict__ rec,
+                                                            uint32_t* __restrict__ counts,
+                                                            uint4* __restrict__ rects) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 c[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) c[k] = curves[5 * (size_t)i + k];
+    const float4 col = curves[5 * (size_t)i + 4];
+    counts[i] = 0u;
+    if (col.w < 0.0009f) return;
+    const float aspect = v.aspect;
+    F4 k4[4];
+    V2 p[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        k4[k] = mat_vec(v.vp, c[k].x, c[k].y, c[k].z, 1.0f);
+        p[k].x = k4[k].x / k4[k].w * aspect;  // project_to_screen (:36-40)
+        p[k].y = k4[k].y / k4[k].w;
+    }
+    if (k4[0].w <= 0.0f || k4[1].w <= 0.0f || k4[2].w <= 0.0f || k4[3].w <= 0.0f) return;
+    BzDraw f;
+    f.width = fmax_w(c[0].w, 0.0001f);
+    f.soft = fmax_w(c[1].w, 0.0001f);
+    const float pad = f.width + f.soft;
+    f.qx0 = fmax_w(fmin_w(fmin_w(p[0].x, p[1].x), fmin_w(p[2].x, p[3].x)) - pad, -4.0f);
+    f.qy0 = fmax_w(fmin_w(fmin_w(p[0].y, p[1].y), fmin_w(p[2].y, p[3].y)) - pad, -4.0f);
+    f.qx1 = fmin_w(fmax_w(fmax_w(p[0].x, p[1].x), fmax_w(p[2].x, p[3].x)) + pad, 4.0f);
+    f.qy1 = fmin_w(fmax_w(fmax_w(p[0].y, p[1].y), fmax_w(p[2].y, p[3].y)) + pad, 4.0f);
+    // every tile holding a pixel centre inside the quad
+    const uint4 tr = tile_rect(f.qx0, f.qx1, f.qy0, f.qy1, v.W, v.H, (uint32_t)kTile, v.tiles_x, v.tiles_y, aspect);
+    if (tr.x >= tr.y || tr.z >= tr.w) return;
+    counts[i] = (tr.y - tr.x) * (tr.w - tr.z);
+    rects[i] = tr;
+    f.x[0] = p[0].x;
+    f.y[0] = p[0].y;
+#pragma unroll
+    for (int k = 1; k <= kBzSeg; ++k) {
+        V2 q = bz_at(p[0], p[1], p[2], p[3], (float)k / (float)kBzSeg);
+        f.x[k] = q.x;
+        f.y[k] = q.y;
+    }
+    float bx0 = f.x[0], bx1 = f.x[0], by0 = f.y[0], by1 = f.y[0];
+#pragma unroll
+    for (int k = 1; k <= kBzSeg; ++k) {
+        bx0 = fmin_w(bx0, f.x[k]);
+        bx1 = fmax_w(bx1, f.x[k]);
+        by0 = fmin_w(by0, f.y[k]);
+        by1 = fmax_w(by1, f.y[k]);
+    }
+    f.op = col.w;
+    f.r = col.x;
+    f.g = col.y;
+    f.b = col.z;
+    f.cx0 = bx0;
+    f.cy0 = by0;
+    f.cx1 = bx1;
+    f.cy1 = by1;
+    // as in training: beyond rcull from the polyline box every computed distance
+    // exceeds width + softness, where alpha is 0
+    const float M = fmax_w(fmax_w(fabsf(bx0), fabsf(bx1)), fmax_w(fabsf(by0), fabsf(by1)));
+    f.rcull = (f.width + f.soft) * 1.0001f + 1e-5f * (4.0f + M);
+    f.pad[0] = f.pad[1] = f.pad[2] = 0.0f;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        float gx0 = f.x[4 * g], gx1 = f.x[4 * g], gy0 = f.y[4 * g], gy1 = f.y[4 * g];
+#pragma unroll
+        for (int j = 1; j <= 4; ++j) {
+            gx0 = fmin_w(gx0, f.x[4 * g + j]);
+            gx1 = fmax_w(gx1, f.x[4 * g + j]);
+            gy0 = fmin_w(gy0, f.y[4 * g + j]);
+            gy1 = fmax_w(gy1, f.y[4 * g + j]);
+        }
+        f.gb[4 * g + 0] = gx0;
+        f.gb[4 * g + 1] = gy0;
+        f.gb[4 * g + 2] = gx1;
+        f.gb[4 * g + 3] = gy1;
+    }
+    rec[i] = f;
+}
+
+// The brush sampler (linear filtering, repeat in u, clamp-to-edge in v, one mip
+// level; GpuBezierForwardPipelineManager.ts:34-41) as FP32 bilinear filtering
+// of the texel values at x = u*bw - 0.5, y = v*bh - 0.5.
+__device__ __forceinline__ float bz_brush(const float* __restrict__ tex, int bw, int bh, float u, float v) {
+    const float x = u * (float)bw - 0.5f, y = v * (float)bh - 0.5f;
+    const float x0f = floorf(x), y0f = floorf(y);
+    const float fx = x - x0f, fy = y - y0f;
+    const int ix = (int)x0f, iy = (int)y0f;
+    const int x0 = ((ix % bw) + bw) % bw, x1 = (((ix + 1) % bw) + bw) % bw;
+    const int y0 = min(max(iy, 0), bh - 1), y1 = min(max(iy + 1, 0), bh - 1);
+    const float t00 = tex[(size_t)y0 * bw + x0], t10 = tex[(size_t)y0 * bw + x1];
+    const float t01 = tex[(size_t)y1 * bw + x0], t11 = tex[(size_t)y1 * bw + x1];
+    const float top = t00 * (1.0f - fx) + t10 * fx;
+    const float bot = t01 * (1.0f - fx) + t11 * fx;
+    return top * (1.0f - fy) + bot * fy;
+}
+
+// fs_main (:115-182) for one pixel: alpha, 0 where the pixel is outside the quad.
+// The closest segment is found as in training (bz_eval: skipped segment groups,
+// square roots only for a possible new minimum, divisions only where the clamp
+// does not decide u), then its signed cross distance is recomputed from the
+// same operands.  A minimum >= fl(width + softness) gives a_geom = 0 and alpha 0.
+__device__ __forceinline__ float bz_draw_alpha(const BzDraw& f, float px, float py, const BzBlock& bk, float thr2,
+                                               const float* __restrict__ brush, int bw, int bh) {
+    const bool cov = (px >= f.qx0) & (px <= f.qx1) & (py >= f.qy0) & (py <= f.qy1);
+    float min_d = 1e9f, min_d2 = __builtin_inff(), min_u = 0.0f;
+    uint32_t min_k = 1u;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const float gx = __builtin_fmaxf(__builtin_fmaxf(f.gb[4 * g + 0] - bk.xb, bk.xa - f.gb[4 * g + 2]), 0.0f);
+        const float gy = __builtin_fmaxf(__builtin_fmaxf(f.gb[4 * g + 1] - bk.yb, bk.ya - f.gb[4 * g + 3]), 0.0f);
+        if (__builtin_amdgcn_readfirstlane((int)(gx * gx + gy * gy > thr2))) continue;
+        float prevx = f.x[4 * g], prevy = f.y[4 * g];
+#pragma unroll
+        for (int k = 4 * g + 1; k <= 4 * g + 4; ++k) {
+            const float cx = f.x[k], cy = f.y[k];
+            const float sx = cx - prevx, sy = cy - prevy;
+            const float len2 = fmax_w(sx * sx + sy * sy, 1e-8f);
+            const float qx = px - prevx, qy = py - prevy;
+            const float num = qx * sx + qy * sy;
+            float u;
+            if (num <= 0.0f) u = 0.0f;
+            else if (num >= len2 && len2 < __builtin_inff()) u = 1.0f;
+            else u = clamp_w(num / len2, 0.0f, 1.0f);
+            const float ox = prevx + sx * u, oy = prevy + sy * u;
+            const float dx = px - ox, dy = py - oy;
+            const float d2 = dx * dx + dy * dy;
+            if (d2 < min_d2) {
+                const float d = sqrtf(d2);
+                if (d < min_d) {
+                    min_d = d;
+                    min_d2 = d2;
+                    min_k = (uint32_t)k;
+                    min_u = u;
+                }
+            }
+            prevx = cx;
+            prevy = cy;
+        }
+    }
+    if (!cov || min_d >= f.width + f.soft) return 0.0f;
+    const float prevx = f.x[min_k - 1u], prevy = f.y[min_k - 1u];
+    const float sx = f.x[min_k] - prevx, sy = f.y[min_k] - prevy;
+    const float len2 = fmax_w(sx * sx + sy * sy, 1e-8f);
+    const float dx = px - (prevx + sx * min_u), dy = py - (prevy + sy * min_u);
+    const float seg_len = sqrtf(len2);
+    const float dirx = sx / seg_len, diry = sy / seg_len;
+    const float cross = dx * (-diry) + dy * dirx;
+    const float t = ((float)(min_k - 1u) + min_u) / (float)kBzSeg;
+    const float dt = t - 0.5f;
+    const float pressure = 1.0f - 4.0f * dt * dt;
+    const float lw = f.width * pressure, ls = f.soft * pressure, lo = f.op * pressure;
+    const float a_geom = 1.0f - bz_smoothstep(lw - ls, lw + ls, min_d);
+    const float bv = clamp_w(cross / fmax_w(lw + ls, 1e-6f) * 0.5f + 0.5f, 0.0f, 1.0f);
+    const float br = bz_brush(brush, bw, bh, t, bv);
+    return clamp_w(a_geom * br * lo, 0.0f, 0.999f);
+}
+
+__device__ __forceinline__ float bz_q8(float x) { return rintf(clamp_w(x, 0.0f, 1.0f) * 255.0f) / 255.0f; }
+
+constexpr int kDrawBatch = 32;
+
+// The instanced draw (:47-182) with its blend state (premultiplied output,
+// one / one-minus-src-alpha, GpuBezierForwardPipelineManager.ts:79-92) as a
+// tiled pass: one 256-thread workgroup per 16x16 tile, each wave an 8x8 block,
+// the tile's curves walked from the highest id down (instance ii draws curve
+// N-1-ii), records staged in LDS 32 at a time.  An entry whose polyline box is
+// farther than the cull radius from the wave's block, or whose quad misses the
+// block, draws nothing there and is skipped by the whole wave.
+__global__ __launch_bounds__(256) void k_bz_draw(ViewParams v, BzDrawArgs A) {
+    __shared__ BzDraw s_f[kDrawBatch];
+    const uint32_t tile = xcd_remap(blockIdx.x, v.num_tiles);
+    const uint32_t tx = tile % v.tiles_x, ty = tile / v.tiles_x;
+    const uint32_t start = A.ranges[2 * tile], end = A.ranges[2 * tile + 1];
+    const int count = (int)(end - start);
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const uint32_t bx = tx * kTile + (wid & 1) * 8, by = ty * kTile + (wid >> 1) * 8;
+    const uint32_t px = bx + (lane & 7), py = by + (lane >> 3);
+    const bool valid = px < v.W && py < v.H;
+    const float pxf = px_to_x(px, v.W, v.aspect);
+    const float pyf = py_to_y(py, v.H);
+    const float xa = px_to_x(bx, v.W, v.aspect), xb = px_to_x(bx + 7, v.W, v.aspect);
+    const float ya = py_to_y(by + 7, v.H), yb = py_to_y(by, v.H);
+    const float blk_margin = 1e-5f * (((fabsf(xa) + fabsf(xb)) + fabsf(ya)) + fabsf(yb));
+    const BzBlock blk = {xa, xb, ya, yb};
+    const size_t pix = valid ? (size_t)py * v.W + px : 0;
+    float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!A.clear && valid) c = A.rgba[pix];
+    constexpr int FQ = (int)(sizeof(BzDraw) / 16);
+    for (int b0 = count > 0 ? ((count - 1) / kDrawBatch) * kDrawBatch : -1; b0 >= 0; b0 -= kDrawBatch) {
+        const int nb = min(kDrawBatch, count - b0);
+        __syncthreads();
+        for (int e = tid; e < nb * FQ; e += 256) {
+            const int j = e / FQ, part = e - j * FQ;
+            const uint32_t id = (uint32_t)A.keys[start + (uint32_t)(b0 + j)];
+            reinterpret_cast<float4*>(&s_f[j])[part] = reinterpret_cast<const float4*>(&A.rec[id])[part];
+        }
+        __syncthreads();
+        for (int j = nb - 1; j >= 0; --j) {
+            const BzDraw& f = s_f[j];
+            const float thr = f.rcull + blk_margin;
+            const float thr2 = thr * thr;
+            const float gx = __builtin_fmaxf(__builtin_fmaxf(f.cx0 - xb, xa - f.cx1), 0.0f);
+            const float gy = __builtin_fmaxf(__builtin_fmaxf(f.cy0 - yb, ya - f.cy1), 0.0f);
+            const bool miss = (gx * gx + gy * gy > thr2) | (f.qx0 > xb) | (f.qx1 < xa) | (f.qy0 > yb) | (f.qy1 < ya);
+            if (__builtin_amdgcn_readfirstlane((int)miss)) continue;
+            const float a = bz_draw_alpha(f, pxf, pyf, blk, thr2, A.brush, A.bw, A.bh);
+            if (valid && !(a < 0.001f)) {
+                const float one_m = 1.0f - a;
+                c.x = f.r * a + c.x * one_m;
+                c.y = f.g * a + c.y * one_m;
+                c.z = f.b * a + c.z * one_m;
+                c.w = a + c.w * one_m;
+                if (A.quantize) c = make_float4(bz_q8(c.x), bz_q8(c.y), bz_q8(c.z), bz_q8(c.w));
+            }
+        }
+    }
+    if (valid) A.rgba[pix] = c;
+}
+
+void launch_bz_draw_preprocess(hipStream_t st, const float* curves, uint32_t n, const ViewParams& v, BzDraw* rec,
+                               uint32_t* counts, uint4* rects) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_bz_draw_preprocess, dim3((n + 255) / 256), dim3(256), 0, st, (const float4*)curves, n, v,
+                       rec, counts, rects);
+}
+
+void launch_bz_draw(hipStream_t st, const ViewParams& v, const BzDrawArgs& a) {
+    if (v.num_tiles == 0) return;
+    hipLaunchKernelGGL(k_bz_draw, dim3(v.num_tiles), dim3(256), 0, st, v, a);
+}
+
 }  // namespace nps

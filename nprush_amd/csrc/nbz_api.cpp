@@ -564,3 +564,113 @@ int nbz_dev_upload(nbz_handle h, void* dst, const void* src, int64_t bytes) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------ display raster (nbf_*)
+
+struct nbf_ctx {
+    nbz_ctx* owner = nullptr;
+    float vp[16] = {0};
+    bool has_vp = false;
+    uint32_t W = 0, H = 0;
+    int32_t bw = 0, bh = 0;
+    DevBuf rgba, rec, brush;
+    Scratch sc;
+};
+
+extern "C" {
+
+int nbf_create(nbz_handle owner, const uint8_t* brush_rgba8, int32_t bw, int32_t bh, nbf_handle* out) {
+    if (!owner || !out) return NPS_ERR_ARG;
+    *out = nullptr;
+    if (!brush_rgba8 || bw <= 0 || bh <= 0) return bz_err(owner, NPS_ERR_ARG, "brush texture missing or empty");
+    NBZ_HIP(owner, hipSetDevice(owner->device));
+    nbf_ctx* f = new nbf_ctx();
+    f->owner = owner;
+    // texel values of the red channel as an rgba8unorm texture returns them
+    std::vector<float> tex((size_t)bw * bh);
+    for (size_t i = 0; i < tex.size(); ++i) tex[i] = (float)brush_rgba8[4 * i] / 255.0f;
+    int rc = bz_ensure(owner, f->brush, 4 * tex.size());
+    if (rc == NPS_OK) rc = bz_copy(owner, f->brush.p, tex.data(), 4 * tex.size(), hipMemcpyHostToDevice);
+    if (rc != NPS_OK) {
+        nbf_destroy(f);
+        return rc;
+    }
+    f->bw = bw;
+    f->bh = bh;
+    *out = f;
+    return NPS_OK;
+}
+
+void nbf_destroy(nbf_handle f) {
+    if (!f) return;
+    (void)hipSetDevice(f->owner->device);
+    (void)hipStreamSynchronize(f->owner->stream);
+    free_buf(f->rgba);
+    free_buf(f->rec);
+    free_buf(f->brush);
+    free_scratch(f->sc);
+    delete f;
+}
+
+int nbf_write_vp(nbf_handle f, const float vp[16]) {
+    if (!f || !vp) return NPS_ERR_ARG;
+    memcpy(f->vp, vp, sizeof(f->vp));
+    f->has_vp = true;
+    return NPS_OK;
+}
+
+int nbf_set_target(nbf_handle f, int32_t w, int32_t hh) {
+    if (!f) return NPS_ERR_ARG;
+    nbz_ctx* h = f->owner;
+    if (w <= 0 || hh <= 0) return bz_err(h, NPS_ERR_ARG, "bad render target size");
+    NBZ_HIP(h, hipSetDevice(h->device));
+    int rc = bz_ensure(h, f->rgba, 16 * (size_t)w * hh);
+    if (rc != NPS_OK) return rc;
+    f->W = (uint32_t)w;
+    f->H = (uint32_t)hh;
+    return NPS_OK;
+}
+
+int nbf_dispatch(nbf_handle f, int32_t clear, int32_t quantize) {
+    if (!f) return NPS_ERR_ARG;
+    nbz_ctx* h = f->owner;
+    if (!f->has_vp || f->W == 0) return bz_err(h, NPS_ERR_STATE, "render target or VP not set");
+    NBZ_HIP(h, hipSetDevice(h->device));
+    const uint32_t n = h->n;
+    const size_t n1 = std::max<uint32_t>(n, 1);
+    ViewParams v = make_view(f->vp, nullptr, f->W, f->H);
+    int rc;
+    if ((rc = bz_ensure(h, f->rec, sizeof(BzDraw) * n1)) != NPS_OK) return rc;
+    if ((rc = bz_ensure(h, f->sc.counts, 4 * n1)) != NPS_OK) return rc;
+    if ((rc = bz_ensure(h, f->sc.rects, 16 * n1)) != NPS_OK) return rc;
+    f->sc.have_full = false;
+    f->sc.splat_block = 0;
+    launch_bz_draw_preprocess(h->stream, h->curves, n, v, (BzDraw*)f->rec.p, (uint32_t*)f->sc.counts.p,
+                              (uint4*)f->sc.rects.p);
+    if ((rc = build_tile_lists_on(h->stream, h->err, f->sc, v, n, nullptr, nullptr)) != NPS_OK) return rc;
+    BzDrawArgs a;
+    a.rec = (const BzDraw*)f->rec.p;
+    a.keys = (const uint64_t*)(f->sc.in_alt ? f->sc.keys_alt.p : f->sc.keys.p);
+    a.ranges = (const uint32_t*)f->sc.ranges.p;
+    a.brush = (const float*)f->brush.p;
+    a.bw = f->bw;
+    a.bh = f->bh;
+    a.rgba = (float4*)f->rgba.p;
+    a.clear = clear != 0;
+    a.quantize = quantize != 0;
+    launch_bz_draw(h->stream, v, a);
+    NBZ_HIP(h, hipGetLastError());
+    return NPS_OK;
+}
+
+int nbf_read_target(nbf_handle f, float* rgba) {
+    if (!f || !rgba) return NPS_ERR_ARG;
+    return bz_copy(f->owner, rgba, f->rgba.p, 16 * (size_t)f->W * f->H, hipMemcpyDeviceToHost);
+}
+
+int nbf_write_target(nbf_handle f, const float* rgba) {
+    if (!f || !rgba) return NPS_ERR_ARG;
+    return bz_copy(f->owner, f->rgba.p, rgba, 16 * (size_t)f->W * f->H, hipMemcpyHostToDevice);
+}
+
+}  // extern "C"

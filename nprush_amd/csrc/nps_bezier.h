@@ -1,5 +1,6 @@
 // nps_bezier.h -- records and launch arguments of the Bezier stroke path
-// (bezier_backward.wgsl, bezier_step.wgsl, bezier_adc.wgsl).
+// (bezier_backward.wgsl, bezier_step.wgsl, bezier_adc.wgsl) and of its display
+// raster (bezier_forward.wgsl).
 //
 // Curve layout in HBM (the reference's Bezier struct, bezier_backward.wgsl:4-10):
 // 20 floats = p0.xyz width | p1.xyz softness | p2.xyz pad | p3.xyz pad | rgba.
@@ -87,6 +88,33 @@ struct BzAdcBuffers {
     uint32_t* cand;       // n
     int64_t* stats;       // 6 (device)
 };
+
+// Display raster (bezier_forward.wgsl): per curve and view, what each pixel of
+// the curve's tiles reads.  Widths are floored at 1e-4 there, 0.001 in training.
+struct BzDraw {
+    float x[kBzSeg + 1], y[kBzSeg + 1];  // P0 = projected p0, Pk = bezier_at(k / 16)
+    float width, soft, op, r, g, b;
+    float qx0, qy0, qx1, qy1;            // the quad: control-point hull -+ (width + softness), clamped to +-4
+    float cx0, cy0, cx1, cy1;            // box of P0..P16
+    float rcull;                         // beyond this distance from the box every alpha is exactly 0
+    float pad[3];
+    float gb[16];                        // boxes (x0, y0, x1, y1) of the points of segments 4g+1..4g+4
+};
+static_assert(sizeof(BzDraw) == 272, "BzDraw layout");
+
+struct BzDrawArgs {
+    const BzDraw* rec;
+    const uint64_t* keys;    // sorted (tile << 32 | curve id)
+    const uint32_t* ranges;  // [2 * num_tiles]
+    const float* brush;      // bh rows of bw texel values (r / 255)
+    int32_t bw, bh;
+    float4* rgba;            // H*W, read when !clear
+    int clear, quantize;
+};
+
+void launch_bz_draw_preprocess(hipStream_t st, const float* curves, uint32_t n, const ViewParams& v, BzDraw* rec,
+                               uint32_t* counts, uint4* rects);
+void launch_bz_draw(hipStream_t st, const ViewParams& v, const BzDrawArgs& a);
 
 void launch_bz_preprocess(hipStream_t st, const float* curves, uint32_t n, const ViewParams& v, BzFwd* fwd,
                           BzBwd* bwd, uint32_t* counts, uint4* rects);

@@ -749,6 +749,65 @@ class GpuBezierOptimizerManager:
                     "nbz_dev_upload")
 
 
+class GpuBezierForwardPipelineManager:
+    """new GpuBezierForwardPipelineManager({device, numBeziers, bezierBuffer, brushTexture})
+    (src/lib/gpu/GpuBezierForwardPipelineManager.ts:17-98): the display raster of the curves.
+
+    ``bezierBuffer`` is the GpuBezierOptimizerManager whose curves are borrowed;
+    ``brushTexture`` is an H x W x 4 uint8 array (rgba8 texels; the red channel
+    is sampled)."""
+
+    def __init__(self, device=None, numBeziers: int | None = None,
+                 bezierBuffer: GpuBezierOptimizerManager = None, brushTexture=None):
+        if bezierBuffer is None:
+            raise ValueError("bezierBuffer (a GpuBezierOptimizerManager) is required")
+        if brushTexture is None:
+            raise ValueError("brushTexture (H x W x 4 uint8) is required")
+        tex = np.ascontiguousarray(brushTexture, dtype=np.uint8)
+        if tex.ndim != 3 or tex.shape[2] != 4:
+            raise ValueError("brushTexture must be H x W x 4 uint8")
+        lib = _lib.load()
+        f = C.c_void_p()
+        bezierBuffer._check(lib.nbf_create(bezierBuffer._h, tex.ctypes.data_as(C.c_void_p), tex.shape[1],
+                                           tex.shape[0], C.byref(f)), "nbf_create")
+        self._lib = lib
+        self._f = f
+        self._owner = bezierBuffer
+        self.numBeziers = bezierBuffer.numBeziers
+        self.dims = (0, 0)
+
+    def close(self):
+        if getattr(self, "_f", None):
+            self._lib.nbf_destroy(self._f)
+            self._f = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def writeVPMatrix(self, mat):
+        vp = np.ascontiguousarray(mat, dtype=np.float32).reshape(16)
+        self._owner._check(self._lib.nbf_write_vp(self._f, fptr(vp)), "writeVPMatrix")
+
+    def setTarget(self, targetView=None, width: int = 0, height: int = 0):
+        self._owner._check(self._lib.nbf_set_target(self._f, int(width), int(height)), "setTarget")
+        self.dims = (int(width), int(height))
+        if targetView is not None:
+            c = np.ascontiguousarray(targetView, dtype=np.float32)
+            self._owner._check(self._lib.nbf_write_target(self._f, fptr(c)), "setTarget")
+
+    def dispatch(self, commandEncoder=None, clear: bool = True, quantize: bool = True):
+        self._owner._check(self._lib.nbf_dispatch(self._f, 1 if clear else 0, 1 if quantize else 0), "dispatch")
+
+    def read_target(self) -> np.ndarray:
+        w, h = self.dims
+        c = np.empty((h, w, 4), dtype=np.float32)
+        self._owner._check(self._lib.nbf_read_target(self._f, fptr(c)), "read_target")
+        return c
+
+
 class DensifierMLP:
     """The densification MLP (nml_*): 32 parent features -> 14 child deltas,
     BF16 matrix-core products with FP32 accumulation, trained with Adam."""

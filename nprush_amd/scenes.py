@@ -128,3 +128,21 @@ def make_bezier_curves(n: int, seed: int, radius_median: float = 0.004, radius_s
     c[:, 16:19] = rng.uniform(0.0, 1.0, size=(n, 3))
     c[:, 19] = np.minimum(1.0 / (1.0 + np.exp(-rng.normal(1.0, 0.5, size=n))), 0.99)
     return c
+
+
+def make_brush(size: int = 256, seed: int = 0) -> np.ndarray:
+    """A synthetic bristle brush of the reference asset's shape (256 x 256 rgba8,
+    red channel sampled): streaks along u with random strengths, a soft falloff
+    towards both edges in v.  Rows are v, columns u."""
+    rng = np.random.default_rng(seed)
+    v = (np.arange(size) + 0.5) / size
+    across = np.clip(1.0 - np.abs(2.0 * v - 1.0) ** 3, 0.0, 1.0)           # soft edges in v
+    bristle = 0.6 + 0.4 * rng.random(size)                                   # per-row strength
+    along = 0.85 + 0.15 * np.sin(np.linspace(0.0, 6.0 * np.pi, size))       # slow variation in u
+    r = np.clip(across[:, None] * bristle[:, None] * along[None, :], 0.0, 1.0)
+    b = np.zeros((size, size, 4), np.uint8)
+    b[..., 0] = np.rint(r * 255.0).astype(np.uint8)
+    b[..., 1] = b[..., 0]
+    b[..., 2] = b[..., 0]
+    b[..., 3] = 255
+    return b

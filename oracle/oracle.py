@@ -402,3 +402,33 @@ def bz_adc_f32(bz, m, v, grad_accum, loss_acc, pixel_loss, W, H, uni, adam_t, no
                             _p(f32(uni), _F), float(adam_t), float(no_kill), _p(st, _L))
     assert rc == 0
     return b, mm, vv, ga, la, pl, st
+
+
+def bz_brush_values(brush_rgba8):
+    """Texel values the sampler returns for an H x W x 4 uint8 brush: red / 255 (float32)."""
+    b = np.ascontiguousarray(brush_rgba8, np.uint8)
+    return np.ascontiguousarray(b[..., 0].astype(np.float32) / np.float32(255.0))
+
+
+def bz_brush_sample(brush_rgba8, u, v):
+    """The restated brush sampler at (u, v) (scalars)."""
+    lib = load()
+    lib.ora_bz_brush_sample.argtypes = [_F, i32, i32, C.c_float, C.c_float]
+    lib.ora_bz_brush_sample.restype = C.c_float
+    t = bz_brush_values(brush_rgba8)
+    return float(lib.ora_bz_brush_sample(_p(t, _F), t.shape[1], t.shape[0], float(u), float(v)))
+
+
+def bz_render_f32(bz, vp, W, H, brush_rgba8, rgba=None, clear=True, quantize=True):
+    """Display raster of the curves (bezier_forward.wgsl); returns H x W x 4 float32."""
+    lib = load()
+    lib.ora_bz_render_f32.argtypes = [_F, i32, _F, i32, i32, _F, i32, i32, _F, i32, i32]
+    lib.ora_bz_render_f32.restype = i32
+    b = f32(bz).reshape(-1, 20)
+    t = bz_brush_values(brush_rgba8)
+    out = np.zeros((H, W, 4), np.float32) if rgba is None else f32(rgba).reshape(H, W, 4).copy()
+    rc = lib.ora_bz_render_f32(_p(b, _F), b.shape[0], _p(f32(vp), _F), W, H, _p(t, _F), t.shape[1], t.shape[0],
+                               _p(out, _F), int(clear), int(quantize))
+    if rc != 0:
+        raise RuntimeError(f"ora_bz_render_f32 failed ({rc})")
+    return out

@@ -746,3 +746,106 @@ int ora_bz_adc_f32(float* bz, int n, float* m, float* v, float* grad_accum, int6
     free(dead);
     return 0;
 }
+
+/*
+ * Display raster of the curves: bezier_forward.wgsl:47-182 with the blend state of
+ * GpuBezierForwardPipelineManager.ts:79-96, brute force per pixel.
+ *   - instances drawn from curve N-1 down to 0 (bezier_forward.wgsl:61);
+ *   - a curve with any control point at clip w <= 0 draws nothing (:74-81);
+ *   - its quad covers the pixel centres inside the closed box hull(P0..P3) -+ pad,
+ *     pad = max(width, 1e-4) + max(softness, 1e-4), clamped to [-4, 4] (:88-96);
+ *     the fragment position is the pixel centre (raster edge rules and varying
+ *     interpolation are not modelled);
+ *   - closest of the 16 polyline segments with the strict <, and the signed cross
+ *     distance on that segment (:133-152);
+ *   - pressure taper, smoothstep edge, brush .r at (t, cross / (w_l + s_l) / 2 + 1/2)
+ *     (:154-173), a = clamp(a_geom * brush * opacity, 0, 0.999), discard a < 0.001;
+ *   - premultiplied output (rgb a, a) blended with (one, one-minus-src-alpha):
+ *     rgb = rgb*a + dst*(1-a), alpha = a + dst_alpha*(1-a).
+ * The sampler (linear filtering, repeat in u, clamp-to-edge in v, one mip level,
+ * GpuBezierForwardPipelineManager.ts:34-41) is restated as FP32 bilinear filtering
+ * of the texel values at x = u*bw - 0.5, y = v*bh - 0.5; hardware filter weights
+ * are implementation-defined.  brush: bh rows of bw texel values (r / 255).
+ * rgba: H*W*4 in/out, cleared to 0 when clear != 0; quantize != 0 rounds every
+ * channel to k/255 after each blend (rgba8unorm target).
+ */
+float ora_bz_brush_sample(const float* tex, int bw, int bh, float u, float v) {
+    float x = u * (float)bw - 0.5f, y = v * (float)bh - 0.5f;
+    float x0f = floorf(x), y0f = floorf(y);
+    float fx = x - x0f, fy = y - y0f;
+    int ix = (int)x0f, iy = (int)y0f;
+    int x0 = ((ix % bw) + bw) % bw, x1 = (((ix + 1) % bw) + bw) % bw;
+    int y0 = iy < 0 ? 0 : (iy > bh - 1 ? bh - 1 : iy);
+    int y1 = iy + 1 < 0 ? 0 : (iy + 1 > bh - 1 ? bh - 1 : iy + 1);
+    float t00 = tex[(size_t)y0 * bw + x0], t10 = tex[(size_t)y0 * bw + x1];
+    float t01 = tex[(size_t)y1 * bw + x0], t11 = tex[(size_t)y1 * bw + x1];
+    float top = t00 * (1.0f - fx) + t10 * fx;
+    float bot = t01 * (1.0f - fx) + t11 * fx;
+    return top * (1.0f - fy) + bot * fy;
+}
+
+int ora_bz_render_f32(const float* bz, int n, const float* vp, int W, int H, const float* brush, int bw, int bh,
+                      float* rgba, int clear, int quantize) {
+    if (bw <= 0 || bh <= 0) return -1;
+    float aspect = (float)W / (float)H;
+    #pragma omp parallel for schedule(dynamic, 4)
+    for (int py = 0; py < H; ++py) {
+        for (int px = 0; px < W; ++px) {
+            float* c = rgba + 4 * ((size_t)py * W + px);
+            if (clear) c[0] = c[1] = c[2] = c[3] = 0.0f;
+            v2 p = pixel_to_p((uint32_t)px, (uint32_t)py, (uint32_t)W, (uint32_t)H, aspect);
+            for (int ii = 0; ii < n; ++ii) {
+                const float* b = bz + 20 * (size_t)(n - 1 - ii);
+                v3 q0 = project_center(vp, b + 0, aspect), q1 = project_center(vp, b + 4, aspect);
+                v3 q2 = project_center(vp, b + 8, aspect), q3 = project_center(vp, b + 12, aspect);
+                if (q0.z <= 0.0f || q1.z <= 0.0f || q2.z <= 0.0f || q3.z <= 0.0f) continue;
+                v2 p0 = { q0.x, q0.y }, p1 = { q1.x, q1.y }, p2 = { q2.x, q2.y }, p3 = { q3.x, q3.y };
+                float width = maxf_(b[3], 0.0001f), soft = maxf_(b[7], 0.0001f);
+                float pad = width + soft;
+                float mnx = maxf_(minf_(minf_(p0.x, p1.x), minf_(p2.x, p3.x)) - pad, -4.0f);
+                float mny = maxf_(minf_(minf_(p0.y, p1.y), minf_(p2.y, p3.y)) - pad, -4.0f);
+                float mxx = minf_(maxf_(maxf_(p0.x, p1.x), maxf_(p2.x, p3.x)) + pad, 4.0f);
+                float mxy = minf_(maxf_(maxf_(p0.y, p1.y), maxf_(p2.y, p3.y)) + pad, 4.0f);
+                if (!(p.x >= mnx && p.x <= mxx && p.y >= mny && p.y <= mxy)) continue;
+                float min_d = 1e9f, min_u = 0.0f, min_cross = 0.0f;
+                uint32_t min_k = 1u;
+                v2 prev = p0;
+                for (uint32_t k = 1u; k <= BZ_NSEG; ++k) {
+                    v2 curr = bezier_at(p0, p1, p2, p3, (float)k / (float)BZ_NSEG);
+                    v2 seg = v2m(curr, prev);
+                    float len2 = maxf_(v2d(seg, seg), 1e-8f);
+                    float uu = clampf_(v2d(v2m(p, prev), seg) / len2, 0.0f, 1.0f);
+                    v2 proj = v2a(prev, v2s(seg, uu));
+                    v2 dv = v2m(p, proj);
+                    float d = sqrtf(v2d(dv, dv));
+                    if (d < min_d) {
+                        min_d = d;
+                        min_k = k;
+                        min_u = uu;
+                        float seg_len = sqrtf(len2);
+                        v2 dir = { seg.x / seg_len, seg.y / seg_len };
+                        min_cross = dv.x * (-dir.y) + dv.y * dir.x;
+                    }
+                    prev = curr;
+                }
+                float t = ((float)(min_k - 1u) + min_u) / (float)BZ_NSEG;
+                float dt = t - 0.5f;
+                float pressure = 1.0f - 4.0f * dt * dt;
+                float lw = width * pressure, ls = soft * pressure, lo = b[19] * pressure;
+                float a_geom = 1.0f - smoothstep_(lw - ls, lw + ls, min_d);
+                float bv = clampf_(min_cross / maxf_(lw + ls, 1e-6f) * 0.5f + 0.5f, 0.0f, 1.0f);
+                float br = ora_bz_brush_sample(brush, bw, bh, t, bv);
+                float a = clampf_(a_geom * br * lo, 0.0f, 0.999f);
+                if (a < 0.001f) continue;
+                float one_m = 1.0f - a;
+                c[0] = b[16] * a + c[0] * one_m;
+                c[1] = b[17] * a + c[1] * one_m;
+                c[2] = b[18] * a + c[2] * one_m;
+                c[3] = a + c[3] * one_m;
+                if (quantize)
+                    for (int k = 0; k < 4; ++k) c[k] = rintf(clampf_(c[k], 0.0f, 1.0f) * 255.0f) / 255.0f;
+            }
+        }
+    }
+    return 0;
+}
