@@ -148,8 +148,7 @@ __global__ __launch_bounds__(256) void k_adc_compute(const float4* __restrict__ 
         o[q] = s.v[q];
         o[16 + q] = child.v[q];
     }
-    if (split) atomicAdd(&flag[2], 1u);
-    else atomicAdd(&flag[3], 1u);
+    wave_count2(split, &flag[2], &flag[3]);
 }
 
 __global__ void k_adc_write(float4* __restrict__ splats, float4* __restrict__ m, float4* __restrict__ v,

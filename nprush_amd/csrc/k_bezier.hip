@@ -431,6 +431,7 @@ __global__ __launch_bounds__(256) void k_bz_forward(ViewParams v, BzCompositeArg
     __shared__ BzFwd s_f[kBatch];
     __shared__ uint32_t s_ids[kBatch];
     __shared__ double s_red[2][4];
+    __shared__ uint32_t s_cnt[4];
 
     const uint32_t tile = xcd_remap(blockIdx.x, v.num_tiles);
     const uint32_t tx = tile % v.tiles_x, ty = tile / v.tiles_x;
@@ -609,7 +610,7 @@ __global__ __launch_bounds__(256) void k_bz_forward(ViewParams v, BzCompositeArg
         A.seeds[so + 7 * ss] = s.fy;
         A.seeds[so + 8 * ss] = s.have_flow ? 1.0f : 0.0f;
     }
-    if (lane == 0 && n_feval) atomicAdd(&A.stats[4], (unsigned long long)n_feval);
+    if (lane == 0) s_cnt[wid] = n_feval;
     __syncthreads();
     if (tid == 0) {
         double a2 = (s_red[0][0] + s_red[0][1]) + (s_red[0][2] + s_red[0][3]);
@@ -617,7 +618,10 @@ __global__ __launch_bounds__(256) void k_bz_forward(ViewParams v, BzCompositeArg
         A.tile_loss[2 * tile] = a2;
         A.tile_loss[2 * tile + 1] = a1;
         uint32_t vw = min(v.W - tx * kTile, (uint32_t)kTile), vh = min(v.H - ty * kTile, (uint32_t)kTile);
-        if (count > 0) atomicAdd(&A.stats[0], (unsigned long long)count * (unsigned long long)(vw * vh));
+        // the tile's counters in its slot of A.part (summed by launch_sum_parts)
+        unsigned long long* pc = A.part + 5 * (size_t)tile;
+        pc[0] = (unsigned long long)count * (unsigned long long)(vw * vh);
+        pc[1] = (unsigned long long)s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
     }
 }
 
@@ -631,12 +635,16 @@ __global__ __launch_bounds__(256) void k_bz_reverse(ViewParams v, BzCompositeArg
     __shared__ uint32_t s_lossp[kBatch][4];
     __shared__ uint32_t s_act[4][kBzCap / 32];
     __shared__ uint32_t s_nz[4][kBzCap / 32];
+    __shared__ unsigned long long s_cnt[3][4];
 
     const uint32_t tile = xcd_remap(blockIdx.x, v.num_tiles);
     const uint32_t tx = tile % v.tiles_x, ty = tile / v.tiles_x;
     const uint32_t start = A.ranges[2 * tile], end = A.ranges[2 * tile + 1];
     const int count = (int)min(end - start, A.cap);
-    if (count == 0) return;
+    if (count == 0) {
+        if (threadIdx.x < 3) A.part[5 * (size_t)tile + 2 + threadIdx.x] = 0ull;
+        return;
+    }
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const uint32_t bx = tx * kTile + (wid & 1) * 8, by = ty * kTile + (wid >> 1) * 8;
     const uint32_t px = bx + (lane & 7), py = by + (lane >> 3);
@@ -755,10 +763,13 @@ __global__ __launch_bounds__(256) void k_bz_reverse(ViewParams v, BzCompositeArg
             }
         }
     if (lane == 0) {
-        if (n_active) atomicAdd(&A.stats[2], n_active);
-        if (n_went) atomicAdd(&A.stats[3], n_went);
-        if (n_reval) atomicAdd(&A.stats[5], n_reval);
+        s_cnt[0][wid] = n_active;
+        s_cnt[1][wid] = n_went;
+        s_cnt[2][wid] = n_reval;
     }
+    __syncthreads();
+    if (tid < 3)
+        A.part[5 * (size_t)tile + 2 + tid] = (s_cnt[tid][0] + s_cnt[tid][1]) + (s_cnt[tid][2] + s_cnt[tid][3]);
 }
 
 void launch_bz_composite(hipStream_t st, const ViewParams& v, const BzCompositeArgs& a) {

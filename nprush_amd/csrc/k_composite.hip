@@ -190,6 +190,7 @@ __global__ __launch_bounds__(256) void k_composite_ref(ViewParams v, CompositeAr
     __shared__ uint32_t s_grad[kBatch][16];
     __shared__ uint32_t s_act[4][kMaxCap / 32];
     __shared__ double s_red[2][4];
+    __shared__ uint32_t s_cnt[3][4];
 
     const uint32_t tile = xcd_remap(blockIdx.x, v.num_tiles);
     const uint32_t tx = tile % v.tiles_x, ty = tile / v.tiles_x;
@@ -374,9 +375,9 @@ __global__ __launch_bounds__(256) void k_composite_ref(ViewParams v, CompositeAr
         }
     }
     if (lane == 0) {
-        if (n_active) atomicAdd(&A.stats[2], (unsigned long long)n_active);
-        if (n_went) atomicAdd(&A.stats[3], (unsigned long long)n_went);
-        if (n_feval) atomicAdd(&A.stats[4], (unsigned long long)n_feval);
+        s_cnt[0][wid] = n_active;
+        s_cnt[1][wid] = n_went;
+        s_cnt[2][wid] = n_feval;
     }
     __syncthreads();
     if (tid == 0) {
@@ -385,7 +386,12 @@ __global__ __launch_bounds__(256) void k_composite_ref(ViewParams v, CompositeAr
         A.tile_loss[2 * tile] = a2;
         A.tile_loss[2 * tile + 1] = a1;
         uint32_t vw = min(v.W - tx * kTile, (uint32_t)kTile), vh = min(v.H - ty * kTile, (uint32_t)kTile);
-        if (count > 0) atomicAdd(&A.stats[0], (unsigned long long)count * (unsigned long long)(vw * vh));
+        // the tile's counters in its slot of A.part (summed by launch_sum_parts)
+        unsigned long long* pc = A.part + 4 * (size_t)tile;
+        pc[0] = (unsigned long long)count * (unsigned long long)(vw * vh);
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            pc[k + 1] = (unsigned long long)s_cnt[k][0] + s_cnt[k][1] + s_cnt[k][2] + s_cnt[k][3];
     }
 }
 

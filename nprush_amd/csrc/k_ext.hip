@@ -329,6 +329,7 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
     __shared__ float s_part[NW][BATCH][9];
     __shared__ double s_red[2][NW];
     __shared__ int s_max[NW];
+    __shared__ unsigned long long s_cnt[4][NW];
 
     const uint32_t tile = A.order[blockIdx.x];
     if (tile >= v.num_tiles) return;  // padding slot of the order (uniform per workgroup)
@@ -669,18 +670,19 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
             }
         }
     }
-    // stats: [0] pairs evaluated by the forward, [2] pairs taken by the reverse
-    // pass, [3] reverse wave-entries, [4] forward wave-entries
-    // pairs evaluated before the pixel stopped (the stopping entry included)
+    // counters of the tile, in its slot of A.part: pairs evaluated by the forward
+    // (those before the pixel stopped, the stopping entry included), forward
+    // wave-entries, pairs taken by the reverse pass, reverse wave-entries
     uint32_t ev = valid ? (uint32_t)(stop > 0 ? stop : len) : 0u;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) ev += __shfl_xor(ev, o, 64);
     if (lane == 0) {
-        if (ev) atomicAdd(&A.stats[0], (unsigned long long)ev);
-        if (n_active) atomicAdd(&A.stats[2], (unsigned long long)n_active);
-        if (n_went) atomicAdd(&A.stats[3], (unsigned long long)n_went);
-        if (n_fw) atomicAdd(&A.stats[4], (unsigned long long)n_fw);
+        s_cnt[0][wid] = ev;
+        s_cnt[1][wid] = n_fw;
+        s_cnt[2][wid] = n_active;
+        s_cnt[3][wid] = n_went;
     }
+    __syncthreads();
     if (tid == 0) {
         double a2 = 0.0, a1 = 0.0;
 #pragma unroll
@@ -692,6 +694,13 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
         A.tile_loss[2 * tile + 1] = a1;
         A.tile_work[2 * tile] = (uint32_t)fw_extent;
         A.tile_work[2 * tile + 1] = (uint32_t)maxlast;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            unsigned long long c = 0ull;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) c += s_cnt[k][w];
+            A.part[4 * (size_t)tile + k] = c;
+        }
     }
 }
 
@@ -1040,14 +1049,13 @@ __global__ __launch_bounds__(256) void k_adc_ext_apply(uint32_t n, float4* __res
                 splats[4 * (size_t)slot + j] = make_float4(out[4 * j], out[4 * j + 1], out[4 * j + 2], out[4 * j + 3]);
         }
         split[k] = 1u;
-        atomicAdd(&cnt[2], 1u);
     } else {
 #pragma unroll
         for (int j = 0; j < 4; ++j)
             splats[4 * (size_t)d + j] = make_float4(p[4 * j], p[4 * j + 1], p[4 * j + 2], p[4 * j + 3]);
         split[k] = 0u;
-        atomicAdd(&cnt[3], 1u);
     }
+    wave_count2(smax > split_scale, &cnt[2], &cnt[3]);
 }
 
 // Adam state and SH of the densification, one thread per float4 (grid-stride,

@@ -282,4 +282,33 @@ bool radix_sort_u64(hipStream_t st, uint64_t* keys, uint64_t* keys_alt, uint32_t
     return alt;
 }
 
+// Sums per-workgroup counter slots (PartMap): one workgroup, stats[dst[k]] +=
+// sum over slots of part[K * slot + k].  Ordered after the producing kernel on
+// the stream, so plain read-modify-writes suffice.
+__global__ __launch_bounds__(1024) void k_sum_parts(const unsigned long long* __restrict__ part, uint32_t nslot,
+                                                    PartMap m, unsigned long long* __restrict__ stats) {
+    __shared__ unsigned long long s[8][16];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    for (uint32_t k = 0; k < m.nk; ++k) {
+        unsigned long long acc = 0ull;
+        for (uint32_t i = tid; i < nslot; i += 1024u) acc += part[(size_t)m.K * i + k];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if (lane == 0) s[k][wid] = acc;
+    }
+    __syncthreads();
+    if ((uint32_t)tid < m.nk) {
+        unsigned long long t = 0ull;
+#pragma unroll
+        for (int w = 0; w < 16; ++w) t += s[tid][w];
+        stats[m.dst[tid]] += t;
+    }
+}
+
+void launch_sum_parts(hipStream_t st, const unsigned long long* part, uint32_t nslot, const PartMap& m,
+                      unsigned long long* stats) {
+    if (nslot == 0 || m.nk == 0) return;
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1024), 0, st, part, nslot, m, stats);
+}
+
 }  // namespace nps

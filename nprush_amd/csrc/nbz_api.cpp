@@ -43,7 +43,7 @@ struct nbz_ctx {
     const float *tgt = nullptr, *tdepth = nullptr, *bg = nullptr, *bgd = nullptr, *normal = nullptr;
     DevBuf pixel_loss;
     uint32_t pl_w = 0, pl_h = 0;
-    DevBuf fwd, bwd, ckpt, c_img, d_img, t_img, tile_loss, stats, pa, pd, pk, act, nzm, seeds;
+    DevBuf fwd, bwd, ckpt, c_img, d_img, t_img, tile_loss, stats, pa, pd, pk, act, nzm, seeds, part;
     Scratch sc;
     uint32_t last_tiles = 0;
     uint32_t img_w = 0, img_h = 0;
@@ -130,6 +130,7 @@ int bz_run(nbz_ctx* h, bool backward) {
     if ((rc = bz_ensure(h, h->d_img, 4 * std::max<size_t>(px, 1))) != NPS_OK) return rc;
     if ((rc = bz_ensure(h, h->t_img, 4 * std::max<size_t>(px, 1))) != NPS_OK) return rc;
     if ((rc = bz_ensure(h, h->tile_loss, 16 * t1)) != NPS_OK) return rc;
+    if ((rc = bz_ensure(h, h->part, 40 * t1)) != NPS_OK) return rc;
     const size_t pslots = (size_t)std::max<uint64_t>(sc.m, 1) * kTilePx;
     if (backward) {
         if ((rc = bz_ensure(h, h->pa, 4 * pslots)) != NPS_OK) return rc;
@@ -165,11 +166,14 @@ int bz_run(nbz_ctx* h, bool backward) {
     a.nzm = (uint32_t*)h->nzm.p;
     a.seeds = (float*)h->seeds.p;
     a.stats = (unsigned long long*)h->stats.p;
+    a.part = (unsigned long long*)h->part.p;
     a.prm = h->prm;
     a.cap = cap;
     a.nchunk_max = nchunk;
     a.backward = backward ? 1 : 0;
     launch_bz_composite(h->stream, v, a);
+    launch_sum_parts(h->stream, a.part, v.num_tiles, PartMap{5, backward ? 5u : 2u, {0, 4, 2, 3, 5, 0, 0, 0}},
+                     a.stats);
     if (h->timing) NBZ_HIP(h, hipEventRecord(h->ev[4], h->stream));
     NBZ_HIP(h, hipGetLastError());
     h->last_tiles = v.num_tiles;
@@ -283,7 +287,7 @@ void nbz_destroy(nbz_handle h) {
     DevBuf* bs[] = {&h->own_tgt, &h->own_tdepth, &h->own_bg, &h->own_bgd, &h->own_normal, &h->pixel_loss,
                     &h->fwd, &h->bwd, &h->ckpt, &h->c_img, &h->d_img, &h->t_img, &h->tile_loss, &h->stats,
                     &h->pa, &h->pd, &h->pk, &h->act, &h->nzm, &h->seeds,
-                    &h->adc_dead, &h->adc_cand, &h->adc_stats};
+                    &h->adc_dead, &h->adc_cand, &h->adc_stats, &h->part};
     for (DevBuf* b : bs) free_buf(*b);
     free_scratch(h->sc);
     for (int k = 0; k < 8; ++k)

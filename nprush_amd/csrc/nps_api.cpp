@@ -32,6 +32,7 @@ namespace {
 struct ExtView {
     DevBuf rec, counts, rects, dkey, dkey_alt, rank_counts, offsets, scan_tmp, keys, keys_alt, sort_tmp, ranges;
     DevBuf order, tile_work, ckpt, act, g2, c_img, tile_loss, stats, flags;  // flags: [0] key overflow
+    DevBuf part;  // per-tile counter slots of the composite
     uint64_t m = 0;    // key count of the last eager run
     uint64_t cap = 0;  // key capacity of a captured run
     bool in_alt = false;
@@ -87,7 +88,7 @@ struct nps_ctx {
     uint32_t W = 0, H = 0;
     bool has_target = false;
     DevBuf tgt_rgba, tgt_depth;
-    DevBuf fwd, bwd, ckpt, c_img, d_img, tile_loss, stats;
+    DevBuf fwd, bwd, ckpt, c_img, d_img, tile_loss, stats, part;
     ExtView xv;      // EXT: buffers of nps_backward / nps_forward
     DevBuf adam_dev;  // EXT: Adam step counter and bias corrections on the device (t, 1-b1^t, 1-b2^t, skip)
     Scratch sc;
@@ -138,7 +139,7 @@ int ensure(nps_ctx* h, DevBuf& b, size_t bytes) { return ensure_dev(b, bytes, h-
 void free_ext_view(ExtView& x) {
     DevBuf* all[] = {&x.rec, &x.counts, &x.rects, &x.dkey, &x.dkey_alt, &x.rank_counts, &x.offsets, &x.scan_tmp,
                      &x.keys, &x.keys_alt, &x.sort_tmp, &x.ranges, &x.order, &x.tile_work, &x.ckpt, &x.act,
-                     &x.g2, &x.c_img, &x.tile_loss, &x.stats, &x.flags};
+                     &x.g2, &x.c_img, &x.tile_loss, &x.stats, &x.flags, &x.part};
     for (DevBuf* b : all) free_buf(*b);
     if (x.ev_count) {
         (void)hipEventDestroy(x.ev_count);
@@ -188,7 +189,7 @@ int ext_view_reserve(ExtView& x, uint32_t n, uint32_t W, uint32_t H, uint64_t ca
         {&x.scan_tmp, scan_temp_bytes(n1) + 64}, {&x.stats, 64}, {&x.flags, 64},
         {&x.sort_tmp, radix_temp_bytes(std::max<uint64_t>(c1, n1))}, {&x.keys, 8 * c1}, {&x.keys_alt, 8 * c1},
         {&x.ranges, 8 * (size_t)nt}, {&x.order, 4 * (size_t)ext_order_slots(tx, ty)}, {&x.tile_work, 8 * (size_t)nt},
-        {&x.c_img, 12 * (size_t)W * H}, {&x.tile_loss, 16 * (size_t)nt},
+        {&x.c_img, 12 * (size_t)W * H}, {&x.tile_loss, 16 * (size_t)nt}, {&x.part, 32 * (size_t)nt},
     };
     int rc;
     for (auto& q : need)
@@ -294,8 +295,10 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
     a.stats = (unsigned long long*)x.stats.p;
     a.order = (const uint32_t*)x.order.p;
     a.tile_work = (uint32_t*)x.tile_work.p;
+    a.part = (unsigned long long*)x.part.p;
     a.backward = grads ? 1 : 0;
     launch_composite_ext(st, v, a, batch);
+    launch_sum_parts(st, a.part, v.num_tiles, PartMap{4, 4, {0, 4, 2, 3, 0, 0, 0, 0}}, a.stats);
     if (ev && (rc = hip(hipEventRecord(ev[4], st), "event")) != NPS_OK) return rc;
     if (grads && !(out && out->defer_chain))
         launch_ext_chain(st, sc.splats, sc.sh, n, sc.deg, v, cam, (const double*)g2, (const uint32_t*)counts, sc.gradx,
@@ -476,7 +479,7 @@ void nps_destroy(nps_handle h) {
     free_ext_view(h->xv);
     free_buf(h->adam_dev);
     free_buf(h->tgt_rgba); free_buf(h->tgt_depth); free_buf(h->fwd); free_buf(h->bwd); free_buf(h->ckpt);
-    free_buf(h->c_img); free_buf(h->d_img); free_buf(h->tile_loss); free_buf(h->stats);
+    free_buf(h->c_img); free_buf(h->d_img); free_buf(h->tile_loss); free_buf(h->stats); free_buf(h->part);
     free_buf(h->adc_u32); free_buf(h->adc_tmp); free_buf(h->adc_scan);
     free_buf(h->adc_feat); free_buf(h->adc_delta); free_buf(h->snap);
     free_scratch(h->sc);
@@ -669,6 +672,7 @@ int nps_backward(nps_handle h) {
     if ((rc = ensure(h, h->c_img, 12 * px)) != NPS_OK) return rc;
     if ((rc = ensure(h, h->d_img, 4 * px)) != NPS_OK) return rc;
     if ((rc = ensure(h, h->tile_loss, 16 * (size_t)std::max<uint32_t>(v.num_tiles, 1))) != NPS_OK) return rc;
+    if ((rc = ensure(h, h->part, 32 * (size_t)std::max<uint32_t>(v.num_tiles, 1))) != NPS_OK) return rc;
     if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[3], h->stream));
     CompositeArgs a;
     a.fwd = (const FwdRec*)h->fwd.p;
@@ -685,7 +689,9 @@ int nps_backward(nps_handle h) {
     a.stats = (unsigned long long*)h->stats.p;
     a.cap = cap;
     a.nchunk_max = nchunk;
+    a.part = (unsigned long long*)h->part.p;
     launch_composite_ref(h->stream, v, a);
+    launch_sum_parts(h->stream, a.part, v.num_tiles, PartMap{4, 4, {0, 2, 3, 4, 0, 0, 0, 0}}, a.stats);
     if (h->timing) NPS_HIP(h, hipEventRecord(h->ev[4], h->stream));
     NPS_HIP(h, hipGetLastError());
     h->last_tiles = v.num_tiles;

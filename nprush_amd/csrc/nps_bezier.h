@@ -75,6 +75,7 @@ struct BzCompositeArgs {
     uint32_t* act;             // per tile, wave and 32 entries: some pixel has alpha >= 0.001
     uint32_t* nzm;             // per tile, wave and 32 entries: pair values stored (some alpha > 0)
     float* seeds;              // 9 planes of num_tiles * 256: dC rgb, dT, dD, is_bg, flow xy, has flow
+    unsigned long long* part;  // 5 per tile (PartMap slots): the counters stats[0], [4], [2], [3], [5]
     unsigned long long* stats; // [0] pairs, [2] reverse pairs, [3] reverse wave-entries, [4] forward and
                                // [5] reverse wave-entries reloaded from the forward's pair buffers
     BzParams prm;

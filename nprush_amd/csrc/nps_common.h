@@ -225,6 +225,19 @@ __device__ __forceinline__ uint4 tile_rect(float minx, float maxx, float miny, f
     return make_uint4(tx0, tx_end, ty0, ty_end);
 }
 
+// Adds the number of active lanes of the wave with `pred` true to *ctr_true and
+// with it false to *ctr_false: one atomic per wave and counter instead of one
+// per lane (same-address atomics from every lane of a launch serialise).
+__device__ __forceinline__ void wave_count2(bool pred, uint32_t* ctr_true, uint32_t* ctr_false) {
+    const uint64_t act = __ballot(1);
+    const uint64_t yes = __ballot(pred);
+    if ((int)(threadIdx.x & 63u) == __ffsll((unsigned long long)act) - 1) {
+        const uint32_t t = (uint32_t)__popcll(yes), f = (uint32_t)__popcll(act) - t;
+        if (t) atomicAdd(ctr_true, t);
+        if (f) atomicAdd(ctr_false, f);
+    }
+}
+
 // A 64-bit value that is the same in every lane, as a scalar.  readfirstlane
 // returns int: each half is taken back to uint32_t before widening, so the low
 // half does not sign-extend into the high one.

@@ -23,6 +23,7 @@ struct CompositeArgs {
     float* ckpt;              // num_tiles * nchunk_max * 256
     unsigned long long* stats;  // [0] pairs, [1] tiles over the cap, [2] pairs in the reverse pass,
                                 // [3] reverse wave-entries, [4] forward wave-entries evaluating pow/exp
+    unsigned long long* part;   // 4 per tile (PartMap slots): the counters stats[0], [2], [3], [4]
     uint32_t cap;
     uint32_t nchunk_max;
 };
@@ -43,6 +44,18 @@ void launch_tile_ranges(hipStream_t st, const uint64_t* keys, uint64_t m, uint32
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint64_t n);
 
 // k_sort.hip
+// Diagnostic counters without same-address global atomics (a device-scope
+// atomic on one address from every wave of a launch serialises across the
+// XCDs and cost more than the work it counted): workgroup slot s of a kernel
+// stores its partial counts in part[K * s + k]; launch_sum_parts then adds the
+// sum over the nslot slots of each k < nk to stats[dst[k]] (one workgroup).
+struct PartMap {
+    uint32_t K, nk;
+    int32_t dst[8];
+};
+void launch_sum_parts(hipStream_t st, const unsigned long long* part, uint32_t nslot, const PartMap& m,
+                      unsigned long long* stats);
+
 size_t scan_temp_bytes(uint64_t n);
 void exclusive_scan_u32_to_u64(hipStream_t st, const uint32_t* in, uint64_t* out, uint64_t n, void* temp);
 void exclusive_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t n, void* temp);
@@ -105,6 +118,7 @@ struct ExtCompositeArgs {
                                 // [4] forward wave-entries
     const uint32_t* order;      // workgroup -> tile (launch_tile_order), ~0 = none
     uint32_t* tile_work;        // 2 per tile: entries staged by the forward, reverse extent
+    unsigned long long* part;   // 4 per tile (PartMap slots): the counters stats[0], [4], [2], [3]
     int backward;               // 0: image and loss only
 };
 // Entries of the composite's workgroup order (= its grid size).
