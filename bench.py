@@ -724,6 +724,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the multi-view (64 views, 4 scenes) comparison")
     ap.add_argument("--no-bezier", action="store_true", help="skip the Bezier stroke workload")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-densification comparison")
+    ap.add_argument("--no-densify", action="store_true", help="skip the timed EXT densification pass")
     ap.add_argument("--cpu-tiles", type=int, default=48, help="tiles in the CPU-baseline sample (0 = skip)")
     ap.add_argument("--tile", type=int, default=0,
                     help="EXT: composite tile edge {8,16,32} instead of the autotuner's choice (with --batch)")
@@ -949,8 +950,10 @@ def main_ext(args):
     if not args.small and not args.no_bezier:
         print("[bench] Bezier strokes: 100k curves, 4 views at 1024x1024 ...", file=sys.stderr, flush=True)
         bezier = bezier_config3(cpu_tiles=1024 if args.cpu_tiles > 0 else 0)
-    print("[bench] EXT densification ...", file=sys.stderr, flush=True)
-    adc = adc_ext_leg(small=args.small)
+    adc = None
+    if not args.no_densify:
+        print("[bench] EXT densification ...", file=sys.stderr, flush=True)
+        adc = adc_ext_leg(small=args.small)
     learned = None
     if not args.small and not args.no_learned:
         print("[bench] learned densification vs plain ...", file=sys.stderr, flush=True)

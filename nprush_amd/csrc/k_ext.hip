@@ -307,7 +307,7 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict_
 
 // Composite of one TILE x TILE tile (one workgroup, one thread per pixel, each
 // wave an 8x8 pixel block), records staged in LDS BATCH at a time.
-//   forward : front to back, alphas of 8 entries at a time (independent), then
+//   forward : front to back, alphas of kFwdGroup entries at a time (independent), then
 //             the transmittance chain with early termination; T checkpointed
 //             every 8 entries ([slot][chunk][pixel]); per wave one bit per entry:
 //             did any of its pixels composite it;
@@ -315,6 +315,9 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict_
 //             skipped, alpha and T recomputed from the checkpoint, 9 FP32
 //             per-pair gradients summed over the wave, the 4..16 wave partials
 //             of an entry summed in FP64 and added to the splat's 2D gradients.
+// Forward: entries whose alphas are computed together (independent LDS reads and
+// arithmetic); 4 keeps the kernel at 5 waves per SIMD, where 8 needs 128 VGPRs (4).
+constexpr int kFwdGroup = 4;
 template <int TILE, int BATCH>
 __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, ExtCompositeArgs A) {
     constexpr int NT = TILE * TILE;            // threads = pixels
@@ -433,8 +436,8 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
 #pragma unroll
         for (int w = 0; w < MW; ++w) mask[w] = 0ull;
         float* ck = A.ckpt + (slot0 + bi) * (size_t)(NCH * NT);
-        // Only the entries that may hit are evaluated, 8 at a time in ascending
-        // order (an entry left out changes no lane's state).  The transmittance
+        // Only the entries that may hit are evaluated, kFwdGroup at a time in
+        // ascending order (an entry left out changes no lane's state).  The transmittance
         // checkpoint of chunk c (entries 8c..8c+7) is T before the first entry
         // of the chunk, written when the walk first reaches the chunk; chunks
         // the walk passes over without an entry are written with the same T.
@@ -444,17 +447,17 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
         for (int w = 0; w < MW; ++w) {
             uint64_t m = hit[w];
             while (m != 0ull && wave_alive) {
-                int e[8];
+                int e[kFwdGroup];
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
+                for (int j = 0; j < kFwdGroup; ++j) {
                     e[j] = m != 0ull ? 64 * w + (int)__builtin_ctzll(m) : -1;
                     m &= m - 1ull;
                 }
                 // alphas of the group's entries are independent: computed first,
                 // branch-free (selects), so their LDS reads and arithmetic overlap
-                float ca[8];
+                float ca[kFwdGroup];
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
+                for (int j = 0; j < kFwdGroup; ++j) {
                     const int q = e[j] < 0 ? 0 : e[j];
                     const float4 p0 = rec4[3 * q];      // scr_x scr_y A B
                     const float4 p1 = rec4[3 * q + 1];  // C op r g
@@ -472,7 +475,7 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
                 }
                 // then the front-to-back chain, entry by entry
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
+                for (int j = 0; j < kFwdGroup; ++j) {
                     if (e[j] < 0) break;  // uniform
                     ++n_fw;
                     if (A.backward)
