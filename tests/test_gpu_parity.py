@@ -96,6 +96,11 @@ def test_config0_tiles_and_backward(oracle):
     assert abs(l2 - loss_ref[0]) <= 1e-9 * abs(loss_ref[0]) and abs(l1 - loss_ref[1]) <= 1e-9 * abs(loss_ref[1])
     st = mgr.read_stats()
     assert st["tiles_over_cap"] == int((o_full > 1024).sum())
+    # pairs = sum over tiles of (capped list length x pixels of the tile), summed from per-tile counter slots
+    tx = (W + 15) // 16
+    pix = np.array([min(16, W - 16 * (t % tx)) * min(16, H - 16 * (t // tx)) for t in range(len(o_cnt))], np.int64)
+    assert st["pairs"] == int(np.dot(o_cnt.astype(np.int64), pix))
+    assert 0 < st["active_pairs"] <= st["pairs"]
     mgr.close()
 
 
