@@ -711,15 +711,24 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
 
 // kStore: the gradient buffer is known to be all zero (the last Adam step
 // cleared it): the splat's gradients are stored, not read and added.
+// flags (nflags words, 16 apart; null = none): when any is set a view of the
+// step dropped keys, the host redoes the step view by view, and this launch
+// leaves gradients and densification statistics untouched.
 template <int DEG, bool kStore>
 __global__ __launch_bounds__(256) void k_ext_chain(const float4* __restrict__ splats, const float4* __restrict__ sh,
                                                    uint32_t n, ViewParams v, float cam0, float cam1, float cam2,
                                                    const double* __restrict__ g2, const uint32_t* __restrict__ counts,
-                                                   float* __restrict__ grads, float2* __restrict__ accum) {
+                                                   float* __restrict__ grads, float2* __restrict__ accum,
+                                                   const uint32_t* __restrict__ flags, uint32_t nflags) {
     constexpr int kNk = (DEG + 1) * (DEG + 1);
     constexpr int kSh4 = (3 * kNk + 3) / 4;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (flags) {
+        uint32_t bad = 0u;
+        for (uint32_t k = 0; k < nflags; ++k) bad |= flags[16 * k];
+        if (bad) return;
+    }
     if (counts[i] == 0u) return;  // culled or off-screen: no pair, no gradient
     double gg[9];
     bool any = false;
@@ -1315,25 +1324,25 @@ size_t ext_act_words(size_t slots, int tile, int batch) {
 template <bool kStore>
 static void launch_ext_chain_t(hipStream_t st, const float4* s4, const float4* h4, uint32_t n, int deg,
                                const ViewParams& v, const float cam[3], const double* g2, const uint32_t* counts,
-                               float* grads, float2* a2) {
+                               float* grads, float2* a2, const uint32_t* fl, uint32_t nfl) {
     dim3 g((n + 255) / 256), b(256);
     switch (deg) {
-        case 0: hipLaunchKernelGGL((k_ext_chain<0, kStore>), g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2); break;
-        case 1: hipLaunchKernelGGL((k_ext_chain<1, kStore>), g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2); break;
-        case 2: hipLaunchKernelGGL((k_ext_chain<2, kStore>), g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2); break;
-        default: hipLaunchKernelGGL((k_ext_chain<3, kStore>), g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2); break;
+        case 0: hipLaunchKernelGGL((k_ext_chain<0, kStore>), g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2, fl, nfl); break;
+        case 1: hipLaunchKernelGGL((k_ext_chain<1, kStore>), g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2, fl, nfl); break;
+        case 2: hipLaunchKernelGGL((k_ext_chain<2, kStore>), g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2, fl, nfl); break;
+        default: hipLaunchKernelGGL((k_ext_chain<3, kStore>), g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2, fl, nfl); break;
     }
 }
 
 void launch_ext_chain(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg, const ViewParams& v,
                       const float cam[3], const double* g2, const uint32_t* counts, float* grads, float* accum,
-                      bool grads_zero) {
+                      bool grads_zero, const uint32_t* flags, uint32_t nflags) {
     if (n == 0) return;
     const float4* s4 = (const float4*)splats;
     const float4* h4 = (const float4*)sh;
     float2* a2 = (float2*)accum;
-    if (grads_zero) launch_ext_chain_t<true>(st, s4, h4, n, deg, v, cam, g2, counts, grads, a2);
-    else launch_ext_chain_t<false>(st, s4, h4, n, deg, v, cam, g2, counts, grads, a2);
+    if (grads_zero) launch_ext_chain_t<true>(st, s4, h4, n, deg, v, cam, g2, counts, grads, a2, flags, nflags);
+    else launch_ext_chain_t<false>(st, s4, h4, n, deg, v, cam, g2, counts, grads, a2, flags, nflags);
 }
 
 void launch_adam_ext(hipStream_t st, float* splats, float* sh, float* grads, float* m, float* v, uint32_t n, int deg,

@@ -1347,9 +1347,12 @@ int npt_plan(npt_handle t) {
                 ViewParams v = npt_view_params(w);
                 float cam[3];
                 ext_camera_centre(w.vp_inv, cam);
+                // gated on the lanes' overflow flags: after an overflow the host
+                // redoes the step view by view, so nothing may be added here
                 launch_ext_chain(st, h->splats, h->sh, h->n, h->deg, v, cam,
                                  (const double*)t->view_g2.p + 9 * t->view_base[vi],
-                                 (const uint32_t*)t->view_counts.p + t->view_base[vi], h->gradx, h->accum);
+                                 (const uint32_t*)t->view_counts.p + t->view_base[vi], h->gradx, h->accum, false,
+                                 (const uint32_t*)t->flags.p, (uint32_t)t->lanes.size());
             }
         }
         for (size_t l = 0; l < std::min(S, L); ++l) {
@@ -1389,9 +1392,10 @@ int npt_step_graph(npt_handle t) {
     bool overflow = false;
     for (size_t l = 0; l < t->lanes.size(); ++l) overflow = overflow || fl[16 * l] != 0;
     if (overflow) {
-        // a view had more keys than its lane holds: Adam was skipped on the
-        // device; clear the partial gradients, grow the lanes, redo the step
-        // view by view and capture again next time
+        // a view had more keys than its lane holds: the chain-rule kernels and
+        // Adam saw the flag and changed nothing (gradients, densification
+        // statistics, parameters); clear the gradients anyway, grow the lanes,
+        // redo the step view by view and capture again next time
         for (nps_handle h : t->scenes) {
             launch_fill_u32(h->stream, (uint32_t*)h->gradx, 0u, 64 * (uint64_t)h->n);
             NPT_HIP(t, hipStreamSynchronize(h->stream));

@@ -140,10 +140,11 @@ void launch_composite_ext(hipStream_t st, const ViewParams& v, const ExtComposit
 size_t ext_ckpt_floats(size_t slots, int tile, int batch);
 size_t ext_act_words(size_t slots, int tile, int batch);
 // grads_zero: the caller knows the 64N gradients are all zero (then they are
-// stored, not read and added)
+// stored, not read and added).  flags/nflags: key-overflow flags (16 words
+// apart) of a captured step; if any is set the launch changes nothing.
 void launch_ext_chain(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg, const ViewParams& v,
                       const float cam[3], const double* g2, const uint32_t* counts, float* grads, float* accum,
-                      bool grads_zero = false);
+                      bool grads_zero = false, const uint32_t* flags = nullptr, uint32_t nflags = 0);
 // ctl = {t, 1 - 0.9^t, 1 - 0.999^t, skip} on the device: t advances by one
 // unless one of the nflags flags (16 words apart) is set, then nothing changes.
 void launch_adam_ext(hipStream_t st, float* splats, float* sh, float* grads, float* m, float* v, uint32_t n, int deg,
