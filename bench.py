@@ -122,15 +122,9 @@ def batched_config2(steps=6, warmup=1, lanes=8):
     launches (strictly serial, and overlapping across scenes)."""
     from nprush_amd import GpuMultiViewStep, GpuSplatOptimizerManager, camera, scenes
 
-    sizes = [(256, 256), (512, 512), (1024, 768), (1920, 1080)]
-    counts = [50_000, 150_000, 300_000, 500_000]
-    rng = np.random.default_rng(4)
-    views = []
-    for k in range(64):
-        w, h = sizes[int(rng.integers(0, 4))]
-        ang = rng.uniform(0.0, 2.0 * np.pi)
-        eye = (3.0 * np.cos(ang), rng.uniform(-1.0, 1.0), 3.0 * np.sin(ang))
-        views.append((k % 4, w, h) + camera.view_proj(eye, aspect=w / h))
+    sizes = scenes.CONFIG2_SIZES
+    counts = scenes.CONFIG2_COUNTS
+    views = scenes.config2_views()
     targets = []
     for sidx, n in enumerate(counts):
         ts, tsh = scenes.make_ext_splats(n, 100 + sidx, sh_degree=3)

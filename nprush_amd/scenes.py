@@ -76,6 +76,25 @@ CONFIG1 = dict(n=1_000_000, capacity=2_000_000, width=1920, height=1080, views=8
                target_seed=1)
 
 
+CONFIG2_SIZES = [(256, 256), (512, 512), (1024, 768), (1920, 1080)]
+CONFIG2_COUNTS = [50_000, 150_000, 300_000, 500_000]
+
+
+def config2_views(num_views: int = 64, seed: int = 4):
+    """Views of the batched uneven workload: view k belongs to scene k % 4, its
+    resolution drawn uniformly (seed 4) from {256^2, 512^2, 1024x768,
+    1920x1080}, its eye on a radius-3 circle at a random angle and height in
+    [-1, 1], looking at the origin.  Returns [(scene, W, H, vp, vp_inv)]."""
+    rng = np.random.default_rng(seed)
+    views = []
+    for k in range(num_views):
+        w, h = CONFIG2_SIZES[int(rng.integers(0, 4))]
+        ang = rng.uniform(0.0, 2.0 * np.pi)
+        eye = (3.0 * np.cos(ang), rng.uniform(-1.0, 1.0), 3.0 * np.sin(ang))
+        views.append((k % 4, w, h) + camera.view_proj(eye, aspect=w / h))
+    return views
+
+
 SH_C0 = 0.28209479177387814
 
 

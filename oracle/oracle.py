@@ -306,6 +306,30 @@ def ext_backward_csr_f32(splats, sh, deg, vp, vp_inv, tgt, W, H, offs, ids, want
     return g, gs, c, loss
 
 
+def ext_backward_csr_f64(splats, sh, deg, vp, vp_inv, tgt, W, H, offs, ids, want_grads=True, tile=16):
+    """FP64 twin of ext_backward_csr_f32 on the same CSR lists (splats, SH and VP
+    widened from FP32, camera centre from cam_from_inv)."""
+    T = ((W + tile - 1) // tile) * ((H + tile - 1) // tile)
+    if len(offs) != T + 1:
+        raise ValueError(f"offs has {len(offs)} entries, {T + 1} expected for {tile}-pixel tiles")
+    lib = load()
+    lib.ora_ext_backward_csr_f64.argtypes = [_D, _D, i32, i32, _D, _D, _F, i32, i32, i32, _U, _L, _D, _D, _D, _D]
+    lib.ora_ext_backward_csr_f64.restype = i32
+    s = np.ascontiguousarray(np.asarray(splats, np.float32), np.float64)
+    h = np.ascontiguousarray(np.asarray(sh, np.float32), np.float64)
+    n = s.shape[0]
+    g = np.zeros((n, 16), np.float64) if want_grads else None
+    gs = np.zeros((n, 48), np.float64) if want_grads else None
+    c = np.zeros((H, W, 3), np.float64)
+    loss = np.zeros(5, np.float64)
+    idsc = np.ascontiguousarray(ids, np.uint32) if len(ids) else np.zeros(1, np.uint32)
+    vp64 = np.ascontiguousarray(np.asarray(vp, np.float32).reshape(16), np.float64)
+    lib.ora_ext_backward_csr_f64(_p(s, _D), _p(h, _D), n, deg, _p(vp64, _D), _p(cam_from_inv(vp_inv), _D),
+                                 _p(f32(tgt), _F), W, H, tile, _p(idsc, _U), _p(np.ascontiguousarray(offs, np.int64), _L),
+                                 _p(g, _D), _p(gs, _D), _p(c, _D), _p(loss, _D))
+    return g, gs, c, loss
+
+
 # ---------------------------------------------------------------- Bezier path
 BZ_CAP = 1024
 

@@ -106,6 +106,22 @@ int ora_ext_backward_csr_f32(const float* splats, const float* sh, int n, int de
     return 0;
 }
 
+/* The FP64 twin over CSR lists: every tile of the view, FP64 splats, SH, VP
+ * and camera centre (for the error statement of the FP32 paths; the lists are
+ * the FP32 binning's). */
+int ora_ext_backward_csr_f64(const double* splats, const double* sh, int n, int deg, const double* vp,
+                             const double* cam, const float* tgt, int W, int H, int tile, const uint32_t* ids,
+                             const int64_t* offs, double* grads, double* gsh, double* c_out, double* loss) {
+    const int ntx = (W + tile - 1) / tile, nty = (H + tile - 1) / tile, T = ntx * nty;
+    int32_t* tiles = (int32_t*)malloc(sizeof(int32_t) * (size_t)T);
+    int32_t* counts = (int32_t*)malloc(sizeof(int32_t) * (size_t)T);
+    for (int t = 0; t < T; ++t) { tiles[t] = t; counts[t] = (int32_t)(offs[t + 1] - offs[t]); }
+    ext_backward_xf64(splats, sh, n, deg, vp, cam, tgt, (uint32_t)W, (uint32_t)H, (uint32_t)tile, tiles, T, ids,
+                      counts, 0, offs, grads, gsh, c_out, loss);
+    free(tiles); free(counts);
+    return 0;
+}
+
 int ora_ext_backward_f64(const double* splats, const double* sh, int n, int deg, const double* vp,
                          const double* cam, const float* tgt, int W, int H, int tile, const int32_t* tiles, int ntiles,
                          const uint32_t* lists, const int32_t* counts, int stride, double* grads, double* gsh,
