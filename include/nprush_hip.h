@@ -299,6 +299,15 @@ int npb_blur(npb_handle b, const float* src_rgba, float* dst_rgba, int32_t w, in
 /* device pointers on the handle's GPU (the output can feed nps_set_target_dev) */
 int npb_blur_dev(npb_handle b, const float* src_dev, float* dst_dev, int32_t w, int32_t hgt, int32_t radius,
                  float sigma, int32_t is_srgb);
+/* npb_blur_dev for callers that cannot allocate device memory (code under
+ * hipGraph stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same arguments and the same
+ * output bits as npb_blur_dev; its 8-bit temp lives in a fixed device array of
+ * 2,097,152 pixels (8 MiB per GPU, 1920x1080 fits) set up by npb_create, so it
+ * calls no hipMalloc/hipFree.  w*hgt above that limit returns NPS_ERR_ARG.
+ * Calls on one GPU share the array: calls that overlap on one GPU race. */
+int npb_blur_dev_noalloc(npb_handle b, const float* src_dev, float* dst_dev, int32_t w, int32_t hgt,
+                         int32_t radius, float sigma, int32_t is_srgb);
 
 /* ---------------------------------------------------------------- Bezier
  * Stroke optimizer: replaces GpuBezierOptimizerManager

@@ -31,7 +31,28 @@ __device__ __forceinline__ float unorm8(float x) {
 }
 }  // namespace
 
-__global__ __launch_bounds__(256) void k_blur_pass(const float4* __restrict__ src, float4* __restrict__ dst,
+// Texel access of the two passes: float4 texels (k/255 per channel), or the
+// 8-bit temp as uchar4 holding k itself.  (float)k / 255 is exactly the value
+// unorm8 produced, so both temps give the same bits.
+__device__ __forceinline__ float4 load_texel(const float4* p, size_t i) { return p[i]; }
+__device__ __forceinline__ float4 load_texel(const uchar4* p, size_t i) {
+    const uchar4 k = p[i];
+    return make_float4((float)k.x / 255.0f, (float)k.y / 255.0f, (float)k.z / 255.0f, 1.0f);
+}
+__device__ __forceinline__ void store_texel(float4* p, size_t i, float r0, float r1, float r2) {
+    p[i] = make_float4(unorm8(r0), unorm8(r1), unorm8(r2), 1.0f);
+}
+__device__ __forceinline__ unsigned char unorm8_k(float x) {
+    if (!(x > 0.0f)) x = 0.0f;
+    if (x > 1.0f) x = 1.0f;
+    return (unsigned char)rintf(x * 255.0f);
+}
+__device__ __forceinline__ void store_texel(uchar4* p, size_t i, float r0, float r1, float r2) {
+    p[i] = make_uchar4(unorm8_k(r0), unorm8_k(r1), unorm8_k(r2), 255);
+}
+
+template <class In, class Out>
+__global__ __launch_bounds__(256) void k_blur_pass(const In* __restrict__ src, Out* __restrict__ dst,
                                                    int W, int H, int dx, int dy, int radius, float sigma,
                                                    int in_srgb, int out_srgb) {
     const int x = blockIdx.x * 16 + (threadIdx.x & 15);
@@ -42,7 +63,7 @@ __global__ __launch_bounds__(256) void k_blur_pass(const float4* __restrict__ sr
     for (int i = -radius; i <= radius; ++i) {
         int cx = min(max(x + dx * i, 0), W - 1);
         int cy = min(max(y + dy * i, 0), H - 1);
-        float4 p = src[(size_t)cy * W + cx];
+        float4 p = load_texel(src, (size_t)cy * W + cx);
         float w = exp_det(-(float)(i * i) / den);
         float c0 = p.x, c1 = p.y, c2 = p.z;
         if (in_srgb) {
@@ -61,15 +82,25 @@ __global__ __launch_bounds__(256) void k_blur_pass(const float4* __restrict__ sr
         r1 = linear_to_srgb(r1);
         r2 = linear_to_srgb(r2);
     }
-    dst[(size_t)y * W + x] = make_float4(unorm8(r0), unorm8(r1), unorm8(r2), 1.0f);
+    store_texel(dst, (size_t)y * W + x, r0, r1, r2);
 }
 
 void launch_blur(hipStream_t st, const float4* src, float4* dst, float4* temp, int W, int H, int radius, float sigma,
                  int is_srgb) {
     if (W <= 0 || H <= 0) return;
     dim3 g((W + 15) / 16, (H + 15) / 16), b(256);
-    hipLaunchKernelGGL(k_blur_pass, g, b, 0, st, src, temp, W, H, 1, 0, radius, sigma, is_srgb, 0);
-    hipLaunchKernelGGL(k_blur_pass, g, b, 0, st, (const float4*)temp, dst, W, H, 0, 1, radius, sigma, 0, is_srgb);
+    hipLaunchKernelGGL((k_blur_pass<float4, float4>), g, b, 0, st, src, temp, W, H, 1, 0, radius, sigma, is_srgb, 0);
+    hipLaunchKernelGGL((k_blur_pass<float4, float4>), g, b, 0, st, (const float4*)temp, dst, W, H, 0, 1, radius,
+                       sigma, 0, is_srgb);
+}
+
+void launch_blur_u8temp(hipStream_t st, const float4* src, float4* dst, uchar4* temp, int W, int H, int radius,
+                        float sigma, int is_srgb) {
+    if (W <= 0 || H <= 0) return;
+    dim3 g((W + 15) / 16, (H + 15) / 16), b(256);
+    hipLaunchKernelGGL((k_blur_pass<float4, uchar4>), g, b, 0, st, src, temp, W, H, 1, 0, radius, sigma, is_srgb, 0);
+    hipLaunchKernelGGL((k_blur_pass<uchar4, float4>), g, b, 0, st, (const uchar4*)temp, dst, W, H, 0, 1, radius,
+                       sigma, 0, is_srgb);
 }
 
 }  // namespace nps

@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1564,6 +1565,47 @@ int npt_read_losses(npt_handle t, double* loss2v) {
 
 // ---------------- target blur ----------------
 
+}  // extern "C"
+
+// Fixed scratch of npb_blur_dev_noalloc: the 8-bit linear temp of the two
+// blur passes (uchar4 per pixel), at most kNoAllocPixels pixels.  Every GPU the
+// library runs on holds its own copy until the process ends; calls on one GPU
+// share it, so calls that overlap on one GPU race.  Nothing in it has to start
+// at zero: the horizontal pass writes every pixel the vertical pass reads.
+namespace {
+constexpr int64_t kNoAllocPixels = 2097152;  // 2^21 pixels (8 MiB): 1920x1080 fits
+constexpr int kMaxScratchDevices = 64;
+}  // namespace
+static __device__ __align__(256) uchar4 scratch[kNoAllocPixels];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on `device` (which must be current), looked up
+// once per device; the host-side name is only a stand-in for it.
+int resolve_scratch(int device, void** out, std::string& err) {
+    if (device < 0 || device >= kMaxScratchDevices) {
+        err = "device index out of range for the no-allocation scratch";
+        return NPS_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return NPS_ERR_HIP;
+        }
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return NPS_OK;
+}
+}  // namespace
+
+extern "C" {
+
 static int npb_fail(npb_handle b, int code, const std::string& msg) {
     b->err = msg;
     return code;
@@ -1593,6 +1635,13 @@ int npb_create(int32_t device, npb_handle* out) {
     npb_ctx* b = new npb_ctx();
     b->device = device;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return NPS_ERR_HIP;
+    }
+    // set-up of npb_blur_dev_noalloc: the scratch address on this device
+    void* sp = nullptr;
+    if (resolve_scratch(device, &sp, b->err) != NPS_OK) {
+        (void)hipStreamDestroy(b->stream);
         delete b;
         return NPS_ERR_HIP;
     }
@@ -1646,6 +1695,26 @@ int npb_blur(npb_handle b, const float* src, float* dst, int32_t w, int32_t hh, 
 int npb_blur_dev(npb_handle b, const float* src, float* dst, int32_t w, int32_t hh, int32_t radius, float sigma,
                  int32_t is_srgb) {
     return npb_run(b, src, dst, w, hh, radius, sigma, is_srgb, true);
+}
+
+int npb_blur_dev_noalloc(npb_handle b, const float* src, float* dst, int32_t w, int32_t hh, int32_t radius,
+                         float sigma, int32_t is_srgb) {
+    if (!b) return NPS_ERR_ARG;
+    if (!src || !dst || w <= 0 || hh <= 0 || radius < 0) return npb_fail(b, NPS_ERR_ARG, "bad blur arguments");
+    if ((int64_t)w * hh > kNoAllocPixels)
+        return npb_fail(b, NPS_ERR_ARG, "npb_blur_dev_noalloc: " + std::to_string(w) + " x " + std::to_string(hh) +
+                                            " pixels exceed the limit of " + std::to_string(kNoAllocPixels) +
+                                            " (use npb_blur_dev)");
+    hipError_t e = hipSetDevice(b->device);
+    if (e != hipSuccess) return npb_fail(b, NPS_ERR_HIP, hipGetErrorString(e));
+    void* sp = nullptr;
+    int rc = resolve_scratch(b->device, &sp, b->err);
+    if (rc != NPS_OK) return rc;
+    launch_blur_u8temp(b->stream, (const float4*)src, (float4*)dst, (uchar4*)sp, w, hh, radius, sigma,
+                       is_srgb ? 1 : 0);
+    if ((e = hipGetLastError()) != hipSuccess) return npb_fail(b, NPS_ERR_HIP, hipGetErrorString(e));
+    if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) return npb_fail(b, NPS_ERR_HIP, hipGetErrorString(e));
+    return NPS_OK;
 }
 
 int nps_dev_alloc(nps_handle h, int64_t bytes, void** out) {

@@ -178,6 +178,10 @@ void launch_adc_ext_deltas(hipStream_t st, float* splats, const AdcExtBuffers& b
 // k_blur.hip: both passes; temp is an H*W float4 scratch image
 void launch_blur(hipStream_t st, const float4* src, float4* dst, float4* temp, int W, int H, int radius, float sigma,
                  int is_srgb);
+// The same two passes with the 8-bit linear temp stored as uchar4 (k of k/255):
+// W*H*4 bytes of temp, bit-identical output.
+void launch_blur_u8temp(hipStream_t st, const float4* src, float4* dst, uchar4* temp, int W, int H, int radius,
+                        float sigma, int is_srgb);
 
 // k_debug.hip
 void launch_debug_math(hipStream_t st, int op, const float* x, const float* y, float* out, uint32_t n);
