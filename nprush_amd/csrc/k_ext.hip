@@ -184,12 +184,11 @@ __device__ __forceinline__ float swap16_sum(float a, float b) {
     return __uint_as_float(r[0]) + __uint_as_float(r[1]);
 }
 
-// Wave-wide sums of 9 floats.  v[0..7] by a transpose reduction: v_permlane32_swap
-// (keep lane bit 5), v_permlane16_swap (bit 4), DPP row_mirror (bit 3), then
-// sums over lane bits 0-2 (row_half_mirror, quad_perm); lanes with (lane & 7) == 0
+// Wave-wide sums of 8 floats by a transpose reduction: v_permlane32_swap (keep
+// lane bit 5), v_permlane16_swap (bit 4), DPP row_mirror (bit 3), then sums
+// over lane bits 0-2 (row_half_mirror, quad_perm); lanes with (lane & 7) == 0
 // return the total of value ((lane>>5)&1)*4 + ((lane>>4)&1)*2 + ((lane>>3)&1).
-// v[8] is summed over all lanes (returned in t8, the same in every lane).
-__device__ __forceinline__ float wave_reduce9(float (&v)[9], int lane, float& t8) {
+__device__ __forceinline__ float wave_reduce8(float (&v)[8], int lane) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) v[k] = swap32_sum(v[k], v[k + 4]);
 #pragma unroll
@@ -204,14 +203,6 @@ __device__ __forceinline__ float wave_reduce9(float (&v)[9], int lane, float& t8
     x = x + dpp_f(x, 2);
     x = x + dpp_f(x, 0);
     x = x + dpp_f(x, 1);
-    float y = v[8];
-    y = y + dpp_f(y, 0);
-    y = y + dpp_f(y, 1);
-    y = y + dpp_f(y, 2);
-    y = y + dpp_f(y, 3);
-    y = swap16_sum(y, y);
-    y = swap32_sum(y, y);
-    t8 = y;
     return x;
 }
 
@@ -459,21 +450,11 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
 #pragma unroll
                 for (int j = 0; j < kFwdGroup; ++j) {
                     const int q = e[j] < 0 ? 0 : e[j];
-                    const float4 p0 = rec4[3 * q];      // scr_x scr_y A B
-                    const float4 p1 = rec4[3 * q + 1];  // C op r g
-                    const float4 p2 = rec4[3 * q + 2];  // b rcut2 ext_x ext_y
-                    const float dx = pxf - p0.x, dy = pyf - p0.y;
-                    const float spx = p0.z * dx + p0.w * dy;
-                    const float spy = p0.w * dx + p1.x * dy;
-                    const float r2 = dx * spx + dy * spy;
-                    // non-short-circuit &: every operand is evaluated, no branch
-                    const bool ok = (e[j] >= 0) & (fabsf(dx) <= p2.z) & (fabsf(dy) <= p2.w) & !(r2 > p2.y);
-                    const float G = exp2_det_in_range(ok ? (-0.5f * r2) * kLog2e : 0.0f);  // r2 <= rcut2 < 12
-                    float a = p1.y * G;
-                    a = a > kExtMaxAlpha ? kExtMaxAlpha : a;
-                    ca[j] = ok ? a : 0.0f;
+                    ca[j] = ext_pair_alpha(pxf, pyf, rec4[3 * q], rec4[3 * q + 1], rec4[3 * q + 2], e[j] >= 0);
                 }
-                // then the front-to-back chain, entry by entry
+                // then the front-to-back chain, entry by entry: a pixel that does
+                // not take the entry composites it with alpha 0, which leaves T
+                // and C unchanged bit for bit (fma(0, x, C) = C)
 #pragma unroll
                 for (int j = 0; j < kFwdGroup; ++j) {
                     if (e[j] < 0) break;  // uniform
@@ -482,20 +463,21 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
                         for (; ck_next <= (e[j] >> 3); ++ck_next) ck[ck_next * NT + tid] = T;
                     const float a = ca[j];
                     const bool take = alive & !(a < kExtMinAlpha);
-                    const float tT = T * (1.0f - a);
+                    const float tT = fmaf(-T, a, T);
                     const bool dies = take & (tT < kExtTMin);
                     const bool inc = take & !dies;
+                    const float Ta = T * (inc ? a : 0.0f);
                     const float4 p1 = rec4[3 * e[j] + 1];
                     const float cb = rec4[3 * e[j] + 2].x;
-                    Cr = inc ? Cr + (T * a) * p1.z : Cr;
-                    Cg = inc ? Cg + (T * a) * p1.w : Cg;
-                    Cb = inc ? Cb + (T * a) * cb : Cb;
+                    Cr = fmaf(Ta, p1.z, Cr);
+                    Cg = fmaf(Ta, p1.w, Cg);
+                    Cb = fmaf(Ta, cb, Cb);
                     T = inc ? tT : T;
                     last = inc ? b0 + e[j] + 1 : last;
                     stop = dies ? b0 + e[j] + 1 : stop;
                     alive = alive & !dies;
-                    const uint64_t any = __ballot(inc) != 0ull ? 1ull : 0ull;
-                    mask[e[j] >> 6] |= any << (e[j] & 63);
+                    const uint64_t bal = __ballot(inc);
+                    if (bal != 0ull) mask[e[j] >> 6] |= 1ull << (e[j] & 63);
                 }
                 wave_alive = __ballot(alive) != 0ull;
             }
@@ -589,7 +571,9 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
                 uint32_t incb = 0u;
                 // an entry whose bit is 0 changed no lane's transmittance before
                 // that lane's last composited entry (it would have been composited):
-                // skipped.  Branch-free inside, as in the forward.
+                // skipped.  Alpha and T are the forward's bit for bit (same
+                // operations); a lane that did not composite the entry gets alpha
+                // and G zero, which zeroes all of its terms below.
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     ra[j] = 0.0f;
@@ -597,26 +581,22 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
                     rT[j] = Tc;
                     if ((bits >> j) & 1u) {
                         const int e = b0 + 8 * c + j;
-                        const float4 p0 = rec4[3 * (8 * c + j)];
-                        const float4 p1 = rec4[3 * (8 * c + j) + 1];
-                        const float4 p2 = rec4[3 * (8 * c + j) + 2];
-                        const float dx = pxf - p0.x, dy = pyf - p0.y;
-                        const float spx = p0.z * dx + p0.w * dy;
-                        const float spy = p0.w * dx + p1.x * dy;
-                        const float r2 = dx * spx + dy * spy;
-                        const bool in = (e < last) & (fabsf(dx) <= p2.z) & (fabsf(dy) <= p2.w) & !(r2 > p2.y);
-                        const float G = exp2_det_in_range(in ? (-0.5f * r2) * kLog2e : 0.0f);  // r2 <= rcut2 < 12
-                        float a = p1.y * G;
-                        a = a > kExtMaxAlpha ? kExtMaxAlpha : a;
-                        const bool ok = in & !(a < kExtMinAlpha);
+                        const int q = 8 * c + j;
+                        float G;
+                        const float a = ext_pair_alpha(pxf, pyf, rec4[3 * q], rec4[3 * q + 1], rec4[3 * q + 2],
+                                                       e < last, &G);
+                        const bool ok = !(a < kExtMinAlpha);
                         ra[j] = ok ? a : 0.0f;
                         rG[j] = ok ? G : 0.0f;
                         incb |= (ok ? 1u : 0u) << j;
-                        Tc = ok ? Tc * (1.0f - a) : Tc;
+                        Tc = ok ? fmaf(-Tc, a, Tc) : Tc;
                     }
                 }
+                // the opacity terms of the chunk's entries, reduced together after it
+                float o8[8];
 #pragma unroll
                 for (int j = 7; j >= 0; --j) {
+                    o8[j] = 0.0f;
                     if ((bits >> j) & 1u) {
                         const int jj = 8 * c + j;
                         const bool inc = (incb >> j) & 1u;
@@ -626,40 +606,44 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
                         const float4 p1 = rec4[3 * jj + 1];
                         const float4 p2 = rec4[3 * jj + 2];
                         const float fr = p1.z, fg = p1.w, fb = p2.x, fop = p1.y;
-                        // per-pair gradient terms (alpha and T above are the forward's
-                        // values bit for bit; these terms only need FP32 accuracy, so
-                        // they use fused multiply-adds and shared subexpressions):
-                        // with s = dC . rgb, dL/dalpha = T (s - dT) and the seed of
-                        // the transmittance before the entry is dT + alpha (s - dT)
+                        // per-pair gradient terms (only FP32 accuracy needed): with
+                        // s = dC . rgb, dL/dalpha = T (s - dT), and the seed of the
+                        // transmittance before the entry is dT + alpha (s - dT)
                         const float dx = pxf - p0.x, dy = pyf - p0.y;
                         const float spx = fmaf(p0.z, dx, p0.w * dy);
                         const float spy = fmaf(p0.w, dx, p1.x * dy);
-                        const float Tk = rT[j], a = ra[j], G = rG[j];
+                        const float Tk = rT[j], a = ra[j];
+                        // no gradient through a clamped alpha
+                        const float G = fop * rG[j] > kExtMaxAlpha ? 0.0f : rG[j];
                         const float u = fmaf(dCb, fb, fmaf(dCg, fg, dCr * fr)) - dT;
                         const float da = Tk * u;
-                        dT = inc ? fmaf(a, u, dT) : dT;
-                        const bool geo = inc & !(fop * G > kExtMaxAlpha);  // alpha not clamped
-                        const float gr2 = ((-0.5f * fop) * G) * da;       // dL / dr^2
+                        dT = fmaf(a, u, dT);
+                        const float gr2 = ((-0.5f * fop) * G) * da;  // dL / dr^2
                         const float Tka = Tk * a;
                         const float m2 = -2.0f * gr2, gx = gr2 * dx;
-                        float gv[9];
-                        gv[5] = inc ? dCr * Tka : 0.0f;
-                        gv[6] = inc ? dCg * Tka : 0.0f;
-                        gv[7] = inc ? dCb * Tka : 0.0f;
-                        gv[8] = geo ? da * G : 0.0f;
-                        gv[0] = geo ? m2 * spx : 0.0f;
-                        gv[1] = geo ? m2 * spy : 0.0f;
-                        gv[2] = geo ? gx * dx : 0.0f;
-                        gv[3] = geo ? (2.0f * gx) * dy : 0.0f;
-                        gv[4] = geo ? (gr2 * dy) * dy : 0.0f;
-                        float t8;
-                        float tot = wave_reduce9(gv, lane, t8);
+                        float gv[8];
+                        gv[5] = dCr * Tka;
+                        gv[6] = dCg * Tka;
+                        gv[7] = dCb * Tka;
+                        gv[0] = m2 * spx;
+                        gv[1] = m2 * spy;
+                        gv[2] = gx * dx;
+                        gv[3] = (2.0f * gx) * dy;
+                        gv[4] = (gr2 * dy) * dy;
+                        o8[j] = da * G;
+                        const float tot = wave_reduce8(gv, lane);
                         if ((lane & 7) == 0) {
                             int idx = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
                             s_part[wid][jj][idx] = tot;
                         }
-                        if (lane == 0) s_part[wid][jj][8] = t8;
                     }
+                }
+                {
+                    // lanes with (lane & 7) == 0 hold the opacity total of entry
+                    // 8c + k, k from lane bits 5, 4, 3 (zero for skipped entries)
+                    const float tot = wave_reduce8(o8, lane);
+                    const int k = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+                    if ((lane & 7) == 0 && ((bits >> k) & 1u)) s_part[wid][8 * c + k][8] = tot;
                 }
                 c = c_next;
             }

@@ -61,6 +61,33 @@ constexpr float kExtBinScale = 1.00001f;
 constexpr float kExtBinPad = 1e-6f;
 static_assert(sizeof(ExtRec) == 48, "ExtRec layout");
 
+// Gaussian falloff exponent: exp(-r^2/2) = exp2(r^2 * kExtNegHalfLog2e)
+// (-0.5 log2 e; -0.5 * log2e is exact in FP32).
+constexpr float kExtNegHalfLog2e = -0.5f * kLog2e;
+
+// EXT alpha of the pixel centre (pxf, pyf) and one record, given as its three
+// float4 pieces p0 = (scr_x, scr_y, A, B), p1 = (C, op, r, g),
+// p2 = (b, rcut2, ext_x, ext_y):
+//   dx, dy = pixel - centre;  r^2 = fma(dx, fma(A, dx, B dy), dy fma(B, dx, C dy));
+//   G = exp2(r^2 * kExtNegHalfLog2e);  alpha = min(op G, 0.99);
+// 0 when `use` is false, the centre of the pixel lies outside the 3-sigma box
+// or r^2 > rcut2 (where alpha < 1/255 is guaranteed).  *Gout = G, or 0 where
+// alpha is 0 by those tests.  Branch-free.
+__device__ __forceinline__ float ext_pair_alpha(float pxf, float pyf, const float4& p0, const float4& p1,
+                                                const float4& p2, bool use, float* Gout = nullptr) {
+    const float dx = pxf - p0.x, dy = pyf - p0.y;
+    const float spx = fmaf(p0.z, dx, p0.w * dy);
+    const float spy = fmaf(p0.w, dx, p1.x * dy);
+    const float r2 = fmaf(dx, spx, dy * spy);
+    // non-short-circuit &: every operand is evaluated, no branch
+    const bool ok = use & (fabsf(dx) <= p2.z) & (fabsf(dy) <= p2.w) & !(r2 > p2.y);
+    const float G = exp2_det_in_range(ok ? r2 * kExtNegHalfLog2e : 0.0f);  // r2 <= rcut2 < 12
+    float a = p1.y * G;
+    a = a > kExtMaxAlpha ? kExtMaxAlpha : a;
+    if (Gout) *Gout = ok ? G : 0.0f;
+    return ok ? a : 0.0f;
+}
+
 struct ExtProj {
     bool valid;
     float s[3], qn[4], qlen, op;

@@ -30,25 +30,37 @@
 #endif
 #include "ref_math.h"
 
+/* FP32 mode: the GPU's operations.  Pair falloff exp(-r2/2) = exp2(r2 * c) with
+ * c = -0.5 log2(e) (exact in FP32), products and sums of r2, the transmittance
+ * and the colour as the explicit fused multiply-adds the kernel uses. */
+#define EXT_NEG_HALF_LOG2E (-0.5f * 0x1.715476p+0f)
 #define R float
 #define SFX(n) n##_xf32
 #define REXP rm_expf
 #define RSQRT sqrtf
+#define RFMA fmaf
+#define RGAUSS(r2) rm_exp2f((r2) * EXT_NEG_HALF_LOG2E)
 #include "oracle_ext_core.inc"
 #undef R
 #undef SFX
 #undef REXP
 #undef RSQRT
+#undef RFMA
+#undef RGAUSS
 
 #define R double
 #define SFX(n) n##_xf64
 #define REXP exp
 #define RSQRT sqrt
+#define RFMA fma
+#define RGAUSS(r2) exp(-0.5 * (r2))
 #include "oracle_ext_core.inc"
 #undef R
 #undef SFX
 #undef REXP
 #undef RSQRT
+#undef RFMA
+#undef RGAUSS
 
 /* Camera centre from the inverse view-projection: VP^-1 (0,0,1,0) is the
  * homogeneous centre, i.e. column 2 of vp_inv, dehomogenised in double. */
