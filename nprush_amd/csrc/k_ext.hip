@@ -18,6 +18,8 @@
 //                      opacity) through the projection to the 64 parameters, in
 //                      FP64, added to the FP32 gradient buffer;
 //   k_adam_ext       : Adam with the per-group learning rates.
+#include <algorithm>
+
 #include "nps_common.h"
 #include "nps_ext.h"
 #include "nps_kernels.h"
@@ -160,6 +162,357 @@ __global__ __launch_bounds__(256) void k_ext_keys(uint32_t n, const uint64_t* __
         const uint64_t id = (uint64_t)(uint32_t)__shfl((int)my_id, (int)s, 64);
         if (g < total && pos0 + g < cap) keys[pos0 + g] = ((uint64_t)(ty * tiles_x + tx) << 32) | id;
     }
+}
+
+// ---------------------------------------------------------------- binning
+//
+// Tile lists in (tile, depth, id) order without sorting 8-byte keys.  A key is
+// (tile, splat) for every tile of a splat's rectangle; keys are enumerated in
+// depth-rank order (a splat's tiles row by row), and each tile's list must keep
+// that order.  Two stable counting passes, the first fused into the
+// enumeration:
+//   k_bin_row_count  : per workgroup of 1024 ranks, its keys per tile row: a
+//                      splat adds its width to rows y0..y1-1 (a difference
+//                      array in LDS);
+//   exclusive scan of the row-major counts: where each (row, workgroup) starts
+//                      in the row buckets;
+//   k_bin_row_emit   : the same workgroups enumerate their keys in rank order
+//                      and write the 4-byte item tx << id_bits | id at its
+//                      place in its row bucket (a running place per wave and
+//                      row; the lanes of a round ranked by a match mask).  The
+//                      tiles of one row of a splat are consecutive keys in one
+//                      bucket: runs of consecutive stores;
+//   k_bin_col_count  : per 4096 items, the count per tile column and per tile
+//                      (an item's row is the bucket its position lies in);
+//   exclusive scans: column positions; tile ranges in column-major order;
+//   k_bin_col_scatter: a stable scatter on the column writing splat ids.
+// Both passes keep the enumeration order within a bucket, so the output holds
+// the tiles column by column, each tile's ids in (depth, id) order.
+constexpr int kBinMaxBits = 8;      // rows and columns of the tile grid: at most 256 each
+constexpr int kBinThreads = 1024;   // ranks per workgroup of the row kernels
+constexpr int kBinWaves = kBinThreads / 64;
+constexpr int kBinColItems = 16;
+constexpr int kBinColBlock = 256 * kBinColItems;  // items per workgroup of the column kernels
+
+// counts1[y * nblk + blk] = keys of ranks [1024 blk, 1024 blk + 1024) in tile row y.
+__global__ __launch_bounds__(kBinThreads) void k_bin_row_count(uint32_t n, const uint64_t* __restrict__ order,
+                                                               const uint32_t* __restrict__ rank_counts,
+                                                               const uint4* __restrict__ rects, uint32_t ny,
+                                                               uint32_t* __restrict__ counts1) {
+    __shared__ int s_diff[(1 << kBinMaxBits) + 1];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t k = tid; k <= ny; k += kBinThreads) s_diff[k] = 0;
+    __syncthreads();
+    const uint32_t r = blockIdx.x * kBinThreads + tid;
+    if (r < n && rank_counts[r]) {
+        const uint4 q = rects[(uint32_t)order[r]];
+        atomicAdd(&s_diff[q.z], (int)(q.y - q.x));
+        atomicAdd(&s_diff[q.w], -(int)(q.y - q.x));
+    }
+    __syncthreads();
+    if (tid < 64) {  // prefix over ny <= 256 rows, one wave
+        int carry = 0;
+        for (uint32_t y0 = 0; y0 < ny; y0 += 64) {
+            const uint32_t y = y0 + tid;
+            int x = y < ny ? s_diff[y] : 0;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(x, o, 64);
+                if ((int)tid >= o) x += t;
+            }
+            if (y < ny) counts1[(size_t)y * gridDim.x + blockIdx.x] = (uint32_t)(carry + x);
+            carry += __shfl(x, 63, 64);
+        }
+    }
+}
+
+// The keys of the same workgroups, written as items at their row places.
+// offs1: exclusive scan of counts1 (row-major).  Items beyond cap are dropped
+// (*overflow = 1).
+__global__ __launch_bounds__(kBinThreads) void k_bin_row_emit(uint32_t n, const uint64_t* __restrict__ order,
+                                                              const uint32_t* __restrict__ rank_counts,
+                                                              const uint4* __restrict__ rects, uint32_t ny,
+                                                              int row_bits, int id_bits,
+                                                              const uint32_t* __restrict__ offs1,
+                                                              uint32_t* __restrict__ items, uint64_t cap,
+                                                              uint32_t* __restrict__ overflow) {
+    __shared__ int s_row[kBinWaves][(1 << kBinMaxBits) + 1];  // per wave: counts, then running places
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
+    for (uint32_t k = tid; k < kBinWaves * ((1u << kBinMaxBits) + 1u); k += kBinThreads) (&s_row[0][0])[k] = 0;
+    __syncthreads();
+    const uint32_t mine = blockIdx.x * kBinThreads + tid;
+    const uint32_t my_cnt = mine < n ? rank_counts[mine] : 0u;
+    uint32_t my_id = 0u, my_x0 = 0u, my_w = 1u, my_y0 = 0u;
+    float my_rcp = 1.0f;
+    if (my_cnt) {
+        my_id = (uint32_t)order[mine];
+        const uint4 q = rects[my_id];
+        my_x0 = q.x;
+        my_w = q.y - q.x;
+        my_y0 = q.z;
+        my_rcp = 1.0f / (float)my_w;
+        atomicAdd(&s_row[wid][q.z], (int)my_w);
+        atomicAdd(&s_row[wid][q.w], -(int)my_w);
+    }
+    __syncthreads();
+    {   // wave w's count per row: prefix of its difference array (lanes over rows)
+        int carry = 0;
+        for (uint32_t y0 = 0; y0 < ny; y0 += 64) {
+            const uint32_t y = y0 + lane;
+            int x = y < ny ? s_row[wid][y] : 0;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(x, o, 64);
+                if ((int)lane >= o) x += t;
+            }
+            const int tot = __shfl(x, 63, 64);
+            __builtin_amdgcn_wave_barrier();
+            if (y < ny) s_row[wid][y] = carry + x;
+            __builtin_amdgcn_wave_barrier();
+            carry += tot;
+        }
+    }
+    __syncthreads();
+    for (uint32_t y = tid; y < ny; y += kBinThreads) {  // places: the waves in order from the workgroup's place
+        uint32_t place = offs1[(size_t)y * gridDim.x + blockIdx.x];
+        for (uint32_t w = 0; w < (uint32_t)kBinWaves; ++w) {
+            const uint32_t c = (uint32_t)s_row[w][y];
+            s_row[w][y] = (int)place;
+            place += c;
+        }
+    }
+    __syncthreads();
+    // enumerate the wave's keys in rank order, 64 per round
+    uint32_t incl = my_cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o, 64);
+        if ((int)lane >= o) incl += y;
+    }
+    const uint32_t total = __shfl(incl, 63, 64);
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (uint32_t g0 = 0u; g0 < total; g0 += 64u) {
+        const uint32_t g = g0 + lane;
+        const bool valid = g < total;
+        // s = first lane whose inclusive count exceeds g (63 when g >= total)
+        uint32_t sl = 0u;
+#pragma unroll
+        for (uint32_t b = 32u; b > 0u; b >>= 1) {
+            const uint32_t v = (uint32_t)__shfl((int)incl, (int)(sl + b - 1u), 64);
+            if (v <= g) sl += b;
+        }
+        const uint32_t k = g - ((uint32_t)__shfl((int)incl, (int)sl, 64) - (uint32_t)__shfl((int)my_cnt, (int)sl, 64));
+        const uint32_t w = (uint32_t)__shfl((int)my_w, (int)sl, 64);
+        const float rcp = __shfl(my_rcp, (int)sl, 64);
+        // k / w from the float reciprocal (within one of the quotient for k < 2^24), corrected
+        uint32_t qq = (uint32_t)((float)k * rcp);
+        int32_t rem = (int32_t)k - (int32_t)(qq * w);
+        if (rem < 0) {
+            --qq;
+            rem += (int32_t)w;
+        } else if (rem >= (int32_t)w) {
+            ++qq;
+            rem -= (int32_t)w;
+        }
+        // every lane runs the cross-lane reads (a read from an inactive lane returns 0)
+        const uint32_t ty_all = (uint32_t)__shfl((int)my_y0, (int)sl, 64) + qq;
+        const uint32_t tx = (uint32_t)__shfl((int)my_x0, (int)sl, 64) + (uint32_t)rem;
+        const uint32_t id = (uint32_t)__shfl((int)my_id, (int)sl, 64);
+        const uint32_t ty = valid ? ty_all : 0u;
+        uint64_t peers = __ballot(valid);
+        for (int q = 0; q < row_bits; ++q) {
+            const uint64_t bal = __ballot((ty >> q) & 1u);
+            peers &= ((ty >> q) & 1u) ? bal : ~bal;
+        }
+        const uint32_t before = (uint32_t)__popcll(peers & lt_mask);
+        const uint32_t place = valid ? (uint32_t)s_row[wid][ty] : 0u;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && before == 0u) s_row[wid][ty] = (int)(place + (uint32_t)__popcll(peers));
+        __builtin_amdgcn_wave_barrier();
+        if (valid) {
+            const uint32_t pos = place + before;
+            if ((uint64_t)pos < cap) items[pos] = (tx << id_bits) | id;
+            else *overflow = 1u;
+        }
+    }
+}
+
+// Per 4096 items: counts2[x * nblk + blk] (tile column x) and tile_cnt[tile] +=.
+// The row buckets start at offs1[y * nblk1] (y < ny; the item count after the
+// last); a workgroup's items lie in the rows y_first..y_last of its first and
+// last item.  Per round of 64 items a wave groups the lanes with equal
+// (row, column) (ballots); the first lane of each group adds its size.
+__global__ __launch_bounds__(256) void k_bin_col_count(const uint32_t* __restrict__ items, uint64_t m,
+                                                       const uint64_t* __restrict__ m_dev,
+                                                       const uint32_t* __restrict__ offs1, uint32_t nblk1,
+                                                       uint32_t ny, uint32_t nx, int row_bits, int col_bits,
+                                                       int id_bits, uint32_t* __restrict__ counts2,
+                                                       uint32_t* __restrict__ tile_cnt) {
+    constexpr int kLocal = 8192;  // per-workgroup tile counters (rows in the workgroup x nx)
+    __shared__ uint32_t s_start[(1 << kBinMaxBits) + 1];
+    __shared__ uint32_t s_c[256];
+    __shared__ uint32_t s_t[kLocal];
+    if (m_dev) m = min(*m_dev, m);
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    for (uint32_t y = tid; y < ny; y += 256) s_start[y] = offs1[(size_t)y * nblk1];
+    s_c[tid] = 0u;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * kBinColBlock;
+    if (base >= m) {
+        if (tid < nx) counts2[(size_t)tid * gridDim.x + blockIdx.x] = 0u;
+        return;
+    }
+    const uint32_t last = (uint32_t)min<uint64_t>(base + kBinColBlock, m) - 1u;
+    auto row_of = [&](uint32_t p) -> uint32_t {  // largest y with start[y] <= p
+        uint32_t y = 0u;
+        for (uint32_t b = 1u << kBinMaxBits; b > 0u; b >>= 1)
+            if (y + b < ny && s_start[y + b] <= p) y += b;
+        return y;
+    };
+    const uint32_t y_first = row_of((uint32_t)base), y_last = row_of(last);
+    const bool local = (y_last - y_first + 1u) * nx <= (uint32_t)kLocal;
+    if (local)
+        for (uint32_t k = tid; k < (y_last - y_first + 1u) * nx; k += 256) s_t[k] = 0u;
+    __syncthreads();
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t it[kBinColItems];
+#pragma unroll
+    for (int r = 0; r < kBinColItems; ++r) {
+        const uint64_t pos = base + (uint64_t)r * 256 + tid;
+        it[r] = pos < m ? items[pos] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kBinColItems; ++r) {
+        const uint64_t pos = base + (uint64_t)r * 256 + tid;
+        const bool valid = pos < m;
+        const uint64_t live = __ballot(valid);
+        const uint32_t x = it[r] >> id_bits;
+        const uint32_t y = (y_first == y_last) ? y_first : row_of((uint32_t)pos);
+        uint64_t peers = live;
+        if (y_first != y_last)  // a row boundary inside the workgroup: group by row too
+            for (int q = 0; q < row_bits; ++q) {
+                const uint64_t bal = __ballot((y >> q) & 1u);
+                peers &= ((y >> q) & 1u) ? bal : ~bal;
+            }
+        for (int q = 0; q < col_bits; ++q) {
+            const uint64_t bal = __ballot((x >> q) & 1u);
+            peers &= ((x >> q) & 1u) ? bal : ~bal;
+        }
+        if (valid && (peers & lt_mask) == 0ull) {
+            const uint32_t c = (uint32_t)__popcll(peers);
+            atomicAdd(&s_c[x], c);
+            if (local) atomicAdd(&s_t[(y - y_first) * nx + x], c);
+            else atomicAdd(&tile_cnt[y * nx + x], c);
+        }
+    }
+    __syncthreads();
+    if (tid < nx) counts2[(size_t)tid * gridDim.x + blockIdx.x] = s_c[tid];
+    if (local)
+        for (uint32_t k = tid; k < (y_last - y_first + 1u) * nx; k += 256) {
+            const uint32_t c = s_t[k];
+            if (c) atomicAdd(&tile_cnt[(y_first + k / nx) * nx + k % nx], c);
+        }
+}
+
+// Tile ranges, tiles taken column by column (the output order):
+// [start, end) of tile y * nx + x = the counts of the tiles before it in
+// column-major order (one workgroup).
+__global__ __launch_bounds__(1024) void k_bin_ranges(const uint32_t* __restrict__ tile_cnt, uint32_t nx, uint32_t ny,
+                                                     uint32_t* __restrict__ ranges) {
+    __shared__ uint32_t s_w[16];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
+    const uint32_t ntiles = nx * ny;
+    uint32_t carry = 0u;
+    for (uint32_t c0 = 0; c0 < ntiles; c0 += 1024u) {
+        const uint32_t c = c0 + tid;
+        const uint32_t t = c < ntiles ? (c % ny) * nx + c / ny : 0u;
+        const uint32_t v = c < ntiles ? tile_cnt[t] : 0u;
+        uint32_t x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if ((int)lane >= o) x += y;
+        }
+        if (lane == 63u) s_w[wid] = x;
+        __syncthreads();
+        uint32_t wb = 0u, tot = 0u;
+        for (uint32_t w = 0; w < 16u; ++w) {
+            if (w < wid) wb += s_w[w];
+            tot += s_w[w];
+        }
+        if (c < ntiles) {
+            ranges[2 * t] = carry + wb + x - v;
+            ranges[2 * t + 1] = carry + wb + x;
+        }
+        carry += tot;
+        __syncthreads();
+    }
+}
+
+// Stable scatter of the items on their column (offs2: exclusive scan of
+// counts2), writing the splat ids: 4 waves x 1024 items, ranked per wave as in
+// k_radix_scatter.
+__global__ __launch_bounds__(256) void k_bin_col_scatter(const uint32_t* __restrict__ items, uint64_t m,
+                                                         const uint64_t* __restrict__ m_dev, uint32_t nx, int col_bits,
+                                                         int id_bits, const uint32_t* __restrict__ offs2,
+                                                         uint32_t* __restrict__ ids) {
+    __shared__ uint32_t wcnt[4][256];
+    if (m_dev) m = min(*m_dev, m);
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
+    for (uint32_t k = tid; k < 4 * 256; k += 256) (&wcnt[0][0])[k] = 0u;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * kBinColBlock;
+    if (base >= m) return;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint64_t wbase = base + (uint64_t)wid * (kBinColBlock / 4);
+    const uint32_t idmask = id_bits >= 32 ? 0xFFFFFFFFu : ((1u << id_bits) - 1u);
+    uint32_t it[kBinColItems], rank[kBinColItems];
+#pragma unroll
+    for (int r = 0; r < kBinColItems; ++r) {
+        const uint64_t pos = wbase + (uint64_t)r * 64 + lane;
+        it[r] = pos < m ? items[pos] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kBinColItems; ++r) {
+        const uint64_t pos = wbase + (uint64_t)r * 64 + lane;
+        const bool valid = pos < m;
+        const uint32_t d = it[r] >> id_bits;
+        uint64_t peers = __ballot(valid);
+        for (int q = 0; q < col_bits; ++q) {
+            const uint64_t bal = __ballot((d >> q) & 1u);
+            peers &= ((d >> q) & 1u) ? bal : ~bal;
+        }
+        const uint32_t before = (uint32_t)__popcll(peers & lt_mask);
+        const uint32_t cnt = valid ? wcnt[wid][d] : 0u;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && before == 0u) wcnt[wid][d] = cnt + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+        rank[r] = cnt + before;
+    }
+    __syncthreads();
+    if (tid < nx) {
+        const uint32_t d = tid;
+        const uint32_t g = offs2[(size_t)d * gridDim.x + blockIdx.x];
+        const uint32_t c0 = wcnt[0][d], c1 = wcnt[1][d], c2 = wcnt[2][d];
+        wcnt[0][d] = g;
+        wcnt[1][d] = g + c0;
+        wcnt[2][d] = g + c0 + c1;
+        wcnt[3][d] = g + c0 + c1 + c2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kBinColItems; ++r) {
+        const uint64_t pos = wbase + (uint64_t)r * 64 + lane;
+        if (pos < m) ids[wcnt[wid][it[r] >> id_bits] + rank[r]] = it[r] & idmask;
+    }
+}
+
+// Splat ids of sorted tile << 32 | id keys (the fallback path).
+__global__ void k_keys_to_ids(const uint64_t* __restrict__ keys, uint64_t m, const uint64_t* __restrict__ m_dev,
+                              uint32_t* __restrict__ ids) {
+    if (m_dev) m = min(*m_dev, m);
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < m) ids[j] = (uint32_t)keys[j];
 }
 
 // ---------------------------------------------------------------- composite
@@ -341,15 +694,17 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
     const uint32_t bx0 = tx * TILE + (wid % BW) * 8, by0 = ty * TILE + (wid / BW) * 8;
     const float blk_x0 = px_to_x(bx0, v.W, v.aspect), blk_x1 = px_to_x(bx0 + 7u, v.W, v.aspect);
     const float blk_y0 = py_to_y(by0 + 7u, v.H), blk_y1 = py_to_y(by0, v.H);
-    // this tile's batch slots: [start/BATCH + tile, ...) never overlap another tile's
-    const size_t slot0 = (size_t)(start / BATCH) + tile;
+    // this tile's batch slots: [start/BATCH + q, ...), q the tile's place in the
+    // order the lists are stored in (starts grow with q), never overlap another tile's
+    const uint32_t qplace = A.col_major ? tx * v.tiles_y + ty : tile;
+    const size_t slot0 = (size_t)(start / BATCH) + qplace;
 
     // Batch staging: piece q = tid + k*NT (< 3*BATCH) is float4 (q % 3) of record
     // q / 3.  The next batch's pieces are loaded into registers while the current
     // batch is composited, with the ids of the batch after that.
     auto load_id = [&](int b0, int k) -> uint32_t {
         const int q = tid + k * NT, j = q / 3;
-        return (q < 3 * BATCH && b0 >= 0 && b0 + j < len) ? (uint32_t)A.keys[start + (uint32_t)(b0 + j)] : 0u;
+        return (q < 3 * BATCH && b0 >= 0 && b0 + j < len) ? A.ids[start + (uint32_t)(b0 + j)] : 0u;
     };
     auto load_rec = [&](uint32_t id, int b0, int k) -> float4 {
         const int q = tid + k * NT, j = q / 3;
@@ -1267,6 +1622,55 @@ void launch_ext_keys(hipStream_t st, uint32_t n, const uint64_t* order, const ui
     if (n == 0) return;
     hipLaunchKernelGGL(k_ext_keys, dim3((n + 255) / 256), dim3(256), 0, st, n, order, rank_counts, rects, tiles_x,
                        offsets, keys, cap, overflow);
+}
+
+// bits needed to write x (0 for x = 0)
+static int bits_for_u32(uint32_t x) { return x ? 32 - __builtin_clz(x) : 0; }
+
+bool ext_bin_fits(uint32_t n, uint32_t tiles_x, uint32_t tiles_y, uint64_t cap) {
+    const int xb = bits_for_u32(tiles_x > 0 ? tiles_x - 1 : 0), yb = bits_for_u32(tiles_y > 0 ? tiles_y - 1 : 0);
+    const int idb = bits_for_u32(n > 1 ? n - 1 : 0);
+    return xb <= kBinMaxBits && yb <= kBinMaxBits && xb + idb <= 32 && cap < 0xFFFFFFFFull;
+}
+
+size_t ext_bin_scan_words(uint32_t n, uint32_t tiles_x, uint32_t tiles_y, uint64_t cap) {
+    const size_t nb1 = ((size_t)n + kBinThreads - 1) / kBinThreads, nb2 = (size_t)((cap + kBinColBlock - 1) / kBinColBlock);
+    // row offsets (+ total) followed by the column offsets (+ total)
+    return (size_t)tiles_y * std::max<size_t>(nb1, 1) + (size_t)tiles_x * std::max<size_t>(nb2, 1) + 2;
+}
+
+void ext_bin(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts, const uint4* rects,
+             uint32_t tiles_x, uint32_t tiles_y, uint64_t cap, const uint64_t* m_dev, uint32_t* items, uint32_t* ids,
+             uint32_t* ranges, uint32_t* cnt, uint32_t* off, uint32_t* tile_cnt, void* scan_tmp,
+             uint32_t* overflow) {
+    const int xb = bits_for_u32(tiles_x > 0 ? tiles_x - 1 : 0), yb = bits_for_u32(tiles_y > 0 ? tiles_y - 1 : 0);
+    const int idb = bits_for_u32(n > 1 ? n - 1 : 0);
+    const uint32_t ntiles = tiles_x * tiles_y;
+    launch_fill_u32(st, tile_cnt, 0u, ntiles);
+    if (n > 0 && cap > 0 && ntiles > 0) {
+        const uint32_t nb1 = (n + kBinThreads - 1) / kBinThreads;
+        hipLaunchKernelGGL(k_bin_row_count, dim3(nb1), dim3(kBinThreads), 0, st, n, order, rank_counts, rects, tiles_y,
+                           cnt);
+        exclusive_scan_u32(st, cnt, off, (uint64_t)tiles_y * nb1, scan_tmp);
+        hipLaunchKernelGGL(k_bin_row_emit, dim3(nb1), dim3(kBinThreads), 0, st, n, order, rank_counts, rects, tiles_y,
+                           yb, idb, (const uint32_t*)off, items, cap, overflow);
+        const uint32_t nb2 = (uint32_t)((cap + kBinColBlock - 1) / kBinColBlock);
+        // column counts reuse cnt; their offsets follow the row offsets (still read below)
+        uint32_t* off2 = off + (size_t)tiles_y * nb1 + 1;
+        hipLaunchKernelGGL(k_bin_col_count, dim3(nb2), dim3(256), 0, st, (const uint32_t*)items, cap, m_dev,
+                           (const uint32_t*)off, nb1, tiles_y, tiles_x, yb, xb, idb, cnt, tile_cnt);
+        exclusive_scan_u32(st, cnt, off2, (uint64_t)tiles_x * nb2, scan_tmp);
+        hipLaunchKernelGGL(k_bin_col_scatter, dim3(nb2), dim3(256), 0, st, (const uint32_t*)items, cap, m_dev,
+                           tiles_x, xb, idb, (const uint32_t*)off2, ids);
+    }
+    if (ntiles > 0)
+        hipLaunchKernelGGL(k_bin_ranges, dim3(1), dim3(1024), 0, st, (const uint32_t*)tile_cnt, tiles_x, tiles_y,
+                           ranges);
+}
+
+void launch_keys_to_ids(hipStream_t st, const uint64_t* keys, uint64_t m, const uint64_t* m_dev, uint32_t* ids) {
+    if (m == 0) return;
+    hipLaunchKernelGGL(k_keys_to_ids, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, keys, m, m_dev, ids);
 }
 
 uint32_t ext_order_slots(uint32_t tiles_x, uint32_t tiles_y) {

@@ -16,6 +16,8 @@
 //                     counter carries the rank across rounds.  Wave offsets per
 //                     digit are then prefix-summed across the 4 waves.  Equal
 //                     digits keep their input order, so passes compose (LSD).
+#include <algorithm>
+
 #include "nps_common.h"
 #include "nps_kernels.h"
 
@@ -151,13 +153,15 @@ void exclusive_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint6
 
 // m_dev (optional): the key count lives in device memory (capacity m); the
 // grid covers the capacity and blocks past the count contribute nothing.
+// RADIX: 256 (8-bit digits) or 2048 (11-bit digits).
+template <int RADIX>
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t* __restrict__ keys, uint64_t m,
                                                              const uint64_t* __restrict__ m_dev, int shift,
                                                              uint32_t mask, uint32_t nblk,
                                                              uint32_t* __restrict__ hist) {
-    __shared__ uint32_t h[kRadix];
+    __shared__ uint32_t h[RADIX];
     if (m_dev) m = min(*m_dev, m);
-    h[threadIdx.x] = 0u;
+    for (int d = threadIdx.x; d < RADIX; d += kSortThreads) h[d] = 0u;
     __syncthreads();
     uint64_t base = (uint64_t)blockIdx.x * kSortBlock;
 #pragma unroll
@@ -169,10 +173,10 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t* __r
         }
     }
     __syncthreads();
-    hist[(uint64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+    for (int d = threadIdx.x; d < RADIX; d += kSortThreads) hist[(uint64_t)d * nblk + blockIdx.x] = h[d];
 }
 
-template <bool kHasVals>
+template <bool kHasVals, int RADIX>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t* __restrict__ keys_in,
                                                                 const uint32_t* __restrict__ vals_in, uint64_t m,
                                                                 const uint64_t* __restrict__ m_dev,
@@ -180,10 +184,10 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t* 
                                                                 const uint32_t* __restrict__ offsets,
                                                                 uint64_t* __restrict__ keys_out,
                                                                 uint32_t* __restrict__ vals_out) {
-    __shared__ uint32_t wcnt[4][kRadix];
+    __shared__ uint32_t wcnt[4][RADIX];
     if (m_dev) m = min(*m_dev, m);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int k = threadIdx.x; k < 4 * kRadix; k += kSortThreads) (&wcnt[0][0])[k] = 0u;
+    for (int k = threadIdx.x; k < 4 * RADIX; k += kSortThreads) (&wcnt[0][0])[k] = 0u;
     __syncthreads();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint64_t base = (uint64_t)blockIdx.x * kSortBlock + (uint64_t)wid * (kSortBlock / 4);
@@ -210,11 +214,9 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t* 
         rank[r] = cnt + before;
     }
     __syncthreads();
-    {
-        const int d = threadIdx.x;  // one digit per thread (kSortThreads == kRadix)
+    for (int d = threadIdx.x; d < RADIX; d += kSortThreads) {
         uint32_t g = offsets[(uint64_t)d * nblk + blockIdx.x];
         uint32_t c0 = wcnt[0][d], c1 = wcnt[1][d], c2 = wcnt[2][d];
-        __syncthreads();
         wcnt[0][d] = g;
         wcnt[1][d] = g + c0;
         wcnt[2][d] = g + c0 + c1;
@@ -233,10 +235,12 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t* 
     }
 }
 
+constexpr int kRadixWide = 2048;  // 11-bit digits (radix_sort_u64 with wide = true)
+
 size_t radix_temp_bytes(uint64_t m) {
     uint64_t nblk = (m + kSortBlock - 1) / kSortBlock;
     if (nblk == 0) nblk = 1;
-    uint64_t hist = (uint64_t)kRadix * nblk;
+    uint64_t hist = (uint64_t)kRadixWide * nblk;
     return sizeof(uint32_t) * 2 * (hist + 1) + scan_temp_bytes(hist) + 256;
 }
 
@@ -245,15 +249,17 @@ size_t radix_temp_bytes(uint64_t m) {
 // min(*m_dev, m) read on the device (m is then the capacity): no host sync,
 // and the launches are the same for every count (graph capture).
 bool radix_sort_u64(hipStream_t st, uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt,
-                    uint64_t m, int bit_lo, int bit_hi, void* temp, const uint64_t* m_dev) {
+                    uint64_t m, int bit_lo, int bit_hi, void* temp, const uint64_t* m_dev, bool wide) {
     if (m == 0 || bit_hi <= bit_lo) return false;
     uint32_t nblk = (uint32_t)((m + kSortBlock - 1) / kSortBlock);
-    uint64_t hist_n = (uint64_t)kRadix * nblk;
+    const int max_bits = wide ? 11 : 8;
+    const int radix = wide ? kRadixWide : kRadix;
+    uint64_t hist_n = (uint64_t)radix * nblk;
     uint32_t* hist = (uint32_t*)temp;
     uint32_t* offs = hist + hist_n + 1;
     void* scan_tmp = (void*)(((uintptr_t)(offs + hist_n + 1) + 255) & ~(uintptr_t)255);
     int total_bits = bit_hi - bit_lo;
-    int passes = (total_bits + 7) / 8;
+    int passes = (total_bits + max_bits - 1) / max_bits;
     int per = (total_bits + passes - 1) / passes;
     bool alt = false;
     uint64_t* kin = keys;
@@ -264,17 +270,25 @@ bool radix_sort_u64(hipStream_t st, uint64_t* keys, uint64_t* keys_alt, uint32_t
         int shift = bit_lo + p * per;
         int bits = (shift + per <= bit_hi) ? per : (bit_hi - shift);
         uint32_t mask = (1u << bits) - 1u;
-        hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(kSortThreads), 0, st, (const uint64_t*)kin, m, m_dev, shift,
-                           mask, nblk, hist);
-        exclusive_scan_u32(st, hist, offs, hist_n, scan_tmp);
-        if (vals)
-            hipLaunchKernelGGL(k_radix_scatter<true>, dim3(nblk), dim3(kSortThreads), 0, st, (const uint64_t*)kin,
-                               (const uint32_t*)vin, m, m_dev, shift, mask, bits, nblk, (const uint32_t*)offs, kout,
-                               vout);
+        const uint64_t hn = (uint64_t)radix * nblk;
+        if (wide)
+            hipLaunchKernelGGL(k_radix_hist<kRadixWide>, dim3(nblk), dim3(kSortThreads), 0, st, (const uint64_t*)kin,
+                               m, m_dev, shift, mask, nblk, hist);
         else
-            hipLaunchKernelGGL(k_radix_scatter<false>, dim3(nblk), dim3(kSortThreads), 0, st, (const uint64_t*)kin,
-                               (const uint32_t*)nullptr, m, m_dev, shift, mask, bits, nblk, (const uint32_t*)offs,
-                               kout, (uint32_t*)nullptr);
+            hipLaunchKernelGGL(k_radix_hist<kRadix>, dim3(nblk), dim3(kSortThreads), 0, st, (const uint64_t*)kin, m,
+                               m_dev, shift, mask, nblk, hist);
+        exclusive_scan_u32(st, hist, offs, hn, scan_tmp);
+#define NPS_SCATTER(V, RX)                                                                                          \
+    hipLaunchKernelGGL((k_radix_scatter<V, RX>), dim3(nblk), dim3(kSortThreads), 0, st, (const uint64_t*)kin,         \
+                       (const uint32_t*)vin, m, m_dev, shift, mask, bits, nblk, (const uint32_t*)offs, kout, vout)
+        if (vals) {
+            if (wide) NPS_SCATTER(true, kRadixWide);
+            else NPS_SCATTER(true, kRadix);
+        } else {
+            if (wide) NPS_SCATTER(false, kRadixWide);
+            else NPS_SCATTER(false, kRadix);
+        }
+#undef NPS_SCATTER
         uint64_t* tk = kin; kin = kout; kout = tk;
         uint32_t* tv = vin; vin = vout; vout = tv;
         alt = !alt;

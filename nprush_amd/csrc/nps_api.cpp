@@ -34,6 +34,9 @@ struct ExtView {
     DevBuf rec, counts, rects, dkey, dkey_alt, rank_counts, offsets, scan_tmp, keys, keys_alt, sort_tmp, ranges;
     DevBuf order, tile_work, ckpt, act, g2, c_img, tile_loss, stats, flags;  // flags: [0] key overflow
     DevBuf part;  // per-tile counter slots of the composite
+    DevBuf bin_cnt, bin_off, bin_tile, bin_scan;  // binning: digit counts, their offsets, per-tile counts
+    uint32_t* ids = nullptr;  // splat ids in (tile, depth, id) order: keys or keys_alt read as 32-bit words
+    bool col_major = false;   // ids stored tile column by tile column (ext_bin)
     uint64_t m = 0;    // key count of the last eager run
     uint64_t cap = 0;  // key capacity of a captured run
     bool in_alt = false;
@@ -140,7 +143,8 @@ int ensure(nps_ctx* h, DevBuf& b, size_t bytes) { return ensure_dev(b, bytes, h-
 void free_ext_view(ExtView& x) {
     DevBuf* all[] = {&x.rec, &x.counts, &x.rects, &x.dkey, &x.dkey_alt, &x.rank_counts, &x.offsets, &x.scan_tmp,
                      &x.keys, &x.keys_alt, &x.sort_tmp, &x.ranges, &x.order, &x.tile_work, &x.ckpt, &x.act,
-                     &x.g2, &x.c_img, &x.tile_loss, &x.stats, &x.flags, &x.part};
+                     &x.g2, &x.c_img, &x.tile_loss, &x.stats, &x.flags, &x.part, &x.bin_cnt, &x.bin_off,
+                     &x.bin_tile, &x.bin_scan};
     for (DevBuf* b : all) free_buf(*b);
     if (x.ev_count) {
         (void)hipEventDestroy(x.ev_count);
@@ -188,9 +192,12 @@ int ext_view_reserve(ExtView& x, uint32_t n, uint32_t W, uint32_t H, uint64_t ca
         {&x.rec, sizeof(ExtRec) * n1}, {&x.counts, 4 * n1}, {&x.rects, 16 * n1}, {&x.dkey, 8 * n1},
         {&x.dkey_alt, 8 * n1}, {&x.rank_counts, 4 * n1}, {&x.offsets, 8 * (n1 + 1)},
         {&x.scan_tmp, scan_temp_bytes(n1) + 64}, {&x.stats, 64}, {&x.flags, 64},
-        {&x.sort_tmp, radix_temp_bytes(std::max<uint64_t>(c1, n1))}, {&x.keys, 8 * c1}, {&x.keys_alt, 8 * c1},
+        {&x.sort_tmp, radix_temp_bytes(std::max<uint64_t>(c1, n1))},
+        {&x.keys, 8 * c1}, {&x.keys_alt, 8 * c1},
         {&x.ranges, 8 * (size_t)nt}, {&x.order, 4 * (size_t)ext_order_slots(tx, ty)}, {&x.tile_work, 8 * (size_t)nt},
         {&x.c_img, 12 * (size_t)W * H}, {&x.tile_loss, 16 * (size_t)nt}, {&x.part, 32 * (size_t)nt},
+        {&x.bin_cnt, 4 * ext_bin_scan_words(n, tx, ty, c1)}, {&x.bin_off, 4 * ext_bin_scan_words(n, tx, ty, c1)},
+        {&x.bin_tile, 4 * (size_t)nt}, {&x.bin_scan, scan_temp_bytes(ext_bin_scan_words(n, tx, ty, c1)) + 64},
     };
     int rc;
     for (auto& q : need)
@@ -235,7 +242,7 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
                           (uint64_t*)x.dkey.p);
     // rank of every splat in (depth, id) order: stable sort of depth << 32 | id on the depth bits
     const bool dalt = radix_sort_u64(st, (uint64_t*)x.dkey.p, (uint64_t*)x.dkey_alt.p, nullptr, nullptr, n, 32, 64,
-                                     x.sort_tmp.p);
+                                     x.sort_tmp.p, nullptr, true);
     const uint64_t* order = (const uint64_t*)(dalt ? x.dkey_alt.p : x.dkey.p);
     launch_ext_rank_counts(st, n, order, (const uint32_t*)counts, (uint32_t*)x.rank_counts.p);
     exclusive_scan_u32_to_u64(st, (const uint32_t*)x.rank_counts.p, (uint64_t*)x.offsets.p, n, x.scan_tmp.p);
@@ -252,7 +259,6 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
                                      hipMemcpyDeviceToHost, st), "count readback")) != NPS_OK)
             return rc;
         if ((rc = hip(hipEventRecord(x.ev_count, st), "event")) != NPS_OK) return rc;
-        launch_fill_u32(st, (uint32_t*)x.ranges.p, 0u, 2 * (uint64_t)v.num_tiles);
         if (grads) launch_fill_u32(st, (uint32_t*)g2, 0u, 18 * (uint64_t)n);
         if ((rc = hip(hipEventSynchronize(x.ev_count), "sync")) != NPS_OK) return rc;
         const uint64_t m = *x.count_host;
@@ -268,24 +274,38 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
         m_dev = (const uint64_t*)x.offsets.p + n;
     }
     x.num_tiles = v.num_tiles;
-    launch_ext_keys(st, n, order, (const uint32_t*)x.rank_counts.p, (const uint4*)x.rects.p, v.tiles_x,
-                    (const uint64_t*)x.offsets.p, (uint64_t*)x.keys.p, cap, flags);
-    if (ev && (rc = hip(hipEventRecord(ev[1], st), "event")) != NPS_OK) return rc;
-    const int tile_bits = bits_for(v.num_tiles > 0 ? v.num_tiles - 1 : 0);
-    x.in_alt = radix_sort_u64(st, (uint64_t*)x.keys.p, (uint64_t*)x.keys_alt.p, nullptr, nullptr, cap, 32,
-                              32 + tile_bits, x.sort_tmp.p, m_dev);
-    if (ev && (rc = hip(hipEventRecord(ev[2], st), "event")) != NPS_OK) return rc;
-    const uint64_t* sorted = (const uint64_t*)(x.in_alt ? x.keys_alt.p : x.keys.p);
-    if (capture) {
+    x.col_major = ext_bin_fits(n, v.tiles_x, v.tiles_y, cap);
+    if (x.col_major) {
+        // keys bucketed by tile row as they are enumerated, then one stable
+        // pass on the tile column: ids and ranges
+        x.ids = (uint32_t*)x.keys_alt.p;
+        if (ev && (rc = hip(hipEventRecord(ev[1], st), "event")) != NPS_OK) return rc;
+        ext_bin(st, n, order, (const uint32_t*)x.rank_counts.p, (const uint4*)x.rects.p, v.tiles_x, v.tiles_y, cap,
+                m_dev, (uint32_t*)x.keys.p, x.ids, (uint32_t*)x.ranges.p, (uint32_t*)x.bin_cnt.p,
+                (uint32_t*)x.bin_off.p, (uint32_t*)x.bin_tile.p, x.bin_scan.p, flags);
+        if (ev && (rc = hip(hipEventRecord(ev[2], st), "event")) != NPS_OK) return rc;
+    } else {
+        // 8-byte keys tile << 32 | id in rank order, stable sort on the tile bits
+        launch_ext_keys(st, n, order, (const uint32_t*)x.rank_counts.p, (const uint4*)x.rects.p, v.tiles_x,
+                        (const uint64_t*)x.offsets.p, (uint64_t*)x.keys.p, cap, flags);
+        if (ev && (rc = hip(hipEventRecord(ev[1], st), "event")) != NPS_OK) return rc;
+        const int tile_bits = bits_for(v.num_tiles > 0 ? v.num_tiles - 1 : 0);
+        x.in_alt = radix_sort_u64(st, (uint64_t*)x.keys.p, (uint64_t*)x.keys_alt.p, nullptr, nullptr, cap, 32,
+                                  32 + tile_bits, x.sort_tmp.p, m_dev);
+        if (ev && (rc = hip(hipEventRecord(ev[2], st), "event")) != NPS_OK) return rc;
+        const uint64_t* sorted = (const uint64_t*)(x.in_alt ? x.keys_alt.p : x.keys.p);
+        x.ids = (uint32_t*)(x.in_alt ? x.keys.p : x.keys_alt.p);
         launch_fill_u32(st, (uint32_t*)x.ranges.p, 0u, 2 * (uint64_t)v.num_tiles);
-        if (grads) launch_fill_u32(st, (uint32_t*)g2, 0u, 18 * (uint64_t)n);
+        launch_tile_ranges(st, sorted, cap, (uint32_t*)x.ranges.p, m_dev);
+        launch_keys_to_ids(st, sorted, cap, m_dev, x.ids);
     }
-    launch_tile_ranges(st, sorted, cap, (uint32_t*)x.ranges.p, m_dev);
+    if (capture && grads) launch_fill_u32(st, (uint32_t*)g2, 0u, 18 * (uint64_t)n);
     launch_tile_order(st, (const uint32_t*)x.ranges.p, v.tiles_x, v.tiles_y, (uint32_t*)x.order.p);
     if (ev && (rc = hip(hipEventRecord(ev[3], st), "event")) != NPS_OK) return rc;
     ExtCompositeArgs a;
     a.rec = (const ExtRec*)x.rec.p;
-    a.keys = sorted;
+    a.ids = x.ids;
+    a.col_major = x.col_major ? 1 : 0;
     a.ranges = (const uint32_t*)x.ranges.p;
     a.tgt_rgba = tgt;
     a.g2 = grads ? g2 : nullptr;
@@ -1015,12 +1035,7 @@ int nps_read_tiles(nps_handle h, uint32_t* ranges, uint32_t* ids, uint32_t* full
         if (ranges) memcpy(ranges, r.data(), 8 * (size_t)h->last_tiles);
         if (full_counts)
             for (size_t t = 0; t < h->last_tiles; ++t) full_counts[t] = r[2 * t + 1] - r[2 * t];
-        if (ids && h->xv.m) {
-            std::vector<uint64_t> k(h->xv.m);
-            NPS_COPY(h, k.data(), h->xv.in_alt ? h->xv.keys_alt.p : h->xv.keys.p, 8 * (size_t)h->xv.m,
-                     hipMemcpyDeviceToHost);
-            for (size_t j = 0; j < k.size(); ++j) ids[j] = (uint32_t)k[j];
-        }
+        if (ids && h->xv.m) NPS_COPY(h, ids, h->xv.ids, 4 * (size_t)h->xv.m, hipMemcpyDeviceToHost);
         return NPS_OK;
     } else {
         if (ranges) NPS_COPY(h, ranges, h->sc.ranges.p, 8 * (size_t)h->last_tiles, hipMemcpyDeviceToHost);

@@ -60,8 +60,11 @@ size_t scan_temp_bytes(uint64_t n);
 void exclusive_scan_u32_to_u64(hipStream_t st, const uint32_t* in, uint64_t* out, uint64_t n, void* temp);
 void exclusive_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t n, void* temp);
 size_t radix_temp_bytes(uint64_t m);
+// wide: digits of up to 11 bits (2048 buckets, fewer passes) instead of 8.
 bool radix_sort_u64(hipStream_t st, uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt,
-                    uint64_t m, int bit_lo, int bit_hi, void* temp, const uint64_t* m_dev = nullptr);
+                    uint64_t m, int bit_lo, int bit_hi, void* temp, const uint64_t* m_dev = nullptr,
+                    bool wide = false);
+
 
 // k_composite.hip
 void launch_composite_ref(hipStream_t st, const ViewParams& v, const CompositeArgs& a);
@@ -106,7 +109,7 @@ void launch_render(hipStream_t st, const ViewParams& v, const RenderArgs& a);
 // k_ext.hip
 struct ExtCompositeArgs {
     const ExtRec* rec;
-    const uint64_t* keys;       // tile << 32 | splat id, in (tile, depth, id) order
+    const uint32_t* ids;        // splat ids in (tile, depth, id) order
     const uint32_t* ranges;     // [2 * num_tiles]
     const float4* tgt_rgba;     // H*W
     double* g2;                 // 9 per splat: d scr.xy, d conic A,2B,C, d rgb, d opacity
@@ -120,7 +123,21 @@ struct ExtCompositeArgs {
     uint32_t* tile_work;        // 2 per tile: entries staged by the forward, reverse extent
     unsigned long long* part;   // 4 per tile (PartMap slots): the counters stats[0], [4], [2], [3]
     int backward;               // 0: image and loss only
+    int col_major;              // lists stored column by column (ext_bin), else tile by tile
 };
+// EXT binning without a key sort (k_bin_* in k_ext.hip): from the depth order
+// (order, rank_counts, rects) to splat ids grouped by tile, each tile in
+// (depth, id) order (tiles column by column), and the tile ranges.
+// ext_bin_fits: at most 256 tile rows and columns, the 4-byte items hold the
+// column and the id, and the counts fit 32 bits (else the caller sorts keys).
+// cnt / off: ext_bin_scan_words(...) words each; scan_tmp: scan_temp_bytes of that.
+bool ext_bin_fits(uint32_t n, uint32_t tiles_x, uint32_t tiles_y, uint64_t cap);
+size_t ext_bin_scan_words(uint32_t n, uint32_t tiles_x, uint32_t tiles_y, uint64_t cap);
+void ext_bin(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts, const uint4* rects,
+             uint32_t tiles_x, uint32_t tiles_y, uint64_t cap, const uint64_t* m_dev, uint32_t* items, uint32_t* ids,
+             uint32_t* ranges, uint32_t* cnt, uint32_t* off, uint32_t* tile_cnt, void* scan_tmp,
+             uint32_t* overflow);
+void launch_keys_to_ids(hipStream_t st, const uint64_t* keys, uint64_t m, const uint64_t* m_dev, uint32_t* ids);
 // Entries of the composite's workgroup order (= its grid size).
 uint32_t ext_order_slots(uint32_t tiles_x, uint32_t tiles_y);
 void launch_tile_order(hipStream_t st, const uint32_t* ranges, uint32_t tiles_x, uint32_t tiles_y, uint32_t* order);
