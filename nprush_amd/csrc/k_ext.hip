@@ -320,7 +320,8 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_row_emit(uint32_t n, const 
         const uint32_t id = (uint32_t)__shfl((int)my_id, (int)sl, 64);
         const uint32_t ty = valid ? ty_all : 0u;
         uint64_t peers = __ballot(valid);
-        for (int q = 0; q < row_bits; ++q) {
+#pragma unroll
+        for (int q = 0; q < kBinMaxBits; ++q) {  // bits above row_bits are 0 in every lane: no-ops
             const uint64_t bal = __ballot((ty >> q) & 1u);
             peers &= ((ty >> q) & 1u) ? bal : ~bal;
         }
@@ -348,7 +349,7 @@ __global__ __launch_bounds__(256) void k_bin_col_count(const uint32_t* __restric
                                                        uint32_t ny, uint32_t nx, int row_bits, int col_bits,
                                                        int id_bits, uint32_t* __restrict__ counts2,
                                                        uint32_t* __restrict__ tile_cnt) {
-    constexpr int kLocal = 8192;  // per-workgroup tile counters (rows in the workgroup x nx)
+    constexpr int kLocal = 2048;  // per-workgroup tile counters (rows in the workgroup x nx)
     __shared__ uint32_t s_start[(1 << kBinMaxBits) + 1];
     __shared__ uint32_t s_c[256];
     __shared__ uint32_t s_t[kLocal];
@@ -394,7 +395,8 @@ __global__ __launch_bounds__(256) void k_bin_col_count(const uint32_t* __restric
                 const uint64_t bal = __ballot((y >> q) & 1u);
                 peers &= ((y >> q) & 1u) ? bal : ~bal;
             }
-        for (int q = 0; q < col_bits; ++q) {
+#pragma unroll
+        for (int q = 0; q < kBinMaxBits; ++q) {  // bits above col_bits are 0 in every lane: no-ops
             const uint64_t bal = __ballot((x >> q) & 1u);
             peers &= ((x >> q) & 1u) ? bal : ~bal;
         }
@@ -451,18 +453,23 @@ __global__ __launch_bounds__(1024) void k_bin_ranges(const uint32_t* __restrict_
 
 // Stable scatter of the items on their column (offs2: exclusive scan of
 // counts2), writing the splat ids: 4 waves x 1024 items, ranked per wave as in
-// k_radix_scatter.
+// k_radix_scatter, staged in LDS in output order and written as runs of
+// consecutive ids per column.
 __global__ __launch_bounds__(256) void k_bin_col_scatter(const uint32_t* __restrict__ items, uint64_t m,
                                                          const uint64_t* __restrict__ m_dev, uint32_t nx, int col_bits,
                                                          int id_bits, const uint32_t* __restrict__ offs2,
                                                          uint32_t* __restrict__ ids) {
     __shared__ uint32_t wcnt[4][256];
+    __shared__ uint32_t s_ids[kBinColBlock];
+    __shared__ uint32_t s_gbase[256];
+    __shared__ uint32_t s_w4[4];
     if (m_dev) m = min(*m_dev, m);
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
     for (uint32_t k = tid; k < 4 * 256; k += 256) (&wcnt[0][0])[k] = 0u;
     __syncthreads();
     const uint64_t base = (uint64_t)blockIdx.x * kBinColBlock;
     if (base >= m) return;
+    const uint32_t nloc = (uint32_t)min<uint64_t>((uint64_t)kBinColBlock, m - base);
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint64_t wbase = base + (uint64_t)wid * (kBinColBlock / 4);
     const uint32_t idmask = id_bits >= 32 ? 0xFFFFFFFFu : ((1u << id_bits) - 1u);
@@ -478,7 +485,8 @@ __global__ __launch_bounds__(256) void k_bin_col_scatter(const uint32_t* __restr
         const bool valid = pos < m;
         const uint32_t d = it[r] >> id_bits;
         uint64_t peers = __ballot(valid);
-        for (int q = 0; q < col_bits; ++q) {
+#pragma unroll
+        for (int q = 0; q < kBinMaxBits; ++q) {  // bits above col_bits are 0 in every lane: no-ops
             const uint64_t bal = __ballot((d >> q) & 1u);
             peers &= ((d >> q) & 1u) ? bal : ~bal;
         }
@@ -490,20 +498,45 @@ __global__ __launch_bounds__(256) void k_bin_col_scatter(const uint32_t* __restr
         rank[r] = cnt + before;
     }
     __syncthreads();
-    if (tid < nx) {
+    {   // one thread per column: the workgroup's local start of column d, the waves' starts
         const uint32_t d = tid;
-        const uint32_t g = offs2[(size_t)d * gridDim.x + blockIdx.x];
-        const uint32_t c0 = wcnt[0][d], c1 = wcnt[1][d], c2 = wcnt[2][d];
-        wcnt[0][d] = g;
-        wcnt[1][d] = g + c0;
-        wcnt[2][d] = g + c0 + c1;
-        wcnt[3][d] = g + c0 + c1 + c2;
+        const bool col = d < nx;
+        const uint32_t c0 = col ? wcnt[0][d] : 0u, c1 = col ? wcnt[1][d] : 0u, c2 = col ? wcnt[2][d] : 0u,
+                       c3 = col ? wcnt[3][d] : 0u;
+        const uint32_t c = c0 + c1 + c2 + c3;
+        uint32_t x = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if ((int)lane >= o) x += y;
+        }
+        if (lane == 63u) s_w4[wid] = x;
+        __syncthreads();
+        uint32_t wb = 0u;
+        for (uint32_t w = 0; w < wid; ++w) wb += s_w4[w];
+        const uint32_t lstart = wb + x - c;
+        if (col) {
+            s_gbase[d] = offs2[(size_t)d * gridDim.x + blockIdx.x] - lstart;  // modulo 2^32
+            wcnt[0][d] = lstart;
+            wcnt[1][d] = lstart + c0;
+            wcnt[2][d] = lstart + c0 + c1;
+            wcnt[3][d] = lstart + c0 + c1 + c2;
+        }
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kBinColItems; ++r) {
         const uint64_t pos = wbase + (uint64_t)r * 64 + lane;
-        if (pos < m) ids[wcnt[wid][it[r] >> id_bits] + rank[r]] = it[r] & idmask;
+        if (pos < m) s_ids[wcnt[wid][it[r] >> id_bits] + rank[r]] = it[r];
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < kBinColItems; ++k) {
+        const uint32_t p = (uint32_t)k * 256u + tid;
+        if (p < nloc) {
+            const uint32_t v = s_ids[p];
+            ids[s_gbase[v >> id_bits] + p] = v & idmask;
+        }
     }
 }
 

@@ -202,7 +202,9 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t* 
         if (kHasVals) val[r] = valid ? vals_in[pos] : 0u;
         uint32_t d = (uint32_t)(key[r] >> shift) & mask;
         uint64_t peers = __ballot(valid);
-        for (int b = 0; b < bits; ++b) {
+        constexpr int kBits = RADIX == 2048 ? 11 : 8;
+#pragma unroll
+        for (int b = 0; b < kBits; ++b) {  // bits at and above `bits` are 0 in every lane: no-ops
             uint64_t bal = __ballot((d >> b) & 1u);
             peers &= ((d >> b) & 1u) ? bal : ~bal;
         }
