@@ -224,7 +224,9 @@ int nps_read_timing(nps_handle h, float ms[8]);
  * new GpuSplatForwardPipelineManager({device, numSplats, splatBuffer})
  * (GpuSplatForwardPipelineManager.ts:17-92): borrows the optimizer handle's
  * splat buffer.  Instanced-quad render of splat_forward.wgsl, reverse index
- * order, "over" blending into colour and encoded-depth targets.
+ * order, "over" blending into colour and encoded-depth targets.  After
+ * nps_destroy(splats_owner) every npf_* call returns NPS_ERR_STATE;
+ * npf_destroy still frees the handle.
  */
 typedef struct npf_ctx* npf_handle;
 int npf_create(nps_handle splats_owner, npf_handle* out);
@@ -257,6 +259,8 @@ int npf_write_target(npf_handle f, const float* color, const float* depth);
  *           mode 0 and the next one re-capture.
  * Both modes give bit-identical results.
  */
+/* After nps_destroy of any of its scenes a step refuses every call
+ * (NPS_ERR_STATE) but npt_destroy. */
 typedef struct npt_ctx* npt_handle;
 int npt_create(nps_handle* scenes, int32_t n_scenes, int32_t lanes, npt_handle* out);
 void npt_destroy(npt_handle t);
@@ -395,7 +399,9 @@ int nbz_dev_upload(nbz_handle h, void* dst, const void* src, int64_t bytes);
  * of bezier_forward.wgsl, drawn from the optimizer handle's curve buffer (the
  * reference's manager borrows bezierBuffer).  brush_rgba8: bh rows of bw rgba8
  * texels (the brush texture; only its red channel is sampled), copied at
- * creation.  Errors are reported through nbz_last_error(owner).
+ * creation.  Errors are reported through nbz_last_error(owner).  After
+ * nbz_destroy(curves_owner) every nbf_* call returns NPS_ERR_STATE; nbf_destroy
+ * still frees the handle.  A target of a new size starts cleared to 0.
  */
 typedef struct nbf_ctx* nbf_handle;
 int nbf_create(nbz_handle curves_owner, const uint8_t* brush_rgba8, int32_t bw, int32_t bh, nbf_handle* out);

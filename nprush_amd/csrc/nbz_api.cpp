@@ -45,6 +45,7 @@ struct nbz_ctx {
     uint32_t pl_w = 0, pl_h = 0;
     DevBuf fwd, bwd, ckpt, c_img, d_img, t_img, tile_loss, stats, pa, pd, pk, act, nzm, seeds, part;
     Scratch sc;
+    std::vector<struct nbf_ctx*> draws;  // display-raster handles drawing from this one's curves
     uint32_t last_tiles = 0;
     uint32_t img_w = 0, img_h = 0;
     bool have_image = false;
@@ -240,6 +241,9 @@ int bz_set_target(nbz_ctx* h, const float* t, const float* td, const float* b, c
 
 }  // namespace
 
+struct nbf_ctx;
+static void nbf_release(nbf_ctx* f);
+
 extern "C" {
 
 int nbz_create(int32_t device, int32_t num_beziers, nbz_handle* out) {
@@ -281,6 +285,10 @@ void nbz_destroy(nbz_handle h) {
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    // display-raster handles of these curves: their device memory goes now, and
+    // every later call on them returns NPS_ERR_STATE (nbf_destroy still frees them)
+    for (nbf_ctx* f : h->draws) nbf_release(f);
+    h->draws.clear();
     void* ps[] = {h->curves, h->grads, h->m, h->v, h->grad_accum, h->loss_acc};
     for (void* p : ps)
         if (p) (void)hipFree(p);
@@ -581,6 +589,19 @@ struct nbf_ctx {
     Scratch sc;
 };
 
+// Device memory of a display-raster handle; the handle is then detached from
+// its (possibly already destroyed) owner.
+static void nbf_release(nbf_ctx* f) {
+    if (!f->owner) return;
+    (void)hipSetDevice(f->owner->device);
+    (void)hipStreamSynchronize(f->owner->stream);
+    free_buf(f->rgba);
+    free_buf(f->rec);
+    free_buf(f->brush);
+    free_scratch(f->sc);
+    f->owner = nullptr;
+}
+
 extern "C" {
 
 int nbf_create(nbz_handle owner, const uint8_t* brush_rgba8, int32_t bw, int32_t bh, nbf_handle* out) {
@@ -601,23 +622,24 @@ int nbf_create(nbz_handle owner, const uint8_t* brush_rgba8, int32_t bw, int32_t
     }
     f->bw = bw;
     f->bh = bh;
+    owner->draws.push_back(f);
     *out = f;
     return NPS_OK;
 }
 
 void nbf_destroy(nbf_handle f) {
     if (!f) return;
-    (void)hipSetDevice(f->owner->device);
-    (void)hipStreamSynchronize(f->owner->stream);
-    free_buf(f->rgba);
-    free_buf(f->rec);
-    free_buf(f->brush);
-    free_scratch(f->sc);
+    if (f->owner) {
+        auto& d = f->owner->draws;
+        d.erase(std::remove(d.begin(), d.end(), f), d.end());
+        nbf_release(f);
+    }
     delete f;
 }
 
 int nbf_write_vp(nbf_handle f, const float vp[16]) {
     if (!f || !vp) return NPS_ERR_ARG;
+    if (!f->owner) return NPS_ERR_STATE;  // its curves' handle is destroyed
     memcpy(f->vp, vp, sizeof(f->vp));
     f->has_vp = true;
     return NPS_OK;
@@ -626,10 +648,13 @@ int nbf_write_vp(nbf_handle f, const float vp[16]) {
 int nbf_set_target(nbf_handle f, int32_t w, int32_t hh) {
     if (!f) return NPS_ERR_ARG;
     nbz_ctx* h = f->owner;
+    if (!h) return NPS_ERR_STATE;
     if (w <= 0 || hh <= 0) return bz_err(h, NPS_ERR_ARG, "bad render target size");
     NBZ_HIP(h, hipSetDevice(h->device));
     int rc = bz_ensure(h, f->rgba, 16 * (size_t)w * hh);
     if (rc != NPS_OK) return rc;
+    // a new size starts cleared to 0 (a dispatch with clear = 0 blends over it)
+    if ((uint32_t)w != f->W || (uint32_t)hh != f->H) launch_fill_u32(h->stream, (uint32_t*)f->rgba.p, 0u, 4 * (uint64_t)w * hh);
     f->W = (uint32_t)w;
     f->H = (uint32_t)hh;
     return NPS_OK;
@@ -638,6 +663,7 @@ int nbf_set_target(nbf_handle f, int32_t w, int32_t hh) {
 int nbf_dispatch(nbf_handle f, int32_t clear, int32_t quantize) {
     if (!f) return NPS_ERR_ARG;
     nbz_ctx* h = f->owner;
+    if (!h) return NPS_ERR_STATE;
     if (!f->has_vp || f->W == 0) return bz_err(h, NPS_ERR_STATE, "render target or VP not set");
     NBZ_HIP(h, hipSetDevice(h->device));
     const uint32_t n = h->n;
@@ -669,11 +695,13 @@ int nbf_dispatch(nbf_handle f, int32_t clear, int32_t quantize) {
 
 int nbf_read_target(nbf_handle f, float* rgba) {
     if (!f || !rgba) return NPS_ERR_ARG;
+    if (!f->owner) return NPS_ERR_STATE;
     return bz_copy(f->owner, rgba, f->rgba.p, 16 * (size_t)f->W * f->H, hipMemcpyDeviceToHost);
 }
 
 int nbf_write_target(nbf_handle f, const float* rgba) {
     if (!f || !rgba) return NPS_ERR_ARG;
+    if (!f->owner) return NPS_ERR_STATE;
     return bz_copy(f->owner, f->rgba.p, rgba, 16 * (size_t)f->W * f->H, hipMemcpyHostToDevice);
 }
 

@@ -67,6 +67,8 @@ struct SceneRef {
 }  // namespace
 
 struct nps_ctx {
+    std::vector<struct npf_ctx*> renders;  // npf handles drawing this handle's splats
+    std::vector<struct npt_ctx*> steps;    // multi-view steps over this scene
     nps_config cfg;
     int device = 0;
     bool ext = false;  // NPS_MODE_EXT
@@ -403,6 +405,9 @@ int run_ext(nps_ctx* h, bool grads) {
 
 }  // namespace
 
+static void npf_release(npf_ctx* f);
+static void npt_detach(npt_ctx* t);
+
 extern "C" {
 
 int nps_config_default(nps_config* cfg) {
@@ -493,6 +498,12 @@ void nps_destroy(nps_handle h) {
     if (!h) return;
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
+    // handles that use this one: their device memory goes now, and every later
+    // call on them returns NPS_ERR_STATE (their destroy calls still free them)
+    for (npf_ctx* f : h->renders) npf_release(f);
+    h->renders.clear();
+    for (npt_ctx* t : std::vector<npt_ctx*>(h->steps)) npt_detach(t);
+    h->steps.clear();
     (void)hipFree(h->splats); (void)hipFree(h->m); (void)hipFree(h->v); (void)hipFree(h->accum);
     if (h->grads) (void)hipFree(h->grads);
     if (h->sh) (void)hipFree(h->sh);
@@ -1093,21 +1104,37 @@ int npf_create(nps_handle owner, npf_handle* out) {
     if (owner->ext) return set_err(owner, NPS_ERR_STATE, "npf_* renders REF splats; EXT renders with nps_forward");
     npf_ctx* f = new npf_ctx();
     f->owner = owner;
+    owner->renders.push_back(f);
     *out = f;
     return NPS_OK;
 }
 
-void npf_destroy(npf_handle f) {
-    if (!f) return;
+}  // extern "C"
+
+static void npf_release(npf_ctx* f) {
+    if (!f->owner) return;
     (void)hipSetDevice(f->owner->device);
     (void)hipStreamSynchronize(f->owner->stream);
     free_buf(f->color); free_buf(f->depth); free_buf(f->rec);
     free_scratch(f->sc);
+    f->owner = nullptr;
+}
+
+extern "C" {
+
+void npf_destroy(npf_handle f) {
+    if (!f) return;
+    if (f->owner) {
+        auto& r = f->owner->renders;
+        r.erase(std::remove(r.begin(), r.end(), f), r.end());
+        npf_release(f);
+    }
     delete f;
 }
 
 int npf_write_vp(npf_handle f, const float vp[16]) {
     if (!f || !vp) return NPS_ERR_ARG;
+    if (!f->owner) return NPS_ERR_STATE;  // its splats' handle is destroyed
     memcpy(f->vp, vp, sizeof(f->vp));
     f->has_vp = true;
     return NPS_OK;
@@ -1115,6 +1142,7 @@ int npf_write_vp(npf_handle f, const float vp[16]) {
 
 int npf_set_target(npf_handle f, int32_t w, int32_t hh) {
     if (!f) return NPS_ERR_ARG;
+    if (!f->owner) return NPS_ERR_STATE;  // its splats' handle is destroyed
     nps_ctx* h = f->owner;
     if (w <= 0 || hh <= 0) return set_err(h, NPS_ERR_ARG, "bad render target size");
     NPS_HIP(h, hipSetDevice(h->device));
@@ -1129,6 +1157,7 @@ int npf_set_target(npf_handle f, int32_t w, int32_t hh) {
 
 int npf_dispatch(npf_handle f, int32_t clear, int32_t quantize) {
     if (!f) return NPS_ERR_ARG;
+    if (!f->owner) return NPS_ERR_STATE;  // its splats' handle is destroyed
     nps_ctx* h = f->owner;
     if (!f->has_vp || f->W == 0) return set_err(h, NPS_ERR_STATE, "render target or VP not set");
     NPS_HIP(h, hipSetDevice(h->device));
@@ -1158,6 +1187,7 @@ int npf_dispatch(npf_handle f, int32_t clear, int32_t quantize) {
 
 int npf_read_target(npf_handle f, float* color, float* depth) {
     if (!f) return NPS_ERR_ARG;
+    if (!f->owner) return NPS_ERR_STATE;  // its splats' handle is destroyed
     nps_ctx* h = f->owner;
     size_t px = (size_t)f->W * f->H;
     if (color) NPS_COPY(h, color, f->color.p, 16 * px, hipMemcpyDeviceToHost);
@@ -1167,6 +1197,7 @@ int npf_read_target(npf_handle f, float* color, float* depth) {
 
 int npf_write_target(npf_handle f, const float* color, const float* depth) {
     if (!f) return NPS_ERR_ARG;
+    if (!f->owner) return NPS_ERR_STATE;  // its splats' handle is destroyed
     nps_ctx* h = f->owner;
     size_t px = (size_t)f->W * f->H;
     if (color) NPS_COPY(h, f->color.p, color, 16 * px, hipMemcpyHostToDevice);
@@ -1208,6 +1239,7 @@ struct npt_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool planned = false;
+    bool detached = false;  // a scene was destroyed: every call but npt_destroy is refused
     double cap_scale = 1.0;
     std::string err;
 };
@@ -1454,12 +1486,19 @@ int npt_create(nps_handle* scenes, int32_t n_scenes, int32_t lanes, npt_handle* 
     if (ensure_dev(t->flags, 64 * (size_t)lanes, t->main, t->err) != NPS_OK) return fail();
     launch_fill_u32(t->main, (uint32_t*)t->flags.p, 0u, 16 * (uint64_t)lanes);
     if (hipStreamSynchronize(t->main) != hipSuccess) return fail();
+    for (nps_handle h : t->scenes)
+        if (std::find(h->steps.begin(), h->steps.end(), t) == h->steps.end()) h->steps.push_back(t);
     *out = t;
     return NPS_OK;
 }
 
 void npt_destroy(npt_handle t) {
     if (!t) return;
+    if (!t->detached)
+        for (nps_handle h : t->scenes) {
+            auto& v = h->steps;
+            v.erase(std::remove(v.begin(), v.end(), t), v.end());
+        }
     (void)hipSetDevice(t->device);
     if (t->main) (void)hipStreamSynchronize(t->main);
     npt_drop_graph(t);
@@ -1484,11 +1523,33 @@ void npt_destroy(npt_handle t) {
     delete t;
 }
 
+}  // extern "C"
+
+// A scene of the step is being destroyed: the step stops using the scenes (its
+// graph may hold their buffers), unregisters from the others and refuses every
+// later call but npt_destroy.
+static void npt_detach(npt_ctx* t) {
+    (void)hipSetDevice(t->device);
+    if (t->main) (void)hipStreamSynchronize(t->main);
+    for (auto& ln : t->lanes)
+        if (ln.st) (void)hipStreamSynchronize(ln.st);
+    npt_drop_graph(t);
+    for (nps_handle h : t->scenes) {
+        auto& v = h->steps;
+        v.erase(std::remove(v.begin(), v.end(), t), v.end());
+    }
+    t->detached = true;
+    t->err = "a scene of this step was destroyed";
+}
+
+extern "C" {
+
 const char* npt_last_error(npt_handle t) { return t ? t->err.c_str() : "invalid handle"; }
 
 int npt_add_view(npt_handle t, int32_t scene, const float vp[16], const float vp_inv[16], const float* rgba_dev,
                  int32_t w, int32_t hgt) {
     if (!t) return NPS_ERR_ARG;
+    if (t->detached) return NPS_ERR_STATE;
     if (scene < 0 || scene >= (int)t->scenes.size() || !vp || !vp_inv || !rgba_dev || w <= 0 || hgt <= 0)
         return npt_fail(t, NPS_ERR_ARG, "bad view");
     npt_view v;
@@ -1524,6 +1585,7 @@ int npt_add_view(npt_handle t, int32_t scene, const float vp[16], const float vp
 
 int npt_step(npt_handle t, int32_t mode) {
     if (!t) return NPS_ERR_ARG;
+    if (t->detached) return NPS_ERR_STATE;
     if (t->views.empty()) return npt_fail(t, NPS_ERR_STATE, "no views");
     NPT_HIP(t, hipSetDevice(t->device));
     // the step's chain kernels add into the scenes' gradients (never store)
@@ -1535,6 +1597,7 @@ int npt_step(npt_handle t, int32_t mode) {
 
 int npt_autotune(npt_handle t) {
     if (!t) return NPS_ERR_ARG;
+    if (t->detached) return NPS_ERR_STATE;
     if (t->views.empty()) return npt_fail(t, NPS_ERR_STATE, "no views");
     NPT_HIP(t, hipSetDevice(t->device));
     std::vector<hipEvent_t> ev(8, nullptr);
@@ -1574,6 +1637,7 @@ int npt_read_tune(npt_handle t, int32_t* tile_batch2v) {
 
 int npt_read_losses(npt_handle t, double* loss2v) {
     if (!t || !loss2v) return NPS_ERR_ARG;
+    if (t->detached) return NPS_ERR_STATE;
     NPT_HIP(t, hipMemcpy(loss2v, t->view_loss.p, 16 * t->views.size(), hipMemcpyDeviceToHost));
     return NPS_OK;
 }
