@@ -163,12 +163,15 @@ def batched_config2(steps=6, warmup=1, lanes=8):
             out["tune"] = {f"{sz[0]}x{sz[1]}": [int(x) for x in tune[[i for i, v in enumerate(views)
                                                                        if (v[1], v[2]) == sz][0]]]
                            for sz in sizes}
+            out["tune_in_context"] = step.graph_info()
         for _ in range(warmup):
             step.step(graph=graph, serial=serial)
         t0 = time.perf_counter()
         for _ in range(steps):
             step.step(graph=graph, serial=serial)
         out[key] = (time.perf_counter() - t0) * 1000.0 / steps
+        if graph:
+            out["nodes_" + key] = step.graph_info()["nodes"]
         losses = step.read_losses()
         out["l1_" + key] = float(losses[:, 1].sum())
         step.close()
@@ -184,6 +187,12 @@ def batched_config2(steps=6, warmup=1, lanes=8):
         "graph_ms_per_step": round(out["graph"], 3),
         "graph_autotuned_ms_per_step": round(out["graph_tuned"], 3),
         "autotuned_tile_batch_by_resolution (scene 0)": out["tune"],
+        # npt_autotune's second stage: whole-step timing graphs (no chain rule, no Adam) for
+        # {scene defaults, per-class choices} x {all lanes, half}; the fastest is kept
+        "autotune_in_context_ms": out["tune_in_context"]["autotune_ms"],
+        "autotuned_lanes": out["tune_in_context"]["lanes"],
+        "graph_nodes": out["nodes_graph"],
+        "graph_nodes_autotuned": out["nodes_graph_tuned"],
         "speedup_graph_autotuned_vs_serial": round(out["per_view_serial"] / out["graph_tuned"], 3),
         "speedup_graph": round(out["per_view"] / out["graph"], 3),
         "speedup_graph_autotuned": round(out["per_view"] / out["graph_tuned"], 3),

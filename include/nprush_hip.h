@@ -274,13 +274,22 @@ int npt_add_view(npt_handle t, int32_t scene, const float vp[16], const float vp
  * scenes overlap on the GPU); mode 2: view by view, each completed before the
  * next is launched (strictly sequential).  Same results in every mode. */
 int npt_step(npt_handle t, int32_t mode);
-/* Autotunes (as nps_autotune) one view of every (scene, resolution) class and
- * uses its choice for the whole class; views start with their scene's setting.
- * tile_batch2v[2 v], [2 v + 1]: the choice of view v. */
+/* Autotunes in two stages: (1) as nps_autotune, one view of every (scene,
+ * resolution) class alone, its choice for the whole class; (2) in context: the
+ * step's views captured as one graph (no chain rule, no Adam, so the scenes are
+ * untouched) for {the scenes' own settings, the per-class choices} x {all
+ * lanes, half of them}, each replayed (best of 3); the fastest is kept.  Views
+ * start with their scene's setting.  tile_batch2v[2 v], [2 v + 1]: the choice
+ * of view v. */
 int npt_autotune(npt_handle t);
 int npt_read_tune(npt_handle t, int32_t* tile_batch2v);
 /* Losses of every view in the last step: loss2v[2 v] = L2, loss2v[2 v + 1] = L1. */
 int npt_read_losses(npt_handle t, double* loss2v);
+/* info[0] = nodes of the last captured graph (a step graph above 8,192 nodes,
+ * the largest tested, is refused with NPS_ERR_CAPACITY: use mode 0), info[1] =
+ * lanes the views are dealt to; tune_ms[4] = npt_autotune's in-context replay
+ * times of {own settings, per-class choices} x {all lanes, half} (-1: not run). */
+int npt_graph_info(npt_handle t, int64_t info[2], float tune_ms[4]);
 
 /*
  * new GpuBlurPipelineManager(device) (src/lib/gpu/GpuBlurPipelineManager.ts:9-49).

@@ -91,6 +91,7 @@ SIGNATURES = {
     "npt_autotune": (i32, [_P]),
     "npt_read_tune": (i32, [_P, _I]),
     "npt_read_losses": (i32, [_P, _D]),
+    "npt_graph_info": (i32, [_P, C.POINTER(C.c_int64), _F]),
     "npb_create": (i32, [i32, C.POINTER(_P)]),
     "npb_destroy": (None, [_P]),
     "npb_last_error": (C.c_char_p, [_P]),

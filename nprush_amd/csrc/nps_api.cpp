@@ -1240,6 +1240,9 @@ struct npt_ctx {
     hipGraphExec_t exec = nullptr;
     bool planned = false;
     bool detached = false;  // a scene was destroyed: every call but npt_destroy is refused
+    int lanes_used = 0;        // lanes the views are dealt to (<= lanes.size(); set by npt_autotune)
+    int64_t graph_nodes = 0;   // nodes of the last captured graph
+    float tune_ms[4] = {-1.0f, -1.0f, -1.0f, -1.0f};  // last npt_autotune's candidates (timing graphs)
     double cap_scale = 1.0;
     std::string err;
 };
@@ -1258,6 +1261,33 @@ int npt_fail(npt_handle t, int code, const std::string& msg) {
     } while (0)
 
 ViewParams npt_view_params(const npt_view& w) { return make_view(w.vp, w.vp_inv, w.W, w.H, (uint32_t)w.tune.tile); }
+
+// Node limit of a captured step graph: the tests capture and replay a step of
+// about 98.5 % of it (about 200 small views), the batched configuration of 4
+// scenes and 64 views (chain rule and Adam included) is below it; a larger
+// plan is refused rather than captured untested.
+constexpr size_t kMaxGraphNodes = 8192;
+
+// Views dealt to the first lanes_used lanes, most pixels first, each to the
+// least-loaded lane (longest-processing-time order); t->order is the step order.
+void npt_assign_lanes(npt_handle t) {
+    const int V = (int)t->views.size();
+    const int L = std::max(1, std::min(t->lanes_used > 0 ? t->lanes_used : (int)t->lanes.size(), (int)t->lanes.size()));
+    t->order.resize(V);
+    for (int i = 0; i < V; ++i) t->order[i] = i;
+    std::stable_sort(t->order.begin(), t->order.end(), [&](int a, int b) {
+        return (uint64_t)t->views[a].W * t->views[a].H > (uint64_t)t->views[b].W * t->views[b].H;
+    });
+    t->lane_of.assign(V, 0);
+    std::vector<uint64_t> load(L, 0);
+    for (int vi : t->order) {
+        int best = 0;
+        for (int l = 1; l < L; ++l)
+            if (load[l] < load[best]) best = l;
+        t->lane_of[vi] = best;
+        load[best] += (uint64_t)t->views[vi].W * t->views[vi].H;
+    }
+}
 
 void npt_drop_graph(npt_handle t) {
     if (t->exec) (void)hipGraphExecDestroy(t->exec);
@@ -1310,7 +1340,7 @@ int npt_step_sequential(npt_handle t, bool serial) {
 // Sizes every lane for its views (key counts measured by a forward-only eager
 // run) and captures the whole step as one graph.
 
-int npt_plan(npt_handle t) {
+int npt_plan(npt_handle t, bool timing) {
     npt_drop_graph(t);
     const int L = (int)t->lanes.size();
     std::vector<uint64_t> mmax(L, 0);
@@ -1381,6 +1411,7 @@ int npt_plan(npt_handle t) {
                 return;
             }
         }
+        if (timing) return;  // timing graph of the autotuner: the views only
         // chain rule: each scene's views in step order on one stream, scenes side by side
         if (hipEventRecord(t->joined, t->main) != hipSuccess) { rc = npt_fail(t, NPS_ERR_HIP, "fork"); return; }
         const size_t S = t->scenes.size(), L = t->lanes.size();
@@ -1421,15 +1452,27 @@ int npt_plan(npt_handle t) {
         return rc;
     }
     if (e != hipSuccess) return npt_fail(t, NPS_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+    size_t nodes = 0;
+    if (hipGraphGetNodes(g, nullptr, &nodes) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return npt_fail(t, NPS_ERR_HIP, "hipGraphGetNodes");
+    }
+    t->graph_nodes = (int64_t)nodes;
+    if (nodes > kMaxGraphNodes) {
+        (void)hipGraphDestroy(g);
+        return npt_fail(t, NPS_ERR_CAPACITY,
+                        "captured step has " + std::to_string(nodes) + " nodes, above the " +
+                            std::to_string(kMaxGraphNodes) + " of the largest graph tested; use npt_step mode 0");
+    }
     t->graph = g;
     NPT_HIP(t, hipGraphInstantiate(&t->exec, t->graph, nullptr, nullptr, 0));
-    t->planned = true;
+    t->planned = !timing;
     return NPS_OK;
 }
 
 int npt_step_graph(npt_handle t) {
     if (!t->planned) {
-        int rc = npt_plan(t);
+        int rc = npt_plan(t, false);
         if (rc != NPS_OK) return rc;
     }
     for (nps_handle h : t->scenes) NPT_HIP(t, hipStreamSynchronize(h->stream));
@@ -1563,22 +1606,8 @@ int npt_add_view(npt_handle t, int32_t scene, const float vp[16], const float vp
     t->views.push_back(v);
     // step order: most pixels first (longest-processing-time order), then index;
     // lanes are filled greedily with the least-loaded lane first
-    const int V = (int)t->views.size();
-    t->order.resize(V);
-    for (int i = 0; i < V; ++i) t->order[i] = i;
-    std::stable_sort(t->order.begin(), t->order.end(), [&](int a, int b) {
-        return (uint64_t)t->views[a].W * t->views[a].H > (uint64_t)t->views[b].W * t->views[b].H;
-    });
-    t->lane_of.assign(V, 0);
-    std::vector<uint64_t> load(t->lanes.size(), 0);
-    for (int vi : t->order) {
-        int best = 0;
-        for (int l = 1; l < (int)t->lanes.size(); ++l)
-            if (load[l] < load[best]) best = l;
-        t->lane_of[vi] = best;
-        load[best] += (uint64_t)t->views[vi].W * t->views[vi].H;
-    }
-    if (ensure_dev(t->view_loss, 16 * (size_t)V, t->main, t->err) != NPS_OK) return NPS_ERR_HIP;
+    npt_assign_lanes(t);
+    if (ensure_dev(t->view_loss, 16 * t->views.size(), t->main, t->err) != NPS_OK) return NPS_ERR_HIP;
     npt_drop_graph(t);
     return NPS_OK;
 }
@@ -1603,6 +1632,13 @@ int npt_autotune(npt_handle t) {
     std::vector<hipEvent_t> ev(8, nullptr);
     for (auto& e : ev) NPT_HIP(t, hipEventCreate(&e));
     int rc = NPS_OK;
+    // 1. per (scene, resolution) class, the fastest (tile, batch) of one view alone
+    const std::vector<ExtTune> untuned = [&] {
+        std::vector<ExtTune> u;
+        for (const auto& w : t->views) u.push_back(t->scenes[w.scene]->tune);
+        return u;
+    }();
+    std::vector<ExtTune> tuned = untuned;
     std::vector<bool> done(t->views.size(), false);
     for (int vi : t->order) {
         if (done[vi]) continue;
@@ -1614,16 +1650,65 @@ int npt_autotune(npt_handle t) {
         if (rc != NPS_OK) break;
         // the same choice for every view of this scene at this resolution
         for (size_t k = 0; k < t->views.size(); ++k) {
-            npt_view& o = t->views[k];
+            const npt_view& o = t->views[k];
             if (o.scene == w.scene && o.W == w.W && o.H == w.H) {
-                o.tune = best;
+                tuned[k] = best;
                 done[k] = true;
             }
         }
     }
+    // 2. in context: the whole step's views as one captured graph (no chain rule,
+    // no Adam: the scenes are untouched) for {scene defaults, per-class choices} x
+    // {all lanes, half of them}; the fastest replay wins
+    const int L = (int)t->lanes.size();
+    const int lane_opts[2] = {L, std::max(1, L / 2)};
+    int best_c = -1;
+    float best_ms = 1e30f;
+    for (int c = 0; rc == NPS_OK && c < 4; ++c) {
+        t->tune_ms[c] = -1.0f;
+        if (c % 2 == 1 && lane_opts[1] == lane_opts[0]) continue;
+        const std::vector<ExtTune>& tu = c < 2 ? untuned : tuned;
+        for (size_t k = 0; k < t->views.size(); ++k) t->views[k].tune = tu[k];
+        t->lanes_used = lane_opts[c % 2];
+        npt_assign_lanes(t);
+        if ((rc = npt_plan(t, true)) != NPS_OK) break;
+        float tmin = 1e30f;
+        for (int rep = 0; rep < 4 && rc == NPS_OK; ++rep) {  // a warm-up replay, then the best of 3
+            if (hipEventRecord(ev[0], t->main) != hipSuccess || hipGraphLaunch(t->exec, t->main) != hipSuccess ||
+                hipEventRecord(ev[1], t->main) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess) {
+                rc = npt_fail(t, NPS_ERR_HIP, "autotune replay");
+                break;
+            }
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+            if (rep > 0) tmin = std::min(tmin, ms);
+        }
+        npt_drop_graph(t);
+        t->tune_ms[c] = tmin;
+        if (tmin < best_ms) {
+            best_ms = tmin;
+            best_c = c;
+        }
+    }
     for (auto& e : ev) (void)hipEventDestroy(e);
+    if (best_c < 0) best_c = 0;
+    const std::vector<ExtTune>& tu = best_c < 2 ? untuned : tuned;
+    for (size_t k = 0; k < t->views.size(); ++k) t->views[k].tune = tu[k];
+    t->lanes_used = lane_opts[best_c % 2];
+    npt_assign_lanes(t);
     npt_drop_graph(t);
     return rc;
+}
+
+int npt_graph_info(npt_handle t, int64_t info[2], float tune_ms[4]) {
+    if (!t) return NPS_ERR_ARG;
+    if (info) {
+        info[0] = t->graph_nodes;
+        info[1] = t->lanes_used > 0 ? t->lanes_used : (int64_t)t->lanes.size();
+    }
+    if (tune_ms)
+        for (int c = 0; c < 4; ++c) tune_ms[c] = t->tune_ms[c];
+    return NPS_OK;
 }
 
 int npt_read_tune(npt_handle t, int32_t* tile_batch2v) {

@@ -449,6 +449,16 @@ class GpuMultiViewStep:
         self._check(self._lib.npt_read_losses(self._t, dptr(out)), "read_losses")
         return out
 
+    def graph_info(self) -> dict:
+        """Nodes of the last captured graph, lanes in use, and the last autotune's
+        in-context replay times (ms) of {own settings, per-class choices} x
+        {all lanes, half of them} (-1: not run)."""
+        info = np.zeros(2, dtype=np.int64)
+        ms = np.zeros(4, dtype=np.float32)
+        self._check(self._lib.npt_graph_info(self._t, info.ctypes.data_as(C.POINTER(C.c_int64)), fptr(ms)),
+                    "graph_info")
+        return {"nodes": int(info[0]), "lanes": int(info[1]), "autotune_ms": [round(float(x), 3) for x in ms]}
+
     def close(self):
         if getattr(self, "_t", None):
             self._lib.npt_destroy(self._t)

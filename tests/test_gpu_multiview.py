@@ -160,3 +160,91 @@ def test_serial_per_view_step_equals_per_view_step():
     sa.close(); sb.close()
     for mgr in ma + mb:
         mgr.close()
+
+
+def test_autotune_in_context_keeps_fastest_and_results():
+    """npt_autotune times whole-step graphs for its candidates and keeps the
+    fastest; the step it chooses gives the per-view results (the autotuner
+    itself leaves the scenes untouched)."""
+    data = [_scene(2500, 6, 3000), _scene(4000, 7, 4500)]
+    views = _views(9)
+    ma, sa = _build(views, data)
+    mb, sb = _build(views, data)
+    before = [m.read_splats() for m in mb]
+    sb.autotune()
+    for m, b in zip(mb, before):
+        assert np.array_equal(m.read_splats(), b), "autotuning changed a scene"
+    info = sb.graph_info()
+    ms = [x for x in info["autotune_ms"] if x >= 0]
+    assert len(ms) >= 2 and min(ms) > 0
+    print("autotune candidates (ms):", info["autotune_ms"], "lanes", info["lanes"])
+    sa.step(graph=False)
+    sb.step(graph=True)
+    assert sb.graph_info()["nodes"] > 0
+    np.testing.assert_allclose(sa.read_losses(), sb.read_losses(), rtol=1e-12, atol=0)
+    for x, y in zip(ma, mb):
+        np.testing.assert_allclose(x.read_splats(), y.read_splats(), rtol=1e-5, atol=1e-6)
+    sa.close(); sb.close()
+    for mgr in ma + mb:
+        mgr.close()
+
+
+def _many_views(count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        w, h = (64, 48) if k % 2 else (48, 64)
+        ang = rng.uniform(0, 2 * np.pi)
+        eye = (3.0 * np.cos(ang), rng.uniform(-0.5, 0.5), 3.0 * np.sin(ang))
+        vp, vi = camera.view_proj(eye, aspect=w / h)
+        tgt = np.ones((h, w, 4), np.float32)
+        tgt[..., :3] = rng.random((h, w, 3)).astype(np.float32)
+        out.append((k % 2, vp, vi, tgt))
+    return out
+
+
+def test_large_graph_near_the_node_limit_and_refusal_above_it():
+    """A step graph close to the node limit is captured, replayed and equals
+    per-view launches; one above it is refused (NPS_ERR_CAPACITY) and per-view
+    mode still works."""
+    from nprush_amd import _lib
+
+    data = [_scene(1500, 8, 1800), _scene(1500, 9, 1800)]
+    # nodes per view from two small graphs, then the view count that brings the step near the limit
+    counts = []
+    for nv in (4, 8):
+        probe_m, probe = _build(_many_views(nv, 10), data)
+        probe.step(graph=True)
+        counts.append(probe.graph_info()["nodes"])
+        probe.close()
+        for m in probe_m:
+            m.close()
+    per_view = (counts[1] - counts[0]) / 4.0
+    fixed = counts[0] - 4 * per_view
+    near = int((0.985 * 8192 - fixed) / per_view)
+    views = _many_views(near, 11)
+    ma, sa = _build(views, data)
+    mb, sb = _build(views, data)
+    sa.step(graph=False)
+    sb.step(graph=True)
+    nodes = sb.graph_info()["nodes"]
+    print("graph nodes:", nodes, "views:", near)
+    assert 0.95 * 8192 <= nodes <= 8192
+    sb.step(graph=True)   # replayed
+    sa.step(graph=False)
+    assert np.array_equal(sa.read_losses(), sb.read_losses())
+    for x, y in zip(ma, mb):
+        assert np.array_equal(x.read_splats(), y.read_splats())
+    sa.close(); sb.close()
+    for mgr in ma + mb:
+        mgr.close()
+    # above the limit
+    views = _many_views(near + 40, 12)
+    mc, sc = _build(views, data)
+    with pytest.raises(_lib.NpsError, match="nodes"):
+        sc.step(graph=True)
+    sc.step(graph=False)
+    assert sc.read_losses()[:, 1].min() > 0
+    sc.close()
+    for mgr in mc:
+        mgr.close()
