@@ -4,7 +4,7 @@ Hot path in hand-written HIP kernels (nprush_amd/csrc) behind a C ABI
 (include/nprush_hip.h, libnprush_hip.so); Python classes mirroring the
 reference's GpuSplatOptimizerManager / GpuSplatForwardPipelineManager.
 """
-from . import camera, scenes  # noqa: F401
+from . import camera, runner, scenes  # noqa: F401
 from .managers import (  # noqa: F401
     DensifierMLP,
     GpuBezierForwardPipelineManager,
