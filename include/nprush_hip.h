@@ -463,6 +463,18 @@ int nml_dev_alloc(nml_handle h, int64_t bytes, void** out);
 int nml_dev_free(nml_handle h, void* p);
 int nml_dev_upload(nml_handle h, void* dst, const void* src, int64_t bytes);
 
+/* The viewer's debug display compositor (src/lib/gpu/splat_render.wgsl:24-181,
+ * drawn by GpuRunner as the final pass).  textures[8]: host rgba float images in
+ * the shader's binding order (target, splat view, target depth, target edges,
+ * edge-layer Bezier view, base-colour Bezier view, colour Bezier view, path
+ * trace), dims[2 k], dims[2 k + 1] their width and height; enabled[3]: the
+ * edge, base-colour and colour Bezier layers (on above 0.5, the uniforms).
+ * Writes the W x H canvas (rgba floats), each pixel shaded at its centre.
+ * Texel reads clamp to the texture (WGSL leaves an out-of-range read
+ * implementation-defined).  Display only: allocates per call. */
+int npd_compose(int32_t device, const float* const* textures, const int32_t* dims, const float enabled[3],
+                float* canvas, int32_t W, int32_t H);
+
 /* Device buffers on the handle's GPU for callers without a GPU allocator of
  * their own (Deno, ctypes), e.g. to keep several targets resident in HBM for
  * nps_set_target_dev.  No reference counterpart (WebGPU's createBuffer). */

@@ -98,6 +98,7 @@ SIGNATURES = {
     "npb_blur": (i32, [_P, _F, _F, i32, i32, i32, C.c_float, i32]),
     "npb_blur_dev": (i32, [_P, _P, _P, i32, i32, i32, C.c_float, i32]),
     "npb_blur_dev_noalloc": (i32, [_P, _P, _P, i32, i32, i32, C.c_float, i32]),
+    "npd_compose": (i32, [i32, C.POINTER(_F), _I, _F, _F, i32, i32]),
     "nps_dev_alloc": (i32, [_P, C.c_int64, C.POINTER(_P)]),
     "nps_dev_free": (i32, [_P, _P]),
     "nps_dev_upload": (i32, [_P, _P, _P, C.c_int64]),

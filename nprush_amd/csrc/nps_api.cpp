@@ -1881,6 +1881,43 @@ int npb_blur_dev_noalloc(npb_handle b, const float* src, float* dst, int32_t w, 
     return NPS_OK;
 }
 
+int npd_compose(int32_t device, const float* const* textures, const int32_t* dims, const float enabled[3],
+                float* canvas, int32_t W, int32_t H) {
+    if (!textures || !dims || !enabled || !canvas || W <= 0 || H <= 0) return NPS_ERR_ARG;
+    for (int k = 0; k < 8; ++k)
+        if (!textures[k] || dims[2 * k] <= 0 || dims[2 * k + 1] <= 0) return NPS_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return NPS_ERR_HIP;
+    hipStream_t st = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return NPS_ERR_HIP;
+    DisplayArgs a;
+    void* bufs[9] = {nullptr};
+    int rc = NPS_OK;
+    for (int k = 0; k < 8 && rc == NPS_OK; ++k) {
+        const size_t bytes = 16 * (size_t)dims[2 * k] * dims[2 * k + 1];
+        if (hipMalloc(&bufs[k], bytes) != hipSuccess ||
+            hipMemcpyAsync(bufs[k], textures[k], bytes, hipMemcpyHostToDevice, st) != hipSuccess)
+            rc = NPS_ERR_HIP;
+        a.tex[k] = DisplayTex{(const float4*)bufs[k], dims[2 * k], dims[2 * k + 1]};
+    }
+    a.edge_on = enabled[0] > 0.5f;
+    a.base_color_on = enabled[1] > 0.5f;
+    a.color_on = enabled[2] > 0.5f;
+    const size_t cbytes = 16 * (size_t)W * H;
+    if (rc == NPS_OK && hipMalloc(&bufs[8], cbytes) != hipSuccess) rc = NPS_ERR_HIP;
+    if (rc == NPS_OK) {
+        launch_compose(st, a, (float4*)bufs[8], W, H);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(canvas, bufs[8], cbytes, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            rc = NPS_ERR_HIP;
+    }
+    (void)hipStreamSynchronize(st);
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
 int nps_dev_alloc(nps_handle h, int64_t bytes, void** out) {
     if (!h || !out || bytes < 0) return NPS_ERR_ARG;
     NPS_HIP(h, hipSetDevice(h->device));

@@ -200,6 +200,19 @@ void launch_blur(hipStream_t st, const float4* src, float4* dst, float4* temp, i
 void launch_blur_u8temp(hipStream_t st, const float4* src, float4* dst, uchar4* temp, int W, int H, int radius,
                         float sigma, int is_srgb);
 
+// k_display.hip: the debug display compositor (splat_render.wgsl).  Textures in
+// binding order: target, splat view, target depth, target edges, edge-layer
+// Bezier view, base-colour Bezier view, colour Bezier view, path trace.
+struct DisplayTex {
+    const float4* p;
+    int w, h;
+};
+struct DisplayArgs {
+    DisplayTex tex[8];
+    int edge_on, base_color_on, color_on;
+};
+void launch_compose(hipStream_t st, const DisplayArgs& a, float4* canvas, int W, int H);
+
 // k_debug.hip
 void launch_debug_math(hipStream_t st, int op, const float* x, const float* y, float* out, uint32_t n);
 

@@ -330,6 +330,22 @@ def ext_backward_csr_f64(splats, sh, deg, vp, vp_inv, tgt, W, H, offs, ids, want
     return g, gs, c, loss
 
 
+def compose_f32(textures, enabled, W, H):
+    """The debug display compositor (splat_render.wgsl) on the CPU: textures = 8
+    H_k x W_k x 4 arrays in binding order, enabled = (edge, base colour, colour)
+    layer flags; returns the H x W x 4 canvas."""
+    lib = load()
+    lib.ora_compose_f32.argtypes = [C.POINTER(_F), _I, _F, _F, i32, i32]
+    lib.ora_compose_f32.restype = i32
+    arrs = [f32(t) for t in textures]
+    ptrs = (_F * 8)(*[_p(a, _F) for a in arrs])
+    dims = np.array([[a.shape[1], a.shape[0]] for a in arrs], np.int32).reshape(16)
+    en = np.asarray(enabled, np.float32)
+    out = np.zeros((H, W, 4), np.float32)
+    lib.ora_compose_f32(ptrs, _p(dims, _I), _p(en, _F), _p(out, _F), W, H)
+    return out
+
+
 # ---------------------------------------------------------------- Bezier path
 BZ_CAP = 1024
 
