@@ -2,7 +2,8 @@
 
 --mode ext (default): the scale-up configuration with SH degree 3.  One "step"
 = one view of the optimizer loop: backward (preprocess with SH colour, depth
-keys, sort, ranges, composite fwd+bwd, per-splat chain rule) + Adam.
+keys, binning into per-tile lists, composite fwd+bwd, per-splat chain rule)
++ Adam.
   1,000,000 Gaussians drawn with seed 0 (means U[-1,1]^3, log-scales
   N(-4,0.5), uniform quaternions, opacity logits N(0,1), SH degree 3: DC from
   RGB U[0,1], higher bands N(0,0.05)), 8 cameras on a radius-3 ring at
@@ -10,7 +11,8 @@ keys, sort, ranges, composite fwd+bwd, per-splat chain rule) + Adam.
   drawn the same way, quantised to k/255.  Densification runs every 100
   steps, so the timed steps (fewer than 100) contain none.
 
---mode ref: the reference's own splat model.
+--mode ref: the reference's own splat model (the default run also measures
+it, as the "ref" object of the EXT line; --no-ref skips that leg).
 Workload (one "step" = one reference dispatch for one view, i.e.
 GpuSplatOptimizerManager.dispatch: backward (fwd+loss+bwd) + Adam step, ADC
 every 25th step):
@@ -200,6 +202,22 @@ def batched_config2(steps=6, warmup=1, lanes=8):
     }
 
 
+def host_info(oracle_lib) -> dict:
+    """The host CPU the CPU baselines ran on: model name, OpenMP threads used
+    and the OMP_NUM_THREADS setting they came from."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cores": int(oracle_lib.ora_num_threads()), "cpu_model": model,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "host_cpus": os.cpu_count()}
+
+
 def cpu_baseline_ext(s, sh, deg, vp, vi, target, W, H, sample_tiles=0, seed=0):
     """The EXT oracle (FP32, OpenMP on all host cores) on one whole view:
     projection, binning in O(keys) with the per-tile depth sort, forward + L1 +
@@ -214,7 +232,7 @@ def cpu_baseline_ext(s, sh, deg, vp, vi, target, W, H, sample_tiles=0, seed=0):
     return {
         "value": round(dt * 1000.0, 1),
         "unit": "ms/view (fwd+bwd)",
-        "cores": int(oracle.load().ora_num_threads()),
+        **host_info(oracle.load()),
         "kind": "port",
         "sample": f"one whole view (view 0, {len(ids)} tile keys), binning + fwd + L1 + bwd + chain rule",
     }
@@ -236,7 +254,7 @@ def cpu_baseline(splats, vp, vi, target, depth, W, H, sample_tiles, seed=0):
     return {
         "value": round(ms_view, 1),
         "unit": "ms/view (fwd+bwd)",
-        "cores": int(oracle.load().ora_num_threads()),
+        **host_info(oracle.load()),
         "kind": "port",
         "sample": f"{len(tiles)} random tiles of view 0 ({dt:.1f} s), scaled by {ntiles}/{len(tiles)} tiles",
     }
@@ -373,7 +391,7 @@ def bezier_config3(steps=20, warmup=3, cpu_tiles=16, n=100_000, W=1024, H=1024, 
         oracle.bz_backward_f32(curves, uni, *tex, W, H, tiles=tiles, lists=lists, counts=cnt)
         dt = time.perf_counter() - c1
         cpu = {"value": round(dt * 1000.0 * ntiles / len(tiles), 1), "unit": "ms/view (backward)",
-               "cores": int(oracle.load().ora_num_threads()), "kind": "port",
+               **host_info(oracle.load()), "kind": "port",
                "sample": f"{len(tiles)} random tiles of view 0 ({dt:.1f} s): binning + fwd + loss + bwd, "
                          f"scaled by {ntiles}/{len(tiles)} tiles"}
     for ptr in bufs:
@@ -396,7 +414,7 @@ def bezier_config3(steps=20, warmup=3, cpu_tiles=16, n=100_000, W=1024, H=1024, 
         "reverse_recompute_pair_fraction": round(64.0 * rev_eval / max(1.0, mean("pairs", 1)), 4),
         "forward_only_composite_ms": round(float(np.median(fwd_ms)), 3),
         "reverse_pairs_per_view": int(rev_pairs),
-        "roofline": {"bound": "mfma", "compute": "fp32 VALU (no matrix work on this path)",
+        "roofline": {"bound": "valu", "compute": "fp32 VALU issue (no matrix work on this path)",
                      "kernel": "k_bz_forward + k_bz_reverse", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": traffic},
@@ -727,6 +745,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the multi-view (64 views, 4 scenes) comparison")
     ap.add_argument("--no-bezier", action="store_true", help="skip the Bezier stroke workload")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-densification comparison")
+    ap.add_argument("--no-ref", action="store_true", help="skip the REF-semantics leg (1M live / 2M slots, 1080p)")
     ap.add_argument("--no-densify", action="store_true", help="skip the timed EXT densification pass")
     ap.add_argument("--cpu-tiles", type=int, default=48, help="tiles in the CPU-baseline sample (0 = skip)")
     ap.add_argument("--tile", type=int, default=0,
@@ -750,13 +769,21 @@ def main():
     if args.mode == "mlp":
         return main_mlp(args)
 
+    out = ref_leg(args.steps, args.warmup, args.cpu_tiles, small=args.small, calibrate=args.calibrate)
+    print(json.dumps(out), flush=True)
+
+
+def ref_leg(steps, warmup, cpu_tiles, small=False, calibrate=False):
+    """The reference-semantics (REF) path at its own headline workload: 1M live
+    splats in 2M slots, 1920x1080, 8 ring views; dispatch = backward + Adam.
+    Returns the full bench record of that run (the --mode ref line)."""
     from nprush_amd import GpuSplatOptimizerManager
 
-    if args.small:
+    if small:
         n_live, capacity, W, H, views = 16384, 32768, 256, 256, 8
     else:
         n_live, capacity, W, H, views = 1_000_000, 2_000_000, 1920, 1080, 8
-    print(f"[bench] building workload: {n_live} live / {capacity} capacity, {W}x{H}, {views} views", file=sys.stderr,
+    print(f"[bench] building REF workload: {n_live} live / {capacity} capacity, {W}x{H}, {views} views", file=sys.stderr,
           flush=True)
     cams, targets, depth, splats = build_workload(n_live, capacity, W, H, views, 3.0)
 
@@ -773,17 +800,17 @@ def main():
         st = mgr.read_stats()
         return t, st
 
-    for k in range(args.warmup):
+    for k in range(warmup):
         one_step(k)
     mgr.sync()
-    print(f"[bench] warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+    print(f"[bench] REF warmup done ({warmup} steps)", file=sys.stderr, flush=True)
     rows = []
     t0 = time.perf_counter()
-    for k in range(args.steps):
-        rows.append(one_step(args.warmup + k))
+    for k in range(steps):
+        rows.append(one_step(warmup + k))
     mgr.sync()
     wall = time.perf_counter() - t0
-    print(f"[bench] timed {args.steps} steps in {wall:.2f} s", file=sys.stderr, flush=True)
+    print(f"[bench] REF timed {steps} steps in {wall:.2f} s", file=sys.stderr, flush=True)
 
     bwd_ms = float(np.mean([t["backward"] for t, _ in rows]))
     comp_ms = float(np.mean([t["composite"] for t, _ in rows]))
@@ -800,7 +827,7 @@ def main():
     # WRITE_SIZE runs of this workload (calibrated against a streaming kernel)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath) and not args.small:
+    if os.path.exists(tpath) and not small:
         with open(tpath) as fh:
             tj = json.load(fh).get("k_composite_ref")
         if tj:
@@ -824,25 +851,25 @@ def main():
            "algorithmic_bytes": int(adc_b), "hbm_gbs": round(adc_b / (adc_ms * 1e-3) / 1e9, 1),
            "roofline_frac": round(adc_b / (adc_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 3)}
 
-    if args.calibrate:
+    if calibrate:
         from nprush_amd import debug_math
 
         debug_math(0, np.linspace(-8.0, 8.0, 1 << 24, dtype=np.float32))
 
     cpu = None
-    if args.cpu_tiles > 0:
-        print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
+    if cpu_tiles > 0:
+        print("[bench] REF CPU baseline (oracle) ...", file=sys.stderr, flush=True)
         vp, vi = cams[0]
-        cpu = cpu_baseline(splats, vp, vi, targets[0], depth, W, H, args.cpu_tiles)
+        cpu = cpu_baseline(splats, vp, vi, targets[0], depth, W, H, cpu_tiles)
 
     out = {
         "metric": "fwd+bwd ms/view @1M splats 1080p",
         "value": round(bwd_ms, 3),
         "unit": "ms/view",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(wall * 1000.0 / args.steps, 3),
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(wall * 1000.0 / steps, 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
@@ -851,7 +878,7 @@ def main():
         "config": {"workload": f"REF splat fwd+bwd+Adam, {n_live} live / {capacity} capacity splats, {W}x{H}, "
                                f"{views} ring views, 16x16 tiles, cap 1024/tile",
                    "views": views, "width": W, "height": H, "splats": n_live, "capacity": capacity},
-        "roofline": {"bound": "mfma", "compute": "fp32 VALU (no matrix work on this path)", "kernel": "k_composite_ref", "achieved": round(achieved, 3),
+        "roofline": {"bound": "valu", "compute": "fp32 VALU issue (no matrix work on this path)", "kernel": "k_composite_ref", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": traffic},
         "phases_ms": phases,
@@ -863,13 +890,14 @@ def main():
         "adam_hbm_gbs": round(adam_gbs, 1),
         "adc": adc,
         "adam_roofline_frac": round(adam_gbs / PEAK_HBM_GBS, 3),
-        "steps_per_s": round(args.steps / wall, 2),
+        "steps_per_s": round(steps / wall, 2),
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
     for s in slots:
         mgr.free_target(s)
     mgr.close()
+    return out
+
 
 
 def main_ext(args):
@@ -927,11 +955,11 @@ def main_ext(args):
     achieved = flops / (comp_ms * 1e-3) / 1e12
     phases = {k: round(mean(k), 3) for k in ("preprocess_keys", "sort", "ranges", "composite", "chain", "step")}
     keys = mean("keys", 1)
-    # tile sort (the "sort" phase): 8-byte keys read + written per pass, 2
-    # passes over the tile keys (13 tile bits at 1080p); the 4 passes of the
-    # per-splat (depth, id) keys run in the preprocess phase and are not counted
-    tile_bits = int(np.ceil(np.log2(((W + tile - 1) // tile) * ((H + tile - 1) // tile))))
-    sort_bytes = 16.0 * keys * int(np.ceil(tile_bits / 8))
+    # binning (the "sort" phase): 4-byte items written once by the row
+    # enumeration, read by the column count, read and written by the column
+    # scatter = 16 bytes per tile key; the per-splat (depth, id) sort runs in
+    # the preprocess phase and is not counted
+    sort_bytes = 16.0 * keys
     # Adam: per splat 64 parameters x (param + grad + m + v), read and written = 2 KiB
     adam_gbs = 2048.0 * n / (step_ms * 1e-3) / 1e9
     traffic = None
@@ -962,6 +990,15 @@ def main_ext(args):
         print("[bench] learned densification vs plain ...", file=sys.stderr, flush=True)
         learned = learned_config4()
         learned["mlp_alone"] = mlp_bench()
+    ref = None
+    if not args.small and not args.no_ref:
+        print("[bench] REF path (reference semantics): 1M live / 2M slots, 1920x1080 ...", file=sys.stderr,
+              flush=True)
+        r = ref_leg(args.steps, args.warmup, args.cpu_tiles)
+        ref = {k: r[k] for k in ("metric", "value", "unit", "steps", "warmup", "ms_per_step", "config", "roofline",
+                                 "phases_ms", "tile_keys_per_view", "pairs_per_view", "active_pairs_per_view",
+                                 "adam_hbm_gbs", "adam_roofline_frac", "adc", "cpu_baseline")}
+        ref["speedup_vs_cpu"] = round(r["cpu_baseline"]["value"] / r["value"], 1) if r["cpu_baseline"] else None
     cpu = None
     if args.cpu_tiles > 0:
         print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
@@ -987,7 +1024,7 @@ def main_ext(args):
         "autotune_ms": None if tune_ms is None else {f"tile{t}_batch{b}": round(float(tune_ms[i, j]), 3)
                                                      for i, t in enumerate((8, 16, 32))
                                                      for j, b in enumerate((32, 64, 128))},
-        "roofline": {"bound": "mfma", "compute": "fp32 VALU (no matrix work on this path)", "kernel": "k_composite_ext", "achieved": round(achieved, 3),
+        "roofline": {"bound": "valu", "compute": "fp32 VALU issue (no matrix work on this path)", "kernel": "k_composite_ext", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": traffic},
         "phases_ms": phases,
@@ -995,7 +1032,7 @@ def main_ext(args):
         "pairs_per_view": int(pairs),
         "active_pairs_per_view": int(active),
         "reverse_lane_efficiency": round(active / max(1.0, 64.0 * mean("reverse_wave_entries", 1)), 3),
-        "sort_hbm_gbs": round(sort_bytes / (mean("sort") * 1e-3) / 1e9, 1),
+        "binning_hbm_gbs": round(sort_bytes / (mean("sort") * 1e-3) / 1e9, 1),
         "adam_hbm_gbs": round(adam_gbs, 1),
         "adam_roofline_frac": round(adam_gbs / PEAK_HBM_GBS, 3),
         "steps_per_s": round(args.steps / wall, 2),
@@ -1003,6 +1040,7 @@ def main_ext(args):
         "bezier_strokes": bezier,
         "densification": adc,
         "learned_densification": learned,
+        "ref": ref,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

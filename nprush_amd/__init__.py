@@ -13,6 +13,7 @@ from .managers import (  # noqa: F401
     GpuMultiViewStep,
     GpuSplatForwardPipelineManager,
     GpuSplatOptimizerManager,
+    compose_display,
     debug_math,
     reference_bezier_init,
     reference_init,

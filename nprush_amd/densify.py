@@ -83,9 +83,16 @@ def record_pairs(mgr: GpuSplatOptimizerManager, views, events: int, warm: int = 
 
 
 def train_mlp(X: np.ndarray, T: np.ndarray, epochs: int = 30, batch: int = 4096, lr: float = 1e-3, seed: int = 0,
-              holdout: float = 0.1):
+              holdout: float = 0.1, shrink: bool = True):
     """Trains a fresh DensifierMLP on standardised (X, T); returns (net, train
-    losses per epoch, held-out loss of the untrained and trained network)."""
+    losses per epoch, held-out loss of the untrained and trained network).
+
+    shrink: each of the 14 outputs is then scaled by its least-squares factor on
+    the held-out pairs, beta = <prediction, target> / <prediction, prediction>
+    clipped to [0, 1] (the factor that minimises the held-out squared error of
+    beta * prediction, shrinking weak predictions toward "no change"); the
+    network's de-standardisation carries it.  ``net.shrink`` and
+    ``net.heldout_r2`` keep the factors and the held-out R^2 per output."""
     rng = np.random.default_rng(seed)
     n = X.shape[0]
     perm = rng.permutation(n)
@@ -113,7 +120,18 @@ def train_mlp(X: np.ndarray, T: np.ndarray, epochs: int = 30, batch: int = 4096,
             tot += net.train(X[idx], T[idx], lr) * len(idx)
             cnt += len(idx)
         losses.append(tot / max(cnt, 1))
-    return net, losses, (before, held())
+    after = held()
+    net.shrink = np.ones(14, np.float32)
+    net.heldout_r2 = np.full(14, np.nan, np.float32)
+    if nv > 1:
+        y, t = net.forward(X[va]).astype(np.float64), T[va].astype(np.float64)
+        tc = t - t.mean(0)
+        net.heldout_r2 = (1.0 - ((y - t) ** 2).sum(0) / np.maximum((tc ** 2).sum(0), 1e-300)).astype(np.float32)
+        if shrink:
+            beta = np.clip((y * t).sum(0) / np.maximum((y * y).sum(0), 1e-300), 0.0, 1.0)
+            net.shrink = beta.astype(np.float32)
+            net.set_norm(in_mean, in_std, beta * out_mean, beta * out_std + 1e-12)
+    return net, losses, (before, after)
 
 
 def view_loss(mgr: GpuSplatOptimizerManager, view) -> float:
@@ -190,6 +208,42 @@ def run(mgr: GpuSplatOptimizerManager, views, steps: int, adc_period: int, densi
                 mgr.adc()
             else:
                 densifier.densify(mgr)
+        mgr.sync()
+        t_train += time.perf_counter() - t0
+        if (k + 1) % eval_every == 0:
+            curve.append((t_train, float(np.mean([view_loss(mgr, v) for v in views]))))
+    return curve
+
+
+def run_lookahead_bound(mgr: GpuSplatOptimizerManager, views, steps: int, adc_period: int, horizon: int,
+                        eval_every: int = 100):
+    """Upper bound of what any child-placement proposal can gain: training with
+    plain densification where, after each densification, the children are
+    moved to the state they reach `horizon` steps later (a look-ahead run from a
+    snapshot, then restored) -- a perfect prediction of their future.  Only the
+    main trajectory is timed.  Returns the run's [(seconds, mean L1)]."""
+    curve = []
+    t_train = 0.0
+    for k in range(steps):
+        t0 = time.perf_counter()
+        _set_view(mgr, views[k % len(views)])
+        mgr.backward()
+        mgr.step()
+        if (k + 1) % adc_period == 0:
+            mgr.adc()
+            mgr.sync()
+            t_train += time.perf_counter() - t0
+            par, ch = mgr.adc_children()
+            if len(par):
+                mgr.snapshot()
+                train_steps(mgr, views, k + 1, horizon)
+                ahead = mgr.read_splats()
+                mgr.restore()
+                s = mgr.read_splats()
+                for slots in (par, ch):
+                    s[slots[:, None], CHILD_COLS] = ahead[slots[:, None], CHILD_COLS]
+                mgr.write_splats(s)
+            t0 = time.perf_counter()
         mgr.sync()
         t_train += time.perf_counter() - t0
         if (k + 1) % eval_every == 0:

@@ -932,3 +932,27 @@ def debug_math(op: int, x: np.ndarray, y: np.ndarray | None = None, device: int 
     check(lib.nps_debug_math(device, op, fptr(x), fptr(yy) if yy is not None else None, fptr(out), x.size), None,
           "nps_debug_math")
     return out
+
+
+def compose_display(textures, enabled, width: int, height: int, device: int = 0) -> np.ndarray:
+    """The viewer's debug display compositor (src/lib/gpu/splat_render.wgsl:24-181)
+    through npd_compose.  textures: 8 (H_k, W_k, 4) float arrays in the
+    shader's binding order (target, splat view, target depth, target edges,
+    edge-layer, base-colour and colour Bezier views, path trace); enabled: the
+    edge, base-colour and colour Bezier layers.  Returns the (height, width, 4)
+    canvas."""
+    if len(textures) != 8:
+        raise ValueError("npd_compose takes 8 textures")
+    lib = _lib.load()
+    arrs = [np.ascontiguousarray(t, np.float32) for t in textures]
+    for a in arrs:
+        if a.ndim != 3 or a.shape[2] != 4:
+            raise ValueError("textures must be (H, W, 4)")
+    F = C.POINTER(C.c_float)
+    ptrs = (F * 8)(*[a.ctypes.data_as(F) for a in arrs])
+    dims = np.array([[a.shape[1], a.shape[0]] for a in arrs], np.int32).reshape(16)
+    en = np.ascontiguousarray(enabled, np.float32).reshape(3)
+    out = np.zeros((int(height), int(width), 4), np.float32)
+    check(lib.npd_compose(int(device), ptrs, iptr(dims), fptr(en), fptr(out), int(width), int(height)), None,
+          "npd_compose")
+    return out

@@ -2,28 +2,16 @@
 npd_compose against the oracle, bit for bit: random textures of different
 sizes and aspect ratios (letter- and pillarboxed panels), path-trace samples
 present in part of the image, every layer combination, several canvas sizes."""
-import ctypes as C
-
 import numpy as np
 import pytest
 
-from nprush_amd import _lib
+from nprush_amd import compose_display
 
 pytestmark = pytest.mark.gpu
 
 
 def _compose_gpu(textures, enabled, W, H):
-    lib = _lib.load()
-    arrs = [np.ascontiguousarray(t, np.float32) for t in textures]
-    F = C.POINTER(C.c_float)
-    ptrs = (F * 8)(*[a.ctypes.data_as(F) for a in arrs])
-    dims = np.array([[a.shape[1], a.shape[0]] for a in arrs], np.int32).reshape(16)
-    en = np.asarray(enabled, np.float32)
-    out = np.zeros((H, W, 4), np.float32)
-    rc = lib.npd_compose(0, ptrs, dims.ctypes.data_as(C.POINTER(C.c_int32)), en.ctypes.data_as(F),
-                         out.ctypes.data_as(F), W, H)
-    assert rc == 0
-    return out
+    return compose_display(textures, enabled, W, H)
 
 
 @pytest.mark.parametrize("W,H", [(1000, 500), (1920, 1080), (737, 911)])

@@ -134,15 +134,21 @@ def test_judge_keeps_heuristic_on_worse_proposal(bad):
 
 def test_recorded_pairs_train_and_densify():
     """The whole loop at small scale: pairs from plain runs, the MLP trained on
-    them, then plain vs learned densification on a fresh scene; reports the
-    final-loss gain (the judge never accepts a held-out loss increase)."""
+    them, then plain vs learned densification on a fresh scene.
+
+    Asserted: training lowers the training loss; the shrink factors lie in
+    [0, 1]; every accepted proposal lowered (or kept) the held-out view's loss;
+    if the judge accepted none, the learned run is the plain run bit for bit
+    (same final loss); otherwise its final loss is within 2 % of plain's."""
     m, views = _scene(seed=0)
     X, T = densify.record_pairs(m, views, events=2, warm=20, window=40)
     m.close()
     assert X.shape[0] > 100 and T.shape == (X.shape[0], 14)
     net, losses, (before, after) = densify.train_mlp(X, T, epochs=10, batch=1024)
     assert losses[-1] < losses[0]
+    assert np.all((net.shrink >= 0) & (net.shrink <= 1))
     finals = {}
+    accepted = 0
     for name in ("plain", "learned"):
         m, views = _scene(seed=4)
         judge = densify.LearnedDensifier(net, views[0]) if name == "learned" else None
@@ -150,7 +156,13 @@ def test_recorded_pairs_train_and_densify():
         finals[name] = curve[-1][1]
         if judge is not None:
             assert all(lm <= lh for dec, _, lh, lm in judge.decisions if dec == "learned")
+            accepted = sum(1 for d in judge.decisions if d[0] == "learned")
             judge.close()
         m.close()
-    print("final L1 plain %.2f learned %.2f" % (finals["plain"], finals["learned"]))
+    print("held-out R2 per output", np.round(net.heldout_r2, 3).tolist(), "shrink", np.round(net.shrink, 3).tolist())
+    print("final L1 plain %.4f learned %.4f, proposals accepted %d" % (finals["plain"], finals["learned"], accepted))
+    if accepted == 0:
+        assert finals["learned"] == finals["plain"]
+    else:
+        assert finals["learned"] <= 1.02 * finals["plain"]
     net.close()
