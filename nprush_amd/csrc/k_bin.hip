@@ -333,6 +333,12 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
     if (j < n) p[j] = v;
 }
 
+// 16-byte stores: p 16-byte aligned, n4 = words / 4
+__global__ void k_fill_u32x4(uint4* p, uint32_t v, uint64_t n4) {
+    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n4) p[j] = make_uint4(v, v, v, v);
+}
+
 void launch_preprocess(hipStream_t st, const float* splats, uint32_t n, const ViewParams& v, FwdRec* fwd,
                        BwdRec* bwd, uint32_t* counts, uint4* rects) {
     if (n == 0) return;
@@ -376,6 +382,12 @@ void launch_tile_ranges(hipStream_t st, const uint64_t* keys, uint64_t m, uint32
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint64_t n) {
     if (n == 0) return;
+    if (n >= 4096 && ((uintptr_t)p & 15u) == 0) {
+        const uint64_t n4 = n / 4;
+        hipLaunchKernelGGL(k_fill_u32x4, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, st, (uint4*)p, v, n4);
+        if (n % 4) hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, p + 4 * n4, v, n % 4);
+        return;
+    }
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
 }
 

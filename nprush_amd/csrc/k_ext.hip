@@ -416,41 +416,6 @@ __global__ __launch_bounds__(256) void k_bin_col_count(const uint32_t* __restric
         }
 }
 
-// Tile ranges, tiles taken column by column (the output order):
-// [start, end) of tile y * nx + x = the counts of the tiles before it in
-// column-major order (one workgroup).
-__global__ __launch_bounds__(1024) void k_bin_ranges(const uint32_t* __restrict__ tile_cnt, uint32_t nx, uint32_t ny,
-                                                     uint32_t* __restrict__ ranges) {
-    __shared__ uint32_t s_w[16];
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
-    const uint32_t ntiles = nx * ny;
-    uint32_t carry = 0u;
-    for (uint32_t c0 = 0; c0 < ntiles; c0 += 1024u) {
-        const uint32_t c = c0 + tid;
-        const uint32_t t = c < ntiles ? (c % ny) * nx + c / ny : 0u;
-        const uint32_t v = c < ntiles ? tile_cnt[t] : 0u;
-        uint32_t x = v;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(x, o, 64);
-            if ((int)lane >= o) x += y;
-        }
-        if (lane == 63u) s_w[wid] = x;
-        __syncthreads();
-        uint32_t wb = 0u, tot = 0u;
-        for (uint32_t w = 0; w < 16u; ++w) {
-            if (w < wid) wb += s_w[w];
-            tot += s_w[w];
-        }
-        if (c < ntiles) {
-            ranges[2 * t] = carry + wb + x - v;
-            ranges[2 * t + 1] = carry + wb + x;
-        }
-        carry += tot;
-        __syncthreads();
-    }
-}
-
 // Stable scatter of the items on their column (offs2: exclusive scan of
 // counts2), writing the splat ids: 4 waves x 1024 items, ranked per wave as in
 // k_radix_scatter, staged in LDS in output order and written as runs of
@@ -604,14 +569,65 @@ __device__ __forceinline__ float wave_reduce8(float (&v)[8], int lane) {
 constexpr uint32_t kOrderMax = 8192;
 constexpr uint32_t kOrderLevels = 1024;
 
-__global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict__ ranges, uint32_t tiles_x,
-                                                     uint32_t tiles_y, uint32_t* __restrict__ order) {
+// kFromCounts: src = per-tile list lengths, and the kernel first writes the
+// tile ranges with the tiles taken column by column (the binning's output
+// order: [start, end) of a tile = the lengths of the tiles before it in
+// column-major order); else src = the ranges.
+template <bool kFromCounts>
+__global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict__ src, uint32_t tiles_x,
+                                                     uint32_t tiles_y, uint32_t* __restrict__ order,
+                                                     uint32_t* __restrict__ ranges_out) {
     __shared__ uint32_t s_rank[kOrderMax];  // super-tile of each rank
     __shared__ uint32_t s_cnt[kOrderLevels];
     __shared__ uint32_t s_wsum[16];
     __shared__ uint32_t s_max;
     const uint32_t sx = (tiles_x + 1u) / 2u, sy = (tiles_y + 1u) / 2u, S = sx * sy;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
+    if (kFromCounts) {
+        // list position c = tx * tiles_y + ty; thread tid owns positions [per tid, per tid + per)
+        const uint32_t ntiles = tiles_x * tiles_y, per = (ntiles + 1023u) / 1024u;
+        const uint32_t c0 = tid * per;
+        constexpr uint32_t kFew = 8;  // up to 8192 tiles: all loads issued at once
+        uint32_t few[kFew];
+        uint32_t own = 0u;
+        if (per <= kFew) {
+#pragma unroll
+            for (uint32_t k = 0; k < kFew; ++k) {
+                const uint32_t c = c0 + k;
+                few[k] = (k < per && c < ntiles) ? src[(c % tiles_y) * tiles_x + c / tiles_y] : 0u;
+                own += few[k];
+            }
+        } else {
+            for (uint32_t k = 0; k < per; ++k) {
+                const uint32_t c = c0 + k;
+                if (c < ntiles) own += src[(c % tiles_y) * tiles_x + c / tiles_y];
+            }
+        }
+        uint32_t x = own;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if ((int)lane >= o) x += y;
+        }
+        if (lane == 63u) s_wsum[wid] = x;
+        __syncthreads();
+        uint32_t run = x - own;
+        for (uint32_t w = 0; w < wid; ++w) run += s_wsum[w];
+        for (uint32_t k = 0; k < per; ++k) {
+            const uint32_t c = c0 + k;
+            if (c < ntiles) {
+                const uint32_t t = (c % tiles_y) * tiles_x + c / tiles_y;
+                const uint32_t len = per <= kFew ? few[k < kFew ? k : 0] : src[t];
+                ranges_out[2 * t] = run;
+                ranges_out[2 * t + 1] = run + len;
+                run += len;
+            }
+        }
+        __syncthreads();
+    }
+    auto tile_len = [&](uint32_t t) -> uint32_t {
+        return kFromCounts ? src[t] : src[2 * t + 1] - src[2 * t];
+    };
     const bool rank = S <= kOrderMax;
     if (rank) {
         constexpr uint32_t kPer = kOrderMax / 1024u;
@@ -625,10 +641,7 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict_
                 const uint32_t stx = sidx % sx, sty = sidx / sx;
                 for (uint32_t j = 0; j < 4u; ++j) {
                     const uint32_t tx = 2u * stx + (j & 1u), ty = 2u * sty + (j >> 1);
-                    if (tx < tiles_x && ty < tiles_y) {
-                        const uint32_t t = ty * tiles_x + tx;
-                        c += ranges[2 * t + 1] - ranges[2 * t];
-                    }
+                    if (tx < tiles_x && ty < tiles_y) c += tile_len(ty * tiles_x + tx);
                 }
             }
             cost[q] = c;
@@ -1110,20 +1123,14 @@ __global__ __launch_bounds__(256) void k_ext_chain(const float4* __restrict__ sp
         any = any || gg[k] != 0.0;
     }
     float s[16];
-    float c[4 * kSh4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         float4 t = splats[4 * (size_t)i + q];
         s[4 * q + 0] = t.x; s[4 * q + 1] = t.y; s[4 * q + 2] = t.z; s[4 * q + 3] = t.w;
     }
-#pragma unroll
-    for (int q = 0; q < kSh4; ++q) {
-        float4 t = sh[12 * (size_t)i + q];
-        c[4 * q + 0] = t.x; c[4 * q + 1] = t.y; c[4 * q + 2] = t.z; c[4 * q + 3] = t.w;
-    }
     const float cam[3] = {cam0, cam1, cam2};
     ExtProj o;
-    ext_project(s, c, DEG, v.vp, cam, v.W, v.H, o);
+    ext_project(s, nullptr, DEG, v.vp, cam, v.W, v.H, o);  // geometry only: the colour follows below
     if (!o.valid) return;
     // densification statistics: |d L / d ndc| of the centre and the view count
     {
@@ -1134,35 +1141,15 @@ __global__ __launch_bounds__(256) void k_ext_chain(const float4* __restrict__ sp
         accum[i] = acc;
     }
     if (!any) return;
+    float* out = grads + 64 * (size_t)i;
     const float* vp = v.vp;
-    double G[16];
+    // geometry first (opacity, scales, quaternion, and the position terms of
+    // the projection); the SH part then needs only the view direction, so the
+    // projection's values and the SH coefficients are never live together
+    double G[12];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) G[k] = 0.0;
+    for (int k = 0; k < 12; ++k) G[k] = 0.0;
     G[3] = gg[8] * (double)(o.op * (1.0f - o.op));
-    float Y[16], Yx[16], Yy[16], Yz[16];
-    sh_basis(o.dir[0], o.dir[1], o.dir[2], DEG, Y);
-    sh_basis_grad(o.dir[0], o.dir[1], o.dir[2], DEG, Yx, Yy, Yz);
-    double gdir[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-        if (!(o.sh_raw[ch] > 0.0f)) continue;
-        const double drgb = gg[5 + ch];
-#pragma unroll
-        for (int k = 0; k < kNk; ++k) {
-            // one term per SH coefficient: written directly
-            float& gk = grads[64 * (size_t)i + 16 + 3 * k + ch];
-            if (kStore) gk = (float)(drgb * (double)Y[k]);
-            else gk += (float)(drgb * (double)Y[k]);
-            const double coef = (double)c[3 * k + ch];
-            gdir[0] += drgb * coef * (double)Yx[k];
-            gdir[1] += drgb * coef * (double)Yy[k];
-            gdir[2] += drgb * coef * (double)Yz[k];
-        }
-    }
-    const double dd = gdir[0] * o.dir[0] + gdir[1] * o.dir[1] + gdir[2] * o.dir[2];
-    double gpos[3];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) gpos[j] = (gdir[j] - dd * (double)o.dir[j]) / (double)o.dlen;
     // conic -> covariance: dL/dSigma = -Q (dL/dQ) Q with dL/dQ symmetric
     const double A = o.A, B = o.B, C = o.C;
     const double gA = gg[2], gB = gg[3] * 0.5, gC = gg[4];
@@ -1228,11 +1215,52 @@ __global__ __launch_bounds__(256) void k_ext_chain(const float4* __restrict__ sp
     for (int j = 0; j < 4; ++j) G[8 + j] += (gqn[j] - dq * (double)o.qn[j]) / (double)o.qlen;
 #pragma unroll
     for (int j = 0; j < 3; ++j)
-        G[j] += gpos[j] + gcx * (double)vp[4 * j + 0] + gcy * (double)vp[4 * j + 1] + gw * (double)vp[4 * j + 3];
-    float* out = grads + 64 * (size_t)i;
+        G[j] += gcx * (double)vp[4 * j + 0] + gcy * (double)vp[4 * j + 1] + gw * (double)vp[4 * j + 3];
 #pragma unroll
-    for (int j = 0; j < 12; ++j)
+    for (int j = 3; j < 12; ++j)
         if (j != 7) out[j] = kStore ? (float)G[j] : out[j] + (float)G[j];
+    // SH colour: forward value (its sign gates the channel, as in the
+    // projection), coefficient gradients, and the view direction's gradient
+    const float dir0 = o.dir[0], dir1 = o.dir[1], dir2 = o.dir[2], dlen = o.dlen;
+    float c[4 * kSh4];
+#pragma unroll
+    for (int q = 0; q < kSh4; ++q) {
+        float4 t = sh[12 * (size_t)i + q];
+        c[4 * q + 0] = t.x; c[4 * q + 1] = t.y; c[4 * q + 2] = t.z; c[4 * q + 3] = t.w;
+    }
+    float Y[16];
+    sh_basis(dir0, dir1, dir2, DEG, Y);
+    // per coefficient k: sum over the drawn channels of dL/drgb * coefficient,
+    // which the direction gradient then weights with dY_k/d dir
+    double wk[kNk];
+#pragma unroll
+    for (int k = 0; k < kNk; ++k) wk[k] = 0.0;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        float raw = 0.0f;
+#pragma unroll
+        for (int k = 0; k < kNk; ++k) raw = raw + Y[k] * c[3 * k + ch];
+        raw = raw + 0.5f;
+        if (!(raw > 0.0f)) continue;
+        const double drgb = gg[5 + ch];
+#pragma unroll
+        for (int k = 0; k < kNk; ++k) {
+            // one term per SH coefficient: written directly
+            float& gk = out[16 + 3 * k + ch];
+            if (kStore) gk = (float)(drgb * (double)Y[k]);
+            else gk += (float)(drgb * (double)Y[k]);
+            wk[k] += drgb * (double)c[3 * k + ch];
+        }
+    }
+    double gdir[3] = {0.0, 0.0, 0.0};
+    sh_grad_weighted(dir0, dir1, dir2, DEG, wk, gdir);
+    const double dd = gdir[0] * dir0 + gdir[1] * dir1 + gdir[2] * dir2;
+    const double dirv[3] = {dir0, dir1, dir2};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const double gpos = (gdir[j] - dd * dirv[j]) / (double)dlen;
+        out[j] = kStore ? (float)(G[j] + gpos) : out[j] + (float)(G[j] + gpos);
+    }
 }
 
 // ---------------------------------------------------------------- Adam
@@ -1666,39 +1694,47 @@ bool ext_bin_fits(uint32_t n, uint32_t tiles_x, uint32_t tiles_y, uint64_t cap) 
     return xb <= kBinMaxBits && yb <= kBinMaxBits && xb + idb <= 32 && cap < 0xFFFFFFFFull;
 }
 
-size_t ext_bin_scan_words(uint32_t n, uint32_t tiles_x, uint32_t tiles_y, uint64_t cap) {
-    const size_t nb1 = ((size_t)n + kBinThreads - 1) / kBinThreads, nb2 = (size_t)((cap + kBinColBlock - 1) / kBinColBlock);
-    // row offsets (+ total) followed by the column offsets (+ total)
-    return (size_t)tiles_y * std::max<size_t>(nb1, 1) + (size_t)tiles_x * std::max<size_t>(nb2, 1) + 2;
+size_t ext_bin_row_words(uint32_t n, uint32_t tiles_y) {
+    const size_t nb1 = ((size_t)n + kBinThreads - 1) / kBinThreads;
+    return (size_t)tiles_y * std::max<size_t>(nb1, 1) + 1;  // + the total
+}
+
+size_t ext_bin_col_words(uint32_t tiles_x, uint64_t cap) {
+    const size_t nb2 = (size_t)((cap + kBinColBlock - 1) / kBinColBlock);
+    return (size_t)tiles_x * std::max<size_t>(nb2, 1) + 1;
+}
+
+void ext_bin_rows(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts, const uint4* rects,
+                  uint32_t tiles_y, uint32_t* rcnt, uint32_t* roff, void* rscan) {
+    if (n == 0 || tiles_y == 0) return;
+    const uint32_t nb1 = (n + kBinThreads - 1) / kBinThreads;
+    hipLaunchKernelGGL(k_bin_row_count, dim3(nb1), dim3(kBinThreads), 0, st, n, order, rank_counts, rects, tiles_y,
+                       rcnt);
+    exclusive_scan_u32(st, rcnt, roff, (uint64_t)tiles_y * nb1, rscan);
 }
 
 void ext_bin(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts, const uint4* rects,
-             uint32_t tiles_x, uint32_t tiles_y, uint64_t cap, const uint64_t* m_dev, uint32_t* items, uint32_t* ids,
-             uint32_t* ranges, uint32_t* cnt, uint32_t* off, uint32_t* tile_cnt, void* scan_tmp,
-             uint32_t* overflow) {
+             uint32_t tiles_x, uint32_t tiles_y, uint64_t cap, const uint64_t* m_dev, const uint32_t* roff,
+             uint32_t* items, uint32_t* ids, uint32_t* ranges, uint32_t* tile_order, uint32_t* ccnt, uint32_t* coff,
+             uint32_t* tile_cnt, void* cscan, uint32_t* overflow) {
     const int xb = bits_for_u32(tiles_x > 0 ? tiles_x - 1 : 0), yb = bits_for_u32(tiles_y > 0 ? tiles_y - 1 : 0);
     const int idb = bits_for_u32(n > 1 ? n - 1 : 0);
     const uint32_t ntiles = tiles_x * tiles_y;
     launch_fill_u32(st, tile_cnt, 0u, ntiles);
     if (n > 0 && cap > 0 && ntiles > 0) {
         const uint32_t nb1 = (n + kBinThreads - 1) / kBinThreads;
-        hipLaunchKernelGGL(k_bin_row_count, dim3(nb1), dim3(kBinThreads), 0, st, n, order, rank_counts, rects, tiles_y,
-                           cnt);
-        exclusive_scan_u32(st, cnt, off, (uint64_t)tiles_y * nb1, scan_tmp);
         hipLaunchKernelGGL(k_bin_row_emit, dim3(nb1), dim3(kBinThreads), 0, st, n, order, rank_counts, rects, tiles_y,
-                           yb, idb, (const uint32_t*)off, items, cap, overflow);
+                           yb, idb, roff, items, cap, overflow);
         const uint32_t nb2 = (uint32_t)((cap + kBinColBlock - 1) / kBinColBlock);
-        // column counts reuse cnt; their offsets follow the row offsets (still read below)
-        uint32_t* off2 = off + (size_t)tiles_y * nb1 + 1;
-        hipLaunchKernelGGL(k_bin_col_count, dim3(nb2), dim3(256), 0, st, (const uint32_t*)items, cap, m_dev,
-                           (const uint32_t*)off, nb1, tiles_y, tiles_x, yb, xb, idb, cnt, tile_cnt);
-        exclusive_scan_u32(st, cnt, off2, (uint64_t)tiles_x * nb2, scan_tmp);
+        hipLaunchKernelGGL(k_bin_col_count, dim3(nb2), dim3(256), 0, st, (const uint32_t*)items, cap, m_dev, roff,
+                           nb1, tiles_y, tiles_x, yb, xb, idb, ccnt, tile_cnt);
+        exclusive_scan_u32(st, ccnt, coff, (uint64_t)tiles_x * nb2, cscan);
         hipLaunchKernelGGL(k_bin_col_scatter, dim3(nb2), dim3(256), 0, st, (const uint32_t*)items, cap, m_dev,
-                           tiles_x, xb, idb, (const uint32_t*)off2, ids);
+                           tiles_x, xb, idb, (const uint32_t*)coff, ids);
     }
-    if (ntiles > 0)
-        hipLaunchKernelGGL(k_bin_ranges, dim3(1), dim3(1024), 0, st, (const uint32_t*)tile_cnt, tiles_x, tiles_y,
-                           ranges);
+    if (ntiles > 0)  // ranges (column-major list order) and the composite's workgroup order
+        hipLaunchKernelGGL(k_tile_order<true>, dim3(1), dim3(1024), 0, st, (const uint32_t*)tile_cnt, tiles_x,
+                           tiles_y, tile_order, ranges);
 }
 
 void launch_keys_to_ids(hipStream_t st, const uint64_t* keys, uint64_t m, const uint64_t* m_dev, uint32_t* ids) {
@@ -1713,7 +1749,7 @@ uint32_t ext_order_slots(uint32_t tiles_x, uint32_t tiles_y) {
 
 void launch_tile_order(hipStream_t st, const uint32_t* ranges, uint32_t tiles_x, uint32_t tiles_y, uint32_t* order) {
     if (tiles_x == 0 || tiles_y == 0) return;
-    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, st, ranges, tiles_x, tiles_y, order);
+    hipLaunchKernelGGL(k_tile_order<false>, dim3(1), dim3(1024), 0, st, ranges, tiles_x, tiles_y, order, nullptr);
 }
 
 template <int TILE, int BATCH>

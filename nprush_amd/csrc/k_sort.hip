@@ -1,15 +1,16 @@
 // k_sort.hip -- device prefix sums and an LSD radix sort for 64-bit keys
 // (optional 32-bit values), written for 64-wide wavefronts.
 //
-// Scan: reduce-then-scan over blocks of 2048 items (256 threads x 8); the
-// block partials are scanned by one workgroup that walks them in order.
+// Scan: reduce-then-scan over blocks of 2048 items (256 threads x 8).  Up to
+// 1024 blocks, each block of the second kernel sums the partials before it
+// (two launches); above that one workgroup scans the partials in between.
 //
 // Radix pass (8-bit or narrower digit at `shift`):
-//   k_radix_hist    : 4096 keys per workgroup (striped: item r of thread t is
-//                     position r*256+t), LDS histogram, digit-major output
+//   k_radix_hist    : 256 KI keys per workgroup (KI = 16; striped: item r of
+//                     thread t is position r*256+t), LDS histogram, digit-major output
 //                     hist[d * nblk + blk] so one exclusive scan yields every
 //                     (digit, block) output offset;
-//   k_radix_scatter : each wave ranks its own contiguous 1024 keys in 16 rounds
+//   k_radix_scatter : each wave ranks its own contiguous 64 KI keys in KI rounds
 //                     of 64; keys with equal digits are found with one
 //                     ballot per digit bit (match mask), their order inside a
 //                     round is popcount(mask & lanes_below), and a per-wave LDS
@@ -131,6 +132,41 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const TI* __restric
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanThreads - 1) out[n] = run;
 }
 
+// As k_scan_apply, for at most kScanDirectBlocks blocks: each block sums the
+// partials of the blocks before it itself (at most kScanDirectBlocks - 1
+// loads spread over its threads), so no separate scan of the partials runs.
+constexpr uint64_t kScanDirectBlocks = 1024;
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply_direct(const TI* __restrict__ in, uint64_t n,
+                                                                    const TO* __restrict__ partials,
+                                                                    TO* __restrict__ out) {
+    __shared__ TO lds4[4];
+    TO pre = 0;
+    for (uint32_t j = threadIdx.x; j < blockIdx.x; j += kScanThreads) pre += partials[j];
+    TO before;
+    (void)block_exclusive_scan<TO>(pre, lds4, &before);
+    uint64_t base = (uint64_t)blockIdx.x * kScanBlock;
+    TO v[kScanItems];
+    TO s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        uint64_t idx = base + (uint64_t)threadIdx.x * kScanItems + k;
+        v[k] = idx < n ? (TO)in[idx] : (TO)0;
+        s += v[k];
+    }
+    TO total;
+    TO ex = block_exclusive_scan<TO>(s, lds4, &total);
+    TO run = before + ex;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        uint64_t idx = base + (uint64_t)threadIdx.x * kScanItems + k;
+        if (idx < n) out[idx] = run;
+        run += v[k];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanThreads - 1) out[n] = run;
+}
+
 size_t scan_temp_bytes(uint64_t n) { return sizeof(uint64_t) * ((n + kScanBlock - 1) / kScanBlock + 1); }
 
 template <typename TI, typename TO>
@@ -139,6 +175,11 @@ static void exclusive_scan(hipStream_t st, const TI* in, TO* out, uint64_t n, vo
     if (nb == 0) nb = 1;
     TO* partials = (TO*)temp;
     hipLaunchKernelGGL((k_scan_reduce<TI, TO>), dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, n, partials);
+    if (nb <= kScanDirectBlocks) {
+        hipLaunchKernelGGL((k_scan_apply_direct<TI, TO>), dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, n,
+                           (const TO*)partials, out);
+        return;
+    }
     hipLaunchKernelGGL((k_scan_partials<TO>), dim3(1), dim3(kScanThreads), 0, st, partials, nb);
     hipLaunchKernelGGL((k_scan_apply<TI, TO>), dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, n,
                        (const TO*)partials, out);
@@ -153,8 +194,9 @@ void exclusive_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint6
 
 // m_dev (optional): the key count lives in device memory (capacity m); the
 // grid covers the capacity and blocks past the count contribute nothing.
-// RADIX: 256 (8-bit digits) or 2048 (11-bit digits).
-template <int RADIX>
+// RADIX: 256 (8-bit digits) or 2048 (11-bit digits).  KI: keys per thread;
+// a workgroup takes 256 KI keys, each of its 4 waves ranks 64 KI of them.
+template <int RADIX, int KI>
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t* __restrict__ keys, uint64_t m,
                                                              const uint64_t* __restrict__ m_dev, int shift,
                                                              uint32_t mask, uint32_t nblk,
@@ -163,9 +205,9 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t* __r
     if (m_dev) m = min(*m_dev, m);
     for (int d = threadIdx.x; d < RADIX; d += kSortThreads) h[d] = 0u;
     __syncthreads();
-    uint64_t base = (uint64_t)blockIdx.x * kSortBlock;
+    uint64_t base = (uint64_t)blockIdx.x * (kSortThreads * KI);
 #pragma unroll
-    for (int r = 0; r < kSortItems; ++r) {
+    for (int r = 0; r < KI; ++r) {
         uint64_t pos = base + (uint64_t)r * kSortThreads + threadIdx.x;
         if (pos < m) {
             uint32_t d = (uint32_t)(keys[pos] >> shift) & mask;
@@ -176,7 +218,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t* __r
     for (int d = threadIdx.x; d < RADIX; d += kSortThreads) hist[(uint64_t)d * nblk + blockIdx.x] = h[d];
 }
 
-template <bool kHasVals, int RADIX>
+template <bool kHasVals, int RADIX, int KI>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t* __restrict__ keys_in,
                                                                 const uint32_t* __restrict__ vals_in, uint64_t m,
                                                                 const uint64_t* __restrict__ m_dev,
@@ -190,16 +232,21 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t* 
     for (int k = threadIdx.x; k < 4 * RADIX; k += kSortThreads) (&wcnt[0][0])[k] = 0u;
     __syncthreads();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    uint64_t base = (uint64_t)blockIdx.x * kSortBlock + (uint64_t)wid * (kSortBlock / 4);
-    uint64_t key[kSortItems];
-    uint32_t val[kSortItems];
-    uint32_t rank[kSortItems];
+    uint64_t base = (uint64_t)blockIdx.x * (kSortThreads * KI) + (uint64_t)wid * (64 * KI);
+    uint64_t key[KI];
+    uint32_t val[KI];
+    uint32_t rank[KI];
 #pragma unroll
-    for (int r = 0; r < kSortItems; ++r) {
+    for (int r = 0; r < KI; ++r) {
         uint64_t pos = base + (uint64_t)r * 64 + lane;
         bool valid = pos < m;
         key[r] = valid ? keys_in[pos] : 0ull;
         if (kHasVals) val[r] = valid ? vals_in[pos] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < KI; ++r) {
+        uint64_t pos = base + (uint64_t)r * 64 + lane;
+        bool valid = pos < m;
         uint32_t d = (uint32_t)(key[r] >> shift) & mask;
         uint64_t peers = __ballot(valid);
         constexpr int kBits = RADIX == 2048 ? 11 : 8;
@@ -226,7 +273,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t* 
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < kSortItems; ++r) {
+    for (int r = 0; r < KI; ++r) {
         uint64_t pos = base + (uint64_t)r * 64 + lane;
         if (pos < m) {
             uint32_t d = (uint32_t)(key[r] >> shift) & mask;
@@ -238,22 +285,29 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t* 
 }
 
 constexpr int kRadixWide = 2048;  // 11-bit digits (radix_sort_u64 with wide = true)
+// Keys per thread of the wide passes.  Smaller workgroups (4: 1024 keys, 4x
+// the waves) measured slower at 1M keys: the 2048-digit histogram written
+// and scanned per workgroup then outweighs the occupancy gained.
+constexpr int kWideItems = 16;
 
-size_t radix_temp_bytes(uint64_t m) {
-    uint64_t nblk = (m + kSortBlock - 1) / kSortBlock;
+size_t radix_temp_bytes(uint64_t m, bool wide) {
+    const uint64_t block = wide ? kSortThreads * kWideItems : kSortBlock;
+    uint64_t nblk = (m + block - 1) / block;
     if (nblk == 0) nblk = 1;
-    uint64_t hist = (uint64_t)kRadixWide * nblk;
+    uint64_t hist = (uint64_t)(wide ? kRadixWide : kRadix) * nblk;
     return sizeof(uint32_t) * 2 * (hist + 1) + scan_temp_bytes(hist) + 256;
 }
 
 // Sorts keys[0..m) (and vals when non-null) by bits [bit_lo, bit_hi).  Returns
 // true when the result is in the *_alt buffers.  With m_dev, the count is
 // min(*m_dev, m) read on the device (m is then the capacity): no host sync,
-// and the launches are the same for every count (graph capture).
+// and the launches are the same for every count (graph capture).  wide:
+// 11-bit digits, else 8-bit digits.
 bool radix_sort_u64(hipStream_t st, uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt,
                     uint64_t m, int bit_lo, int bit_hi, void* temp, const uint64_t* m_dev, bool wide) {
     if (m == 0 || bit_hi <= bit_lo) return false;
-    uint32_t nblk = (uint32_t)((m + kSortBlock - 1) / kSortBlock);
+    const uint64_t block = wide ? kSortThreads * kWideItems : kSortBlock;
+    uint32_t nblk = (uint32_t)((m + block - 1) / block);
     const int max_bits = wide ? 11 : 8;
     const int radix = wide ? kRadixWide : kRadix;
     uint64_t hist_n = (uint64_t)radix * nblk;
@@ -274,21 +328,21 @@ bool radix_sort_u64(hipStream_t st, uint64_t* keys, uint64_t* keys_alt, uint32_t
         uint32_t mask = (1u << bits) - 1u;
         const uint64_t hn = (uint64_t)radix * nblk;
         if (wide)
-            hipLaunchKernelGGL(k_radix_hist<kRadixWide>, dim3(nblk), dim3(kSortThreads), 0, st, (const uint64_t*)kin,
-                               m, m_dev, shift, mask, nblk, hist);
+            hipLaunchKernelGGL((k_radix_hist<kRadixWide, kWideItems>), dim3(nblk), dim3(kSortThreads), 0, st,
+                               (const uint64_t*)kin, m, m_dev, shift, mask, nblk, hist);
         else
-            hipLaunchKernelGGL(k_radix_hist<kRadix>, dim3(nblk), dim3(kSortThreads), 0, st, (const uint64_t*)kin, m,
-                               m_dev, shift, mask, nblk, hist);
+            hipLaunchKernelGGL((k_radix_hist<kRadix, kSortItems>), dim3(nblk), dim3(kSortThreads), 0, st,
+                               (const uint64_t*)kin, m, m_dev, shift, mask, nblk, hist);
         exclusive_scan_u32(st, hist, offs, hn, scan_tmp);
-#define NPS_SCATTER(V, RX)                                                                                          \
-    hipLaunchKernelGGL((k_radix_scatter<V, RX>), dim3(nblk), dim3(kSortThreads), 0, st, (const uint64_t*)kin,         \
+#define NPS_SCATTER(V, RX, KI)                                                                                      \
+    hipLaunchKernelGGL((k_radix_scatter<V, RX, KI>), dim3(nblk), dim3(kSortThreads), 0, st, (const uint64_t*)kin,     \
                        (const uint32_t*)vin, m, m_dev, shift, mask, bits, nblk, (const uint32_t*)offs, kout, vout)
         if (vals) {
-            if (wide) NPS_SCATTER(true, kRadixWide);
-            else NPS_SCATTER(true, kRadix);
+            if (wide) NPS_SCATTER(true, kRadixWide, kWideItems);
+            else NPS_SCATTER(true, kRadix, kSortItems);
         } else {
-            if (wide) NPS_SCATTER(false, kRadixWide);
-            else NPS_SCATTER(false, kRadix);
+            if (wide) NPS_SCATTER(false, kRadixWide, kWideItems);
+            else NPS_SCATTER(false, kRadix, kSortItems);
         }
 #undef NPS_SCATTER
         uint64_t* tk = kin; kin = kout; kout = tk;

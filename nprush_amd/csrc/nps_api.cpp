@@ -34,7 +34,8 @@ struct ExtView {
     DevBuf rec, counts, rects, dkey, dkey_alt, rank_counts, offsets, scan_tmp, keys, keys_alt, sort_tmp, ranges;
     DevBuf order, tile_work, ckpt, act, g2, c_img, tile_loss, stats, flags;  // flags: [0] key overflow
     DevBuf part;  // per-tile counter slots of the composite
-    DevBuf bin_cnt, bin_off, bin_tile, bin_scan;  // binning: digit counts, their offsets, per-tile counts
+    DevBuf bin_rcnt, bin_roff, bin_rscan;         // binning, row pass: counts, offsets, scan temp
+    DevBuf bin_cnt, bin_off, bin_tile, bin_scan;  // binning, column pass: counts, offsets, per-tile counts, temp
     uint32_t* ids = nullptr;  // splat ids in (tile, depth, id) order: keys or keys_alt read as 32-bit words
     bool col_major = false;   // ids stored tile column by tile column (ext_bin)
     uint64_t m = 0;    // key count of the last eager run
@@ -146,7 +147,7 @@ void free_ext_view(ExtView& x) {
     DevBuf* all[] = {&x.rec, &x.counts, &x.rects, &x.dkey, &x.dkey_alt, &x.rank_counts, &x.offsets, &x.scan_tmp,
                      &x.keys, &x.keys_alt, &x.sort_tmp, &x.ranges, &x.order, &x.tile_work, &x.ckpt, &x.act,
                      &x.g2, &x.c_img, &x.tile_loss, &x.stats, &x.flags, &x.part, &x.bin_cnt, &x.bin_off,
-                     &x.bin_tile, &x.bin_scan};
+                     &x.bin_tile, &x.bin_scan, &x.bin_rcnt, &x.bin_roff, &x.bin_rscan};
     for (DevBuf* b : all) free_buf(*b);
     if (x.ev_count) {
         (void)hipEventDestroy(x.ev_count);
@@ -194,12 +195,14 @@ int ext_view_reserve(ExtView& x, uint32_t n, uint32_t W, uint32_t H, uint64_t ca
         {&x.rec, sizeof(ExtRec) * n1}, {&x.counts, 4 * n1}, {&x.rects, 16 * n1}, {&x.dkey, 8 * n1},
         {&x.dkey_alt, 8 * n1}, {&x.rank_counts, 4 * n1}, {&x.offsets, 8 * (n1 + 1)},
         {&x.scan_tmp, scan_temp_bytes(n1) + 64}, {&x.stats, 64}, {&x.flags, 64},
-        {&x.sort_tmp, radix_temp_bytes(std::max<uint64_t>(c1, n1))},
+        {&x.sort_tmp, std::max(radix_temp_bytes(c1), radix_temp_bytes(n1, true))},
         {&x.keys, 8 * c1}, {&x.keys_alt, 8 * c1},
         {&x.ranges, 8 * (size_t)nt}, {&x.order, 4 * (size_t)ext_order_slots(tx, ty)}, {&x.tile_work, 8 * (size_t)nt},
         {&x.c_img, 12 * (size_t)W * H}, {&x.tile_loss, 16 * (size_t)nt}, {&x.part, 32 * (size_t)nt},
-        {&x.bin_cnt, 4 * ext_bin_scan_words(n, tx, ty, c1)}, {&x.bin_off, 4 * ext_bin_scan_words(n, tx, ty, c1)},
-        {&x.bin_tile, 4 * (size_t)nt}, {&x.bin_scan, scan_temp_bytes(ext_bin_scan_words(n, tx, ty, c1)) + 64},
+        {&x.bin_rcnt, 4 * ext_bin_row_words(n, ty)}, {&x.bin_roff, 4 * ext_bin_row_words(n, ty)},
+        {&x.bin_rscan, scan_temp_bytes(ext_bin_row_words(n, ty)) + 64},
+        {&x.bin_cnt, 4 * ext_bin_col_words(tx, c1)}, {&x.bin_off, 4 * ext_bin_col_words(tx, c1)},
+        {&x.bin_tile, 4 * (size_t)nt}, {&x.bin_scan, scan_temp_bytes(ext_bin_col_words(tx, c1)) + 64},
     };
     int rc;
     for (auto& q : need)
@@ -248,6 +251,13 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
     const uint64_t* order = (const uint64_t*)(dalt ? x.dkey_alt.p : x.dkey.p);
     launch_ext_rank_counts(st, n, order, (const uint32_t*)counts, (uint32_t*)x.rank_counts.p);
     exclusive_scan_u32_to_u64(st, (const uint32_t*)x.rank_counts.p, (uint64_t*)x.offsets.p, n, x.scan_tmp.p);
+    // the binning's row pass needs no key count: it runs while the host waits for it
+    const bool rows_early = ext_bin_fits(n, v.tiles_x, v.tiles_y, 0);
+    auto bin_rows = [&]() {
+        if (rows_early)
+            ext_bin_rows(st, n, order, (const uint32_t*)x.rank_counts.p, (const uint4*)x.rects.p, v.tiles_y,
+                         (uint32_t*)x.bin_rcnt.p, (uint32_t*)x.bin_roff.p, x.bin_rscan.p);
+    };
     uint64_t cap;
     const uint64_t* m_dev = nullptr;
     if (!capture) {
@@ -261,6 +271,7 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
                                      hipMemcpyDeviceToHost, st), "count readback")) != NPS_OK)
             return rc;
         if ((rc = hip(hipEventRecord(x.ev_count, st), "event")) != NPS_OK) return rc;
+        bin_rows();
         if (grads) launch_fill_u32(st, (uint32_t*)g2, 0u, 18 * (uint64_t)n);
         if ((rc = hip(hipEventSynchronize(x.ev_count), "sync")) != NPS_OK) return rc;
         const uint64_t m = *x.count_host;
@@ -274,17 +285,19 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
     } else {
         cap = x.cap;
         m_dev = (const uint64_t*)x.offsets.p + n;
+        bin_rows();
     }
     x.num_tiles = v.num_tiles;
-    x.col_major = ext_bin_fits(n, v.tiles_x, v.tiles_y, cap);
+    x.col_major = rows_early && ext_bin_fits(n, v.tiles_x, v.tiles_y, cap);
     if (x.col_major) {
         // keys bucketed by tile row as they are enumerated, then one stable
         // pass on the tile column: ids and ranges
         x.ids = (uint32_t*)x.keys_alt.p;
         if (ev && (rc = hip(hipEventRecord(ev[1], st), "event")) != NPS_OK) return rc;
         ext_bin(st, n, order, (const uint32_t*)x.rank_counts.p, (const uint4*)x.rects.p, v.tiles_x, v.tiles_y, cap,
-                m_dev, (uint32_t*)x.keys.p, x.ids, (uint32_t*)x.ranges.p, (uint32_t*)x.bin_cnt.p,
-                (uint32_t*)x.bin_off.p, (uint32_t*)x.bin_tile.p, x.bin_scan.p, flags);
+                m_dev, (const uint32_t*)x.bin_roff.p, (uint32_t*)x.keys.p, x.ids, (uint32_t*)x.ranges.p,
+                (uint32_t*)x.order.p, (uint32_t*)x.bin_cnt.p, (uint32_t*)x.bin_off.p, (uint32_t*)x.bin_tile.p,
+                x.bin_scan.p, flags);
         if (ev && (rc = hip(hipEventRecord(ev[2], st), "event")) != NPS_OK) return rc;
     } else {
         // 8-byte keys tile << 32 | id in rank order, stable sort on the tile bits
@@ -302,7 +315,7 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
         launch_keys_to_ids(st, sorted, cap, m_dev, x.ids);
     }
     if (capture && grads) launch_fill_u32(st, (uint32_t*)g2, 0u, 18 * (uint64_t)n);
-    launch_tile_order(st, (const uint32_t*)x.ranges.p, v.tiles_x, v.tiles_y, (uint32_t*)x.order.p);
+    if (!x.col_major) launch_tile_order(st, (const uint32_t*)x.ranges.p, v.tiles_x, v.tiles_y, (uint32_t*)x.order.p);
     if (ev && (rc = hip(hipEventRecord(ev[3], st), "event")) != NPS_OK) return rc;
     ExtCompositeArgs a;
     a.rec = (const ExtRec*)x.rec.p;

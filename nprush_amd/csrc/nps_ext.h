@@ -80,7 +80,7 @@ __device__ __forceinline__ float ext_pair_alpha(float pxf, float pyf, const floa
     const float spy = fmaf(p0.w, dx, p1.x * dy);
     const float r2 = fmaf(dx, spx, dy * spy);
     // non-short-circuit &: every operand is evaluated, no branch
-    const bool ok = use & (fabsf(dx) <= p2.z) & (fabsf(dy) <= p2.w) & !(r2 > p2.y);
+    const bool ok = (int)use & (int)(fabsf(dx) <= p2.z) & (int)(fabsf(dy) <= p2.w) & (int)!(r2 > p2.y);
     const float G = exp2_det_in_range(ok ? r2 * kExtNegHalfLog2e : 0.0f);  // r2 <= rcut2 < 12
     float a = p1.y * G;
     a = a > kExtMaxAlpha ? kExtMaxAlpha : a;
@@ -128,35 +128,40 @@ NPS_HD void sh_basis(float x, float y, float z, int deg, float* Y) {
     Y[15] = kShC3_6 * x * (xx - 3.0f * yy);
 }
 
-// dY_k / d(x, y, z) for k < (deg+1)^2 (entries above stay untouched; callers zero them).
-NPS_HD void sh_basis_grad(float x, float y, float z, int deg, float* gx, float* gy, float* gz) {
-    for (int k = 0; k < 16; ++k) gx[k] = gy[k] = gz[k] = 0.0f;
+// g += sum_k w[k] dY_k/d(x, y, z), k < (deg+1)^2: the derivatives of the
+// sh_basis polynomials (each evaluated in FP32), the terms that are not
+// identically zero, accumulated in FP64.
+NPS_HD void sh_grad_weighted(float x, float y, float z, int deg, const double* w, double* g) {
     if (deg < 1) return;
-    gy[1] = -kShC1;
-    gz[2] = kShC1;
-    gx[3] = -kShC1;
+    g[1] += w[1] * (double)(-kShC1);
+    g[2] += w[2] * (double)kShC1;
+    g[0] += w[3] * (double)(-kShC1);
     if (deg < 2) return;
-    gx[4] = kShC2_0 * y; gy[4] = kShC2_0 * x;
-    gy[5] = kShC2_1 * z; gz[5] = kShC2_1 * y;
-    gx[6] = kShC2_2 * (-2.0f * x); gy[6] = kShC2_2 * (-2.0f * y); gz[6] = kShC2_2 * (4.0f * z);
-    gx[7] = kShC2_3 * z; gz[7] = kShC2_3 * x;
-    gx[8] = kShC2_4 * (2.0f * x); gy[8] = kShC2_4 * (-2.0f * y);
+    g[0] += w[4] * (double)(kShC2_0 * y); g[1] += w[4] * (double)(kShC2_0 * x);
+    g[1] += w[5] * (double)(kShC2_1 * z); g[2] += w[5] * (double)(kShC2_1 * y);
+    g[0] += w[6] * (double)(kShC2_2 * (-2.0f * x)); g[1] += w[6] * (double)(kShC2_2 * (-2.0f * y));
+    g[2] += w[6] * (double)(kShC2_2 * (4.0f * z));
+    g[0] += w[7] * (double)(kShC2_3 * z); g[2] += w[7] * (double)(kShC2_3 * x);
+    g[0] += w[8] * (double)(kShC2_4 * (2.0f * x)); g[1] += w[8] * (double)(kShC2_4 * (-2.0f * y));
     if (deg < 3) return;
-    float xx = x * x, yy = y * y, zz = z * z;
-    gx[9] = kShC3_0 * (6.0f * x * y); gy[9] = kShC3_0 * (3.0f * xx - 3.0f * yy);
-    gx[10] = kShC3_1 * (y * z); gy[10] = kShC3_1 * (x * z); gz[10] = kShC3_1 * (x * y);
-    gx[11] = kShC3_2 * (-2.0f * x * y); gy[11] = kShC3_2 * (4.0f * zz - xx - 3.0f * yy);
-    gz[11] = kShC3_2 * (8.0f * y * z);
-    gx[12] = kShC3_3 * (-6.0f * x * z); gy[12] = kShC3_3 * (-6.0f * y * z);
-    gz[12] = kShC3_3 * (6.0f * zz - 3.0f * xx - 3.0f * yy);
-    gx[13] = kShC3_4 * (4.0f * zz - 3.0f * xx - yy); gy[13] = kShC3_4 * (-2.0f * x * y);
-    gz[13] = kShC3_4 * (8.0f * x * z);
-    gx[14] = kShC3_5 * (2.0f * x * z); gy[14] = kShC3_5 * (-2.0f * y * z); gz[14] = kShC3_5 * (xx - yy);
-    gx[15] = kShC3_6 * (3.0f * xx - 3.0f * yy); gy[15] = kShC3_6 * (-6.0f * x * y);
+    const float xx = x * x, yy = y * y, zz = z * z;
+    g[0] += w[9] * (double)(kShC3_0 * (6.0f * x * y)); g[1] += w[9] * (double)(kShC3_0 * (3.0f * xx - 3.0f * yy));
+    g[0] += w[10] * (double)(kShC3_1 * (y * z)); g[1] += w[10] * (double)(kShC3_1 * (x * z));
+    g[2] += w[10] * (double)(kShC3_1 * (x * y));
+    g[0] += w[11] * (double)(kShC3_2 * (-2.0f * x * y)); g[1] += w[11] * (double)(kShC3_2 * (4.0f * zz - xx - 3.0f * yy));
+    g[2] += w[11] * (double)(kShC3_2 * (8.0f * y * z));
+    g[0] += w[12] * (double)(kShC3_3 * (-6.0f * x * z)); g[1] += w[12] * (double)(kShC3_3 * (-6.0f * y * z));
+    g[2] += w[12] * (double)(kShC3_3 * (6.0f * zz - 3.0f * xx - 3.0f * yy));
+    g[0] += w[13] * (double)(kShC3_4 * (4.0f * zz - 3.0f * xx - yy)); g[1] += w[13] * (double)(kShC3_4 * (-2.0f * x * y));
+    g[2] += w[13] * (double)(kShC3_4 * (8.0f * x * z));
+    g[0] += w[14] * (double)(kShC3_5 * (2.0f * x * z)); g[1] += w[14] * (double)(kShC3_5 * (-2.0f * y * z));
+    g[2] += w[14] * (double)(kShC3_5 * (xx - yy));
+    g[0] += w[15] * (double)(kShC3_6 * (3.0f * xx - 3.0f * yy)); g[1] += w[15] * (double)(kShC3_6 * (-6.0f * x * y));
 }
 
 // Projection of splat s (16 floats) with SH block sh (48 floats).  cam is the
-// camera centre.  On return o.valid says whether the splat is drawn.
+// camera centre.  On return o.valid says whether the splat is drawn.  With
+// sh == nullptr the colour (sh_raw, rgb) is left unset.
 NPS_HD void ext_project(const float* s, const float* sh, int deg, const float* vp, const float* cam, uint32_t W,
                         uint32_t H, ExtProj& o) {
     o.valid = false;
@@ -210,6 +215,8 @@ NPS_HD void ext_project(const float* s, const float* sh, int deg, const float* v
     o.dir[0] = v0 / o.dlen;
     o.dir[1] = v1 / o.dlen;
     o.dir[2] = v2 / o.dlen;
+    o.valid = true;
+    if (!sh) return;
     float Y[16];
     sh_basis(o.dir[0], o.dir[1], o.dir[2], deg, Y);
     const int nk = (deg + 1) * (deg + 1);
@@ -219,7 +226,6 @@ NPS_HD void ext_project(const float* s, const float* sh, int deg, const float* v
         o.sh_raw[c] = acc + 0.5f;
         o.rgb[c] = o.sh_raw[c] > 0.0f ? o.sh_raw[c] : 0.0f;
     }
-    o.valid = true;
 }
 
 // r^2 of a pair, d = p - centre, sp = conic * d.

@@ -59,7 +59,7 @@ void launch_sum_parts(hipStream_t st, const unsigned long long* part, uint32_t n
 size_t scan_temp_bytes(uint64_t n);
 void exclusive_scan_u32_to_u64(hipStream_t st, const uint32_t* in, uint64_t* out, uint64_t n, void* temp);
 void exclusive_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t n, void* temp);
-size_t radix_temp_bytes(uint64_t m);
+size_t radix_temp_bytes(uint64_t m, bool wide = false);
 // wide: digits of up to 11 bits (2048 buckets, fewer passes) instead of 8.
 bool radix_sort_u64(hipStream_t st, uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt,
                     uint64_t m, int bit_lo, int bit_hi, void* temp, const uint64_t* m_dev = nullptr,
@@ -130,13 +130,20 @@ struct ExtCompositeArgs {
 // (depth, id) order (tiles column by column), and the tile ranges.
 // ext_bin_fits: at most 256 tile rows and columns, the 4-byte items hold the
 // column and the id, and the counts fit 32 bits (else the caller sorts keys).
-// cnt / off: ext_bin_scan_words(...) words each; scan_tmp: scan_temp_bytes of that.
+// ext_bin_rows (row counts and their offsets) needs no key count, so it can
+// run before the count is known; ext_bin then does the rest, ending with the
+// ranges and the composite's workgroup order (launch_tile_order's output).
+// rcnt / roff: ext_bin_row_words(...) words each, rscan: scan_temp_bytes of
+// that; ccnt / coff: ext_bin_col_words(...) words each, cscan likewise.
 bool ext_bin_fits(uint32_t n, uint32_t tiles_x, uint32_t tiles_y, uint64_t cap);
-size_t ext_bin_scan_words(uint32_t n, uint32_t tiles_x, uint32_t tiles_y, uint64_t cap);
+size_t ext_bin_row_words(uint32_t n, uint32_t tiles_y);
+size_t ext_bin_col_words(uint32_t tiles_x, uint64_t cap);
+void ext_bin_rows(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts, const uint4* rects,
+                  uint32_t tiles_y, uint32_t* rcnt, uint32_t* roff, void* rscan);
 void ext_bin(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts, const uint4* rects,
-             uint32_t tiles_x, uint32_t tiles_y, uint64_t cap, const uint64_t* m_dev, uint32_t* items, uint32_t* ids,
-             uint32_t* ranges, uint32_t* cnt, uint32_t* off, uint32_t* tile_cnt, void* scan_tmp,
-             uint32_t* overflow);
+             uint32_t tiles_x, uint32_t tiles_y, uint64_t cap, const uint64_t* m_dev, const uint32_t* roff,
+             uint32_t* items, uint32_t* ids, uint32_t* ranges, uint32_t* tile_order, uint32_t* ccnt, uint32_t* coff,
+             uint32_t* tile_cnt, void* cscan, uint32_t* overflow);
 void launch_keys_to_ids(hipStream_t st, const uint64_t* keys, uint64_t m, const uint64_t* m_dev, uint32_t* ids);
 // Entries of the composite's workgroup order (= its grid size).
 uint32_t ext_order_slots(uint32_t tiles_x, uint32_t tiles_y);
