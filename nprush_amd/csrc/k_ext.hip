@@ -341,13 +341,13 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_row_emit(uint32_t n, const 
 // Per 4096 items: counts2[x * nblk + blk] (tile column x) and tile_cnt[tile] +=.
 // The row buckets start at offs1[y * nblk1] (y < ny; the item count after the
 // last); a workgroup's items lie in the rows y_first..y_last of its first and
-// last item.  Per round of 64 items a wave groups the lanes with equal
-// (row, column) (ballots); the first lane of each group adds its size.
+// last item.  Counts need no order: every item adds 1 to its column's and its
+// tile's LDS counter (LDS atomics; lanes that hit one address serialise).
 __global__ __launch_bounds__(256) void k_bin_col_count(const uint32_t* __restrict__ items, uint64_t m,
                                                        const uint64_t* __restrict__ m_dev,
                                                        const uint32_t* __restrict__ offs1, uint32_t nblk1,
-                                                       uint32_t ny, uint32_t nx, int row_bits, int col_bits,
-                                                       int id_bits, uint32_t* __restrict__ counts2,
+                                                       uint32_t ny, uint32_t nx, int id_bits,
+                                                       uint32_t* __restrict__ counts2,
                                                        uint32_t* __restrict__ tile_cnt) {
     constexpr int kLocal = 2048;  // per-workgroup tile counters (rows in the workgroup x nx)
     __shared__ uint32_t s_start[(1 << kBinMaxBits) + 1];
@@ -375,7 +375,6 @@ __global__ __launch_bounds__(256) void k_bin_col_count(const uint32_t* __restric
     if (local)
         for (uint32_t k = tid; k < (y_last - y_first + 1u) * nx; k += 256) s_t[k] = 0u;
     __syncthreads();
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint32_t it[kBinColItems];
 #pragma unroll
     for (int r = 0; r < kBinColItems; ++r) {
@@ -385,26 +384,12 @@ __global__ __launch_bounds__(256) void k_bin_col_count(const uint32_t* __restric
 #pragma unroll
     for (int r = 0; r < kBinColItems; ++r) {
         const uint64_t pos = base + (uint64_t)r * 256 + tid;
-        const bool valid = pos < m;
-        const uint64_t live = __ballot(valid);
-        const uint32_t x = it[r] >> id_bits;
-        const uint32_t y = (y_first == y_last) ? y_first : row_of((uint32_t)pos);
-        uint64_t peers = live;
-        if (y_first != y_last)  // a row boundary inside the workgroup: group by row too
-            for (int q = 0; q < row_bits; ++q) {
-                const uint64_t bal = __ballot((y >> q) & 1u);
-                peers &= ((y >> q) & 1u) ? bal : ~bal;
-            }
-#pragma unroll
-        for (int q = 0; q < kBinMaxBits; ++q) {  // bits above col_bits are 0 in every lane: no-ops
-            const uint64_t bal = __ballot((x >> q) & 1u);
-            peers &= ((x >> q) & 1u) ? bal : ~bal;
-        }
-        if (valid && (peers & lt_mask) == 0ull) {
-            const uint32_t c = (uint32_t)__popcll(peers);
-            atomicAdd(&s_c[x], c);
-            if (local) atomicAdd(&s_t[(y - y_first) * nx + x], c);
-            else atomicAdd(&tile_cnt[y * nx + x], c);
+        if (pos < m) {
+            const uint32_t x = it[r] >> id_bits;
+            const uint32_t y = (y_first == y_last) ? y_first : row_of((uint32_t)pos);
+            atomicAdd(&s_c[x], 1u);
+            if (local) atomicAdd(&s_t[(y - y_first) * nx + x], 1u);
+            else atomicAdd(&tile_cnt[y * nx + x], 1u);
         }
     }
     __syncthreads();
@@ -568,6 +553,7 @@ __device__ __forceinline__ float wave_reduce8(float (&v)[8], int lane) {
 // kOrderMax super-tiles the ranking is skipped.
 constexpr uint32_t kOrderMax = 8192;
 constexpr uint32_t kOrderLevels = 1024;
+constexpr uint32_t kOrderTiles = 8192;  // tile lengths kept in LDS up to this many tiles
 
 // kFromCounts: src = per-tile list lengths, and the kernel first writes the
 // tile ranges with the tiles taken column by column (the binning's output
@@ -581,26 +567,39 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict_
     __shared__ uint32_t s_cnt[kOrderLevels];
     __shared__ uint32_t s_wsum[16];
     __shared__ uint32_t s_max;
+    __shared__ uint32_t s_len[kFromCounts ? kOrderTiles : 1];  // tile lengths (up to kOrderTiles tiles)
     const uint32_t sx = (tiles_x + 1u) / 2u, sy = (tiles_y + 1u) / 2u, S = sx * sy;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
+    const uint32_t ntiles = tiles_x * tiles_y;
+    const bool in_lds = kFromCounts && ntiles <= kOrderTiles;
     if (kFromCounts) {
         // list position c = tx * tiles_y + ty; thread tid owns positions [per tid, per tid + per)
-        const uint32_t ntiles = tiles_x * tiles_y, per = (ntiles + 1023u) / 1024u;
+        const uint32_t per = (ntiles + 1023u) / 1024u;
         const uint32_t c0 = tid * per;
+        const uint32_t tx0 = c0 / tiles_y, ty0 = c0 - tx0 * tiles_y;
         constexpr uint32_t kFew = 8;  // up to 8192 tiles: all loads issued at once
         uint32_t few[kFew];
         uint32_t own = 0u;
         if (per <= kFew) {
+            uint32_t tx = tx0, ty = ty0;
 #pragma unroll
             for (uint32_t k = 0; k < kFew; ++k) {
                 const uint32_t c = c0 + k;
-                few[k] = (k < per && c < ntiles) ? src[(c % tiles_y) * tiles_x + c / tiles_y] : 0u;
+                few[k] = (k < per && c < ntiles) ? src[ty * tiles_x + tx] : 0u;
                 own += few[k];
+                if (++ty == tiles_y) {
+                    ty = 0u;
+                    ++tx;
+                }
             }
         } else {
+            uint32_t tx = tx0, ty = ty0;
             for (uint32_t k = 0; k < per; ++k) {
-                const uint32_t c = c0 + k;
-                if (c < ntiles) own += src[(c % tiles_y) * tiles_x + c / tiles_y];
+                if (c0 + k < ntiles) own += src[ty * tiles_x + tx];
+                if (++ty == tiles_y) {
+                    ty = 0u;
+                    ++tx;
+                }
             }
         }
         uint32_t x = own;
@@ -613,20 +612,26 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict_
         __syncthreads();
         uint32_t run = x - own;
         for (uint32_t w = 0; w < wid; ++w) run += s_wsum[w];
+        uint32_t tx = tx0, ty = ty0;
         for (uint32_t k = 0; k < per; ++k) {
-            const uint32_t c = c0 + k;
-            if (c < ntiles) {
-                const uint32_t t = (c % tiles_y) * tiles_x + c / tiles_y;
+            if (c0 + k < ntiles) {
+                const uint32_t t = ty * tiles_x + tx;
                 const uint32_t len = per <= kFew ? few[k < kFew ? k : 0] : src[t];
                 ranges_out[2 * t] = run;
                 ranges_out[2 * t + 1] = run + len;
+                if (in_lds) s_len[t] = len;
                 run += len;
+            }
+            if (++ty == tiles_y) {
+                ty = 0u;
+                ++tx;
             }
         }
         __syncthreads();
     }
     auto tile_len = [&](uint32_t t) -> uint32_t {
-        return kFromCounts ? src[t] : src[2 * t + 1] - src[2 * t];
+        if (kFromCounts) return in_lds ? s_len[t] : src[t];
+        return src[2 * t + 1] - src[2 * t];
     };
     const bool rank = S <= kOrderMax;
     if (rank) {
@@ -1727,7 +1732,7 @@ void ext_bin(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* 
                            yb, idb, roff, items, cap, overflow);
         const uint32_t nb2 = (uint32_t)((cap + kBinColBlock - 1) / kBinColBlock);
         hipLaunchKernelGGL(k_bin_col_count, dim3(nb2), dim3(256), 0, st, (const uint32_t*)items, cap, m_dev, roff,
-                           nb1, tiles_y, tiles_x, yb, xb, idb, ccnt, tile_cnt);
+                           nb1, tiles_y, tiles_x, idb, ccnt, tile_cnt);
         exclusive_scan_u32(st, ccnt, coff, (uint64_t)tiles_x * nb2, cscan);
         hipLaunchKernelGGL(k_bin_col_scatter, dim3(nb2), dim3(256), 0, st, (const uint32_t*)items, cap, m_dev,
                            tiles_x, xb, idb, (const uint32_t*)coff, ids);
