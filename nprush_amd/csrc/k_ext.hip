@@ -559,6 +559,17 @@ constexpr uint32_t kOrderTiles = 8192;  // tile lengths kept in LDS up to this m
 // tile ranges with the tiles taken column by column (the binning's output
 // order: [start, end) of a tile = the lengths of the tiles before it in
 // column-major order); else src = the ranges.
+// n / d and n % d for n < 2^32 with m = floor((2^32 - 1) / d): umulhi(n, m)
+// is the quotient or one less, fixed by one comparison.
+__device__ __forceinline__ void divmod_u32(uint32_t n, uint32_t d, uint32_t m, uint32_t& q, uint32_t& r) {
+    q = __umulhi(n, m);
+    r = n - q * d;
+    if (r >= d) {
+        ++q;
+        r -= d;
+    }
+}
+
 template <bool kFromCounts>
 __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict__ src, uint32_t tiles_x,
                                                      uint32_t tiles_y, uint32_t* __restrict__ order,
@@ -569,6 +580,7 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict_
     __shared__ uint32_t s_max;
     __shared__ uint32_t s_len[kFromCounts ? kOrderTiles : 1];  // tile lengths (up to kOrderTiles tiles)
     const uint32_t sx = (tiles_x + 1u) / 2u, sy = (tiles_y + 1u) / 2u, S = sx * sy;
+    const uint32_t msx = 0xFFFFFFFFu / sx;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
     const uint32_t ntiles = tiles_x * tiles_y;
     const bool in_lds = kFromCounts && ntiles <= kOrderTiles;
@@ -643,7 +655,8 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict_
             const uint32_t sidx = tid + 1024u * q;
             uint32_t c = 0u;
             if (sidx < S) {
-                const uint32_t stx = sidx % sx, sty = sidx / sx;
+                uint32_t sty, stx;
+                divmod_u32(sidx, sx, msx, sty, stx);
                 for (uint32_t j = 0; j < 4u; ++j) {
                     const uint32_t tx = 2u * stx + (j & 1u), ty = 2u * sty + (j >> 1);
                     if (tx < tiles_x && ty < tiles_y) c += tile_len(ty * tiles_x + tx);
@@ -689,10 +702,12 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict_
     for (uint32_t r = tid; r < R; r += blockDim.x) {
         const uint32_t q = r / 8u, x = r % 8u;
         const uint32_t sidx = r < S ? (rank ? s_rank[r] : r) : ~0u;
+        uint32_t sty = 0u, stx = 0u;
+        if (sidx != ~0u) divmod_u32(sidx, sx, msx, sty, stx);
         for (uint32_t j = 0; j < 4u; ++j) {
             uint32_t t = ~0u;
             if (sidx != ~0u) {
-                const uint32_t tx = 2u * (sidx % sx) + (j & 1u), ty = 2u * (sidx / sx) + (j >> 1);
+                const uint32_t tx = 2u * stx + (j & 1u), ty = 2u * sty + (j >> 1);
                 if (tx < tiles_x && ty < tiles_y) t = ty * tiles_x + tx;
             }
             order[8u * (4u * q + j) + x] = t;
