@@ -165,8 +165,11 @@ int nps_set_adc_ext(nps_handle h, float grad_threshold, float scene_extent, floa
  * nps_adc_apply_deltas - adds 14 deltas per placement (child in the parent
  *   slot: position xyz, log-scale xyz, opacity logit; then the child in the dead
  *   slot) to the children of the last nps_adc;
- * nps_snapshot - restore = 0 saves splats, SH, Adam m/v and accumulators on the
- *   device, restore = 1 copies them back. */
+ * nps_snapshot - restore = 0 saves splats, SH, Adam m/v, Adam's step counter
+ *   and bias corrections, the densification accumulators and the dispatch
+ *   cadence's step count; restore = 1 puts them back (training steps taken in
+ *   between are undone).  The gradient buffer is not saved: between steps it
+ *   holds zeros. */
 int nps_adc_features(nps_handle h, float* feats32, int32_t max_rows, int32_t* nq);
 int nps_adc_children(nps_handle h, int32_t* parents, int32_t* children, int32_t max_rows, int32_t* count);
 int nps_adc_apply_deltas(nps_handle h, const float* deltas14, int32_t rows);

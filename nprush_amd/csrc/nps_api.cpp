@@ -105,6 +105,8 @@ struct nps_ctx {
     DevBuf adc_u32, adc_tmp, adc_scan;
     DevBuf adc_feat, adc_delta, snap;  // learned densification: features, deltas, state snapshot
     bool have_snap = false;
+    float snap_adam_t = 0.0f;     // host step counters at the snapshot
+    int64_t snap_step_count = 0;
     int64_t stats_host[12] = {0};
     bool timing = false;
     hipEvent_t ev[8] = {nullptr};
@@ -934,8 +936,9 @@ int nps_snapshot(nps_handle h, int32_t restore) {
     if (!h->ext) return set_err(h, NPS_ERR_STATE, "EXT mode only");
     NPS_HIP(h, hipSetDevice(h->device));
     const size_t n = h->n;
-    const size_t sz[5] = {64 * n, 192 * n, 256 * n, 256 * n, 8 * n};
-    void* live[5] = {h->splats, h->sh, h->m, h->v, h->accum};
+    // splats, SH, Adam m and v, accumulators, Adam's (t, 1 - b1^t, 1 - b2^t, skip)
+    const size_t sz[6] = {64 * n, 192 * n, 256 * n, 256 * n, 8 * n, 16};
+    void* live[6] = {h->splats, h->sh, h->m, h->v, h->accum, h->adam_dev.p};
     size_t tot = 0;
     for (size_t s : sz) tot += s;
     if (restore) {
@@ -945,14 +948,21 @@ int nps_snapshot(nps_handle h, int32_t restore) {
         if ((rc = ensure(h, h->snap, tot)) != NPS_OK) return rc;
     }
     size_t off = 0;
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < 6; ++k) {
         char* s = (char*)h->snap.p + off;
         if (restore) NPS_HIP(h, hipMemcpyAsync(live[k], s, sz[k], hipMemcpyDeviceToDevice, h->stream));
         else NPS_HIP(h, hipMemcpyAsync(s, live[k], sz[k], hipMemcpyDeviceToDevice, h->stream));
         off += sz[k];
     }
     NPS_HIP(h, hipStreamSynchronize(h->stream));
-    if (!restore) h->have_snap = true;
+    if (restore) {
+        h->adam_t = h->snap_adam_t;
+        h->step_count = h->snap_step_count;
+    } else {
+        h->have_snap = true;
+        h->snap_adam_t = h->adam_t;
+        h->snap_step_count = h->step_count;
+    }
     return NPS_OK;
 }
 

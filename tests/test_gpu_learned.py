@@ -108,6 +108,40 @@ def test_features_placements_deltas_snapshot():
     m.close()
 
 
+def test_snapshot_undoes_training_steps():
+    """A restore after training steps gives back the whole optimizer state,
+    Adam's step counter included: snapshot, 5 steps, restore, 5 steps equals a
+    twin's 5 steps bit for bit (splats, SH, Adam m/v and t)."""
+    m, views = _scene()
+    twin, tviews = _scene()
+    m.snapshot()
+    densify.train_steps(m, views, 0, 5)
+    m.restore()
+    densify.train_steps(m, views, 0, 5)
+    densify.train_steps(twin, tviews, 0, 5)
+    assert np.array_equal(m.read_splats(), twin.read_splats())
+    assert np.array_equal(m.read_sh(), twin.read_sh())
+    am, av, at = m.read_adam()
+    bm, bv, bt = twin.read_adam()
+    assert np.array_equal(am, bm) and np.array_equal(av, bv) and at == bt == 5.0
+    m.close()
+    twin.close()
+
+
+def test_lookahead_bound_with_no_move_is_the_plain_run():
+    """run_lookahead_bound trains ahead from a snapshot at every densification;
+    moving the children 0 of the way there must leave the plain trajectory
+    untouched (same losses, bit for bit), so the bound's gains come from the
+    children's placement only."""
+    m, views = _scene(seed=2)
+    plain = densify.run(m, views, 90, 30, None, eval_every=30)
+    m.close()
+    m, views = _scene(seed=2)
+    bound0 = densify.run_lookahead_bound(m, views, 90, 30, horizon=15, eval_every=30, fraction=0.0)
+    m.close()
+    assert [l for _, l in plain] == [l for _, l in bound0]
+
+
 @pytest.mark.parametrize("bad", [True, False])
 def test_judge_keeps_heuristic_on_worse_proposal(bad):
     m, views = _scene()
