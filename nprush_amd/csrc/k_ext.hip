@@ -8,10 +8,12 @@
 //                      monotone in w) << 32 | id;
 //   depth order      : the per-splat keys sorted on their high 32 bits (stable,
 //                      so equal depths stay in id order) give every splat its
-//                      rank in (depth, id) order;
-//   k_ext_keys       : one key tile << 32 | id per (splat, covered tile),
-//                      written in rank order, so a stable LSD sort on the tile
-//                      bits alone leaves every tile list in (depth, id) order;
+//                      rank in (depth, id) order; the last pass gathers the
+//                      tile counts in rank order;
+//   k_bin_*          : binning in rank order without a key sort (row buckets,
+//                      then a stable column pass) into per-tile lists in
+//                      (depth, id) order; k_ext_keys + a stable LSD sort on the
+//                      tile bits is the fallback for grids above 256 x 256 tiles;
 //   k_composite_ext  : per 16x16 tile, front-to-back compositing with early
 //                      termination, image + L1/L2 loss, then the reverse pass;
 //   k_ext_chain      : per splat, the 2D gradients (screen centre, conic, colour,
@@ -90,14 +92,6 @@ __global__ __launch_bounds__(256) void k_ext_preprocess(const float4* __restrict
     f.ext_x = o.ext[0];
     f.ext_y = o.ext[1];
     rec[i] = f;
-}
-
-// Tile counts in rank order: rank_counts[r] = counts[id of rank r].
-__global__ __launch_bounds__(256) void k_ext_rank_counts(uint32_t n, const uint64_t* __restrict__ order,
-                                                         const uint32_t* __restrict__ counts,
-                                                         uint32_t* __restrict__ rank_counts) {
-    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < n) rank_counts[r] = counts[(uint32_t)order[r]];
 }
 
 // One key tile << 32 | id per (splat, covered tile), splats taken in rank
@@ -344,7 +338,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_row_emit(uint32_t n, const 
 // last item.  Counts need no order: every item adds 1 to its column's and its
 // tile's LDS counter (LDS atomics; lanes that hit one address serialise).
 __global__ __launch_bounds__(256) void k_bin_col_count(const uint32_t* __restrict__ items, uint64_t m,
-                                                       const uint64_t* __restrict__ m_dev,
+                                                       const uint32_t* __restrict__ m_dev,
                                                        const uint32_t* __restrict__ offs1, uint32_t nblk1,
                                                        uint32_t ny, uint32_t nx, int id_bits,
                                                        uint32_t* __restrict__ counts2,
@@ -353,7 +347,7 @@ __global__ __launch_bounds__(256) void k_bin_col_count(const uint32_t* __restric
     __shared__ uint32_t s_start[(1 << kBinMaxBits) + 1];
     __shared__ uint32_t s_c[256];
     __shared__ uint32_t s_t[kLocal];
-    if (m_dev) m = min(*m_dev, m);
+    if (m_dev) m = min((uint64_t)*m_dev, m);
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     for (uint32_t y = tid; y < ny; y += 256) s_start[y] = offs1[(size_t)y * nblk1];
     s_c[tid] = 0u;
@@ -406,14 +400,14 @@ __global__ __launch_bounds__(256) void k_bin_col_count(const uint32_t* __restric
 // k_radix_scatter, staged in LDS in output order and written as runs of
 // consecutive ids per column.
 __global__ __launch_bounds__(256) void k_bin_col_scatter(const uint32_t* __restrict__ items, uint64_t m,
-                                                         const uint64_t* __restrict__ m_dev, uint32_t nx, int col_bits,
+                                                         const uint32_t* __restrict__ m_dev, uint32_t nx, int col_bits,
                                                          int id_bits, const uint32_t* __restrict__ offs2,
                                                          uint32_t* __restrict__ ids) {
     __shared__ uint32_t wcnt[4][256];
     __shared__ uint32_t s_ids[kBinColBlock];
     __shared__ uint32_t s_gbase[256];
     __shared__ uint32_t s_w4[4];
-    if (m_dev) m = min(*m_dev, m);
+    if (m_dev) m = min((uint64_t)*m_dev, m);
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
     for (uint32_t k = tid; k < 4 * 256; k += 256) (&wcnt[0][0])[k] = 0u;
     __syncthreads();
@@ -1691,12 +1685,6 @@ void launch_ext_preprocess(hipStream_t st, const float* splats, const float* sh,
     }
 }
 
-void launch_ext_rank_counts(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* counts,
-                            uint32_t* rank_counts) {
-    if (n == 0) return;
-    hipLaunchKernelGGL(k_ext_rank_counts, dim3((n + 255) / 256), dim3(256), 0, st, n, order, counts, rank_counts);
-}
-
 void launch_ext_keys(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts,
                      const uint4* rects, uint32_t tiles_x, const uint64_t* offsets, uint64_t* keys, uint64_t cap,
                      uint32_t* overflow) {
@@ -1726,7 +1714,10 @@ size_t ext_bin_col_words(uint32_t tiles_x, uint64_t cap) {
 
 void ext_bin_rows(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts, const uint4* rects,
                   uint32_t tiles_y, uint32_t* rcnt, uint32_t* roff, void* rscan) {
-    if (n == 0 || tiles_y == 0) return;
+    if (n == 0 || tiles_y == 0) {  // no keys: the count word alone
+        launch_fill_u32(st, roff + (ext_bin_row_words(n, tiles_y) - 1), 0u, 1);
+        return;
+    }
     const uint32_t nb1 = (n + kBinThreads - 1) / kBinThreads;
     hipLaunchKernelGGL(k_bin_row_count, dim3(nb1), dim3(kBinThreads), 0, st, n, order, rank_counts, rects, tiles_y,
                        rcnt);
@@ -1734,7 +1725,7 @@ void ext_bin_rows(hipStream_t st, uint32_t n, const uint64_t* order, const uint3
 }
 
 void ext_bin(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts, const uint4* rects,
-             uint32_t tiles_x, uint32_t tiles_y, uint64_t cap, const uint64_t* m_dev, const uint32_t* roff,
+             uint32_t tiles_x, uint32_t tiles_y, uint64_t cap, const uint32_t* m_dev, const uint32_t* roff,
              uint32_t* items, uint32_t* ids, uint32_t* ranges, uint32_t* tile_order, uint32_t* ccnt, uint32_t* coff,
              uint32_t* tile_cnt, void* cscan, uint32_t* overflow) {
     const int xb = bits_for_u32(tiles_x > 0 ? tiles_x - 1 : 0), yb = bits_for_u32(tiles_y > 0 ? tiles_y - 1 : 0);

@@ -225,7 +225,9 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t* 
                                                                 int shift, uint32_t mask, int bits, uint32_t nblk,
                                                                 const uint32_t* __restrict__ offsets,
                                                                 uint64_t* __restrict__ keys_out,
-                                                                uint32_t* __restrict__ vals_out) {
+                                                                uint32_t* __restrict__ vals_out,
+                                                                const uint32_t* __restrict__ gsrc,
+                                                                uint32_t* __restrict__ gdst) {
     __shared__ uint32_t wcnt[4][RADIX];
     if (m_dev) m = min(*m_dev, m);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -280,6 +282,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t* 
             uint32_t dst = wcnt[wid][d] + rank[r];
             keys_out[dst] = key[r];
             if (kHasVals) vals_out[dst] = val[r];
+            if (gsrc) gdst[dst] = gsrc[(uint32_t)key[r]];
         }
     }
 }
@@ -304,7 +307,8 @@ size_t radix_temp_bytes(uint64_t m, bool wide) {
 // and the launches are the same for every count (graph capture).  wide:
 // 11-bit digits, else 8-bit digits.
 bool radix_sort_u64(hipStream_t st, uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt,
-                    uint64_t m, int bit_lo, int bit_hi, void* temp, const uint64_t* m_dev, bool wide) {
+                    uint64_t m, int bit_lo, int bit_hi, void* temp, const uint64_t* m_dev, bool wide,
+                    const uint32_t* gather_src, uint32_t* gather_dst) {
     if (m == 0 || bit_hi <= bit_lo) return false;
     const uint64_t block = wide ? kSortThreads * kWideItems : kSortBlock;
     uint32_t nblk = (uint32_t)((m + block - 1) / block);
@@ -334,9 +338,11 @@ bool radix_sort_u64(hipStream_t st, uint64_t* keys, uint64_t* keys_alt, uint32_t
             hipLaunchKernelGGL((k_radix_hist<kRadix, kSortItems>), dim3(nblk), dim3(kSortThreads), 0, st,
                                (const uint64_t*)kin, m, m_dev, shift, mask, nblk, hist);
         exclusive_scan_u32(st, hist, offs, hn, scan_tmp);
+        const uint32_t* gs = p == passes - 1 ? gather_src : nullptr;
 #define NPS_SCATTER(V, RX, KI)                                                                                      \
     hipLaunchKernelGGL((k_radix_scatter<V, RX, KI>), dim3(nblk), dim3(kSortThreads), 0, st, (const uint64_t*)kin,     \
-                       (const uint32_t*)vin, m, m_dev, shift, mask, bits, nblk, (const uint32_t*)offs, kout, vout)
+                       (const uint32_t*)vin, m, m_dev, shift, mask, bits, nblk, (const uint32_t*)offs, kout, vout,    \
+                       gs, gather_dst)
         if (vals) {
             if (wide) NPS_SCATTER(true, kRadixWide, kWideItems);
             else NPS_SCATTER(true, kRadix, kSortItems);

@@ -247,19 +247,28 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
     double* g2 = (out && out->g2) ? out->g2 : (double*)x.g2.p;
     launch_ext_preprocess(st, sc.splats, sc.sh, n, sc.deg, v, cam, (ExtRec*)x.rec.p, counts, (uint4*)x.rects.p,
                           (uint64_t*)x.dkey.p);
-    // rank of every splat in (depth, id) order: stable sort of depth << 32 | id on the depth bits
+    // rank of every splat in (depth, id) order: stable sort of depth << 32 | id on
+    // the depth bits; its last pass also gathers the tile counts in rank order
     const bool dalt = radix_sort_u64(st, (uint64_t*)x.dkey.p, (uint64_t*)x.dkey_alt.p, nullptr, nullptr, n, 32, 64,
-                                     x.sort_tmp.p, nullptr, true);
+                                     x.sort_tmp.p, nullptr, true, (const uint32_t*)counts,
+                                     (uint32_t*)x.rank_counts.p);
     const uint64_t* order = (const uint64_t*)(dalt ? x.dkey_alt.p : x.dkey.p);
-    launch_ext_rank_counts(st, n, order, (const uint32_t*)counts, (uint32_t*)x.rank_counts.p);
-    exclusive_scan_u32_to_u64(st, (const uint32_t*)x.rank_counts.p, (uint64_t*)x.offsets.p, n, x.scan_tmp.p);
-    // the binning's row pass needs no key count: it runs while the host waits for it
+    // The binning's row pass needs no key count, and its last row offset is the
+    // count (u32).  The key path (sorted tile << 32 | id keys) scans per-rank key
+    // offsets instead and takes the count from them (u64).
     const bool rows_early = ext_bin_fits(n, v.tiles_x, v.tiles_y, 0);
-    auto bin_rows = [&]() {
-        if (rows_early)
-            ext_bin_rows(st, n, order, (const uint32_t*)x.rank_counts.p, (const uint4*)x.rects.p, v.tiles_y,
-                         (uint32_t*)x.bin_rcnt.p, (uint32_t*)x.bin_roff.p, x.bin_rscan.p);
-    };
+    const bool key_path = !rows_early || (capture && x.cap >= 0xFFFFFFFFull);
+    const uint32_t* count32 = nullptr;
+    const uint64_t* count64 = nullptr;
+    if (rows_early) {
+        ext_bin_rows(st, n, order, (const uint32_t*)x.rank_counts.p, (const uint4*)x.rects.p, v.tiles_y,
+                     (uint32_t*)x.bin_rcnt.p, (uint32_t*)x.bin_roff.p, x.bin_rscan.p);
+        count32 = (const uint32_t*)x.bin_roff.p + (ext_bin_row_words(n, v.tiles_y) - 1);
+    }
+    if (key_path) {
+        exclusive_scan_u32_to_u64(st, (const uint32_t*)x.rank_counts.p, (uint64_t*)x.offsets.p, n, x.scan_tmp.p);
+        count64 = (const uint64_t*)x.offsets.p + n;
+    }
     uint64_t cap;
     const uint64_t* m_dev = nullptr;
     if (!capture) {
@@ -269,14 +278,15 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
             return rc;
         if (!x.count_host && (rc = hip(hipHostMalloc((void**)&x.count_host, 64, hipHostMallocDefault), "pinned")) != NPS_OK)
             return rc;
-        if ((rc = hip(hipMemcpyAsync(x.count_host, (uint64_t*)x.offsets.p + n, sizeof(uint64_t),
-                                     hipMemcpyDeviceToHost, st), "count readback")) != NPS_OK)
+        uint32_t* count_host32 = (uint32_t*)(x.count_host + 1);
+        if ((rc = hip(count64 ? hipMemcpyAsync(x.count_host, count64, sizeof(uint64_t), hipMemcpyDeviceToHost, st)
+                              : hipMemcpyAsync(count_host32, count32, sizeof(uint32_t), hipMemcpyDeviceToHost, st),
+                      "count readback")) != NPS_OK)
             return rc;
         if ((rc = hip(hipEventRecord(x.ev_count, st), "event")) != NPS_OK) return rc;
-        bin_rows();
         if (grads) launch_fill_u32(st, (uint32_t*)g2, 0u, 18 * (uint64_t)n);
         if ((rc = hip(hipEventSynchronize(x.ev_count), "sync")) != NPS_OK) return rc;
-        const uint64_t m = *x.count_host;
+        const uint64_t m = count64 ? x.count_host[0] : (uint64_t)*count_host32;
         if (m >= 0xFFFFFFFFull) {
             err = "tile key count exceeds 2^32";
             return NPS_ERR_CAPACITY;
@@ -286,8 +296,7 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
         if ((rc = ext_view_reserve(x, n, v.W, v.H, m, grads, tu, st, err)) != NPS_OK) return rc;
     } else {
         cap = x.cap;
-        m_dev = (const uint64_t*)x.offsets.p + n;
-        bin_rows();
+        m_dev = count64;
     }
     x.num_tiles = v.num_tiles;
     x.col_major = rows_early && ext_bin_fits(n, v.tiles_x, v.tiles_y, cap);
@@ -297,7 +306,7 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
         x.ids = (uint32_t*)x.keys_alt.p;
         if (ev && (rc = hip(hipEventRecord(ev[1], st), "event")) != NPS_OK) return rc;
         ext_bin(st, n, order, (const uint32_t*)x.rank_counts.p, (const uint4*)x.rects.p, v.tiles_x, v.tiles_y, cap,
-                m_dev, (const uint32_t*)x.bin_roff.p, (uint32_t*)x.keys.p, x.ids, (uint32_t*)x.ranges.p,
+                capture ? count32 : nullptr, (const uint32_t*)x.bin_roff.p, (uint32_t*)x.keys.p, x.ids, (uint32_t*)x.ranges.p,
                 (uint32_t*)x.order.p, (uint32_t*)x.bin_cnt.p, (uint32_t*)x.bin_off.p, (uint32_t*)x.bin_tile.p,
                 x.bin_scan.p, flags);
         if (ev && (rc = hip(hipEventRecord(ev[2], st), "event")) != NPS_OK) return rc;

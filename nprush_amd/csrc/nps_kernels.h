@@ -61,9 +61,12 @@ void exclusive_scan_u32_to_u64(hipStream_t st, const uint32_t* in, uint64_t* out
 void exclusive_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t n, void* temp);
 size_t radix_temp_bytes(uint64_t m, bool wide = false);
 // wide: digits of up to 11 bits (2048 buckets, fewer passes) instead of 8.
+// gather_src/dst (optional): the last pass also writes gather_dst[r] =
+// gather_src[low 32 bits of the key placed at r].
 bool radix_sort_u64(hipStream_t st, uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt,
                     uint64_t m, int bit_lo, int bit_hi, void* temp, const uint64_t* m_dev = nullptr,
-                    bool wide = false);
+                    bool wide = false,
+                    const uint32_t* gather_src = nullptr, uint32_t* gather_dst = nullptr);
 
 
 // k_composite.hip
@@ -130,9 +133,10 @@ struct ExtCompositeArgs {
 // (depth, id) order (tiles column by column), and the tile ranges.
 // ext_bin_fits: at most 256 tile rows and columns, the 4-byte items hold the
 // column and the id, and the counts fit 32 bits (else the caller sorts keys).
-// ext_bin_rows (row counts and their offsets) needs no key count, so it can
-// run before the count is known; ext_bin then does the rest, ending with the
-// ranges and the composite's workgroup order (launch_tile_order's output).
+// ext_bin_rows (row counts and their offsets) needs no key count; the last
+// offset, roff[ext_bin_row_words(n, tiles_y) - 1], is the key count (0 when n
+// is 0).  ext_bin then does the rest, ending with the ranges and the
+// composite's workgroup order (launch_tile_order's output).
 // rcnt / roff: ext_bin_row_words(...) words each, rscan: scan_temp_bytes of
 // that; ccnt / coff: ext_bin_col_words(...) words each, cscan likewise.
 bool ext_bin_fits(uint32_t n, uint32_t tiles_x, uint32_t tiles_y, uint64_t cap);
@@ -141,7 +145,7 @@ size_t ext_bin_col_words(uint32_t tiles_x, uint64_t cap);
 void ext_bin_rows(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts, const uint4* rects,
                   uint32_t tiles_y, uint32_t* rcnt, uint32_t* roff, void* rscan);
 void ext_bin(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts, const uint4* rects,
-             uint32_t tiles_x, uint32_t tiles_y, uint64_t cap, const uint64_t* m_dev, const uint32_t* roff,
+             uint32_t tiles_x, uint32_t tiles_y, uint64_t cap, const uint32_t* m_dev, const uint32_t* roff,
              uint32_t* items, uint32_t* ids, uint32_t* ranges, uint32_t* tile_order, uint32_t* ccnt, uint32_t* coff,
              uint32_t* tile_cnt, void* cscan, uint32_t* overflow);
 void launch_keys_to_ids(hipStream_t st, const uint64_t* keys, uint64_t m, const uint64_t* m_dev, uint32_t* ids);
@@ -151,8 +155,6 @@ void launch_tile_order(hipStream_t st, const uint32_t* ranges, uint32_t tiles_x,
 void launch_ext_preprocess(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg,
                            const ViewParams& v, const float cam[3], ExtRec* rec, uint32_t* counts, uint4* rects,
                            uint64_t* dkey);
-void launch_ext_rank_counts(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* counts,
-                            uint32_t* rank_counts);
 // keys beyond `cap` are not written; *overflow is then set to 1
 void launch_ext_keys(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts,
                      const uint4* rects, uint32_t tiles_x, const uint64_t* offsets, uint64_t* keys, uint64_t cap,
