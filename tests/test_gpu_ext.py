@@ -356,3 +356,38 @@ def test_ext_block_cull_shapes(oracle, tile, batch):
     assert st["active_pairs"] == int(loss[3])       # composited pairs, as the reverse pass takes them
     _check_grads(mgr.read_grads_f32(), g, gs, deg)
     mgr.close()
+
+
+@pytest.mark.parametrize("W,H", [(2064, 48), (48, 2064)])
+def test_ext_key_path_beyond_256_tiles(oracle, W, H):
+    """A tile grid with more than 256 columns (or rows) at 8-pixel tiles is
+    binned by the key path (tile << 32 | id keys in depth order, a stable sort
+    on the tile bits) instead of the row/column buckets: same lists, image and
+    gradients as the oracle."""
+    deg = 1
+    n = 5000
+    s, sh = scenes.make_ext_splats(n, 21, sh_degree=deg)
+    s[:, 0] *= 1.0 if W > H else 0.05      # spread the scene over the long side
+    s[:, 1] *= 1.0 if H > W else 0.05
+    vp, vi = camera.view_proj((0.0, 0.0, 2.5), aspect=W / H)
+    rng = np.random.default_rng(6)
+    tgt = np.ones((H, W, 4), np.float32)
+    tgt[..., :3] = rng.random((H, W, 3)).astype(np.float32)
+    mgr = GpuSplatOptimizerManager(numSplats=n, mode="ext", sh_degree=deg, splats=s, sh=sh, adc_enabled=False,
+                                   ext_tile=8, ext_batch=32)
+    mgr.writeSplatVPMatrix(vp, vi)
+    mgr.setBackwardTarget(tgt, np.ones((H, W), np.float32), W, H)
+    mgr.backward()
+    ranges, ids, full = mgr.read_tiles()
+    tiles, lists, counts = oracle.ext_bin(s, sh, deg, vp, vi, W, H, tile=8)
+    assert max((W + 7) // 8, (H + 7) // 8) > 256 and counts.sum() > 0
+    assert np.array_equal(full.astype(np.int64), counts.astype(np.int64))
+    for t in range(len(tiles)):
+        a, b = ranges[t]
+        assert np.array_equal(ids[a:b], lists[t, :counts[t]]), t
+    t16, l16, c16 = oracle.ext_bin(s, sh, deg, vp, vi, W, H)
+    g, gs, c, loss = oracle.ext_backward_f32(s, sh, deg, vp, vi, tgt, W, H, t16, l16, c16)
+    img, _ = mgr.read_image()
+    assert np.array_equal(img, c)
+    _check_grads(mgr.read_grads_f32(), g, gs, deg)
+    mgr.close()
