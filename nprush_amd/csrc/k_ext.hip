@@ -804,7 +804,8 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
     float T = 1.0f, Cr = 0.0f, Cg = 0.0f, Cb = 0.0f;
     int last = 0;
     bool alive = valid;
-    uint32_t n_fw = 0u;  // entries the wave evaluated (uniform)
+    uint32_t n_fw = 0u;    // entries the wave evaluated (uniform)
+    uint32_t n_take = 0u;  // pairs the wave composited (uniform)
     int stop = 0;        // 1 + index of the entry that stopped this pixel (0: not stopped)
     int fw_extent = 0;   // entries staged by the forward (uniform)
     prime(0, BATCH);
@@ -862,14 +863,16 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
                 // alphas of the group's entries are independent: computed first,
                 // branch-free (selects), so their LDS reads and arithmetic overlap
                 float ca[kFwdGroup];
+                bool ctk[kFwdGroup];
 #pragma unroll
                 for (int j = 0; j < kFwdGroup; ++j) {
                     const int q = e[j] < 0 ? 0 : e[j];
-                    ca[j] = ext_pair_alpha(pxf, pyf, rec4[3 * q], rec4[3 * q + 1], rec4[3 * q + 2], e[j] >= 0);
+                    float G;
+                    ctk[j] = ext_pair_eval(pxf, pyf, rec4[3 * q], rec4[3 * q + 1], rec4[3 * q + 2], e[j] >= 0, ca[j], G);
                 }
                 // then the front-to-back chain, entry by entry: a pixel that does
-                // not take the entry composites it with alpha 0, which leaves T
-                // and C unchanged bit for bit (fma(0, x, C) = C)
+                // not composite the entry adds 0 to C, which leaves C unchanged bit
+                // for bit (fma(0, x, C) = C)
 #pragma unroll
                 for (int j = 0; j < kFwdGroup; ++j) {
                     if (e[j] < 0) break;  // uniform
@@ -877,7 +880,7 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
                     if (A.backward)
                         for (; ck_next <= (e[j] >> 3); ++ck_next) ck[ck_next * NT + tid] = T;
                     const float a = ca[j];
-                    const bool take = alive & !(a < kExtMinAlpha);
+                    const bool take = alive & ctk[j];
                     const float tT = fmaf(-T, a, T);
                     const bool dies = take & (tT < kExtTMin);
                     const bool inc = take & !dies;
@@ -891,8 +894,11 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
                     last = inc ? b0 + e[j] + 1 : last;
                     stop = dies ? b0 + e[j] + 1 : stop;
                     alive = alive & !dies;
-                    const uint64_t bal = __ballot(inc);
-                    if (bal != 0ull) mask[e[j] >> 6] |= 1ull << (e[j] & 63);
+                    const uint64_t bal = ballot_b(inc);
+                    if (bal != 0ull) {
+                        mask[e[j] >> 6] |= 1ull << (e[j] & 63);
+                        n_take += (uint32_t)__popcll(bal);
+                    }
                 }
                 wave_alive = __ballot(alive) != 0ull;
             }
@@ -941,7 +947,7 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
     for (int w = 0; w < NW; ++w) maxlast = max(maxlast, s_max[w]);
 
     // ---------------- reverse pass ----------------
-    uint32_t n_active = 0u, n_went = 0u;
+    uint32_t n_went = 0u;
     if (A.backward) {
         float dT = (dCr * kExtBg + dCg * kExtBg) + dCb * kExtBg;
         const int nbr = (maxlast + BATCH - 1) / BATCH;
@@ -983,12 +989,12 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
                 if (c_next >= 0) T_next = ck[c_next * NT + tid];
                 const uint32_t bits = (uint32_t)(mask[c >> 3] >> (8 * (c & 7))) & 0xFFu;
                 float ra[8], rT[8], rG[8];
-                uint32_t incb = 0u;
                 // an entry whose bit is 0 changed no lane's transmittance before
                 // that lane's last composited entry (it would have been composited):
                 // skipped.  Alpha and T are the forward's bit for bit (same
                 // operations); a lane that did not composite the entry gets alpha
-                // and G zero, which zeroes all of its terms below.
+                // and G zero, which zeroes all of its terms below and leaves T
+                // unchanged (fma(-T, 0, T) = T).
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     ra[j] = 0.0f;
@@ -997,14 +1003,12 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
                     if ((bits >> j) & 1u) {
                         const int e = b0 + 8 * c + j;
                         const int q = 8 * c + j;
-                        float G;
-                        const float a = ext_pair_alpha(pxf, pyf, rec4[3 * q], rec4[3 * q + 1], rec4[3 * q + 2],
-                                                       e < last, &G);
-                        const bool ok = !(a < kExtMinAlpha);
+                        float a, G;
+                        const bool ok =
+                            ext_pair_eval(pxf, pyf, rec4[3 * q], rec4[3 * q + 1], rec4[3 * q + 2], e < last, a, G);
                         ra[j] = ok ? a : 0.0f;
                         rG[j] = ok ? G : 0.0f;
-                        incb |= (ok ? 1u : 0u) << j;
-                        Tc = ok ? fmaf(-Tc, a, Tc) : Tc;
+                        Tc = fmaf(-Tc, ra[j], Tc);
                     }
                 }
                 // the opacity terms of the chunk's entries, reduced together after it
@@ -1014,8 +1018,6 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
                     o8[j] = 0.0f;
                     if ((bits >> j) & 1u) {
                         const int jj = 8 * c + j;
-                        const bool inc = (incb >> j) & 1u;
-                        n_active += (uint32_t)__popcll(__ballot(inc));
                         n_went += 1u;
                         const float4 p0 = rec4[3 * jj];
                         const float4 p1 = rec4[3 * jj + 1];
@@ -1023,29 +1025,31 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
                         const float fr = p1.z, fg = p1.w, fb = p2.x, fop = p1.y;
                         // per-pair gradient terms (only FP32 accuracy needed): with
                         // s = dC . rgb, dL/dalpha = T (s - dT), and the seed of the
-                        // transmittance before the entry is dT + alpha (s - dT)
+                        // transmittance before the entry is dT + alpha (s - dT).
+                        // The geometry enters through r^2 = d^T Q d only: with
+                        // w = dL/dalpha G (= dL/dop) and dL/dr^2 = -op w / 2, the wave
+                        // sums of w d and w d d^T give the centre and conic gradients
+                        // once multiplied by the entry's op and conic (after the sum).
                         const float dx = pxf - p0.x, dy = pyf - p0.y;
-                        const float spx = fmaf(p0.z, dx, p0.w * dy);
-                        const float spy = fmaf(p0.w, dx, p1.x * dy);
                         const float Tk = rT[j], a = ra[j];
                         // no gradient through a clamped alpha
                         const float G = fop * rG[j] > kExtMaxAlpha ? 0.0f : rG[j];
                         const float u = fmaf(dCb, fb, fmaf(dCg, fg, dCr * fr)) - dT;
                         const float da = Tk * u;
                         dT = fmaf(a, u, dT);
-                        const float gr2 = ((-0.5f * fop) * G) * da;  // dL / dr^2
+                        const float w = da * G;
+                        const float wx = w * dx, wy = w * dy;
                         const float Tka = Tk * a;
-                        const float m2 = -2.0f * gr2, gx = gr2 * dx;
                         float gv[8];
                         gv[5] = dCr * Tka;
                         gv[6] = dCg * Tka;
                         gv[7] = dCb * Tka;
-                        gv[0] = m2 * spx;
-                        gv[1] = m2 * spy;
-                        gv[2] = gx * dx;
-                        gv[3] = (2.0f * gx) * dy;
-                        gv[4] = (gr2 * dy) * dy;
-                        o8[j] = da * G;
+                        gv[0] = wx;
+                        gv[1] = wy;
+                        gv[2] = wx * dx;
+                        gv[3] = wx * dy;
+                        gv[4] = wy * dy;
+                        o8[j] = w;
                         const float tot = wave_reduce8(gv, lane);
                         if ((lane & 7) == 0) {
                             int idx = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
@@ -1063,11 +1067,29 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
                 c = c_next;
             }
             __syncthreads();
+            // per entry and term: the wave partials summed in FP64 in wave order;
+            // slots 0-4 hold the sums of w dx, w dy, w dx^2, w dx dy, w dy^2, taken
+            // to d scr (= -2 Q (dL/dr^2) d) and d conic (A, B, C) here
             for (int e = tid; e < nb * 9; e += NT) {
                 const int j = e / 9, k = e - 9 * j;
-                double sum = 0.0;
+                auto part_sum = [&](int kk) {
+                    double s = 0.0;
 #pragma unroll
-                for (int w = 0; w < NW; ++w) sum += (double)s_part[w][j][k];
+                    for (int w = 0; w < NW; ++w) s += (double)s_part[w][j][kk];
+                    return s;
+                };
+                const double op = (double)s_rec[j].op;
+                double sum;
+                if (k == 0)
+                    sum = op * ((double)s_rec[j].A * part_sum(0) + (double)s_rec[j].B * part_sum(1));
+                else if (k == 1)
+                    sum = op * ((double)s_rec[j].B * part_sum(0) + (double)s_rec[j].C * part_sum(1));
+                else if (k == 2 || k == 4)
+                    sum = -0.5 * op * part_sum(k);
+                else if (k == 3)
+                    sum = -op * part_sum(3);
+                else
+                    sum = part_sum(k);
                 if (sum != 0.0) unsafeAtomicAdd(&A.g2[9 * (size_t)s_ids[j] + k], sum);
             }
         }
@@ -1081,7 +1103,7 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
     if (lane == 0) {
         s_cnt[0][wid] = ev;
         s_cnt[1][wid] = n_fw;
-        s_cnt[2][wid] = n_active;
+        s_cnt[2][wid] = A.backward ? n_take : 0u;  // = the pairs the reverse pass takes
         s_cnt[3][wid] = n_went;
     }
     __syncthreads();

@@ -65,28 +65,31 @@ static_assert(sizeof(ExtRec) == 48, "ExtRec layout");
 // (-0.5 log2 e; -0.5 * log2e is exact in FP32).
 constexpr float kExtNegHalfLog2e = -0.5f * kLog2e;
 
-// EXT alpha of the pixel centre (pxf, pyf) and one record, given as its three
+// EXT pair of the pixel centre (pxf, pyf) and one record, given as its three
 // float4 pieces p0 = (scr_x, scr_y, A, B), p1 = (C, op, r, g),
 // p2 = (b, rcut2, ext_x, ext_y):
 //   dx, dy = pixel - centre;  r^2 = fma(dx, fma(A, dx, B dy), dy fma(B, dx, C dy));
-//   G = exp2(r^2 * kExtNegHalfLog2e);  alpha = min(op G, 0.99);
-// 0 when `use` is false, the centre of the pixel lies outside the 3-sigma box
-// or r^2 > rcut2 (where alpha < 1/255 is guaranteed).  *Gout = G, or 0 where
-// alpha is 0 by those tests.  Branch-free.
-__device__ __forceinline__ float ext_pair_alpha(float pxf, float pyf, const float4& p0, const float4& p1,
-                                                const float4& p2, bool use, float* Gout = nullptr) {
+//   G = exp2(r^2 * kExtNegHalfLog2e);  alpha = min(op G, 0.99).
+// Returns whether the pixel takes the pair: `use`, the centre of the pixel in the
+// 3-sigma box, r^2 <= rcut2 (beyond it alpha < 1/255 is guaranteed) and
+// alpha >= 1/255.  a and G are set to alpha and G; they are meaningful only
+// where the pair is taken (r^2 <= rcut2 < 12 keeps exp2's argument in range
+// there).  Branch-free.
+__device__ __forceinline__ bool ext_pair_eval(float pxf, float pyf, const float4& p0, const float4& p1,
+                                              const float4& p2, bool use, float& a, float& G) {
     const float dx = pxf - p0.x, dy = pyf - p0.y;
     const float spx = fmaf(p0.z, dx, p0.w * dy);
     const float spy = fmaf(p0.w, dx, p1.x * dy);
     const float r2 = fmaf(dx, spx, dy * spy);
+    G = exp2_det_in_range(r2 * kExtNegHalfLog2e);
+    a = fminf(p1.y * G, kExtMaxAlpha);  // op G is never NaN where the pair is taken
     // non-short-circuit &: every operand is evaluated, no branch
-    const bool ok = (int)use & (int)(fabsf(dx) <= p2.z) & (int)(fabsf(dy) <= p2.w) & (int)!(r2 > p2.y);
-    const float G = exp2_det_in_range(ok ? r2 * kExtNegHalfLog2e : 0.0f);  // r2 <= rcut2 < 12
-    float a = p1.y * G;
-    a = a > kExtMaxAlpha ? kExtMaxAlpha : a;
-    if (Gout) *Gout = ok ? G : 0.0f;
-    return ok ? a : 0.0f;
+    return (int)use & (int)(fabsf(dx) <= p2.z) & (int)(fabsf(dy) <= p2.w) & (int)!(r2 > p2.y) &
+           (int)!(a < kExtMinAlpha);
 }
+
+// Wave ballot of a predicate (no round trip through a 0/1 integer).
+__device__ __forceinline__ uint64_t ballot_b(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 struct ExtProj {
     bool valid;

@@ -146,6 +146,45 @@ def test_ext_dense_early_termination(oracle):
     mgr.close()
 
 
+def test_ext_speculative_key_buffers_overflow_redo(oracle):
+    """A view of the same shape as the last one sizes its key buffers from that
+    view's count (+25 % + 64K) and keeps its own count on the device.  A view
+    with far more keys overflows them: the dropped keys are flagged, the chain
+    rule adds nothing, and the view is redone with buffers for its count.  The
+    results (tile lists, image, gradients, densification statistics) are then
+    those of the oracle, counted once; the next view of the first camera fits."""
+    W = H = 256
+    deg = 3
+    n = 16384
+    s, sh = scenes.make_ext_splats(n, 0, sh_degree=deg)
+    s[:, 4:7] += 1.0                                  # larger splats: many keys up close
+    far = camera.view_proj((0.0, 0.0, 20.0), aspect=1.0)
+    near = camera.view_proj((0.0, 0.0, 2.0), aspect=1.0)
+    tgt = np.ones((H, W, 4), np.float32)
+    tgt[..., :3] = 0.4
+    mgr = _run(s, sh, deg, *far, tgt, W, H)          # first view: waits for its key count
+    m_far = mgr.read_stats()["keys"]
+    for k, (vp, vi) in enumerate((near, far)):
+        mgr.write_grads_f32(np.zeros((n, 64), np.float32))
+        mgr.write_accum(np.zeros_like(mgr.read_accum()))
+        mgr.writeSplatVPMatrix(vp, vi)
+        mgr.backward()
+        tiles, lists, counts = _check_tiles(mgr, oracle, s, sh, deg, vp, vi, W, H)
+        if k == 0:                                    # the near view overflowed the far view's capacity
+            assert counts.sum() > 1.25 * m_far + 65536
+        assert mgr.read_stats()["keys"] == int(counts.sum())
+        g, gs, c, loss = oracle.ext_backward_f32(s, sh, deg, vp, vi, tgt, W, H, tiles, lists, counts)
+        img, _ = mgr.read_image()
+        assert np.array_equal(img, c)
+        _check_grads(mgr.read_grads_f32(), g, gs, deg)
+        on_screen = np.zeros(n, bool)
+        for t in range(len(tiles)):
+            on_screen[lists[t, :counts[t]]] = True
+        acc = mgr.read_accum()
+        assert np.array_equal(acc[:, 1], on_screen.astype(np.float32))  # one view counted once
+    mgr.close()
+
+
 def test_ext_forward_only_and_accumulation(oracle):
     W = H = 96
     deg = 3
