@@ -478,6 +478,53 @@ int nml_dev_upload(nml_handle h, void* dst, const void* src, int64_t bytes);
 int npd_compose(int32_t device, const float* const* textures, const int32_t* dims, const float enabled[3],
                 float* canvas, int32_t W, int32_t H);
 
+/* ---------------------------------------------------------------- path trace
+ * Progressive path-traced target generator: replaces GpuPathTracePipelineManager
+ * (src/lib/gpu/GpuPathTracePipelineManager.ts:8-248) with its BVH builder
+ * (src/lib/gpu/bvh.ts:144-223) and its two compute pipelines (path_trace.wgsl,
+ * path_trace_resolve.wgsl).  Vertices: 12 floats each [px py pz, nx ny nz,
+ * r g b a, u v] (alpha < 0.5 = mirror, else diffuse with albedo rgb); indices:
+ * 3 per triangle.  The environment texture (envTexture of the constructor) is
+ * ew x eh rgba floats (rgb used; HDR values allowed), sampled equirectangular,
+ * Z up, with the sampler restated as FP32 bilinear filtering (repeat in u,
+ * clamp in v).  The output texture is W x H rgba floats holding rgba8unorm
+ * values (k/255), ready for nps_set_target / npb_blur.
+ * The reference's methods return silently when unbound; these return
+ * NPS_ERR_STATE (no meshes, no output size, no environment). */
+typedef struct npp_ctx* npp_handle;
+int npp_create(int32_t device, npp_handle* out);
+void npp_destroy(npp_handle p);
+const char* npp_last_error(npp_handle p);
+/* envTexture (GpuPathTracePipelineManager.ts:42-45) */
+int npp_set_env(npp_handle p, const float* rgba, int32_t ew, int32_t eh);
+/* setMeshes (GpuPathTracePipelineManager.ts:101-150): the meshes already
+ * concatenated (indices offset by the vertex counts before them); builds the
+ * SAH BVH on the host (8 bins, at most 4 triangles per leaf, bvh.ts) */
+int npp_set_meshes(npp_handle p, const float* vertices, int32_t nverts, const uint32_t* indices, int32_t ntris);
+/* setOutputSize (:152-172): a new size clears the accumulator and the frame count */
+int npp_set_output_size(npp_handle p, int32_t w, int32_t hgt);
+/* reset (:174-180): clears the accumulator and the frame count */
+int npp_reset(npp_handle p);
+/* writeInvViewProjMat (:182-184): column-major inverse view-projection */
+int npp_write_inv_vp(npp_handle p, const float inv_vp[16]);
+/* dispatch (:221-247): one sample per pixel added to the accumulator, then the
+ * resolve into the output texture; the frame count advances */
+int npp_dispatch(npp_handle p);
+/* the output texture (W*H*4 floats), the accumulator (W*H*4: radiance sums,
+ * sample count), and the output's device address (valid until the next
+ * npp_set_output_size; for nps_set_target_dev / npb_blur_dev) */
+int npp_read_output(npp_handle p, float* rgba);
+int npp_read_accum(npp_handle p, float* accum);
+int npp_output_dev(npp_handle p, const float** rgba_dev);
+/* the BVH as built: nodes (8 words each: min xyz, data0, max xyz, data1, f32
+ * bits / u32) and the reordered triangle indices; counts[0] = nodes,
+ * counts[1] = triangles.  Null arrays: counts only. */
+int npp_read_bvh(npp_handle p, uint32_t* nodes, uint32_t* tris, int32_t counts[2]);
+/* Equirectangular environment background of the raster target (envmap.wgsl,
+ * drawn first in GpuRunner's mesh passes): W x H rgba8 values (k/255) of
+ * Reinhard(4 env(dir)) per pixel centre, on the handle's environment. */
+int npp_env_background(npp_handle p, const float inv_vp[16], float* rgba, int32_t w, int32_t hgt);
+
 /* Device buffers on the handle's GPU for callers without a GPU allocator of
  * their own (Deno, ctypes), e.g. to keep several targets resident in HBM for
  * nps_set_target_dev.  No reference counterpart (WebGPU's createBuffer). */
@@ -487,7 +534,8 @@ int nps_dev_upload(nps_handle h, void* dst, const void* src, int64_t bytes);
 
 /* Deterministic transcendentals used by the kernels, evaluated on the GPU
  * over x[n] (and y[n] for pow): op 0=exp2 1=log2 2=pow 3=exp 4=log 5=sin,
- * 6=float->int32 conversion of the gradient path (result bits in out), 7=cos.
+ * 6=float->int32 conversion of the gradient path (result bits in out), 7=cos,
+ * 8=atan2(x, y) (x the ordinate), 9=asin.
  * For tests (bit-identity with the CPU oracle). */
 int nps_debug_math(int32_t device, int32_t op, const float* x, const float* y, float* out, int32_t n);
 

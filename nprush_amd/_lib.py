@@ -92,6 +92,20 @@ SIGNATURES = {
     "npt_read_tune": (i32, [_P, _I]),
     "npt_read_losses": (i32, [_P, _D]),
     "npt_graph_info": (i32, [_P, C.POINTER(C.c_int64), _F]),
+    "npp_create": (i32, [i32, C.POINTER(_P)]),
+    "npp_destroy": (None, [_P]),
+    "npp_last_error": (C.c_char_p, [_P]),
+    "npp_set_env": (i32, [_P, _F, i32, i32]),
+    "npp_set_meshes": (i32, [_P, _F, i32, _U, i32]),
+    "npp_set_output_size": (i32, [_P, i32, i32]),
+    "npp_reset": (i32, [_P]),
+    "npp_write_inv_vp": (i32, [_P, _F]),
+    "npp_dispatch": (i32, [_P]),
+    "npp_read_output": (i32, [_P, _F]),
+    "npp_read_accum": (i32, [_P, _F]),
+    "npp_output_dev": (i32, [_P, C.POINTER(_P)]),
+    "npp_read_bvh": (i32, [_P, _U, _U, _I]),
+    "npp_env_background": (i32, [_P, _F, _F, i32, i32]),
     "npb_create": (i32, [i32, C.POINTER(_P)]),
     "npb_destroy": (None, [_P]),
     "npb_last_error": (C.c_char_p, [_P]),

@@ -19,6 +19,8 @@ __global__ void k_debug_math(int op, const float* x, const float* y, float* out,
         case 6: r = __int_as_float(f2i_hw(a)); break;
         case 5: r = sin_det(a); break;
         case 7: r = cos_det(a); break;
+        case 8: r = atan2_det(a, y[i]); break;
+        case 9: r = asin_det(a); break;
         default: r = 0.0f; break;
     }
     out[i] = r;

@@ -2012,7 +2012,7 @@ int nps_dev_upload(nps_handle h, void* dst, const void* src, int64_t bytes) {
 }
 
 int nps_debug_math(int32_t device, int32_t op, const float* x, const float* y, float* out, int32_t n) {
-    if (!x || !out || n < 0 || (op == 2 && !y)) return NPS_ERR_ARG;
+    if (!x || !out || n < 0 || ((op == 2 || op == 8) && !y)) return NPS_ERR_ARG;
     if (hipSetDevice(device) != hipSuccess) return NPS_ERR_HIP;
     float *dx = nullptr, *dy = nullptr, *dout = nullptr;
     size_t bytes = 4 * (size_t)std::max(n, 1);

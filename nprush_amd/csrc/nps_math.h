@@ -15,6 +15,8 @@
 //   pow(x,y) = exp2(y log2 x); log(x) = log2(x) ln 2; exp(x) = exp2(x log2 e).
 //   sin: double-precision Cody-Waite reduction by pi/2 (three-part constant,
 //        exact for |x| < 2^27) and Taylor polynomials, rounded to float.
+//   atan2, asin: double precision (octant reduction, argument halving, odd
+//        Taylor series), rounded to float.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -153,6 +155,52 @@ NPS_HD float cos_det(float xf) {
     double cosr = 1.0 + r2 * c;
     double out = (q == 0) ? sinr : (q == 1) ? cosr : (q == 2) ? -sinr : -cosr;
     return (float)out;
+}
+
+// atan of t in [0, 1] in double: above tan(pi/8), atan t = pi/4 + atan((t-1)/(t+1));
+// then atan u = 2 atan(u / (1 + sqrt(1 + u^2))) brings |u| below tan(pi/16) < 0.2,
+// where the odd Taylor series to the u^21 term leaves an error below 1e-17.
+NPS_HD double atan01_det(double t) {
+    double base = 0.0, u = t;
+    if (t > 0x1.a827999fcef32p-2) {  // tan(pi/8) = sqrt(2) - 1
+        base = 0x1.921fb54442d18p-1;  // pi/4
+        u = (t - 1.0) / (t + 1.0);
+    }
+    const double v = u / (1.0 + sqrt(1.0 + u * u));
+    const double v2 = v * v;
+    double p = -1.0 / 21.0;
+    p = p * v2 + 1.0 / 19.0;
+    p = p * v2 - 1.0 / 17.0;
+    p = p * v2 + 1.0 / 15.0;
+    p = p * v2 - 1.0 / 13.0;
+    p = p * v2 + 1.0 / 11.0;
+    p = p * v2 - 1.0 / 9.0;
+    p = p * v2 + 1.0 / 7.0;
+    p = p * v2 - 1.0 / 5.0;
+    p = p * v2 + 1.0 / 3.0;
+    p = 1.0 - v2 * p;
+    return base + 2.0 * (v * p);
+}
+
+// atan2 in double by octant: t = min(|x|,|y|) / max(|x|,|y|), then the quadrant.
+// (0, 0) gives 0; a NaN operand gives NaN.
+NPS_HD double atan2_det_d(double y, double x) {
+    if (y != y || x != x) return NAN;
+    const double ax = fabs(x), ay = fabs(y);
+    if (ax == 0.0 && ay == 0.0) return 0.0;
+    const bool swap = ay > ax;
+    double a = atan01_det(swap ? ax / ay : ay / ax);
+    if (swap) a = 0x1.921fb54442d18p+0 - a;  // pi/2
+    if (x < 0.0) a = 0x1.921fb54442d18p+1 - a;  // pi
+    return y < 0.0 ? -a : a;
+}
+
+// FP32 atan2 and asin: the double-precision value rounded to float.
+// asin z = atan2(z, sqrt((1 - z)(1 + z))) for z in [-1, 1].
+NPS_HD float atan2_det(float y, float x) { return (float)atan2_det_d((double)y, (double)x); }
+NPS_HD float asin_det(float zf) {
+    const double z = (double)zf;
+    return (float)atan2_det_d(z, sqrt((1.0 - z) * (1.0 + z)));
 }
 
 NPS_HD float fract_det(float x) { return x - floorf(x); }

@@ -513,6 +513,109 @@ class GpuBlurPipelineManager:
         return out
 
 
+class GpuPathTracePipelineManager:
+    """new GpuPathTracePipelineManager({device, envTexture})
+    (src/lib/gpu/GpuPathTracePipelineManager.ts:8-248): the progressive path
+    tracer that produces the optimisation target of the reference's frame.
+    envTexture is an H x W x 4 float array (rgb used, HDR allowed)."""
+
+    def __init__(self, device=None, envTexture: np.ndarray | None = None, *, hip_device: int = 0):
+        lib = _lib.load()
+        p = C.c_void_p()
+        check(lib.npp_create(int(hip_device), C.byref(p)), None, "npp_create")
+        self._lib = lib
+        self._p = p
+        self.accumWidth = 0
+        self.accumHeight = 0
+        if envTexture is not None:
+            self.setEnvTexture(envTexture)
+
+    def _ok(self, rc, what):
+        if rc != 0:
+            m = self._lib.npp_last_error(self._p)
+            raise _lib.NpsError(f"{what} failed ({rc}): {m.decode() if m else ''}")
+
+    def close(self):
+        if getattr(self, "_p", None):
+            self._lib.npp_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def setEnvTexture(self, env: np.ndarray):
+        e = np.ascontiguousarray(env, dtype=np.float32)
+        self._ok(self._lib.npp_set_env(self._p, fptr(e), int(e.shape[1]), int(e.shape[0])), "npp_set_env")
+
+    def setMeshes(self, meshes):
+        """setMeshes (:101-150): meshes is a list of (vertices [n, 12], indices [m, 3])
+        concatenated with each mesh's indices offset by the vertices before it."""
+        vs, ids, off = [], [], 0
+        for v, i in meshes:
+            v = np.asarray(v, np.float32).reshape(-1, 12)
+            vs.append(v)
+            ids.append(np.asarray(i, np.uint32).reshape(-1) + np.uint32(off))
+            off += v.shape[0]
+        v = np.ascontiguousarray(np.concatenate(vs) if vs else np.zeros((0, 12), np.float32))
+        i = np.ascontiguousarray(np.concatenate(ids) if ids else np.zeros(0, np.uint32))
+        self._ok(self._lib.npp_set_meshes(self._p, fptr(v), int(v.shape[0]),
+                                          i.ctypes.data_as(C.POINTER(C.c_uint32)), int(i.size // 3)),
+                 "npp_set_meshes")
+
+    def setOutputSize(self, width: int, height: int):
+        self._ok(self._lib.npp_set_output_size(self._p, int(width), int(height)), "npp_set_output_size")
+        self.accumWidth, self.accumHeight = int(width), int(height)
+
+    def reset(self):
+        self._ok(self._lib.npp_reset(self._p), "npp_reset")
+
+    def writeInvViewProjMat(self, mat):
+        m = np.ascontiguousarray(np.asarray(mat, np.float32).reshape(16))
+        self._ok(self._lib.npp_write_inv_vp(self._p, fptr(m)), "npp_write_inv_vp")
+
+    def dispatch(self, commandEncoder=None):
+        self._ok(self._lib.npp_dispatch(self._p), "npp_dispatch")
+
+    def read_output(self) -> np.ndarray:
+        """outputTextureView: H x W x 4 rgba8 texel values (k/255)."""
+        out = np.empty((self.accumHeight, self.accumWidth, 4), np.float32)
+        self._ok(self._lib.npp_read_output(self._p, fptr(out)), "npp_read_output")
+        return out
+
+    def read_accum(self) -> np.ndarray:
+        out = np.empty((self.accumHeight, self.accumWidth, 4), np.float32)
+        self._ok(self._lib.npp_read_accum(self._p, fptr(out)), "npp_read_accum")
+        return out
+
+    def output_dev(self) -> int:
+        """Device address of the output texture (for set_target_dev / blur_dev)."""
+        q = C.c_void_p()
+        self._ok(self._lib.npp_output_dev(self._p, C.byref(q)), "npp_output_dev")
+        return int(q.value)
+
+    def read_bvh(self):
+        counts = np.zeros(2, np.int32)
+        I = C.POINTER(C.c_int32)
+        U = C.POINTER(C.c_uint32)
+        self._ok(self._lib.npp_read_bvh(self._p, None, None, counts.ctypes.data_as(I)), "npp_read_bvh")
+        nodes = np.zeros((int(counts[0]), 8), np.uint32)
+        tris = np.zeros((max(int(counts[1]), 1), 3), np.uint32)
+        self._ok(self._lib.npp_read_bvh(self._p, nodes.ctypes.data_as(U), tris.ctypes.data_as(U),
+                                        counts.ctypes.data_as(I)), "npp_read_bvh")
+        return nodes, tris[:int(counts[1])]
+
+    def env_background(self, inv_vp, width: int, height: int) -> np.ndarray:
+        """The environment background pass of the raster target (envmap.wgsl)."""
+        m = np.ascontiguousarray(np.asarray(inv_vp, np.float32).reshape(16))
+        out = np.empty((height, width, 4), np.float32)
+        self._ok(self._lib.npp_env_background(self._p, fptr(m), fptr(out), int(width), int(height)),
+                 "npp_env_background")
+        return out
+
+
 def reference_bezier_init(num_beziers: int, seed: int = 0) -> np.ndarray:
     """The reference's initial curves (GpuBezierOptimizerManager.ts:56-95: short
     randomly oriented squiggles near the origin, width 0.02, softness 0.005,

@@ -472,3 +472,124 @@ def bz_render_f32(bz, vp, W, H, brush_rgba8, rgba=None, clear=True, quantize=Tru
     if rc != 0:
         raise RuntimeError(f"ora_bz_render_f32 failed ({rc})")
     return out
+
+
+# ---------------------------------------------------------------- path trace
+def _pt_sig(lib):
+    if getattr(lib, "_pt_ready", False):
+        return
+    lib.ora_pt_bvh.argtypes = [_F, i32, _U, i32, _U, _U]
+    lib.ora_pt_bvh.restype = i32
+    lib.ora_pt_trace.argtypes = [_F, C.c_uint32, i32, i32, _F, _U, i32, _U, _F, i32, i32, _F]
+    lib.ora_pt_trace.restype = None
+    lib.ora_pt_resolve.argtypes = [_F, i32, _F]
+    lib.ora_pt_resolve.restype = None
+    lib.ora_env_background.argtypes = [_F, _F, i32, i32, i32, i32, _F]
+    lib.ora_env_background.restype = None
+    lib.ora_pt_closest_brute.argtypes = [_F, _F, _F, _U, i32, _F, _I]
+    lib.ora_pt_closest_brute.restype = None
+    lib.ora_pt_closest_bvh.argtypes = [_F, _F, _F, _U, i32, _U, _F]
+    lib.ora_pt_closest_bvh.restype = None
+    lib.ora_pt_hemisphere.argtypes = [_F, C.c_uint32, i32, _F]
+    lib.ora_pt_hemisphere.restype = None
+    lib.ora_pt_pcg.argtypes = [C.c_uint32]
+    lib.ora_pt_pcg.restype = C.c_uint32
+    lib.ora_atan2_asin.argtypes = [_F, _F, i32, _F, _F]
+    lib.ora_atan2_asin.restype = None
+    lib._pt_ready = True
+
+
+def pt_bvh(verts, indices):
+    """SAH BVH of bvh.ts: (nodes u32[n, 8], tris u32[m, 3])."""
+    lib = load()
+    _pt_sig(lib)
+    v = f32(verts).reshape(-1, 12)
+    idx = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1)
+    nt = idx.size // 3
+    nodes = np.zeros((2 * nt + 1, 8), np.uint32)
+    tris = np.zeros((max(nt, 1), 3), np.uint32)
+    nn = lib.ora_pt_bvh(_p(v, _F), v.shape[0], _p(idx, _U), nt, _p(nodes, _U), _p(tris, _U))
+    return nodes[:nn].copy(), tris[:nt].copy()
+
+
+def pt_trace(inv_vp, frame, W, H, verts, nodes, tris, env, accum=None):
+    """One frame of path_trace.wgsl: returns accum (H, W, 4) += radiance, 1."""
+    lib = load()
+    _pt_sig(lib)
+    acc = np.zeros((H, W, 4), np.float32) if accum is None else f32(accum).copy()
+    v = f32(verts).reshape(-1, 12)
+    nd = np.ascontiguousarray(nodes, dtype=np.uint32)
+    tr = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1)
+    if tr.size == 0:
+        tr = np.zeros(3, np.uint32)
+    e = f32(env)
+    eh, ew = e.shape[0], e.shape[1]
+    m = f32(inv_vp).reshape(16)
+    lib.ora_pt_trace(_p(m, _F), C.c_uint32(frame), W, H, _p(v, _F), _p(nd, _U), nd.shape[0], _p(tr, _U), _p(e, _F),
+                     ew, eh, _p(acc, _F))
+    return acc
+
+
+def pt_resolve(accum):
+    lib = load()
+    _pt_sig(lib)
+    a = f32(accum)
+    out = np.empty_like(a)
+    lib.ora_pt_resolve(_p(a, _F), a.shape[0] * a.shape[1], _p(out, _F))
+    return out
+
+
+def env_background(inv_vp, env, W, H):
+    lib = load()
+    _pt_sig(lib)
+    e = f32(env)
+    out = np.empty((H, W, 4), np.float32)
+    lib.ora_env_background(_p(f32(inv_vp).reshape(16), _F), _p(e, _F), e.shape[1], e.shape[0], W, H, _p(out, _F))
+    return out
+
+
+def pt_closest_brute(ro, rd, verts, indices):
+    lib = load()
+    _pt_sig(lib)
+    t = np.zeros(1, np.float32)
+    k = np.zeros(1, np.int32)
+    idx = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1)
+    lib.ora_pt_closest_brute(_p(f32(ro), _F), _p(f32(rd), _F), _p(f32(verts), _F), _p(idx, _U), idx.size // 3,
+                             _p(t, _F), _p(k, _I))
+    return float(t[0]), int(k[0])
+
+
+def pt_closest_bvh(ro, rd, verts, nodes, tris):
+    lib = load()
+    _pt_sig(lib)
+    out = np.zeros(8, np.float32)
+    nd = np.ascontiguousarray(nodes, dtype=np.uint32)
+    tr = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1)
+    lib.ora_pt_closest_bvh(_p(f32(ro), _F), _p(f32(rd), _F), _p(f32(verts), _F), _p(nd, _U), nd.shape[0],
+                           _p(tr, _U), _p(out, _F))
+    return out
+
+
+def pt_hemisphere(n, seed, count):
+    lib = load()
+    _pt_sig(lib)
+    d = np.zeros((count, 3), np.float32)
+    lib.ora_pt_hemisphere(_p(f32(n), _F), C.c_uint32(seed), count, _p(d, _F))
+    return d
+
+
+def pt_pcg(v):
+    lib = load()
+    _pt_sig(lib)
+    return int(lib.ora_pt_pcg(C.c_uint32(v)))
+
+
+def atan2_asin_f32(y, x):
+    lib = load()
+    _pt_sig(lib)
+    y = f32(y)
+    x = f32(x)
+    a = np.empty_like(y)
+    s = np.empty_like(y)
+    lib.ora_atan2_asin(_p(y, _F), _p(x, _F), y.size, _p(a, _F), _p(s, _F))
+    return a, s

@@ -164,6 +164,45 @@ static inline float rm_cosf(float xf) {
     return (float)out;
 }
 
+/* atan of t in [0,1] in double: t > tan(pi/8) -> pi/4 + atan((t-1)/(t+1));
+ * then atan(u) = 2 atan(u / (1 + sqrt(1 + u^2))) and the odd Taylor series of
+ * atan to the u^21 term (|u| < tan(pi/16)).  Same double operations as the
+ * kernels' definition, so the rounded float results agree bit for bit. */
+static inline double rm_atan01(double t) {
+    double base = 0.0, u = t;
+    if (t > 0x1.a827999fcef32p-2) {
+        base = 0x1.921fb54442d18p-1;
+        u = (t - 1.0) / (t + 1.0);
+    }
+    double h = u / (1.0 + sqrt(1.0 + u * u));
+    double h2 = h * h;
+    static const double c[10] = { 1.0 / 3.0, -1.0 / 5.0, 1.0 / 7.0, -1.0 / 9.0, 1.0 / 11.0,
+                                  -1.0 / 13.0, 1.0 / 15.0, -1.0 / 17.0, 1.0 / 19.0, -1.0 / 21.0 };
+    /* q = c0 + h2 (c1 + h2 (... + h2 c9)) evaluated from the inside out */
+    double q = c[9];
+    for (int k = 8; k >= 0; --k) q = q * h2 + c[k];
+    q = 1.0 - h2 * q;
+    return base + 2.0 * (h * q);
+}
+
+static inline double rm_atan2d(double y, double x) {
+    if (y != y || x != x) return NAN;
+    double ax = fabs(x), ay = fabs(y);
+    if (ax == 0.0 && ay == 0.0) return 0.0;
+    int swap = ay > ax;
+    double a = rm_atan01(swap ? ax / ay : ay / ax);
+    if (swap) a = 0x1.921fb54442d18p+0 - a;
+    if (x < 0.0) a = 0x1.921fb54442d18p+1 - a;
+    return y < 0.0 ? -a : a;
+}
+
+static inline float rm_atan2f(float y, float x) { return (float)rm_atan2d((double)y, (double)x); }
+/* asin z = atan2(z, sqrt((1 - z)(1 + z))), z in [-1, 1] */
+static inline float rm_asinf(float zf) {
+    double z = (double)zf;
+    return (float)rm_atan2d(z, sqrt((1.0 - z) * (1.0 + z)));
+}
+
 /* WGSL fract(e) = e - floor(e) */
 static inline float rm_fractf(float x) { return x - floorf(x); }
 

@@ -10,7 +10,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def _declared():
     src = open(os.path.join(ROOT, "include", "nprush_hip.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b((?:nps|npf|npb|npt|nbz|nbf|nml|npd)_[a-z0-9_]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b((?:nps|npf|npb|npt|nbz|nbf|nml|npd|npp)_[a-z0-9_]+)\s*\(", src)))
 
 
 def test_header_symbols_exported():

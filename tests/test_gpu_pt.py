@@ -1,0 +1,126 @@
+"""Path-traced target generator on the GPU (npp_*, GpuPathTracePipelineManager)
+against the path-trace oracle: the BVH as built (nodes and reordered
+triangles), the accumulator after every frame and the resolved rgba8 output,
+bit for bit; the environment background; atan2/asin of the shared definition;
+set-output-size and reset semantics; the error paths."""
+import numpy as np
+import pytest
+
+from nprush_amd import camera, meshes
+from nprush_amd import _lib
+from nprush_amd.managers import GpuPathTracePipelineManager, debug_math
+
+pytestmark = pytest.mark.gpu
+
+
+def _flat(ms):
+    vs, ids, off = [], [], 0
+    for v, i in ms:
+        vs.append(np.asarray(v, np.float32).reshape(-1, 12))
+        ids.append(np.asarray(i, np.uint32).reshape(-1, 3) + np.uint32(off))
+        off += vs[-1].shape[0]
+    return np.concatenate(vs), np.concatenate(ids)
+
+
+def _cam(W, H, eye=(3.0, -2.5, 1.8)):
+    vp, vi = camera.view_proj(eye, target=(0.2, 0.0, 0.6), up=(0.0, 0.0, 1.0), aspect=W / H)
+    return vi
+
+
+@pytest.mark.parametrize("W,H,subdiv", [(64, 48, 2), (100, 60, 3)])
+def test_pt_frames_bit_exact(oracle, W, H, subdiv):
+    ms = meshes.scene(subdiv=subdiv, seed=1)
+    env = meshes.environment(64, 32)
+    v, idx = _flat(ms)
+    pt = GpuPathTracePipelineManager(envTexture=env)
+    pt.setMeshes(ms)
+    nodes, tris = pt.read_bvh()
+    on, ot = oracle.pt_bvh(v, idx)
+    assert np.array_equal(nodes, on)
+    assert np.array_equal(tris, ot)
+    vi = _cam(W, H)
+    pt.writeInvViewProjMat(vi)
+    pt.setOutputSize(W, H)
+    acc = None
+    for f in range(3):
+        pt.dispatch()
+        acc = oracle.pt_trace(vi, f, W, H, v, on, ot, env, accum=acc)
+        got = pt.read_accum()
+        assert np.array_equal(got, acc), (f, int((got != acc).sum()))
+        assert np.array_equal(pt.read_output(), oracle.pt_resolve(acc))
+    # paths hit the scene (not only the environment): diffuse bounces tint them
+    assert acc[..., :3].std() > 0.0
+    pt.close()
+
+
+def test_pt_reset_resize_and_empty_scene(oracle):
+    W, H = 40, 24
+    E = np.array([0.25, 0.5, 2.0], np.float32)
+    env = meshes.constant_environment(E)
+    pt = GpuPathTracePipelineManager(envTexture=env)
+    pt.setMeshes([])
+    nodes, tris = pt.read_bvh()
+    assert nodes.shape[0] == 1 and tris.shape[0] == 0
+    vi = _cam(W, H)
+    pt.writeInvViewProjMat(vi)
+    pt.setOutputSize(W, H)
+    pt.dispatch()
+    pt.dispatch()
+    acc = pt.read_accum()
+    assert np.all(acc[..., :3] == 2 * E) and np.all(acc[..., 3] == 2.0)
+    pt.reset()                                   # clears the accumulator and the frame count
+    assert np.all(pt.read_accum() == 0.0)
+    pt.dispatch()
+    assert np.all(pt.read_accum()[..., 3] == 1.0)
+    pt.setOutputSize(W, H)                       # same size: nothing changes
+    assert np.all(pt.read_accum()[..., 3] == 1.0)
+    pt.setOutputSize(W + 8, H)                   # new size: cleared
+    assert np.all(pt.read_accum() == 0.0)
+    # the frame count restarted: frame 0 of the oracle on the new size
+    ms = meshes.scene(subdiv=1, seed=2)
+    pt.setMeshes(ms)
+    v, idx = _flat(ms)
+    on, ot = oracle.pt_bvh(v, idx)
+    pt.dispatch()
+    assert np.array_equal(pt.read_accum(), oracle.pt_trace(vi, 0, W + 8, H, v, on, ot, env))
+    pt.close()
+
+
+def test_pt_environment_background_and_math(oracle):
+    env = meshes.environment(48, 24)
+    pt = GpuPathTracePipelineManager(envTexture=env)
+    for W, H, eye in [(33, 17, (2.0, 1.0, 0.5)), (16, 16, (0.0, -3.0, 3.0))]:
+        vi = _cam(W, H, eye)
+        assert np.array_equal(pt.env_background(vi, W, H), oracle.env_background(vi, env, W, H))
+    pt.close()
+    rng = np.random.default_rng(3)
+    y = rng.normal(size=4096).astype(np.float32)
+    x = rng.normal(size=4096).astype(np.float32)
+    y[:4] = [0.0, 1.0, -1.0, 0.0]
+    x[:4] = [0.0, 0.0, -1.0, -2.0]
+    a_ref, _ = oracle.atan2_asin_f32(y, x)
+    assert np.array_equal(debug_math(8, y, x), a_ref)
+    z = np.clip(rng.normal(size=4096), -1, 1).astype(np.float32)
+    _, s_ref = oracle.atan2_asin_f32(z, np.ones_like(z))
+    assert np.array_equal(debug_math(9, z), s_ref)
+
+
+def test_pt_error_paths():
+    pt = GpuPathTracePipelineManager()
+    with pytest.raises(_lib.NpsError):
+        pt.dispatch()                            # no meshes, no environment, no size
+    pt.setMeshes(meshes.scene(subdiv=0))
+    with pytest.raises(_lib.NpsError):
+        pt.dispatch()                            # no environment
+    pt.setEnvTexture(meshes.constant_environment())
+    with pytest.raises(_lib.NpsError):
+        pt.dispatch()                            # no output size
+    with pytest.raises(_lib.NpsError):
+        pt.setOutputSize(0, 4)
+    bad_v = np.zeros((3, 12), np.float32)
+    with pytest.raises(_lib.NpsError):
+        pt.setMeshes([(bad_v, np.array([[0, 1, 3]], np.uint32))])   # index out of range
+    pt.setOutputSize(8, 8)
+    pt.dispatch()
+    assert pt.read_output().shape == (8, 8, 4)
+    pt.close()
