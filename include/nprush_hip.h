@@ -525,6 +525,23 @@ int npp_read_bvh(npp_handle p, uint32_t* nodes, uint32_t* tris, int32_t counts[2
  * Reinhard(4 env(dir)) per pixel centre, on the handle's environment. */
 int npp_env_background(npp_handle p, const float inv_vp[16], float* rgba, int32_t w, int32_t hgt);
 
+/* Mesh raster target: replaces GpuMeshRenderPipelineManager's draw
+ * (src/lib/gpu/GpuMeshRenderPipelineManager.ts:246-280, mesh.wgsl:33-74) over
+ * the envmap background, on the handle's meshes (npp_set_meshes) with the
+ * handle's environment as the matcap texture (GpuRunner passes the same
+ * texture to both, GpuRunner.svelte.ts:163-171).  Rasterisation is restated per
+ * pixel centre: the closest front-facing (counter-clockwise) triangle along the
+ * centre's ray, its varyings blended with the hit's barycentric weights (the
+ * perspective-correct weights there); coverage on shared edges and hardware
+ * interpolation rounding are parity unpinned.  Writes W x H rgba8 values
+ * (k/255): colour = Reinhard(4 c) with c the normal colour (shading < 0.5) or
+ * matcap x vertex colour, and depth (d, d, d, 1), d the blended
+ * 1 - 0.1 / max(w, 0.1), or the envmap colour and depth 1 where no triangle is.
+ * The ground mesh's PBR variant (frag_pbr) needs texture assets the reference
+ * does not ship and is not built. */
+int npp_raster(npp_handle p, const float vp[16], const float view[16], const float inv_vp[16], float shading_mode,
+               float* color, float* depth, int32_t w, int32_t hgt);
+
 /* Device buffers on the handle's GPU for callers without a GPU allocator of
  * their own (Deno, ctypes), e.g. to keep several targets resident in HBM for
  * nps_set_target_dev.  No reference counterpart (WebGPU's createBuffer). */

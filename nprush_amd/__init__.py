@@ -10,6 +10,7 @@ from .managers import (  # noqa: F401
     GpuBezierForwardPipelineManager,
     GpuBezierOptimizerManager,
     GpuBlurPipelineManager,
+    GpuMeshRenderPipelineManager,
     GpuMultiViewStep,
     GpuPathTracePipelineManager,
     GpuSplatForwardPipelineManager,

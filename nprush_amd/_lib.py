@@ -106,6 +106,7 @@ SIGNATURES = {
     "npp_output_dev": (i32, [_P, C.POINTER(_P)]),
     "npp_read_bvh": (i32, [_P, _U, _U, _I]),
     "npp_env_background": (i32, [_P, _F, _F, i32, i32]),
+    "npp_raster": (i32, [_P, _F, _F, _F, C.c_float, _F, _F, i32, i32]),
     "npb_create": (i32, [i32, C.POINTER(_P)]),
     "npb_destroy": (None, [_P]),
     "npb_last_error": (C.c_char_p, [_P]),

@@ -65,11 +65,13 @@ __device__ __forceinline__ float2 rand2(uint32_t& seed) {
     return make_float2((float)a / 4294967296.0f, (float)b / 4294967296.0f);
 }
 
+// Closest hit of a walk: distance, barycentric weights and the triangle's
+// vertices (its attributes are blended once the walk is over; the blend of the
+// closest hit is the shader's, which blends at every accepted hit).
 struct PtHit {
     bool hit;
-    float t;
-    V3 n;
-    float4 c;
+    float t, u, v, w;
+    uint32_t i0, i1, i2;
 };
 
 __device__ __forceinline__ void load_vert(const float4* __restrict__ vt, uint32_t i, V3& p, V3& n, float4& c) {
@@ -78,17 +80,20 @@ __device__ __forceinline__ void load_vert(const float4* __restrict__ vt, uint32_
     n = v3(a.w, b.x, b.y);
     c = make_float4(b.z, b.w, d.x, d.y);
 }
+__device__ __forceinline__ V3 load_pos(const float4* __restrict__ vt, uint32_t i) {
+    const float4 a = vt[3u * i];
+    return v3(a.x, a.y, a.z);
+}
 
 // Watertight ray/triangle test (path_trace.wgsl:60-129): shear to the ray's
 // dominant axis, scaled barycentrics, hit if all of U, V, W share a sign,
-// 1e-4 <= t < t_max.
+// 1e-4 <= t < t_max.  kCull: a triangle whose ccw normal (p1 - p0) x (p2 - p0)
+// does not face the ray (dot >= 0) is skipped (the raster's back-face cull).
+template <bool kCull>
 __device__ __forceinline__ void intersect_tri(V3 ro, V3 rd, const float4* __restrict__ vt, uint32_t i0, uint32_t i1,
                                               uint32_t i2, float t_max, PtHit& best) {
-    V3 p0, p1, p2, n0, n1, n2;
-    float4 c0, c1, c2;
-    load_vert(vt, i0, p0, n0, c0);
-    load_vert(vt, i1, p1, n1, c1);
-    load_vert(vt, i2, p2, n2, c2);
+    const V3 p0 = load_pos(vt, i0), p1 = load_pos(vt, i1), p2 = load_pos(vt, i2);
+    if (kCull && !(dot3(cross3(sub(p1, p0), sub(p2, p0)), rd) < 0.0f)) return;
     const float ax = fabsf(rd.x), ay = fabsf(rd.y), az = fabsf(rd.z);
     uint32_t kz = 0u;
     if (ay > ax) kz = 1u;
@@ -114,12 +119,14 @@ __device__ __forceinline__ void intersect_tri(V3 ro, V3 rd, const float4* __rest
     const float inv_det = 1.0f / det;
     const float t = T * inv_det;
     if (t < 1e-4f || t >= t_max) return;
-    const float u = U * inv_det, v = V * inv_det, w = W * inv_det;
     best.hit = true;
     best.t = t;
-    best.n = normalize3(add(add(mul(n0, u), mul(n1, v)), mul(n2, w)));
-    best.c = make_float4((u * c0.x + v * c1.x) + w * c2.x, (u * c0.y + v * c1.y) + w * c2.y,
-                         (u * c0.z + v * c1.z) + w * c2.z, (u * c0.w + v * c1.w) + w * c2.w);
+    best.u = U * inv_det;
+    best.v = V * inv_det;
+    best.w = W * inv_det;
+    best.i0 = i0;
+    best.i1 = i1;
+    best.i2 = i2;
 }
 
 // Slab test (path_trace.wgsl:132-142) with WGSL min/max (a NaN operand loses).
@@ -132,13 +139,14 @@ __device__ __forceinline__ bool aabb_hit(V3 ro, V3 inv, const float4& mn, const 
 }
 
 // Closest hit (path_trace.wgsl:146-185).
+template <bool kCull>
 __device__ PtHit scene_hit(V3 ro, V3 rd, const float4* __restrict__ nodes, const uint32_t* __restrict__ tris,
                            const float4* __restrict__ vt, uint32_t nnodes) {
     PtHit best;
     best.hit = false;
     best.t = 1e30f;
-    best.n = v3(0.f, 0.f, 0.f);
-    best.c = make_float4(0.f, 0.f, 0.f, 0.f);
+    best.u = best.v = best.w = 0.0f;
+    best.i0 = best.i1 = best.i2 = 0u;
     if (nnodes == 0u) return best;
     const V3 inv = v3(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
     uint32_t stack[kPtMaxStack];
@@ -153,7 +161,7 @@ __device__ PtHit scene_hit(V3 ro, V3 rd, const float4* __restrict__ nodes, const
             const uint32_t count = d1 & ~kPtLeafFlag;
             for (uint32_t k = 0u; k < count; ++k) {
                 const uint32_t b = 3u * (d0 + k);
-                intersect_tri(ro, rd, vt, tris[b], tris[b + 1u], tris[b + 2u], best.t, best);
+                intersect_tri<kCull>(ro, rd, vt, tris[b], tris[b + 1u], tris[b + 2u], best.t, best);
             }
         } else {
             if (top + 1u < kPtMaxStack) stack[top++] = d1;
@@ -161,6 +169,19 @@ __device__ PtHit scene_hit(V3 ro, V3 rd, const float4* __restrict__ nodes, const
         }
     }
     return best;
+}
+
+// The hit's blended normal (normalised) and colour (path_trace.wgsl:126-127).
+__device__ __forceinline__ void hit_attributes(const PtHit& h, const float4* __restrict__ vt, V3& n, float4& c) {
+    V3 p0, p1, p2, n0, n1, n2;
+    float4 c0, c1, c2;
+    load_vert(vt, h.i0, p0, n0, c0);
+    load_vert(vt, h.i1, p1, n1, c1);
+    load_vert(vt, h.i2, p2, n2, c2);
+    const float u = h.u, v = h.v, w = h.w;
+    n = normalize3(add(add(mul(n0, u), mul(n1, v)), mul(n2, w)));
+    c = make_float4((u * c0.x + v * c1.x) + w * c2.x, (u * c0.y + v * c1.y) + w * c2.y,
+                    (u * c0.z + v * c1.z) + w * c2.z, (u * c0.w + v * c1.w) + w * c2.w);
 }
 
 // Equirectangular environment, Z up (path_trace.wgsl:190-196): bilinear filtering
@@ -205,17 +226,20 @@ __global__ __launch_bounds__(256) void k_pt_trace(PtArgs a) {
     V3 rd = normalize3(sub(far, ro));
     V3 rad = v3(0.f, 0.f, 0.f), thr = v3(1.f, 1.f, 1.f), o = ro;
     for (int bounce = 0; bounce < 8; ++bounce) {
-        const PtHit h = scene_hit(o, rd, a.nodes, a.tris, a.verts, a.nnodes);
+        const PtHit h = scene_hit<false>(o, rd, a.nodes, a.tris, a.verts, a.nnodes);
         if (!h.hit) {
             rad = add(rad, mulv(thr, sample_env(a.env, a.ew, a.eh, rd)));
             break;
         }
-        const V3 n = dot3(h.n, rd) > 0.0f ? v3(-h.n.x, -h.n.y, -h.n.z) : h.n;
+        V3 hn;
+        float4 hc;
+        hit_attributes(h, a.verts, hn, hc);
+        const V3 n = dot3(hn, rd) > 0.0f ? v3(-hn.x, -hn.y, -hn.z) : hn;
         o = add(add(o, mul(rd, h.t)), mul(n, 5e-4f));
-        if (h.c.w < 0.5f) {
+        if (hc.w < 0.5f) {
             rd = sub(rd, mul(n, 2.0f * dot3(n, rd)));  // reflect(rd, n)
         } else {
-            thr = mulv(thr, v3(h.c.x, h.c.y, h.c.z));
+            thr = mulv(thr, v3(hc.x, hc.y, hc.z));
             rd = cosine_hemisphere(n, seed);
         }
     }
@@ -251,6 +275,76 @@ __global__ __launch_bounds__(256) void k_env_background(EnvArgs a) {
     const V3 c = sample_env(a.env, a.ew, a.eh, normalize3(sub(fw, nw)));
     a.out[(size_t)py * a.W + px] = make_float4(pt_unorm8(pt_reinhard(c.x * 4.0f)), pt_unorm8(pt_reinhard(c.y * 4.0f)),
                                                pt_unorm8(pt_reinhard(c.z * 4.0f)), 1.0f);
+}
+
+// clip-space w of a point (row 3 of the column-major VP)
+__device__ __forceinline__ float clip_w(const float* m, V3 p) {
+    return ((m[3] * p.x + m[7] * p.y) + m[11] * p.z) + m[15] * 1.0f;
+}
+// view-space direction of a normal (view * vec4(n, 0)).xyz
+__device__ __forceinline__ V3 view_dir(const float* m, V3 n) {
+    float r[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) r[i] = ((m[i] * n.x + m[4 + i] * n.y) + m[8 + i] * n.z) + m[12 + i] * 0.0f;
+    return v3(r[0], r[1], r[2]);
+}
+__device__ __forceinline__ float view_depth(float w) {
+    return clamp_w(1.0f - 0.1f / fmax_w(w, 0.1f), 0.0f, 1.0f);  // mesh.wgsl:42-43
+}
+
+// Per pixel: the envmap background, then the closest front-facing triangle
+// along the pixel centre's ray shaded as mesh.wgsl frag (:55-74): normals
+// colour (normalize(n) + 1) / 2, or the matcap at the view normal's xy (v
+// flipped) times the vertex colour; Reinhard(4 c); depth the blended vertex
+// depths 1 - 0.1 / max(w, 0.1).  No triangle: background colour, depth 1.
+__global__ __launch_bounds__(256) void k_mesh_raster(RasterArgs a) {
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    const uint32_t px = blockIdx.x * 16u + (wid & 1u) * 8u + (lane & 7u);
+    const uint32_t py = blockIdx.y * 16u + (wid >> 1) * 8u + (lane >> 3);
+    if (px >= a.W || py >= a.H) return;
+    const float nx = (((float)px + 0.5f) / (float)a.W) * 2.0f - 1.0f;
+    const float ny = 1.0f - (((float)py + 0.5f) / (float)a.H) * 2.0f;
+    const V3 nw = unproject(a.ivp, nx, ny, 0.0f), fw = unproject(a.ivp, nx, ny, 1.0f);
+    const V3 rd = normalize3(sub(fw, nw));
+    const size_t pix = (size_t)py * a.W + px;
+    const PtHit h = scene_hit<true>(nw, rd, a.nodes, a.tris, a.verts, a.nnodes);
+    if (!h.hit) {
+        const V3 c = sample_env(a.env, a.ew, a.eh, rd);
+        a.color[pix] = make_float4(pt_unorm8(pt_reinhard(c.x * 4.0f)), pt_unorm8(pt_reinhard(c.y * 4.0f)),
+                                   pt_unorm8(pt_reinhard(c.z * 4.0f)), 1.0f);
+        a.depth[pix] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        return;
+    }
+    V3 p[3], n[3];
+    float4 c[3];
+    load_vert(a.verts, h.i0, p[0], n[0], c[0]);
+    load_vert(a.verts, h.i1, p[1], n[1], c[1]);
+    load_vert(a.verts, h.i2, p[2], n[2], c[2]);
+    const float u = h.u, v = h.v, w = h.w;
+    const float d = (u * view_depth(clip_w(a.vp, p[0])) + v * view_depth(clip_w(a.vp, p[1]))) +
+                    w * view_depth(clip_w(a.vp, p[2]));
+    const V3 nn = add(add(mul(n[0], u), mul(n[1], v)), mul(n[2], w));
+    V3 fc;
+    if (a.shading < 0.5f) {
+        const V3 q = normalize3(nn);
+        fc = v3((q.x + 1.0f) * 0.5f, (q.y + 1.0f) * 0.5f, (q.z + 1.0f) * 0.5f);
+    } else {
+        const V3 vn = normalize3(
+            add(add(mul(view_dir(a.view, n[0]), u), mul(view_dir(a.view, n[1]), v)), mul(view_dir(a.view, n[2]), w)));
+        const float mu = vn.x * 0.5f + 0.5f, mv = vn.y * 0.5f + 0.5f;
+        const float3 m = pt_env_bilinear_raw<false>(a.env, a.ew, a.eh, mu, 1.0f - mv);
+        fc = v3(m.x * ((u * c[0].x + v * c[1].x) + w * c[2].x), m.y * ((u * c[0].y + v * c[1].y) + w * c[2].y),
+                m.z * ((u * c[0].z + v * c[1].z) + w * c[2].z));
+    }
+    a.color[pix] = make_float4(pt_unorm8(pt_reinhard(fc.x * 4.0f)), pt_unorm8(pt_reinhard(fc.y * 4.0f)),
+                               pt_unorm8(pt_reinhard(fc.z * 4.0f)), 1.0f);
+    const float dq = pt_unorm8(d);
+    a.depth[pix] = make_float4(dq, dq, dq, 1.0f);
+}
+
+void launch_mesh_raster(hipStream_t st, const RasterArgs& a) {
+    if (a.W == 0 || a.H == 0) return;
+    hipLaunchKernelGGL(k_mesh_raster, dim3((a.W + 15) / 16, (a.H + 15) / 16), dim3(256), 0, st, a);
 }
 
 void launch_env_background(hipStream_t st, const float* ivp, const float4* env, int ew, int eh, uint32_t W,

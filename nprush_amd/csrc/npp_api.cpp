@@ -197,7 +197,7 @@ struct npp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    DevBuf verts, nodes, tris, env, accum, out, bg;
+    DevBuf verts, nodes, tris, env, accum, out, bg, rc, rd;
     std::vector<uint32_t> host_nodes, host_tris;
     bool have_meshes = false, have_env = false;
     int ew = 0, eh = 0;
@@ -241,7 +241,7 @@ void npp_destroy(npp_handle p) {
     (void)hipSetDevice(p->device);
     (void)hipStreamSynchronize(p->stream);
     free_buf(p->verts); free_buf(p->nodes); free_buf(p->tris); free_buf(p->env);
-    free_buf(p->accum); free_buf(p->out); free_buf(p->bg);
+    free_buf(p->accum); free_buf(p->out); free_buf(p->bg); free_buf(p->rc); free_buf(p->rd);
     (void)hipStreamDestroy(p->stream);
     delete p;
 }
@@ -395,6 +395,40 @@ int npp_env_background(npp_handle p, const float inv_vp[16], float* rgba, int32_
                           (float4*)p->bg.p);
     NPP_HIP(p, hipGetLastError());
     NPP_HIP(p, hipMemcpyAsync(rgba, p->bg.p, bytes, hipMemcpyDeviceToHost, p->stream));
+    NPP_HIP(p, hipStreamSynchronize(p->stream));
+    return NPS_OK;
+}
+
+int npp_raster(npp_handle p, const float vp[16], const float view[16], const float inv_vp[16], float shading_mode,
+               float* color, float* depth, int32_t w, int32_t hgt) {
+    if (!p || !vp || !view || !inv_vp || !color || !depth || w <= 0 || hgt <= 0) return NPS_ERR_ARG;
+    if (!p->have_meshes) return perr(p, NPS_ERR_STATE, "no meshes (npp_set_meshes)");
+    if (!p->have_env) return perr(p, NPS_ERR_STATE, "no environment / matcap texture (npp_set_env)");
+    NPP_HIP(p, hipSetDevice(p->device));
+    const size_t bytes = 16 * (size_t)w * hgt;
+    int rc;
+    if ((rc = ensure_dev(p->rc, bytes, p->stream, p->err)) != NPS_OK) return rc;
+    if ((rc = ensure_dev(p->rd, bytes, p->stream, p->err)) != NPS_OK) return rc;
+    RasterArgs a;
+    memcpy(a.vp, vp, sizeof(a.vp));
+    memcpy(a.view, view, sizeof(a.view));
+    memcpy(a.ivp, inv_vp, sizeof(a.ivp));
+    a.shading = shading_mode;
+    a.W = (uint32_t)w;
+    a.H = (uint32_t)hgt;
+    a.nnodes = (uint32_t)(p->host_nodes.size() / 8);
+    a.verts = (const float4*)p->verts.p;
+    a.nodes = (const float4*)p->nodes.p;
+    a.tris = (const uint32_t*)p->tris.p;
+    a.env = (const float4*)p->env.p;
+    a.ew = p->ew;
+    a.eh = p->eh;
+    a.color = (float4*)p->rc.p;
+    a.depth = (float4*)p->rd.p;
+    launch_mesh_raster(p->stream, a);
+    NPP_HIP(p, hipGetLastError());
+    NPP_HIP(p, hipMemcpyAsync(color, p->rc.p, bytes, hipMemcpyDeviceToHost, p->stream));
+    NPP_HIP(p, hipMemcpyAsync(depth, p->rd.p, bytes, hipMemcpyDeviceToHost, p->stream));
     NPP_HIP(p, hipStreamSynchronize(p->stream));
     return NPS_OK;
 }

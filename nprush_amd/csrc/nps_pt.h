@@ -47,16 +47,20 @@ __device__ __forceinline__ float pt_tex_coord(float x) {
     return fminf(fmaxf(x, -1048576.0f), 1048576.0f);
 }
 
-// Environment texel filter (linear, repeat in u, clamp-to-edge in v, one mip
-// level; GpuPathTracePipelineManager.ts sampler) as FP32 bilinear filtering of
-// the texels at x = u ew - 1/2, y = v eh - 1/2.
+// Linear texture filtering (one mip level) as FP32 bilinear filtering of the
+// texels at x = u ew - 1/2, y = v eh - 1/2, clamp-to-edge in v and, with
+// kRepeatU, repeat in u (the path tracer's environment sampler,
+// GpuPathTracePipelineManager.ts:205-208), else clamp-to-edge (the mesh's
+// matcap sampler, GpuMeshRenderPipelineManager.ts:254).
+template <bool kRepeatU = true>
 __device__ __forceinline__ float3 pt_env_bilinear_raw(const float4* __restrict__ env, int ew, int eh, float u,
                                                       float v) {
     const float x = pt_tex_coord(u * (float)ew - 0.5f), y = pt_tex_coord(v * (float)eh - 0.5f);
     const float x0f = floorf(x), y0f = floorf(y);
     const float fx = x - x0f, fy = y - y0f;
     const int ix = (int)x0f, iy = (int)y0f;
-    const int x0 = ((ix % ew) + ew) % ew, x1 = (((ix + 1) % ew) + ew) % ew;
+    const int x0 = kRepeatU ? ((ix % ew) + ew) % ew : min(max(ix, 0), ew - 1);
+    const int x1 = kRepeatU ? (((ix + 1) % ew) + ew) % ew : min(max(ix + 1, 0), ew - 1);
     const int y0 = min(max(iy, 0), eh - 1), y1 = min(max(iy + 1, 0), eh - 1);
     const float4 t00 = env[(size_t)y0 * ew + x0], t10 = env[(size_t)y0 * ew + x1];
     const float4 t01 = env[(size_t)y1 * ew + x0], t11 = env[(size_t)y1 * ew + x1];
@@ -77,7 +81,26 @@ __device__ __forceinline__ float pt_unorm8(float x) {
     return rintf(x * 255.0f) / 255.0f;
 }
 
+// Mesh raster target (mesh.wgsl vert/frag, GpuMeshRenderPipelineManager's
+// pipeline: back faces culled, depth test "less", over the envmap background),
+// restated per pixel centre as the closest front-facing triangle along the
+// centre's ray, with the varyings blended by its barycentric weights (the
+// perspective-correct interpolation weights at that point).
+struct RasterArgs {
+    float vp[16], view[16], ivp[16];
+    float shading;              // < 0.5: normal colours, else matcap x vertex colour
+    uint32_t W, H, nnodes;
+    const float4* verts;
+    const float4* nodes;
+    const uint32_t* tris;
+    const float4* env;          // the matcap / environment texture
+    int ew, eh;
+    float4* color;              // W x H rgba8 values (k/255)
+    float4* depth;              // W x H: (d, d, d, 1), d = 1 - 0.1 / max(w, 0.1) blended, rgba8
+};
+
 void launch_pt_trace(hipStream_t st, const PtArgs& a);
+void launch_mesh_raster(hipStream_t st, const RasterArgs& a);
 void launch_pt_resolve(hipStream_t st, const float4* accum, uint32_t npix, float4* out);
 void launch_env_background(hipStream_t st, const float* ivp, const float4* env, int ew, int eh, uint32_t W,
                            uint32_t H, float4* out);

@@ -616,6 +616,91 @@ class GpuPathTracePipelineManager:
         return out
 
 
+class GpuMeshRenderPipelineManager:
+    """new GpuMeshRenderPipelineManager({device, format, uniformsManager, mesh})
+    (src/lib/gpu/GpuMeshRenderPipelineManager.ts:17-280): the raster target of
+    the reference's frame (colour and encoded depth of the mesh over the envmap
+    background).  mesh and setGroundMesh take (vertices [n, 12], indices [m, 3]);
+    the matcap texture is an H x W x 4 float array (GpuRunner passes its
+    environment panorama).  The uniforms of GpuUniformsBufferManager (view-
+    projection, view matrix, shading mode, inverse view-projection) are given
+    with writeUniforms.  setPbrMesh's textured ground is not built (its texture
+    assets are not shipped with the reference)."""
+
+    def __init__(self, device=None, format=None, uniformsManager=None, mesh=None, *,
+                 matcapTexture: np.ndarray | None = None, hip_device: int = 0):
+        lib = _lib.load()
+        p = C.c_void_p()
+        check(lib.npp_create(int(hip_device), C.byref(p)), None, "npp_create")
+        self._lib = lib
+        self._p = p
+        self._meshes = [] if mesh is None else [mesh]
+        self._ground = None
+        self._u = None
+        if mesh is not None:
+            self._upload()
+        if matcapTexture is not None:
+            self.setMatcapTexture(matcapTexture)
+
+    def _ok(self, rc, what):
+        if rc != 0:
+            m = self._lib.npp_last_error(self._p)
+            raise _lib.NpsError(f"{what} failed ({rc}): {m.decode() if m else ''}")
+
+    def close(self):
+        if getattr(self, "_p", None):
+            self._lib.npp_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _upload(self):
+        ms = list(self._meshes) + ([self._ground] if self._ground is not None else [])
+        vs, ids, off = [], [], 0
+        for v, i in ms:
+            v = np.asarray(v, np.float32).reshape(-1, 12)
+            vs.append(v)
+            ids.append(np.asarray(i, np.uint32).reshape(-1) + np.uint32(off))
+            off += v.shape[0]
+        v = np.ascontiguousarray(np.concatenate(vs) if vs else np.zeros((0, 12), np.float32))
+        i = np.ascontiguousarray(np.concatenate(ids) if ids else np.zeros(0, np.uint32))
+        self._ok(self._lib.npp_set_meshes(self._p, fptr(v), int(v.shape[0]), i.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                          int(i.size // 3)), "npp_set_meshes")
+
+    def setGroundMesh(self, mesh):
+        self._ground = mesh
+        self._upload()
+
+    def setPbrMesh(self, *args, **kwargs):
+        raise NotImplementedError("the textured (PBR) ground needs texture assets the reference does not ship")
+
+    def setMatcapTexture(self, tex: np.ndarray):
+        e = np.ascontiguousarray(tex, dtype=np.float32)
+        self._ok(self._lib.npp_set_env(self._p, fptr(e), int(e.shape[1]), int(e.shape[0])), "npp_set_env")
+
+    def writeUniforms(self, viewProjMat, viewMat, invViewProjMat, shadingMode: float = 1.0):
+        self._u = tuple(np.ascontiguousarray(np.asarray(m, np.float32).reshape(16))
+                        for m in (viewProjMat, viewMat, invViewProjMat)) + (float(shadingMode),)
+
+    def addDraw(self, renderPassEncoder=None, matcapTextureView=None, width: int = 0, height: int = 0):
+        """The envmap pass and this manager's draws into a cleared W x H pass:
+        returns (colour, depth), each H x W x 4 rgba8 texel values."""
+        if self._u is None:
+            raise _lib.NpsError("no uniforms (writeUniforms)")
+        if matcapTextureView is not None:
+            self.setMatcapTexture(matcapTextureView)
+        vp, view, ivp, mode = self._u
+        col = np.empty((height, width, 4), np.float32)
+        dep = np.empty((height, width, 4), np.float32)
+        self._ok(self._lib.npp_raster(self._p, fptr(vp), fptr(view), fptr(ivp), mode, fptr(col), fptr(dep),
+                                      int(width), int(height)), "npp_raster")
+        return col, dep
+
+
 def reference_bezier_init(num_beziers: int, seed: int = 0) -> np.ndarray:
     """The reference's initial curves (GpuBezierOptimizerManager.ts:56-95: short
     randomly oriented squiggles near the origin, width 0.02, softness 0.005,

@@ -593,3 +593,26 @@ def atan2_asin_f32(y, x):
     s = np.empty_like(y)
     lib.ora_atan2_asin(_p(y, _F), _p(x, _F), y.size, _p(a, _F), _p(s, _F))
     return a, s
+
+
+def mesh_raster(vp, view, inv_vp, shading, W, H, verts, nodes, tris, env):
+    """mesh.wgsl over the envmap background per pixel centre: (colour, depth),
+    each (H, W, 4) rgba8 values."""
+    lib = load()
+    _pt_sig(lib)
+    if not getattr(lib, "_raster_ready", False):
+        lib.ora_mesh_raster.argtypes = [_F, _F, _F, C.c_float, i32, i32, _F, _U, i32, _U, _F, i32, i32, _F, _F]
+        lib.ora_mesh_raster.restype = None
+        lib._raster_ready = True
+    v = f32(verts).reshape(-1, 12)
+    nd = np.ascontiguousarray(nodes, dtype=np.uint32)
+    tr = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1)
+    if tr.size == 0:
+        tr = np.zeros(3, np.uint32)
+    e = f32(env)
+    col = np.empty((H, W, 4), np.float32)
+    dep = np.empty((H, W, 4), np.float32)
+    lib.ora_mesh_raster(_p(f32(vp).reshape(16), _F), _p(f32(view).reshape(16), _F), _p(f32(inv_vp).reshape(16), _F),
+                        float(shading), W, H, _p(v, _F), _p(nd, _U), nd.shape[0], _p(tr, _U), _p(e, _F), e.shape[1],
+                        e.shape[0], _p(col, _F), _p(dep, _F))
+    return col, dep

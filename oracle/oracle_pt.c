@@ -206,10 +206,18 @@ static void prand2(uint32_t* seed, float* a, float* b) {
     *b = (float)y / 4294967296.0f;
 }
 
-typedef struct { int hit; float t; f3 n; float c[4]; } phit;
+typedef struct { int hit; float t; f3 n; float c[4]; float bu, bv, bw; uint32_t tri[3]; } phit;
 
-static void ptri_test(f3 ro, f3 rd, const float* V, uint32_t i0, uint32_t i1, uint32_t i2, float tmax, phit* best) {
+/* cull: skip a triangle whose ccw normal (p1 - p0) x (p2 - p0) does not face
+ * the ray (the raster's back-face cull, GpuMeshRenderPipelineManager.ts:99-103) */
+static void ptri_test(f3 ro, f3 rd, const float* V, uint32_t i0, uint32_t i1, uint32_t i2, float tmax, phit* best,
+                      int cull) {
     const float* q[3] = { V + 12 * (size_t)i0, V + 12 * (size_t)i1, V + 12 * (size_t)i2 };
+    if (cull) {
+        f3 e1 = vsub(mk(q[1][0], q[1][1], q[1][2]), mk(q[0][0], q[0][1], q[0][2]));
+        f3 e2 = vsub(mk(q[2][0], q[2][1], q[2][2]), mk(q[0][0], q[0][1], q[0][2]));
+        if (!(vdot(vcross(e1, e2), rd) < 0.0f)) return;
+    }
     float ax = fabsf(rd.x), ay = fabsf(rd.y), az = fabsf(rd.z);
     int kz = 0;
     if (ay > ax) kz = 1;
@@ -239,6 +247,8 @@ static void ptri_test(f3 ro, f3 rd, const float* V, uint32_t i0, uint32_t i1, ui
     best->t = t;
     best->n = vnorm(vadd(vadd(vscale(n0, u), vscale(n1, v)), vscale(n2, w)));
     for (int k = 0; k < 4; ++k) best->c[k] = (u * q[0][6 + k] + v * q[1][6 + k]) + w * q[2][6 + k];
+    best->bu = u; best->bv = v; best->bw = w;
+    best->tri[0] = i0; best->tri[1] = i1; best->tri[2] = i2;
 }
 
 static int pslab(f3 ro, f3 inv, const uint32_t* nd, float tmax) {
@@ -252,7 +262,7 @@ static int pslab(f3 ro, f3 inv, const uint32_t* nd, float tmax) {
     return tm >= 0.0f && tm >= tmin && tmin < tmax;
 }
 
-static phit pscene(f3 ro, f3 rd, const uint32_t* nodes, int nn, const uint32_t* tris, const float* V) {
+static phit pscene(f3 ro, f3 rd, const uint32_t* nodes, int nn, const uint32_t* tris, const float* V, int cull) {
     phit best;
     memset(&best, 0, sizeof(best));
     best.t = 1e30f;
@@ -269,7 +279,7 @@ static phit pscene(f3 ro, f3 rd, const uint32_t* nodes, int nn, const uint32_t* 
             uint32_t cnt = nd[7] & ~PT_LEAF;
             for (uint32_t k = 0; k < cnt; ++k) {
                 size_t b = 3 * ((size_t)nd[3] + k);
-                ptri_test(ro, rd, V, tris[b], tris[b + 1], tris[b + 2], best.t, &best);
+                ptri_test(ro, rd, V, tris[b], tris[b + 1], tris[b + 2], best.t, &best, cull);
             }
         } else {
             if (top + 1 < PT_STACK) st[top++] = nd[7];
@@ -285,12 +295,19 @@ static float ptexc(float x) {
     return x < -1048576.0f ? -1048576.0f : (x > 1048576.0f ? 1048576.0f : x);
 }
 
-static f3 penv_bilinear(const float* env, int ew, int eh, float u, float v) {
+static f3 penv_bilinear_m(const float* env, int ew, int eh, float u, float v, int repeat_u) {
     float x = ptexc(u * (float)ew - 0.5f), y = ptexc(v * (float)eh - 0.5f);
     float x0f = floorf(x), y0f = floorf(y);
     float fx = x - x0f, fy = y - y0f;
     int ix = (int)x0f, iy = (int)y0f;
-    int x0 = ((ix % ew) + ew) % ew, x1 = (((ix + 1) % ew) + ew) % ew;
+    int x0, x1;
+    if (repeat_u) {
+        x0 = ((ix % ew) + ew) % ew;
+        x1 = (((ix + 1) % ew) + ew) % ew;
+    } else {
+        x0 = ix < 0 ? 0 : (ix > ew - 1 ? ew - 1 : ix);
+        x1 = ix + 1 < 0 ? 0 : (ix + 1 > ew - 1 ? ew - 1 : ix + 1);
+    }
     int y0 = iy < 0 ? 0 : (iy > eh - 1 ? eh - 1 : iy);
     int y1 = iy + 1 < 0 ? 0 : (iy + 1 > eh - 1 ? eh - 1 : iy + 1);
     const float* t00 = env + 4 * ((size_t)y0 * ew + x0);
@@ -300,6 +317,10 @@ static f3 penv_bilinear(const float* env, int ew, int eh, float u, float v) {
     float gx = 1.0f - fx, gy = 1.0f - fy, r[3];
     for (int k = 0; k < 3; ++k) r[k] = (t00[k] * gx + t10[k] * fx) * gy + (t01[k] * gx + t11[k] * fx) * fy;
     return mk(r[0], r[1], r[2]);
+}
+
+static f3 penv_bilinear(const float* env, int ew, int eh, float u, float v) {
+    return penv_bilinear_m(env, ew, eh, u, v, 1);
 }
 
 static f3 penv(const float* env, int ew, int eh, f3 d) {
@@ -340,7 +361,7 @@ void ora_pt_trace(const float* ivp, uint32_t frame, int W, int H, const float* v
             f3 rd = vnorm(vsub(fw, ro));
             f3 rad = mk(0, 0, 0), thr = mk(1, 1, 1), o = ro;
             for (int bounce = 0; bounce < 8; ++bounce) {
-                phit h = pscene(o, rd, nodes, nn, tris, verts);
+                phit h = pscene(o, rd, nodes, nn, tris, verts, 0);
                 if (!h.hit) {
                     f3 e = penv(env, ew, eh, rd);
                     rad = vadd(rad, mk(thr.x * e.x, thr.y * e.y, thr.z * e.z));
@@ -407,7 +428,7 @@ void ora_pt_closest_brute(const float* ro3, const float* rd3, const float* verts
         int before = best.hit;
         float tb = best.t;
         ptri_test(ro, rd, verts, indices[3 * (size_t)k], indices[3 * (size_t)k + 1], indices[3 * (size_t)k + 2], best.t,
-                  &best);
+                  &best, 0);
         if (best.hit && (!before || best.t != tb)) which = k;
     }
     *t_out = best.hit ? best.t : -1.0f;
@@ -417,7 +438,7 @@ void ora_pt_closest_brute(const float* ro3, const float* rd3, const float* verts
 /* Closest hit of one ray through the BVH: t (or -1) and the normal/colour. */
 void ora_pt_closest_bvh(const float* ro3, const float* rd3, const float* verts, const uint32_t* nodes, int nn,
                         const uint32_t* tris, float* out8) {
-    phit h = pscene(mk(ro3[0], ro3[1], ro3[2]), mk(rd3[0], rd3[1], rd3[2]), nodes, nn, tris, verts);
+    phit h = pscene(mk(ro3[0], ro3[1], ro3[2]), mk(rd3[0], rd3[1], rd3[2]), nodes, nn, tris, verts, 0);
     out8[0] = h.hit ? h.t : -1.0f;
     out8[1] = h.n.x; out8[2] = h.n.y; out8[3] = h.n.z;
     for (int k = 0; k < 4; ++k) out8[4 + k] = h.c[k];
@@ -437,4 +458,62 @@ void ora_atan2_asin(const float* y, const float* x, int n, float* atan2_out, flo
         if (atan2_out) atan2_out[i] = rm_atan2f(y[i], x[i]);
         if (asin_out) asin_out[i] = rm_asinf(y[i]);
     }
+}
+
+/* ------------------------------------------------------------ mesh raster */
+static float pclipw(const float* m, const float* p) { return ((m[3] * p[0] + m[7] * p[1]) + m[11] * p[2]) + m[15] * 1.0f; }
+static f3 pviewdir(const float* m, const float* n) {
+    float r[3];
+    for (int i = 0; i < 3; ++i) r[i] = ((m[i] * n[0] + m[4 + i] * n[1]) + m[8 + i] * n[2]) + m[12 + i] * 0.0f;
+    return mk(r[0], r[1], r[2]);
+}
+static float pviewdepth(float w) { return wmin(wmax(1.0f - 0.1f / wmax(w, 0.1f), 0.0f), 1.0f); }
+
+/* mesh.wgsl vert/frag over the envmap background, per pixel centre: the
+ * closest front-facing triangle along the centre's ray; varyings blended with
+ * its barycentric weights. */
+void ora_mesh_raster(const float* vp, const float* view, const float* ivp, float shading, int W, int H,
+                     const float* verts, const uint32_t* nodes, int nn, const uint32_t* tris, const float* env, int ew,
+                     int eh, float* color, float* depth) {
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int py = 0; py < H; ++py)
+        for (int px = 0; px < W; ++px) {
+            float nx = (((float)px + 0.5f) / (float)W) * 2.0f - 1.0f;
+            float ny = 1.0f - (((float)py + 0.5f) / (float)H) * 2.0f;
+            f3 nw = unproj(ivp, nx, ny, 0.0f), fw = unproj(ivp, nx, ny, 1.0f);
+            f3 rd = vnorm(vsub(fw, nw));
+            float* oc = color + 4 * ((size_t)py * W + px);
+            float* od = depth + 4 * ((size_t)py * W + px);
+            phit h = pscene(nw, rd, nodes, nn, tris, verts, 1);
+            f3 fc;
+            if (!h.hit) {
+                fc = penv(env, ew, eh, rd);
+                for (int k = 0; k < 3; ++k) od[k] = 1.0f;
+            } else {
+                const float* q[3] = { verts + 12 * (size_t)h.tri[0], verts + 12 * (size_t)h.tri[1], verts + 12 * (size_t)h.tri[2] };
+                float u = h.bu, v = h.bv, w = h.bw;
+                float d = (u * pviewdepth(pclipw(vp, q[0])) + v * pviewdepth(pclipw(vp, q[1]))) +
+                          w * pviewdepth(pclipw(vp, q[2]));
+                float dq = punorm8(d);
+                for (int k = 0; k < 3; ++k) od[k] = dq;
+                if (shading < 0.5f) {
+                    f3 nn3 = vadd(vadd(vscale(mk(q[0][3], q[0][4], q[0][5]), u), vscale(mk(q[1][3], q[1][4], q[1][5]), v)),
+                                  vscale(mk(q[2][3], q[2][4], q[2][5]), w));
+                    f3 qq = vnorm(nn3);
+                    fc = mk((qq.x + 1.0f) * 0.5f, (qq.y + 1.0f) * 0.5f, (qq.z + 1.0f) * 0.5f);
+                } else {
+                    f3 vn = vnorm(vadd(vadd(vscale(pviewdir(view, q[0] + 3), u), vscale(pviewdir(view, q[1] + 3), v)),
+                                       vscale(pviewdir(view, q[2] + 3), w)));
+                    float mu = vn.x * 0.5f + 0.5f, mv = vn.y * 0.5f + 0.5f;
+                    f3 m = penv_bilinear_m(env, ew, eh, mu, 1.0f - mv, 0);
+                    fc = mk(m.x * ((u * q[0][6] + v * q[1][6]) + w * q[2][6]), m.y * ((u * q[0][7] + v * q[1][7]) + w * q[2][7]),
+                            m.z * ((u * q[0][8] + v * q[1][8]) + w * q[2][8]));
+                }
+            }
+            oc[0] = punorm8(preinhard(fc.x * 4.0f));
+            oc[1] = punorm8(preinhard(fc.y * 4.0f));
+            oc[2] = punorm8(preinhard(fc.z * 4.0f));
+            oc[3] = 1.0f;
+            od[3] = 1.0f;
+        }
 }

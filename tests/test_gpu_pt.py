@@ -124,3 +124,25 @@ def test_pt_error_paths():
     pt.dispatch()
     assert pt.read_output().shape == (8, 8, 4)
     pt.close()
+
+
+@pytest.mark.parametrize("W,H,shading", [(64, 48, 1.0), (50, 70, 0.0)])
+def test_mesh_raster_bit_exact(oracle, W, H, shading):
+    from nprush_amd.managers import GpuMeshRenderPipelineManager
+
+    ms = meshes.scene(subdiv=3, seed=4)
+    env = meshes.environment(64, 32)
+    eye, tgt, up = (3.0, -2.5, 1.8), (0.2, 0.0, 0.6), (0.0, 0.0, 1.0)
+    vp, vi = camera.view_proj(eye, target=tgt, up=up, aspect=W / H)
+    view = camera.look_at(eye, tgt, up)
+    mr = GpuMeshRenderPipelineManager(mesh=ms[0], matcapTexture=env)
+    mr.setGroundMesh(ms[2])
+    mr.writeUniforms(vp, view, vi, shading)
+    col, dep = mr.addDraw(None, None, W, H)
+    v, idx = _flat([ms[0], ms[2]])
+    nodes, tris = oracle.pt_bvh(v, idx)
+    oc, od = oracle.mesh_raster(vp, view, vi, shading, W, H, v, nodes, tris, env)
+    assert np.array_equal(col, oc)
+    assert np.array_equal(dep, od)
+    assert (dep[..., 0] < 1.0).mean() > 0.3                  # the mesh covers part of the view
+    mr.close()

@@ -289,3 +289,63 @@ def test_cosine_hemisphere_distribution(oracle):
         assert abs(c.mean() - 2.0 / 3.0) < 0.01                        # E[cos] of a cosine lobe
         t = d - c[:, None] * n.astype(np.float64)[None, :]
         assert np.linalg.norm(t.mean(0)) < 0.01
+
+
+# ------------------------------------------------------------ mesh raster
+def _rgba8(x):
+    x = np.clip(np.asarray(x, np.float32), 0, 1)
+    return (np.rint(x * np.float32(255.0)) / np.float32(255.0)).astype(np.float32)
+
+
+def _reinhard(c):
+    c = np.asarray(c, np.float32)
+    return c / (c + np.float32(1.0))
+
+
+def _down_camera(h, W, H):
+    vp, vi = camera.view_proj((0.0, 0.0, h), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), aspect=W / H)
+    view = camera.look_at((0.0, 0.0, h), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0))
+    return vp, view, vi
+
+
+def test_raster_ground_closed_forms(oracle):
+    """A ground quad facing a camera that looks straight down from height h:
+    normal colour (0, 0, 1) -> (1/2, 1/2, 1); matcap mode with a constant matcap
+    M and vertex colour c -> M c; depth 1 - 0.1 / h (every vertex has clip w = h);
+    the reversed quad is culled (background, depth 1)."""
+    W, H, h = 24, 16, 2.5
+    vp, view, vi = _down_camera(h, W, H)
+    c = np.array([0.6, 0.3, 0.9], np.float32)          # 4 M c away from the 8-bit rounding boundaries
+    gv, gi = meshes.ground(40.0, 0.0, color=tuple(c) + (1.0,))
+    nodes, tris = oracle.pt_bvh(gv, gi)
+    M = np.array([0.5, 1.0, 0.25], np.float32)
+    env = meshes.constant_environment(M)
+    col, dep = oracle.mesh_raster(vp, view, vi, 0.0, W, H, gv, nodes, tris, env)
+    assert np.all(col[..., :3] == _rgba8(_reinhard(np.float32([0.5, 0.5, 1.0]) * np.float32(4.0))))
+    d = _rgba8(np.float32(1.0) - np.float32(0.1) / np.float32(h))
+    assert np.all(dep[..., :3] == d) and np.all(dep[..., 3] == 1.0) and np.all(col[..., 3] == 1.0)
+    col, dep = oracle.mesh_raster(vp, view, vi, 1.0, W, H, gv, nodes, tris, env)
+    assert np.all(col[..., :3] == _rgba8(_reinhard(M * c * np.float32(4.0))))
+    # reversed winding: back-facing -> culled
+    nodes_r, tris_r = oracle.pt_bvh(gv, gi[:, ::-1].copy())
+    col, dep = oracle.mesh_raster(vp, view, vi, 1.0, W, H, gv, nodes_r, tris_r, env)
+    assert np.all(dep[..., :3] == 1.0)
+    assert np.array_equal(col, oracle.env_background(vi, env, W, H))
+
+
+def test_raster_nearer_surface_wins(oracle):
+    W, H, h = 20, 20, 3.0
+    vp, view, vi = _down_camera(h, W, H)
+    lo = meshes.ground(40.0, 0.0, color=(1.0, 0.0, 0.0, 1.0))
+    hi = meshes.ground(0.5, 1.0, color=(0.0, 1.0, 0.0, 1.0))     # a small square 1 unit above the ground
+    v = np.concatenate([lo[0], hi[0]])
+    i = np.concatenate([lo[1], hi[1] + len(lo[0])])
+    nodes, tris = oracle.pt_bvh(v, i)
+    env = meshes.constant_environment((1.0, 1.0, 1.0))
+    col, dep = oracle.mesh_raster(vp, view, vi, 1.0, W, H, v, nodes, tris, env)
+    centre = col[H // 2, W // 2, :3]
+    corner = col[0, 0, :3]
+    assert centre[1] > 0 and centre[0] == 0          # the raised square (green) covers the centre
+    assert corner[0] > 0 and corner[1] == 0          # the ground (red) elsewhere
+    assert dep[H // 2, W // 2, 0] == _rgba8(np.float32(1.0) - np.float32(0.1) / np.float32(h - 1.0))
+    assert dep[0, 0, 0] == _rgba8(np.float32(1.0) - np.float32(0.1) / np.float32(h))
