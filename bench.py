@@ -675,6 +675,69 @@ def mlp_bench(n=757_052, steps=20, warmup=3, batch=4096, train_steps=20):
     }
 
 
+def pt_target_leg(frames=20, small=False, cpu=True):
+    """The frame's target generators on a generated scene: the mesh raster (colour
+    and encoded depth over the environment) and one path-traced sample per pixel
+    at the optimisation resolution of a 16:9 panel (455 x 256), an icosphere of
+    81,920 triangles + mirror box + ground (81,964 triangles)."""
+    import time as _t
+
+    from nprush_amd import camera, meshes
+    from nprush_amd.managers import GpuMeshRenderPipelineManager, GpuPathTracePipelineManager
+
+    W, H = (128, 72) if small else (455, 256)
+    ms = meshes.scene(subdiv=3 if small else 6, seed=1)
+    env = meshes.environment(512, 256)
+    ntri = int(sum(len(i) for _, i in ms))
+    eye, tgt, up = (3.0, -2.5, 1.8), (0.2, 0.0, 0.6), (0.0, 0.0, 1.0)
+    vp, vi = camera.view_proj(eye, target=tgt, up=up, aspect=W / H)
+    view = camera.look_at(eye, tgt, up)
+    pt = GpuPathTracePipelineManager(envTexture=env)
+    t0 = _t.perf_counter()
+    pt.setMeshes(ms)
+    build_ms = (_t.perf_counter() - t0) * 1e3
+    pt.writeInvViewProjMat(vi)
+    pt.setOutputSize(W, H)
+    for _ in range(3):
+        pt.dispatch()
+    pt.read_accum()
+    t0 = _t.perf_counter()
+    for _ in range(frames):
+        pt.dispatch()
+    acc = pt.read_accum()
+    pt_ms = (_t.perf_counter() - t0) * 1e3 / frames
+    mr = GpuMeshRenderPipelineManager(mesh=ms[0], matcapTexture=env)
+    mr.setGroundMesh(ms[2])
+    mr.writeUniforms(vp, view, vi, 1.0)
+    mr.addDraw(None, None, W, H)
+    t0 = _t.perf_counter()
+    for _ in range(frames):
+        mr.addDraw(None, None, W, H)
+    raster_ms = (_t.perf_counter() - t0) * 1e3 / frames
+    mr.close()
+    pt.close()
+    out = {"workload": f"path-traced target: {ntri} triangles (icosphere + mirror box + ground), 1 sample per pixel "
+                       f"and frame, 8 bounces, at {W}x{H}; mesh raster target of the same scene",
+           "width": W, "height": H, "triangles": ntri, "bvh_build_ms_host": round(build_ms, 1),
+           "pt_ms_per_frame": round(pt_ms, 3), "paths_per_s": round(W * H / (pt_ms * 1e-3), 1),
+           "raster_ms_incl_readback": round(raster_ms, 3), "mean_samples": float(acc[..., 3].mean()),
+           "cpu_baseline": None}
+    if cpu:
+        from oracle import oracle
+
+        v = np.concatenate([m[0] for m in ms])
+        offs = np.cumsum([0] + [len(m[0]) for m in ms[:-1]])
+        idx = np.concatenate([m[1] + np.uint32(o) for m, o in zip(ms, offs)])
+        nodes, tris = oracle.pt_bvh(v, idx)
+        t0 = _t.perf_counter()
+        oracle.pt_trace(vi, 0, W, H, v, nodes, tris, env)
+        dt = _t.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(dt * 1e3, 1), "unit": "ms/frame", "cores": int(oracle.load().ora_num_threads()),
+                               "kind": "port", "sample": f"one whole frame at {W}x{H} (oracle, OpenMP)"}
+        out["speedup_vs_cpu"] = round(dt * 1e3 / pt_ms, 1)
+    return out
+
+
 def main_mlp(args):
     print("[bench] densification MLP ...", file=sys.stderr, flush=True)
     r = mlp_bench(steps=args.steps, warmup=args.warmup, n=40_000 if args.small else 757_052)
@@ -747,6 +810,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-densification comparison")
     ap.add_argument("--no-ref", action="store_true", help="skip the REF-semantics leg (1M live / 2M slots, 1080p)")
     ap.add_argument("--no-densify", action="store_true", help="skip the timed EXT densification pass")
+    ap.add_argument("--no-pt", action="store_true", help="skip the path-traced / raster target leg")
     ap.add_argument("--cpu-tiles", type=int, default=48, help="tiles in the CPU-baseline sample (0 = skip)")
     ap.add_argument("--tile", type=int, default=0,
                     help="EXT: composite tile edge {8,16,32} instead of the autotuner's choice (with --batch)")
@@ -999,6 +1063,10 @@ def main_ext(args):
                                  "phases_ms", "tile_keys_per_view", "pairs_per_view", "active_pairs_per_view",
                                  "adam_hbm_gbs", "adam_roofline_frac", "adc", "cpu_baseline")}
         ref["speedup_vs_cpu"] = round(r["cpu_baseline"]["value"] / r["value"], 1) if r["cpu_baseline"] else None
+    ptleg = None
+    if not args.no_pt:
+        print("[bench] path-traced and raster target ...", file=sys.stderr, flush=True)
+        ptleg = pt_target_leg(small=args.small, cpu=args.cpu_tiles > 0)
     cpu = None
     if args.cpu_tiles > 0:
         print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
@@ -1041,6 +1109,7 @@ def main_ext(args):
         "densification": adc,
         "learned_densification": learned,
         "ref": ref,
+        "target_generators": ptleg,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

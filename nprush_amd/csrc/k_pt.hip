@@ -16,7 +16,8 @@
 // (oracle/oracle_pt.c, -ffp-contract=off on both sides): dot(a, b) =
 // (ax bx + ay by) + az bz, M v = ((c0 x + c1 y) + c2 z) + c3 w, normalize(v) =
 // v / sqrt(dot(v, v)), atan2/asin/sin/cos from nps_math.h.  A wave traces an 8x8
-// pixel block (neighbouring primary rays walk the same nodes).
+// pixel block (neighbouring primary rays walk the same nodes); the BVH walk's
+// stack lives in LDS (scratch memory was 20-25 % slower).
 #include "nps_common.h"
 #include "nps_kernels.h"
 #include "nps_pt.h"
@@ -138,10 +139,12 @@ __device__ __forceinline__ bool aabb_hit(V3 ro, V3 inv, const float4& mn, const 
     return tmax >= 0.0f && tmax >= tmin && tmin < t_max;
 }
 
-// Closest hit (path_trace.wgsl:146-185).
+// Closest hit (path_trace.wgsl:146-185).  The walk's stack (64 entries) is the
+// lane's column of an LDS array: entry k at stack[64 k] (lanes side by side, so
+// a wave's pushes and pops hit distinct banks).
 template <bool kCull>
 __device__ PtHit scene_hit(V3 ro, V3 rd, const float4* __restrict__ nodes, const uint32_t* __restrict__ tris,
-                           const float4* __restrict__ vt, uint32_t nnodes) {
+                           const float4* __restrict__ vt, uint32_t nnodes, uint32_t* stack) {
     PtHit best;
     best.hit = false;
     best.t = 1e30f;
@@ -149,11 +152,10 @@ __device__ PtHit scene_hit(V3 ro, V3 rd, const float4* __restrict__ nodes, const
     best.i0 = best.i1 = best.i2 = 0u;
     if (nnodes == 0u) return best;
     const V3 inv = v3(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
-    uint32_t stack[kPtMaxStack];
     uint32_t top = 0u;
-    stack[top++] = 0u;
+    stack[64u * top++] = 0u;
     while (top > 0u) {
-        const uint32_t node = stack[--top];
+        const uint32_t node = stack[64u * --top];
         const float4 mn = nodes[2u * node], mx = nodes[2u * node + 1u];
         if (!aabb_hit(ro, inv, mn, mx, best.t)) continue;
         const uint32_t d1 = __float_as_uint(mx.w), d0 = __float_as_uint(mn.w);
@@ -164,8 +166,8 @@ __device__ PtHit scene_hit(V3 ro, V3 rd, const float4* __restrict__ nodes, const
                 intersect_tri<kCull>(ro, rd, vt, tris[b], tris[b + 1u], tris[b + 2u], best.t, best);
             }
         } else {
-            if (top + 1u < kPtMaxStack) stack[top++] = d1;
-            if (top < kPtMaxStack) stack[top++] = d0;
+            if (top + 1u < kPtMaxStack) stack[64u * top++] = d1;
+            if (top < kPtMaxStack) stack[64u * top++] = d0;
         }
     }
     return best;
@@ -209,10 +211,11 @@ __device__ __forceinline__ V3 cosine_hemisphere(V3 n, uint32_t& seed) {
 }  // namespace
 
 // One thread per pixel, 16x16 pixels per workgroup, each wave an 8x8 block.
-__global__ __launch_bounds__(256) void k_pt_trace(PtArgs a) {
-    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
-    const uint32_t px = blockIdx.x * 16u + (wid & 1u) * 8u + (lane & 7u);
-    const uint32_t py = blockIdx.y * 16u + (wid >> 1) * 8u + (lane >> 3);
+__global__ __launch_bounds__(64) void k_pt_trace(PtArgs a) {
+    __shared__ uint32_t s_stack[kPtMaxStack * 64];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t px = blockIdx.x * 8u + (lane & 7u);
+    const uint32_t py = blockIdx.y * 8u + (lane >> 3);
     if (px >= a.W || py >= a.H) return;
     const uint32_t pix = py * a.W + px;
     uint32_t seed = pcg(pix ^ (a.frame * 2654435761u));
@@ -226,7 +229,7 @@ __global__ __launch_bounds__(256) void k_pt_trace(PtArgs a) {
     V3 rd = normalize3(sub(far, ro));
     V3 rad = v3(0.f, 0.f, 0.f), thr = v3(1.f, 1.f, 1.f), o = ro;
     for (int bounce = 0; bounce < 8; ++bounce) {
-        const PtHit h = scene_hit<false>(o, rd, a.nodes, a.tris, a.verts, a.nnodes);
+        const PtHit h = scene_hit<false>(o, rd, a.nodes, a.tris, a.verts, a.nnodes, s_stack + lane);
         if (!h.hit) {
             rad = add(rad, mulv(thr, sample_env(a.env, a.ew, a.eh, rd)));
             break;
@@ -298,6 +301,7 @@ __device__ __forceinline__ float view_depth(float w) {
 // flipped) times the vertex colour; Reinhard(4 c); depth the blended vertex
 // depths 1 - 0.1 / max(w, 0.1).  No triangle: background colour, depth 1.
 __global__ __launch_bounds__(256) void k_mesh_raster(RasterArgs a) {
+    __shared__ uint32_t s_stack[kPtMaxStack * 64 * 4];
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
     const uint32_t px = blockIdx.x * 16u + (wid & 1u) * 8u + (lane & 7u);
     const uint32_t py = blockIdx.y * 16u + (wid >> 1) * 8u + (lane >> 3);
@@ -307,7 +311,7 @@ __global__ __launch_bounds__(256) void k_mesh_raster(RasterArgs a) {
     const V3 nw = unproject(a.ivp, nx, ny, 0.0f), fw = unproject(a.ivp, nx, ny, 1.0f);
     const V3 rd = normalize3(sub(fw, nw));
     const size_t pix = (size_t)py * a.W + px;
-    const PtHit h = scene_hit<true>(nw, rd, a.nodes, a.tris, a.verts, a.nnodes);
+    const PtHit h = scene_hit<true>(nw, rd, a.nodes, a.tris, a.verts, a.nnodes, s_stack + wid * 64u * kPtMaxStack + lane);
     if (!h.hit) {
         const V3 c = sample_env(a.env, a.ew, a.eh, rd);
         a.color[pix] = make_float4(pt_unorm8(pt_reinhard(c.x * 4.0f)), pt_unorm8(pt_reinhard(c.y * 4.0f)),
@@ -363,7 +367,7 @@ void launch_env_background(hipStream_t st, const float* ivp, const float4* env, 
 
 void launch_pt_trace(hipStream_t st, const PtArgs& a) {
     if (a.W == 0 || a.H == 0) return;
-    hipLaunchKernelGGL(k_pt_trace, dim3((a.W + 15) / 16, (a.H + 15) / 16), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_pt_trace, dim3((a.W + 7) / 8, (a.H + 7) / 8), dim3(64), 0, st, a);
 }
 
 void launch_pt_resolve(hipStream_t st, const float4* accum, uint32_t npix, float4* out) {

@@ -64,3 +64,34 @@ def test_runner_frames_match_oracle(oracle):
         assert np.array_equal(runner.optimizer.read_splats(), s), k
         assert np.array_equal(out_c, ec) and np.array_equal(out_d, ed), k
     runner.close()
+
+
+def test_runner_scene_target_matches_oracle(oracle):
+    """The frame's target from an attached scene (GpuRunner.svelte.ts:578-642):
+    the raster (colour, encoded depth) and the path tracer's output, which
+    accumulates while the camera holds still and restarts when it moves."""
+    from nprush_amd import meshes
+
+    runner = GpuRunner(400, 1.0, splats=scenes.with_dead_capacity(scenes.make_splats(200, 3), 400))
+    W, H = runner.optim_width, runner.optim_height
+    ms = meshes.scene(subdiv=2, seed=5)
+    env = meshes.environment(64, 32)
+    runner.attach_scene([ms[0], ms[2]], env)
+    v = np.concatenate([ms[0][0], ms[2][0]])
+    idx = np.concatenate([ms[0][1], ms[2][1] + len(ms[0][0])])
+    nodes, tris = oracle.pt_bvh(v, idx)
+    eyes = [(3.0, -2.5, 1.8), (-2.0, 3.0, 2.2)]
+    for eye in eyes:
+        vp, vi = camera.view_proj(eye, target=(0.0, 0.0, 0.6), up=(0.0, 0.0, 1.0), aspect=W / H)
+        view = camera.look_at(eye, (0.0, 0.0, 0.6), (0.0, 0.0, 1.0))
+        runner.set_camera(vp, vi, view)
+        acc = None
+        for f in range(2):                      # still camera: samples accumulate from frame 0
+            pt_col, col, dep = runner.scene_textures()
+            oc, od = oracle.mesh_raster(vp, view, vi, 1.0, W, H, v, nodes, tris, env)
+            assert np.array_equal(col, oc) and np.array_equal(dep, od)
+            acc = oracle.pt_trace(vi, f, W, H, v, nodes, tris, env, accum=acc)
+            assert np.array_equal(pt_col, oracle.pt_resolve(acc))
+        out_c, out_d = runner.frame(pt_col, dep)
+        assert out_c.shape == (H, W, 4) and np.isfinite(out_c).all()
+    runner.close()
