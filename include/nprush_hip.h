@@ -556,6 +556,17 @@ int nps_dev_upload(nps_handle h, void* dst, const void* src, int64_t bytes);
  * For tests (bit-identity with the CPU oracle). */
 int nps_debug_math(int32_t device, int32_t op, const float* x, const float* y, float* out, int32_t n);
 
+/* The EXT depth order on the GPU for tests: keys[n] = depth_bits << 32 | id
+ * (depth_bits 0xFFFFFFFF: not drawn) sorted stably on the depth bits, by the
+ * 32-bit pass sort (narrow = 0) or the two-pass range-compressed sort of
+ * speculative views (narrow = 1; the per-256-key minima of the drawn depths
+ * stand in for the preprocess's).  flags_out: [0] any overflow, [1] depth range
+ * too wide for the narrow sort (the keys are then not in depth order); n < 2^31.
+ * No reference counterpart (EXT's depth-ordered lists; the reference's lists
+ * are in id order). */
+int nps_debug_depth_sort(int32_t device, const uint64_t* keys, uint32_t n, int32_t narrow, uint64_t* out,
+                         uint32_t flags_out[2]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -118,6 +118,7 @@ SIGNATURES = {
     "nps_dev_free": (i32, [_P, _P]),
     "nps_dev_upload": (i32, [_P, _P, _P, C.c_int64]),
     "nps_debug_math": (i32, [i32, i32, _F, _F, _F, i32]),
+    "nps_debug_depth_sort": (i32, [i32, C.POINTER(C.c_uint64), C.c_uint32, i32, C.POINTER(C.c_uint64), _U]),
     "nps_adc_features": (i32, [_P, _F, i32, _I]),
     "nps_adc_children": (i32, [_P, _I, _I, i32, _I]),
     "nps_adc_apply_deltas": (i32, [_P, _F, i32]),

@@ -30,16 +30,16 @@ namespace nps {
 
 // ---------------------------------------------------------------- preprocess
 
+// One splat; returns the float bits of its depth w when it is drawn (a tile
+// rectangle), else 0xFFFFFFFF.
 template <int DEG>
-__global__ __launch_bounds__(256) void k_ext_preprocess(const float4* __restrict__ splats,
-                                                        const float4* __restrict__ sh, uint32_t n, ViewParams v,
-                                                        float cam0, float cam1, float cam2,
-                                                        ExtRec* __restrict__ rec, uint32_t* __restrict__ counts,
-                                                        uint4* __restrict__ rects, uint64_t* __restrict__ dkey) {
+__device__ __forceinline__ uint32_t ext_preprocess_splat(uint32_t i, const float4* __restrict__ splats,
+                                                         const float4* __restrict__ sh, const ViewParams& v,
+                                                         float cam0, float cam1, float cam2,
+                                                         ExtRec* __restrict__ rec, uint32_t* __restrict__ counts,
+                                                         uint4* __restrict__ rects, uint64_t* __restrict__ dkey) {
     constexpr int kNk = (DEG + 1) * (DEG + 1);
     constexpr int kSh4 = (3 * kNk + 3) / 4;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
     float s[16];
     float c[4 * kSh4];
 #pragma unroll
@@ -57,16 +57,16 @@ __global__ __launch_bounds__(256) void k_ext_preprocess(const float4* __restrict
     const float cam[3] = {cam0, cam1, cam2};
     ExtProj o;
     ext_project(s, c, DEG, v.vp, cam, v.W, v.H, o);
-    if (!o.valid) return;
+    if (!o.valid) return 0xFFFFFFFFu;
     const float ebx = o.ext[0] * kExtBinScale + kExtBinPad, eby = o.ext[1] * kExtBinScale + kExtBinPad;
     const float sminx = o.scr[0] - ebx, smaxx = o.scr[0] + ebx;
     const float sminy = o.scr[1] - eby, smaxy = o.scr[1] + eby;
-    if (!(sminx == sminx && smaxx == smaxx && sminy == sminy && smaxy == smaxy)) return;
+    if (!(sminx == sminx && smaxx == smaxx && sminy == sminy && smaxy == smaxy)) return 0xFFFFFFFFu;
     const uint32_t W = v.W, H = v.H, T = v.tile;
     const float aspect = v.aspect;
     const uint4 tr = tile_rect(sminx, smaxx, sminy, smaxy, W, H, T, v.tiles_x, v.tiles_y, aspect);
     const uint32_t tx0 = tr.x, tx_end = tr.y, ty0 = tr.z, ty_end = tr.w;
-    if (tx0 >= tx_end || ty0 >= ty_end) return;
+    if (tx0 >= tx_end || ty0 >= ty_end) return 0xFFFFFFFFu;
     counts[i] = (tx_end - tx0) * (ty_end - ty0);
     rects[i] = make_uint4(tx0, tx_end, ty0, ty_end);
     dkey[i] = ((uint64_t)__float_as_uint(o.w) << 32) | i;
@@ -92,6 +92,29 @@ __global__ __launch_bounds__(256) void k_ext_preprocess(const float4* __restrict
     f.ext_x = o.ext[0];
     f.ext_y = o.ext[1];
     rec[i] = f;
+    return __float_as_uint(o.w);
+}
+
+// Per splat the projection of ext_preprocess_splat; per workgroup (256 splats)
+// the least depth bits of its drawn splats (0xFFFFFFFF: none) into wmin_part,
+// the base of the depth order's compressed keys (null: not needed).
+template <int DEG>
+__global__ __launch_bounds__(256) void k_ext_preprocess(const float4* __restrict__ splats,
+                                                        const float4* __restrict__ sh, uint32_t n, ViewParams v,
+                                                        float cam0, float cam1, float cam2,
+                                                        ExtRec* __restrict__ rec, uint32_t* __restrict__ counts,
+                                                        uint4* __restrict__ rects, uint64_t* __restrict__ dkey,
+                                                        uint32_t* __restrict__ wmin_part) {
+    __shared__ uint32_t s_min[4];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t wb = 0xFFFFFFFFu;
+    if (i < n) wb = ext_preprocess_splat<DEG>(i, splats, sh, v, cam0, cam1, cam2, rec, counts, rects, dkey);
+    if (!wmin_part) return;  // uniform
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) wb = min(wb, (uint32_t)__shfl_xor((int)wb, o, 64));
+    if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = wb;
+    __syncthreads();
+    if (threadIdx.x == 0) wmin_part[blockIdx.x] = min(min(s_min[0], s_min[1]), min(s_min[2], s_min[3]));
 }
 
 // One key tile << 32 | id per (splat, covered tile), splats taken in rank
@@ -189,8 +212,9 @@ constexpr int kBinColItems = 16;
 constexpr int kBinColBlock = 256 * kBinColItems;  // items per workgroup of the column kernels
 
 // counts1[y * nblk + blk] = keys of ranks [1024 blk, 1024 blk + 1024) in tile row y.
+// A rank's splat is drawn when its depth bits (order's high word) are not
+// 0xFFFFFFFF: drawn depths are at least 0.1.
 __global__ __launch_bounds__(kBinThreads) void k_bin_row_count(uint32_t n, const uint64_t* __restrict__ order,
-                                                               const uint32_t* __restrict__ rank_counts,
                                                                const uint4* __restrict__ rects, uint32_t ny,
                                                                uint32_t* __restrict__ counts1) {
     __shared__ int s_diff[(1 << kBinMaxBits) + 1];
@@ -198,8 +222,9 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_row_count(uint32_t n, const
     for (uint32_t k = tid; k <= ny; k += kBinThreads) s_diff[k] = 0;
     __syncthreads();
     const uint32_t r = blockIdx.x * kBinThreads + tid;
-    if (r < n && rank_counts[r]) {
-        const uint4 q = rects[(uint32_t)order[r]];
+    const uint64_t o = r < n ? order[r] : ~0ull;
+    if ((uint32_t)(o >> 32) != 0xFFFFFFFFu) {
+        const uint4 q = rects[(uint32_t)o];
         atomicAdd(&s_diff[q.z], (int)(q.y - q.x));
         atomicAdd(&s_diff[q.w], -(int)(q.y - q.x));
     }
@@ -224,7 +249,6 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_row_count(uint32_t n, const
 // offs1: exclusive scan of counts1 (row-major).  Items beyond cap are dropped
 // (*overflow = 1).
 __global__ __launch_bounds__(kBinThreads) void k_bin_row_emit(uint32_t n, const uint64_t* __restrict__ order,
-                                                              const uint32_t* __restrict__ rank_counts,
                                                               const uint4* __restrict__ rects, uint32_t ny,
                                                               int row_bits, int id_bits,
                                                               const uint32_t* __restrict__ offs1,
@@ -235,15 +259,16 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_row_emit(uint32_t n, const 
     for (uint32_t k = tid; k < kBinWaves * ((1u << kBinMaxBits) + 1u); k += kBinThreads) (&s_row[0][0])[k] = 0;
     __syncthreads();
     const uint32_t mine = blockIdx.x * kBinThreads + tid;
-    const uint32_t my_cnt = mine < n ? rank_counts[mine] : 0u;
-    uint32_t my_id = 0u, my_x0 = 0u, my_w = 1u, my_y0 = 0u;
+    const uint64_t my_o = mine < n ? order[mine] : ~0ull;
+    uint32_t my_cnt = 0u, my_id = 0u, my_x0 = 0u, my_w = 1u, my_y0 = 0u;
     float my_rcp = 1.0f;
-    if (my_cnt) {
-        my_id = (uint32_t)order[mine];
+    if ((uint32_t)(my_o >> 32) != 0xFFFFFFFFu) {  // drawn (as k_bin_row_count)
+        my_id = (uint32_t)my_o;
         const uint4 q = rects[my_id];
         my_x0 = q.x;
         my_w = q.y - q.x;
         my_y0 = q.z;
+        my_cnt = my_w * (q.w - q.z);
         my_rcp = 1.0f / (float)my_w;
         atomicAdd(&s_row[wid][q.z], (int)my_w);
         atomicAdd(&s_row[wid][q.w], -(int)my_w);
@@ -348,7 +373,7 @@ __global__ __launch_bounds__(256) void k_bin_col_count(const uint32_t* __restric
     __shared__ uint32_t s_c[256];
     __shared__ uint32_t s_t[kLocal];
     if (m_dev) m = min((uint64_t)*m_dev, m);
-    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t tid = threadIdx.x;
     for (uint32_t y = tid; y < ny; y += 256) s_start[y] = offs1[(size_t)y * nblk1];
     s_c[tid] = 0u;
     __syncthreads();
@@ -1694,16 +1719,16 @@ void launch_sum_tile_loss(hipStream_t st, const double* tile_loss, uint32_t ntil
 
 void launch_ext_preprocess(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg,
                            const ViewParams& v, const float cam[3], ExtRec* rec, uint32_t* counts, uint4* rects,
-                           uint64_t* dkey) {
+                           uint64_t* dkey, uint32_t* wmin_part) {
     if (n == 0) return;
     dim3 g((n + 255) / 256), b(256);
     const float4* s4 = (const float4*)splats;
     const float4* h4 = (const float4*)sh;
     switch (deg) {
-        case 0: hipLaunchKernelGGL(k_ext_preprocess<0>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], rec, counts, rects, dkey); break;
-        case 1: hipLaunchKernelGGL(k_ext_preprocess<1>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], rec, counts, rects, dkey); break;
-        case 2: hipLaunchKernelGGL(k_ext_preprocess<2>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], rec, counts, rects, dkey); break;
-        default: hipLaunchKernelGGL(k_ext_preprocess<3>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], rec, counts, rects, dkey); break;
+        case 0: hipLaunchKernelGGL(k_ext_preprocess<0>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], rec, counts, rects, dkey, wmin_part); break;
+        case 1: hipLaunchKernelGGL(k_ext_preprocess<1>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], rec, counts, rects, dkey, wmin_part); break;
+        case 2: hipLaunchKernelGGL(k_ext_preprocess<2>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], rec, counts, rects, dkey, wmin_part); break;
+        default: hipLaunchKernelGGL(k_ext_preprocess<3>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], rec, counts, rects, dkey, wmin_part); break;
     }
 }
 
@@ -1734,19 +1759,18 @@ size_t ext_bin_col_words(uint32_t tiles_x, uint64_t cap) {
     return (size_t)tiles_x * std::max<size_t>(nb2, 1) + 1;
 }
 
-void ext_bin_rows(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts, const uint4* rects,
-                  uint32_t tiles_y, uint32_t* rcnt, uint32_t* roff, void* rscan) {
+void ext_bin_rows(hipStream_t st, uint32_t n, const uint64_t* order, const uint4* rects, uint32_t tiles_y,
+                  uint32_t* rcnt, uint32_t* roff, void* rscan) {
     if (n == 0 || tiles_y == 0) {  // no keys: the count word alone
         launch_fill_u32(st, roff + (ext_bin_row_words(n, tiles_y) - 1), 0u, 1);
         return;
     }
     const uint32_t nb1 = (n + kBinThreads - 1) / kBinThreads;
-    hipLaunchKernelGGL(k_bin_row_count, dim3(nb1), dim3(kBinThreads), 0, st, n, order, rank_counts, rects, tiles_y,
-                       rcnt);
+    hipLaunchKernelGGL(k_bin_row_count, dim3(nb1), dim3(kBinThreads), 0, st, n, order, rects, tiles_y, rcnt);
     exclusive_scan_u32(st, rcnt, roff, (uint64_t)tiles_y * nb1, rscan);
 }
 
-void ext_bin(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts, const uint4* rects,
+void ext_bin(hipStream_t st, uint32_t n, const uint64_t* order, const uint4* rects,
              uint32_t tiles_x, uint32_t tiles_y, uint64_t cap, const uint32_t* m_dev, const uint32_t* roff,
              uint32_t* items, uint32_t* ids, uint32_t* ranges, uint32_t* tile_order, uint32_t* ccnt, uint32_t* coff,
              uint32_t* tile_cnt, void* cscan, uint32_t* overflow) {
@@ -1756,8 +1780,8 @@ void ext_bin(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* 
     launch_fill_u32(st, tile_cnt, 0u, ntiles);
     if (n > 0 && cap > 0 && ntiles > 0) {
         const uint32_t nb1 = (n + kBinThreads - 1) / kBinThreads;
-        hipLaunchKernelGGL(k_bin_row_emit, dim3(nb1), dim3(kBinThreads), 0, st, n, order, rank_counts, rects, tiles_y,
-                           yb, idb, roff, items, cap, overflow);
+        hipLaunchKernelGGL(k_bin_row_emit, dim3(nb1), dim3(kBinThreads), 0, st, n, order, rects, tiles_y, yb, idb,
+                           roff, items, cap, overflow);
         const uint32_t nb2 = (uint32_t)((cap + kBinColBlock - 1) / kBinColBlock);
         hipLaunchKernelGGL(k_bin_col_count, dim3(nb2), dim3(256), 0, st, (const uint32_t*)items, cap, m_dev, roff,
                            nb1, tiles_y, tiles_x, idb, ccnt, tile_cnt);

@@ -358,6 +358,283 @@ bool radix_sort_u64(hipStream_t st, uint64_t* keys, uint64_t* keys_alt, uint32_t
     return alt;
 }
 
+// ---------------------------------------------------------------- depth order
+// Keys depth_bits << 32 | id (depth_bits: the float bits of a positive depth,
+// 0xFFFFFFFF for a splat that is not drawn) sorted on the depth bits in two
+// 12-bit passes over a range-compressed code: code = min(hi - base, 2^24 - 1)
+// with base the least drawn depth bits, and 2^24 - 1 for a splat not drawn.
+// The code keeps the order of the depth bits as long as every drawn depth lies
+// below base + 2^24 - 1 (a depth ratio of about 4 and more: two binades and
+// more of float spacing); the first histogram pass checks that and otherwise
+// stores 1 to flags[0] and flags[1] (the order is then a permutation, but not
+// the depth order, and the caller sorts again on all 32 bits).
+//
+// Per pass: k_depth_hist (4096 keys per workgroup) writes its digit counts
+// block-major, hist[blk][digit] (one contiguous 16 KB row per workgroup), and
+// adds the counts of each group of 16 digits to gsum[group];
+// k_depth_offsets (a workgroup per digit group, i.e. per contiguous range of
+// the digit-major scan order) turns its 16 columns into the exclusive
+// offsets of every (digit, block) in place; k_depth_scatter reads its block's
+// row of offsets (contiguous) and scatters stably.
+
+constexpr int kDepthRadix = 4096;
+constexpr int kDepthItems = 16;
+constexpr int kDepthBlock = kSortThreads * kDepthItems;
+constexpr int kDepthGroups = kDepthRadix / 16;     // 256 groups of 16 digits
+constexpr int kDepthTileRows = 256;                // rows (blocks) per LDS tile of k_depth_offsets
+constexpr uint32_t kDepthCap = (1u << 24) - 1u;
+
+__device__ __forceinline__ uint32_t depth_code(uint64_t key, uint32_t base) {
+    const uint32_t hi = (uint32_t)(key >> 32);
+    return hi == 0xFFFFFFFFu ? kDepthCap : min(hi - base, kDepthCap);
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, o, 64));
+    return x;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x += (uint32_t)__shfl_xor((int)x, o, 64);
+    return x;
+}
+
+// kFirst: reduces the per-workgroup minima part[0..npart) to base (block 0
+// stores it to *base_slot) and checks the range; else reads *base_slot.
+template <bool kFirst>
+__global__ __launch_bounds__(kSortThreads) void k_depth_hist(const uint64_t* __restrict__ keys, uint32_t m, int shift,
+                                                             const uint32_t* __restrict__ part, uint32_t npart,
+                                                             uint32_t* __restrict__ base_slot,
+                                                             uint32_t* __restrict__ flags,
+                                                             uint32_t* __restrict__ hist,
+                                                             uint32_t* __restrict__ gsum) {
+    __shared__ uint32_t h[kDepthRadix];
+    __shared__ uint32_t s_min[4];
+    __shared__ uint32_t s_over;
+    const uint32_t tid = threadIdx.x;
+    for (int d = tid; d < kDepthRadix; d += kSortThreads) h[d] = 0u;
+    uint32_t base;
+    if (kFirst) {
+        uint32_t b = 0xFFFFFFFFu;
+        for (uint32_t k = tid; k < npart; k += kSortThreads) b = min(b, part[k]);
+        b = wave_min_u32(b);
+        if ((tid & 63) == 0) s_min[tid >> 6] = b;
+        if (tid == 0) s_over = 0u;
+        __syncthreads();
+        base = min(min(s_min[0], s_min[1]), min(s_min[2], s_min[3]));
+        if (blockIdx.x == 0 && tid == 0) *base_slot = base;
+    } else {
+        __syncthreads();
+        base = *base_slot;
+    }
+    bool over = false;
+    const uint32_t b0 = blockIdx.x * (uint32_t)kDepthBlock;
+#pragma unroll
+    for (int r = 0; r < kDepthItems; ++r) {
+        const uint32_t pos = b0 + (uint32_t)r * kSortThreads + tid;
+        if (pos < m) {
+            const uint64_t key = keys[pos];
+            if (kFirst) {
+                const uint32_t hi = (uint32_t)(key >> 32);
+                over |= hi != 0xFFFFFFFFu && hi - base >= kDepthCap;
+            }
+            atomicAdd(&h[(depth_code(key, base) >> shift) & (kDepthRadix - 1)], 1u);
+        }
+    }
+    if (kFirst && over) s_over = 1u;
+    __syncthreads();
+    uint32_t* row = hist + (size_t)blockIdx.x * kDepthRadix;
+    for (int d = tid; d < kDepthRadix; d += kSortThreads) row[d] = h[d];
+    // digit group tid: 16 counts (read rotated: no LDS bank conflicts)
+    uint32_t g = 0u;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) g += h[16 * tid + ((j + tid) & 15)];
+    if (g) atomicAdd(&gsum[tid], g);
+    if (kFirst && tid == 0 && s_over) {
+        flags[0] = 1u;
+        flags[1] = 1u;
+    }
+}
+
+// One workgroup per group of 16 digits: hist[b][d] becomes the exclusive
+// offset of (digit d, block b) in digit-major order, in place.
+// Thread t: digit column j = t & 15 of the group, row lane r = t >> 4.  Up to
+// kDepthTileRows blocks the group's columns are read once into LDS; beyond,
+// a first sweep sums the columns and a second one scans them tile by tile.
+__global__ __launch_bounds__(kSortThreads) void k_depth_offsets(uint32_t* __restrict__ hist, uint32_t nblk,
+                                                                const uint32_t* __restrict__ gsum) {
+    __shared__ uint32_t tile[kDepthTileRows][17];
+    __shared__ uint32_t s_w[4];
+    __shared__ uint32_t s_col[16][17];
+    __shared__ uint32_t s_tot[16];
+    __shared__ uint32_t s_base[16];
+    const uint32_t tid = threadIdx.x, j = tid & 15u, r = tid >> 4, g = blockIdx.x;
+    uint32_t* col = hist + 16u * g + j;
+    const bool one = nblk <= (uint32_t)kDepthTileRows;
+    // keys with a digit below the group
+    uint32_t gb = tid < g ? gsum[tid] : 0u;
+    gb = wave_sum_u32(gb);
+    if ((tid & 63) == 0) s_w[tid >> 6] = gb;
+    // tile rows t0 .. t0 + rows: thread (j, r) scans rows 16 r .. 16 r + 15 of
+    // column j serially (exclusive, in the tile), segment sum to s_col[r][j]
+    auto load_scan = [&](uint32_t t0) {
+        const uint32_t rows = min((uint32_t)kDepthTileRows, nblk - t0);
+#pragma unroll 4
+        for (uint32_t q = r; q < kDepthTileRows; q += 16)
+            tile[q][j] = q < rows ? col[(size_t)(t0 + q) * kDepthRadix] : 0u;
+        __syncthreads();
+        uint32_t seg = 0u;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const uint32_t v = tile[16 * r + q][j];
+            tile[16 * r + q][j] = seg;
+            seg += v;
+        }
+        s_col[r][j] = seg;
+    };
+    if (one) {
+        load_scan(0u);
+    } else {
+        uint32_t tot = 0u;
+        for (uint32_t row = r; row < nblk; row += 16) tot += col[(size_t)row * kDepthRadix];
+        s_col[r][j] = tot;
+    }
+    __syncthreads();
+    if (tid < 16) {
+        uint32_t t = 0u;
+        for (int q = 0; q < 16; ++q) t += s_col[q][tid];
+        s_tot[tid] = t;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t acc = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        for (int q = 0; q < 16; ++q) {
+            s_base[q] = acc;
+            acc += s_tot[q];
+        }
+    }
+    __syncthreads();
+    uint32_t carry = s_base[j];
+    for (uint32_t t0 = 0; t0 < nblk; t0 += kDepthTileRows) {
+        if (!one) {
+            load_scan(t0);
+            __syncthreads();
+        }
+        const uint32_t rows = min((uint32_t)kDepthTileRows, nblk - t0);
+        uint32_t pre = carry, all = carry;
+        for (uint32_t q = 0; q < 16; ++q) {
+            const uint32_t v = s_col[q][j];
+            pre += q < r ? v : 0u;
+            all += v;
+        }
+#pragma unroll 4
+        for (uint32_t q = 0; q < 16; ++q) {
+            const uint32_t row = 16 * r + q;
+            if (row < rows) col[(size_t)(t0 + row) * kDepthRadix] = pre + tile[row][j];
+        }
+        carry = all;
+        __syncthreads();
+    }
+}
+
+// Stable scatter on the 12-bit digit of the code at `shift` (ranking as in
+// k_radix_scatter); per-wave digit counts are 16-bit (a wave ranks 1024 keys).
+__global__ __launch_bounds__(kSortThreads) void k_depth_scatter(const uint64_t* __restrict__ keys_in, uint32_t m,
+                                                                int shift, const uint32_t* __restrict__ base_slot,
+                                                                const uint32_t* __restrict__ offsets,
+                                                                uint64_t* __restrict__ keys_out,
+                                                                const uint32_t* __restrict__ gsrc,
+                                                                uint32_t* __restrict__ gdst) {
+    __shared__ uint16_t wcnt[4][kDepthRadix];
+    __shared__ uint32_t goff[kDepthRadix];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int k = threadIdx.x; k < 2 * kDepthRadix; k += kSortThreads) ((uint32_t*)&wcnt[0][0])[k] = 0u;
+    const uint32_t base = *base_slot;
+    const uint32_t* orow = offsets + (size_t)blockIdx.x * kDepthRadix;
+    for (int d = threadIdx.x; d < kDepthRadix; d += kSortThreads) goff[d] = orow[d];
+    __syncthreads();
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t b0 = blockIdx.x * (uint32_t)kDepthBlock + (uint32_t)wid * (64 * kDepthItems);
+    uint64_t key[kDepthItems];
+    uint32_t rank[kDepthItems];
+#pragma unroll
+    for (int r = 0; r < kDepthItems; ++r) {
+        const uint32_t pos = b0 + (uint32_t)r * 64 + lane;
+        key[r] = pos < m ? keys_in[pos] : 0ull;
+    }
+#pragma unroll
+    for (int r = 0; r < kDepthItems; ++r) {
+        const uint32_t pos = b0 + (uint32_t)r * 64 + lane;
+        const bool valid = pos < m;
+        const uint32_t d = (depth_code(key[r], base) >> shift) & (kDepthRadix - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 12; ++b) {
+            const uint64_t bal = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bal : ~bal;
+        }
+        const uint32_t before = (uint32_t)__popcll(peers & lt_mask);
+        const uint32_t cnt = valid ? (uint32_t)wcnt[wid][d] : 0u;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && before == 0u) wcnt[wid][d] = (uint16_t)(cnt + (uint32_t)__popcll(peers));
+        __builtin_amdgcn_wave_barrier();
+        rank[r] = cnt + before;
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < kDepthRadix; d += kSortThreads) {
+        const uint32_t c0 = wcnt[0][d], c1 = wcnt[1][d], c2 = wcnt[2][d];
+        wcnt[0][d] = 0;
+        wcnt[1][d] = (uint16_t)c0;
+        wcnt[2][d] = (uint16_t)(c0 + c1);
+        wcnt[3][d] = (uint16_t)(c0 + c1 + c2);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kDepthItems; ++r) {
+        const uint32_t pos = b0 + (uint32_t)r * 64 + lane;
+        if (pos < m) {
+            const uint32_t d = (depth_code(key[r], base) >> shift) & (kDepthRadix - 1);
+            const uint32_t dst = goff[d] + (uint32_t)wcnt[wid][d] + rank[r];
+            keys_out[dst] = key[r];
+            if (gsrc) gdst[dst] = gsrc[(uint32_t)key[r]];
+        }
+    }
+}
+
+// temp: base slot and two passes' group sums (1024 words), then the block-major
+// histogram / offsets (4096 words per block of 4096 keys)
+size_t depth_sort_temp_bytes(uint64_t m) {
+    uint64_t nblk = (m + kDepthBlock - 1) / kDepthBlock;
+    if (nblk == 0) nblk = 1;
+    return sizeof(uint32_t) * (1024 + (uint64_t)kDepthRadix * nblk);
+}
+
+void depth_sort_narrow(hipStream_t st, uint64_t* keys, uint64_t* keys_alt, uint32_t m, void* temp,
+                       const uint32_t* wmin_part, uint32_t npart, uint32_t* flags, const uint32_t* gather_src,
+                       uint32_t* gather_dst) {
+    if (m == 0) return;
+    const uint32_t nblk = (m + kDepthBlock - 1) / kDepthBlock;
+    uint32_t* base_slot = (uint32_t*)temp;
+    uint32_t* gsum = base_slot + 256;               // [2][kDepthGroups]
+    uint32_t* hist = base_slot + 1024;
+    launch_fill_u32(st, gsum, 0u, 2 * kDepthGroups);
+    hipLaunchKernelGGL(k_depth_hist<true>, dim3(nblk), dim3(kSortThreads), 0, st, (const uint64_t*)keys, m, 0,
+                       wmin_part, npart, base_slot, flags, hist, gsum);
+    hipLaunchKernelGGL(k_depth_offsets, dim3(kDepthGroups), dim3(kSortThreads), 0, st, hist, nblk,
+                       (const uint32_t*)gsum);
+    hipLaunchKernelGGL(k_depth_scatter, dim3(nblk), dim3(kSortThreads), 0, st, (const uint64_t*)keys, m, 0,
+                       (const uint32_t*)base_slot, (const uint32_t*)hist, keys_alt, (const uint32_t*)nullptr,
+                       (uint32_t*)nullptr);
+    hipLaunchKernelGGL(k_depth_hist<false>, dim3(nblk), dim3(kSortThreads), 0, st, (const uint64_t*)keys_alt, m, 12,
+                       wmin_part, npart, base_slot, flags, hist, gsum + kDepthGroups);
+    hipLaunchKernelGGL(k_depth_offsets, dim3(kDepthGroups), dim3(kSortThreads), 0, st, hist, nblk,
+                       (const uint32_t*)(gsum + kDepthGroups));
+    hipLaunchKernelGGL(k_depth_scatter, dim3(nblk), dim3(kSortThreads), 0, st, (const uint64_t*)keys_alt, m, 12,
+                       (const uint32_t*)base_slot, (const uint32_t*)hist, keys, gather_src, gather_dst);
+}
+
 // Sums per-workgroup counter slots (PartMap): one workgroup, stats[dst[k]] +=
 // sum over slots of part[K * slot + k].  Ordered after the producing kernel on
 // the stream, so plain read-modify-writes suffice.

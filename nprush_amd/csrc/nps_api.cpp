@@ -35,6 +35,7 @@ struct ExtView {
     DevBuf order, tile_work, ckpt, act, g2, c_img, tile_loss, stats, flags;  // flags: [0] key overflow
     DevBuf part;  // per-tile counter slots of the composite
     DevBuf bin_rcnt, bin_roff, bin_rscan;         // binning, row pass: counts, offsets, scan temp
+    DevBuf wmin;                                  // per preprocess workgroup: least drawn depth bits
     DevBuf bin_cnt, bin_off, bin_tile, bin_scan;  // binning, column pass: counts, offsets, per-tile counts, temp
     uint32_t* ids = nullptr;  // splat ids in (tile, depth, id) order: keys or keys_alt read as 32-bit words
     bool col_major = false;   // ids stored tile column by tile column (ext_bin)
@@ -44,6 +45,13 @@ struct ExtView {
     // the next run of the same shape sizes its key buffers from that run's
     // count instead of waiting for its own.
     uint64_t shape = 0;
+    // Tiles of the last composite whose counter slots (part) are not summed into
+    // stats yet: the statistics are summed when they are read (nps_read_stats).
+    uint32_t part_tiles = 0;
+    // Speculative views sort the depths in two narrow passes (depth_sort_narrow)
+    // unless a recent one found its depth range too wide: then the next
+    // narrow_skip speculative views use the full 32-bit sort.
+    uint32_t narrow_skip = 0;
     bool in_alt = false;
     uint32_t num_tiles = 0;
     uint32_t* flags_at = nullptr;  // flags in a shared buffer instead of `flags` (16 words)
@@ -153,7 +161,7 @@ void free_ext_view(ExtView& x) {
     DevBuf* all[] = {&x.rec, &x.counts, &x.rects, &x.dkey, &x.dkey_alt, &x.rank_counts, &x.offsets, &x.scan_tmp,
                      &x.keys, &x.keys_alt, &x.sort_tmp, &x.ranges, &x.order, &x.tile_work, &x.ckpt, &x.act,
                      &x.g2, &x.c_img, &x.tile_loss, &x.stats, &x.flags, &x.part, &x.bin_cnt, &x.bin_off,
-                     &x.bin_tile, &x.bin_scan, &x.bin_rcnt, &x.bin_roff, &x.bin_rscan};
+                     &x.bin_tile, &x.bin_scan, &x.bin_rcnt, &x.bin_roff, &x.bin_rscan, &x.wmin};
     for (DevBuf* b : all) free_buf(*b);
     if (x.ev_count) {
         (void)hipEventDestroy(x.ev_count);
@@ -201,7 +209,8 @@ int ext_view_reserve(ExtView& x, uint32_t n, uint32_t W, uint32_t H, uint64_t ca
         {&x.rec, sizeof(ExtRec) * n1}, {&x.counts, 4 * n1}, {&x.rects, 16 * n1}, {&x.dkey, 8 * n1},
         {&x.dkey_alt, 8 * n1}, {&x.rank_counts, 4 * n1}, {&x.offsets, 8 * (n1 + 1)},
         {&x.scan_tmp, scan_temp_bytes(n1) + 64}, {&x.stats, 64}, {&x.flags, 64},
-        {&x.sort_tmp, std::max(radix_temp_bytes(c1), radix_temp_bytes(n1, true))},
+        {&x.sort_tmp, std::max({radix_temp_bytes(c1), radix_temp_bytes(n1, true), depth_sort_temp_bytes(n1)})},
+        {&x.wmin, 4 * ((n1 + 255) / 256)},
         {&x.keys, 8 * c1}, {&x.keys_alt, 8 * c1},
         {&x.ranges, 8 * (size_t)nt}, {&x.order, 4 * (size_t)ext_order_slots(tx, ty)}, {&x.tile_work, 8 * (size_t)nt},
         {&x.c_img, 12 * (size_t)W * H}, {&x.tile_loss, 16 * (size_t)nt}, {&x.part, 32 * (size_t)nt},
@@ -252,30 +261,41 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
     const bool dev_count = capture || spec;  // the key count stays on the device
     if (!capture && (rc = ext_view_reserve(x, n, v.W, v.H, spec_cap, grads, tu, st, err)) != NPS_OK) return rc;
     if (ev && (rc = hip(hipEventRecord(ev[0], st), "event")) != NPS_OK) return rc;
-    launch_fill_u32(st, (uint32_t*)x.stats.p, 0u, 16);
+    x.part_tiles = 0;
     // flags: cleared per view when eager; a captured step clears them once, so a
     // dropped key anywhere in a lane survives to the Adam update
     uint32_t* flags = x.flags_at ? x.flags_at : (uint32_t*)x.flags.p;
     if (!capture) launch_fill_u32(st, flags, 0u, 16);
     uint32_t* counts = (out && out->counts) ? out->counts : (uint32_t*)x.counts.p;
     double* g2 = (out && out->g2) ? out->g2 : (double*)x.g2.p;
+    // narrow depth sort: only where a too-wide depth range can be caught after
+    // the view (speculative runs redo the view then)
+    const bool narrow = spec && x.narrow_skip == 0 && n <= 0x7FFFFFFFu;
+    if (spec && x.narrow_skip) --x.narrow_skip;
     launch_ext_preprocess(st, sc.splats, sc.sh, n, sc.deg, v, cam, (ExtRec*)x.rec.p, counts, (uint4*)x.rects.p,
-                          (uint64_t*)x.dkey.p);
-    // rank of every splat in (depth, id) order: stable sort of depth << 32 | id on
-    // the depth bits; its last pass also gathers the tile counts in rank order
-    const bool dalt = radix_sort_u64(st, (uint64_t*)x.dkey.p, (uint64_t*)x.dkey_alt.p, nullptr, nullptr, n, 32, 64,
-                                     x.sort_tmp.p, nullptr, true, (const uint32_t*)counts,
-                                     (uint32_t*)x.rank_counts.p);
-    const uint64_t* order = (const uint64_t*)(dalt ? x.dkey_alt.p : x.dkey.p);
+                          (uint64_t*)x.dkey.p, narrow ? (uint32_t*)x.wmin.p : nullptr);
     // The binning's row pass needs no key count, and its last row offset is the
     // count (u32).  The key path (sorted tile << 32 | id keys) scans per-rank key
     // offsets instead and takes the count from them (u64).
     const bool rows_early = ext_bin_fits(n, v.tiles_x, v.tiles_y, 0);
     const bool key_path = !rows_early || (capture && x.cap >= 0xFFFFFFFFull) || (spec && spec_cap >= 0xFFFFFFFFull);
+    // rank of every splat in (depth, id) order: stable sort of depth << 32 | id on
+    // the depth bits; for the key path its last pass also gathers the tile
+    // counts in rank order (the binning reads the rectangles instead)
+    const uint32_t* gsrc = key_path ? (const uint32_t*)counts : nullptr;
+    uint32_t* gdst = key_path ? (uint32_t*)x.rank_counts.p : nullptr;
+    bool dalt = false;
+    if (narrow)
+        depth_sort_narrow(st, (uint64_t*)x.dkey.p, (uint64_t*)x.dkey_alt.p, n, x.sort_tmp.p, (const uint32_t*)x.wmin.p,
+                          (n + 255) / 256, flags, gsrc, gdst);
+    else
+        dalt = radix_sort_u64(st, (uint64_t*)x.dkey.p, (uint64_t*)x.dkey_alt.p, nullptr, nullptr, n, 32, 64,
+                              x.sort_tmp.p, nullptr, true, gsrc, gdst);
+    const uint64_t* order = (const uint64_t*)(dalt ? x.dkey_alt.p : x.dkey.p);
     const uint32_t* count32 = nullptr;
     const uint64_t* count64 = nullptr;
     if (rows_early) {
-        ext_bin_rows(st, n, order, (const uint32_t*)x.rank_counts.p, (const uint4*)x.rects.p, v.tiles_y,
+        ext_bin_rows(st, n, order, (const uint4*)x.rects.p, v.tiles_y,
                      (uint32_t*)x.bin_rcnt.p, (uint32_t*)x.bin_roff.p, x.bin_rscan.p);
         count32 = (const uint32_t*)x.bin_roff.p + (ext_bin_row_words(n, v.tiles_y) - 1);
     }
@@ -319,13 +339,18 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
         // pass on the tile column: ids and ranges
         x.ids = (uint32_t*)x.keys_alt.p;
         if (ev && (rc = hip(hipEventRecord(ev[1], st), "event")) != NPS_OK) return rc;
-        ext_bin(st, n, order, (const uint32_t*)x.rank_counts.p, (const uint4*)x.rects.p, v.tiles_x, v.tiles_y, cap,
+        ext_bin(st, n, order, (const uint4*)x.rects.p, v.tiles_x, v.tiles_y, cap,
                 dev_count ? count32 : nullptr, (const uint32_t*)x.bin_roff.p, (uint32_t*)x.keys.p, x.ids, (uint32_t*)x.ranges.p,
                 (uint32_t*)x.order.p, (uint32_t*)x.bin_cnt.p, (uint32_t*)x.bin_off.p, (uint32_t*)x.bin_tile.p,
                 x.bin_scan.p, flags);
         if (ev && (rc = hip(hipEventRecord(ev[2], st), "event")) != NPS_OK) return rc;
     } else {
         // 8-byte keys tile << 32 | id in rank order, stable sort on the tile bits
+        // (key_path holds here: col_major fails only for a capacity of 2^32 - 1 and more)
+        if (!key_path) {
+            err = "EXT view: key path without rank counts";
+            return NPS_ERR_ARG;
+        }
         launch_ext_keys(st, n, order, (const uint32_t*)x.rank_counts.p, (const uint4*)x.rects.p, v.tiles_x,
                         (const uint64_t*)x.offsets.p, (uint64_t*)x.keys.p, cap, flags);
         if (ev && (rc = hip(hipEventRecord(ev[1], st), "event")) != NPS_OK) return rc;
@@ -359,7 +384,7 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
     a.part = (unsigned long long*)x.part.p;
     a.backward = grads ? 1 : 0;
     launch_composite_ext(st, v, a, batch);
-    launch_sum_parts(st, a.part, v.num_tiles, PartMap{4, 4, {0, 4, 2, 3, 0, 0, 0, 0}}, a.stats);
+    x.part_tiles = v.num_tiles;
     if (ev && (rc = hip(hipEventRecord(ev[4], st), "event")) != NPS_OK) return rc;
     if (grads && !(out && out->defer_chain))
         launch_ext_chain(st, sc.splats, sc.sh, n, sc.deg, v, cam, (const double*)g2, (const uint32_t*)counts, sc.gradx,
@@ -368,12 +393,13 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
     if ((rc = hip(hipGetLastError(), "launch")) != NPS_OK) return rc;
     if (capture) return NPS_OK;
     if (spec) {
-        // the overflow flag and the key count, read once the view is done
+        // the overflow flags (a dropped key or a too-wide depth range: flags[0];
+        // the latter alone: flags[1]) and the key count, read once the view is done
         if (!x.count_host && (rc = hip(hipHostMalloc((void**)&x.count_host, 64, hipHostMallocDefault), "pinned")) != NPS_OK)
             return rc;
         uint32_t* h32 = (uint32_t*)(x.count_host + 1);
-        if ((rc = hip(hipMemcpyAsync(h32 + 1, flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "flag readback")) !=
-            NPS_OK)
+        if ((rc = hip(hipMemcpyAsync(h32 + 1, flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st),
+                      "flag readback")) != NPS_OK)
             return rc;
         if ((rc = hip(count64 ? hipMemcpyAsync(x.count_host, count64, sizeof(uint64_t), hipMemcpyDeviceToHost, st)
                               : hipMemcpyAsync(h32, count32, sizeof(uint32_t), hipMemcpyDeviceToHost, st),
@@ -382,7 +408,9 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
         if ((rc = hip(hipStreamSynchronize(st), "sync")) != NPS_OK) return rc;
         const uint64_t m = count64 ? x.count_host[0] : (uint64_t)h32[0];
         if (h32[1] != 0u) {
-            // a key was dropped: size for the count just seen and run the view again
+            // a key was dropped or the depth order was not exact: size for the
+            // count just seen and run the view again (32-bit depth sort)
+            if (h32[2] != 0u) x.narrow_skip = 64;
             x.m = m;
             x.shape = 0;
             return run_ext_view(sc, v, batch, cam, tgt, x, st, grads, false, err, out, ev, false);
@@ -1143,6 +1171,15 @@ int nps_read_tile_work(nps_handle h, uint32_t* work) {
 int nps_read_stats(nps_handle h, int64_t stats[12]) {
     if (!h || !stats) return NPS_ERR_ARG;
     const DevBuf& sb = h->ext ? h->xv.stats : h->stats;
+    if (h->ext && h->have_backward && sb.p) {
+        // the last EXT composite's per-tile counter slots, summed now
+        NPS_HIP(h, hipSetDevice(h->device));
+        launch_fill_u32(h->stream, (uint32_t*)sb.p, 0u, 16);
+        if (h->xv.part_tiles)
+            launch_sum_parts(h->stream, (const unsigned long long*)h->xv.part.p, h->xv.part_tiles,
+                             PartMap{4, 4, {0, 4, 2, 3, 0, 0, 0, 0}}, (unsigned long long*)sb.p);
+        NPS_HIP(h, hipGetLastError());
+    }
     if (h->have_backward && sb.p) {
         unsigned long long s[5] = {0, 0, 0, 0, 0};
         NPS_COPY(h, s, sb.p, sizeof(s), hipMemcpyDeviceToHost);
@@ -2030,6 +2067,48 @@ int nps_debug_math(int32_t device, int32_t op, const float* x, const float* y, f
     if (dx) hipFree(dx);
     if (dy) hipFree(dy);
     if (dout) hipFree(dout);
+    return rc;
+}
+
+int nps_debug_depth_sort(int32_t device, const uint64_t* keys, uint32_t n, int32_t narrow, uint64_t* out,
+                         uint32_t flags_out[2]) {
+    if ((!keys && n) || (!out && n) || !flags_out || n > 0x7FFFFFFFu) return NPS_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return NPS_ERR_HIP;
+    flags_out[0] = flags_out[1] = 0u;
+    if (n == 0) return NPS_OK;
+    // per 256 keys: the least depth bits of the drawn ones (the preprocess's
+    // workgroup minima)
+    const uint32_t npart = (n + 255) / 256;
+    std::vector<uint32_t> part(npart, 0xFFFFFFFFu);
+    for (uint32_t i = 0; i < n; ++i) part[i / 256] = std::min(part[i / 256], (uint32_t)(keys[i] >> 32));
+    const size_t tmp_bytes = std::max(radix_temp_bytes(n, true), depth_sort_temp_bytes(n));
+    uint64_t *dk = nullptr, *da = nullptr;
+    uint32_t *dp = nullptr, *df = nullptr;
+    void* tmp = nullptr;
+    int rc = NPS_OK;
+    if (hipMalloc(&dk, 8 * (size_t)n) != hipSuccess || hipMalloc(&da, 8 * (size_t)n) != hipSuccess ||
+        hipMalloc(&dp, 4 * (size_t)npart) != hipSuccess || hipMalloc(&df, 64) != hipSuccess ||
+        hipMalloc(&tmp, tmp_bytes) != hipSuccess) {
+        rc = NPS_ERR_HIP;
+    } else {
+        hipMemcpy(dk, keys, 8 * (size_t)n, hipMemcpyHostToDevice);
+        hipMemcpy(dp, part.data(), 4 * (size_t)npart, hipMemcpyHostToDevice);
+        launch_fill_u32(nullptr, df, 0u, 16);
+        bool alt = false;
+        if (narrow) depth_sort_narrow(nullptr, dk, da, n, tmp, dp, npart, df, nullptr, nullptr);
+        else alt = radix_sort_u64(nullptr, dk, da, nullptr, nullptr, n, 32, 64, tmp, nullptr, true);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+            rc = NPS_ERR_HIP;
+        } else {
+            hipMemcpy(out, alt ? da : dk, 8 * (size_t)n, hipMemcpyDeviceToHost);
+            hipMemcpy(flags_out, df, 8, hipMemcpyDeviceToHost);
+        }
+    }
+    if (dk) hipFree(dk);
+    if (da) hipFree(da);
+    if (dp) hipFree(dp);
+    if (df) hipFree(df);
+    if (tmp) hipFree(tmp);
     return rc;
 }
 

@@ -67,6 +67,16 @@ bool radix_sort_u64(hipStream_t st, uint64_t* keys, uint64_t* keys_alt, uint32_t
                     uint64_t m, int bit_lo, int bit_hi, void* temp, const uint64_t* m_dev = nullptr,
                     bool wide = false,
                     const uint32_t* gather_src = nullptr, uint32_t* gather_dst = nullptr);
+// Depth order of m keys depth_bits << 32 | id (0xFFFFFFFF: not drawn) in two
+// 12-bit passes over min(depth_bits - base, 2^24 - 1), base = the least of
+// wmin_part[0..npart) (the preprocess's per-workgroup least drawn depth bits).
+// A drawn depth at or beyond base + 2^24 - 1 stores 1 to flags[0] and flags[1]
+// (the result is then not the depth order).  Result in keys; the last pass
+// gathers as radix_sort_u64 does.  temp: depth_sort_temp_bytes(m).
+size_t depth_sort_temp_bytes(uint64_t m);
+void depth_sort_narrow(hipStream_t st, uint64_t* keys, uint64_t* keys_alt, uint32_t m, void* temp,
+                       const uint32_t* wmin_part, uint32_t npart, uint32_t* flags, const uint32_t* gather_src,
+                       uint32_t* gather_dst);
 
 
 // k_composite.hip
@@ -129,7 +139,8 @@ struct ExtCompositeArgs {
     int col_major;              // lists stored column by column (ext_bin), else tile by tile
 };
 // EXT binning without a key sort (k_bin_* in k_ext.hip): from the depth order
-// (order, rank_counts, rects) to splat ids grouped by tile, each tile in
+// (order: depth_bits << 32 | id, sorted) and the tile rectangles to splat ids
+// grouped by tile, each tile in
 // (depth, id) order (tiles column by column), and the tile ranges.
 // ext_bin_fits: at most 256 tile rows and columns, the 4-byte items hold the
 // column and the id, and the counts fit 32 bits (else the caller sorts keys).
@@ -142,9 +153,9 @@ struct ExtCompositeArgs {
 bool ext_bin_fits(uint32_t n, uint32_t tiles_x, uint32_t tiles_y, uint64_t cap);
 size_t ext_bin_row_words(uint32_t n, uint32_t tiles_y);
 size_t ext_bin_col_words(uint32_t tiles_x, uint64_t cap);
-void ext_bin_rows(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts, const uint4* rects,
-                  uint32_t tiles_y, uint32_t* rcnt, uint32_t* roff, void* rscan);
-void ext_bin(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts, const uint4* rects,
+void ext_bin_rows(hipStream_t st, uint32_t n, const uint64_t* order, const uint4* rects, uint32_t tiles_y,
+                  uint32_t* rcnt, uint32_t* roff, void* rscan);
+void ext_bin(hipStream_t st, uint32_t n, const uint64_t* order, const uint4* rects,
              uint32_t tiles_x, uint32_t tiles_y, uint64_t cap, const uint32_t* m_dev, const uint32_t* roff,
              uint32_t* items, uint32_t* ids, uint32_t* ranges, uint32_t* tile_order, uint32_t* ccnt, uint32_t* coff,
              uint32_t* tile_cnt, void* cscan, uint32_t* overflow);
@@ -154,7 +165,7 @@ uint32_t ext_order_slots(uint32_t tiles_x, uint32_t tiles_y);
 void launch_tile_order(hipStream_t st, const uint32_t* ranges, uint32_t tiles_x, uint32_t tiles_y, uint32_t* order);
 void launch_ext_preprocess(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg,
                            const ViewParams& v, const float cam[3], ExtRec* rec, uint32_t* counts, uint4* rects,
-                           uint64_t* dkey);
+                           uint64_t* dkey, uint32_t* wmin_part = nullptr);
 // keys beyond `cap` are not written; *overflow is then set to 1
 void launch_ext_keys(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts,
                      const uint4* rects, uint32_t tiles_x, const uint64_t* offsets, uint64_t* keys, uint64_t cap,

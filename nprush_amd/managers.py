@@ -1122,6 +1122,19 @@ def debug_math(op: int, x: np.ndarray, y: np.ndarray | None = None, device: int 
     return out
 
 
+def debug_depth_sort(keys: np.ndarray, narrow: bool, device: int = 0):
+    """EXT depth order of keys depth_bits << 32 | id on the GPU (nps_debug_depth_sort):
+    (sorted keys, flags [any overflow, depth range too wide])."""
+    lib = _lib.load()
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    out = np.empty_like(k)
+    fl = np.zeros(2, np.uint32)
+    check(lib.nps_debug_depth_sort(device, k.ctypes.data_as(C.POINTER(C.c_uint64)), k.size, 1 if narrow else 0,
+                                   out.ctypes.data_as(C.POINTER(C.c_uint64)), uptr(fl)), None,
+          "nps_debug_depth_sort")
+    return out, fl
+
+
 def compose_display(textures, enabled, width: int, height: int, device: int = 0) -> np.ndarray:
     """The viewer's debug display compositor (src/lib/gpu/splat_render.wgsl:24-181)
     through npd_compose.  textures: 8 (H_k, W_k, 4) float arrays in the

@@ -185,6 +185,55 @@ def test_ext_speculative_key_buffers_overflow_redo(oracle):
     mgr.close()
 
 
+def _drawn_depth_span(oracle, s, sh, deg, vp, vi, W, H, eye):
+    """float-bit span of the depths (distance along the view axis, here -z of
+    a camera on the +z axis) of the splats in some tile list"""
+    tiles, lists, counts = oracle.ext_bin(s, sh, deg, vp, vi, W, H)
+    drawn = np.zeros(len(s), bool)
+    for t in range(len(tiles)):
+        drawn[lists[t, :counts[t]]] = True
+    w = (np.float32(eye[2]) - s[drawn, 2]).astype(np.float32).view(np.uint32)
+    return int(w.max()) - int(w.min())
+
+
+def test_ext_narrow_depth_order_and_redo(oracle):
+    """A view of the shape seen last sorts its depths in two 12-bit passes over
+    the depth bits less the least drawn depth's when they span fewer than
+    2^24 - 1 float steps; a wider span is flagged, the chain rule adds nothing,
+    and the view is redone with the 32-bit sort (as are the next views of that
+    shape).  Lists (pairs of splats at one position: equal depths, id order),
+    image, gradients and statistics are the oracle's, each view counted once."""
+    W = H = 128
+    deg = 1
+    n = 8192
+    s, sh = scenes.make_ext_splats(n, 3, sh_degree=deg)
+    s[1::2, 0:3] = s[0::2, 0:3]
+    eyes = {"tight": (0.0, 0.0, 6.0), "wide": (0.0, 0.0, 1.3)}
+    cams = {k: camera.view_proj(e, aspect=1.0) for k, e in eyes.items()}
+    assert _drawn_depth_span(oracle, s, sh, deg, *cams["tight"], W, H, eyes["tight"]) < (1 << 23)
+    assert _drawn_depth_span(oracle, s, sh, deg, *cams["wide"], W, H, eyes["wide"]) > (1 << 24) + (1 << 21)
+    tgt = np.ones((H, W, 4), np.float32)
+    tgt[..., :3] = 0.3
+    mgr = _run(s, sh, deg, *cams["tight"], tgt, W, H)
+    for name in ("tight", "wide", "wide", "tight"):
+        vp, vi = cams[name]
+        mgr.write_grads_f32(np.zeros((n, 64), np.float32))
+        mgr.write_accum(np.zeros_like(mgr.read_accum()))
+        mgr.writeSplatVPMatrix(vp, vi)
+        mgr.backward()
+        tiles, lists, counts = _check_tiles(mgr, oracle, s, sh, deg, vp, vi, W, H)
+        assert mgr.read_stats()["keys"] == int(counts.sum())
+        g, gs, c, loss = oracle.ext_backward_f32(s, sh, deg, vp, vi, tgt, W, H, tiles, lists, counts)
+        img, _ = mgr.read_image()
+        assert np.array_equal(img, c), name
+        _check_grads(mgr.read_grads_f32(), g, gs, deg)
+        on_screen = np.zeros(n, bool)
+        for t in range(len(tiles)):
+            on_screen[lists[t, :counts[t]]] = True
+        assert np.array_equal(mgr.read_accum()[:, 1], on_screen.astype(np.float32)), name
+    mgr.close()
+
+
 def test_ext_forward_only_and_accumulation(oracle):
     W = H = 96
     deg = 3
