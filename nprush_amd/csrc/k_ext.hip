@@ -97,16 +97,26 @@ __device__ __forceinline__ uint32_t ext_preprocess_splat(uint32_t i, const float
 
 // Per splat the projection of ext_preprocess_splat; per workgroup (256 splats)
 // the least depth bits of its drawn splats (0xFFFFFFFF: none) into wmin_part,
-// the base of the depth order's compressed keys (null: not needed).
+// the base of the depth order's compressed keys (null: not needed).  The grid
+// also clears clr.n[c] words at clr.p[c] (null: none), the view's small
+// counters, so they need no launch of their own.
+struct ClearPair {
+    uint32_t* p[3];
+    uint32_t n[3];
+};
 template <int DEG>
 __global__ __launch_bounds__(256) void k_ext_preprocess(const float4* __restrict__ splats,
                                                         const float4* __restrict__ sh, uint32_t n, ViewParams v,
                                                         float cam0, float cam1, float cam2,
                                                         ExtRec* __restrict__ rec, uint32_t* __restrict__ counts,
                                                         uint4* __restrict__ rects, uint64_t* __restrict__ dkey,
-                                                        uint32_t* __restrict__ wmin_part) {
+                                                        uint32_t* __restrict__ wmin_part, ClearPair clr) {
     __shared__ uint32_t s_min[4];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        if (clr.p[c])
+            for (uint32_t k = i; k < clr.n[c]; k += gridDim.x * blockDim.x) clr.p[c][k] = 0u;
     uint32_t wb = 0xFFFFFFFFu;
     if (i < n) wb = ext_preprocess_splat<DEG>(i, splats, sh, v, cam0, cam1, cam2, rec, counts, rects, dkey);
     if (!wmin_part) return;  // uniform
@@ -1160,21 +1170,19 @@ __global__ __launch_bounds__(TILE * TILE) void k_composite_ext(ViewParams v, Ext
 // flags (nflags words, 16 apart; null = none): when any is set a view of the
 // step dropped keys, the host redoes the step view by view, and this launch
 // leaves gradients and densification statistics untouched.
+// g2_clear (null = keep): the 2D gradients read are cleared, so g2 is all zero
+// again for the next view (only splats with a tile count can have any).
 template <int DEG, bool kStore>
 __global__ __launch_bounds__(256) void k_ext_chain(const float4* __restrict__ splats, const float4* __restrict__ sh,
                                                    uint32_t n, ViewParams v, float cam0, float cam1, float cam2,
                                                    const double* __restrict__ g2, const uint32_t* __restrict__ counts,
                                                    float* __restrict__ grads, float2* __restrict__ accum,
-                                                   const uint32_t* __restrict__ flags, uint32_t nflags) {
+                                                   const uint32_t* __restrict__ flags, uint32_t nflags,
+                                                   double* __restrict__ g2_clear) {
     constexpr int kNk = (DEG + 1) * (DEG + 1);
     constexpr int kSh4 = (3 * kNk + 3) / 4;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (flags) {
-        uint32_t bad = 0u;
-        for (uint32_t k = 0; k < nflags; ++k) bad |= flags[16 * k];
-        if (bad) return;
-    }
     if (counts[i] == 0u) return;  // culled or off-screen: no pair, no gradient
     double gg[9];
     bool any = false;
@@ -1182,6 +1190,15 @@ __global__ __launch_bounds__(256) void k_ext_chain(const float4* __restrict__ sp
     for (int k = 0; k < 9; ++k) {
         gg[k] = g2[9 * (size_t)i + k];
         any = any || gg[k] != 0.0;
+    }
+    if (g2_clear && any) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) g2_clear[9 * (size_t)i + k] = 0.0;
+    }
+    if (flags) {
+        uint32_t bad = 0u;
+        for (uint32_t k = 0; k < nflags; ++k) bad |= flags[16 * k];
+        if (bad) return;
     }
     float s[16];
 #pragma unroll
@@ -1719,16 +1736,24 @@ void launch_sum_tile_loss(hipStream_t st, const double* tile_loss, uint32_t ntil
 
 void launch_ext_preprocess(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg,
                            const ViewParams& v, const float cam[3], ExtRec* rec, uint32_t* counts, uint4* rects,
-                           uint64_t* dkey, uint32_t* wmin_part) {
-    if (n == 0) return;
+                           uint64_t* dkey, uint32_t* wmin_part, uint32_t* clr0, uint32_t nclr0, uint32_t* clr1,
+                           uint32_t nclr1, uint32_t* clr2, uint32_t nclr2) {
+    if (n == 0) {
+        if (nclr0) launch_fill_u32(st, clr0, 0u, nclr0);
+        if (nclr1) launch_fill_u32(st, clr1, 0u, nclr1);
+        if (nclr2) launch_fill_u32(st, clr2, 0u, nclr2);
+        return;
+    }
+    const ClearPair clr{{nclr0 ? clr0 : nullptr, nclr1 ? clr1 : nullptr, nclr2 ? clr2 : nullptr},
+                        {nclr0, nclr1, nclr2}};
     dim3 g((n + 255) / 256), b(256);
     const float4* s4 = (const float4*)splats;
     const float4* h4 = (const float4*)sh;
     switch (deg) {
-        case 0: hipLaunchKernelGGL(k_ext_preprocess<0>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], rec, counts, rects, dkey, wmin_part); break;
-        case 1: hipLaunchKernelGGL(k_ext_preprocess<1>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], rec, counts, rects, dkey, wmin_part); break;
-        case 2: hipLaunchKernelGGL(k_ext_preprocess<2>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], rec, counts, rects, dkey, wmin_part); break;
-        default: hipLaunchKernelGGL(k_ext_preprocess<3>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], rec, counts, rects, dkey, wmin_part); break;
+        case 0: hipLaunchKernelGGL(k_ext_preprocess<0>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], rec, counts, rects, dkey, wmin_part, clr); break;
+        case 1: hipLaunchKernelGGL(k_ext_preprocess<1>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], rec, counts, rects, dkey, wmin_part, clr); break;
+        case 2: hipLaunchKernelGGL(k_ext_preprocess<2>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], rec, counts, rects, dkey, wmin_part, clr); break;
+        default: hipLaunchKernelGGL(k_ext_preprocess<3>, g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], rec, counts, rects, dkey, wmin_part, clr); break;
     }
 }
 
@@ -1773,11 +1798,11 @@ void ext_bin_rows(hipStream_t st, uint32_t n, const uint64_t* order, const uint4
 void ext_bin(hipStream_t st, uint32_t n, const uint64_t* order, const uint4* rects,
              uint32_t tiles_x, uint32_t tiles_y, uint64_t cap, const uint32_t* m_dev, const uint32_t* roff,
              uint32_t* items, uint32_t* ids, uint32_t* ranges, uint32_t* tile_order, uint32_t* ccnt, uint32_t* coff,
-             uint32_t* tile_cnt, void* cscan, uint32_t* overflow) {
+             uint32_t* tile_cnt, void* cscan, uint32_t* overflow, bool tile_cnt_clear) {
     const int xb = bits_for_u32(tiles_x > 0 ? tiles_x - 1 : 0), yb = bits_for_u32(tiles_y > 0 ? tiles_y - 1 : 0);
     const int idb = bits_for_u32(n > 1 ? n - 1 : 0);
     const uint32_t ntiles = tiles_x * tiles_y;
-    launch_fill_u32(st, tile_cnt, 0u, ntiles);
+    if (!tile_cnt_clear) launch_fill_u32(st, tile_cnt, 0u, ntiles);
     if (n > 0 && cap > 0 && ntiles > 0) {
         const uint32_t nb1 = (n + kBinThreads - 1) / kBinThreads;
         hipLaunchKernelGGL(k_bin_row_emit, dim3(nb1), dim3(kBinThreads), 0, st, n, order, rects, tiles_y, yb, idb,
@@ -1838,25 +1863,25 @@ size_t ext_act_words(size_t slots, int tile, int batch) {
 template <bool kStore>
 static void launch_ext_chain_t(hipStream_t st, const float4* s4, const float4* h4, uint32_t n, int deg,
                                const ViewParams& v, const float cam[3], const double* g2, const uint32_t* counts,
-                               float* grads, float2* a2, const uint32_t* fl, uint32_t nfl) {
+                               float* grads, float2* a2, const uint32_t* fl, uint32_t nfl, double* g2c) {
     dim3 g((n + 255) / 256), b(256);
     switch (deg) {
-        case 0: hipLaunchKernelGGL((k_ext_chain<0, kStore>), g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2, fl, nfl); break;
-        case 1: hipLaunchKernelGGL((k_ext_chain<1, kStore>), g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2, fl, nfl); break;
-        case 2: hipLaunchKernelGGL((k_ext_chain<2, kStore>), g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2, fl, nfl); break;
-        default: hipLaunchKernelGGL((k_ext_chain<3, kStore>), g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2, fl, nfl); break;
+        case 0: hipLaunchKernelGGL((k_ext_chain<0, kStore>), g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2, fl, nfl, g2c); break;
+        case 1: hipLaunchKernelGGL((k_ext_chain<1, kStore>), g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2, fl, nfl, g2c); break;
+        case 2: hipLaunchKernelGGL((k_ext_chain<2, kStore>), g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2, fl, nfl, g2c); break;
+        default: hipLaunchKernelGGL((k_ext_chain<3, kStore>), g, b, 0, st, s4, h4, n, v, cam[0], cam[1], cam[2], g2, counts, grads, a2, fl, nfl, g2c); break;
     }
 }
 
 void launch_ext_chain(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg, const ViewParams& v,
                       const float cam[3], const double* g2, const uint32_t* counts, float* grads, float* accum,
-                      bool grads_zero, const uint32_t* flags, uint32_t nflags) {
+                      bool grads_zero, const uint32_t* flags, uint32_t nflags, double* g2_clear) {
     if (n == 0) return;
     const float4* s4 = (const float4*)splats;
     const float4* h4 = (const float4*)sh;
     float2* a2 = (float2*)accum;
-    if (grads_zero) launch_ext_chain_t<true>(st, s4, h4, n, deg, v, cam, g2, counts, grads, a2, flags, nflags);
-    else launch_ext_chain_t<false>(st, s4, h4, n, deg, v, cam, g2, counts, grads, a2, flags, nflags);
+    if (grads_zero) launch_ext_chain_t<true>(st, s4, h4, n, deg, v, cam, g2, counts, grads, a2, flags, nflags, g2_clear);
+    else launch_ext_chain_t<false>(st, s4, h4, n, deg, v, cam, g2, counts, grads, a2, flags, nflags, g2_clear);
 }
 
 void launch_adam_ext(hipStream_t st, float* splats, float* sh, float* grads, float* m, float* v, uint32_t n, int deg,

@@ -611,15 +611,18 @@ size_t depth_sort_temp_bytes(uint64_t m) {
     return sizeof(uint32_t) * (1024 + (uint64_t)kDepthRadix * nblk);
 }
 
+uint32_t* depth_sort_zero_region(void* temp) { return (uint32_t*)temp + 256; }
+
 void depth_sort_narrow(hipStream_t st, uint64_t* keys, uint64_t* keys_alt, uint32_t m, void* temp,
                        const uint32_t* wmin_part, uint32_t npart, uint32_t* flags, const uint32_t* gather_src,
-                       uint32_t* gather_dst) {
+                       uint32_t* gather_dst, bool counters_clear) {
+    static_assert(2 * kDepthGroups == depth_sort_zero_words, "zero region: the two passes' group sums");
     if (m == 0) return;
     const uint32_t nblk = (m + kDepthBlock - 1) / kDepthBlock;
     uint32_t* base_slot = (uint32_t*)temp;
-    uint32_t* gsum = base_slot + 256;               // [2][kDepthGroups]
+    uint32_t* gsum = depth_sort_zero_region(temp);  // [2][kDepthGroups]
     uint32_t* hist = base_slot + 1024;
-    launch_fill_u32(st, gsum, 0u, 2 * kDepthGroups);
+    if (!counters_clear) launch_fill_u32(st, gsum, 0u, 2 * kDepthGroups);
     hipLaunchKernelGGL(k_depth_hist<true>, dim3(nblk), dim3(kSortThreads), 0, st, (const uint64_t*)keys, m, 0,
                        wmin_part, npart, base_slot, flags, hist, gsum);
     hipLaunchKernelGGL(k_depth_offsets, dim3(kDepthGroups), dim3(kSortThreads), 0, st, hist, nblk,

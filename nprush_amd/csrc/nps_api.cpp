@@ -52,6 +52,10 @@ struct ExtView {
     // unless a recent one found its depth range too wide: then the next
     // narrow_skip speculative views use the full 32-bit sort.
     uint32_t narrow_skip = 0;
+    // entries [0, g2_zero_n) of g2 are zero: the chain rule of the last view that
+    // wrote it cleared them (eager views with their own g2 and the chain right
+    // after the composite)
+    uint32_t g2_zero_n = 0;
     bool in_alt = false;
     uint32_t num_tiles = 0;
     uint32_t* flags_at = nullptr;  // flags in a shared buffer instead of `flags` (16 words)
@@ -227,7 +231,9 @@ int ext_view_reserve(ExtView& x, uint32_t n, uint32_t W, uint32_t H, uint64_t ca
             return rc;
         if ((rc = ensure_dev(x.act, sizeof(uint64_t) * ext_act_words(slots, tu.tile, tu.batch), st, err)) != NPS_OK)
             return rc;
+        void* g2_old = x.g2.p;
         if ((rc = ensure_dev(x.g2, sizeof(double) * 9 * n1, st, err)) != NPS_OK) return rc;
+        if (x.g2.p != g2_old) x.g2_zero_n = 0;
     }
     return NPS_OK;
 }
@@ -262,23 +268,26 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
     if (!capture && (rc = ext_view_reserve(x, n, v.W, v.H, spec_cap, grads, tu, st, err)) != NPS_OK) return rc;
     if (ev && (rc = hip(hipEventRecord(ev[0], st), "event")) != NPS_OK) return rc;
     x.part_tiles = 0;
-    // flags: cleared per view when eager; a captured step clears them once, so a
+    // flags: cleared per view when eager (by the preprocess); a captured step clears them once, so a
     // dropped key anywhere in a lane survives to the Adam update
     uint32_t* flags = x.flags_at ? x.flags_at : (uint32_t*)x.flags.p;
-    if (!capture) launch_fill_u32(st, flags, 0u, 16);
     uint32_t* counts = (out && out->counts) ? out->counts : (uint32_t*)x.counts.p;
     double* g2 = (out && out->g2) ? out->g2 : (double*)x.g2.p;
     // narrow depth sort: only where a too-wide depth range can be caught after
     // the view (speculative runs redo the view then)
     const bool narrow = spec && x.narrow_skip == 0 && n <= 0x7FFFFFFFu;
     if (spec && x.narrow_skip) --x.narrow_skip;
-    launch_ext_preprocess(st, sc.splats, sc.sh, n, sc.deg, v, cam, (ExtRec*)x.rec.p, counts, (uint4*)x.rects.p,
-                          (uint64_t*)x.dkey.p, narrow ? (uint32_t*)x.wmin.p : nullptr);
     // The binning's row pass needs no key count, and its last row offset is the
     // count (u32).  The key path (sorted tile << 32 | id keys) scans per-rank key
     // offsets instead and takes the count from them (u64).
     const bool rows_early = ext_bin_fits(n, v.tiles_x, v.tiles_y, 0);
     const bool key_path = !rows_early || (capture && x.cap >= 0xFFFFFFFFull) || (spec && spec_cap >= 0xFFFFFFFFull);
+    // the preprocess also clears the flags (eager), the narrow sort's counters
+    // and the binning's per-tile counts
+    launch_ext_preprocess(st, sc.splats, sc.sh, n, sc.deg, v, cam, (ExtRec*)x.rec.p, counts, (uint4*)x.rects.p,
+                          (uint64_t*)x.dkey.p, narrow ? (uint32_t*)x.wmin.p : nullptr, flags, capture ? 0u : 16u,
+                          depth_sort_zero_region(x.sort_tmp.p), narrow ? depth_sort_zero_words : 0u,
+                          (uint32_t*)x.bin_tile.p, rows_early ? v.num_tiles : 0u);
     // rank of every splat in (depth, id) order: stable sort of depth << 32 | id on
     // the depth bits; for the key path its last pass also gathers the tile
     // counts in rank order (the binning reads the rectangles instead)
@@ -287,7 +296,7 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
     bool dalt = false;
     if (narrow)
         depth_sort_narrow(st, (uint64_t*)x.dkey.p, (uint64_t*)x.dkey_alt.p, n, x.sort_tmp.p, (const uint32_t*)x.wmin.p,
-                          (n + 255) / 256, flags, gsrc, gdst);
+                          (n + 255) / 256, flags, gsrc, gdst, true);
     else
         dalt = radix_sort_u64(st, (uint64_t*)x.dkey.p, (uint64_t*)x.dkey_alt.p, nullptr, nullptr, n, 32, 64,
                               x.sort_tmp.p, nullptr, true, gsrc, gdst);
@@ -342,7 +351,7 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
         ext_bin(st, n, order, (const uint4*)x.rects.p, v.tiles_x, v.tiles_y, cap,
                 dev_count ? count32 : nullptr, (const uint32_t*)x.bin_roff.p, (uint32_t*)x.keys.p, x.ids, (uint32_t*)x.ranges.p,
                 (uint32_t*)x.order.p, (uint32_t*)x.bin_cnt.p, (uint32_t*)x.bin_off.p, (uint32_t*)x.bin_tile.p,
-                x.bin_scan.p, flags);
+                x.bin_scan.p, flags, true);
         if (ev && (rc = hip(hipEventRecord(ev[2], st), "event")) != NPS_OK) return rc;
     } else {
         // 8-byte keys tile << 32 | id in rank order, stable sort on the tile bits
@@ -364,7 +373,13 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
         launch_tile_ranges(st, sorted, cap, (uint32_t*)x.ranges.p, m_dev);
         launch_keys_to_ids(st, sorted, cap, m_dev, x.ids);
     }
-    if (dev_count && grads) launch_fill_u32(st, (uint32_t*)g2, 0u, 18 * (uint64_t)n);
+    // the chain rule clears what it reads when it runs right after the
+    // composite on the view's own g2; then no fill is needed while that holds
+    const bool own_g2 = !(out && out->g2);
+    const bool clear_chain = grads && own_g2 && !capture && !(out && out->defer_chain);
+    const uint32_t zero_n = x.g2_zero_n;
+    if (dev_count && grads && !(clear_chain && zero_n >= n)) launch_fill_u32(st, (uint32_t*)g2, 0u, 18 * (uint64_t)n);
+    if (grads && own_g2) x.g2_zero_n = 0;
     if (!x.col_major) launch_tile_order(st, (const uint32_t*)x.ranges.p, v.tiles_x, v.tiles_y, (uint32_t*)x.order.p);
     if (ev && (rc = hip(hipEventRecord(ev[3], st), "event")) != NPS_OK) return rc;
     ExtCompositeArgs a;
@@ -388,7 +403,9 @@ int run_ext_view(const SceneRef& sc, const ViewParams& v, int batch, const float
     if (ev && (rc = hip(hipEventRecord(ev[4], st), "event")) != NPS_OK) return rc;
     if (grads && !(out && out->defer_chain))
         launch_ext_chain(st, sc.splats, sc.sh, n, sc.deg, v, cam, (const double*)g2, (const uint32_t*)counts, sc.gradx,
-                         sc.accum, out && out->grads_zero, spec ? flags : nullptr, spec ? 1u : 0u);
+                         sc.accum, out && out->grads_zero, spec ? flags : nullptr, spec ? 1u : 0u,
+                         clear_chain ? g2 : nullptr);
+    if (clear_chain) x.g2_zero_n = std::max(n, zero_n);
     if (ev && (rc = hip(hipEventRecord(ev[7], st), "event")) != NPS_OK) return rc;
     if ((rc = hip(hipGetLastError(), "launch")) != NPS_OK) return rc;
     if (capture) return NPS_OK;

@@ -73,10 +73,15 @@ bool radix_sort_u64(hipStream_t st, uint64_t* keys, uint64_t* keys_alt, uint32_t
 // A drawn depth at or beyond base + 2^24 - 1 stores 1 to flags[0] and flags[1]
 // (the result is then not the depth order).  Result in keys; the last pass
 // gathers as radix_sort_u64 does.  temp: depth_sort_temp_bytes(m).
+// The sort's counters that start at zero: depth_sort_zero_words words at
+// depth_sort_zero_region(temp); with counters_clear the caller has cleared
+// them on the stream, else the sort clears them with a launch of its own.
 size_t depth_sort_temp_bytes(uint64_t m);
+uint32_t* depth_sort_zero_region(void* temp);
+constexpr uint32_t depth_sort_zero_words = 512;
 void depth_sort_narrow(hipStream_t st, uint64_t* keys, uint64_t* keys_alt, uint32_t m, void* temp,
                        const uint32_t* wmin_part, uint32_t npart, uint32_t* flags, const uint32_t* gather_src,
-                       uint32_t* gather_dst);
+                       uint32_t* gather_dst, bool counters_clear = false);
 
 
 // k_composite.hip
@@ -158,14 +163,16 @@ void ext_bin_rows(hipStream_t st, uint32_t n, const uint64_t* order, const uint4
 void ext_bin(hipStream_t st, uint32_t n, const uint64_t* order, const uint4* rects,
              uint32_t tiles_x, uint32_t tiles_y, uint64_t cap, const uint32_t* m_dev, const uint32_t* roff,
              uint32_t* items, uint32_t* ids, uint32_t* ranges, uint32_t* tile_order, uint32_t* ccnt, uint32_t* coff,
-             uint32_t* tile_cnt, void* cscan, uint32_t* overflow);
+             uint32_t* tile_cnt, void* cscan, uint32_t* overflow, bool tile_cnt_clear = false);
 void launch_keys_to_ids(hipStream_t st, const uint64_t* keys, uint64_t m, const uint64_t* m_dev, uint32_t* ids);
 // Entries of the composite's workgroup order (= its grid size).
 uint32_t ext_order_slots(uint32_t tiles_x, uint32_t tiles_y);
 void launch_tile_order(hipStream_t st, const uint32_t* ranges, uint32_t tiles_x, uint32_t tiles_y, uint32_t* order);
 void launch_ext_preprocess(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg,
                            const ViewParams& v, const float cam[3], ExtRec* rec, uint32_t* counts, uint4* rects,
-                           uint64_t* dkey, uint32_t* wmin_part = nullptr);
+                           uint64_t* dkey, uint32_t* wmin_part = nullptr, uint32_t* clr0 = nullptr,
+                           uint32_t nclr0 = 0, uint32_t* clr1 = nullptr, uint32_t nclr1 = 0,
+                           uint32_t* clr2 = nullptr, uint32_t nclr2 = 0);
 // keys beyond `cap` are not written; *overflow is then set to 1
 void launch_ext_keys(hipStream_t st, uint32_t n, const uint64_t* order, const uint32_t* rank_counts,
                      const uint4* rects, uint32_t tiles_x, const uint64_t* offsets, uint64_t* keys, uint64_t cap,
@@ -181,7 +188,8 @@ size_t ext_act_words(size_t slots, int tile, int batch);
 // apart) of a captured step; if any is set the launch changes nothing.
 void launch_ext_chain(hipStream_t st, const float* splats, const float* sh, uint32_t n, int deg, const ViewParams& v,
                       const float cam[3], const double* g2, const uint32_t* counts, float* grads, float* accum,
-                      bool grads_zero = false, const uint32_t* flags = nullptr, uint32_t nflags = 0);
+                      bool grads_zero = false, const uint32_t* flags = nullptr, uint32_t nflags = 0,
+                      double* g2_clear = nullptr);
 // ctl = {t, 1 - 0.9^t, 1 - 0.999^t, skip} on the device: t advances by one
 // unless one of the nflags flags (16 words apart) is set, then nothing changes.
 void launch_adam_ext(hipStream_t st, float* splats, float* sh, float* grads, float* m, float* v, uint32_t n, int deg,
