@@ -400,9 +400,18 @@ def test_ext_autotune(oracle):
     assert tile in (8, 16, 32) and batch in (32, 64, 128)
     assert np.all(ms > 0) and ms[(8, 16, 32).index(tile), (32, 64, 128).index(batch)] == ms.min()
     assert np.array_equal(mgr.read_grads_f32(), g0)      # timing runs add no gradients
+    mgr.write_grads_f32(np.zeros_like(g0))
     mgr.backward()
     img1, _ = mgr.read_image()
     assert np.array_equal(img1, img0)
+    # the timing runs' 2D gradients (no chain rule after them) do not leak into
+    # the next view: its gradients are the first view's (FP32 sums of FP64
+    # accumulators in another order)
+    g1 = mgr.read_grads_f32()
+    assert np.all(np.abs(g1 - g0) <= 1e-4 * np.abs(g0).max(axis=0) + 1e-5 * np.abs(g0))
+    mgr.backward()                                        # again, without a fill in between
+    g2 = mgr.read_grads_f32()
+    assert np.all(np.abs(g2 - 2 * g0) <= 2e-4 * np.abs(g0).max(axis=0) + 1e-5 * np.abs(g0))
     mgr.close()
 
 
