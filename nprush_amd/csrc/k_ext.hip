@@ -697,15 +697,24 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict_
         s_cnt[tid] = 0u;
         if (tid == 0) s_max = 0u;
         __syncthreads();
-        atomicMax(&s_max, cmax);
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) cmax = max(cmax, (uint32_t)__shfl_xor((int)cmax, o, 64));
+        if (lane == 0u) atomicMax(&s_max, cmax);  // one LDS atomic per wave, not per thread
         __syncthreads();
         uint32_t shift = 0u;
         while ((s_max >> shift) >= kOrderLevels) ++shift;
+        // Super-tiles of the last level (no work, common at the frame's edges) are
+        // counted and placed with one LDS atomic per wave, the others per thread.
+        const uint64_t lt = (lane == 0u) ? 0ull : (~0ull >> (64u - lane));
         uint32_t level[kPer];  // 0 = longest
 #pragma unroll
         for (uint32_t q = 0; q < kPer; ++q) {
             level[q] = kOrderLevels - 1u - (cost[q] >> shift);
-            if (tid + 1024u * q < S) atomicAdd(&s_cnt[level[q]], 1u);
+            const bool in = tid + 1024u * q < S;
+            const bool last_lv = in && level[q] == kOrderLevels - 1u;
+            const uint64_t bl = __ballot(last_lv);
+            if (in && !last_lv) atomicAdd(&s_cnt[level[q]], 1u);
+            if (bl && lane == (uint32_t)__builtin_ctzll(bl)) atomicAdd(&s_cnt[kOrderLevels - 1u], (uint32_t)__popcll(bl));
         }
         __syncthreads();
         // exclusive scan of the 1024 level counts (one per thread)
@@ -723,8 +732,17 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict_
         s_cnt[tid] = base + x - v;
         __syncthreads();
 #pragma unroll
-        for (uint32_t q = 0; q < kPer; ++q)
-            if (tid + 1024u * q < S) s_rank[atomicAdd(&s_cnt[level[q]], 1u)] = tid + 1024u * q;
+        for (uint32_t q = 0; q < kPer; ++q) {
+            const bool in = tid + 1024u * q < S;
+            const bool last_lv = in && level[q] == kOrderLevels - 1u;
+            const uint64_t bl = __ballot(last_lv);
+            uint32_t first = 0u;
+            if (bl && lane == (uint32_t)__builtin_ctzll(bl))
+                first = atomicAdd(&s_cnt[kOrderLevels - 1u], (uint32_t)__popcll(bl));
+            first = (uint32_t)__shfl((int)first, bl ? (int)__builtin_ctzll(bl) : 0, 64);
+            if (in && !last_lv) s_rank[atomicAdd(&s_cnt[level[q]], 1u)] = tid + 1024u * q;
+            if (last_lv) s_rank[first + (uint32_t)__popcll(bl & lt)] = tid + 1024u * q;
+        }
         __syncthreads();
     }
     const uint32_t R = (S + 7u) / 8u * 8u;
