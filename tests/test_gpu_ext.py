@@ -234,6 +234,37 @@ def test_ext_narrow_depth_order_and_redo(oracle):
     mgr.close()
 
 
+@pytest.mark.parametrize("n", [1, 37, 300])
+def test_ext_small_scenes_speculative_views(oracle, n):
+    """Scenes of fewer Gaussians than one preprocess workgroup, a camera that sees
+    none of them, then one that sees them: each view twice, so the second runs
+    speculatively (narrow depth order over one or two histogram blocks, no fill
+    of the 2D gradients).  Lists, image and gradients are the oracle's."""
+    W = H = 64
+    deg = 2
+    s, sh = scenes.make_ext_splats(n, 11, sh_degree=deg)
+    s[:, 4:7] += 1.5
+    away = camera.view_proj((0.0, 0.0, 3.0), target=(0.0, 0.0, 6.0), aspect=1.0)
+    front = camera.view_proj((0.3, 0.2, 3.0), aspect=1.0)
+    tgt = np.ones((H, W, 4), np.float32)
+    tgt[..., :3] = 0.5
+    mgr = _run(s, sh, deg, *away, tgt, W, H)
+    for vp, vi in (away, front, front):
+        mgr.write_grads_f32(np.zeros((n, 64), np.float32))
+        mgr.writeSplatVPMatrix(vp, vi)
+        mgr.backward()
+        tiles, lists, counts = _check_tiles(mgr, oracle, s, sh, deg, vp, vi, W, H)
+        g, gs, c, loss = oracle.ext_backward_f32(s, sh, deg, vp, vi, tgt, W, H, tiles, lists, counts)
+        img, _ = mgr.read_image()
+        assert np.array_equal(img, c)
+        if counts.sum():
+            _check_grads(mgr.read_grads_f32(), g, gs, deg)
+        else:
+            assert np.all(mgr.read_grads_f32() == 0)
+    assert counts.sum() > 0                           # the front camera sees the scene
+    mgr.close()
+
+
 def test_ext_forward_only_and_accumulation(oracle):
     W = H = 96
     deg = 3
