@@ -1253,9 +1253,11 @@ __global__ __launch_bounds__(256) void k_ext_chain(const float4* __restrict__ sp
     const double s00 = -(t00 * A + t01 * B), s01 = -(t00 * B + t01 * C), s11 = -(t10 * B + t11 * C);
     const double gcov00 = s00, gcov01 = 2.0 * s01, gcov11 = s11;
     const double aspect = (double)v.aspect, w = o.w, cx = o.cx, cy = o.cy;
-    double gcx = gg[0] * aspect / w;
-    double gcy = gg[1] / w;
-    double gw = -gg[0] * cx * aspect / (w * w) - gg[1] * cy / (w * w);
+    // powers of 1/w once (FP64 divisions are long instruction sequences)
+    const double iw = 1.0 / w, iw2 = iw * iw, iw3 = iw2 * iw;
+    double gcx = gg[0] * aspect * iw;
+    double gcy = gg[1] * iw;
+    double gw = -gg[0] * cx * aspect * iw2 - gg[1] * cy * iw2;
     double gqn[4] = {0.0, 0.0, 0.0, 0.0};
     const double qw = o.qn[0], qx = o.qn[1], qy = o.qn[2], qz = o.qn[3];
 #pragma unroll
@@ -1270,12 +1272,11 @@ __global__ __launch_bounds__(256) void k_ext_chain(const float4* __restrict__ sp
         const double acx = (double)((vp[0] * a[0] + vp[4] * a[1]) + vp[8] * a[2]);
         const double acy = (double)((vp[1] * a[0] + vp[5] * a[1]) + vp[9] * a[2]);
         const double acw = (double)((vp[3] * a[0] + vp[7] * a[1]) + vp[11] * a[2]);
-        const double gacx = gax * aspect / w, gacy = gay / w;
-        const double gacw = -gax * aspect * cx / (w * w) - gay * cy / (w * w);
-        gcx += -gax * aspect * acw / (w * w);
-        gcy += -gay * acw / (w * w);
-        gw += gax * aspect * (-acx / (w * w) + 2.0 * cx * acw / (w * w * w)) +
-              gay * (-acy / (w * w) + 2.0 * cy * acw / (w * w * w));
+        const double gacx = gax * aspect * iw, gacy = gay * iw;
+        const double gacw = -gax * aspect * cx * iw2 - gay * cy * iw2;
+        gcx += -gax * aspect * acw * iw2;
+        gcy += -gay * acw * iw2;
+        gw += gax * aspect * (-acx * iw2 + 2.0 * cx * acw * iw3) + gay * (-acy * iw2 + 2.0 * cy * acw * iw3);
         double gaw[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j)
@@ -1307,8 +1308,9 @@ __global__ __launch_bounds__(256) void k_ext_chain(const float4* __restrict__ sp
         }
     }
     const double dq = gqn[0] * qw + gqn[1] * qx + gqn[2] * qy + gqn[3] * qz;
+    const double iql = 1.0 / (double)o.qlen;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) G[8 + j] += (gqn[j] - dq * (double)o.qn[j]) / (double)o.qlen;
+    for (int j = 0; j < 4; ++j) G[8 + j] += (gqn[j] - dq * (double)o.qn[j]) * iql;
 #pragma unroll
     for (int j = 0; j < 3; ++j)
         G[j] += gcx * (double)vp[4 * j + 0] + gcy * (double)vp[4 * j + 1] + gw * (double)vp[4 * j + 3];
@@ -1352,9 +1354,10 @@ __global__ __launch_bounds__(256) void k_ext_chain(const float4* __restrict__ sp
     sh_grad_weighted(dir0, dir1, dir2, DEG, wk, gdir);
     const double dd = gdir[0] * dir0 + gdir[1] * dir1 + gdir[2] * dir2;
     const double dirv[3] = {dir0, dir1, dir2};
+    const double idl = 1.0 / (double)dlen;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        const double gpos = (gdir[j] - dd * dirv[j]) / (double)dlen;
+        const double gpos = (gdir[j] - dd * dirv[j]) * idl;
         out[j] = kStore ? (float)(G[j] + gpos) : out[j] + (float)(G[j] + gpos);
     }
 }
