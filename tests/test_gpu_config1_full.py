@@ -79,3 +79,42 @@ def test_config1_view_full_size(oracle):
               f"rel {err.max() / scale:.2e})  GPU-FP64 max-abs {e64.max():.3e} (mean-L1 {e64.max() * norm:.3e}, "
               f"rel {e64.max() / scale:.2e})  FP32-FP64 max-abs {np.abs(ref - ref64).max():.3e}")
     assert np.all(gx[:, [7, 12, 13, 14, 15]] == 0)
+
+
+def test_config1_speculative_view_full_size(oracle):
+    """The benchmark's steady state at full size: a second view of the same
+    shape runs speculatively (key buffers sized from the first view's count,
+    the depth order in two 12-bit passes over the range-compressed depth, no
+    fill of the 2D gradients).  Its lists and image are the oracle's bit for
+    bit, its gradients within the EXT tolerance."""
+    cams = camera.ring_cameras(8, 3.0, W, H)
+    s, sh = scenes.make_ext_splats(N, 0, sh_degree=DEG)
+    tgt = np.ones((H, W, 4), np.float32)
+    tgt[..., :3] = 0.5
+    mgr = GpuSplatOptimizerManager(numSplats=N, mode="ext", sh_degree=DEG, splats=s, sh=sh, adc_enabled=False)
+    mgr.setBackwardTarget(tgt, np.ones((H, W), np.float32), W, H)
+    mgr.writeSplatVPMatrix(*cams[0])
+    mgr.backward()                                    # waits for its key count
+    vp, vi = cams[3]
+    mgr.write_grads_f32(np.zeros((N, 64), np.float32))
+    mgr.writeSplatVPMatrix(vp, vi)
+    mgr.backward()                                    # speculative
+    ranges, ids, full = mgr.read_tiles()
+    img, _ = mgr.read_image()
+    gx = mgr.read_grads_f32().astype(np.float64)
+    mgr.close()
+    offs, oids = oracle.ext_bin_all(s, sh, DEG, vp, vi, W, H)
+    counts = np.diff(offs).astype(np.int64)
+    assert np.array_equal(full.astype(np.int64), counts)
+    assert np.array_equal((ranges[:, 1] - ranges[:, 0]).astype(np.int64), counts)
+    gather = np.concatenate([np.arange(a, b) for a, b in ranges if b > a])
+    assert np.array_equal(ids[gather], oids)
+    g, gs, c32, loss = oracle.ext_backward_csr_f32(s, sh, DEG, vp, vi, tgt, W, H, offs, oids)
+    assert np.array_equal(img, c32)
+    for name, cols in GROUPS.items():
+        ref = g[:, cols]
+        err = np.abs(gx[:, cols] - ref)
+        assert np.all(err <= 1e-4 * np.abs(ref).max() + 1e-5 * np.abs(ref)), (name, err.max())
+    ref = gs[:, :48]
+    err = np.abs(gx[:, 16:64] - ref)
+    assert np.all(err <= 1e-4 * np.abs(ref).max() + 1e-5 * np.abs(ref)), ("sh", err.max())
